@@ -1,0 +1,150 @@
+// Oracle command-line driver (src/main.rs:322-849 subset): chunking, per-chunk
+// RNG/insert-size state, ordered output, N worker threads.
+// TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+#include "oracle.hpp"
+
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <thread>
+
+using namespace orc;
+
+static void usage() {
+    std::fprintf(stderr,
+                 "usage: sa_oracle [-t N] [-x|--aemb] [-r N] [-N N] [-U] [--eqx] [--ssw] [-C] [--details]\n"
+                 "                 [--rg-id ID] [--rg TAG] [--chunk-size N] [--mcs MODE] [-o OUT]\n"
+                 "                 [--dump-index PATH] [--time] ref.fa reads1 [reads2]\n");
+}
+
+int main(int argc, char** argv) {
+    size_t threads = 1, chunk_size = 1000000, read_length = 0;
+    int mode = 0;
+    bool time_it = false;
+    MappingParams mp;
+    SamOpts so;
+    std::vector<std::string> rg_fields;
+    std::string out_path, dump_index;
+    std::vector<std::string> pos;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto val = [&](const std::string& name) -> std::string {
+            size_t eq = a.find('=');
+            if (eq != std::string::npos && a.compare(0, eq, name) == 0) return a.substr(eq + 1);
+            if (i + 1 >= argc) { usage(); std::exit(2); }
+            return argv[++i];
+        };
+        auto is = [&](const char* name) { return a == name || a.rfind(std::string(name) + "=", 0) == 0; };
+        if (a == "-t") threads = std::stoul(val("-t"));
+        else if (a == "-x") mode = 1;
+        else if (a == "--aemb") mode = 2;
+        else if (a == "-r") read_length = std::stoul(val("-r"));
+        else if (a == "-N") mp.max_secondary = std::stoul(val("-N"));
+        else if (a == "-U") mp.output_unmapped = false;
+        else if (a == "--eqx") so.eqx = true;
+        else if (a == "--ssw") mp.use_ssw = true;
+        else if (a == "-C") so.fastq_comments = true;
+        else if (a == "--details") so.details = true;
+        else if (a == "--no-PG") {}
+        else if (a == "--time") time_it = true;
+        else if (is("--rg-id")) { so.has_rg = true; so.rg_id = val("--rg-id"); }
+        else if (is("--rg")) rg_fields.push_back(val("--rg"));
+        else if (is("--chunk-size")) chunk_size = std::stoul(val("--chunk-size"));
+        else if (is("--mcs")) {
+            std::string m = val("--mcs");
+            mp.mcs = m == "rescue" ? Mcs::Rescue : m == "off" ? Mcs::Off : m == "first-strobe" ? Mcs::FirstStrobe : Mcs::Always;
+        } else if (a == "-o") out_path = val("-o");
+        else if (is("--dump-index")) dump_index = val("--dump-index");
+        else if (!a.empty() && a[0] == '-' && a.size() > 1) { usage(); return 2; }
+        else pos.push_back(a);
+    }
+    if (!rg_fields.empty() && !so.has_rg) { so.has_rg = true; so.rg_id = "1"; }
+    if (pos.size() < 2) { usage(); return 2; }
+    std::string err;
+    std::vector<Record> r1, r2;
+    if (!read_reads(pos[1], r1, err)) { std::fprintf(stderr, "error: %s\n", err.c_str()); return 1; }
+    if (pos.size() > 2 && !read_reads(pos[2], r2, err)) { std::fprintf(stderr, "error: %s\n", err.c_str()); return 1; }
+    if (read_length == 0) read_length = estimate_read_length(r1);
+    SeedingParams sp = seeding_params_for_read_length(read_length);
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<RefSeq> refs = read_fasta(pos[0]);
+    if (refs.empty()) { std::fprintf(stderr, "error: No sequences found in the reference FASTA\n"); return 1; }
+    int bits = pick_bits(refs, sp.syncmer);
+    Index index = make_index(refs, sp, bits, 0.0002);
+    auto t1 = std::chrono::steady_clock::now();
+    if (!dump_index.empty()) {
+        std::ofstream f(dump_index, std::ios::binary);
+        uint64_t n = index.randstrobes.size(), nb = index.bucket_starts.size();
+        uint64_t hdr[4] = {n, nb, (uint64_t)index.bits, (uint64_t)index.filter_cutoff};
+        f.write((const char*)hdr, sizeof hdr);
+        f.write((const char*)index.randstrobes.data(), (std::streamsize)(n * 16));
+        f.write((const char*)index.bucket_starts.data(), (std::streamsize)(nb * 8));
+    }
+
+    Mapper m;
+    m.index = &index;
+    m.refs = &refs;
+    m.mp = mp;
+    m.so = so;
+    m.mode = mode;
+    m.abundances.assign(refs.size(), 0.0);
+
+    // src/main.rs:591-604: a chunk ends with the record that makes its nucleotide count exceed chunk_size
+    using Pair = std::pair<Record, std::optional<Record>>;
+    std::vector<std::vector<Pair>> chunks;
+    {
+        std::vector<Pair> cur;
+        size_t nt = 0;
+        size_t n = r2.empty() ? r1.size() : std::min(r1.size(), r2.size());
+        for (size_t i = 0; i < n; ++i) {
+            if (r2.empty()) { nt += r1[i].seq.size(); cur.push_back({r1[i], std::nullopt}); }
+            else { nt += r1[i].seq.size() + r2[i].seq.size(); cur.push_back({r1[i], r2[i]}); }
+            if (nt > chunk_size) { chunks.push_back(std::move(cur)); cur.clear(); nt = 0; }
+        }
+        if (!cur.empty()) chunks.push_back(std::move(cur));
+    }
+    std::vector<std::string> outs(chunks.size());
+    std::vector<std::vector<double>> abund(threads, std::vector<double>(refs.size(), 0.0));
+    std::atomic<size_t> next{0};
+    auto t2 = std::chrono::steady_clock::now();
+    auto worker = [&](size_t tid) {
+        Mapper mm = m;
+        mm.abundances.assign(refs.size(), 0.0);
+        for (;;) {
+            size_t c = next.fetch_add(1);
+            if (c >= chunks.size()) break;
+            mm.map_chunk(chunks[c], outs[c]);
+        }
+        abund[tid] = mm.abundances;
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < threads; ++t) th.emplace_back(worker, t);
+    for (auto& t : th) t.join();
+    auto t3 = std::chrono::steady_clock::now();
+
+    FILE* out = out_path.empty() ? stdout : std::fopen(out_path.c_str(), "wb");
+    if (!out) { std::fprintf(stderr, "error: cannot open %s\n", out_path.c_str()); return 1; }
+    if (mode == 0) {
+        std::string h = sam_header(refs, so, rg_fields);
+        std::fwrite(h.data(), 1, h.size(), out);
+    }
+    for (const std::string& s : outs) std::fwrite(s.data(), 1, s.size(), out);
+    if (mode == 2) {
+        std::vector<double> ab(refs.size(), 0.0);
+        for (size_t t = 0; t < threads; ++t)
+            for (size_t i = 0; i < refs.size(); ++i) ab[i] += abund[t][i];
+        for (size_t i = 0; i < refs.size(); ++i) std::fprintf(out, "%s\t%.6f\n", refs[i].name.c_str(), ab[i] / (double)refs[i].seq.len);
+    }
+    if (out != stdout) std::fclose(out);
+    if (time_it) {
+        double ti = std::chrono::duration<double>(t1 - t0).count();
+        double tm = std::chrono::duration<double>(t3 - t2).count();
+        size_t n = r2.empty() ? r1.size() : std::min(r1.size(), r2.size());
+        std::fprintf(stderr, "{\"index_s\": %.4f, \"map_s\": %.4f, \"records\": %zu, \"threads\": %zu}\n", ti, tm, n, threads);
+    }
+    return 0;
+}
