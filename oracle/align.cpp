@@ -417,7 +417,8 @@ std::optional<AlignmentInfo> aligner_align(const uint8_t* query, size_t qlen, co
 // Affine-gap DP standing in for block-aligner (parity unpinned): gap of
 // length L costs gap_open + (L-1)*gap_extend, as block-aligner's Gaps and the
 // profile gap settings in src/piecewisealigner.rs:589-624 both give.
-// Traceback order at ties: diagonal, then insertion (query gap), then deletion.
+// Traceback order at ties (walking back from the end): deletion, then insertion, then diagonal;
+// a gap run is closed as soon as opening it explains the score.
 namespace {
 const int NEG = INT_MIN / 4;
 
@@ -459,13 +460,13 @@ Cigar trace(const Dp& d, int i, int j, int go, int ge, Sub sub, const uint8_t* q
     while (i > 0 || j > 0) {
         if (state == 0) {
             int h = d.H[ix(i, j)];
-            if (i > 0 && j > 0 && h == d.H[ix(i - 1, j - 1)] + sub(i, j)) {
-                rev.push_back(q[i - 1] == r[j - 1] ? 7 : 8);
-                --i; --j;
+            if (j > 0 && h == d.E[ix(i, j)]) {
+                state = 1;
             } else if (i > 0 && h == d.F[ix(i, j)]) {
                 state = 2;
             } else {
-                state = 1;
+                rev.push_back(q[i - 1] == r[j - 1] ? 7 : 8);
+                --i; --j;
             }
         } else if (state == 2) {
             rev.push_back(1);

@@ -1,0 +1,449 @@
+"""Pins the CPU oracle before it is trusted as the checker.
+
+Sources of truth, in order: the reference's golden files (tests/golden/phix.*,
+copied from the reference's tests/), the reference's unit known-answer tests
+(cited per test), vectors written by the reference's own ssw.c
+(tests/golden/ssw_vectors.json), and brute-force restatements of definitions.
+"""
+import ctypes
+import json
+import os
+import random
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN
+
+PHIX = os.path.join(GOLDEN, "phix.fasta")
+R1 = os.path.join(GOLDEN, "phix.1.fastq")
+R2 = os.path.join(GOLDEN, "phix.2.fastq")
+SAM_FLAGS = ["--no-PG", "--eqx", "--rg-id=1", "--rg=SM:sample", "--rg=LB:library"]
+SC = (ctypes.c_int * 5)(2, 8, 12, 1, 10)
+
+
+def run(oracle_bin, *args):
+    return subprocess.run([oracle_bin, *args], check=True, capture_output=True).stdout.decode()
+
+
+def read(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return f.read()
+
+
+# ------------------------------------------------------------ end-to-end ---
+# tests/test.rs:66-100 and tests/run.sh:37-105 of the reference
+
+def test_single_end_sam_golden(oracle_bin):
+    assert run(oracle_bin, "--chunk-size=3", *SAM_FLAGS, PHIX, R1) == read("phix.se.sam")
+
+
+def test_paired_end_sam_golden(oracle_bin):
+    assert run(oracle_bin, *SAM_FLAGS, PHIX, R1, R2) == read("phix.pe.sam")
+
+
+def test_single_end_paf_golden(oracle_bin):
+    out = run(oracle_bin, "-x", PHIX, R1).splitlines(keepends=True)
+    assert "".join(out[-11:]) == read("phix.se.paf")
+
+
+def test_mcs_rescue_paf_golden(oracle_bin):
+    out = run(oracle_bin, "-x", "--mcs=rescue", PHIX, R1).splitlines(keepends=True)
+    assert "".join(out[-11:]) == read("phix.mcsrescue.se.paf")
+
+
+def test_abundance_se_golden(oracle_bin):
+    assert run(oracle_bin, "--aemb", PHIX, R1) == read("phix.abun.se.txt")
+
+
+def test_abundance_pe_golden(oracle_bin):
+    assert run(oracle_bin, "--aemb", PHIX, R1, R2) == read("phix.abun.pe.txt")
+
+
+def test_tags_sam_golden(oracle_bin):
+    out = run(oracle_bin, "-C", "--no-PG", "--rg-id=1", "--rg=SM:sample", "--rg=LB:library", PHIX,
+              os.path.join(GOLDEN, "phix.tags.fastq"))
+    assert out == read("phix.tags.sam")
+
+
+def test_secondary_removal_equals_no_secondary(oracle_bin, tmp_path):
+    # tests/run.sh:86-100: a duplicated phiX gives secondary records; dropping them
+    # gives the output produced without -N.
+    rep = tmp_path / "rep.fa"
+    fa = read("phix.fasta")
+    rep.write_text(fa + ">repeated_NC_001422\n" + "".join(fa.splitlines(keepends=True)[1:]))
+    no_sec = run(oracle_bin, "--no-PG", str(rep), R1)
+    with_sec = run(oracle_bin, "--no-PG", "-N", "5", str(rep), R1)
+    sec = [l for l in with_sec.splitlines() if not l.startswith("@") and int(l.split("\t")[1]) & 0x100]
+    assert len(sec) > 0
+    prim = "".join(l + "\n" for l in with_sec.splitlines() if l.startswith("@") or not int(l.split("\t")[1]) & 0x100)
+    assert prim == no_sec
+
+
+# ------------------------------------------------------------- seeding ---
+
+def test_xxh64_kat(orc):
+    # src/seeding/hash.rs:29-33 ("hello123")
+    orc.orc_xxh64.restype = ctypes.c_uint64
+    orc.orc_xxh64.argtypes = [ctypes.c_uint64]
+    assert orc.orc_xxh64(0x3332316F6C6C6568) == 0x2D119FCED5EE39CE
+
+
+def params(orc, rl):
+    out = (ctypes.c_int64 * 8)()
+    orc.orc_seeding_params(ctypes.c_uint64(rl), out)
+    return list(out)
+
+
+def test_profiles(orc):
+    # src/seeding/parameters.rs:330-421: r250 -> k22 s18 w6..16 max_dist 180, mask 0xfffffffffc000000
+    p = params(orc, 250)
+    assert p[:6] == [22, 18, 3, 6, 16, 180]
+    assert p[6] == 255
+    assert p[7] & 0xFFFFFFFFFFFFFFFF == 0xFFFFFFFFFC000000
+    assert params(orc, 150)[:6] == [20, 16, 3, 5, 11, 80]
+    assert params(orc, 289)[:6] == [22, 18, 3, 6, 16, 180]  # phix.1.fastq mean length (src/main.rs:1045)
+    assert params(orc, 50)[:6] == [18, 14, 3, 1, 4, 18]
+    assert params(orc, 100)[:6] == [20, 16, 3, 2, 6, 30]
+    assert params(orc, 1000)[:6] == [23, 17, 4, 5, 15, 255]
+
+
+def syncmers(orc, seq, k, s):
+    cap = len(seq) + 1
+    h = (ctypes.c_uint64 * cap)()
+    f = (ctypes.c_uint8 * cap)()
+    p = (ctypes.c_uint64 * cap)()
+    orc.orc_syncmers.restype = ctypes.c_int64
+    n = orc.orc_syncmers(seq, ctypes.c_uint64(len(seq)), k, s, h, f, p, ctypes.c_int64(cap))
+    return [(h[i], f[i], p[i]) for i in range(n)]
+
+
+def xxh(orc, v):
+    orc.orc_xxh64.restype = ctypes.c_uint64
+    orc.orc_xxh64.argtypes = [ctypes.c_uint64]
+    return orc.orc_xxh64(v)
+
+
+def enc(seq):
+    return [{"A": 0, "C": 1, "G": 2, "T": 3}.get(c, 4) for c in seq]
+
+
+def brute_syncmers(orc, seq, k, s):
+    """Definition: the k-mer at p (no N inside) is a syncmer iff the canonical s-mer
+    starting at p+t-1 has the minimum xxh64 among the k-s+1 s-mers of the k-mer
+    (src/seeding/syncmers.rs:171-237)."""
+    t = (k - s) // 2 + 1
+    c = enc(seq)
+
+    def canon(i, n):
+        f = r = 0
+        for j in range(n):
+            f = (f << 2) | c[i + j]
+            r = (r << 2) | (3 - c[i + n - 1 - j])
+        return min(f, r), f, r
+
+    out = []
+    for p in range(len(seq) - k + 1):
+        if any(x == 4 for x in c[p:p + k]):
+            continue
+        hs = [xxh(orc, canon(p + j, s)[0]) for j in range(k - s + 1)]
+        if hs[t - 1] == min(hs):
+            y, f, r = canon(p, k)
+            out.append((xxh(orc, y), int(f <= r), p))
+    return out
+
+
+def test_syncmers_brute_force(orc):
+    rng = random.Random(5)
+    for _ in range(30):
+        seq = "".join(rng.choice("ACGT" if rng.random() < 0.9 else "ACGTN") for _ in range(rng.randint(5, 120)))
+        assert syncmers(orc, seq.encode(), 20, 16) == brute_syncmers(orc, seq, 20, 16)
+        assert syncmers(orc, seq.encode(), 8, 4) == brute_syncmers(orc, seq, 8, 4)
+
+
+def test_syncmer_poly_a(orc):
+    # src/seeding/syncmers.rs:258-269
+    assert syncmers(orc, b"A" * 20, 8, 4)[0][2] == 0
+
+
+def revcomp(s):
+    return s[::-1].translate(str.maketrans("ACGTN", "TGCAN"))
+
+
+def test_canonical_syncmers_rc_symmetry(orc):
+    # src/seeding/syncmers.rs:276-300
+    phix = "".join(read("phix.fasta").splitlines()[1:]).upper()  # decode_all() yields uppercase
+    for seq in ["A" * 41, phix]:
+        fw = syncmers(orc, seq.encode(), 20, 16)
+        rv = syncmers(orc, revcomp(seq).encode(), 20, 16)[::-1]
+        assert len(fw) == len(rv)
+        for a, b in zip(fw, rv):
+            assert a[0] == b[0]
+            assert a[2] == len(seq) - 20 - b[2]
+            assert a[1] != b[1]
+
+
+def randstrobes(orc, seq, rl):
+    cap = len(seq) + 1
+    arr = [(ctypes.c_uint64 * (2 * cap))() for _ in range(4)]
+    counts = (ctypes.c_int64 * 2)()
+    orc.orc_randstrobes_query(seq, ctypes.c_uint64(len(seq)), ctypes.c_uint64(rl), *arr, ctypes.c_int64(cap), counts)
+    return [[(arr[0][o * cap + i], arr[1][o * cap + i], arr[2][o * cap + i], arr[3][o * cap + i])
+             for i in range(counts[o])] for o in range(2)]
+
+
+def brute_randstrobe_hash(h1, h2, f1, f2):
+    main = 0xFFFFFFFFFC000000
+    fmain = main | (1 << 25)
+    return ((h1 & main) | (h2 & ~fmain & 0xFFFFFFFFFFFFFFFF) | (f1 << 25) | (f2 << 8)) & 0xFFFFFFFFFFFFFE00
+
+
+def test_randstrobes_brute_force(orc):
+    """Window definition (src/seeding/strobes.rs:145-176) restated in Python."""
+    rng = random.Random(9)
+    k, s, wmin, wmax, md = 20, 16, 5, 11, 80
+    for _ in range(20):
+        seq = "".join(rng.choice("ACGT") for _ in range(rng.randint(30, 300)))
+        sy = syncmers(orc, seq.encode(), k, s)
+        got = randstrobes(orc, seq.encode(), 150)
+        exp = []
+        for a in range(len(sy)):
+            h1, f1, p1 = sy[a]
+            best, s2 = None, sy[a]
+            for i in range(wmin, min(len(sy) - a, wmax + 1)):
+                h, f, p = sy[a + i]
+                if p > p1 + md:
+                    break
+                ones = bin((h1 ^ h) & 255).count("1")
+                if best is None or ones < best:
+                    best, s2 = ones, sy[a + i]
+            exp.append((brute_randstrobe_hash(h1, s2[0], f1, s2[1]), brute_randstrobe_hash(s2[0], h1, f1 ^ 1, s2[1] ^ 1),
+                        p1, s2[2] + k))
+        assert got[0] == exp
+        assert len(got[1]) == len(got[0])  # count == syncmer count (src/seeding/strobes.rs:217-238)
+
+
+# ------------------------------------------------------------- chaining ---
+
+def chain_best(orc, anchors, k, rl):
+    arr = (ctypes.c_uint64 * (3 * len(anchors)))(*[x for a in anchors for x in a])
+    orc.orc_chain_best.restype = ctypes.c_float
+    return orc.orc_chain_best(arr, ctypes.c_int64(len(anchors)), ctypes.c_uint64(k), ctypes.c_uint64(rl))
+
+
+def test_chainer_early_break(orc):
+    # src/chainer.rs:455-474
+    assert chain_best(orc, [(0, 0, 0), (0, 30, 20), (0, 60, 0), (0, 95, 35)], 20, 2000) > 42.0
+
+
+def test_chainer_linear_adjacent(orc):
+    # src/chainer.rs:476-499
+    a = [(0, 0, 0), (0, 20, 20), (0, 40, 40)]
+    s1 = chain_best(orc, a[:1], 20, 200)
+    assert chain_best(orc, a[:2], 20, 200) == s1 * 2
+    assert chain_best(orc, a[:3], 20, 200) == s1 * 3
+
+
+def test_chainer_diagonal_ratio(orc):
+    # src/chainer.rs:501-512
+    assert chain_best(orc, [(0, 0, 0), (0, 11, 1)], 20, 2000) == 20.0
+
+
+def test_chain_score_closed_form(orc):
+    orc.orc_chain_score.restype = ctypes.c_float
+    orc.orc_chain_score.argtypes = [ctypes.c_uint64] * 3
+    import numpy as np
+    for dq, dr in [(5, 5), (30, 25), (10, 40), (100, 100), (3, 200)]:
+        dd, dg = abs(dr - dq), min(dq, dr)
+        f = np.float32
+        s = f(min(20, dg))
+        lin = f(0.1) * f(dd) + f(0.05) * f(max(dg - 20, 0))
+        lg = np.log2(f(dd) + f(1)) if dd >= 1 else f(0)
+        exp = s - (lin + f(0.5) * lg)
+        assert orc.orc_chain_score(dq, dr, 20) == pytest.approx(float(exp), abs=1e-6)
+
+
+# ------------------------------------------------------------- hamming ---
+
+def hss(orc, q, r, m, mm, eb):
+    s, e, sc = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+    orc.orc_highest_scoring_segment(q, r, ctypes.c_uint64(len(q)), m, mm, eb, ctypes.byref(s), ctypes.byref(e),
+                                    ctypes.byref(sc))
+    return s.value, e.value, sc.value
+
+
+def test_highest_scoring_segment_kats(orc):
+    # src/aligner.rs:321-372
+    assert hss(orc, b"AAAAAAAAAA", b"AAAAAATTTT", 5, 7, 0) == (0, 6, 30)
+    assert hss(orc, b"AAAAAAAAAA", b"TTTTAAAAAA", 5, 7, 0) == (4, 10, 30)
+    assert hss(orc, b"AAAAAAAAAA", b"TTAAAAAATT", 5, 7, 0) == (2, 8, 30)
+    assert hss(orc, b"AAAAAAAAAAAAAAA", b"TAAAAAATTTAAAAT", 5, 7, 0) == (1, 7, 30)
+    assert hss(orc, b"", b"", 2, 4, 5) == (0, 0, 10)
+    assert hss(orc, b"TAAT", b"TAAA", 2, 4, 5) == (0, 4, 12)
+    assert hss(orc, b"AAA", b"AAA", 2, 4, 5) == (0, 3, 16)
+    assert hss(orc, b"TAAT", b"AAAA", 2, 4, 5) == (0, 4, 6)
+    assert hss(orc, b"ATAATA", b"AAAAAA", 2, 4, 5) == (0, 6, 10)
+    assert hss(orc, b"TTAATA", b"AAAAAA", 2, 4, 5) == (2, 6, 7)
+
+
+def ham(orc, q, r, m, mm, eb):
+    sc = (ctypes.c_int * 5)(m, mm, 12, 1, eb)
+    qs, qe, ed = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    buf = ctypes.create_string_buffer(256)
+    s = orc.orc_hamming_align(q, r, ctypes.c_uint64(len(q)), sc, ctypes.byref(qs), ctypes.byref(qe), ctypes.byref(ed),
+                              buf, ctypes.c_uint64(256))
+    return buf.value.decode(), ed.value, s, qs.value, qe.value
+
+
+def test_hamming_align_kats(orc):
+    # src/aligner.rs:374-470
+    assert ham(orc, b"AAXGGG", b"AAYGGG", 1, 1, 0) == ("2=1X3=", 1, 4, 0, 6)
+    assert ham(orc, b"AXGGG", b"AYGGG", 1, 4, 0) == ("3=", 0, 3, 2, 5)
+    assert ham(orc, b"NAACCG", b"TAACCG", 3, 7, 0) == ("5=", 0, 15, 1, 6)
+    assert ham(orc, b"AACCGN", b"AACCGT", 3, 7, 0) == ("5=", 0, 15, 0, 5)
+    assert ham(orc, b"NAAAAAAAAAAAAAA", b"TAAAATTTTTTTTTT", 3, 7, 0) == ("4=", 0, 12, 1, 5)
+    assert ham(orc, b"", b"", 7, 5, 0) == ("", 0, 0, 0, 0)
+
+
+# ----------------------------------------------------------------- SSW ---
+
+def ssw_raw(orc, q, r):
+    mat = bytes([(2 if (a == b and a < 4) else 256 - 8) for a in range(5) for b in range(5)])
+    out = (ctypes.c_int64 * 7)()
+    cig = (ctypes.c_uint32 * 4096)()
+    orc.orc_ssw_raw(bytes(q), len(q), bytes(r), len(r), mat, 5, 12, 1, out, cig, ctypes.c_int64(4096))
+    return [out[i] for i in range(6)], [cig[i] for i in range(out[6])]
+
+
+def test_ssw_against_reference_vectors(orc):
+    """Vectors written by the reference's ext/ssw/ssw.c (tests/golden/gen_ssw_vectors.py)."""
+    with open(os.path.join(GOLDEN, "ssw_vectors.json")) as f:
+        vecs = json.load(f)
+    assert len(vecs) >= 300
+    n_word = 0
+    for v in vecs:
+        q = [int(c) for c in v["q"]]
+        r = [int(c) for c in v["r"]]
+        res, cig = ssw_raw(orc, q, r)
+        assert res == v["res"], v
+        if v["res"][1] >= 0:
+            assert cig == v["cigar"], v
+        n_word += v["res"][0] > 247
+    assert n_word > 10  # the i16 kernel is exercised
+
+
+def ssw(orc, q, r):
+    out = (ctypes.c_int64 * 6)()
+    buf = ctypes.create_string_buffer(8192)
+    ok = orc.orc_ssw_align(q, ctypes.c_uint64(len(q)), r, ctypes.c_uint64(len(r)), SC, out, buf, ctypes.c_uint64(8192))
+    return (list(out), buf.value.decode()) if ok else None
+
+
+def test_ssw_align_no_result(orc):
+    # src/ssw.rs:191-201 and src/aligner.rs:299-306
+    assert ssw(orc, b"TCTCTCCCTCTCTCTCTCTCCCTCCCTCTCTCTCCCTCTCTCTCTCTCTCTCCCTCCCTT",
+               b"GAGGGAGAGAGAGAGAGGGAGAGAGAGAGAGAG") is None
+    assert ssw(orc, b"CCCCC", b"AAAAAACCCCCGGGGG") is not None  # src/aligner.rs:308-319
+
+
+def test_ssw_end_bonus_extension(orc):
+    """A clipped mismatch at the read start is re-extended when the end bonus pays for it."""
+    ref = b"ACGTTGCAAGGCTTACCGATAGCTAGCTAGGATCCATGCA"
+    q = b"T" + ref[11:30]
+    out, cig = ssw(orc, q, ref)
+    # SSW alone starts at query 1; extending one X to the start costs 8 and gains 10
+    assert out[1] == 0 and cig.startswith("1X")
+    assert out[0] == 19 * 2 - 8 + 10 + 10
+
+
+# ------------------------------------------------------------- piecewise ---
+
+def pw(orc, q, r, mode, eb=10, xdrop=100):
+    sc = (ctypes.c_int * 5)(2, 8, 12, 1, eb)
+    out = (ctypes.c_int64 * 5)()
+    buf = ctypes.create_string_buffer(1024)
+    orc.orc_pw_align(q, ctypes.c_uint64(len(q)), r, ctypes.c_uint64(len(r)), sc, mode, xdrop, out, buf,
+                     ctypes.c_uint64(1024))
+    return list(out), buf.value.decode()
+
+
+@pytest.mark.parametrize("q,r,score,qe,re_,cigar", [
+    # src/piecewisealigner.rs:725-933
+    (b"AAATTT", b"AAATTT", 12, 6, 6, "6="),
+    (b"AAA", b"TTT", -24, 3, 3, "3X"),
+    (b"AAATAA", b"AAAAAA", 2, 6, 6, "3=1X2="),
+    (b"AAATTT", b"AAAATTTT", -1, 6, 8, "3=2D3="),
+    (b"AAAATTTT", b"AAATTT", -1, 8, 6, "3=2I3="),
+    (b"AAATTT", b"TTAAATTT", -1, 6, 8, "2D6="),
+    (b"AAATTT", b"AAATTTAA", -1, 6, 8, "6=2D"),
+    (b"TTAAATTT", b"AAATTT", -1, 8, 6, "2I6="),
+    (b"AAATTTAA", b"AAATTT", -1, 8, 6, "6=2I"),
+    (b"ATATATATAT", b"AAAAAAAAAA", -30, 10, 10, "1=1X1=1X1=1X1=1X1=1X"),
+    (b"AAACTTAAACCTT", b"AAAATTGAAATT", -13, 13, 12, "3=1X2=1D3=2I2="),
+])
+def test_global_alignment_kats(orc, q, r, score, qe, re_, cigar):
+    out, cig = pw(orc, q, r, 0)
+    assert out == [score, 0, qe, 0, re_]
+    assert cig == cigar
+
+
+@pytest.mark.parametrize("q,r,mode,eb,exp,cigar", [
+    # src/piecewisealigner.rs:935-1215
+    (b"AAATTT", b"AAATTT", 1, 10, [22, 0, 6, 0, 6], "6="),
+    (b"AAATAA", b"AAAAAA", 1, 10, [12, 0, 6, 0, 6], "3=1X2="),
+    (b"AAAAAAATTTTTTT", b"AAAAAAACCCCCCC", 1, 10, [14, 0, 7, 0, 7], "7="),
+    (b"AAAAAAAATT", b"AAAAAAAAAA", 1, 50, [50, 0, 10, 0, 10], "8=2X"),
+    (b"AAAAACCTTT", b"AAAAATTT", 1, 10, [13, 0, 10, 0, 8], "5=2I3="),
+    (b"AAAAATTT", b"AAAAACCTTT", 1, 10, [13, 0, 8, 0, 10], "5=2D3="),
+    (b"AAATTT", b"AAATTT", 2, 10, [22, 0, 6, 0, 6], "6="),
+    (b"AATAAA", b"AAAAAA", 2, 10, [12, 0, 6, 0, 6], "2=1X3="),
+    (b"TTTTTTTAAAAAAA", b"CCCCCCCAAAAAAA", 2, 10, [14, 7, 14, 7, 14], "7="),
+    (b"TTAAAAAAAA", b"AAAAAAAAAA", 2, 50, [50, 0, 10, 0, 10], "2X8="),
+    (b"TTTCCAAAAA", b"TTTAAAAA", 2, 10, [13, 0, 10, 0, 8], "3=2I5="),
+    (b"TTTAAAAA", b"TTTCCAAAAA", 2, 10, [13, 0, 8, 0, 10], "3=2D5="),
+])
+def test_xdrop_alignment_kats(orc, q, r, mode, eb, exp, cigar):
+    out, cig = pw(orc, q, r, mode, eb)
+    assert out == exp
+    assert cig == cigar
+
+
+def rsa(orc, anchors):
+    arr = (ctypes.c_uint64 * (2 * len(anchors)))(*[x for a in anchors for x in a])
+    n = orc.orc_remove_spurious_anchors(arr, ctypes.c_int64(len(anchors)))
+    return [(arr[2 * i], arr[2 * i + 1]) for i in range(n)]
+
+
+def test_remove_spurious_anchors_kats(orc):
+    # src/piecewisealigner.rs:1217-1519 (pairs are (ref_start, query_start))
+    ctrl = [(i, i) for i in range(0, 101, 10)]
+    assert rsa(orc, ctrl) == ctrl
+    inside = [(0, 0), (10, 10), (20, 20), (130, 30), (140, 40), (150, 50), (60, 60), (70, 50), (80, 80), (90, 90),
+              (100, 100)]
+    assert rsa(orc, inside) == [(0, 0), (10, 10), (20, 20), (60, 60), (80, 80), (90, 90), (100, 100)]
+    ends = [(5, 0), (15, 10), (20, 20), (30, 30), (40, 40), (50, 50), (60, 60), (70, 70), (80, 80), (90, 95),
+            (100, 105)]
+    assert rsa(orc, ends) == [(i, i) for i in range(20, 81, 10)]
+
+
+def ext_pw(orc, q, r, anchors, padding=5, k=5):
+    arr = (ctypes.c_uint64 * (2 * len(anchors)))(*[x for a in anchors for x in a])
+    out = (ctypes.c_int64 * 6)()
+    buf = ctypes.create_string_buffer(1024)
+    ok = orc.orc_extend_piecewise(q, ctypes.c_uint64(len(q)), r, ctypes.c_uint64(len(r)), arr,
+                                  ctypes.c_int64(len(anchors)), ctypes.c_uint64(padding), SC, ctypes.c_uint64(k), 100,
+                                  out, buf, ctypes.c_uint64(1024))
+    return (list(out), buf.value.decode()) if ok else None
+
+
+def test_extend_piecewise_kats(orc):
+    # src/piecewisealigner.rs:1521-1711
+    q = b"A" * 50
+    assert ext_pw(orc, q, b"A" * 70, [(40, 30), (30, 20), (20, 10), (10, 0)]) == ([120, 0, 50, 10, 60, 0], "50=")
+    assert ext_pw(orc, q, b"T" * 50, [(30, 20), (20, 20), (10, 10)]) is None
+    out = ext_pw(orc, q, b"A" * 81, [(40, 30), (37, 27), (35, 25), (30, 20), (25, 20), (20, 15), (10, 5)])
+    assert out == ([50 * 2 + 20 - 12 - 4, 0, 50, 5, 60, 5], "25=5D25=")
+    q = b"CTTTTAAAAATTTTAAAAATGGTTTCAAAAATTCCTAAAAATTTTTCCCCC"
+    r = b"TTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAA"
+    out = ext_pw(orc, q, r, [(35, 36), (15, 14), (5, 5)])
+    assert out == ([(13 + 6 + 3 + 7 + 11) * 2 + 10 - 12 * 2 - 1 - 8 * 4, 0, 46, 0, 45, 7], "1X13=1D6=2I3=1X7=2X11=")
