@@ -1,0 +1,175 @@
+/*
+ * sa_hip.h -- C ABI of the MI355X (gfx950) drop-in for strobealign's read-mapping
+ * path: seeding -> index lookup -> chaining/NAMs -> extension -> SAM/PAF.
+ *
+ * Conventions (mirroring the one native boundary of the reference, the SSW FFI
+ * declared in src/ssw/raw.rs:26-51 and linked by build.rs:2):
+ *   - every function returns 0 on success and a negative code on error; the text
+ *     of the last error on the calling thread is available via sa_last_error();
+ *   - the library allocates handles and result buffers; the caller releases them
+ *     with the matching *_destroy / sa_free function;
+ *   - no C++ exception crosses this boundary.
+ * Threading: handles may be used from one thread at a time; different contexts
+ * (one per device) may be used concurrently.
+ */
+#ifndef SA_HIP_H
+#define SA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sa_ctx sa_ctx;         /* one device: stream, buffers */
+typedef struct sa_index sa_index;     /* device-resident reference + randstrobe index */
+
+#define SA_OK 0
+#define SA_ERR_ARG -1
+#define SA_ERR_IO -2
+#define SA_ERR_DEVICE -3
+#define SA_ERR_INTERNAL -4
+
+/* Text of the last error on this thread (empty if none). Returns its length. */
+size_t sa_last_error(char* buf, size_t cap);
+
+/* Library version string, e.g. "0.1.0 (gfx950)". */
+const char* sa_version(void);
+
+/* Device context. Replaces nothing in the reference (which has no device). */
+int sa_ctx_create(int device, sa_ctx** out);
+void sa_ctx_destroy(sa_ctx* ctx);
+
+/* Seeding parameters (src/seeding/parameters.rs:163-176; RandstrobeParameters,
+ * src/seeding/strobes.rs:8-23), plus the bucket bits and filter cutoff of the
+ * index (src/index.rs:68-87). */
+typedef struct sa_seeding_params {
+    int32_t k, s, t;
+    int32_t w_min, w_max, max_dist;
+    uint64_t q;
+    uint64_t main_hash_mask;
+    uint64_t forward_main_hash_mask;
+    int32_t partial_orientation_pos;
+    int32_t bits;
+    uint64_t filter_cutoff;
+} sa_seeding_params;
+
+/* SeedingParameters::new(read_length) (src/seeding/parameters.rs:181-221). */
+int sa_seeding_params_for_read_length(uint64_t read_length, sa_seeding_params* out);
+
+/* Loads a FASTA reference (src/io/fasta.rs:116-165) and builds the randstrobe
+ * index for the profile of `read_length` (src/indexer.rs:18-170, make_index;
+ * bits from pick_bits, src/indexer.rs:294-300, when bits <= 0), then uploads
+ * reference and index to the context's device. */
+int sa_index_build(sa_ctx* ctx, const char* fasta_path, uint64_t read_length, int bits, int threads,
+                   sa_index** out);
+
+/* Uploads an existing index: `entries` are n 16-byte RefRandstrobe records
+ * (src/index.rs:15-30, sorted as make_index sorts them), `bucket_starts` has
+ * 2^bits + 1 entries (src/index.rs:76-83), and the packed reference uses the
+ * PackedSeq layout (src/packed_seq.rs:1-15) with each contig starting on a
+ * 64-bit word boundary. These are the in-memory and .sti layouts of the
+ * reference (src/index.rs:374-409), so a .sti file can be streamed in. */
+int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* entries, uint64_t n,
+                    const uint64_t* bucket_starts, const uint64_t* packed_ref_words, uint64_t n_words,
+                    const uint64_t* contig_word_offsets, const uint64_t* contig_lengths,
+                    const char* const* contig_names, uint32_t n_contigs, sa_index** out);
+void sa_index_destroy(sa_index* index);
+int sa_index_info(const sa_index* index, sa_seeding_params* params, uint64_t* n_entries, uint32_t* n_contigs);
+
+/* Mapping options: the flags of src/main.rs:61-283 that the mapping path uses. */
+typedef struct sa_map_options {
+    int32_t mode;            /* 0 SAM, 1 PAF (-x), 2 abundances (--aemb) */
+    int32_t eqx;             /* --eqx */
+    int32_t details;         /* --details */
+    int32_t fastq_comments;  /* -C */
+    int32_t only_mapped;     /* -U */
+    int32_t use_ssw;         /* --ssw */
+    const char* rg_id;       /* --rg-id (NULL: none) */
+    const char* const* rg;   /* --rg fields */
+    int32_t n_rg;
+    uint32_t max_secondary;  /* -N */
+    float dropoff;           /* -S (0.5) */
+    uint32_t rescue_distance;/* -R (100) */
+    uint32_t max_tries;      /* -M (20) */
+    uint32_t mcs;            /* --mcs: 0 always, 1 rescue, 2 off, 3 first-strobe */
+    uint64_t chunk_size;     /* --chunk-size (1000000) */
+    int32_t threads;         /* host threads for the decision replay and output */
+    uint64_t batch_bases;    /* bases per device batch (0: default) */
+} sa_map_options;
+
+/* Fills `opt` with the reference defaults. */
+void sa_map_options_default(sa_map_options* opt);
+
+/* Maps FASTQ/FASTA files (reads2 may be NULL for single-end) and returns the
+ * complete output (SAM with header but without @PG, PAF, or the abundance
+ * table) in *out, which the caller releases with sa_free. Replaces the per-chunk
+ * Mapper::map_chunk loop of src/main.rs:873-988 with its calls into
+ * align_single_end_read / align_paired_end_read (src/mapper.rs:349, 624) and
+ * map_*_end_read / abundances_* (src/maponly.rs:18, 57, 114, 198). */
+int sa_map_files(sa_index* index, const sa_map_options* opt, const char* reads1, const char* reads2, char** out,
+                 size_t* out_len);
+
+/* Same for records held in memory (uppercase or lowercase ASCII sequences,
+ * qualities may be NULL). r2 arrays are NULL for single-end input. */
+int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const char* const* names1,
+                   const char* const* seqs1, const char* const* quals1, const char* const* names2,
+                   const char* const* seqs2, const char* const* quals2, char** out, size_t* out_len);
+
+/* Per-stage probes (parity tests, and a finer-grained drop-in for a caller that
+ * keeps its own decision logic). Reads are concatenated ASCII with offsets[n+1]. */
+
+/* QueryRandstrobe (src/seeding/mod.rs:10-16). */
+typedef struct sa_qrand {
+    uint64_t hash, hash_revcomp;
+    uint32_t start, end;
+} sa_qrand;
+
+/* randstrobes_query (src/seeding/mod.rs:19-75) for n reads. out_fwd/out_rc are
+ * sized offsets[n] each; read i's entries start at offsets[i]; counts[2i],
+ * counts[2i+1] receive the forward / reverse-complement counts. */
+int sa_randstrobes_query_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
+                               sa_qrand* out_fwd, sa_qrand* out_rc, uint32_t* counts);
+
+/* NAM (src/nam.rs:18-31) as produced by get_nams_by_chaining (src/nam.rs:138-166)
+ * and the stable score sort of sort_nams (src/nam.rs:170), before the per-chunk
+ * shuffle of the tied best (src/shuffle.rs:7-25, which needs the caller's RNG). */
+typedef struct sa_nam {
+    uint32_t nam_id, ref_id;
+    uint32_t ref_start, ref_end;
+    uint32_t query_start, query_end;
+    uint32_t matching_bases, n_anchors;
+    float score;
+    uint32_t flags; /* bit0 is_revcomp, bit1 is_consistent, bits2-3 reverse_nam_if_needed outcome */
+    uint64_t anchor_offset;
+} sa_nam;
+
+/* NAMs of n reads; *nams (released with sa_free) holds nam_offsets[n] records,
+ * read i's at [nam_offsets[i], nam_offsets[i+1]). mcs as in sa_map_options. */
+int sa_find_nams_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, uint32_t mcs,
+                       uint32_t rescue_distance, sa_nam** nams, uint64_t* nam_offsets);
+
+/* Aligner::align (src/aligner.rs:80-151: SSW of ext/ssw/ssw.c plus the end-bonus
+ * extension) of n (query, reference) pairs. Queries/references are ASCII,
+ * concatenated with offsets. Each result: found, score, query_start, query_end,
+ * ref_start, ref_end, edit_distance; CIGARs (BAM-encoded len<<4|op with =/X)
+ * are concatenated in *cigars with cigar_offsets[n+1]. */
+typedef struct sa_alignment {
+    uint32_t found, score, query_start, query_end, ref_start, ref_end, edit_distance, n_cigar;
+} sa_alignment;
+int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
+                       const uint64_t* ref_offsets, uint32_t n, sa_alignment* out, uint32_t** cigars,
+                       uint64_t* cigar_offsets);
+
+/* Kernel time (ms) spent by the index's engine since creation, and the number
+ * of device launches; for measurement. */
+int sa_index_stats(const sa_index* index, double* kernel_ms);
+
+void sa_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SA_HIP_H */

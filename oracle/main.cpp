@@ -3,6 +3,7 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -28,7 +29,7 @@ int main(int argc, char** argv) {
     MappingParams mp;
     SamOpts so;
     std::vector<std::string> rg_fields;
-    std::string out_path, dump_index;
+    std::string out_path, dump_index, dump_nams;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -59,6 +60,7 @@ int main(int argc, char** argv) {
             mp.mcs = m == "rescue" ? Mcs::Rescue : m == "off" ? Mcs::Off : m == "first-strobe" ? Mcs::FirstStrobe : Mcs::Always;
         } else if (a == "-o") out_path = val("-o");
         else if (is("--dump-index")) dump_index = val("--dump-index");
+        else if (is("--dump-nams")) dump_nams = val("--dump-nams");
         else if (!a.empty() && a[0] == '-' && a.size() > 1) { usage(); return 2; }
         else pos.push_back(a);
     }
@@ -85,6 +87,26 @@ int main(int argc, char** argv) {
         f.write((const char*)index.bucket_starts.data(), (std::streamsize)(nb * 8));
     }
 
+    if (!dump_nams.empty()) {
+        // NAMs of every read after the stable score sort, before the shuffle of the tied best
+        FILE* f = std::fopen(dump_nams.c_str(), "wb");
+        ChainingParams cp;
+        for (size_t i = 0; i < r1.size(); ++i) {
+            NamDetails nd;
+            std::vector<Nam> nams = get_nams_by_chaining((const uint8_t*)r1[i].seq.data(), r1[i].seq.size(), index, cp,
+                                                         mp.rescue_distance, mp.mcs, nd);
+            std::stable_sort(nams.begin(), nams.end(), [](const Nam& a, const Nam& b) { return a.score > b.score; });
+            for (const Nam& n : nams) {
+                uint32_t sb;
+                std::memcpy(&sb, &n.score, 4);
+                std::fprintf(f, "%zu\t%zu\t%zu\t%zu\t%zu\t%zu\t%zu\t%zu\t%zu\t%08x\t%u\n", i, n.nam_id, n.ref_id,
+                             n.ref_start, n.ref_end, n.query_start, n.query_end, n.matching_bases, n.anchors.size(), sb,
+                             (unsigned)n.is_revcomp);
+            }
+        }
+        std::fclose(f);
+        return 0;
+    }
     Mapper m;
     m.index = &index;
     m.refs = &refs;

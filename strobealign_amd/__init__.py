@@ -1,0 +1,223 @@
+"""MI355X (gfx950) drop-in for strobealign's read-mapping path.
+
+The mapping path -- randstrobe seeding, index lookups, hit filtering and
+rescue, anchor chaining into NAMs, and SSW / piecewise extension -- runs as HIP
+kernels in ``lib/libsa_hip.so``; the host library replays the reference's
+per-chunk decisions and writes SAM, PAF or abundance output. This module is a
+thin ctypes binding of the C ABI in ``include/sa_hip.h``; it raises instead of
+falling back when the library is missing or no GPU is present.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsa_hip.so")
+CLI_PATH = os.path.join(HERE, "bin", "strobealign-hip")
+CSRC = os.path.join(HERE, "csrc")
+
+_lib = None
+
+
+class SaError(RuntimeError):
+    pass
+
+
+class SeedingParams(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int32), ("s", ctypes.c_int32), ("t", ctypes.c_int32), ("w_min", ctypes.c_int32),
+                ("w_max", ctypes.c_int32), ("max_dist", ctypes.c_int32), ("q", ctypes.c_uint64),
+                ("main_hash_mask", ctypes.c_uint64), ("forward_main_hash_mask", ctypes.c_uint64),
+                ("partial_orientation_pos", ctypes.c_int32), ("bits", ctypes.c_int32),
+                ("filter_cutoff", ctypes.c_uint64)]
+
+
+class MapOptions(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("eqx", ctypes.c_int32), ("details", ctypes.c_int32),
+                ("fastq_comments", ctypes.c_int32), ("only_mapped", ctypes.c_int32), ("use_ssw", ctypes.c_int32),
+                ("rg_id", ctypes.c_char_p), ("rg", ctypes.POINTER(ctypes.c_char_p)), ("n_rg", ctypes.c_int32),
+                ("max_secondary", ctypes.c_uint32), ("dropoff", ctypes.c_float),
+                ("rescue_distance", ctypes.c_uint32), ("max_tries", ctypes.c_uint32), ("mcs", ctypes.c_uint32),
+                ("chunk_size", ctypes.c_uint64), ("threads", ctypes.c_int32), ("batch_bases", ctypes.c_uint64)]
+
+
+class QRand(ctypes.Structure):
+    _fields_ = [("hash", ctypes.c_uint64), ("hash_revcomp", ctypes.c_uint64), ("start", ctypes.c_uint32),
+                ("end", ctypes.c_uint32)]
+
+
+class Nam(ctypes.Structure):
+    _fields_ = [("nam_id", ctypes.c_uint32), ("ref_id", ctypes.c_uint32), ("ref_start", ctypes.c_uint32),
+                ("ref_end", ctypes.c_uint32), ("query_start", ctypes.c_uint32), ("query_end", ctypes.c_uint32),
+                ("matching_bases", ctypes.c_uint32), ("n_anchors", ctypes.c_uint32), ("score", ctypes.c_float),
+                ("flags", ctypes.c_uint32), ("anchor_offset", ctypes.c_uint64)]
+
+
+class Alignment(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("found", "score", "query_start", "query_end", "ref_start", "ref_end", "edit_distance", "n_cigar")]
+
+
+EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
+           "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
+           "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
+           "sa_align_ssw_batch", "sa_index_stats", "sa_free"]
+
+
+def load(path=LIB_PATH):
+    """Loads the HIP library (raises SaError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SaError("libsa_hip.so not built (run __graft_entry__.build() or make -C strobealign_amd/csrc)")
+    lib = ctypes.CDLL(path)
+    lib.sa_last_error.restype = ctypes.c_size_t
+    lib.sa_version.restype = ctypes.c_char_p
+    lib.sa_map_options_default.restype = None
+    lib.sa_index_destroy.restype = None
+    lib.sa_ctx_destroy.restype = None
+    lib.sa_free.restype = None
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(4096)
+        _lib.sa_last_error(buf, ctypes.c_size_t(4096))
+        raise SaError("sa_hip error %d: %s" % (rc, buf.value.decode(errors="replace")))
+
+
+class Context:
+    """A device context (one GPU)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        self._p = ctypes.c_void_p()
+        _check(lib.sa_ctx_create(device, ctypes.byref(self._p)))
+
+    def close(self):
+        if self._p:
+            _lib.sa_ctx_destroy(self._p)
+            self._p = None
+
+    def align_ssw(self, pairs):
+        """Aligner::align on the device for [(query, ref)] byte strings; returns
+        [(found, score, qs, qe, rs, re, edit_distance, cigar_ops)]."""
+        q = b"".join(p[0] for p in pairs)
+        r = b"".join(p[1] for p in pairs)
+        qo = [0]
+        ro = [0]
+        for a, b in pairs:
+            qo.append(qo[-1] + len(a))
+            ro.append(ro[-1] + len(b))
+        n = len(pairs)
+        out = (Alignment * max(1, n))()
+        cig = ctypes.POINTER(ctypes.c_uint32)()
+        coff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_align_ssw_batch(self._p, q, (ctypes.c_uint64 * (n + 1))(*qo), r,
+                                       (ctypes.c_uint64 * (n + 1))(*ro), ctypes.c_uint32(n), out,
+                                       ctypes.byref(cig), coff))
+        res = []
+        for i in range(n):
+            a = out[i]
+            ops = [cig[x] for x in range(coff[i], coff[i + 1])]
+            res.append((a.found, a.score, a.query_start, a.query_end, a.ref_start, a.ref_end, a.edit_distance, ops))
+        _lib.sa_free(cig)
+        return res
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Index:
+    """Device-resident reference + randstrobe index (make_index, src/indexer.rs:18-170)."""
+
+    def __init__(self, ctx, fasta, read_length, bits=0, threads=4):
+        self._p = ctypes.c_void_p()
+        _check(_lib.sa_index_build(ctx._p, fasta.encode(), ctypes.c_uint64(read_length), bits, threads,
+                                   ctypes.byref(self._p)))
+
+    def close(self):
+        if self._p:
+            _lib.sa_index_destroy(self._p)
+            self._p = None
+
+    def params(self):
+        p = SeedingParams()
+        n = ctypes.c_uint64()
+        c = ctypes.c_uint32()
+        _check(_lib.sa_index_info(self._p, ctypes.byref(p), ctypes.byref(n), ctypes.byref(c)))
+        return p, n.value, c.value
+
+    def map_files(self, reads1, reads2=None, mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
+                  use_ssw=False, max_secondary=0, mcs=0, details=False, fastq_comments=False, only_mapped=False,
+                  batch_bases=0):
+        o = MapOptions()
+        _lib.sa_map_options_default(ctypes.byref(o))
+        o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
+        o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
+        o.batch_bases = batch_bases
+        if rg_id is not None:
+            o.rg_id = rg_id.encode()
+        rgs = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])
+        o.rg, o.n_rg = rgs, len(rg)
+        out = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _check(_lib.sa_map_files(self._p, ctypes.byref(o), reads1.encode(),
+                                 reads2.encode() if reads2 else None, ctypes.byref(out), ctypes.byref(n)))
+        s = ctypes.string_at(out, n.value)
+        _lib.sa_free(out)
+        return s
+
+    @staticmethod
+    def _pack(seqs):
+        data = b"".join(seqs)
+        off = [0]
+        for s in seqs:
+            off.append(off[-1] + len(s))
+        return data, (ctypes.c_uint64 * len(off))(*off)
+
+    def randstrobes(self, seqs):
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        tot = max(1, off[n])
+        fwd, rc = (QRand * tot)(), (QRand * tot)()
+        counts = (ctypes.c_uint32 * (2 * n + 1))()
+        _check(_lib.sa_randstrobes_query_batch(self._p, data, off, ctypes.c_uint32(n), fwd, rc, counts))
+        res = []
+        for i in range(n):
+            o = off[i]
+            res.append(([(fwd[o + x].hash, fwd[o + x].hash_revcomp, fwd[o + x].start, fwd[o + x].end)
+                         for x in range(counts[2 * i])],
+                        [(rc[o + x].hash, rc[o + x].hash_revcomp, rc[o + x].start, rc[o + x].end)
+                         for x in range(counts[2 * i + 1])]))
+        return res
+
+    def nams(self, seqs, mcs=0, rescue_distance=100):
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        p = ctypes.POINTER(Nam)()
+        noff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_find_nams_batch(self._p, data, off, ctypes.c_uint32(n), mcs, rescue_distance,
+                                       ctypes.byref(p), noff))
+        res = []
+        for i in range(n):
+            res.append([(p[x].nam_id, p[x].ref_id, p[x].ref_start, p[x].ref_end, p[x].query_start, p[x].query_end,
+                         p[x].matching_bases, p[x].n_anchors, p[x].score, p[x].flags & 1)
+                        for x in range(noff[i], noff[i + 1])])
+        _lib.sa_free(p)
+        return res
+
+    def kernel_ms(self):
+        v = ctypes.c_double()
+        _check(_lib.sa_index_stats(self._p, ctypes.byref(v)))
+        return v.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

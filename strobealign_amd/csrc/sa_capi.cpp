@@ -1,0 +1,339 @@
+// extern "C" entry points declared in include/sa_hip.h.
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "sa_hip.h"
+#include "sa_host.hpp"
+
+struct sa_ctx {
+    int device;
+};
+
+struct sa_index {
+    int device;
+    std::unique_ptr<sa::Engine> eng;
+    sa::PackedRef ref;
+    sa::SeedParams p;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+template <typename F>
+int guard(F f) {
+    try {
+        g_err.clear();
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(SA_ERR_INTERNAL, "out of memory");
+    } catch (const std::exception& e) {
+        std::string m = e.what();
+        return fail(m.rfind("HIP error", 0) == 0 ? SA_ERR_DEVICE : SA_ERR_INTERNAL, m);
+    }
+}
+
+sa::SeedParams from_c(const sa_seeding_params& c) {
+    sa::SeedParams p{};
+    p.k = c.k; p.s = c.s; p.t = c.t; p.w_min = c.w_min; p.w_max = c.w_max; p.max_dist = c.max_dist; p.q = c.q;
+    p.main_mask = c.main_hash_mask; p.fwd_main_mask = c.forward_main_hash_mask;
+    p.partial_pos = c.partial_orientation_pos; p.bits = c.bits; p.filter_cutoff = c.filter_cutoff;
+    return p;
+}
+
+void to_c(const sa::SeedParams& p, sa_seeding_params* c) {
+    c->k = p.k; c->s = p.s; c->t = p.t; c->w_min = p.w_min; c->w_max = p.w_max; c->max_dist = p.max_dist; c->q = p.q;
+    c->main_hash_mask = p.main_mask; c->forward_main_hash_mask = p.fwd_main_mask;
+    c->partial_orientation_pos = p.partial_pos; c->bits = p.bits; c->filter_cutoff = p.filter_cutoff;
+}
+
+sa::Options options_of(const sa_map_options* o) {
+    sa::Options r;
+    if (!o) return r;
+    r.mode = o->mode;
+    r.eqx = o->eqx; r.details = o->details; r.fastq_comments = o->fastq_comments; r.only_mapped = o->only_mapped;
+    r.use_ssw = o->use_ssw;
+    if (o->rg_id) { r.has_rg = true; r.rg_id = o->rg_id; }
+    for (int i = 0; i < o->n_rg; ++i) r.rg_fields.push_back(o->rg[i]);
+    if (!r.rg_fields.empty() && !r.has_rg) { r.has_rg = true; r.rg_id = "1"; }
+    r.max_secondary = o->max_secondary; r.dropoff = o->dropoff; r.rescue_distance = o->rescue_distance;
+    r.max_tries = o->max_tries; r.mcs = o->mcs; r.chunk_size = o->chunk_size;
+    r.threads = o->threads > 0 ? o->threads : 1;
+    if (o->batch_bases) r.batch_bases = o->batch_bases;
+    return r;
+}
+
+char* dup_out(const std::string& s, size_t* len) {
+    char* p = (char*)std::malloc(s.size() + 1);
+    if (!p) throw std::bad_alloc();
+    std::memcpy(p, s.data(), s.size());
+    p[s.size()] = 0;
+    *len = s.size();
+    return p;
+}
+}  // namespace
+
+extern "C" {
+
+size_t sa_last_error(char* buf, size_t cap) {
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, g_err.size());
+        std::memcpy(buf, g_err.data(), n);
+        buf[n] = 0;
+    }
+    return g_err.size();
+}
+
+const char* sa_version(void) { return "0.1.0 (gfx950)"; }
+
+int sa_ctx_create(int device, sa_ctx** out) {
+    if (!out) return fail(SA_ERR_ARG, "out is NULL");
+    return guard([&] {
+        // validates the device by creating (and dropping) an engine on it
+        sa::Engine probe(device);
+        *out = new sa_ctx{device};
+        return SA_OK;
+    });
+}
+
+void sa_ctx_destroy(sa_ctx* ctx) { delete ctx; }
+
+int sa_seeding_params_for_read_length(uint64_t read_length, sa_seeding_params* out) {
+    if (!out) return fail(SA_ERR_ARG, "out is NULL");
+    to_c(sa::seed_params_for_read_length(read_length), out);
+    return SA_OK;
+}
+
+int sa_index_build(sa_ctx* ctx, const char* fasta_path, uint64_t read_length, int bits, int threads, sa_index** out) {
+    if (!ctx || !fasta_path || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        auto ix = std::make_unique<sa_index>();
+        ix->device = ctx->device;
+        try {
+            ix->ref = sa::read_fasta(fasta_path);
+        } catch (const std::exception& e) {
+            return fail(SA_ERR_IO, e.what());
+        }
+        sa::SeedParams sp = sa::seed_params_for_read_length(read_length);
+        if (bits <= 0) bits = sa::pick_bits(ix->ref, sp);
+        sa::HostIndex hi;
+        sa::build_index(ix->ref, sp, bits, 0.0002, threads > 0 ? threads : 1, hi);
+        ix->p = hi.p;
+        ix->eng = std::make_unique<sa::Engine>(ctx->device);
+        ix->eng->upload_reference(ix->ref.words.data(), ix->ref.words.size(), ix->ref.word_off.data(),
+                                  ix->ref.len.data(), (uint32_t)ix->ref.names.size());
+        ix->eng->upload_index(hi.entries.data(), hi.entries.size(), hi.bucket_starts.data(), hi.p);
+        *out = ix.release();
+        return SA_OK;
+    });
+}
+
+int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* entries, uint64_t n,
+                    const uint64_t* bucket_starts, const uint64_t* packed_ref_words, uint64_t n_words,
+                    const uint64_t* contig_word_offsets, const uint64_t* contig_lengths,
+                    const char* const* contig_names, uint32_t n_contigs, sa_index** out) {
+    if (!ctx || !params || (!entries && n) || !bucket_starts || !packed_ref_words || !contig_word_offsets ||
+        !contig_lengths || !out)
+        return fail(SA_ERR_ARG, "NULL argument");
+    if (params->bits < 1 || params->bits > 32) return fail(SA_ERR_ARG, "bits must be in 1..32");
+    return guard([&] {
+        auto ix = std::make_unique<sa_index>();
+        ix->device = ctx->device;
+        ix->p = from_c(*params);
+        ix->p.n_entries = n;
+        ix->ref.words.assign(packed_ref_words, packed_ref_words + n_words);
+        ix->ref.word_off.assign(contig_word_offsets, contig_word_offsets + n_contigs + 1);
+        ix->ref.len.assign(contig_lengths, contig_lengths + n_contigs);
+        for (uint32_t i = 0; i < n_contigs; ++i)
+            ix->ref.names.push_back(contig_names ? contig_names[i] : ("contig" + std::to_string(i)));
+        ix->eng = std::make_unique<sa::Engine>(ctx->device);
+        ix->eng->upload_reference(packed_ref_words, n_words, contig_word_offsets, contig_lengths, n_contigs);
+        ix->eng->upload_index((const sa::RefEntry*)entries, n, bucket_starts, ix->p);
+        *out = ix.release();
+        return SA_OK;
+    });
+}
+
+void sa_index_destroy(sa_index* index) { delete index; }
+
+int sa_index_info(const sa_index* index, sa_seeding_params* params, uint64_t* n_entries, uint32_t* n_contigs) {
+    if (!index) return fail(SA_ERR_ARG, "index is NULL");
+    if (params) to_c(index->p, params);
+    if (n_entries) *n_entries = index->eng->n_entries();
+    if (n_contigs) *n_contigs = (uint32_t)index->ref.names.size();
+    return SA_OK;
+}
+
+void sa_map_options_default(sa_map_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->dropoff = 0.5f;
+    o->rescue_distance = 100;
+    o->max_tries = 20;
+    o->chunk_size = 1000000;
+    o->threads = 1;
+}
+
+int sa_map_files(sa_index* index, const sa_map_options* opt, const char* reads1, const char* reads2, char** out,
+                 size_t* out_len) {
+    if (!index || !reads1 || !out || !out_len) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        std::string err;
+        std::vector<sa::Record> r1, r2;
+        if (!sa::read_records(reads1, r1, err)) return fail(SA_ERR_IO, err);
+        if (reads2 && !sa::read_records(reads2, r2, err)) return fail(SA_ERR_IO, err);
+        sa::Options o = options_of(opt);
+        std::string s;
+        if (o.mode == 0) s = sa::sam_header(index->ref, o);
+        sa::map_records(*index->eng, index->ref, r1, r2, o, s, nullptr);
+        *out = dup_out(s, out_len);
+        return SA_OK;
+    });
+}
+
+int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const char* const* names1,
+                   const char* const* seqs1, const char* const* quals1, const char* const* names2,
+                   const char* const* seqs2, const char* const* quals2, char** out, size_t* out_len) {
+    if (!index || !names1 || !seqs1 || !out || !out_len) return fail(SA_ERR_ARG, "NULL argument");
+    if ((names2 == nullptr) != (seqs2 == nullptr)) return fail(SA_ERR_ARG, "names2 and seqs2 must both be set");
+    return guard([&] {
+        auto mk = [](const char* name, const char* seq, const char* qual) {
+            sa::Record r;
+            std::string h = name;
+            size_t p = h.find_first_of(" \t");
+            if (p == std::string::npos) r.name = h;
+            else { r.name = h.substr(0, p); r.comment = h.substr(p + 1); r.has_comment = true; }
+            r.seq = seq;
+            for (char& c : r.seq) c = (char)toupper((unsigned char)c);
+            if (qual) r.qual = qual; else r.has_qual = false;
+            return r;
+        };
+        std::vector<sa::Record> r1, r2;
+        for (uint64_t i = 0; i < n; ++i) {
+            r1.push_back(mk(names1[i], seqs1[i], quals1 ? quals1[i] : nullptr));
+            if (seqs2) r2.push_back(mk(names2[i], seqs2[i], quals2 ? quals2[i] : nullptr));
+        }
+        sa::Options o = options_of(opt);
+        std::string s;
+        sa::map_records(*index->eng, index->ref, r1, r2, o, s, nullptr);
+        *out = dup_out(s, out_len);
+        return SA_OK;
+    });
+}
+
+int sa_randstrobes_query_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
+                               sa_qrand* out_fwd, sa_qrand* out_rc, uint32_t* counts) {
+    if (!index || (!seqs && n) || !offsets || !out_fwd || !out_rc || !counts) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        index->eng->upload_reads(seqs, offsets, n);
+        std::vector<sa::QRand> qr;
+        std::vector<uint32_t> cnt;
+        index->eng->randstrobes(qr, cnt);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t o = offsets[i], L = offsets[i + 1] - o;
+            counts[2 * i] = cnt[2 * i];
+            counts[2 * i + 1] = cnt[2 * i + 1];
+            for (uint32_t x = 0; x < cnt[2 * i]; ++x) std::memcpy(&out_fwd[o + x], &qr[2 * o + x], sizeof(sa_qrand));
+            for (uint32_t x = 0; x < cnt[2 * i + 1]; ++x) std::memcpy(&out_rc[o + x], &qr[2 * o + L + x], sizeof(sa_qrand));
+        }
+        return SA_OK;
+    });
+}
+
+int sa_find_nams_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, uint32_t mcs,
+                       uint32_t rescue_distance, sa_nam** nams, uint64_t* nam_offsets) {
+    if (!index || (!seqs && n) || !offsets || !nams || !nam_offsets) return fail(SA_ERR_ARG, "NULL argument");
+    if (mcs > 3) return fail(SA_ERR_ARG, "mcs must be 0..3");
+    return guard([&] {
+        index->eng->upload_reads(seqs, offsets, n);
+        sa::NamBatch nb;
+        sa::MapParams mp{mcs, rescue_distance};
+        sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+        index->eng->find_nams(mp, cp, nb);
+        static_assert(sizeof(sa_nam) == sizeof(sa::NamRec), "sa_nam layout");
+        sa_nam* p = (sa_nam*)std::malloc(std::max<size_t>(1, nb.nams.size()) * sizeof(sa_nam));
+        if (!p) throw std::bad_alloc();
+        if (!nb.nams.empty()) std::memcpy(p, nb.nams.data(), nb.nams.size() * sizeof(sa_nam));
+        for (uint32_t i = 0; i <= n; ++i) nam_offsets[i] = nb.nam_off[i];
+        *nams = p;
+        return SA_OK;
+    });
+}
+
+int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
+                       const uint64_t* ref_offsets, uint32_t n, sa_alignment* out, uint32_t** cigars,
+                       uint64_t* cigar_offsets) {
+    if (!ctx || !query_offsets || !ref_offsets || !out || !cigars || !cigar_offsets) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        // references are packed as one contig per pair (ACGT only)
+        std::vector<uint64_t> words, woff{0}, len;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint64_t L = ref_offsets[i + 1] - ref_offsets[i];
+            for (uint64_t x = 0; x < L; ++x) {
+                uint32_t c = sa::nt2(refs[ref_offsets[i] + x]);
+                if (c > 3) { throw std::invalid_argument("reference windows must be ACGT"); }
+                if ((x & 31) == 0) words.push_back(0);
+                words.back() |= (uint64_t)c << ((x & 31) * 2);
+            }
+            woff.push_back(words.size());
+            len.push_back(L);
+        }
+        sa::Engine eng(ctx->device);
+        eng.upload_reference(words.data(), words.size(), woff.data(), len.data(), n);
+        sa::SeedParams sp = sa::seed_params_for_read_length(150);
+        uint64_t bs[3] = {0, 0, 0};
+        sp.bits = 1;
+        eng.upload_index(nullptr, 0, bs, sp);
+        eng.upload_reads(queries, query_offsets, n);
+        std::vector<sa::ExtRequest> req(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            sa::ExtRequest& r = req[i];
+            std::memset(&r, 0, sizeof r);
+            r.kind = sa::EXT_RESCUE;
+            r.read_id = i;
+            r.flags = 8;  // plain alignment: no shared-substring precheck, query as given
+            r.ref_id = i;
+            r.win_start = 0;
+            r.win_end = (uint32_t)len[i];
+        }
+        std::vector<sa::ExtResult> res;
+        std::vector<uint32_t> cig;
+        std::vector<uint64_t> coff;
+        sa::Scores sc{2, 8, 12, 1, 10, 500};
+        eng.extend(req.data(), n, sc, res, cig, coff);
+        for (uint32_t i = 0; i < n; ++i) {
+            out[i].found = res[i].found;
+            out[i].score = res[i].score;
+            out[i].query_start = res[i].query_start;
+            out[i].query_end = res[i].query_end;
+            out[i].ref_start = res[i].ref_start;
+            out[i].ref_end = res[i].ref_start + res[i].length;
+            out[i].edit_distance = res[i].edit_distance;
+            out[i].n_cigar = res[i].found ? res[i].n_cigar : 0;
+            cigar_offsets[i] = coff[i];
+        }
+        cigar_offsets[n] = coff[n];
+        uint32_t* p = (uint32_t*)std::malloc(std::max<size_t>(1, cig.size()) * 4);
+        if (!p) throw std::bad_alloc();
+        if (!cig.empty()) std::memcpy(p, cig.data(), cig.size() * 4);
+        *cigars = p;
+        return SA_OK;
+    });
+}
+
+int sa_index_stats(const sa_index* index, double* kernel_ms) {
+    if (!index) return fail(SA_ERR_ARG, "index is NULL");
+    if (kernel_ms) *kernel_ms = index->eng->total_kernel_ms();
+    return SA_OK;
+}
+
+void sa_free(void* p) { std::free(p); }
+
+}  // extern "C"

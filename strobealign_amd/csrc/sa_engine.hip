@@ -1,0 +1,340 @@
+// Device engine: buffer management, kernel launches and the device-side scans
+// that size each stage's output.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "sa_engine.hpp"
+#include "sa_kernels.hip"
+
+#define SA_CHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess)                                                                        \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " +   \
+                                     __FILE__ + ":" + std::to_string(__LINE__) + " (" #x ")");       \
+    } while (0)
+
+namespace sa {
+
+namespace {
+__global__ void k_pair_sum(const uint32_t* __restrict__ a, uint64_t* __restrict__ out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint64_t)a[2 * i] + a[2 * i + 1];
+}
+__global__ void k_ncigar(const ExtResult* __restrict__ r, uint64_t* __restrict__ out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = r[i].found ? r[i].n_cigar : 0;
+}
+__global__ void k_compact_cigars(const ExtRequest* __restrict__ req, const ExtResult* __restrict__ res,
+                                 const uint32_t* __restrict__ slots, const uint64_t* __restrict__ coff,
+                                 uint32_t* __restrict__ out, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !res[i].found) return;
+    const uint32_t* s = slots + req[i].cigar_off;
+    uint32_t* d = out + coff[i];
+    for (uint32_t x = 0; x < res[i].n_cigar; ++x) d[x] = s[x];
+}
+
+inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+enum Slot {
+    S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_DP, S_PRED, S_CAND, S_USED,
+    S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_COUNT
+};
+}  // namespace
+
+Engine::Engine(int device) : device_(device) {
+    int n = 0;
+    SA_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw std::runtime_error("no HIP device " + std::to_string(device));
+    SA_CHECK(hipSetDevice(device));
+    SA_CHECK(hipStreamCreate(&stream_));
+    buf_.assign(S_COUNT, nullptr);
+    cap_.assign(S_COUNT, 0);
+}
+
+Engine::~Engine() {
+    hipSetDevice(device_);
+    for (void* p : buf_)
+        if (p) hipFree(p);
+    for (void* p : {(void*)d_entries_, (void*)d_buckets_, (void*)d_words_, (void*)d_word_off_, (void*)d_len_,
+                    (void*)d_log2_, (void*)d_same_diag_})
+        if (p) hipFree(p);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+void* Engine::grow(int slot, uint64_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (cap_[slot] < bytes) {
+        if (buf_[slot]) SA_CHECK(hipFree(buf_[slot]));
+        uint64_t want = bytes + bytes / 4;
+        SA_CHECK(hipMalloc(&buf_[slot], want));
+        cap_[slot] = want;
+    }
+    return buf_[slot];
+}
+
+void Engine::scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total) {
+    if (n == 0) { *total = 0; return; }
+    size_t tmp = 0;
+    SA_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, stream_));
+    void* t = grow(S_SCANTMP, tmp);
+    SA_CHECK(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, out, (int)n, stream_));
+    uint64_t last[2];
+    SA_CHECK(hipMemcpyAsync(&last[0], in + n - 1, 8, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(&last[1], out + n - 1, 8, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+    *total = last[0] + last[1];
+}
+
+void Engine::upload_reference(const uint64_t* words, uint64_t n_words, const uint64_t* word_off, const uint64_t* len,
+                              uint32_t n_contigs) {
+    SA_CHECK(hipSetDevice(device_));
+    SA_CHECK(hipMalloc(&d_words_, std::max<uint64_t>(n_words, 1) * 8));
+    SA_CHECK(hipMalloc(&d_word_off_, (uint64_t)(n_contigs + 1) * 8));
+    SA_CHECK(hipMalloc(&d_len_, std::max<uint64_t>(n_contigs, 1) * 8));
+    SA_CHECK(hipMemcpy(d_words_, words, n_words * 8, hipMemcpyHostToDevice));
+    SA_CHECK(hipMemcpy(d_word_off_, word_off, (uint64_t)(n_contigs + 1) * 8, hipMemcpyHostToDevice));
+    SA_CHECK(hipMemcpy(d_len_, len, (uint64_t)n_contigs * 8, hipMemcpyHostToDevice));
+    n_contigs_ = n_contigs;
+}
+
+void Engine::upload_index(const RefEntry* entries, uint64_t n, const uint64_t* bucket_starts, const SeedParams& p) {
+    SA_CHECK(hipSetDevice(device_));
+    p_ = p;
+    p_.n_entries = n;
+    n_entries_ = n;
+    uint64_t nb = (1ull << p.bits) + 1;
+    SA_CHECK(hipMalloc(&d_entries_, std::max<uint64_t>(n, 1) * sizeof(RefEntry)));
+    SA_CHECK(hipMalloc(&d_buckets_, nb * 8));
+    if (n) SA_CHECK(hipMemcpy(d_entries_, entries, n * sizeof(RefEntry), hipMemcpyHostToDevice));
+    SA_CHECK(hipMemcpy(d_buckets_, bucket_starts, nb * 8, hipMemcpyHostToDevice));
+    // log2(d + 1) for the chaining gap penalty, from the host libm (src/chainer.rs:294-298)
+    std::vector<float> lg(65536);
+    for (size_t d = 0; d < lg.size(); ++d) lg[d] = std::log2((float)d + 1.0f);
+    SA_CHECK(hipMalloc(&d_log2_, lg.size() * sizeof(float)));
+    SA_CHECK(hipMemcpy(d_log2_, lg.data(), lg.size() * sizeof(float), hipMemcpyHostToDevice));
+    SA_CHECK(hipMalloc(&d_same_diag_, 1024 * sizeof(float)));
+}
+
+void Engine::upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_reads) {
+    SA_CHECK(hipSetDevice(device_));
+    n_reads_ = n_reads;
+    h_off_.assign(off, off + n_reads + 1);
+    n_bases_ = h_off_[n_reads];
+    max_len_ = 0;
+    for (uint32_t i = 0; i < n_reads; ++i) max_len_ = std::max<uint32_t>(max_len_, (uint32_t)(off[i + 1] - off[i]));
+    if (max_len_ >= 65536) throw std::runtime_error("reads longer than 65535 bp are not supported");
+    void* s = grow(S_SEQ, n_bases_ + 16);
+    void* o = grow(S_OFF, (uint64_t)(n_reads + 1) * 8);
+    if (n_bases_) SA_CHECK(hipMemcpyAsync(s, seq, n_bases_, hipMemcpyHostToDevice, stream_));
+    SA_CHECK(hipMemcpyAsync(o, off, (uint64_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) {
+    const uint32_t n = n_reads_;
+    auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    auto* d_off = (const uint64_t*)buf_[S_OFF];
+    auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
+    auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
+    auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
+    if (n) k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
+    SA_CHECK(hipGetLastError());
+    qr.resize(2 * n_bases_);
+    counts.resize(2ull * n);
+    if (n_bases_) SA_CHECK(hipMemcpyAsync(qr.data(), d_qr, 2 * n_bases_ * sizeof(QRand), hipMemcpyDeviceToHost, stream_));
+    if (n) SA_CHECK(hipMemcpyAsync(counts.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint32_t n = n_reads_;
+    out.nams.clear();
+    out.nam_off.assign(1, 0);
+    out.info.clear();
+    if (n == 0) return;
+    // same-diagonal score table (src/chainer.rs:63-74), host f32 arithmetic
+    const uint32_t k = (uint32_t)p_.k;
+    if (tab_k_ != k || std::memcmp(&tab_cp_, &cp, sizeof cp) != 0) {
+        std::vector<float> lg(1025);
+        for (size_t d = 0; d < lg.size(); ++d) lg[d] = std::log2((float)d + 1.0f);
+        std::vector<float> sd(1024);
+        for (uint32_t i = 0; i < 1024; ++i) sd[i] = chain_score(i, i, k, cp, lg.data());
+        SA_CHECK(hipMemcpy(d_same_diag_, sd.data(), sd.size() * 4, hipMemcpyHostToDevice));
+        tab_k_ = k;
+        tab_cp_ = cp;
+    }
+    hipEvent_t ev[6];
+    for (auto& e : ev) SA_CHECK(hipEventCreate(&e));
+    IndexView ix{d_entries_, d_buckets_, p_};
+    RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    auto* d_off = (const uint64_t*)buf_[S_OFF];
+    auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
+    auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
+    auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
+    auto* d_hits = (Hit*)grow(S_HITS, (2 * n_bases_ + 1) * sizeof(Hit));
+    auto* d_cnt = (uint64_t*)grow(S_CNT, (2 * n_bases_ + 1) * 8);
+    auto* d_hc = (HitCounts*)grow(S_HC, (2ull * n + 1) * sizeof(HitCounts));
+    auto* d_acap = (uint64_t*)grow(S_ACAP, (2ull * n + 1) * 8);
+    auto* d_aoff = (uint64_t*)grow(S_AOFF, (2ull * n + 1) * 8);
+    SA_CHECK(hipEventRecord(ev[0], stream_));
+    k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipEventRecord(ev[1], stream_));
+    k_hits<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, mp, d_off, n, d_qr, d_qrn, d_hits, d_cnt, d_hc, d_acap);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipEventRecord(ev[2], stream_));
+    uint64_t A = 0;
+    scan_u64(d_acap, d_aoff, 2ull * n, &A);
+    ChainBufs b;
+    b.anchors = (Anchor*)grow(S_ANCH, (A + 1) * sizeof(Anchor));
+    b.dp = (float*)grow(S_DP, (A + 1) * 4);
+    b.pred = (uint32_t*)grow(S_PRED, (A + 1) * 4);
+    b.cand = (uint32_t*)grow(S_CAND, (A + 1) * 4);
+    b.used = (uint8_t*)grow(S_USED, A + 1);
+    b.nams = (NamRec*)grow(S_NAMS_IN, (A + 1) * sizeof(NamRec));
+    b.chain_anchors = (uint32_t*)grow(S_CHAIN, (4 * A + 4) * 4);
+    auto* d_namn = (uint32_t*)grow(S_NAMN, (2ull * n + 1) * 4);
+    auto* d_araw = (uint32_t*)grow(S_ARAW, (2ull * n + 1) * 4);
+    SA_CHECK(hipEventRecord(ev[3], stream_));
+    k_chain<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, cp, d_off, n, d_hits, d_hc, d_acap, d_aoff, b, d_namn,
+                                                         d_araw, d_log2_, d_same_diag_);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipEventRecord(ev[4], stream_));
+    auto* d_tot = (uint64_t*)grow(S_TOT, (uint64_t)(n + 1) * 8);
+    auto* d_outoff = (uint64_t*)grow(S_OUTOFF, (uint64_t)(n + 1) * 8);
+    k_pair_sum<<<blocks(n, 256), 256, 0, stream_>>>(d_namn, d_tot, n);
+    uint64_t N = 0;
+    scan_u64(d_tot, d_outoff, n, &N);
+    auto* d_nams = (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec));
+    hipEvent_t fe0, fe1;
+    SA_CHECK(hipEventCreate(&fe0));
+    SA_CHECK(hipEventCreate(&fe1));
+    SA_CHECK(hipEventRecord(fe0, stream_));
+    k_finalize<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, ref, k, d_namn, d_aoff, b.nams, d_outoff, d_nams);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipEventRecord(fe1, stream_));
+    out.nams.resize(N);
+    out.nam_off.resize(n + 1);
+    if (N) SA_CHECK(hipMemcpyAsync(out.nams.data(), d_nams, N * sizeof(NamRec), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(out.nam_off.data(), d_outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
+    std::vector<HitCounts> hc(2ull * n);
+    std::vector<uint32_t> qrn(2ull * n), namn(2ull * n), araw(2ull * n);
+    SA_CHECK(hipMemcpyAsync(hc.data(), d_hc, 2ull * n * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(qrn.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(namn.data(), d_namn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(araw.data(), d_araw, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+    out.nam_off[n] = N;
+    out.info.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        ReadNamInfo& r = out.info[i];
+        r.hits[0] = hc[2 * i];
+        r.hits[1] = hc[2 * i + 1];
+        r.n_randstrobes = qrn[2 * i] + qrn[2 * i + 1];
+        r.n_anchors = araw[2 * i] + araw[2 * i + 1];
+        r.n_nams = namn[2 * i] + namn[2 * i + 1];
+        r.both_orientations = !better_orientation(hc[2 * i], hc[2 * i + 1]) && !better_orientation(hc[2 * i + 1], hc[2 * i]);
+    }
+    float t;
+    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t;
+    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t;
+    SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t;
+    SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t;
+    total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
+    for (auto& e : ev) hipEventDestroy(e);
+    hipEventDestroy(fe0);
+    hipEventDestroy(fe1);
+}
+
+void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std::vector<ExtResult>& res,
+                    std::vector<uint32_t>& cigars, std::vector<uint64_t>& cigar_off) {
+    SA_CHECK(hipSetDevice(device_));
+    res.assign(n, ExtResult{});
+    cigars.clear();
+    cigar_off.assign(n + 1, 0);
+    if (n == 0) return;
+    // CIGAR slots: at most 2*L+1 ops for a read of length L
+    std::vector<ExtRequest> req(req_in, req_in + n);
+    uint64_t slots = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t L = (uint32_t)(h_off_[req[i].read_id + 1] - h_off_[req[i].read_id]);
+        req[i].cigar_cap = 2 * L + 8;
+        req[i].cigar_off = slots;
+        slots += req[i].cigar_cap;
+    }
+    auto* d_req = (ExtRequest*)grow(S_REQ, (uint64_t)n * sizeof(ExtRequest));
+    auto* d_res = (ExtResult*)grow(S_RES, (uint64_t)n * sizeof(ExtResult));
+    auto* d_slots = (uint32_t*)grow(S_CIGSLOT, slots * 4);
+    SA_CHECK(hipMemcpyAsync(d_req, req.data(), (uint64_t)n * sizeof(ExtRequest), hipMemcpyHostToDevice, stream_));
+    const uint64_t L = max_len_;
+    uint64_t scratch_per = 16 * L + 256 + 65536 + (L + 1) * (L + L / 10 + 2) + 16 * (2 * L + L / 10 + 16) + 16 * L;
+    scratch_per = (scratch_per + 255) & ~255ull;
+    const uint64_t budget = 6ull << 30;
+    uint64_t wave = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / scratch_per));
+    auto* d_scr = (uint8_t*)grow(S_SCRATCH, wave * scratch_per);
+    RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    const uint32_t* d_chain = (const uint32_t*)buf_[S_CHAIN];
+    hipEvent_t e0, e1;
+    SA_CHECK(hipEventCreate(&e0));
+    SA_CHECK(hipEventCreate(&e1));
+    SA_CHECK(hipEventRecord(e0, stream_));
+    for (uint64_t first = 0; first < n; first += wave) {
+        uint32_t cnt = (uint32_t)std::min<uint64_t>(wave, n - first);
+        k_extend<<<blocks(cnt, 128), 128, 0, stream_>>>((const uint8_t*)buf_[S_SEQ], (const uint64_t*)buf_[S_OFF], ref,
+                                                        sc, (uint32_t)p_.k, d_req, cnt, d_res, d_slots, d_chain, d_scr,
+                                                        scratch_per, (uint32_t)first);
+        SA_CHECK(hipGetLastError());
+    }
+    SA_CHECK(hipEventRecord(e1, stream_));
+    SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+    float t;
+    SA_CHECK(hipEventElapsedTime(&t, e0, e1));
+    times_.extend = t;
+    total_ms_ += t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    // requests that ran out of scratch are re-run one at a time with a large slot
+    std::vector<uint32_t> big;
+    for (uint32_t i = 0; i < n; ++i)
+        if (res[i].status == 2) big.push_back(i);
+    if (!big.empty()) {
+        const uint64_t bigs = 256ull << 20;
+        auto* d_big = (uint8_t*)grow(S_SCRATCH, std::max<uint64_t>(bigs, wave * scratch_per));
+        for (uint32_t i : big) {
+            k_extend<<<1, 1, 0, stream_>>>((const uint8_t*)buf_[S_SEQ], (const uint64_t*)buf_[S_OFF], ref, sc,
+                                           (uint32_t)p_.k, d_req, 1, d_res, d_slots, d_chain, d_big, bigs, i);
+            SA_CHECK(hipGetLastError());
+        }
+        SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
+        SA_CHECK(hipStreamSynchronize(stream_));
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (res[i].found && res[i].status != 0) throw std::runtime_error("extension request failed (status)");
+    // compact the CIGARs
+    auto* d_ncig = (uint64_t*)grow(S_NCIG, (uint64_t)n * 8);
+    auto* d_coff = (uint64_t*)grow(S_CIGOFF, (uint64_t)n * 8);
+    k_ncigar<<<blocks(n, 256), 256, 0, stream_>>>(d_res, d_ncig, n);
+    uint64_t total = 0;
+    scan_u64(d_ncig, d_coff, n, &total);
+    auto* d_cout = (uint32_t*)grow(S_CIGOUT, (total + 1) * 4);
+    k_compact_cigars<<<blocks(n, 256), 256, 0, stream_>>>(d_req, d_res, d_slots, d_coff, d_cout, n);
+    SA_CHECK(hipGetLastError());
+    cigars.resize(total);
+    if (total) SA_CHECK(hipMemcpyAsync(cigars.data(), d_cout, total * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipStreamSynchronize(stream_));
+    cigar_off[n] = total;
+}
+
+}  // namespace sa

@@ -1,0 +1,93 @@
+// Device engine: owns the device-resident reference and index, the per-batch
+// buffers, and launches the kernels of sa_kernels.hip. Host code above this
+// layer (sa_mapper.cpp, the C ABI) never touches device pointers directly.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sa_types.hpp"
+
+struct ihipStream_t;
+
+namespace sa {
+
+struct Scores;
+
+// Host copy of what the chainer/NAM stage returns for one batch of reads.
+struct NamBatch {
+    std::vector<NamRec> nams;       // per read, stable-sorted by score (pre-shuffle)
+    std::vector<uint64_t> nam_off;  // n_reads + 1
+    std::vector<ReadNamInfo> info;  // n_reads
+};
+
+// Per-stage timings of the last call, in milliseconds (HIP events on the engine stream).
+struct StageTimes {
+    float seed = 0, hits = 0, chain = 0, finalize = 0, extend = 0, scans = 0;
+};
+
+class Engine {
+public:
+    explicit Engine(int device);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Packed reference: words of each contig start on a word boundary.
+    void upload_reference(const uint64_t* words, uint64_t n_words, const uint64_t* word_off, const uint64_t* len,
+                          uint32_t n_contigs);
+    void upload_index(const RefEntry* entries, uint64_t n, const uint64_t* bucket_starts, const SeedParams& p);
+
+    // Seeding -> hits -> chaining -> NAMs for reads held in host memory (or already
+    // resident when `seq == nullptr`, see upload_reads). Keeps the reads and the chain
+    // anchors on the device for the extension calls that follow.
+    void upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_reads);
+    void find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out);
+
+    // Fine-grained stages for parity probes (valid after upload_reads).
+    void randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts);
+
+    // Extension requests over the resident reads. `cigars` receives each result's ops
+    // at `cigar_off` positions computed by this call (compact).
+    void extend(const ExtRequest* req, uint32_t n, const Scores& sc, std::vector<ExtResult>& res,
+                std::vector<uint32_t>& cigars, std::vector<uint64_t>& cigar_off);
+
+    const StageTimes& times() const { return times_; }
+    const SeedParams& params() const { return p_; }
+    uint64_t n_entries() const { return n_entries_; }
+    ihipStream_t* stream() const { return stream_; }
+    // Kernel time accumulated over all find_nams/extend calls, milliseconds.
+    double total_kernel_ms() const { return total_ms_; }
+    uint64_t total_lookup_bytes() const { return lookup_bytes_; }
+
+private:
+    void* grow(int slot, uint64_t bytes);
+    void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total);
+
+    int device_;
+    ihipStream_t* stream_ = nullptr;
+    SeedParams p_{};
+    uint64_t n_entries_ = 0;
+    RefEntry* d_entries_ = nullptr;
+    uint64_t* d_buckets_ = nullptr;
+    uint64_t* d_words_ = nullptr;
+    uint64_t* d_word_off_ = nullptr;
+    uint64_t* d_len_ = nullptr;
+    uint32_t n_contigs_ = 0;
+    float* d_log2_ = nullptr;
+    float* d_same_diag_ = nullptr;
+    ChainParams tab_cp_{};
+    uint32_t tab_k_ = 0;
+    uint32_t n_reads_ = 0;
+    uint64_t n_bases_ = 0;
+    uint32_t max_len_ = 0;
+    std::vector<uint64_t> h_off_;
+    // growable device buffers indexed by slot
+    std::vector<void*> buf_;
+    std::vector<uint64_t> cap_;
+    StageTimes times_;
+    double total_ms_ = 0;
+    uint64_t lookup_bytes_ = 0;
+};
+
+}  // namespace sa

@@ -1,0 +1,716 @@
+// Per-request base-level extension: ungapped (Hamming) alignment, SSW with
+// the reference's striped lane semantics, banded global traceback, and
+// piecewise extension between chain anchors. One request per GPU thread;
+// all working memory lives in a per-request scratch slot.
+#pragma once
+#include "sa_seed.hpp"
+
+namespace sa {
+
+struct Scores {
+    int32_t match, mismatch, gap_open, gap_extend, end_bonus, xdrop;
+};
+
+// CIGAR writer with the merge-equal / drop-empty rule of src/cigar.rs:96-105.
+// Ops are BAM-encoded (len << 4 | op); op codes 0 M, 1 I, 2 D, 4 S, 7 =, 8 X.
+struct CigarOut {
+    uint32_t* ops;
+    uint32_t cap, n;
+    bool overflow;
+    SA_HD void push(uint32_t op, uint32_t len) {
+        if (len == 0) return;
+        if (n > 0 && (ops[n - 1] & 0xF) == op) {
+            ops[n - 1] += len << 4;
+            return;
+        }
+        if (n >= cap) { overflow = true; ++n; return; }
+        ops[n++] = (len << 4) | op;
+    }
+    SA_HD uint32_t edit_distance() const {
+        uint32_t d = 0;
+        for (uint32_t i = 0; i < n && i < cap; ++i) {
+            uint32_t op = ops[i] & 0xF;
+            if (op == 8 || op == 1 || op == 2) d += ops[i] >> 4;
+        }
+        return d;
+    }
+};
+
+// Views of the query (read or its reverse complement) and of a reference window.
+struct QueryView {
+    const uint8_t* seq;
+    uint32_t len;
+    bool rc;
+    SA_HD uint8_t operator[](uint32_t i) const { return read_char(seq, len, rc, i); }
+};
+struct RefWin {
+    const RefView* ref;
+    uint32_t id;
+    uint64_t start;  // absolute position of window index 0
+    uint32_t len;
+    SA_HD uint8_t operator[](uint32_t i) const { return ref_char(*ref, id, start + i); }
+};
+
+// src/ssw.rs:8-22 translate: A/U -> 0, C 1, G 2, T 3, other 4
+SA_HD int32_t ssw_code(uint8_t c) {
+    switch (c) {
+        case 'A': case 'a': case 'U': case 'u': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return 4;
+    }
+}
+SA_HD int32_t ssw_mat(int32_t a, int32_t b, const Scores& sc) { return (a == b && a < 4) ? sc.match : -sc.mismatch; }
+
+// ------------------------------------------------------------------ SSW ---
+struct SswEnd {
+    int32_t score, ref, read;
+};
+
+// One striped pass of ext/ssw/ssw.c:197-588 with LANES lanes per vector
+// (16 unsigned bytes with bias 8, or 8 signed words without bias). The query
+// is `qcode(j)` for j < qlen; the reference index runs forward over [0, rlen)
+// or, for the reverse pass, backward from rlen-1 to 0. H/E/Hmax live in
+// `buf` (4 * segLen * LANES ints).
+template <int LANES, bool BYTE, typename QC, typename RC>
+SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool reverse, const Scores& sc, int32_t terminate,
+                      int32_t* buf) {
+    const int32_t bias = BYTE ? sc.mismatch : 0;
+    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
+    const int32_t segLen = (qlen + LANES - 1) / LANES;
+    const int32_t W = segLen * LANES;
+    int32_t* HS = buf;
+    int32_t* HL = buf + W;
+    int32_t* E = buf + 2 * W;
+    int32_t* Hmax = buf + 3 * W;
+    for (int32_t i = 0; i < 4 * W; ++i) buf[i] = 0;
+    auto sat = [](int32_t v) { return BYTE ? (v < 0 ? 0 : (v > 255 ? 255 : v)) : (v < -32768 ? -32768 : (v > 32767 ? 32767 : v)); };
+    auto subs = [](int32_t a, int32_t b) { int32_t v = a - b; return v < 0 ? 0 : v; };
+    int32_t maxv = 0, end_read = qlen - 1, end_ref = BYTE ? -1 : 0;
+    int32_t vMaxScore[LANES], vMaxMark[LANES];
+    for (int l = 0; l < LANES; ++l) vMaxScore[l] = vMaxMark[l] = 0;
+    int32_t begin = reverse ? rlen - 1 : 0, end = reverse ? -1 : rlen, step = reverse ? -1 : 1;
+    for (int32_t i = begin; i != end; i += step) {
+        int32_t vF[LANES], vMaxCol[LANES], vH[LANES];
+        for (int l = LANES - 1; l > 0; --l) vH[l] = HS[(segLen - 1) * LANES + l - 1];
+        vH[0] = 0;
+        for (int l = 0; l < LANES; ++l) { vF[l] = 0; vMaxCol[l] = 0; }
+        int32_t* t = HS; HS = HL; HL = t;
+        const int32_t rc = rcode(i);
+        for (int32_t j = 0; j < segLen; ++j) {
+            for (int l = 0; l < LANES; ++l) {
+                int32_t qi = j + l * segLen;
+                int32_t prof = qi >= qlen ? 0 : ssw_mat(rc, qcode(qi), sc);
+                int32_t h = BYTE ? subs(sat(vH[l] + prof + bias), bias) : sat(vH[l] + prof);
+                int32_t e = E[j * LANES + l];
+                h = h > e ? h : e;
+                h = h > vF[l] ? h : vF[l];
+                vMaxCol[l] = vMaxCol[l] > h ? vMaxCol[l] : h;
+                HS[j * LANES + l] = h;
+                int32_t hg = subs(h, gapO);
+                int32_t en = subs(e, gapE);
+                E[j * LANES + l] = en > hg ? en : hg;
+                int32_t fn = subs(vF[l], gapE);
+                vF[l] = fn > hg ? fn : hg;
+                vH[l] = HL[j * LANES + l];
+            }
+        }
+        bool done = false;
+        for (int kk = 0; kk < LANES && !done; ++kk) {
+            for (int l = LANES - 1; l > 0; --l) vF[l] = vF[l - 1];
+            vF[0] = 0;
+            for (int32_t j = 0; j < segLen; ++j) {
+                bool all_le = true;
+                for (int l = 0; l < LANES; ++l) {
+                    int32_t h = HS[j * LANES + l];
+                    h = h > vF[l] ? h : vF[l];
+                    vMaxCol[l] = vMaxCol[l] > h ? vMaxCol[l] : h;
+                    HS[j * LANES + l] = h;
+                    int32_t hg = subs(h, gapO);
+                    vF[l] = subs(vF[l], gapE);
+                    if (vF[l] > hg) all_le = false;
+                }
+                if (all_le) { done = true; break; }
+            }
+        }
+        bool changed = false;
+        for (int l = 0; l < LANES; ++l) {
+            vMaxScore[l] = vMaxScore[l] > vMaxCol[l] ? vMaxScore[l] : vMaxCol[l];
+            if (vMaxMark[l] != vMaxScore[l]) changed = true;
+        }
+        if (changed) {
+            int32_t temp = 0;
+            for (int l = 0; l < LANES; ++l) { vMaxMark[l] = vMaxScore[l]; temp = temp > vMaxScore[l] ? temp : vMaxScore[l]; }
+            if (temp > maxv) {
+                maxv = temp;
+                if (BYTE && maxv + bias >= 255) break;
+                end_ref = i;
+                for (int32_t x = 0; x < W; ++x) Hmax[x] = HS[x];
+            }
+        }
+        int32_t colmax = 0;
+        for (int l = 0; l < LANES; ++l) colmax = colmax > vMaxCol[l] ? colmax : vMaxCol[l];
+        if (colmax == terminate) break;
+    }
+    for (int32_t idx = 0; idx < W; ++idx) {
+        if (Hmax[idx] == maxv) {
+            int32_t t = idx / LANES + (idx % LANES) * segLen;
+            if (t < end_read) end_read = t;
+        }
+    }
+    SswEnd r;
+    r.score = BYTE ? (maxv + bias >= 255 ? 255 : maxv) : maxv;
+    r.ref = end_ref;
+    r.read = end_read;
+    return r;
+}
+
+// Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774).
+// Emits M/I/D ops in forward order through `emit(op, len)`. Returns 0 ok,
+// 1 traceback failure, 2 scratch too small.
+template <typename QC, typename RC, typename Emit>
+SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score, const Scores& sc, int32_t bw,
+                    uint8_t* scratch, uint64_t scratch_bytes, Emit emit) {
+    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
+    int32_t len = refLen > readLen ? refLen : readLen;
+    int32_t maxv = 0;
+    int32_t width = 0, width_d = 0;
+    int32_t *hb = nullptr, *eb = nullptr, *hc = nullptr;
+    int8_t* dir = nullptr;
+    auto U = [&](int32_t i, int32_t j) { int32_t x = i - bw; x = x > 0 ? x : 0; return j - x + 1; };
+    auto D = [&](int32_t i, int32_t j, int32_t p) { int32_t x = i - bw; x = x > 0 ? x : 0; return (j - x) * 3 + p; };
+    do {
+        width = bw * 2 + 3;
+        width_d = bw * 2 + 1;
+        uint64_t need = 3ull * (uint64_t)(width + 1) * 4 + (uint64_t)width_d * (uint64_t)readLen * 3 + 3;
+        if (need > scratch_bytes) return 2;
+        hb = (int32_t*)scratch;
+        eb = hb + (width + 1);
+        hc = eb + (width + 1);
+        dir = (int8_t*)(hc + (width + 1));
+        for (int32_t x = 0; x <= width; ++x) hb[x] = eb[x] = hc[x] = 0;
+        for (uint64_t x = 0; x < (uint64_t)width_d * readLen * 3 + 3; ++x) dir[x] = 0;
+        for (int32_t i = 0; i < readLen; ++i) {
+            int32_t beg = i - bw > 0 ? i - bw : 0, end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
+            int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
+            int32_t f = 0;
+            hb[0] = eb[0] = hb[edge] = eb[edge] = hc[0] = 0;
+            int8_t* dl = dir + (uint64_t)width_d * i * 3;
+            int32_t u = 0;
+            const int32_t qi = qc(i);
+            for (int32_t j = beg; j <= end; ++j) {
+                u = U(i, j);
+                int32_t e = U(i - 1, j), b = U(i, j - 1), d = U(i - 1, j - 1);
+                int32_t de = D(i, j, 0), df = D(i, j, 1), dh = D(i, j, 2);
+                int32_t t1 = i == 0 ? -gapO : hb[e] - gapO;
+                int32_t t2 = i == 0 ? -gapE : eb[e] - gapE;
+                eb[u] = t1 > t2 ? t1 : t2;
+                dl[de] = t1 > t2 ? 3 : 2;
+                t1 = hc[b] - gapO;
+                t2 = f - gapE;
+                f = t1 > t2 ? t1 : t2;
+                dl[df] = t1 > t2 ? 5 : 4;
+                int32_t e1 = eb[u] > 0 ? eb[u] : 0, f1 = f > 0 ? f : 0;
+                t1 = e1 > f1 ? e1 : f1;
+                t2 = hb[d] + ssw_mat(rc(j), qi, sc);
+                hc[u] = t1 > t2 ? t1 : t2;
+                if (hc[u] > maxv) maxv = hc[u];
+                if (t1 <= t2) dl[dh] = 1;
+                else dl[dh] = e1 > f1 ? dl[de] : dl[df];
+            }
+            for (int32_t j = 1; j <= u; ++j) hb[j] = hc[j];
+        }
+        bw *= 2;
+    } while (maxv < score && bw <= len);
+    bw /= 2;
+    // traceback (ops collected backwards into the tail of scratch, emitted forwards)
+    uint64_t used = 3ull * (uint64_t)(width + 1) * 4 + (uint64_t)width_d * (uint64_t)readLen * 3 + 3;
+    uint64_t room = (scratch_bytes - used) / 8;
+    uint32_t* st = (uint32_t*)(scratch + ((used + 7) & ~7ull));
+    if (room < 4) return 2;
+    room -= 1;
+    uint32_t nst = 0;
+    int32_t i = readLen - 1, j = refLen - 1, e = 0, t2 = 2, row = readLen - 1;
+    char op = 'M', prev = 'M';
+    auto code = [](char o) { return o == 'M' ? 0u : (o == 'I' ? 1u : 2u); };
+    while (i >= 0 && j > 0) {
+        int8_t v = dir[(uint64_t)width_d * row * 3 + D(i, j, t2)];
+        switch (v) {
+            case 1: --i; --j; t2 = 2; --row; op = 'M'; break;
+            case 2: --i; t2 = 0; --row; op = 'I'; break;
+            case 3: --i; t2 = 2; --row; op = 'I'; break;
+            case 4: --j; t2 = 1; op = 'D'; break;
+            case 5: --j; t2 = 2; op = 'D'; break;
+            default: return 1;
+        }
+        if (op == prev) ++e;
+        else {
+            if (nst >= room) return 2;
+            st[nst++] = ((uint32_t)e << 4) | code(prev);
+            prev = op;
+            e = 1;
+        }
+    }
+    if (nst + 2 >= room) return 2;
+    if (op == 'M') st[nst++] = ((uint32_t)(e + 1) << 4) | code(op);
+    else { st[nst++] = ((uint32_t)e << 4) | code(op); st[nst++] = (1u << 4) | 0u; }
+    for (uint32_t x = nst; x-- > 0;) emit(st[x] & 0xF, st[x] >> 4);
+    return 0;
+}
+
+struct AlignInfo {
+    bool found;
+    uint32_t score, query_start, query_end, ref_start, ref_end, edit_distance;
+    int status;  // 0 ok, 1 cigar overflow, 2 scratch overflow
+};
+
+// SSW alignment with the end-bonus extension (src/ssw.rs:156-191 + src/aligner.rs:80-151).
+// Coordinates are relative to the query view and the reference window.
+SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Scores& sc, uint8_t* scratch,
+                                  uint64_t scratch_bytes, CigarOut& cig) {
+    AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
+    cig.n = 0;
+    cig.overflow = false;
+    if (r.len > 2000 || q.len == 0) return a;
+    const int32_t qlen = (int32_t)q.len, rlen = (int32_t)r.len;
+    uint64_t ssw_bytes = 4ull * 4 * (uint64_t)(((qlen + 15) / 16) * 16);
+    if (ssw_bytes + 64 > scratch_bytes) { a.status = 2; return a; }
+    int32_t* buf = (int32_t*)scratch;
+    auto qf = [&](int32_t j) { return ssw_code(q[(uint32_t)j]); };
+    auto rf = [&](int32_t i) { return ssw_code(r[(uint32_t)i]); };
+    SswEnd best = ssw_pass<16, true>(qf, qlen, rf, rlen, false, sc, 255, buf);
+    bool word = false;
+    if (best.score == 255) {
+        best = ssw_pass<8, false>(qf, qlen, rf, rlen, false, sc, 65535, buf);
+        word = true;
+    }
+    const int32_t score1 = best.score, ref_end1 = best.ref, read_end1 = best.read;
+    // reverse pass over the reversed query prefix [0, read_end1]
+    auto qrev = [&](int32_t j) { return ssw_code(q[(uint32_t)(read_end1 - j)]); };
+    SswEnd br = word ? ssw_pass<8, false>(qrev, read_end1 + 1, rf, ref_end1 + 1, true, sc, score1, buf)
+                     : ssw_pass<16, true>(qrev, read_end1 + 1, rf, ref_end1 + 1, true, sc, (uint8_t)score1, buf);
+    const int32_t ref_begin1 = br.ref, read_begin1 = read_end1 - br.read;
+    if (ref_begin1 == -1 || read_begin1 == -1 || ref_begin1 < 0 || read_begin1 < 0) return a;
+    if (score1 > br.score) return a;  // flag 2
+    const int32_t refL = ref_end1 - ref_begin1 + 1, readL = read_end1 - read_begin1 + 1;
+    int32_t bw = refL > readL ? refL - readL + 1 : readL - refL + 1;
+    // decide the end-bonus extensions first so the CIGAR can be written front to back
+    uint32_t qs = (uint32_t)read_begin1, rs = (uint32_t)ref_begin1;
+    uint32_t qe = (uint32_t)read_end1 + 1, re = (uint32_t)ref_end1 + 1;
+    // The SSW part is written after room reserved for a possible front extension
+    // (at most qs ops), then moved down once the extensions are decided.
+    uint32_t qi = qs, ri = rs, edits = 0;
+    CigarOut mainc = cig;
+    uint32_t front_room = qs <= rs ? qs : 0;
+    if (front_room >= cig.cap) { a.status = 1; return a; }
+    mainc.ops = cig.ops + front_room;
+    mainc.cap = cig.cap - front_room;
+    mainc.n = 0;
+    // emit main alignment into mainc
+    {
+        auto emit_main = [&](uint32_t op, uint32_t len) {
+            if (op == 0) {
+                for (uint32_t t = 0; t < len; ++t) {
+                    bool eq = q[qi] == r[ri];
+                    mainc.push(eq ? 7u : 8u, 1);
+                    edits += eq ? 0 : 1;
+                    ++qi; ++ri;
+                }
+            } else if (op == 1) {
+                mainc.push(1, len); qi += len; edits += len;
+            } else {
+                mainc.push(2, len); ri += len; edits += len;
+            }
+        };
+        uint64_t used = ssw_bytes;
+        int st = banded_sw([&](int32_t i) { return ssw_code(q[qs + (uint32_t)i]); },
+                           [&](int32_t j) { return ssw_code(r[rs + (uint32_t)j]); }, refL, readL, score1, sc, bw,
+                           scratch + used, scratch_bytes - used, emit_main);
+        if (st == 1) return a;  // flag 1: no alignment
+        if (st == 2) { a.status = 2; return a; }
+    }
+    if (mainc.overflow) { a.status = 1; return a; }
+    int32_t score = score1;
+    // front extension towards query start
+    uint32_t fq = qs, fr = rs;
+    int32_t fscore = score;
+    uint32_t fedits = edits;
+    while (fq > 0 && fr > 0) {
+        --fq; --fr;
+        if (q[fq] == r[fr]) fscore += sc.match; else { fscore -= sc.mismatch; ++fedits; }
+    }
+    bool front = fq == 0 && fscore + sc.end_bonus > score;
+    // final CIGAR: [front ops][main ops][back ops]
+    cig.n = 0;
+    cig.overflow = false;
+    if (front && qs > 0) {
+        for (uint32_t t = 0; t < qs; ++t) cig.push(q[t] == r[rs - qs + t] ? 7u : 8u, 1);
+    }
+    // append main ops (they were written at an offset; copy down with merging)
+    for (uint32_t x = 0; x < mainc.n; ++x) {
+        uint32_t v = mainc.ops[x];
+        cig.push(v & 0xF, v >> 4);
+    }
+    if (front) {
+        qs = 0;
+        rs = fr;
+        score = fscore + sc.end_bonus;
+        edits = fedits;
+    }
+    // back extension towards query end
+    uint32_t bq = qe, br2 = re;
+    int32_t bscore = score;
+    uint32_t bedits = edits;
+    while (bq < q.len && br2 < r.len) {
+        if (q[bq] == r[br2]) bscore += sc.match; else { bscore -= sc.mismatch; ++bedits; }
+        ++bq; ++br2;
+    }
+    if (bq == q.len && bscore + sc.end_bonus > score) {
+        if (qe < q.len)
+            for (uint32_t t = 0; t < bq - qe; ++t) cig.push(q[qe + t] == r[re + t] ? 7u : 8u, 1);
+        qe = q.len;
+        re = br2;
+        score = bscore + sc.end_bonus;
+        edits = bedits;
+    }
+    if (cig.overflow) { a.status = 1; return a; }
+    a.found = true;
+    a.score = (uint32_t)score;
+    a.query_start = qs;
+    a.query_end = qe;
+    a.ref_start = rs;
+    a.ref_end = re;
+    a.edit_distance = edits;
+    return a;
+}
+
+// src/aligner.rs:181-259 (segment scores with the end bonus at both ends)
+template <typename Q, typename R>
+SA_HD AlignInfo hamming_align(const Q& q, const R& r, uint32_t n, const Scores& sc, CigarOut& cig) {
+    uint32_t start = 0, bs = 0, be = 0;
+    int32_t score = sc.end_bonus, best = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (q[i] == r[i]) score += sc.match; else score -= sc.mismatch;
+        if (score < 0) { start = i + 1; score = 0; }
+        if (score > best) { bs = start; best = score; be = i + 1; }
+    }
+    if (score + sc.end_bonus > best) { best = score + sc.end_bonus; be = n; bs = start; }
+    cig.n = 0;
+    cig.overflow = false;
+    uint32_t mm = 0;
+    for (uint32_t i = bs; i < be; ++i) {
+        bool eq = q[i] == r[i];
+        mm += eq ? 0 : 1;
+        cig.push(eq ? 7u : 8u, 1);
+    }
+    AlignInfo a{true, (uint32_t)best, bs, be, bs, be, mm, cig.overflow ? 1 : 0};
+    return a;
+}
+
+// --------------------------------------------------------- piecewise DP ---
+// Affine-gap DP (gap of length L costs open + (L-1)*extend) with a global start
+// at (0,0); `gsle`: local end (max cell, end bonus on the last query row);
+// otherwise global (cell (m,n)). Traceback preference at ties, walking back:
+// deletion, then insertion, then diagonal; a gap run closes as soon as opening
+// it explains the score. Ops are written to `tb` in traceback order.
+struct DpOut {
+    int32_t score;
+    uint32_t bi, bj;  // end cell
+    uint32_t ntb;     // ops in tb
+    int status;
+};
+
+template <typename Q, typename R>
+SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scores& sc, bool local_end,
+                      uint8_t* scratch, uint64_t scratch_bytes, uint8_t* tb, uint32_t tb_cap) {
+    DpOut o{0, 0, 0, 0, 0};
+    const int32_t go = sc.gap_open, ge = sc.gap_extend;
+    const int32_t NEG = -(1 << 28);
+    uint64_t W = (uint64_t)n + 1;
+    uint64_t need = (uint64_t)(m + 1) * W + 3 * W * 4 + 16;
+    if (need > scratch_bytes) { o.status = 2; return o; }
+    uint8_t* dirs = scratch;  // bit0 H==E, bit1 H==F, bit2 E opened, bit3 F opened
+    int32_t* Hp = (int32_t*)(scratch + (((uint64_t)(m + 1) * W + 15) & ~15ull));
+    int32_t* Hc = Hp + W;
+    int32_t* F = Hc + W;
+    int32_t best = 0;
+    uint32_t bi = 0, bj = 0;
+    // row 0
+    Hp[0] = 0;
+    F[0] = NEG;
+    dirs[0] = 0;
+    for (uint32_t j = 1; j <= n; ++j) {
+        Hp[j] = -(go + (int32_t)(j - 1) * ge);
+        F[j] = NEG;
+        dirs[j] = (uint8_t)(1 | (j == 1 ? 4 : 0));
+        if (local_end && Hp[j] > best) { best = Hp[j]; bi = 0; bj = j; }
+    }
+    for (uint32_t i = 1; i <= m; ++i) {
+        uint8_t* dr = dirs + (uint64_t)i * W;
+        int32_t Fi0 = -(go + (int32_t)(i - 1) * ge);
+        Hc[0] = Fi0;
+        F[0] = Fi0;
+        dr[0] = (uint8_t)(2 | (i == 1 ? 8 : 0));
+        if (local_end && Hc[0] > best) { best = Hc[0]; bi = i; bj = 0; }
+        int32_t E = NEG;
+        const uint8_t qc = q[i - 1];
+        const int32_t bonus = (local_end && i == m) ? sc.end_bonus : 0;
+        for (uint32_t j = 1; j <= n; ++j) {
+            int32_t eo = Hc[j - 1] - go, ee = E - ge;
+            E = eo > ee ? eo : ee;
+            int32_t fo = Hp[j] - go, fe = F[j] - ge;
+            int32_t f = fo > fe ? fo : fe;
+            F[j] = f;
+            int32_t h = Hp[j - 1] + (qc == r[j - 1] ? sc.match : -sc.mismatch) + bonus;
+            int32_t hh = h > E ? h : E;
+            hh = hh > f ? hh : f;
+            Hc[j] = hh;
+            dr[j] = (uint8_t)((hh == E ? 1 : 0) | (hh == f ? 2 : 0) | (E == eo ? 4 : 0) | (f == fo ? 8 : 0));
+            if (local_end && hh > best) { best = hh; bi = i; bj = j; }
+        }
+        int32_t* t = Hp; Hp = Hc; Hc = t;
+    }
+    if (!local_end) { best = Hp[n]; bi = m; bj = n; }
+    o.score = best;
+    o.bi = bi;
+    o.bj = bj;
+    // traceback
+    uint32_t i = bi, j = bj, nt = 0;
+    int state = 0;
+    while (i > 0 || j > 0) {
+        uint8_t d = dirs[(uint64_t)i * W + j];
+        if (state == 0) {
+            if (j > 0 && (d & 1)) state = 1;
+            else if (i > 0 && (d & 2)) state = 2;
+            else {
+                if (nt >= tb_cap) { o.status = 2; return o; }
+                tb[nt++] = q[i - 1] == r[j - 1] ? 7 : 8;
+                --i; --j;
+            }
+        } else if (state == 2) {
+            if (nt >= tb_cap) { o.status = 2; return o; }
+            tb[nt++] = 1;
+            bool open = (d & 8) != 0;
+            --i;
+            if (open) state = 0;
+        } else {
+            if (nt >= tb_cap) { o.status = 2; return o; }
+            tb[nt++] = 2;
+            bool open = (d & 4) != 0;
+            --j;
+            if (open) state = 0;
+        }
+    }
+    o.ntb = nt;
+    return o;
+}
+
+// Reversed views used by the LocalStartGlobalEnd mode.
+template <typename V>
+struct RevView {
+    const V* v;
+    uint32_t len;
+    SA_HD uint8_t operator[](uint32_t i) const { return (*v)[len - 1 - i]; }
+};
+
+// Offset view (sub-range of another view).
+template <typename V>
+struct SubView {
+    const V* v;
+    uint32_t off;
+    SA_HD uint8_t operator[](uint32_t i) const { return (*v)[off + i]; }
+};
+
+// src/piecewisealigner.rs:645-719 on (ref_start, query_start) pairs, last anchor first.
+SA_HD uint32_t remove_spurious_anchors(uint32_t* a, uint32_t n) {
+    if (n < 2) return n;
+    auto erase = [&](uint32_t from, uint32_t to) {  // [from, to)
+        uint32_t d = to - from;
+        for (uint32_t x = to; x < n; ++x) { a[2 * (x - d)] = a[2 * x]; a[2 * (x - d) + 1] = a[2 * x + 1]; }
+        n -= d;
+    };
+    const int64_t tol = 5;
+    int64_t tracked = 0, dev_start = -1;
+    uint32_t i = 1;
+    while (i < n) {
+        int64_t qd = (int64_t)a[2 * (i - 1) + 1] - (int64_t)a[2 * i + 1];
+        int64_t rd = (int64_t)a[2 * (i - 1)] - (int64_t)a[2 * i];
+        int64_t indel = qd - rd;
+        int64_t ai = indel < 0 ? -indel : indel;
+        if (ai > tol) {
+            int64_t s = tracked + indel;
+            if (dev_start >= 0 && (s < 0 ? -s : s) <= tol) {
+                erase((uint32_t)dev_start, i);
+                i = (uint32_t)dev_start;
+                dev_start = -1;
+                tracked = 0;
+                continue;
+            }
+            dev_start = i;
+            tracked = indel;
+        }
+        ++i;
+    }
+    // max_prune_count = ceil(len * 0.20) in f32
+    float fl = (float)n * 0.20f;
+    uint32_t maxp = (uint32_t)fl;
+    if ((float)maxp < fl) ++maxp;
+    for (uint32_t t = 1; t < (n < maxp ? n : maxp); ++t) {
+        int64_t qd = (int64_t)a[2 * t + 1] - (int64_t)a[2 * (t - 1) + 1];
+        int64_t rd = (int64_t)a[2 * t] - (int64_t)a[2 * (t - 1)];
+        if (qd - rd != 0) { erase(0, t); break; }
+    }
+    uint32_t lo = (n > maxp ? n - maxp : 0) + 1;
+    for (uint32_t t = n; t-- > lo;) {
+        int64_t qd = (int64_t)a[2 * t + 1] - (int64_t)a[2 * (t - 1) + 1];
+        int64_t rd = (int64_t)a[2 * t] - (int64_t)a[2 * (t - 1)];
+        if (qd - rd != 0) { erase(t, n); break; }
+    }
+    return n;
+}
+
+// Emit DP traceback ops in forward order into `cig`; `reversed_space` means the DP
+// ran on reversed sequences (its traceback order already is forward order).
+SA_HD void emit_tb(const uint8_t* tb, uint32_t ntb, bool reversed_space, CigarOut& cig) {
+    if (reversed_space) for (uint32_t x = 0; x < ntb; ++x) cig.push(tb[x], 1);
+    else for (uint32_t x = ntb; x-- > 0;) cig.push(tb[x], 1);
+}
+
+// src/piecewisealigner.rs:396-510. `anch` holds (ref_start, query_start) pairs,
+// last anchor first, with absolute reference coordinates; `win` is the decoded
+// reference window (its start is the decode start).
+SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t* anch, uint32_t na, uint32_t padding,
+                                 const Scores& sc, uint32_t k, uint8_t* scratch, uint64_t scratch_bytes,
+                                 CigarOut& cig) {
+    AlignInfo res{false, 0, 0, 0, 0, 0, 0, 0};
+    cig.n = 0;
+    cig.overflow = false;
+    if (na == 0) return res;
+    const uint32_t ql = q.len;
+    const uint32_t rl = win.len;
+    // anchors relative to the window
+    auto A_rs = [&](uint32_t x) { return anch[2 * x] - (uint32_t)win.start; };
+    auto A_qs = [&](uint32_t x) { return anch[2 * x + 1]; };
+    uint8_t* tb = scratch;
+    uint32_t tb_cap = 2 * (ql + rl) + 16;
+    uint64_t tb_bytes = ((uint64_t)tb_cap + 15) & ~15ull;
+    if (tb_bytes >= scratch_bytes) { res.status = 2; return res; }
+    uint8_t* dps = scratch + tb_bytes;
+    uint64_t dps_bytes = scratch_bytes - tb_bytes;
+    int32_t score = 0;
+    uint32_t ref_start, query_start;
+    {
+        uint32_t fq = A_qs(na - 1), fr = A_rs(na - 1);
+        if (fq > 0 && fr > 0) {
+            uint32_t qlen = fq;
+            uint32_t rs = fr > qlen + padding ? fr - (qlen + padding) : 0;
+            uint32_t rlen = fr - rs;
+            SubView<QueryView> qv{&q, 0};
+            SubView<RefWin> rv{&win, rs};
+            RevView<SubView<QueryView>> qr{&qv, qlen};
+            RevView<SubView<RefWin>> rr{&rv, rlen};
+            DpOut d = affine_dp(qr, qlen, rr, rlen, sc, true, dps, dps_bytes, tb, tb_cap);
+            if (d.status) { res.status = d.status; return res; }
+            if (d.score == 0) { query_start = fq; ref_start = fr; }
+            else {
+                emit_tb(tb, d.ntb, true, cig);
+                score = d.score;
+                query_start = qlen - d.bi;
+                ref_start = rlen - d.bj + rs;
+            }
+        } else {
+            query_start = fq;
+            ref_start = fr;
+            score = fq == 0 ? sc.end_bonus : 0;
+        }
+    }
+    score += (int32_t)k * sc.match;
+    cig.push(7, k);
+    for (uint32_t i = na - 1; i >= 1; --i) {
+        uint32_t cqs = A_qs(i - 1), crs = A_rs(i - 1);
+        uint32_t pqe = A_qs(i) + k, pre = A_rs(i) + k;
+        int64_t qd = (int64_t)cqs - (int64_t)pqe, rd = (int64_t)crs - (int64_t)pre;
+        if (rd > 0 && qd > 0) {
+            bool done = false;
+            if (rd == qd) {
+                int32_t hs = 0;
+                for (uint32_t t = 0; t < (uint32_t)qd; ++t) hs += q[pqe + t] == win[pre + t] ? sc.match : -sc.mismatch;
+                uint32_t hsc = hs > 0 ? (uint32_t)hs : 0;
+                float fl = (float)qd * 0.85f;
+                uint32_t c = (uint32_t)fl;
+                if ((float)c < fl) ++c;
+                if (hsc > 0 && hsc > (uint32_t)sc.match * c) {
+                    score += (int32_t)hsc;
+                    for (uint32_t t = 0; t < (uint32_t)qd; ++t) cig.push(q[pqe + t] == win[pre + t] ? 7u : 8u, 1);
+                    score += (int32_t)k * sc.match;
+                    cig.push(7, k);
+                    done = true;
+                }
+            }
+            if (!done) {
+                SubView<QueryView> qv{&q, pqe};
+                SubView<RefWin> rv{&win, pre};
+                DpOut d = affine_dp(qv, (uint32_t)qd, rv, (uint32_t)rd, sc, false, dps, dps_bytes, tb, tb_cap);
+                if (d.status) { res.status = d.status; return res; }
+                score += d.score;
+                emit_tb(tb, d.ntb, false, cig);
+                score += (int32_t)k * sc.match;
+                cig.push(7, k);
+            }
+        } else if (rd < qd) {
+            uint32_t ins = (uint32_t)(qd - rd);
+            score += -sc.gap_open + ((int32_t)ins - 1) * -sc.gap_extend;
+            cig.push(1, ins);
+            uint32_t mp = (uint32_t)((int64_t)k + rd);
+            score += (int32_t)mp * sc.match;
+            cig.push(7, mp);
+        } else if (rd > qd) {
+            uint32_t del = (uint32_t)(rd - qd);
+            score += -sc.gap_open + ((int32_t)del - 1) * -sc.gap_extend;
+            cig.push(2, del);
+            uint32_t mp = (uint32_t)((int64_t)k + qd);
+            score += (int32_t)mp * sc.match;
+            cig.push(7, mp);
+        } else {
+            uint32_t mp = (uint32_t)((int64_t)k + qd);
+            score += (int32_t)mp * sc.match;
+            cig.push(7, mp);
+        }
+    }
+    uint32_t ref_end, query_end;
+    {
+        uint32_t lqe = A_qs(0) + k, lre = A_rs(0) + k;
+        if (lqe < ql && lre < rl) {
+            uint32_t qlen = ql - lqe;
+            uint32_t re = rl < lre + qlen + padding ? rl : lre + qlen + padding;
+            SubView<QueryView> qv{&q, lqe};
+            SubView<RefWin> rv{&win, lre};
+            DpOut d = affine_dp(qv, qlen, rv, re - lre, sc, true, dps, dps_bytes, tb, tb_cap);
+            if (d.status) { res.status = d.status; return res; }
+            if (d.score == 0) { query_end = lqe; ref_end = lre; }
+            else {
+                emit_tb(tb, d.ntb, false, cig);
+                score += d.score;
+                query_end = d.bi + lqe;
+                ref_end = d.bj + lre;
+            }
+        } else {
+            query_end = lqe;
+            ref_end = lre;
+            score += lqe == ql ? sc.end_bonus : 0;
+        }
+    }
+    if (cig.overflow) { res.status = 1; return res; }
+    if (score <= 0) return res;
+    res.found = true;
+    res.score = (uint32_t)score;
+    res.query_start = query_start;
+    res.query_end = query_end;
+    res.ref_start = ref_start;
+    res.ref_end = ref_end;
+    res.edit_distance = cig.edit_distance();
+    return res;
+}
+
+}  // namespace sa

@@ -1,0 +1,81 @@
+// Host side of the drop-in: reference and read I/O, index construction, and
+// the mapper that turns device NAMs and device extension results into SAM,
+// PAF or abundance output with the reference's decision logic.
+#pragma once
+#include <cstdint>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "sa_engine.hpp"
+#include "sa_extend.hpp"
+
+namespace sa {
+
+// src/packed_seq.rs:12-126
+struct PackedRef {
+    std::vector<std::string> names;
+    std::vector<uint64_t> len;
+    std::vector<uint64_t> word_off;  // n + 1
+    std::vector<uint64_t> words;
+    uint8_t base(uint32_t id, uint64_t i) const {
+        return (uint8_t)"ACGT"[(words[word_off[id] + (i >> 5)] >> ((i & 31) * 2)) & 3];
+    }
+};
+
+// src/io/fasta.rs:116-165
+PackedRef read_fasta(const std::string& path);
+
+// src/seeding/parameters.rs:181-221 (SeedingParameters::new(read_length))
+SeedParams seed_params_for_read_length(uint64_t read_length);
+// src/indexer.rs:294-300
+int pick_bits(const PackedRef& ref, const SeedParams& p);
+
+struct HostIndex {
+    SeedParams p;
+    std::vector<RefEntry> entries;
+    std::vector<uint64_t> bucket_starts;
+};
+// src/indexer.rs:18-170 (host construction with multiple threads)
+void build_index(const PackedRef& ref, SeedParams p, int bits, double filter_fraction, int threads, HostIndex& out);
+
+// src/io/record.rs + src/io/fastq.rs:22-95
+struct Record {
+    std::string name, comment, seq, qual;
+    bool has_comment = false, has_qual = true;
+    int end = 0;
+};
+bool read_records(const std::string& path, std::vector<Record>& out, std::string& err);
+uint64_t estimate_read_length(const std::vector<Record>& recs);
+
+struct Options {
+    int mode = 0;  // 0 SAM, 1 PAF (-x), 2 abundances (--aemb)
+    bool eqx = false, details = false, fastq_comments = false, only_mapped = false, use_ssw = false;
+    bool has_rg = false;
+    std::string rg_id;
+    std::vector<std::string> rg_fields;
+    uint64_t max_secondary = 0;
+    float dropoff = 0.5f;
+    uint32_t rescue_distance = 100;
+    uint64_t max_tries = 20;
+    uint32_t mcs = 0;
+    uint64_t chunk_size = 1000000;
+    int threads = 1;
+    uint64_t batch_bases = 64ull << 20;  // bases per device batch
+};
+
+// src/io/sam.rs:177-201 (no @PG line)
+std::string sam_header(const PackedRef& ref, const Options& o);
+
+struct RunStats {
+    double nam_ms = 0, extend_ms = 0, host_ms = 0, upload_ms = 0, total_ms = 0;
+    uint64_t records = 0, requests = 0, rounds = 0, nams = 0;
+};
+
+// Maps all records (paired when r2 is non-empty) and appends the output
+// (without header) to `out`, in input order. Abundances (mode 2) are written
+// as the final table.
+void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
+                 const Options& o, std::string& out, RunStats* stats);
+
+}  // namespace sa

@@ -1,0 +1,273 @@
+// Reference/read input and host-side index construction for the drop-in.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <thread>
+
+#include "sa_host.hpp"
+
+namespace sa {
+
+namespace {
+void rstrip(std::string& s) {
+    while (!s.empty() && (s.back() == ' ' || s.back() == '\t' || s.back() == '\n' || s.back() == '\r' ||
+                          s.back() == '\f' || s.back() == '\v'))
+        s.pop_back();
+}
+
+// src/io/mod.rs:32-38
+void split_header(const std::string& h0, std::string& name, std::string& comment, bool& has_comment) {
+    std::string h = h0;
+    rstrip(h);
+    size_t p = h.find_first_of(" \t");
+    if (p == std::string::npos) { name = h; comment.clear(); has_comment = false; }
+    else { name = h.substr(0, p); comment = h.substr(p + 1); has_comment = true; }
+}
+
+template <typename F>
+void parallel_for(int threads, size_t n, F f) {
+    if (threads <= 1 || n <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    for (int t = 0; t < threads; ++t)
+        th.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto& t : th) t.join();
+}
+}  // namespace
+
+// src/io/fasta.rs:116-165 and src/packed_seq.rs:56-92 (wildcards -> position hash)
+PackedRef read_fasta(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + path);
+    PackedRef r;
+    std::string line;
+    uint64_t cur_len = 0;
+    bool have = false;
+    r.word_off.push_back(0);
+    auto finish = [&]() {
+        if (!have) return;
+        r.len.push_back(cur_len);
+        r.word_off.push_back(r.words.size());
+    };
+    while (std::getline(f, line)) {
+        if (!line.empty() && line[0] == '>') {
+            finish();
+            std::string name, comment;
+            bool hc;
+            split_header(line.substr(1), name, comment, hc);
+            r.names.push_back(name);
+            cur_len = 0;
+            have = true;
+        } else if (have) {
+            rstrip(line);
+            for (unsigned char c : line) {
+                uint64_t b;
+                switch (c) {
+                    case 'A': case 'a': b = 0; break;
+                    case 'C': case 'c': b = 1; break;
+                    case 'G': case 'g': b = 2; break;
+                    case 'T': case 't': case 'U': case 'u': b = 3; break;
+                    default: {
+                        uint64_t h = cur_len * 0x9e3779b97f4a7c15ULL;
+                        b = (h ^ (h >> 30)) & 3;
+                    }
+                }
+                uint64_t bit = (cur_len & 31) * 2;
+                if (bit == 0) r.words.push_back(b);
+                else r.words.back() |= b << bit;
+                ++cur_len;
+            }
+        }
+    }
+    finish();
+    if (r.names.empty()) throw std::runtime_error("No sequences found in the reference FASTA");
+    return r;
+}
+
+// src/seeding/parameters.rs:116-159, 181-221
+SeedParams seed_params_for_read_length(uint64_t read_length) {
+    struct Row { uint64_t canon, thr; int k, s, wmin, wmax; };
+    static const Row rows[] = {
+        {50, 70, 18, 14, 1, 4},   {75, 90, 20, 16, 1, 6},    {100, 110, 20, 16, 2, 6},
+        {125, 135, 20, 16, 3, 8}, {150, 175, 20, 16, 5, 11}, {250, 375, 22, 18, 6, 16},
+        {400, ~0ull, 23, 17, 5, 15},
+    };
+    const Row* r = &rows[6];
+    for (const Row& x : rows)
+        if (read_length <= x.thr) { r = &x; break; }
+    SeedParams p{};
+    p.k = r->k;
+    p.s = r->s;
+    p.t = (r->k - r->s) / 2 + 1;
+    p.w_min = r->wmin;
+    p.w_max = r->wmax;
+    int64_t md = (int64_t)r->canon - 70;
+    if (md < 0) md = 0;
+    md = std::max<int64_t>(md, r->k);
+    md = std::min<int64_t>(md, 255);
+    p.max_dist = (int32_t)md;
+    p.q = 255;
+    p.main_mask = ~0ULL << (9 + 17);
+    p.partial_pos = __builtin_ctzll(p.main_mask) - 1;
+    p.fwd_main_mask = p.main_mask | (1ULL << p.partial_pos);
+    return p;
+}
+
+int pick_bits(const PackedRef& ref, const SeedParams& p) {
+    uint64_t total = 0;
+    for (uint64_t l : ref.len) total += l;
+    uint64_t est = total / (uint64_t)(p.k - p.s + 1) + 1;
+    int64_t b = (int64_t)std::log2((double)est);
+    b = std::min<int64_t>(std::max<int64_t>(b, 9), 32);
+    return (int)b - 1;
+}
+
+// src/indexer.rs:18-170, 186-253
+void build_index(const PackedRef& ref, SeedParams p, int bits, double filter_fraction, int threads, HostIndex& out) {
+    const uint32_t nc = (uint32_t)ref.names.size();
+    p.bits = bits;
+    std::vector<std::vector<RefEntry>> per(nc);
+    parallel_for(threads, nc, [&](size_t c) {
+        uint64_t L = ref.len[c];
+        if (L < (uint64_t)p.w_max) return;
+        if (L >= (1ull << 31)) throw std::runtime_error("contigs longer than 2^31 bp are not supported");
+        const uint64_t* w = ref.words.data() + ref.word_off[c];
+        std::vector<Syncmer> sy;
+        sy.reserve(L / 4 + 16);
+        syncmer_scan((uint32_t)L, p, [&](uint32_t i) { return (uint32_t)(w[i >> 5] >> ((i & 31) * 2)) & 3; },
+                     [&](uint64_t h, bool fwd, uint32_t pos) { sy.push_back(Syncmer{h, (pos << 1) | (fwd ? 1u : 0u), 0}); });
+        std::vector<RefEntry>& v = per[c];
+        v.resize(sy.size());
+        const uint32_t n = (uint32_t)sy.size();
+        for (uint32_t a = 0; a < n; ++a) {
+            uint32_t b = pick_strobe2(a, n, p, [&](uint32_t i) { return sy[i]; });
+            uint64_t h = randstrobe_hash(sy[a].hash, sy[b].hash, sy[a].pos_fwd & 1, sy[b].pos_fwd & 1, p);
+            uint32_t p1 = sy[a].pos_fwd >> 1, p2 = sy[b].pos_fwd >> 1;
+            v[a] = RefEntry{(h & REF_HASH_MASK) | (uint64_t)((p2 - p1) & 0xFF), p1, (uint32_t)c};
+        }
+    });
+    std::vector<RefEntry>& rs = out.entries;
+    size_t total = 0;
+    for (auto& v : per) total += v.size();
+    rs.clear();
+    rs.reserve(total);
+    for (auto& v : per) { rs.insert(rs.end(), v.begin(), v.end()); std::vector<RefEntry>().swap(v); }
+    auto less = [](const RefEntry& a, const RefEntry& b) {
+        if (a.hash_offset != b.hash_offset) return a.hash_offset < b.hash_offset;
+        if (a.position != b.position) return a.position < b.position;
+        return a.ref_index < b.ref_index;
+    };
+    // parallel sort: sort T slices, then merge pairwise (the key is a total order)
+    int T = std::max(1, std::min(threads, 64));
+    std::vector<size_t> bounds;
+    for (int t = 0; t <= T; ++t) bounds.push_back(total * (size_t)t / (size_t)T);
+    parallel_for(T, (size_t)T, [&](size_t t) { std::sort(rs.begin() + bounds[t], rs.begin() + bounds[t + 1], less); });
+    for (size_t width = 1; width < (size_t)T; width *= 2) {
+        std::vector<size_t> starts;
+        for (size_t t = 0; t + width < (size_t)T; t += 2 * width) starts.push_back(t);
+        parallel_for(T, starts.size(), [&](size_t x) {
+            size_t t = starts[x];
+            size_t lo = bounds[t], mid = bounds[t + width], hi = bounds[std::min((size_t)T, t + 2 * width)];
+            std::inplace_merge(rs.begin() + lo, rs.begin() + mid, rs.begin() + hi, less);
+        });
+    }
+    // bucket directory exactly as src/indexer.rs:77-129 builds it
+    std::vector<uint64_t>& bs = out.bucket_starts;
+    bs.clear();
+    bs.reserve((1ull << bits) + 1);
+    std::vector<uint32_t> counts;
+    uint64_t unique = rs.empty() ? 0 : 1;
+    uint64_t prev = rs.empty() ? 0 : (rs[0].hash_offset & REF_HASH_MASK);
+    uint64_t count = 1;
+    if (!rs.empty()) bs.push_back(0);
+    for (size_t pos = 1; pos < rs.size(); ++pos) {
+        uint64_t cur = rs[pos].hash_offset & REF_HASH_MASK;
+        if (cur == prev) { ++count; continue; }
+        ++unique;
+        if (count != 1) counts.push_back((uint32_t)std::min<uint64_t>(count, 0xFFFFFFFFu));
+        count = 1;
+        uint64_t top = cur >> (64 - bits);
+        while (bs.size() <= top) bs.push_back(pos);
+        prev = cur;
+    }
+    if (!rs.empty() && count != 1) counts.push_back((uint32_t)std::min<uint64_t>(count, 0xFFFFFFFFu));
+    while (bs.size() < (1ull << bits) + 1) bs.push_back(rs.size());
+    std::sort(counts.begin(), counts.end(), std::greater<uint32_t>());
+    uint64_t index_cutoff = (uint64_t)((double)unique * filter_fraction);
+    uint64_t fc = index_cutoff < counts.size() ? counts[index_cutoff] : (counts.empty() ? 30 : counts.back());
+    p.filter_cutoff = std::min<uint64_t>(std::max<uint64_t>(fc, 30), 100);
+    p.n_entries = rs.size();
+    out.p = p;
+}
+
+// src/io/fastq.rs:22-95 and FASTA reads (src/io/reads.rs:121-135)
+bool read_records(const std::string& path, std::vector<Record>& out, std::string& err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { err = "cannot open " + path; return false; }
+    std::string line;
+    if (f.peek() == '>') {
+        Record cur;
+        bool have = false;
+        while (std::getline(f, line)) {
+            if (!line.empty() && line[0] == '>') {
+                if (have) out.push_back(std::move(cur));
+                cur = Record();
+                split_header(line.substr(1), cur.name, cur.comment, cur.has_comment);
+                cur.has_qual = false;
+                have = true;
+            } else if (have) {
+                rstrip(line);
+                for (char& c : line) c = (char)toupper((unsigned char)c);
+                cur.seq += line;
+            }
+        }
+        if (have) out.push_back(std::move(cur));
+        return true;
+    }
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] != '@') { err = "FASTQ record must start with '@'"; return false; }
+        Record r;
+        split_header(line.substr(1), r.name, r.comment, r.has_comment);
+        if (r.name.empty()) { err = "FASTQ record identifier is empty"; return false; }
+        if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/1") == 0) { r.name.resize(r.name.size() - 2); r.end = 1; }
+        else if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/2") == 0) { r.name.resize(r.name.size() - 2); r.end = 2; }
+        std::string plus;
+        std::getline(f, r.seq);
+        if (!r.seq.empty() && r.seq.back() == '\r') r.seq.pop_back();
+        std::getline(f, plus);
+        std::getline(f, r.qual);
+        if (!r.qual.empty() && r.qual.back() == '\r') r.qual.pop_back();
+        if (r.seq.size() != r.qual.size()) { err = "sequence and quality lengths differ in record " + r.name; return false; }
+        for (char& c : r.seq) c = (char)toupper((unsigned char)c);
+        out.push_back(std::move(r));
+    }
+    return true;
+}
+
+// src/main.rs:1001-1012
+uint64_t estimate_read_length(const std::vector<Record>& recs) {
+    uint64_t s = 0, n = 0;
+    for (size_t i = 0; i < recs.size() && i < 500; ++i) { s += recs[i].seq.size(); ++n; }
+    return n ? s / n : 0;
+}
+
+std::string sam_header(const PackedRef& ref, const Options& o) {
+    std::string h = "@HD\tVN:1.6\tSO:unsorted\n";
+    for (size_t i = 0; i < ref.names.size(); ++i) h += "@SQ\tSN:" + ref.names[i] + "\tLN:" + std::to_string(ref.len[i]) + "\n";
+    if (o.has_rg) {
+        h += "@RG\tID:" + o.rg_id;
+        for (const auto& f : o.rg_fields) h += "\t" + f;
+        h += "\n";
+    }
+    return h;
+}
+
+}  // namespace sa

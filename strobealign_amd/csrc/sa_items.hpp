@@ -1,0 +1,217 @@
+// Work items of the mapping kernels: one call handles one read, one read
+// orientation or one extension request. The __global__ kernels in
+// sa_kernels.hip map threads onto these items.
+#pragma once
+#include "sa_extend.hpp"
+
+namespace sa {
+
+// --------------------------------------------------------------- seeding ---
+SA_HD void item_seed(uint32_t i, const uint8_t* seq, const uint64_t* off,
+                                             uint32_t n, SeedParams p, Syncmer* sy,
+                                             QRand* qr, uint32_t* qr_n) {
+    if (i >= n) return;
+    uint64_t o = off[i];
+    uint32_t L = (uint32_t)(off[i + 1] - o);
+    read_randstrobes(seq + o, L, p, sy + o, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1]);
+}
+
+// ------------------------------------------------------------------ hits ---
+SA_HD void item_hits(uint32_t t, IndexView ix, MapParams mp, const uint64_t* off,
+                                             uint32_t n, const QRand* qr,
+                                             const uint32_t* qr_n, Hit* hits,
+                                             uint64_t* cnt, HitCounts* hc,
+                                             uint64_t* acap) {
+    if (t >= 2 * n) return;
+    uint32_t i = t >> 1, o = t & 1;
+    uint64_t ro = off[i];
+    uint32_t L = (uint32_t)(off[i + 1] - ro);
+    uint64_t base = 2 * ro + (uint64_t)o * L;
+    HitCounts d;
+    uint32_t nh = find_hits(ix, qr + base, qr_n[t], mp, hits + base, cnt + base, d);
+    hc[t] = d;
+    acap[t] = hit_runs(ix, hits + base, nh);
+}
+
+// -------------------------------------------------------------- chaining ---
+struct ChainBufs {
+    Anchor* anchors;
+    float* dp;
+    uint32_t* pred;
+    uint32_t* cand;
+    uint8_t* used;
+    NamRec* nams;
+    uint32_t* chain_anchors;
+};
+
+SA_HD void item_chain(uint32_t t, IndexView ix, ChainParams cp, const uint64_t* off,
+                                              uint32_t n, const Hit* hits,
+                                              const HitCounts* hc, const uint64_t* acap,
+                                              const uint64_t* aoff, ChainBufs b,
+                                              uint32_t* nam_n, uint32_t* anchors_raw,
+                                              const float* log2tab,
+                                              const float* same_diag) {
+    if (t >= 2 * n) return;
+    uint32_t i = t >> 1, o = t & 1;
+    const HitCounts& h0 = hc[2 * i];
+    const HitCounts& h1 = hc[2 * i + 1];
+    bool sel;
+    if (better_orientation(h0, h1)) sel = (o == 0);
+    else if (better_orientation(h1, h0)) sel = (o == 1);
+    else sel = true;
+    nam_n[t] = 0;
+    anchors_raw[t] = 0;
+    if (!sel) return;
+    uint64_t ro = off[i];
+    uint32_t L = (uint32_t)(off[i + 1] - ro);
+    uint64_t base = 2 * ro + (uint64_t)o * L;
+    uint64_t ab = aoff[t];
+    ChainScratch s{b.anchors + ab, b.dp + ab, b.pred + ab, b.cand + ab, b.used + ab, (uint32_t)acap[t]};
+    uint32_t na = hits_to_anchors(ix, hits + base, hc[t].n_hits, s.anchors);
+    anchors_raw[t] = na;
+    na = sort_dedup_anchors(s.anchors, na);
+    nam_n[t] = chain_orientation(s, na, (uint32_t)ix.p.k, L, o == 1, cp, log2tab, same_diag, b.nams + ab,
+                                 b.chain_anchors + 2 * (2 * ab), 2 * ab);
+}
+
+// Merge both orientations, number NAMs like src/chainer.rs:435-436, stable-sort by
+// score (src/nam.rs:170) and record the consistency checks of src/nam.rs:49-135.
+SA_HD void item_finalize(uint32_t i, const uint8_t* seq, const uint64_t* off,
+                                                 uint32_t n, RefView ref, uint32_t k,
+                                                 const uint32_t* nam_n,
+                                                 const uint64_t* aoff, const NamRec* nams_in,
+                                                 const uint64_t* out_off, NamRec* nams_out) {
+    if (i >= n) return;
+    uint32_t n0 = nam_n[2 * i], n1 = nam_n[2 * i + 1];
+    NamRec* out = nams_out + out_off[i];
+    const NamRec* a0 = nams_in + aoff[2 * i];
+    const NamRec* a1 = nams_in + aoff[2 * i + 1];
+    for (uint32_t x = 0; x < n0; ++x) out[x] = a0[x];
+    for (uint32_t x = 0; x < n1; ++x) {
+        out[n0 + x] = a1[x];
+        out[n0 + x].nam_id += n0;
+    }
+    uint32_t m = n0 + n1;
+    // stable: order by (score desc, nam_id asc); nam_id equals the concatenation index
+    heap_sort(out, m, [](const NamRec& a, const NamRec& b) {
+        return a.score > b.score || (a.score == b.score && a.nam_id < b.nam_id);
+    });
+    uint64_t ro = off[i];
+    uint32_t L = (uint32_t)(off[i + 1] - ro);
+    for (uint32_t x = 0; x < m; ++x) {
+        NamRec& r = out[x];
+        bool cons = nam_consistent(ref, seq + ro, L, r.flags & 1, r.ref_id, r.ref_start, r.ref_end, r.query_start,
+                                   r.query_end, k);
+        uint32_t rev = cons ? 0u : nam_reverse_state(ref, seq + ro, L, r, k);
+        r.flags = (r.flags & 1u) | (cons ? 2u : 0u) | (rev << 2);
+    }
+}
+
+// ------------------------------------------------------------- extension ---
+// extend_seed (src/mapper.rs:532-620) and rescue_align (src/mapper.rs:1094-1153).
+SA_HD void item_extend(uint32_t t, const uint8_t* seq, const uint64_t* off,
+                                               RefView ref, Scores sc, uint32_t k,
+                                               const ExtRequest* req, uint32_t nreq,
+                                               ExtResult* res, uint32_t* cigars,
+                                               const uint32_t* chain_anchors, uint8_t* scratch,
+                                               uint64_t scratch_per, uint32_t first) {
+    if (t >= nreq) return;
+    const uint32_t ri = first + t;
+    const ExtRequest r = req[ri];
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    uint8_t* scr = scratch + (uint64_t)t * scratch_per;
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+    const uint64_t rlen = ref.len[r.ref_id];
+    if (r.kind == EXT_SEED) {
+        const bool use_ssw = r.flags & 1, consistent = (r.flags & 2) != 0;
+        uint64_t prs = r.ref_start > r.query_start ? r.ref_start - r.query_start : 0;
+        uint64_t pre = (uint64_t)r.ref_end + L - r.query_end;
+        if (pre > rlen) pre = rlen;
+        bool gapped = true;
+        AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
+        uint64_t result_ref_start = 0;
+        if (prs + L == pre && consistent) {
+            RefWin w{&ref, r.ref_id, prs, L};
+            uint32_t hd = 0;
+            for (uint32_t x = 0; x < L; ++x) hd += q[x] != w[x];
+            if ((float)hd / (float)L < 0.05f) {
+                a = hamming_align(q, w, L, sc, cig);
+                result_ref_start = prs + a.ref_start;
+                gapped = false;
+            }
+        }
+        if (gapped) {
+            uint32_t padding = L / 10;
+            if (use_ssw) {
+                uint64_t ws = prs > padding ? prs - padding : 0;
+                uint64_t we = pre + padding < rlen ? pre + padding : rlen;
+                RefWin w{&ref, r.ref_id, ws, (uint32_t)(we - ws)};
+                a = ssw_aligner_align(q, w, sc, scr, scratch_per, cig);
+                result_ref_start = ws + a.ref_start;
+            } else {
+                uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
+                uint64_t de = (uint64_t)r.ref_end + L + padding;
+                if (de > rlen) de = rlen;
+                RefWin w{&ref, r.ref_id, ds, (uint32_t)(de - ds)};
+                // copy the chain anchors into scratch, prune them, then extend
+                uint32_t* an = (uint32_t*)scr;
+                uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
+                if (an_bytes >= scratch_per) {
+                    a.status = 2;
+                } else {
+                    for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
+                    uint32_t na = remove_spurious_anchors(an, r.n_anchors);
+                    a = extend_piecewise(q, w, an, na, padding, sc, k, scr + an_bytes, scratch_per - an_bytes, cig);
+                    result_ref_start = a.ref_start + ds;
+                }
+            }
+        }
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) {
+            out.found = 1;
+            out.gapped = gapped ? 1 : 0;
+            out.score = a.score;
+            out.ref_start = (uint32_t)result_ref_start;
+            out.length = a.ref_end - a.ref_start;
+            out.query_start = a.query_start;
+            out.query_end = a.query_end;
+            out.edit_distance = a.edit_distance;
+            out.n_cigar = cig.n;
+        }
+    } else {
+        // rescue: shared-substring test, then SSW over the window
+        uint32_t ws = r.win_start, we = r.win_end;
+        RefWin w{&ref, r.ref_id, ws, we - ws};
+        uint32_t sub = 2 * k / 3, step = k / 3;
+        bool shared = (r.flags & 8) != 0;  // bit3: plain alignment without the substring precheck
+        uint32_t lim = L > sub ? L - sub : 0;
+        for (uint32_t i0 = 0; i0 < lim && !shared; i0 += step) {
+            for (uint32_t p = 0; p + sub <= w.len && !shared; ++p) {
+                uint32_t x = 0;
+                while (x < sub && w[p + x] == q[i0 + x]) ++x;
+                if (x == sub) shared = true;
+            }
+        }
+        if (shared) {
+            AlignInfo a = ssw_aligner_align(q, w, sc, scr, scratch_per, cig);
+            out.status = (uint32_t)a.status;
+            if (a.found && a.status == 0) {
+                out.found = 1;
+                out.gapped = 1;
+                out.score = a.score;
+                out.ref_start = ws + a.ref_start;
+                out.length = a.ref_end - a.ref_start;
+                out.query_start = a.query_start;
+                out.query_end = a.query_end;
+                out.edit_distance = a.edit_distance;
+                out.n_cigar = cig.n;
+            }
+        }
+    }
+    res[ri] = out;
+}
+
+}  // namespace sa

@@ -1,0 +1,1243 @@
+// Read-level decisions of src/mapper.rs and src/maponly.rs, replayed on the
+// host over device results.
+//
+// The device computes NAMs and every base-level extension. The decisions that
+// depend on chunk-sequential state -- the per-chunk random generator (fastrand
+// Rng, src/main.rs:878) and insert-size estimate (src/main.rs:879) -- are taken
+// here, in input order, one host thread per chunk. An extension the replay needs
+// but the device has not computed yet is a "miss": the chunk stops at that
+// record, the remaining records are replayed in probe mode to predict their
+// misses too, all misses of all chunks go to the device as one batch, and the
+// chunk resumes. Output is only written by a replay without misses, so it is
+// identical to a sequential run.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <array>
+#include <deque>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <thread>
+#include <unordered_map>
+
+#include "sa_host.hpp"
+
+namespace sa {
+
+namespace {
+
+const uint16_t PAIRED = 1, PROPER_PAIR = 2, UNMAP = 4, MUNMAP = 8, REVERSE = 0x10, MREVERSE = 0x20, READ1 = 0x40,
+               READ2 = 0x80, SECONDARY = 0x100;  // src/io/sam.rs:7-18
+
+// ------------------------------------------------------------------ rng ---
+// fastrand 2.3.0 Rng: wyrand (v4.2 constants) with Lemire bounded sampling.
+struct Rng {
+    uint64_t s = 0;
+    uint64_t u64() {
+        s += 0x2d358dccaa6c78a5ULL;
+        unsigned __int128 t = (unsigned __int128)s * (unsigned __int128)(s ^ 0x8bb84b93962eacc9ULL);
+        return (uint64_t)t ^ (uint64_t)(t >> 64);
+    }
+    uint32_t mod32(uint32_t n) {
+        uint32_t r = (uint32_t)u64();
+        uint64_t m = (uint64_t)r * n;
+        uint32_t hi = (uint32_t)(m >> 32), lo = (uint32_t)m;
+        if (lo < n) {
+            uint32_t t = (uint32_t)(-n) % n;
+            while (lo < t) {
+                r = (uint32_t)u64();
+                m = (uint64_t)r * n;
+                hi = (uint32_t)(m >> 32);
+                lo = (uint32_t)m;
+            }
+        }
+        return hi;
+    }
+    uint64_t mod64(uint64_t n) {
+        uint64_t r = u64();
+        unsigned __int128 m = (unsigned __int128)r * n;
+        uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
+        if (lo < n) {
+            uint64_t t = (uint64_t)(-n) % n;
+            while (lo < t) {
+                r = u64();
+                m = (unsigned __int128)r * n;
+                hi = (uint64_t)(m >> 64);
+                lo = (uint64_t)m;
+            }
+        }
+        return hi;
+    }
+};
+
+// src/insertsize.rs:3-50
+struct InsertSize {
+    uint64_t sample_size = 0;
+    float mu = 300.0f, sigma = 100.0f, variance = 10000.0f, sse = 10000.0f;
+    void update(uint64_t x0) {
+        if (x0 >= 2000) return;
+        if (sample_size > 0) {
+            float x = (float)x0;
+            float e = x - mu;
+            mu += e / (float)sample_size;
+            sse += e * (x - mu);
+            if (sample_size > 1) variance = sse / ((float)sample_size - 1.0f);
+            else variance = sse;
+            sigma = std::sqrt(variance);
+        }
+        sample_size += 1;
+    }
+};
+
+// src/math.rs:1-7
+float normal_pdf(float x, float mu, float sigma) {
+    float a = (x - mu) / sigma;
+    return (float)0.3989422804014327 / sigma * std::exp(-0.5f * a * a);
+}
+
+struct Nam {
+    uint32_t nam_id, ref_id, ref_start, ref_end, query_start, query_end, matching_bases, n_anchors;
+    float score;
+    bool is_revcomp;
+    uint8_t cons, rev;  // is_consistent as is; reverse_nam_if_needed outcome (0 as is, 1 flip, 2 none)
+    uint64_t anchor_off;
+    uint32_t projected_ref_start() const { return ref_start > query_start ? ref_start - query_start : 0; }
+};
+
+Nam to_nam(const NamRec& r) {
+    return Nam{r.nam_id, r.ref_id, r.ref_start, r.ref_end, r.query_start, r.query_end, r.matching_bases, r.n_anchors,
+               r.score, (r.flags & 1) != 0, (uint8_t)((r.flags >> 1) & 1), (uint8_t)((r.flags >> 2) & 3), r.anchor_off};
+}
+
+// src/nam.rs:94-135 using the device's consistency verdict
+bool reverse_nam_if_needed(Nam& n, uint32_t L) {
+    if (n.rev == 0) return true;
+    if (n.rev == 1) {
+        n.is_revcomp = !n.is_revcomp;
+        uint32_t qs = L - n.query_end, qe = L - n.query_start;
+        n.query_start = qs;
+        n.query_end = qe;
+        n.cons = 1;
+        n.rev = 0;
+        return true;
+    }
+    return false;
+}
+
+struct Alignment {
+    uint32_t ref_id = 0, ref_start = 0;
+    const uint32_t* cigar = nullptr;  // BAM ops (=, X, I, D)
+    uint32_t n_cigar = 0;
+    uint32_t edit_distance = 0, soft_clip_left = 0, soft_clip_right = 0, score = 0, length = 0;
+    bool is_revcomp = false, gapped = false;
+    uint32_t global_ed() const { return edit_distance + soft_clip_left + soft_clip_right; }
+};
+
+struct Details {
+    ReadNamInfo nam{};
+    uint32_t inconsistent_nams = 0, mate_rescue = 0, tried_alignment = 0, gapped = 0, best_alignments = 0;
+};
+
+// ------------------------------------------------------------ cache key ---
+struct Key {
+    uint32_t kind, read, flags, ref_id, a, b, c, d;
+    uint64_t e;
+    bool operator==(const Key& o) const { return std::memcmp(this, &o, sizeof(Key)) == 0; }
+};
+struct KeyHash {
+    size_t operator()(const Key& k) const {
+        const uint64_t* w = (const uint64_t*)&k;
+        uint64_t h = 0x9e3779b97f4a7c15ULL;
+        for (size_t i = 0; i < sizeof(Key) / 8; ++i) h = (h ^ w[i]) * 0xff51afd7ed558ccdULL, h ^= h >> 29;
+        return (size_t)h;
+    }
+};
+static_assert(sizeof(Key) == 40, "Key packing");
+
+struct Miss {};
+
+struct CachedResult {
+    ExtResult r;
+    std::vector<uint32_t> cigar;
+};
+
+// Per-chunk state of the replay.
+struct Chunk {
+    uint64_t first, count;  // records (pairs) [first, first+count)
+    uint64_t next = 0;       // next record to commit (relative)
+    Rng rng;
+    InsertSize isize;
+    std::string out;
+    std::vector<double> abundances;
+    std::unordered_map<Key, CachedResult, KeyHash> cache;
+    std::vector<Key> pending;
+    std::unordered_map<Key, char, KeyHash> pending_set;
+    bool done() const { return next >= count; }
+};
+
+struct Ctx {
+    const Options& o;
+    const PackedRef& ref;
+    const NamBatch& nb;
+    const std::vector<Record>& r1;
+    const std::vector<Record>& r2;
+    bool paired;
+    uint64_t batch_first;  // first record index of the engine batch
+    uint32_t k;
+    Chunk* ch;
+    bool probe;
+    Rng* rng;
+    InsertSize* isize;
+};
+
+// Look up (or register a miss for) an extension result.
+const CachedResult* lookup(Ctx& c, const Key& key) {
+    auto it = c.ch->cache.find(key);
+    if (it != c.ch->cache.end()) return &it->second;
+    if (!c.ch->pending_set.count(key)) {
+        c.ch->pending_set.emplace(key, 1);
+        c.ch->pending.push_back(key);
+    }
+    if (c.probe) return nullptr;
+    throw Miss{};
+}
+
+// src/mapper.rs:532-620 (device request kind EXT_SEED)
+std::optional<Alignment> extend_seed(Ctx& c, const Nam& nam, uint32_t read_id, uint32_t L, bool consistent,
+                                     bool use_ssw) {
+    Key key{EXT_SEED, read_id, (use_ssw ? 1u : 0u) | (consistent ? 2u : 0u) | (nam.is_revcomp ? 4u : 0u), nam.ref_id,
+            nam.ref_start, nam.ref_end, nam.query_start, nam.query_end, use_ssw ? 0 : nam.anchor_off};
+    if (!use_ssw) key.e |= (uint64_t)nam.n_anchors << 48;
+    const CachedResult* r = lookup(c, key);
+    if (!r || !r->r.found) return std::nullopt;
+    Alignment a;
+    a.ref_id = nam.ref_id;
+    a.ref_start = r->r.ref_start;
+    a.cigar = r->cigar.data();
+    a.n_cigar = (uint32_t)r->cigar.size();
+    a.edit_distance = r->r.edit_distance;
+    a.soft_clip_left = r->r.query_start;
+    a.soft_clip_right = L - r->r.query_end;
+    a.score = r->r.score;
+    a.length = r->r.length;
+    a.is_revcomp = nam.is_revcomp;
+    a.gapped = r->r.gapped != 0;
+    return a;
+}
+
+// src/mapper.rs:1094-1153 (device request kind EXT_RESCUE)
+std::optional<Alignment> rescue_align(Ctx& c, const Nam& mate, uint32_t read_id, uint32_t L, float mu, float sigma) {
+    float w = mu + 5.0f * sigma;
+    uint64_t wi = w > 0 ? (uint64_t)w : 0;
+    uint64_t rs, re;
+    bool rc;
+    if (mate.is_revcomp) {
+        rc = false;
+        uint64_t p = mate.projected_ref_start();
+        rs = p > wi ? p - wi : 0;
+        re = p + L / 2;
+    } else {
+        rc = true;
+        uint64_t b = (uint64_t)mate.ref_end + L - mate.query_end;
+        rs = b > L / 2 ? b - L / 2 : 0;
+        re = b + wi;
+    }
+    uint64_t rl = c.ref.len[mate.ref_id];
+    rs = std::min(rs, rl);
+    re = std::min(re, rl);
+    if (re < rs + c.k) return std::nullopt;
+    Key key{EXT_RESCUE, read_id, rc ? 4u : 0u, mate.ref_id, (uint32_t)rs, (uint32_t)re, 0, 0, 0};
+    const CachedResult* r = lookup(c, key);
+    if (!r || !r->r.found) return std::nullopt;
+    Alignment a;
+    a.ref_id = mate.ref_id;
+    a.ref_start = r->r.ref_start;
+    a.cigar = r->cigar.data();
+    a.n_cigar = (uint32_t)r->cigar.size();
+    a.edit_distance = r->r.edit_distance;
+    a.soft_clip_left = r->r.query_start;
+    a.soft_clip_right = L - r->r.query_end;
+    a.score = r->r.score;
+    a.length = r->r.length;
+    a.is_revcomp = !mate.is_revcomp;
+    a.gapped = true;
+    return a;
+}
+
+// ----------------------------------------------------------------- output ---
+void put_u(std::string& s, uint64_t v) {
+    char b[24];
+    int n = 0;
+    do { b[n++] = (char)('0' + v % 10); v /= 10; } while (v);
+    while (n) s += b[--n];
+}
+void put_i(std::string& s, int64_t v) {
+    if (v < 0) { s += '-'; put_u(s, (uint64_t)(-v)); }
+    else put_u(s, (uint64_t)v);
+}
+
+// SAM record fields (src/io/sam.rs:21-43)
+struct Sam {
+    const Record* rec;
+    uint16_t flags = 0;
+    int64_t rname = -1;  // reference id
+    int64_t pos = -1;
+    uint8_t mapq = 0;
+    const Alignment* al = nullptr;  // CIGAR source (mapped)
+    bool rnext_eq = false;
+    int64_t rnext = -1;
+    int64_t mate_pos = -1;
+    bool has_tlen = false;
+    int32_t tlen = 0;
+    bool revseq = false;
+    const Details* det = nullptr;
+    bool is_mapped() const { return (flags & UNMAP) == 0; }
+};
+
+void write_cigar(std::string& s, const Alignment& a, bool eqx) {
+    static const char L[] = "MIDNSHP=X";
+    uint32_t pend_op = 99, pend_len = 0;
+    auto push = [&](uint32_t op, uint32_t len) {
+        if (len == 0) return;
+        if (!eqx && (op == 7 || op == 8)) op = 0;
+        if (op == pend_op) { pend_len += len; return; }
+        if (pend_len) { put_u(s, pend_len); s += L[pend_op]; }
+        pend_op = op;
+        pend_len = len;
+    };
+    push(4, a.soft_clip_left);
+    for (uint32_t i = 0; i < a.n_cigar; ++i) push(a.cigar[i] & 0xF, a.cigar[i] >> 4);
+    push(4, a.soft_clip_right);
+    if (pend_len) { put_u(s, pend_len); s += L[pend_op]; }
+}
+
+// src/io/sam.rs:53-138
+void write_sam(std::string& s, const Sam& r, const Ctx& c) {
+    s += r.rec->name; s += '\t';
+    put_u(s, r.flags); s += '\t';
+    if (r.rname >= 0) s += c.ref.names[r.rname]; else s += '*';
+    s += '\t';
+    put_u(s, r.pos >= 0 ? (uint64_t)r.pos + 1 : 0); s += '\t';
+    put_u(s, r.mapq); s += '\t';
+    if (r.al) write_cigar(s, *r.al, c.o.eqx); else s += '*';
+    s += '\t';
+    if (r.rnext_eq) s += '=';
+    else if (r.rnext >= 0) s += c.ref.names[r.rnext];
+    else s += '*';
+    s += '\t';
+    put_u(s, r.mate_pos >= 0 ? (uint64_t)r.mate_pos + 1 : 0); s += '\t';
+    put_i(s, r.has_tlen ? r.tlen : 0); s += '\t';
+    bool sec = (r.flags & SECONDARY) != 0;
+    const std::string& seq = r.rec->seq;
+    if (sec || seq.empty()) s += '*';
+    else if (r.revseq) for (size_t i = seq.size(); i-- > 0;) s += (char)comp_ascii((uint8_t)seq[i]);
+    else s += seq;
+    s += '\t';
+    const std::string& q = r.rec->qual;
+    if (sec || !r.rec->has_qual || q.empty()) s += '*';
+    else if (r.revseq) s.append(q.rbegin(), q.rend());
+    else s += q;
+    if (r.al) { s += "\tNM:i:"; put_u(s, r.al->edit_distance); s += "\tAS:i:"; put_u(s, r.al->score); }
+    if (c.o.details && r.det) {
+        s += "\tna:i:"; put_u(s, r.det->nam.n_nams);
+        s += "\tal:i:"; put_u(s, r.det->tried_alignment);
+        s += "\tga:i:"; put_u(s, r.det->gapped);
+        s += "\tX0:i:"; put_u(s, r.det->best_alignments);
+        if (r.flags & PAIRED) { s += "\tmr:i:"; put_u(s, r.det->mate_rescue); }
+    }
+    if (c.o.has_rg) { s += "\tRG:Z:"; s += c.o.rg_id; }
+    if (c.o.fastq_comments && r.rec->has_comment) { s += '\t'; s += r.rec->comment; }
+    s += '\n';
+}
+
+// src/mapper.rs:115-243
+Sam make_unmapped(const Record& rec, const Details* d) {
+    Sam r;
+    r.rec = &rec;
+    r.flags = UNMAP;
+    r.det = d;
+    return r;
+}
+Sam make_mapped(const Alignment& a, const Record& rec, uint8_t mapq, bool secondary, const Details* d) {
+    Sam r;
+    r.rec = &rec;
+    if (a.is_revcomp) r.flags |= REVERSE;
+    if (secondary) { mapq = 0; r.flags |= SECONDARY; }
+    r.revseq = a.is_revcomp;
+    r.al = &a;
+    r.rname = a.ref_id;
+    r.pos = a.ref_start;
+    r.mapq = mapq;
+    r.det = d;
+    return r;
+}
+
+// src/mapper.rs:245-345
+void make_paired(const Alignment* al[2], const Record* recs[2], const uint8_t mq[2], const Details det[2],
+                 bool secondary, bool proper, std::vector<Sam>& v) {
+    size_t b = v.size();
+    for (int i = 0; i < 2; ++i)
+        v.push_back(al[i] ? make_mapped(*al[i], *recs[i], mq[i], secondary, &det[i]) : make_unmapped(*recs[i], &det[i]));
+    v[b].flags |= READ1;
+    v[b + 1].flags |= READ2;
+    for (int i = 0; i < 2; ++i) {
+        Sam& s = v[b + i];
+        s.flags |= PAIRED;
+        if (proper) s.flags |= PROPER_PAIR;
+        const Alignment* mate = al[1 - i];
+        if (mate) {
+            if (mate->is_revcomp) s.flags |= MREVERSE;
+            s.mate_pos = mate->ref_start;
+            s.rnext = mate->ref_id;
+            if (al[i]) {
+                const Alignment* th = al[i];
+                if (th->ref_id == mate->ref_id) {
+                    s.rnext_eq = true;
+                    int64_t tl;
+                    if (mate->ref_start > th->ref_start) tl = (int64_t)(mate->ref_start - th->ref_start + mate->length);
+                    else if (th->ref_start > mate->ref_start) tl = -(int64_t)(th->ref_start - mate->ref_start + th->length);
+                    else if (i == 0) tl = -(int64_t)mate->length;
+                    else tl = (int64_t)th->length;
+                    s.has_tlen = true;
+                    s.tlen = (int32_t)tl;
+                }
+            } else {
+                s.rnext_eq = true;
+                s.rname = mate->ref_id;
+                s.pos = mate->ref_start;
+            }
+        } else {
+            s.flags |= MUNMAP;
+            if (al[i]) { s.rnext_eq = true; s.mate_pos = al[i]->ref_start; }
+        }
+    }
+    if (secondary) {
+        if (!v[b + 1].is_mapped()) v.pop_back();
+        else if (!v[b].is_mapped()) { v[b] = v[b + 1]; v.pop_back(); }
+    }
+}
+
+// src/mapper.rs:1302-1313
+uint8_t mapping_quality(const std::vector<Nam>& nams) {
+    if (nams.size() <= 1) return 60;
+    float s1 = nams[0].score, s2 = nams[1].score;
+    float mm = (float)std::min<uint32_t>(nams[0].n_anchors, 10) / 10.0f;
+    float u = 40.0f * (1.0f - s2 / s1) * mm * std::log(s1);
+    u = std::min(u, 60.0f);
+    if (!(u > 0.0f)) return 0;
+    return (uint8_t)u;
+}
+
+// src/nam.rs:168-188 + src/shuffle.rs:7-25 (the device already did the stable sort)
+void shuffle_best(std::vector<Nam>& nams, Rng& rng) {
+    if (nams.empty()) return;
+    size_t nb = 0;
+    while (nb < nams.size() && nams[nb].score == nams[0].score) ++nb;
+    if (nb > 1)
+        for (size_t i = 1; i < nb; ++i) std::swap(nams[i], nams[rng.mod64(i + 1)]);
+}
+
+void nams_of(const Ctx& c, uint64_t read_id, std::vector<Nam>& v) {
+    v.clear();
+    for (uint64_t x = c.nb.nam_off[read_id]; x < c.nb.nam_off[read_id + 1]; ++x) v.push_back(to_nam(c.nb.nams[x]));
+}
+
+// ----------------------------------------------------------------- SE ---
+// src/mapper.rs:349-528
+void align_single_end(Ctx& c, uint64_t rec_idx, std::vector<Sam>& out, std::deque<Alignment>& store,
+                      std::deque<std::array<Details, 2>>& dstore) {
+    const Record& rec = c.r1[rec_idx];
+    const uint32_t read_id = (uint32_t)(rec_idx - c.batch_first);
+    const uint32_t L = (uint32_t)rec.seq.size();
+    std::vector<Nam> nams;
+    nams_of(c, read_id, nams);
+    shuffle_best(nams, *c.rng);
+    dstore.emplace_back();
+    Details& det = dstore.back()[0];
+    det.nam = c.nb.info[read_id];
+    if (nams.empty()) { out.push_back(make_unmapped(rec, &det)); return; }
+    std::vector<Alignment> alignments;
+    float nam_max = nams[0].score;
+    uint32_t best_ed = 0xFFFFFFFFu, best_score = 0, second = 0, n_best = 0;
+    size_t best_index = 0;
+    std::optional<Alignment> best;
+    for (size_t tries = 0; tries < nams.size() && tries < c.o.max_tries; ++tries) {
+        Nam& nam = nams[tries];
+        float dropoff = nam.score / nam_max;
+        if ((tries > 1 && best_ed == 0) || dropoff < c.o.dropoff) break;
+        if (!nam.cons) { det.inconsistent_nams++; continue; }
+        auto al = extend_seed(c, nam, read_id, L, true, c.o.use_ssw);
+        if (!al) continue;
+        det.tried_alignment++;
+        det.gapped += al->gapped ? 1 : 0;
+        if (al->score >= best_score) {
+            second = best_score;
+            bool upd = false;
+            if (al->score > best_score) { n_best = 1; upd = true; }
+            else {
+                n_best++;
+                if (c.rng->mod32(n_best) == 0) upd = true;
+            }
+            if (upd) {
+                best_score = al->score;
+                best = *al;
+                best_index = alignments.size();
+                if (c.o.max_secondary == 0) best_ed = al->global_ed();
+            }
+        } else if (al->score > second) {
+            second = al->score;
+        }
+        if (c.o.max_secondary > 0) alignments.push_back(*al);
+    }
+    if (!best) { out.push_back(make_unmapped(rec, &det)); return; }
+    uint32_t num = 60 * (best_score - second);
+    uint8_t mapq = (uint8_t)((num + best_score - 1) / best_score);
+    store.push_back(*best);
+    out.push_back(make_mapped(store.back(), rec, mapq, false, &det));
+    if (c.o.max_secondary > 0) {
+        alignments[best_index] = alignments.back();
+        alignments.pop_back();
+        std::stable_sort(alignments.begin(), alignments.end(),
+                         [](const Alignment& a, const Alignment& b) { return a.score > b.score; });
+        for (size_t i = 0; i < alignments.size() && i < c.o.max_secondary; ++i) {
+            const Alignment& a = alignments[i];
+            uint32_t diff = a.score > best_score ? a.score - best_score : 0;
+            if (diff > 2 * 8 + 12) break;  // saturating_sub never exceeds: kept for parity
+            store.push_back(a);
+            out.push_back(make_mapped(store.back(), rec, mapq, true, &det));
+        }
+    }
+}
+
+// ----------------------------------------------------------------- PE ---
+struct ScoredPair {
+    double score;
+    std::optional<Alignment> a1, a2;
+};
+
+// src/mapper.rs:1018-1091
+std::vector<ScoredPair> rescue_read(Ctx& c, uint32_t read2, uint32_t L2, uint32_t read1, uint32_t L1,
+                                    std::vector<Nam>& nams1, Details det[2], float mu, float sigma) {
+    float max1 = nams1[0].score;
+    std::vector<Alignment> al1;
+    std::vector<std::optional<Alignment>> al2;
+    for (size_t t = 0; t < nams1.size() && t < c.o.max_tries; ++t) {
+        Nam& nam = nams1[t];
+        if (nam.score / max1 < c.o.dropoff) break;
+        bool cons = reverse_nam_if_needed(nam, L1);
+        det[0].inconsistent_nams += !cons;
+        auto a = extend_seed(c, nam, read1, L1, cons, true);
+        if (a) {
+            det[0].gapped += a->gapped;
+            al1.push_back(*a);
+            det[0].tried_alignment++;
+            auto b = rescue_align(c, nam, read2, L2, mu, sigma);
+            if (b) det[1].mate_rescue++;
+            al2.push_back(b);
+        }
+    }
+    std::vector<ScoredPair> pairs;
+    for (const Alignment& a1 : al1) {
+        for (const auto& a2 : al2) {
+            if (a2) {
+                uint32_t dist = a1.ref_start > a2->ref_start ? a1.ref_start - a2->ref_start : a2->ref_start - a1.ref_start;
+                float score = (float)(a1.score + a2->score);
+                if ((a1.is_revcomp ^ a2->is_revcomp) && (float)dist < mu + 4.0f * sigma)
+                    score += std::log(normal_pdf((float)dist, mu, sigma));
+                else
+                    score -= 10.0f;
+                pairs.push_back({(double)score, a1, *a2});
+            } else {
+                pairs.push_back({(double)a1.score - 10.0, a1, std::nullopt});
+            }
+        }
+    }
+    return pairs;
+}
+
+// src/mapper.rs:1184-1198
+bool is_proper_pair(const Alignment* a1, const Alignment* a2, float mu, float sigma) {
+    if (!a1 || !a2) return false;
+    int64_t dist = (int64_t)a2->ref_start - (int64_t)a1->ref_start;
+    bool same = a1->ref_id == a2->ref_id;
+    bool r1r2 = !a1->is_revcomp && a2->is_revcomp && dist >= 0;
+    bool r2r1 = !a2->is_revcomp && a1->is_revcomp && dist <= 0;
+    float lim = mu + sigma * 6.0f;
+    uint64_t limi = lim > 0 ? (uint64_t)lim : 0;
+    return same && (uint64_t)(dist < 0 ? -dist : dist) <= limi && (r1r2 || r2r1);
+}
+
+// src/mapper.rs:1200-1220
+bool is_proper_nam_pair(const Nam& n1, const Nam& n2, float mu, float sigma) {
+    if (n1.ref_id != n2.ref_id || n1.is_revcomp == n2.is_revcomp) return false;
+    uint32_t a = n1.projected_ref_start(), b = n2.projected_ref_start();
+    bool r1r2 = n2.is_revcomp && a <= b && (float)(b - a) < mu + 10.0f * sigma;
+    bool r2r1 = n1.is_revcomp && b <= a && (float)(a - b) < mu + 10.0f * sigma;
+    return r1r2 || r2r1;
+}
+
+struct NamPair { float score; const Nam* n1; const Nam* n2; };
+
+// src/mapper.rs:1224-1299
+std::vector<NamPair> best_scoring_nam_pairs(const std::vector<Nam>& n1, const std::vector<Nam>& n2, float mu, float sigma) {
+    std::vector<NamPair> pairs;
+    if (n1.empty() && n2.empty()) return pairs;
+    std::vector<uint32_t> added1, added2;
+    size_t best_joint = 0;
+    for (size_t a = 0; a < n1.size() && a < 1000; ++a) {
+        for (size_t b = 0; b < n2.size() && b < 1000; ++b) {
+            size_t joint = (size_t)n1[a].n_anchors + n2[b].n_anchors;
+            if (joint < best_joint / 2) break;
+            if (is_proper_nam_pair(n1[a], n2[b], mu, sigma)) {
+                pairs.push_back({n1[a].score + n2[b].score, &n1[a], &n2[b]});
+                added1.push_back(n1[a].nam_id);
+                added2.push_back(n2[b].nam_id);
+                best_joint = std::max(joint, best_joint);
+            }
+        }
+    }
+    auto has = [](const std::vector<uint32_t>& v, uint32_t x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    if (!n1.empty()) {
+        size_t bh = best_joint > 0 ? best_joint : n1[0].n_anchors;
+        for (const Nam& n : n1) {
+            if (n.n_anchors < bh / 2) break;
+            if (has(added1, n.nam_id)) continue;
+            pairs.push_back({n.score, &n, nullptr});
+        }
+    }
+    if (!n2.empty()) {
+        size_t bh = best_joint > 0 ? best_joint : n2[0].n_anchors;
+        for (const Nam& n : n2) {
+            if (n.n_anchors < bh / 2) break;
+            if (has(added2, n.nam_id)) continue;
+            pairs.push_back({n.score, nullptr, &n});
+        }
+    }
+    std::stable_sort(pairs.begin(), pairs.end(), [](const NamPair& x, const NamPair& y) { return x.score > y.score; });
+    return pairs;
+}
+
+// src/mapper.rs:1456-1470
+float top_dropoff(const std::vector<Nam>& nams) {
+    const Nam& n = nams[0];
+    if (n.n_anchors <= 2) return 1.0f;
+    if (nams.size() > 1) return (float)nams[1].n_anchors / (float)n.n_anchors;
+    return 0.0f;
+}
+
+struct PairedResult {
+    bool proper = false;
+    Alignment p1, p2;
+    std::vector<ScoredPair> pairs;
+};
+
+// src/mapper.rs:778-1014
+PairedResult extend_paired_seeds(Ctx& c, std::vector<Nam> nams[2], const uint32_t rid[2], const uint32_t L[2],
+                                 Details det[2], float mu, float sigma) {
+    PairedResult res;
+    const float dropoff = c.o.dropoff;
+    if (nams[0].empty() && nams[1].empty()) return res;
+    if (!nams[0].empty() && nams[1].empty()) {
+        res.pairs = rescue_read(c, rid[1], L[1], rid[0], L[0], nams[0], det, mu, sigma);
+        return res;
+    }
+    if (nams[0].empty() && !nams[1].empty()) {
+        Details sw[2] = {det[1], det[0]};
+        res.pairs = rescue_read(c, rid[0], L[0], rid[1], L[1], nams[1], sw, mu, sigma);
+        det[0] = sw[1];
+        det[1] = sw[0];
+        for (auto& p : res.pairs) std::swap(p.a1, p.a2);
+        return res;
+    }
+    if (top_dropoff(nams[0]) < dropoff && top_dropoff(nams[1]) < dropoff &&
+        is_proper_nam_pair(nams[0][0], nams[1][0], mu, sigma)) {
+        const Nam& m1 = nams[0][0];
+        const Nam& m2 = nams[1][0];
+        bool c1 = m1.cons, c2 = m2.cons;
+        det[0].inconsistent_nams += !c1;
+        det[1].inconsistent_nams += !c2;
+        // both requests are registered before a miss stops the record
+        bool saved = c.probe;
+        c.probe = true;
+        size_t before = c.ch->pending.size();
+        auto a1 = extend_seed(c, m1, rid[0], L[0], c1, true);
+        auto a2 = extend_seed(c, m2, rid[1], L[1], c2, true);
+        c.probe = saved;
+        if (c.ch->pending.size() != before) throw Miss{};
+        if (a1 && a2) {
+            det[0].tried_alignment++;
+            det[0].gapped += a1->gapped;
+            det[1].tried_alignment++;
+            det[1].gapped += a2->gapped;
+            res.proper = true;
+            res.p1 = *a1;
+            res.p2 = *a2;
+            return res;
+        }
+    }
+    std::map<uint32_t, std::optional<Alignment>> cache[2];
+    std::optional<Alignment> indv[2];
+    for (int i = 0; i < 2; ++i) {
+        bool cons = reverse_nam_if_needed(nams[i][0], L[i]);
+        det[i].inconsistent_nams += !cons;
+        indv[i] = extend_seed(c, nams[i][0], rid[i], L[i], cons, true);
+        det[i].tried_alignment++;
+        det[i].gapped += indv[i] ? indv[i]->gapped : 0;
+        cache[i][nams[i][0].nam_id] = indv[i];
+    }
+    std::vector<NamPair> np = best_scoring_nam_pairs(nams[0], nams[1], mu, sigma);
+    std::vector<ScoredPair>& out = res.pairs;
+    float max_score = np[0].score;
+    for (const NamPair& p : np) {
+        float sd = p.score / max_score;
+        if (out.size() >= c.o.max_tries || sd < dropoff) break;
+        std::optional<Alignment> als[2];
+        const Nam* ns[2] = {p.n1, p.n2};
+        for (int i = 0; i < 2; ++i) {
+            std::optional<Alignment> al;
+            if (ns[i]) {
+                Nam th = *ns[i];
+                auto it = cache[i].find(th.nam_id);
+                if (it == cache[i].end()) {
+                    bool cons = reverse_nam_if_needed(th, L[i]);
+                    det[i].inconsistent_nams += !cons;
+                    al = extend_seed(c, th, rid[i], L[i], cons, true);
+                    det[i].tried_alignment++;
+                    det[i].gapped += al ? al->gapped : 0;
+                    cache[i][th.nam_id] = al;
+                } else {
+                    al = it->second;
+                }
+            } else {
+                Nam other = *ns[1 - i];
+                det[1 - i].inconsistent_nams += !reverse_nam_if_needed(other, L[1 - i]);
+                al = rescue_align(c, other, rid[i], L[i], mu, sigma);
+                if (al) { det[i].mate_rescue++; det[i].tried_alignment++; }
+            }
+            if ((al ? al->score : 0) > (indv[i] ? indv[i]->score : 0)) indv[i] = al;
+            als[i] = al;
+        }
+        if (!als[0] || !als[1]) continue;
+        const Alignment& a1 = *als[0];
+        const Alignment& a2 = *als[1];
+        float lim = mu + 10.0f * sigma;
+        uint64_t limi = lim > 0 ? (uint64_t)lim : 0;
+        bool r1r2 = a2.is_revcomp && !a1.is_revcomp && a1.ref_start <= a2.ref_start && (uint64_t)(a2.ref_start - a1.ref_start) < limi;
+        bool r2r1 = a1.is_revcomp && !a2.is_revcomp && a2.ref_start <= a1.ref_start && (uint64_t)(a1.ref_start - a2.ref_start) < limi;
+        double cs;
+        if (r1r2 || r2r1) {
+            uint32_t x = a1.ref_start > a2.ref_start ? a1.ref_start - a2.ref_start : a2.ref_start - a1.ref_start;
+            cs = (double)a1.score + (double)a2.score + std::max(-20.0 + 0.001, (double)std::log(normal_pdf((float)x, mu, sigma)));
+        } else {
+            cs = (double)a1.score + (double)a2.score - 20.0;
+        }
+        out.push_back({cs, a1, a2});
+    }
+    if (indv[0] && indv[1]) out.push_back({(double)indv[0]->score + (double)indv[1]->score - 20.0, indv[0], indv[1]});
+    return res;
+}
+
+// src/mapper.rs:1343-1373
+void dedup_pairs(std::vector<ScoredPair>& v) {
+    if (v.size() < 2) return;
+    auto same = [](const std::optional<Alignment>& a, const std::optional<Alignment>& b) {
+        if (!a && !b) return true;
+        if (!a || !b) return false;
+        return a->ref_start == b->ref_start && a->ref_id == b->ref_id;
+    };
+    size_t k = 0, j = 1;
+    for (size_t i = 1; i < v.size(); ++i)
+        if (!same(v[k].a1, v[i].a1) || !same(v[k].a2, v[i].a2)) { v[j] = v[i]; ++j; k = i; }
+    v.resize(j);
+}
+
+// src/mapper.rs:1436-1453
+uint8_t alignment_quality(const std::vector<ScoredPair>& v) {
+    if (v.size() <= 1) return 60;
+    float s1 = (float)v[0].score, s2 = (float)v[1].score;
+    if (s1 > 0 && s2 > 0) {
+        float d = std::min(s1 - s2, 60.0f);
+        if (!(d > 0)) return 0;
+        return (uint8_t)d;
+    }
+    if (s1 > 0 && s2 <= 0) return 60;
+    return 1;
+}
+
+// src/mapper.rs:624-768
+void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Alignment>& store,
+                      std::deque<std::array<Details, 2>>& dstore) {
+    const Record* recs[2] = {&c.r1[idx], &c.r2[idx]};
+    const uint64_t rel = idx - c.batch_first;
+    const uint32_t rid[2] = {(uint32_t)(2 * rel), (uint32_t)(2 * rel + 1)};
+    const uint32_t L[2] = {(uint32_t)recs[0]->seq.size(), (uint32_t)recs[1]->seq.size()};
+    dstore.emplace_back();
+    Details* det = dstore.back().data();
+    std::vector<Nam> nams[2];
+    for (int i = 0; i < 2; ++i) {
+        nams_of(c, rid[i], nams[i]);
+        shuffle_best(nams[i], *c.rng);
+        det[i].nam = c.nb.info[rid[i]];
+    }
+    InsertSize& isz = *c.isize;
+    float mu = isz.mu, sigma = isz.sigma;
+    PairedResult pr = extend_paired_seeds(c, nams, rid, L, det, mu, sigma);
+    if (pr.proper) {
+        bool ps = is_proper_pair(&pr.p1, &pr.p2, mu, sigma);
+        if (ps && isz.sample_size < 400 && pr.p1.edit_distance + pr.p2.edit_distance < 3) {
+            uint32_t d = pr.p1.ref_start > pr.p2.ref_start ? pr.p1.ref_start - pr.p2.ref_start : pr.p2.ref_start - pr.p1.ref_start;
+            isz.update(d);
+        }
+        uint8_t mq[2] = {mapping_quality(nams[0]), mapping_quality(nams[1])};
+        det[0].best_alignments = 1;
+        det[1].best_alignments = 1;
+        store.push_back(pr.p1);
+        store.push_back(pr.p2);
+        const Alignment* al[2] = {&store[store.size() - 2], &store[store.size() - 1]};
+        make_paired(al, recs, mq, det, false, ps, out);
+        return;
+    }
+    std::vector<ScoredPair>& v = pr.pairs;
+    std::stable_sort(v.begin(), v.end(), [](const ScoredPair& a, const ScoredPair& b) { return a.score > b.score; });
+    dedup_pairs(v);
+    size_t nb = 0;
+    while (nb < v.size() && v[nb].score == v[0].score) ++nb;
+    det[0].best_alignments = (uint32_t)nb;
+    det[1].best_alignments = (uint32_t)nb;
+    if (nb > 1) std::swap(v[0], v[(size_t)c.rng->mod64(nb)]);
+    uint8_t mq[2];
+    if (v.size() == 1 && !is_proper_pair(v[0].a1 ? &*v[0].a1 : nullptr, v[0].a2 ? &*v[0].a2 : nullptr, mu, sigma)) {
+        mq[0] = mapping_quality(nams[0]);
+        mq[1] = mapping_quality(nams[1]);
+    } else {
+        mq[0] = mq[1] = alignment_quality(v);
+    }
+    if (v.empty()) {
+        Sam a = make_unmapped(*recs[0], &det[0]), b = make_unmapped(*recs[1], &det[1]);
+        a.flags |= PAIRED | READ1 | MUNMAP;
+        b.flags |= PAIRED | READ2 | MUNMAP;
+        out.push_back(a);
+        out.push_back(b);
+        return;
+    }
+    const double dropoff = 2.0 * 8 + 12;
+    size_t limit = c.o.max_secondary == 0 ? 1 : c.o.max_secondary;
+    bool secondary = false;
+    double smax = v[0].score;
+    for (size_t i = 0; i < v.size() && i < limit; ++i) {
+        if (c.o.max_secondary != 0 && !(smax - v[i].score < dropoff)) break;
+        const Alignment* al[2] = {nullptr, nullptr};
+        if (v[i].a1) { store.push_back(*v[i].a1); al[0] = &store.back(); }
+        if (v[i].a2) { store.push_back(*v[i].a2); al[1] = &store.back(); }
+        bool ps = is_proper_pair(al[0], al[1], mu, sigma);
+        uint8_t emq[2] = {secondary ? (uint8_t)0 : mq[0], secondary ? (uint8_t)0 : mq[1]};
+        make_paired(al, recs, emq, det, secondary, ps, out);
+        secondary = true;
+    }
+}
+
+// -------------------------------------------------------------- maponly ---
+// src/io/paf.rs:37-60
+void write_paf(std::string& s, const Nam& n, const Record& rec, int end, const PackedRef& ref, int mapq) {
+    s += rec.name;
+    if (end == 1) s += "/1";
+    else if (end == 2) s += "/2";
+    s += '\t'; put_u(s, rec.seq.size());
+    s += '\t'; put_u(s, n.query_start);
+    s += '\t'; put_u(s, n.query_end);
+    s += '\t'; s += n.is_revcomp ? '-' : '+';
+    s += '\t'; s += ref.names[n.ref_id];
+    s += '\t'; put_u(s, ref.len[n.ref_id]);
+    s += '\t'; put_u(s, n.ref_start);
+    s += '\t'; put_u(s, n.ref_end);
+    s += '\t'; put_u(s, n.matching_bases);
+    s += '\t'; put_u(s, n.ref_end - n.ref_start);
+    s += '\t'; put_u(s, (uint64_t)mapq);
+    s += '\n';
+}
+
+struct NamPairO { Nam n1, n2; double score; };
+
+// src/maponly.rs:380-458
+void find_pairs(const std::vector<Nam*>& fwd, const std::vector<Nam*>& rev, float mu, float sigma, bool swap,
+                std::vector<NamPairO>& out) {
+    float md = std::ceil(mu + 10.0f * sigma);
+    uint64_t max_dist = md > 0 ? (uint64_t)md : 0;
+    size_t rp = 0;
+    bool have_last = false;
+    size_t last_id = 0;
+    double last_score = 0;
+    for (const Nam* f : fwd) {
+        while (rp < rev.size() && (rev[rp]->ref_id < f->ref_id ||
+                                   (rev[rp]->ref_id == f->ref_id && rev[rp]->projected_ref_start() < f->projected_ref_start())))
+            ++rp;
+        if (rp == rev.size()) break;
+        if (rev[rp]->ref_id > f->ref_id) continue;
+        bool have = false;
+        size_t best_id = 0;
+        double best_score = 0;
+        for (size_t i = rp; i < rev.size() && rev[i]->ref_id == f->ref_id &&
+                            (uint64_t)(rev[i]->projected_ref_start() - f->projected_ref_start()) <= max_dist;
+             ++i) {
+            const Nam* r = rev[i];
+            uint32_t x = f->ref_start > r->ref_start ? f->ref_start - r->ref_start : r->ref_start - f->ref_start;
+            double s = (double)f->score + (double)r->score +
+                       std::max(0.001, (double)std::log(normal_pdf((float)x, mu, sigma) + 1.0f));
+            if (!have || s > best_score) { have = true; best_id = i; best_score = s; }
+        }
+        if (!have) continue;
+        if (have_last && last_id == best_id) {
+            if (best_score <= last_score) continue;
+            out.pop_back();
+        }
+        if (swap) out.push_back({*rev[best_id], *f, best_score});
+        else out.push_back({*f, *rev[best_id], best_score});
+        have_last = true;
+        last_id = best_id;
+        last_score = best_score;
+        rp = best_id;
+    }
+}
+
+// src/maponly.rs:314-375 (tie order of Rust's sort_unstable: parity unpinned)
+std::vector<NamPairO> get_nam_pairs(std::vector<Nam>& n1, std::vector<Nam>& n2, float mu, float sigma, bool both1, bool both2) {
+    std::vector<NamPairO> res;
+    if (n1.empty() || n2.empty()) return res;
+    auto split = [](std::vector<Nam>& v, bool both) -> size_t {
+        if (both) {
+            size_t left = 0, right = v.size();
+            while (left < right) {
+                if (v[left].is_revcomp) { --right; std::swap(v[left], v[right]); }
+                else ++left;
+            }
+            return left;
+        }
+        return v[0].is_revcomp ? 0 : v.size();
+    };
+    size_t e1 = split(n1, both1), e2 = split(n2, both2);
+    auto key_sort = [](std::vector<Nam>& v, size_t a, size_t b) {
+        std::sort(v.begin() + a, v.begin() + b, [](const Nam& x, const Nam& y) {
+            if (x.ref_id != y.ref_id) return x.ref_id < y.ref_id;
+            return x.projected_ref_start() < y.projected_ref_start();
+        });
+    };
+    auto run = [&](size_t fa, size_t fb, std::vector<Nam>& fv, size_t ra, size_t rb, std::vector<Nam>& rv, bool swap) {
+        if (fa == fb || ra == rb) return;
+        key_sort(fv, fa, fb);
+        key_sort(rv, ra, rb);
+        std::vector<Nam*> f, r;
+        for (size_t i = fa; i < fb; ++i) f.push_back(&fv[i]);
+        for (size_t i = ra; i < rb; ++i) r.push_back(&rv[i]);
+        find_pairs(f, r, mu, sigma, swap, res);
+    };
+    run(0, e1, n1, e2, n2.size(), n2, false);
+    run(0, e2, n2, e1, n1.size(), n1, true);
+    std::stable_sort(res.begin(), res.end(), [](const NamPairO& a, const NamPairO& b) { return a.score > b.score; });
+    return res;
+}
+
+void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
+    const PackedRef& ref = c.ref;
+    if (!c.paired) {
+        const Record& r1 = c.r1[idx];
+        uint32_t id = (uint32_t)(idx - c.batch_first);
+        std::vector<Nam> nams;
+        nams_of(c, id, nams);
+        shuffle_best(nams, *c.rng);
+        if (c.o.mode == 1) {
+            if (!nams.empty()) write_paf(s, nams[0], r1, 0, ref, mapping_quality(nams));
+        } else {
+            size_t nb = 0;
+            while (nb < nams.size() && nams[nb].score == nams[0].score) ++nb;
+            double w = (double)r1.seq.size() / (double)nb;
+            for (size_t i = 0; i < nb; ++i) ab[nams[i].ref_id] += w;
+        }
+        return;
+    }
+    const Record& r1 = c.r1[idx];
+    const Record& r2 = c.r2[idx];
+    uint64_t rel = idx - c.batch_first;
+    uint32_t id1 = (uint32_t)(2 * rel), id2 = (uint32_t)(2 * rel + 1);
+    std::vector<Nam> n1, n2;
+    nams_of(c, id1, n1);
+    nams_of(c, id2, n2);
+    if (n1.empty() && n2.empty()) return;
+    InsertSize& isz = *c.isize;
+    std::vector<NamPairO> np = get_nam_pairs(n1, n2, isz.mu, isz.sigma, c.nb.info[id1].both_orientations != 0,
+                                             c.nb.info[id2].both_orientations != 0);
+    // sort_nams on the partitioned vectors: stable by score, then shuffle the best
+    auto sort_shuffle = [&](std::vector<Nam>& v) {
+        std::stable_sort(v.begin(), v.end(), [](const Nam& a, const Nam& b) { return a.score > b.score; });
+        shuffle_best(v, *c.rng);
+    };
+    sort_shuffle(n1);
+    sort_shuffle(n2);
+    double indiv = (n1.empty() ? 0.0 : (double)n1[0].score) + (n2.empty() ? 0.0 : (double)n2[0].score);
+    bool pair = !np.empty() && np[0].score >= indiv / 2.0;
+    if (pair && isz.sample_size < 400) {
+        const Nam& a = np[0].n1;
+        const Nam& b = np[0].n2;
+        isz.update(a.ref_start > b.ref_start ? a.ref_start - b.ref_start : b.ref_start - a.ref_start);
+    }
+    if (c.o.mode == 1) {
+        if (pair) {
+            write_paf(s, np[0].n1, r1, 1, ref, 255);
+            write_paf(s, np[0].n2, r2, 2, ref, 255);
+        } else {
+            if (!n1.empty()) write_paf(s, n1[0], r1, 1, ref, 255);
+            if (!n2.empty()) write_paf(s, n2[0], r2, 2, ref, 255);
+        }
+    } else if (pair) {
+        size_t nb = 0;
+        while (nb < np.size() && np[nb].score == np[0].score) ++nb;
+        double w1 = (double)r1.seq.size() / (double)nb, w2 = (double)r2.seq.size() / (double)nb;
+        for (size_t i = 0; i < nb; ++i) { ab[np[i].n1.ref_id] += w1; ab[np[i].n2.ref_id] += w2; }
+    } else {
+        for (int e = 0; e < 2; ++e) {
+            const std::vector<Nam>& v = e == 0 ? n1 : n2;
+            size_t Lr = e == 0 ? r1.seq.size() : r2.seq.size();
+            size_t nb = 0;
+            while (nb < v.size() && v[nb].score == v[0].score) ++nb;
+            double w = (double)Lr / (double)nb;
+            for (size_t i = 0; i < nb; ++i) ab[v[i].ref_id] += w;
+        }
+    }
+}
+
+// One pass over a chunk: commit records until the first miss, then probe the rest.
+void run_chunk(Ctx base, Chunk& ch) {
+    std::vector<Sam> recs;
+    std::deque<Alignment> store;
+    std::deque<std::array<Details, 2>> dstore;
+    Ctx c = base;
+    c.ch = &ch;
+    c.probe = false;
+    Rng rng = ch.rng;
+    InsertSize isz = ch.isize;
+    bool probing = false;
+    for (uint64_t r = ch.next; r < ch.count; ++r) {
+        const uint64_t idx = ch.first + r;
+        Rng rng0 = rng;
+        InsertSize isz0 = isz;
+        c.rng = &rng;
+        c.isize = &isz;
+        c.probe = probing;
+        recs.clear();
+        store.clear();
+        dstore.clear();
+        try {
+            if (c.o.mode == 0) {
+                if (c.paired) {
+                    align_paired_end(c, idx, recs, store, dstore);
+                    if (c.o.only_mapped && !recs[0].is_mapped() && !recs[1].is_mapped()) recs.clear();
+                } else {
+                    align_single_end(c, idx, recs, store, dstore);
+                    if (c.o.only_mapped && !recs[0].is_mapped()) recs.clear();
+                }
+                if (!probing) {
+                    for (const Sam& s : recs) write_sam(ch.out, s, c);
+                }
+            } else {
+                std::string tmp;
+                std::vector<double> ab_probe;
+                if (probing) ab_probe.assign(ch.abundances.size(), 0.0);
+                map_only(c, idx, probing ? tmp : ch.out, probing ? ab_probe : ch.abundances);
+            }
+            if (!probing) {
+                ch.next = r + 1;
+                ch.rng = rng;
+                ch.isize = isz;
+            }
+        } catch (const Miss&) {
+            // restore the state of this record, probe it for its other misses, and
+            // continue in probe mode
+            rng = rng0;
+            isz = isz0;
+            if (!probing) {
+                probing = true;
+                --r;
+                continue;
+            }
+        }
+    }
+}
+
+template <typename F>
+void parallel_for(int threads, size_t n, F f) {
+    if (threads <= 1 || n <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::atomic<size_t> next{0};
+    for (int t = 0; t < std::min<int>(threads, (int)n); ++t)
+        th.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+        });
+    for (auto& t : th) t.join();
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
+                 const Options& o, std::string& out, RunStats* stats) {
+    const double t_start = now_ms();
+    const bool paired = !r2.empty();
+    const uint64_t n = paired ? std::min(r1.size(), r2.size()) : r1.size();
+    // src/main.rs:591-604 chunking
+    std::vector<Chunk> chunks;
+    {
+        uint64_t nt = 0, first = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            nt += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
+            if (nt > o.chunk_size) {
+                Chunk c;
+                c.first = first;
+                c.count = i + 1 - first;
+                chunks.push_back(std::move(c));
+                first = i + 1;
+                nt = 0;
+            }
+        }
+        if (first < n) {
+            Chunk c;
+            c.first = first;
+            c.count = n - first;
+            chunks.push_back(std::move(c));
+        }
+    }
+    std::vector<double> abundances(ref.names.size(), 0.0);
+    Scores sc{2, 8, 12, 1, 10, 500};
+    MapParams mp{o.mcs, o.rescue_distance};
+    ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+    const uint32_t k = (uint32_t)eng.params().k;
+    // device batches of whole chunks
+    size_t cb = 0;
+    std::vector<uint8_t> seq;
+    std::vector<uint64_t> off;
+    while (cb < chunks.size()) {
+        size_t ce = cb;
+        uint64_t bases = 0;
+        while (ce < chunks.size() && (ce == cb || bases < o.batch_bases)) {
+            for (uint64_t i = chunks[ce].first; i < chunks[ce].first + chunks[ce].count; ++i)
+                bases += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
+            ++ce;
+        }
+        const uint64_t bfirst = chunks[cb].first;
+        const uint64_t blast = chunks[ce - 1].first + chunks[ce - 1].count;
+        double t0 = now_ms();
+        seq.clear();
+        off.assign(1, 0);
+        seq.reserve(bases);
+        for (uint64_t i = bfirst; i < blast; ++i) {
+            seq.insert(seq.end(), r1[i].seq.begin(), r1[i].seq.end());
+            off.push_back(seq.size());
+            if (paired) {
+                seq.insert(seq.end(), r2[i].seq.begin(), r2[i].seq.end());
+                off.push_back(seq.size());
+            }
+        }
+        eng.upload_reads(seq.data(), off.data(), (uint32_t)(off.size() - 1));
+        double t1 = now_ms();
+        NamBatch nb;
+        eng.find_nams(mp, cp, nb);
+        double t2 = now_ms();
+        if (stats) {
+            stats->upload_ms += t1 - t0;
+            stats->nam_ms += t2 - t1;
+            stats->nams += nb.nams.size();
+        }
+        Ctx base{o, ref, nb, r1, r2, paired, bfirst, k, nullptr, false, nullptr, nullptr};
+        for (size_t c = cb; c < ce; ++c) {
+            chunks[c].abundances.assign(ref.names.size(), 0.0);
+            chunks[c].out.reserve(chunks[c].count * (paired ? 1400 : 700));
+        }
+        // replay rounds
+        for (int round = 0;; ++round) {
+            double h0 = now_ms();
+            parallel_for(o.threads, ce - cb, [&](size_t x) {
+                Chunk& ch = chunks[cb + x];
+                if (!ch.done() || round == 0) run_chunk(base, ch);
+            });
+            double h1 = now_ms();
+            if (stats) stats->host_ms += h1 - h0;
+            std::vector<ExtRequest> req;
+            std::vector<std::pair<size_t, size_t>> back;
+            for (size_t cix = cb; cix < ce; ++cix) {
+                Chunk& ch = chunks[cix];
+                for (size_t x = 0; x < ch.pending.size(); ++x) {
+                    const Key& kk = ch.pending[x];
+                    ExtRequest r{};
+                    r.kind = kk.kind;
+                    r.read_id = kk.read;
+                    r.flags = kk.flags;
+                    r.ref_id = kk.ref_id;
+                    if (kk.kind == EXT_SEED) {
+                        r.ref_start = kk.a; r.ref_end = kk.b; r.query_start = kk.c; r.query_end = kk.d;
+                        r.anchor_off = kk.e & ((1ull << 48) - 1);
+                        r.n_anchors = (uint32_t)(kk.e >> 48);
+                    } else {
+                        r.win_start = kk.a; r.win_end = kk.b;
+                    }
+                    req.push_back(r);
+                    back.push_back({cix, x});
+                }
+            }
+            bool all_done = true;
+            for (size_t cix = cb; cix < ce; ++cix) all_done &= chunks[cix].done();
+            if (req.empty()) {
+                if (!all_done) throw std::runtime_error("replay stalled without pending extensions");
+                break;
+            }
+            if (stats) { stats->rounds++; stats->requests += req.size(); }
+            std::vector<ExtResult> res;
+            std::vector<uint32_t> cig;
+            std::vector<uint64_t> coff;
+            double g0 = now_ms();
+            eng.extend(req.data(), (uint32_t)req.size(), sc, res, cig, coff);
+            double g1 = now_ms();
+            if (stats) stats->extend_ms += g1 - g0;
+            for (size_t x = 0; x < req.size(); ++x) {
+                Chunk& ch = chunks[back[x].first];
+                CachedResult cr;
+                cr.r = res[x];
+                cr.cigar.assign(cig.begin() + (ptrdiff_t)coff[x], cig.begin() + (ptrdiff_t)coff[x + 1]);
+                ch.cache.emplace(ch.pending[back[x].second], std::move(cr));
+            }
+            for (size_t cix = cb; cix < ce; ++cix) {
+                chunks[cix].pending.clear();
+                chunks[cix].pending_set.clear();
+            }
+            if (stats) stats->host_ms += now_ms() - g1;
+        }
+        for (size_t c = cb; c < ce; ++c) {
+            out += chunks[c].out;
+            std::string().swap(chunks[c].out);
+            chunks[c].cache.clear();
+            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += chunks[c].abundances[i];
+        }
+        cb = ce;
+    }
+    if (o.mode == 2) {
+        char buf[64];
+        for (size_t i = 0; i < ref.names.size(); ++i) {
+            snprintf(buf, sizeof buf, "\t%.6f\n", abundances[i] / (double)ref.len[i]);
+            out += ref.names[i];
+            out += buf;
+        }
+    }
+    if (stats) {
+        stats->records += n;
+        stats->total_ms += now_ms() - t_start;
+    }
+}
+
+}  // namespace sa

@@ -1,0 +1,167 @@
+"""Synthetic references and reads for the benchmark and parity workloads.
+
+Workloads (all randomness from numpy's PCG64 with the stated seeds):
+  small   : 1 Mbp single contig (seed 42); 10,000 single-end 150 bp reads,
+            1% substitutions, 0.1% indel events of 1-3 bp, Phred 30 qualities.
+  pe100m  : 100 Mbp in 20 contigs of 5 Mbp (seed 7), about 30% of the bases
+            overwritten by copies of 1-10 kbp segments with 2% divergence;
+            paired-end 2x150 reads, fragment length round(Normal(300, 30))
+            clipped at >= 150, FR orientation, 0.5% substitutions, 0.05% indels.
+  se_var  : same reference; single-end reads of uniform length 50-500 bp,
+            15% starting inside copied repeat segments.
+Reads are generated from both strands with equal probability (single-end) or
+as FR pairs (paired-end). Indel events insert or delete 1-3 bases (50/50,
+uniform inserted bases); reads are cut or extended to keep their length.
+"""
+import argparse
+import os
+
+import numpy as np
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+COMP = np.array([3, 2, 1, 0], dtype=np.uint8)
+
+
+def random_reference(rng, n_contigs, contig_len, repeat_fraction=0.0, seg_min=1000, seg_max=10000, divergence=0.02):
+    contigs = [rng.integers(0, 4, contig_len, dtype=np.uint8) for _ in range(n_contigs)]
+    segments = []  # (contig, start, length) of copied destinations
+    target = int(repeat_fraction * n_contigs * contig_len)
+    copied = 0
+    while copied < target:
+        L = int(rng.integers(seg_min, seg_max + 1))
+        sc, dc = int(rng.integers(0, n_contigs)), int(rng.integers(0, n_contigs))
+        ss, ds = int(rng.integers(0, contig_len - L)), int(rng.integers(0, contig_len - L))
+        seg = contigs[sc][ss:ss + L].copy()
+        mut = rng.random(L) < divergence
+        seg[mut] = (seg[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+        contigs[dc][ds:ds + L] = seg
+        segments.append((dc, ds, L))
+        copied += L
+    return contigs, segments
+
+
+def write_fasta(path, names, contigs, width=80):
+    with open(path, "wb") as f:
+        for name, c in zip(names, contigs):
+            f.write(b">" + name.encode() + b"\n")
+            s = ACGT[c].tobytes()
+            for i in range(0, len(s), width):
+                f.write(s[i:i + width] + b"\n")
+
+
+def mutate(rng, template, length, sub_rate, indel_rate):
+    """Apply indel events then substitutions to a code array; returns `length` codes."""
+    t = template
+    if indel_rate > 0:
+        events = np.nonzero(rng.random(len(t)) < indel_rate)[0]
+        if len(events):
+            out, last = [], 0
+            for e in events:
+                if e < last:
+                    continue
+                out.append(t[last:e])
+                L = int(rng.integers(1, 4))
+                if rng.random() < 0.5:
+                    out.append(rng.integers(0, 4, L, dtype=np.uint8))
+                    last = e
+                else:
+                    last = e + L
+            out.append(t[last:])
+            t = np.concatenate(out)
+    t = t[:length].copy()
+    if len(t) < length:  # extend with random bases (template ran short)
+        t = np.concatenate([t, rng.integers(0, 4, length - len(t), dtype=np.uint8)])
+    mut = rng.random(length) < sub_rate
+    t[mut] = (t[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+    return t
+
+
+def write_fastq(f, name, codes, qual_char=b"?"):
+    f.write(b"@" + name + b"\n" + ACGT[codes].tobytes() + b"\n+\n" + qual_char * len(codes) + b"\n")
+
+
+def single_end(rng, contigs, n_reads, lengths, sub_rate, indel_rate, out_path, repeat_segments=None,
+               repeat_share=0.0, prefix="r"):
+    sizes = np.array([len(c) for c in contigs], dtype=np.float64)
+    p = sizes / sizes.sum()
+    margin = 12
+    with open(out_path, "wb") as f:
+        for i in range(n_reads):
+            L = int(lengths[i])
+            if repeat_segments and rng.random() < repeat_share:
+                dc, ds, sl = repeat_segments[int(rng.integers(0, len(repeat_segments)))]
+                c = dc
+                start = ds + int(rng.integers(0, max(1, sl - 1)))
+                start = min(start, len(contigs[c]) - L - margin - 1)
+            else:
+                c = int(rng.choice(len(contigs), p=p))
+                start = int(rng.integers(0, len(contigs[c]) - L - margin))
+            tmpl = contigs[c][start:start + L + margin]
+            if rng.random() < 0.5:
+                tmpl = COMP[tmpl[::-1]]
+            write_fastq(f, ("%s%d" % (prefix, i)).encode(), mutate(rng, tmpl, L, sub_rate, indel_rate))
+
+
+def paired_end(rng, contigs, n_pairs, read_len, sub_rate, indel_rate, out1, out2, mean=300, sd=30,
+               len_range=None, prefix="p"):
+    sizes = np.array([len(c) for c in contigs], dtype=np.float64)
+    p = sizes / sizes.sum()
+    margin = 12
+    with open(out1, "wb") as f1, open(out2, "wb") as f2:
+        for i in range(n_pairs):
+            L = read_len if len_range is None else int(rng.integers(len_range[0], len_range[1] + 1))
+            frag = max(L, int(round(rng.normal(mean, sd))))
+            c = int(rng.choice(len(contigs), p=p))
+            start = int(rng.integers(margin, len(contigs[c]) - frag - margin))
+            fwd = contigs[c][start:start + frag + margin]
+            rev = COMP[contigs[c][start + frag - L - margin:start + frag][::-1]]
+            if rng.random() < 0.5:  # fragment from the reverse strand: mates swap roles
+                fwd, rev = COMP[contigs[c][start - margin:start + frag][::-1]], contigs[c][start:start + L + margin]
+            name = ("%s%d" % (prefix, i)).encode()
+            write_fastq(f1, name, mutate(rng, fwd, L, sub_rate, indel_rate))
+            write_fastq(f2, name, mutate(rng, rev, L, sub_rate, indel_rate))
+
+
+def make(workload, out_dir, n_reads=None):
+    os.makedirs(out_dir, exist_ok=True)
+    ref = os.path.join(out_dir, "ref.fa")
+    if workload == "small":
+        rng = np.random.default_rng(42)
+        contigs, _ = random_reference(rng, 1, 1_000_000)
+        write_fasta(ref, ["chr1"], contigs)
+        n = n_reads or 10_000
+        single_end(rng, contigs, n, np.full(n, 150), 0.01, 0.001, os.path.join(out_dir, "reads.fq"))
+        return {"ref": ref, "reads1": os.path.join(out_dir, "reads.fq")}
+    rng = np.random.default_rng(7)
+    contigs, segs = random_reference(rng, 20, 5_000_000, repeat_fraction=0.30)
+    names = ["ctg%02d" % (i + 1) for i in range(20)]
+    if not os.path.exists(ref):
+        write_fasta(ref, names, contigs)
+    if workload == "pe100m":
+        n = n_reads or 1_000_000
+        r1, r2 = os.path.join(out_dir, "pe_%d_1.fq" % n), os.path.join(out_dir, "pe_%d_2.fq" % n)
+        if not (os.path.exists(r1) and os.path.exists(r2)):
+            paired_end(np.random.default_rng(70), contigs, n, 150, 0.005, 0.0005, r1, r2)
+        return {"ref": ref, "reads1": r1, "reads2": r2}
+    if workload == "se_var":
+        n = n_reads or 2_000_000
+        r = os.path.join(out_dir, "sevar_%d.fq" % n)
+        if not os.path.exists(r):
+            g = np.random.default_rng(71)
+            single_end(g, contigs, n, g.integers(50, 501, n), 0.005, 0.0005, r, repeat_segments=segs,
+                       repeat_share=0.15, prefix="v")
+        return {"ref": ref, "reads1": r}
+    raise ValueError(workload)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("workload", choices=["small", "pe100m", "se_var"])
+    ap.add_argument("out_dir")
+    ap.add_argument("-n", type=int, default=None, help="number of reads / pairs")
+    a = ap.parse_args()
+    print(make(a.workload, a.out_dir, a.n))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle and the
+reference's golden files. Needs an MI355X."""
+import ctypes
+import json
+import os
+import random
+import subprocess
+import sys
+
+import pytest
+
+from conftest import GOLDEN, ORACLE_BIN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+PHIX = os.path.join(GOLDEN, "phix.fasta")
+R1 = os.path.join(GOLDEN, "phix.1.fastq")
+R2 = os.path.join(GOLDEN, "phix.2.fastq")
+RG = dict(rg_id="1", rg=["SM:sample", "LB:library"])
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def sa():
+    import strobealign_amd
+    strobealign_amd.load()
+    return strobealign_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(sa):
+    c = sa.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def phix(sa, ctx):
+    ix = sa.Index(ctx, PHIX, read_length=289)
+    yield ix
+    ix.close()
+
+
+def read(name):
+    with open(os.path.join(GOLDEN, name), "rb") as f:
+        return f.read()
+
+
+def oracle(*args):
+    return subprocess.run([ORACLE_BIN, *args], check=True, capture_output=True).stdout
+
+
+def test_phix_single_end_sam(phix):
+    say("case: phix SE SAM golden")
+    assert phix.map_files(R1, eqx=True, chunk_size=3, **RG) == read("phix.se.sam")
+
+
+def test_phix_paired_end_sam(phix):
+    say("case: phix PE SAM golden")
+    assert phix.map_files(R1, R2, eqx=True, **RG) == read("phix.pe.sam")
+
+
+def test_phix_paf_and_abundances(phix):
+    say("case: phix PAF/abundance goldens")
+    out = phix.map_files(R1, mode=1).splitlines(keepends=True)
+    assert b"".join(out[-11:]) == read("phix.se.paf")
+    out = phix.map_files(R1, mode=1, mcs=1).splitlines(keepends=True)
+    assert b"".join(out[-11:]) == read("phix.mcsrescue.se.paf")
+    assert phix.map_files(R1, mode=2) == read("phix.abun.se.txt")
+    assert phix.map_files(R1, R2, mode=2) == read("phix.abun.pe.txt")
+
+
+def test_phix_tags(sa, ctx):
+    say("case: phix tags golden")
+    tags = os.path.join(GOLDEN, "phix.tags.fastq")
+    with sa.Index(ctx, PHIX, read_length=151) as ix:
+        assert ix.map_files(tags, fastq_comments=True, **RG) == read("phix.tags.sam")
+
+
+def test_randstrobes_match_oracle(phix, orc):
+    say("case: randstrobes vs oracle")
+    rng = random.Random(11)
+    seqs = [b""] + [bytes(rng.choice(b"ACGTN" if rng.random() < 0.1 else b"ACGT") for _ in range(rng.randint(10, 500)))
+                    for _ in range(200)] + [b"A" * 300]
+    got = phix.randstrobes(seqs)
+    from test_oracle import randstrobes as orc_randstrobes
+    for s, g in zip(seqs, got):
+        exp = orc_randstrobes(orc, s, 289) if s else [[], []]
+        assert [list(map(tuple, exp[0])), list(map(tuple, exp[1]))] == [g[0], g[1]]
+
+
+@pytest.fixture(scope="module")
+def small(tmp_path_factory):
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("small"))
+    return synth.make("small", d, 3000)
+
+
+def test_small_single_end_vs_oracle(sa, ctx, small):
+    say("case: 1 Mbp, 3000 SE 150 bp reads with indels (piecewise path)")
+    exp = oracle("--no-PG", "--eqx", small["ref"], small["reads1"])
+    with sa.Index(ctx, small["ref"], read_length=150) as ix:
+        got = ix.map_files(small["reads1"], eqx=True)
+    assert b"\tNM:i:" in got
+    assert got == exp
+    exp_ssw = oracle("--no-PG", "--ssw", small["ref"], small["reads1"])
+    with sa.Index(ctx, small["ref"], read_length=150) as ix:
+        assert ix.map_files(small["reads1"], use_ssw=True) == exp_ssw
+
+
+def test_small_nams_vs_oracle(sa, ctx, small, tmp_path):
+    say("case: NAM lists vs oracle")
+    dump = str(tmp_path / "o.nams")
+    oracle("--dump-nams", dump, small["ref"], small["reads1"])
+    exp = [l.split("\t") for l in open(dump).read().splitlines()]
+    seqs = []
+    with open(small["reads1"], "rb") as f:
+        lines = f.read().split(b"\n")
+    seqs = [lines[i] for i in range(1, len(lines), 4)]
+    with sa.Index(ctx, small["ref"], read_length=150) as ix:
+        got = ix.nams(seqs)
+    flat = []
+    for i, nams in enumerate(got):
+        for n in nams:
+            sb = ctypes.c_uint32.from_buffer(ctypes.c_float(n[8])).value
+            flat.append([str(i)] + [str(x) for x in n[:8]] + ["%08x" % sb, str(n[9])])
+    assert flat == exp
+
+
+def test_paired_end_100m_vs_oracle(sa, ctx, tmp_path_factory):
+    say("case: 100 Mbp repetitive reference, 3000 pairs 2x150")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("pe"))
+    w = synth.make("pe100m", d, 3000)
+    exp = oracle("-t", "8", "--no-PG", w["ref"], w["reads1"], w["reads2"])
+    with sa.Index(ctx, w["ref"], read_length=150, threads=8) as ix:
+        got = ix.map_files(w["reads1"], w["reads2"], threads=8)
+    assert got == exp
+
+
+def test_ssw_against_reference_vectors(ctx, orc):
+    """Aligner::align on the device vs the oracle, over the queries/references of
+    the vectors written by the reference's ssw.c (ACGT-only references)."""
+    say("case: SSW aligner vs oracle")
+    vecs = json.load(open(os.path.join(GOLDEN, "ssw_vectors.json")))
+    pairs = []
+    for v in vecs:
+        q = bytes(b"ACGTN"[int(c)] for c in v["q"])
+        r = bytes(b"ACGTN"[int(c)] for c in v["r"])
+        pairs.append((q, r))
+    got = ctx.align_ssw(pairs)
+    from test_oracle import ssw as orc_ssw
+    L = "MIDNSHP=X"
+    for (q, r), g in zip(pairs, got):
+        exp = orc_ssw(orc, q, r)
+        if exp is None:
+            assert g[0] == 0
+            continue
+        out, cig = exp
+        assert g[0] == 1
+        assert [g[1], g[2], g[3], g[4], g[5], g[6]] == [out[0], out[1], out[2], out[3], out[4], out[5]]
+        assert "".join("%d%s" % (o >> 4, L[o & 15]) for o in g[7]) == cig
