@@ -1,0 +1,164 @@
+"""Benchmark: read pairs/s mapping 2x150 paired-end reads to a 100 Mbp reference.
+
+Workload: 100 Mbp in 20 contigs of 5 Mbp (seed 7) with ~30% copied 1-10 kbp
+segments at 2% divergence; 1,000,000 pairs 2x150 (fragment Normal(300,30),
+0.5% substitutions, 0.05% indels) generated by strobealign_amd/synth.py. One
+step maps all pairs through the HIP library (seeding, lookups, chaining, NAMs,
+extension on the GPU; decision replay and SAM formatting on the host) with the
+reads already parsed in host memory and the index resident in HBM. Output SAM
+is produced in memory and discarded.
+
+Prints one JSON line (see the contract in the repository docs) with a roofline
+object for the dominant kernel (HIP-event timing on the engine's stream) and a
+CPU baseline: the oracle (a C++ restatement of the reference) on the host's
+cores over a bounded sample of the same reads.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(ref, r1, r2, pairs, threads, workdir):
+    """Times the oracle on the first `pairs` pairs (index build excluded)."""
+    from strobealign_amd import synth  # noqa: F401
+    oracle = os.path.join(ROOT, "oracle", "build", "sa_oracle")
+    if not os.path.exists(oracle):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    s1, s2 = os.path.join(workdir, "cpu_1.fq"), os.path.join(workdir, "cpu_2.fq")
+    for src, dst in ((r1, s1), (r2, s2)):
+        with open(src, "rb") as f, open(dst, "wb") as g:
+            for i, line in enumerate(f):
+                if i >= 4 * pairs:
+                    break
+                g.write(line)
+    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "-o", os.devnull, ref, s1, s2],
+                       check=True, capture_output=True, text=True)
+    t = json.loads(p.stderr.strip().splitlines()[-1])
+    return pairs / t["map_s"], t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--threads", type=int, default=0, help="host threads (default: all cores)")
+    ap.add_argument("--data-dir", default=os.environ.get("SA_BENCH_DATA", "/tmp/strobealign_amd_bench"))
+    ap.add_argument("--cpu-pairs", type=int, default=40_000, help="oracle sample size (0: skip)")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import strobealign_amd as sa
+    from strobealign_amd import synth
+
+    cores = host_cores()
+    threads = a.threads or cores
+    t0 = time.time()
+    w = synth.make("pe100m", a.data_dir, a.pairs)
+    gen_s = time.time() - t0
+    print("data ready in %.1fs: %s" % (gen_s, w), file=sys.stderr, flush=True)
+    ctx = sa.Context(0)
+    t0 = time.time()
+    ix = sa.Index(ctx, w["ref"], read_length=150, threads=threads)
+    index_s = time.time() - t0
+    reads = sa.Reads(w["reads1"], w["reads2"])
+    opts = sa.options(threads=threads)
+    print("index %.1fs, %d pairs loaded" % (index_s, len(reads)), file=sys.stderr, flush=True)
+    for i in range(a.warmup):
+        _, st = ix.map_reads(reads, opts, keep_output=False)
+        print("warmup %d: %.1f ms" % (i, st.total_ms), file=sys.stderr, flush=True)
+    k0 = ix.kernel_times()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        _, st = ix.map_reads(reads, opts, keep_output=False)
+        stats.append(st)
+        print("step %d: %.1f ms (upload %.1f, nams %.1f, extend %.1f, host %.1f, rounds %d)" %
+              (i, st.total_ms, st.upload_ms, st.nam_ms, st.extend_ms, st.host_ms, st.rounds), file=sys.stderr,
+              flush=True)
+    wall = time.perf_counter() - t0
+    k1 = ix.kernel_times()
+    n_pairs = len(reads)
+    upload_s = sum(s.upload_ms for s in stats) / 1000.0
+    value = n_pairs * a.steps / (wall - upload_s)
+    # dominant kernel over the timed steps
+    delta = {k: (k1[k][0] - k0.get(k, (0, 0, 0))[0], k1[k][1] - k0.get(k, (0, 0, 0))[1],
+                 k1[k][2] - k0.get(k, (0, 0, 0))[2]) for k in k1}
+    dom = max(delta, key=lambda k: delta[k][0])
+    ms, launches, items = delta[dom]
+    per_launch_ms = ms / max(1, launches)
+    # algorithmic bytes per launch ("Roofline accounting")
+    traffic = None
+    if dom == "k_extend":
+        # per request: query bases (1 B) + packed reference window (~1.2 L / 4 B) + request and
+        # result records (64 + 40 B) + CIGAR ops (4 B each, ~3 per request)
+        per_item = 150 + (180 / 4) + 64 + 40 + 12
+    elif dom == "k_hits":
+        per_item = 26 * (3 * 16 + 2 * 8) + 26 * 24 + 26 * 56
+    elif dom == "k_chain":
+        per_item = 60 * (16 + 16 + 16) + 26 * 56
+    else:
+        per_item = 150 * 1 + 52 * 24 + 30 * 16
+    bytes_per_launch = per_item * items / max(1, launches)
+    achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+            "avg_launch_ms": round(per_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
+            "kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in delta.items()}}
+    cpu = None
+    if not a.no_cpu and a.cpu_pairs > 0:
+        try:
+            v, t = cpu_baseline(w["ref"], w["reads1"], w["reads2"], min(a.cpu_pairs, n_pairs), cores, a.data_dir)
+            cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
+                   "sample": "first %d pairs of the same workload, oracle -t %d, index build excluded (%.1f s map)"
+                             % (min(a.cpu_pairs, n_pairs), cores, t["map_s"])}
+        except Exception as e:  # reported, never fatal
+            cpu = {"value": None, "unit": "read pairs/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
+    out = {
+        "metric": "read pairs/sec at 2x150 on 100 Mbp ref, bit-exact SAM; p50/p99 latency",
+        "value": round(value, 1),
+        "unit": "read pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1000.0 * wall / a.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (strobealign_amd/synth.py pe100m, seed 7)",
+        "config": {"workload": "100 Mbp reference (20x5 Mbp, 30% repeats), %d pairs 2x150, SAM output" % n_pairs,
+                   "pairs": n_pairs, "read_len": 150, "host_threads": threads,
+                   "upload_ms_per_step": round(1000 * upload_s / a.steps, 2),
+                   "pairs_per_s_incl_pcie": round(n_pairs * a.steps / wall, 1),
+                   "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    reads.close()
+    ix.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -117,6 +117,24 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
                    const char* const* seqs1, const char* const* quals1, const char* const* names2,
                    const char* const* seqs2, const char* const* quals2, char** out, size_t* out_len);
 
+/* A read set parsed once (FASTQ or FASTA; reads2 may be NULL) and mapped by
+ * sa_map_reads; it is the input of one request of the serving loop. */
+typedef struct sa_reads sa_reads;
+int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out);
+uint64_t sa_reads_count(const sa_reads* reads);
+void sa_reads_destroy(sa_reads* reads);
+
+/* Wall-clock breakdown of one sa_map_reads call (milliseconds). */
+typedef struct sa_run_stats {
+    double total_ms, upload_ms, nam_ms, extend_ms, host_ms;
+    uint64_t records, requests, rounds, nams;
+} sa_run_stats;
+
+/* Maps a read set. With out == NULL the output is produced and discarded
+ * (measurement); otherwise as sa_map_files. stats may be NULL. */
+int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* reads, char** out, size_t* out_len,
+                 sa_run_stats* stats);
+
 /* Per-stage probes (parity tests, and a finer-grained drop-in for a caller that
  * keeps its own decision logic). Reads are concatenated ASCII with offsets[n+1]. */
 
@@ -165,6 +183,11 @@ int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* quer
 /* Kernel time (ms) spent by the index's engine since creation, and the number
  * of device launches; for measurement. */
 int sa_index_stats(const sa_index* index, double* kernel_ms);
+
+/* Per-kernel totals since the index was created, as JSON text
+ * {"k_seed": [milliseconds, launches, work_items], ...}, timed with HIP events on
+ * the engine's stream. Returns the JSON length (truncated to cap-1 in buf). */
+size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap);
 
 void sa_free(void* p);
 

@@ -59,7 +59,46 @@ class Alignment(ctypes.Structure):
 EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
            "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
-           "sa_align_ssw_batch", "sa_index_stats", "sa_free"]
+           "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
+           "sa_reads_destroy", "sa_map_reads", "sa_free"]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("upload_ms", ctypes.c_double), ("nam_ms", ctypes.c_double),
+                ("extend_ms", ctypes.c_double), ("host_ms", ctypes.c_double), ("records", ctypes.c_uint64),
+                ("requests", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("nams", ctypes.c_uint64)]
+
+
+class Reads:
+    """A read set parsed once (sa_reads_load)."""
+
+    def __init__(self, reads1, reads2=None):
+        load()
+        self._p = ctypes.c_void_p()
+        _check(_lib.sa_reads_load(reads1.encode(), reads2.encode() if reads2 else None, ctypes.byref(self._p)))
+        self.paired = reads2 is not None
+
+    def __len__(self):
+        return _lib.sa_reads_count(self._p)
+
+    def close(self):
+        if self._p:
+            _lib.sa_reads_destroy(self._p)
+            self._p = None
+
+
+def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000, use_ssw=False, max_secondary=0,
+            mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0):
+    o = MapOptions()
+    _lib.sa_map_options_default(ctypes.byref(o))
+    o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
+    o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
+    o.batch_bases = batch_bases
+    if rg_id is not None:
+        o.rg_id = rg_id.encode()
+    o._rg_keep = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])
+    o.rg, o.n_rg = o._rg_keep, len(rg)
+    return o
 
 
 def load(path=LIB_PATH):
@@ -76,6 +115,9 @@ def load(path=LIB_PATH):
     lib.sa_index_destroy.restype = None
     lib.sa_ctx_destroy.restype = None
     lib.sa_free.restype = None
+    lib.sa_index_kernel_times.restype = ctypes.c_size_t
+    lib.sa_reads_count.restype = ctypes.c_uint64
+    lib.sa_reads_destroy.restype = None
     _lib = lib
     return lib
 
@@ -172,6 +214,20 @@ class Index:
         _lib.sa_free(out)
         return s
 
+    def map_reads(self, reads, opts, keep_output=True):
+        """Maps a Reads set; returns (output bytes or None, RunStats)."""
+        st = RunStats()
+        if keep_output:
+            out = ctypes.c_void_p()
+            n = ctypes.c_size_t()
+            _check(_lib.sa_map_reads(self._p, ctypes.byref(opts), reads._p, ctypes.byref(out), ctypes.byref(n),
+                                     ctypes.byref(st)))
+            s = ctypes.string_at(out, n.value)
+            _lib.sa_free(out)
+            return s, st
+        _check(_lib.sa_map_reads(self._p, ctypes.byref(opts), reads._p, None, None, ctypes.byref(st)))
+        return None, st
+
     @staticmethod
     def _pack(seqs):
         data = b"".join(seqs)
@@ -210,6 +266,14 @@ class Index:
                         for x in range(noff[i], noff[i + 1])])
         _lib.sa_free(p)
         return res
+
+    def kernel_times(self):
+        """{kernel: (total_ms, launches, items)} timed with HIP events."""
+        import json
+        n = _lib.sa_index_kernel_times(self._p, None, ctypes.c_size_t(0))
+        buf = ctypes.create_string_buffer(n + 1)
+        _lib.sa_index_kernel_times(self._p, buf, ctypes.c_size_t(n + 1))
+        return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
 
     def kernel_ms(self):
         v = ctypes.c_double()

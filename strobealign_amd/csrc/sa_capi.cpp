@@ -1,4 +1,5 @@
 // extern "C" entry points declared in include/sa_hip.h.
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -10,6 +11,10 @@
 
 struct sa_ctx {
     int device;
+};
+
+struct sa_reads {
+    std::vector<sa::Record> r1, r2;
 };
 
 struct sa_index {
@@ -228,6 +233,45 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
     });
 }
 
+int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out) {
+    if (!reads1 || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        auto r = std::make_unique<sa_reads>();
+        std::string err;
+        if (!sa::read_records(reads1, r->r1, err)) return fail(SA_ERR_IO, err);
+        if (reads2 && !sa::read_records(reads2, r->r2, err)) return fail(SA_ERR_IO, err);
+        if (reads2 && r->r1.size() != r->r2.size()) return fail(SA_ERR_IO, "reads1 and reads2 differ in length");
+        *out = r.release();
+        return SA_OK;
+    });
+}
+
+uint64_t sa_reads_count(const sa_reads* reads) { return reads ? reads->r1.size() : 0; }
+
+void sa_reads_destroy(sa_reads* reads) { delete reads; }
+
+int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* reads, char** out, size_t* out_len,
+                 sa_run_stats* stats) {
+    if (!index || !reads) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        sa::Options o = options_of(opt);
+        std::string s;
+        if (o.mode == 0 && out) s = sa::sam_header(index->ref, o);
+        sa::RunStats st;
+        sa::map_records(*index->eng, index->ref, reads->r1, reads->r2, o, s, &st);
+        if (stats) {
+            stats->total_ms = st.total_ms; stats->upload_ms = st.upload_ms; stats->nam_ms = st.nam_ms;
+            stats->extend_ms = st.extend_ms; stats->host_ms = st.host_ms; stats->records = st.records;
+            stats->requests = st.requests; stats->rounds = st.rounds; stats->nams = st.nams;
+        }
+        if (out) {
+            if (!out_len) return fail(SA_ERR_ARG, "out_len is NULL");
+            *out = dup_out(s, out_len);
+        }
+        return SA_OK;
+    });
+}
+
 int sa_randstrobes_query_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
                                sa_qrand* out_fwd, sa_qrand* out_rc, uint32_t* counts) {
     if (!index || (!seqs && n) || !offsets || !out_fwd || !out_rc || !counts) return fail(SA_ERR_ARG, "NULL argument");
@@ -332,6 +376,26 @@ int sa_index_stats(const sa_index* index, double* kernel_ms) {
     if (!index) return fail(SA_ERR_ARG, "index is NULL");
     if (kernel_ms) *kernel_ms = index->eng->total_kernel_ms();
     return SA_OK;
+}
+
+size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
+    if (!index) return 0;
+    std::string s = "{";
+    bool first = true;
+    for (const auto& k : index->eng->kernel_stats()) {
+        char tmp[256];
+        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu]", first ? "" : ", ", k.first.c_str(), k.second.ms,
+                      (unsigned long long)k.second.launches, (unsigned long long)k.second.items);
+        s += tmp;
+        first = false;
+    }
+    s += "}";
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return s.size();
 }
 
 void sa_free(void* p) { std::free(p); }

@@ -69,6 +69,12 @@ Engine::~Engine() {
     if (stream_) hipStreamDestroy(stream_);
 }
 
+void Engine::add_kernel(const char* name, double ms, uint64_t items) {
+    for (auto& k : kstats_)
+        if (k.first == name) { k.second.ms += ms; k.second.launches += 1; k.second.items += items; return; }
+    kstats_.push_back({name, KernelStat{ms, 1, items}});
+}
+
 void* Engine::grow(int slot, uint64_t bytes) {
     if (bytes == 0) bytes = 16;
     if (cap_[slot] < bytes) {
@@ -246,10 +252,10 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         r.both_orientations = !better_orientation(hc[2 * i], hc[2 * i + 1]) && !better_orientation(hc[2 * i + 1], hc[2 * i]);
     }
     float t;
-    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t;
-    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t;
-    SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t;
-    SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t;
+    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n);
+    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n);
+    SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n);
+    SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
     for (auto& e : ev) hipEventDestroy(e);
     hipEventDestroy(fe0);
@@ -302,6 +308,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     SA_CHECK(hipEventElapsedTime(&t, e0, e1));
     times_.extend = t;
     total_ms_ += t;
+    add_kernel("k_extend", t, n);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     // requests that ran out of scratch are re-run one at a time with a large slot

@@ -53,6 +53,10 @@ public:
                 std::vector<uint32_t>& cigars, std::vector<uint64_t>& cigar_off);
 
     const StageTimes& times() const { return times_; }
+    // Per-kernel totals since creation: name -> (milliseconds, launches, work items).
+    struct KernelStat { double ms = 0; uint64_t launches = 0, items = 0; };
+    const std::vector<std::pair<std::string, KernelStat>>& kernel_stats() const { return kstats_; }
+    void add_kernel(const char* name, double ms, uint64_t items);
     const SeedParams& params() const { return p_; }
     uint64_t n_entries() const { return n_entries_; }
     ihipStream_t* stream() const { return stream_; }
@@ -86,6 +90,7 @@ private:
     std::vector<void*> buf_;
     std::vector<uint64_t> cap_;
     StageTimes times_;
+    std::vector<std::pair<std::string, KernelStat>> kstats_;
     double total_ms_ = 0;
     uint64_t lookup_bytes_ = 0;
 };

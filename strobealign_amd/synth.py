@@ -102,24 +102,72 @@ def single_end(rng, contigs, n_reads, lengths, sub_rate, indel_rate, out_path, r
             write_fastq(f, ("%s%d" % (prefix, i)).encode(), mutate(rng, tmpl, L, sub_rate, indel_rate))
 
 
-def paired_end(rng, contigs, n_pairs, read_len, sub_rate, indel_rate, out1, out2, mean=300, sd=30,
-               len_range=None, prefix="p"):
-    sizes = np.array([len(c) for c in contigs], dtype=np.float64)
-    p = sizes / sizes.sum()
+def _apply_errors(rng, tmpl, length, sub_rate, indel_rate):
+    """Errors for a batch of templates (rows): indels row by row where they occur,
+    substitutions vectorised. Returns an (n, length) code array."""
+    n = tmpl.shape[0]
+    out = tmpl[:, :length].copy()
+    if indel_rate > 0:
+        ev = rng.random(tmpl.shape) < indel_rate
+        for r in np.nonzero(ev.any(axis=1))[0]:
+            out[r] = _indels(rng, tmpl[r], length, np.nonzero(ev[r])[0])
+    mut = rng.random((n, length)) < sub_rate
+    out[mut] = (out[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
+    return out
+
+
+def _indels(rng, t, length, events):
+    parts, last = [], 0
+    for e in events:
+        if e < last:
+            continue
+        parts.append(t[last:e])
+        L = int(rng.integers(1, 4))
+        if rng.random() < 0.5:
+            parts.append(rng.integers(0, 4, L, dtype=np.uint8))
+            last = e
+        else:
+            last = e + L
+    parts.append(t[last:])
+    t = np.concatenate(parts)[:length]
+    if len(t) < length:
+        t = np.concatenate([t, rng.integers(0, 4, length - len(t), dtype=np.uint8)])
+    return t
+
+
+def _write_fastq_batch(f, prefix, first, codes, qual_char=b"?"):
+    seqs = ACGT[codes]
+    q = qual_char * codes.shape[1]
+    f.write(b"".join(b"@%s%d\n%s\n+\n%s\n" % (prefix, first + i, seqs[i].tobytes(), q) for i in range(codes.shape[0])))
+
+
+def paired_end(rng, contigs, n_pairs, read_len, sub_rate, indel_rate, out1, out2, mean=300, sd=30, prefix="p",
+               batch=200_000):
+    sizes = np.array([len(c) for c in contigs], dtype=np.int64)
+    genome = np.concatenate(contigs)
+    base = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     margin = 12
+    L = read_len
+    W = L + margin
     with open(out1, "wb") as f1, open(out2, "wb") as f2:
-        for i in range(n_pairs):
-            L = read_len if len_range is None else int(rng.integers(len_range[0], len_range[1] + 1))
-            frag = max(L, int(round(rng.normal(mean, sd))))
-            c = int(rng.choice(len(contigs), p=p))
-            start = int(rng.integers(margin, len(contigs[c]) - frag - margin))
-            fwd = contigs[c][start:start + frag + margin]
-            rev = COMP[contigs[c][start + frag - L - margin:start + frag][::-1]]
-            if rng.random() < 0.5:  # fragment from the reverse strand: mates swap roles
-                fwd, rev = COMP[contigs[c][start - margin:start + frag][::-1]], contigs[c][start:start + L + margin]
-            name = ("%s%d" % (prefix, i)).encode()
-            write_fastq(f1, name, mutate(rng, fwd, L, sub_rate, indel_rate))
-            write_fastq(f2, name, mutate(rng, rev, L, sub_rate, indel_rate))
+        for b0 in range(0, n_pairs, batch):
+            n = min(batch, n_pairs - b0)
+            c = rng.choice(len(contigs), size=n, p=sizes / sizes.sum())
+            frag = np.maximum(L, np.round(rng.normal(mean, sd, n))).astype(np.int64)
+            start = margin + (rng.random(n) * (sizes[c] - frag - 2 * margin)).astype(np.int64)
+            g0 = base[c] + start
+            ar = np.arange(W)
+            # mate 1 reads forward from the fragment start, mate 2 is the reverse
+            # complement of the fragment end; fragments on the reverse strand swap roles
+            fwd_t = genome[g0[:, None] + ar[None, :]]
+            rev_t = COMP[genome[(g0 + frag)[:, None] - 1 - ar[None, :]]]
+            flip = rng.random(n) < 0.5
+            m1 = np.where(flip[:, None], COMP[genome[(g0 + frag)[:, None] - 1 - ar[None, :]]], fwd_t)
+            m2 = np.where(flip[:, None], genome[g0[:, None] + ar[None, :]], rev_t)
+            r1 = _apply_errors(rng, m1, L, sub_rate, indel_rate)
+            r2 = _apply_errors(rng, m2, L, sub_rate, indel_rate)
+            _write_fastq_batch(f1, prefix.encode(), b0, r1)
+            _write_fastq_batch(f2, prefix.encode(), b0, r2)
 
 
 def make(workload, out_dir, n_reads=None):
