@@ -146,7 +146,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic (strobealign_amd/synth.py pe100m, seed 7)",
-        "config": {"workload": "100 Mbp reference (20x5 Mbp, 30% repeats), %d pairs 2x150, SAM output" % n_pairs,
+        "config": {"workload": "100 Mbp reference (20x5 Mbp, 30%% repeats), %d pairs 2x150, SAM output" % n_pairs,
                    "pairs": n_pairs, "read_len": 150, "host_threads": threads,
                    "upload_ms_per_step": round(1000 * upload_s / a.steps, 2),
                    "pairs_per_s_incl_pcie": round(n_pairs * a.steps / wall, 1),
