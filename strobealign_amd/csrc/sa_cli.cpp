@@ -132,6 +132,18 @@ int main(int argc, char** argv) {
                          t1 - t0, t2 - t1, t3 - t2, (unsigned long long)st.records, (unsigned long long)st.nams,
                          (unsigned long long)st.requests, (unsigned long long)st.rounds, st.nam_ms, st.extend_ms,
                          st.host_ms, st.upload_ms);
+        if (time_it) {
+            // per-kernel totals: milliseconds, launches, work items
+            std::string k = "{";
+            for (const auto& ks : eng.kernel_stats()) {
+                char b[160];
+                std::snprintf(b, sizeof b, "%s\"%s\": [%.3f, %llu, %llu]", k.size() > 1 ? ", " : "", ks.first.c_str(),
+                              ks.second.ms, (unsigned long long)ks.second.launches,
+                              (unsigned long long)ks.second.items);
+                k += b;
+            }
+            std::fprintf(stderr, "%s}\n", k.c_str());
+        }
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

@@ -45,7 +45,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_COUNT
 };
 }  // namespace
 
@@ -262,6 +262,25 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     hipEventDestroy(fe1);
 }
 
+namespace {
+// Launches k_ssw_pass<G, BYTE, seg> for a run-time stripe count 1 <= seg <= MAX.
+template <int G, bool BYTE, int S, int MAX>
+void launch_ssw_pass(uint32_t seg, uint32_t count, hipStream_t st, const uint8_t* seq, const uint64_t* off,
+                     RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, const uint32_t* list,
+                     uint32_t rev, SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap) {
+    if constexpr (S > MAX) {
+        throw std::logic_error("SSW stripe count out of range");
+    } else if (seg != S) {
+        launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, count, st, seq, off, ref, sc, req, jobs, list, rev, fwd, ends,
+                                             lists, counts, cap);
+    } else {
+        k_ssw_pass<G, BYTE, S><<<blocks((uint64_t)count * G, 256), 256, 0, st>>>(seq, off, ref, sc, req, jobs, list,
+                                                                                 count, rev, fwd, ends, lists,
+                                                                                 counts, cap);
+    }
+}
+}  // namespace
+
 void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std::vector<ExtResult>& res,
                     std::vector<uint32_t>& cigars, std::vector<uint64_t>& cigar_off) {
     SA_CHECK(hipSetDevice(device_));
@@ -281,34 +300,125 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_req = (ExtRequest*)grow(S_REQ, (uint64_t)n * sizeof(ExtRequest));
     auto* d_res = (ExtResult*)grow(S_RES, (uint64_t)n * sizeof(ExtResult));
     auto* d_slots = (uint32_t*)grow(S_CIGSLOT, slots * 4);
+    auto* d_jobs = (ExtJob*)grow(S_JOBS, (uint64_t)n * sizeof(ExtJob));
+    auto* d_ljobs = (ExtJob*)grow(S_LJOBS, (uint64_t)n * sizeof(ExtJob));
+    auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
+    auto* d_cnt = (uint32_t*)grow(S_XCNT, 16);
     SA_CHECK(hipMemcpyAsync(d_req, req.data(), (uint64_t)n * sizeof(ExtRequest), hipMemcpyHostToDevice, stream_));
-    const uint64_t L = max_len_;
-    uint64_t scratch_per = 16 * L + 256 + 65536 + (L + 1) * (L + L / 10 + 2) + 16 * (2 * L + L / 10 + 16) + 16 * L;
-    scratch_per = (scratch_per + 255) & ~255ull;
-    const uint64_t budget = 6ull << 30;
-    uint64_t wave = std::max<uint64_t>(1, std::min<uint64_t>(n, budget / scratch_per));
-    auto* d_scr = (uint8_t*)grow(S_SCRATCH, wave * scratch_per);
+    SA_CHECK(hipMemsetAsync(d_cnt, 0, 16, stream_));
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    const auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    const auto* d_off = (const uint64_t*)buf_[S_OFF];
     const uint32_t* d_chain = (const uint32_t*)buf_[S_CHAIN];
+    const uint32_t k = (uint32_t)p_.k;
+    const uint64_t L = max_len_;
     hipEvent_t e0, e1;
     SA_CHECK(hipEventCreate(&e0));
     SA_CHECK(hipEventCreate(&e1));
-    SA_CHECK(hipEventRecord(e0, stream_));
-    for (uint64_t first = 0; first < n; first += wave) {
-        uint32_t cnt = (uint32_t)std::min<uint64_t>(wave, n - first);
-        k_extend<<<blocks(cnt, 128), 128, 0, stream_>>>((const uint8_t*)buf_[S_SEQ], (const uint64_t*)buf_[S_OFF], ref,
-                                                        sc, (uint32_t)p_.k, d_req, cnt, d_res, d_slots, d_chain, d_scr,
-                                                        scratch_per, (uint32_t)first);
+    auto timed = [&](const char* name, uint64_t items, auto&& launch) {
+        SA_CHECK(hipEventRecord(e0, stream_));
+        launch();
         SA_CHECK(hipGetLastError());
+        SA_CHECK(hipEventRecord(e1, stream_));
+        SA_CHECK(hipEventSynchronize(e1));
+        float t = 0;
+        SA_CHECK(hipEventElapsedTime(&t, e0, e1));
+        add_kernel(name, t, items);
+        total_ms_ += t;
+        return t;
+    };
+    float ext_ms = 0;
+    // 1. Hamming fast path, rescue prefilter, job lists
+    ext_ms += timed("k_ext_prep", n, [&] {
+        k_ext_prep<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res, d_slots, d_jobs,
+                                                        d_ljobs, d_pjobs, d_cnt);
+    });
+    uint32_t cnt[4] = {0, 0, 0, 0};
+    SA_CHECK(hipMemcpy(cnt, d_cnt, 16, hipMemcpyDeviceToHost));
+    const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
+    // thread-per-job kernels run in waves sized to the scratch budget
+    const uint64_t budget = 16ull << 30;
+    auto waves = [&](uint32_t njobs, uint64_t per, auto&& launch) {
+        uint64_t wave = std::max<uint64_t>(1, std::min<uint64_t>(njobs, budget / per));
+        auto* d_scr = (uint8_t*)grow(S_SCRATCH, wave * per);
+        for (uint64_t first = 0; first < njobs; first += wave)
+            launch((uint32_t)std::min<uint64_t>(wave, njobs - first), (uint32_t)first, d_scr);
+    };
+    // 2. SSW: score passes per lane group, bucketed by stripe count, then
+    //    traceback per thread
+    if (nssw) {
+        const uint32_t cap = nssw;
+        auto* d_ends = (SswEnds*)grow(S_ENDS, (uint64_t)nssw * sizeof(SswEnds));
+        auto* d_fwd = (SswEnd*)grow(S_FWD, (uint64_t)nssw * sizeof(SswEnd));
+        auto* d_lists = (uint32_t*)grow(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
+        auto* d_bcnt = (uint32_t*)grow(S_BCNT, SSW_BUCKETS * 4);
+        SA_CHECK(hipMemsetAsync(d_bcnt, 0, SSW_BUCKETS * 4, stream_));
+        k_ssw_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, nssw, d_lists, d_bcnt, cap);
+        SA_CHECK(hipGetLastError());
+        uint32_t bc[SSW_BUCKETS];
+        auto read_counts = [&] { SA_CHECK(hipMemcpy(bc, d_bcnt, sizeof bc, hipMemcpyDeviceToHost)); };
+        // launches every non-empty bucket of one (mode, pass) family
+        auto run = [&](bool word, bool rev) {
+            for (uint32_t seg = 1; seg <= (word ? 32u : 16u); ++seg) {
+                const uint32_t b = ssw_bucket(word, rev, seg), c = bc[b];
+                if (!c) continue;
+                const uint32_t* list = d_lists + (uint64_t)b * cap;
+                if (word)
+                    launch_ssw_pass<8, false, 1, 32>(seg, c, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, list, rev,
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap);
+                else
+                    launch_ssw_pass<16, true, 1, 16>(seg, c, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, list, rev,
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap);
+                SA_CHECK(hipGetLastError());
+            }
+        };
+        read_counts();
+        ext_ms += timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); });
+        read_counts();
+        uint64_t nrb = 0, nfw = 0;
+        for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1)];
+        for (uint32_t x = 0; x < 32; ++x) nfw += bc[ssw_bucket(true, false, x + 1)];
+        if (nrb) ext_ms += timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
+        if (nfw) {
+            ext_ms += timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });
+            read_counts();
+            ext_ms += timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
+        }
+        // banded traceback: band up to 64 without a retry
+        uint64_t per = 3ull * (2 * 64 + 4) * 4 + 129ull * L * 3 + 8ull * (2 * L + 64) + 4096;
+        per = (per + 255) & ~255ull;
+        ext_ms += timed("k_ssw_finish", nssw, [&] {
+            waves(nssw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
+                k_ssw_finish<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, cnt_,
+                                                                    d_res, d_slots, scr, per, first);
+            });
+        });
     }
-    SA_CHECK(hipEventRecord(e1, stream_));
+    // 3. SSW of queries too long for a lane group, one thread each
+    if (nlong) {
+        uint64_t per = 16ull * ((L + 15) / 16 * 16) + 256 + 65536 + 8 * (2 * L + 64) + 129ull * L * 3;
+        per = (per + 255) & ~255ull;
+        ext_ms += timed("k_ext_long", nlong, [&] {
+            waves(nlong, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
+                k_ext_long<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_ljobs, cnt_, d_res,
+                                                                  d_slots, scr, per, first);
+            });
+        });
+    }
+    // 4. piecewise extension, one thread per job
+    if (npw) {
+        uint64_t per = 16 * L + 256 + (L + 1) * (L + L / 10 + 2) + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
+        per = (per + 255) & ~255ull;
+        ext_ms += timed("k_ext_pw", npw, [&] {
+            waves(npw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
+                k_ext_pw<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res,
+                                                                d_slots, d_chain, scr, per, first);
+            });
+        });
+    }
+    times_.extend = ext_ms;
     SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipStreamSynchronize(stream_));
-    float t;
-    SA_CHECK(hipEventElapsedTime(&t, e0, e1));
-    times_.extend = t;
-    total_ms_ += t;
-    add_kernel("k_extend", t, n);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     // requests that ran out of scratch are re-run one at a time with a large slot
@@ -317,17 +427,16 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         if (res[i].status == 2) big.push_back(i);
     if (!big.empty()) {
         const uint64_t bigs = 256ull << 20;
-        auto* d_big = (uint8_t*)grow(S_SCRATCH, std::max<uint64_t>(bigs, wave * scratch_per));
+        auto* d_big = (uint8_t*)grow(S_SCRATCH, bigs);
         for (uint32_t i : big) {
-            k_extend<<<1, 1, 0, stream_>>>((const uint8_t*)buf_[S_SEQ], (const uint64_t*)buf_[S_OFF], ref, sc,
-                                           (uint32_t)p_.k, d_req, 1, d_res, d_slots, d_chain, d_big, bigs, i);
+            k_extend<<<1, 1, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, 1, d_res, d_slots, d_chain, d_big, bigs, i);
             SA_CHECK(hipGetLastError());
         }
         SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
         SA_CHECK(hipStreamSynchronize(stream_));
     }
     for (uint32_t i = 0; i < n; ++i)
-        if (res[i].found && res[i].status != 0) throw std::runtime_error("extension request failed (status)");
+        if (res[i].status != 0) throw std::runtime_error("extension request failed (status)");
     // compact the CIGARs
     auto* d_ncig = (uint64_t*)grow(S_NCIG, (uint64_t)n * 8);
     auto* d_coff = (uint64_t*)grow(S_CIGOFF, (uint64_t)n * 8);

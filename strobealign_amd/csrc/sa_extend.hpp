@@ -53,13 +53,11 @@ struct RefWin {
 
 // src/ssw.rs:8-22 translate: A/U -> 0, C 1, G 2, T 3, other 4
 SA_HD int32_t ssw_code(uint8_t c) {
-    switch (c) {
-        case 'A': case 'a': case 'U': case 'u': return 0;
-        case 'C': case 'c': return 1;
-        case 'G': case 'g': return 2;
-        case 'T': case 't': return 3;
-        default: return 4;
-    }
+    return nt_is_acgt(c) ? (int32_t)nt_acgt_code(c) : (nt_is_u(c) ? 0 : 4);
+}
+// ssw_code of the complement of c (comp_ascii), without the ASCII round trip
+SA_HD int32_t ssw_code_comp(uint8_t c) {
+    return nt_is_acgt(c) ? 3 - (int32_t)nt_acgt_code(c) : (nt_is_u(c) ? 0 : 4);
 }
 SA_HD int32_t ssw_mat(int32_t a, int32_t b, const Scores& sc) { return (a == b && a < 4) ? sc.match : -sc.mismatch; }
 
@@ -265,34 +263,25 @@ struct AlignInfo {
     int status;  // 0 ok, 1 cigar overflow, 2 scratch overflow
 };
 
-// SSW alignment with the end-bonus extension (src/ssw.rs:156-191 + src/aligner.rs:80-151).
-// Coordinates are relative to the query view and the reference window.
-SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Scores& sc, uint8_t* scratch,
-                                  uint64_t scratch_bytes, CigarOut& cig) {
+// Outcome of the SSW score passes (ext/ssw/ssw.c:818-888): best score, its end
+// (first column reaching it, then smallest query row) and the start found by the
+// reverse pass.
+struct SswEnds {
+    int32_t score1, ref_end1, read_end1, ref_begin1, read_begin1;
+    bool rev_short;  // score1 > reverse-pass score (the reference returns flag 2)
+};
+
+// Banded traceback, with_eqx CIGAR and the end-bonus extensions (ext/ssw/ssw.c:
+// 895-915 then src/ssw.rs:181-188 and src/aligner.rs:90-148), given the passes' result.
+SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e, const Scores& sc, uint8_t* scratch,
+                           uint64_t scratch_bytes, CigarOut& cig) {
     AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
     cig.n = 0;
     cig.overflow = false;
-    if (r.len > 2000 || q.len == 0) return a;
-    const int32_t qlen = (int32_t)q.len, rlen = (int32_t)r.len;
-    uint64_t ssw_bytes = 4ull * 4 * (uint64_t)(((qlen + 15) / 16) * 16);
-    if (ssw_bytes + 64 > scratch_bytes) { a.status = 2; return a; }
-    int32_t* buf = (int32_t*)scratch;
-    auto qf = [&](int32_t j) { return ssw_code(q[(uint32_t)j]); };
-    auto rf = [&](int32_t i) { return ssw_code(r[(uint32_t)i]); };
-    SswEnd best = ssw_pass<16, true>(qf, qlen, rf, rlen, false, sc, 255, buf);
-    bool word = false;
-    if (best.score == 255) {
-        best = ssw_pass<8, false>(qf, qlen, rf, rlen, false, sc, 65535, buf);
-        word = true;
-    }
-    const int32_t score1 = best.score, ref_end1 = best.ref, read_end1 = best.read;
-    // reverse pass over the reversed query prefix [0, read_end1]
-    auto qrev = [&](int32_t j) { return ssw_code(q[(uint32_t)(read_end1 - j)]); };
-    SswEnd br = word ? ssw_pass<8, false>(qrev, read_end1 + 1, rf, ref_end1 + 1, true, sc, score1, buf)
-                     : ssw_pass<16, true>(qrev, read_end1 + 1, rf, ref_end1 + 1, true, sc, (uint8_t)score1, buf);
-    const int32_t ref_begin1 = br.ref, read_begin1 = read_end1 - br.read;
-    if (ref_begin1 == -1 || read_begin1 == -1 || ref_begin1 < 0 || read_begin1 < 0) return a;
-    if (score1 > br.score) return a;  // flag 2
+    const int32_t score1 = e.score1, ref_end1 = e.ref_end1, read_end1 = e.read_end1;
+    const int32_t ref_begin1 = e.ref_begin1, read_begin1 = e.read_begin1;
+    if (ref_begin1 < 0 || read_begin1 < 0) return a;
+    if (e.rev_short) return a;
     const int32_t refL = ref_end1 - ref_begin1 + 1, readL = read_end1 - read_begin1 + 1;
     int32_t bw = refL > readL ? refL - readL + 1 : readL - refL + 1;
     // decide the end-bonus extensions first so the CIGAR can be written front to back
@@ -323,10 +312,9 @@ SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Sco
                 mainc.push(2, len); ri += len; edits += len;
             }
         };
-        uint64_t used = ssw_bytes;
         int st = banded_sw([&](int32_t i) { return ssw_code(q[qs + (uint32_t)i]); },
                            [&](int32_t j) { return ssw_code(r[rs + (uint32_t)j]); }, refL, readL, score1, sc, bw,
-                           scratch + used, scratch_bytes - used, emit_main);
+                           scratch, scratch_bytes, emit_main);
         if (st == 1) return a;  // flag 1: no alignment
         if (st == 2) { a.status = 2; return a; }
     }
@@ -383,6 +371,40 @@ SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Sco
     a.ref_end = re;
     a.edit_distance = edits;
     return a;
+}
+
+// Scalar SSW: both passes run in one thread (a loop over the vector lanes), then
+// ssw_finish. Used for queries longer than the lane-parallel kernel supports.
+SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Scores& sc, uint8_t* scratch,
+                                  uint64_t scratch_bytes, CigarOut& cig) {
+    AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
+    cig.n = 0;
+    cig.overflow = false;
+    if (r.len > 2000 || q.len == 0) return a;
+    const int32_t qlen = (int32_t)q.len, rlen = (int32_t)r.len;
+    uint64_t ssw_bytes = 4ull * 4 * (uint64_t)(((qlen + 15) / 16) * 16);
+    ssw_bytes = (ssw_bytes + 255) & ~255ull;
+    if (ssw_bytes + 64 > scratch_bytes) { a.status = 2; return a; }
+    int32_t* buf = (int32_t*)scratch;
+    auto qf = [&](int32_t j) { return ssw_code(q[(uint32_t)j]); };
+    auto rf = [&](int32_t i) { return ssw_code(r[(uint32_t)i]); };
+    SswEnd best = ssw_pass<16, true>(qf, qlen, rf, rlen, false, sc, 255, buf);
+    bool word = false;
+    if (best.score == 255) {
+        best = ssw_pass<8, false>(qf, qlen, rf, rlen, false, sc, 65535, buf);
+        word = true;
+    }
+    SswEnds e;
+    e.score1 = best.score;
+    e.ref_end1 = best.ref;
+    e.read_end1 = best.read;
+    auto qrev = [&](int32_t j) { return ssw_code(q[(uint32_t)(e.read_end1 - j)]); };
+    SswEnd br = word ? ssw_pass<8, false>(qrev, e.read_end1 + 1, rf, e.ref_end1 + 1, true, sc, e.score1, buf)
+                     : ssw_pass<16, true>(qrev, e.read_end1 + 1, rf, e.ref_end1 + 1, true, sc, (uint8_t)e.score1, buf);
+    e.ref_begin1 = br.ref;
+    e.read_begin1 = e.read_end1 - br.read;
+    e.rev_short = e.score1 > br.score;
+    return ssw_finish(q, r, e, sc, scratch + ssw_bytes, scratch_bytes - ssw_bytes, cig);
 }
 
 // src/aligner.rs:181-259 (segment scores with the end bonus at both ends)

@@ -108,7 +108,164 @@ SA_HD void item_finalize(uint32_t i, const uint8_t* seq, const uint64_t* off,
 }
 
 // ------------------------------------------------------------- extension ---
-// extend_seed (src/mapper.rs:532-620) and rescue_align (src/mapper.rs:1094-1153).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SA_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#else
+#define SA_ATOMIC_ADD(p, v) ((*(p) += (v)) - (v))
+#endif
+
+// Longest query the lane-parallel SSW kernel handles; longer ones use the scalar form.
+constexpr uint32_t SSW_GROUP_MAX_QUERY = 256;
+
+SA_HD void write_found(ExtResult& out, const AlignInfo& a, uint64_t ref_base, bool gapped, uint32_t n_cigar) {
+    out.found = 1;
+    out.gapped = gapped ? 1 : 0;
+    out.score = a.score;
+    out.ref_start = (uint32_t)(ref_base + a.ref_start);
+    out.length = a.ref_end - a.ref_start;
+    out.query_start = a.query_start;
+    out.query_end = a.query_end;
+    out.edit_distance = a.edit_distance;
+    out.n_cigar = n_cigar;
+}
+
+// First stage of extend_seed (src/mapper.rs:532-620) and rescue_align
+// (src/mapper.rs:1094-1153): the ungapped (Hamming) case and the rescue
+// substring prefilter are finished here; gapped alignments become jobs for the
+// SSW kernel (ssw_jobs, or long_jobs for queries above SSW_GROUP_MAX_QUERY) or
+// the piecewise kernel (pw_jobs).
+SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc, uint32_t k,
+                         const ExtRequest* req, uint32_t nreq, ExtResult* res, uint32_t* cigars, ExtJob* ssw_jobs,
+                         ExtJob* long_jobs, ExtJob* pw_jobs, uint32_t* counters) {
+    if (t >= nreq) return;
+    const ExtRequest r = req[t];
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    const uint64_t rlen = ref.len[r.ref_id];
+    auto push_ssw = [&](uint64_t ws, uint64_t we) {
+        if (we - ws > 2000 || L == 0) return;  // Aligner::align returns None (src/aligner.rs:83-86)
+        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), 0};
+        if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000) long_jobs[SA_ATOMIC_ADD(&counters[1], 1u)] = j;
+        else ssw_jobs[SA_ATOMIC_ADD(&counters[0], 1u)] = j;
+    };
+    if (r.kind == EXT_SEED) {
+        const bool use_ssw = r.flags & 1, consistent = (r.flags & 2) != 0;
+        uint64_t prs = r.ref_start > r.query_start ? r.ref_start - r.query_start : 0;
+        uint64_t pre = (uint64_t)r.ref_end + L - r.query_end;
+        if (pre > rlen) pre = rlen;
+        if (prs + L == pre && consistent) {
+            RefWin w{&ref, r.ref_id, prs, L};
+            uint32_t hd = 0;
+            for (uint32_t x = 0; x < L; ++x) hd += q[x] != w[x];
+            if ((float)hd / (float)L < 0.05f) {
+                CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+                AlignInfo a = hamming_align(q, w, L, sc, cig);
+                out.status = (uint32_t)a.status;
+                if (a.found && a.status == 0) write_found(out, a, prs, false, cig.n);
+                res[t] = out;
+                return;
+            }
+        }
+        uint32_t padding = L / 10;
+        if (use_ssw) {
+            uint64_t ws = prs > padding ? prs - padding : 0;
+            uint64_t we = pre + padding < rlen ? pre + padding : rlen;
+            push_ssw(ws, we);
+        } else {
+            uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
+            uint64_t de = (uint64_t)r.ref_end + L + padding;
+            if (de > rlen) de = rlen;
+            pw_jobs[SA_ATOMIC_ADD(&counters[2], 1u)] = ExtJob{t, r.ref_id, ds, (uint32_t)(de - ds), 0};
+        }
+    } else {
+        // has_shared_substring (src/mapper.rs:1156-1170) unless bit3 asks for a plain alignment
+        uint32_t ws = r.win_start, we = r.win_end;
+        RefWin w{&ref, r.ref_id, ws, we - ws};
+        uint32_t sub = 2 * k / 3, step = k / 3;
+        bool shared = (r.flags & 8) != 0;
+        uint32_t lim = L > sub ? L - sub : 0;
+        for (uint32_t i0 = 0; i0 < lim && !shared; i0 += step) {
+            for (uint32_t p = 0; p + sub <= w.len && !shared; ++p) {
+                uint32_t x = 0;
+                while (x < sub && w[p + x] == q[i0 + x]) ++x;
+                if (x == sub) shared = true;
+            }
+        }
+        if (shared) push_ssw(ws, we);
+    }
+    res[t] = out;
+}
+
+// Piecewise extension job (thread per job).
+SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc, uint32_t k,
+                       const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res, uint32_t* cigars,
+                       const uint32_t* chain_anchors, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
+    if (t >= njobs) return;
+    const ExtJob j = jobs[first + t];
+    const ExtRequest r = req[j.req];
+    uint8_t* scr = scratch + (uint64_t)t * scratch_per;
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RefWin w{&ref, r.ref_id, j.ws, j.wlen};
+    uint32_t* an = (uint32_t*)scr;
+    uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
+    if (an_bytes >= scratch_per) {
+        out.status = 2;
+    } else {
+        for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
+        uint32_t na = remove_spurious_anchors(an, r.n_anchors);
+        AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig);
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    }
+    res[j.req] = out;
+}
+
+// SSW job in one thread (scalar lanes): queries longer than SSW_GROUP_MAX_QUERY.
+SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
+                               const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res,
+                               uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
+    if (t >= njobs) return;
+    const ExtJob j = jobs[first + t];
+    const ExtRequest r = req[j.req];
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RefWin w{&ref, r.ref_id, j.ws, j.wlen};
+    AlignInfo a = ssw_aligner_align(q, w, sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
+    out.status = (uint32_t)a.status;
+    if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    res[j.req] = out;
+}
+
+// Traceback, CIGAR and end bonus of an SSW job whose score passes are done.
+SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
+                           const ExtRequest* req, const ExtJob* jobs, const SswEnds* ends, uint32_t njobs,
+                           ExtResult* res, uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
+    if (t >= njobs) return;
+    const ExtJob j = jobs[first + t];
+    const ExtRequest r = req[j.req];
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RefWin w{&ref, r.ref_id, j.ws, j.wlen};
+    AlignInfo a = ssw_finish(q, w, ends[first + t], sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
+    out.status = (uint32_t)a.status;
+    if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    res[j.req] = out;
+}
+
+// extend_seed (src/mapper.rs:532-620) and rescue_align (src/mapper.rs:1094-1153) for one
+// request in one thread: the fallback for requests whose scratch slot was too small.
 SA_HD void item_extend(uint32_t t, const uint8_t* seq, const uint64_t* off,
                                                RefView ref, Scores sc, uint32_t k,
                                                const ExtRequest* req, uint32_t nreq,

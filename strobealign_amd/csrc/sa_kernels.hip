@@ -1,9 +1,11 @@
 // HIP kernels of the mapping path: seeding -> index lookups/hits -> anchors ->
-// collinear chaining -> NAMs, and the extension kernel. Each thread runs one
-// work item of sa_items.hpp (a read, a read orientation or an extension request).
+// collinear chaining -> NAMs, then extension. Most threads run one work item of
+// sa_items.hpp (a read, a read orientation, an extension request or job); the
+// SSW score passes run one alignment per lane group (sa_ssw_group.hpp).
 #include <hip/hip_runtime.h>
 
 #include "sa_items.hpp"
+#include "sa_ssw_group.hpp"
 
 namespace sa {
 
@@ -48,6 +50,110 @@ __global__ void __launch_bounds__(128) k_extend(const uint8_t* __restrict__ seq,
                                                uint64_t scratch_per, uint32_t first) {
     item_extend(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, nreq, res, cigars, chain_anchors,
                 scratch, scratch_per, first);
+}
+
+__global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                 RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
+                                                 uint32_t nreq, ExtResult* __restrict__ res,
+                                                 uint32_t* __restrict__ cigars, ExtJob* __restrict__ ssw_jobs,
+                                                 ExtJob* __restrict__ long_jobs, ExtJob* __restrict__ pw_jobs,
+                                                 uint32_t* __restrict__ counters) {
+    item_ext_prep(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, nreq, res, cigars, ssw_jobs,
+                  long_jobs, pw_jobs, counters);
+}
+
+__global__ void __launch_bounds__(128) k_ext_pw(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                               RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
+                                               const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                               ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                               const uint32_t* __restrict__ chain_anchors,
+                                               uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first) {
+    item_ext_pw(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, jobs, njobs, res, cigars,
+                chain_anchors, scratch, scratch_per, first);
+}
+
+__global__ void __launch_bounds__(128) k_ext_long(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                 RefView ref, Scores sc, const ExtRequest* __restrict__ req,
+                                                 const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                                 ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                                 uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first) {
+    item_ext_long(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, req, jobs, njobs, res, cigars,
+                        scratch, scratch_per, first);
+}
+
+__global__ void __launch_bounds__(128) k_ssw_finish(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                   RefView ref, Scores sc, const ExtRequest* __restrict__ req,
+                                                   const ExtJob* __restrict__ jobs, const SswEnds* __restrict__ ends,
+                                                   uint32_t njobs, ExtResult* __restrict__ res,
+                                                   uint32_t* __restrict__ cigars, uint8_t* __restrict__ scratch,
+                                                   uint64_t scratch_per, uint32_t first) {
+    item_ssw_finish(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, req, jobs, ends, njobs, res, cigars,
+                    scratch, scratch_per, first);
+}
+
+// SSW score-pass buckets: jobs are listed by (pass, stripe count) so that each
+// launch of k_ssw_pass runs one compile-time stripe count.
+constexpr uint32_t SSW_BUCKETS = 96;
+// byte forward 0..15, byte reverse 16..31, word forward 32..63, word reverse 64..95
+SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg) {
+    return (rev ? (word ? 64u : 16u) : (word ? 32u : 0u)) + seg - 1;
+}
+
+// Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY).
+__global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__ off,
+                                                   const ExtRequest* __restrict__ req,
+                                                   const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                                   uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
+                                                   uint32_t cap) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= njobs) return;
+    const uint32_t rid = req[jobs[t].req].read_id;
+    const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
+    const uint32_t b = ssw_bucket(false, false, (L + 15) / 16);
+    lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
+}
+
+// One striped score pass (ext/ssw/ssw.c:197-588) per group of G lanes: the
+// forward pass over the whole window, or the reverse pass that finds the start
+// (ext/ssw/ssw.c:818-888). A saturated forward byte pass sends the job to the
+// word buckets; a finished forward pass sends it to the reverse bucket of its
+// stripe count; the reverse pass writes the job's SswEnds.
+template <int G, bool BYTE, int SEG>
+__global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                 RefView ref, Scores sc, const ExtRequest* __restrict__ req,
+                                                 const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ list,
+                                                 uint32_t count, uint32_t rev, SswEnd* __restrict__ fwd,
+                                                 SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
+                                                 uint32_t* __restrict__ counts, uint32_t cap) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t slot = tid / G, l = tid % G;
+    if (slot >= count) return;
+    const uint32_t job = list[slot];
+    const ExtJob j = jobs[job];
+    const ExtRequest r = req[j.req];
+    const uint64_t ro = off[r.read_id];
+    const int32_t L = (int32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, (uint32_t)L, (r.flags & 4) != 0};
+    RefCursor w{ref.words + ref.word_off[j.ref_id], j.ws, -1, 0};
+    const uint32_t gbase = (__lane_id() & 63u) & ~(uint32_t)(G - 1);
+    if (!rev) {
+        const SswEnd f = ssw_pass_seg<G, BYTE, SEG>(q, L, false, 0, w, (int32_t)j.wlen, sc, BYTE ? 255 : 65535, l,
+                                                    gbase);
+        if (l == 0) {
+            uint32_t b;
+            if (BYTE && f.score == 255) {
+                b = ssw_bucket(true, false, (uint32_t)(L + 7) / 8);
+            } else {
+                fwd[job] = f;
+                b = ssw_bucket(!BYTE, true, (uint32_t)(f.read + 1 + G - 1) / G);
+            }
+            lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = job;
+        }
+    } else {
+        const SswEnd f = fwd[job];
+        const SswEnd b = ssw_pass_seg<G, BYTE, SEG>(q, f.read + 1, true, f.read, w, f.ref + 1, sc, f.score, l, gbase);
+        if (l == 0) ends[job] = SswEnds{f.score, f.ref, f.read, b.ref, f.read - b.read, f.score > b.score};
+    }
 }
 
 }  // namespace sa

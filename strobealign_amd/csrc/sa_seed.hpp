@@ -38,14 +38,23 @@ SA_HD uint32_t nt2(uint8_t c) {
 }
 
 // Complement used for read reverse complements (src/revcomp.rs:1-29).
+// Nucleotide letters without a switch (a switch over a per-lane value becomes a
+// branch tree in device code). d = lowercase letter - 0x60: a=1, c=3, g=7, t=20,
+// u=21; for a/c/g/t, ((x >> 1) ^ (x >> 2)) & 3 gives 0, 1, 2, 3.
+SA_HD bool nt_is_acgt(uint8_t c) {
+    const uint32_t d = (uint32_t)(c | 0x20u) - 0x60u;
+    return d < 32u && ((0x10008Au >> d) & 1u);
+}
+SA_HD bool nt_is_u(uint8_t c) { return (c | 0x20u) == 'u'; }
+SA_HD uint32_t nt_acgt_code(uint8_t c) {
+    const uint32_t x = c | 0x20u;
+    return ((x >> 1) ^ (x >> 2)) & 3u;
+}
+
 SA_HD uint8_t comp_ascii(uint8_t c) {
-    switch (c) {
-        case 'A': case 'a': return 'T';
-        case 'C': case 'c': return 'G';
-        case 'G': case 'g': return 'C';
-        case 'T': case 't': case 'U': case 'u': return 'A';
-        default: return 'N';
-    }
+    // A<->T, C<->G, U->A, anything else N
+    const uint8_t tgca = (uint8_t)((0x41434754u >> (8 * nt_acgt_code(c))) & 0xFFu);
+    return nt_is_acgt(c) ? tgca : (nt_is_u(c) ? (uint8_t)'A' : (uint8_t)'N');
 }
 
 // ------------------------------------------------------------- syncmers ---

@@ -114,6 +114,16 @@ struct ExtRequest {
     uint64_t cigar_off;
 };
 
+// A gapped alignment the prep kernel hands on: SSW over a reference window, or
+// piecewise extension (src/piecewisealigner.rs:396-510) over a decoded window.
+struct ExtJob {
+    uint32_t req;      // request index (query and result slot)
+    uint32_t ref_id;
+    uint64_t ws;       // window start (absolute)
+    uint32_t wlen;     // window length
+    uint32_t pad;
+};
+
 struct ExtResult {
     uint32_t found;        // 0: None
     uint32_t gapped;

@@ -165,3 +165,54 @@ def test_ssw_against_reference_vectors(ctx, orc):
         assert g[0] == 1
         assert [g[1], g[2], g[3], g[4], g[5], g[6]] == [out[0], out[1], out[2], out[3], out[4], out[5]]
         assert "".join("%d%s" % (o >> 4, L[o & 15]) for o in g[7]) == cig
+
+
+def _mutate(rng, s, sub, indel):
+    out = bytearray()
+    for c in s:
+        x = rng.random()
+        if x < indel / 2:
+            continue
+        if x < indel:
+            out.append(rng.choice(b"ACGT"))
+        out.append(rng.choice(b"ACGT") if rng.random() < sub else c)
+    return bytes(out)
+
+
+def test_ssw_lane_groups_vs_oracle(ctx, orc):
+    """Random (query, window) pairs through the lane-group SSW kernels (byte and
+    word passes, every stripe count up to 256 bp queries, N in queries) and the
+    one-thread path for longer queries, against the oracle's SSW."""
+    say("case: SSW lane groups, 3000 random pairs")
+    rng = random.Random(2024)
+    pairs = []
+    for i in range(3000):
+        ql = rng.randint(1, 256) if i % 10 else rng.randint(257, 400)
+        q = bytes(rng.choice(b"ACGT") for _ in range(ql))
+        if rng.random() < 0.05:
+            q = bytes(c if rng.random() > 0.05 else ord("N") for c in q)
+        kind = i % 3
+        if kind == 0:  # unrelated window: low scores, byte passes only
+            r = bytes(rng.choice(b"ACGT") for _ in range(rng.randint(1, 800)))
+        else:  # the query planted with errors: high scores, word passes
+            core = _mutate(rng, q.replace(b"N", b"A"), 0.03 if kind == 1 else 0.0, 0.02 if kind == 1 else 0.0)
+            left = bytes(rng.choice(b"ACGT") for _ in range(rng.randint(0, 300)))
+            right = bytes(rng.choice(b"ACGT") for _ in range(rng.randint(0, 300)))
+            r = (left + core + right)[:2000] or b"A"
+        pairs.append((q, r))
+    got = ctx.align_ssw(pairs)
+    from test_oracle import ssw as orc_ssw
+    L = "MIDNSHP=X"
+    n_found = n_word = 0
+    for idx, ((q, r), g) in enumerate(zip(pairs, got)):
+        exp = orc_ssw(orc, q, r)
+        if exp is None:
+            assert g[0] == 0, idx
+            continue
+        out, cig = exp
+        n_found += 1
+        n_word += out[0] >= 255
+        assert g[0] == 1, idx
+        assert [g[1], g[2], g[3], g[4], g[5], g[6]] == [out[0], out[1], out[2], out[3], out[4], out[5]], idx
+        assert "".join("%d%s" % (o >> 4, L[o & 15]) for o in g[7]) == cig, idx
+    assert n_found > 2000 and n_word > 500
