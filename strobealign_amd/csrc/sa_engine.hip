@@ -330,7 +330,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     float ext_ms = 0;
     // 1. Hamming fast path, rescue prefilter, job lists
     ext_ms += timed("k_ext_prep", n, [&] {
-        k_ext_prep<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res, d_slots, d_jobs,
+        k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res, d_slots, d_jobs,
                                                         d_ljobs, d_pjobs, d_cnt);
     });
     uint32_t cnt[4] = {0, 0, 0, 0};

@@ -61,7 +61,7 @@ struct Options {
     uint32_t mcs = 0;
     uint64_t chunk_size = 1000000;
     int threads = 1;
-    uint64_t batch_bases = 64ull << 20;  // bases per device batch
+    uint64_t batch_bases = 512ull << 20;  // bases per device batch (about 90 GB of device buffers at most)
 };
 
 // src/io/sam.rs:177-201 (no @PG line)

@@ -147,7 +147,8 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
     auto push_ssw = [&](uint64_t ws, uint64_t we) {
         if (we - ws > 2000 || L == 0) return;  // Aligner::align returns None (src/aligner.rs:83-86)
         ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), 0};
-        if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000) long_jobs[SA_ATOMIC_ADD(&counters[1], 1u)] = j;
+        if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000 || sc.match > 127 || sc.mismatch > 127)
+            long_jobs[SA_ATOMIC_ADD(&counters[1], 1u)] = j;
         else ssw_jobs[SA_ATOMIC_ADD(&counters[0], 1u)] = j;
     };
     if (r.kind == EXT_SEED) {

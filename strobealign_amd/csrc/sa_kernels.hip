@@ -52,14 +52,125 @@ __global__ void __launch_bounds__(128) k_extend(const uint8_t* __restrict__ seq,
                 scratch, scratch_per, first);
 }
 
+// Mismatch bits as a sequence view for hamming_align: q[i] == r[i] exactly when
+// bit i of the mask is clear.
+struct MismatchBits {
+    uint64_t w[4];
+    __device__ uint8_t operator[](uint32_t i) const { return (uint8_t)((w[i >> 6] >> (i & 63)) & 1u); }
+};
+struct ZeroBits {
+    __device__ uint8_t operator[](uint32_t) const { return 0; }
+};
+
+// First stage of extension, one wave per request (item_ext_prep is the same
+// logic in one thread): the ungapped case of extend_seed (src/mapper.rs:567-588)
+// compares the read with the reference in 64-base ballots, and the rescue
+// prefilter has_shared_substring (src/mapper.rs:1156-1170) tests 64 window
+// positions per step. Gapped alignments are queued as jobs.
 __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                                  RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
                                                  uint32_t nreq, ExtResult* __restrict__ res,
                                                  uint32_t* __restrict__ cigars, ExtJob* __restrict__ ssw_jobs,
                                                  ExtJob* __restrict__ long_jobs, ExtJob* __restrict__ pw_jobs,
                                                  uint32_t* __restrict__ counters) {
-    item_ext_prep(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, nreq, res, cigars, ssw_jobs,
-                  long_jobs, pw_jobs, counters);
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if (t >= nreq) return;
+    const ExtRequest r = req[t];
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    const uint64_t rlen = ref.len[r.ref_id];
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    auto push_ssw = [&](uint64_t ws, uint64_t we) {
+        if (we - ws > 2000 || L == 0) return;  // Aligner::align returns None (src/aligner.rs:83-86)
+        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), 0};
+        if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000 || sc.match > 127 || sc.mismatch > 127)
+            long_jobs[atomicAdd(&counters[1], 1u)] = j;
+        else
+            ssw_jobs[atomicAdd(&counters[0], 1u)] = j;
+    };
+    if (r.kind == EXT_SEED) {
+        const bool use_ssw = r.flags & 1, consistent = (r.flags & 2) != 0;
+        const uint64_t prs = r.ref_start > r.query_start ? r.ref_start - r.query_start : 0;
+        uint64_t pre = (uint64_t)r.ref_end + L - r.query_end;
+        if (pre > rlen) pre = rlen;
+        if (prs + L == pre && consistent) {
+            RefWin w{&ref, r.ref_id, prs, L};
+            if (L <= 256) {
+                MismatchBits m;
+                uint32_t hd = 0;
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    const uint32_t pos = c * 64 + lane;
+                    m.w[c] = __ballot(pos < L && q[pos] != w[pos]);
+                    hd += (uint32_t)__popcll(m.w[c]);
+                }
+                if ((float)hd / (float)L < 0.05f) {
+                    if (lane == 0) {
+                        CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+                        AlignInfo a = hamming_align(ZeroBits{}, m, L, sc, cig);
+                        out.status = (uint32_t)a.status;
+                        if (a.found && a.status == 0) write_found(out, a, prs, false, cig.n);
+                        res[t] = out;
+                    }
+                    return;
+                }
+            } else {
+                bool done = false;
+                if (lane == 0) {
+                    uint32_t hd = 0;
+                    for (uint32_t x = 0; x < L; ++x) hd += q[x] != w[x];
+                    if ((float)hd / (float)L < 0.05f) {
+                        CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+                        AlignInfo a = hamming_align(q, w, L, sc, cig);
+                        out.status = (uint32_t)a.status;
+                        if (a.found && a.status == 0) write_found(out, a, prs, false, cig.n);
+                        res[t] = out;
+                        done = true;
+                    }
+                }
+                if (__ballot(done)) return;
+            }
+        }
+        if (lane == 0) {
+            const uint32_t padding = L / 10;
+            if (use_ssw) {
+                const uint64_t ws = prs > padding ? prs - padding : 0;
+                const uint64_t we = pre + padding < rlen ? pre + padding : rlen;
+                push_ssw(ws, we);
+            } else {
+                const uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
+                uint64_t de = (uint64_t)r.ref_end + L + padding;
+                if (de > rlen) de = rlen;
+                pw_jobs[atomicAdd(&counters[2], 1u)] = ExtJob{t, r.ref_id, ds, (uint32_t)(de - ds), 0};
+            }
+            res[t] = out;
+        }
+    } else {
+        // has_shared_substring unless bit3 asks for a plain alignment
+        const uint32_t ws = r.win_start, we = r.win_end;
+        RefWin w{&ref, r.ref_id, ws, we - ws};
+        const uint32_t sub = 2 * k / 3, step = k / 3;
+        bool shared = (r.flags & 8) != 0;
+        const uint32_t lim = L > sub ? L - sub : 0;
+        for (uint32_t i0 = 0; i0 < lim && !shared; i0 += step) {
+            for (uint32_t p0 = 0; p0 + sub <= w.len && !shared; p0 += 64) {
+                const uint32_t p = p0 + lane;
+                bool hit = false;
+                if (p + sub <= w.len) {
+                    uint32_t x = 0;
+                    while (x < sub && w[p + x] == q[i0 + x]) ++x;
+                    hit = x == sub;
+                }
+                shared = __ballot(hit) != 0;
+            }
+        }
+        if (lane == 0) {
+            if (shared) push_ssw(ws, we);
+            res[t] = out;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(128) k_ext_pw(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,

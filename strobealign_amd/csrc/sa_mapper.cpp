@@ -162,7 +162,7 @@ struct Miss {};
 
 struct CachedResult {
     ExtResult r;
-    std::vector<uint32_t> cigar;
+    uint64_t cigar_off;  // into Chunk::cigar_pool
 };
 
 // Per-chunk state of the replay.
@@ -174,6 +174,7 @@ struct Chunk {
     std::string out;
     std::vector<double> abundances;
     std::unordered_map<Key, CachedResult, KeyHash> cache;
+    std::vector<uint32_t> cigar_pool;  // CIGARs of the cached results (grows only between passes)
     std::vector<Key> pending;
     std::unordered_map<Key, char, KeyHash> pending_set;
     bool done() const { return next >= count; }
@@ -217,8 +218,8 @@ std::optional<Alignment> extend_seed(Ctx& c, const Nam& nam, uint32_t read_id, u
     Alignment a;
     a.ref_id = nam.ref_id;
     a.ref_start = r->r.ref_start;
-    a.cigar = r->cigar.data();
-    a.n_cigar = (uint32_t)r->cigar.size();
+    a.cigar = c.ch->cigar_pool.data() + r->cigar_off;
+    a.n_cigar = r->r.found ? r->r.n_cigar : 0;
     a.edit_distance = r->r.edit_distance;
     a.soft_clip_left = r->r.query_start;
     a.soft_clip_right = L - r->r.query_end;
@@ -256,8 +257,8 @@ std::optional<Alignment> rescue_align(Ctx& c, const Nam& mate, uint32_t read_id,
     Alignment a;
     a.ref_id = mate.ref_id;
     a.ref_start = r->r.ref_start;
-    a.cigar = r->cigar.data();
-    a.n_cigar = (uint32_t)r->cigar.size();
+    a.cigar = c.ch->cigar_pool.data() + r->cigar_off;
+    a.n_cigar = r->r.found ? r->r.n_cigar : 0;
     a.edit_distance = r->r.edit_distance;
     a.soft_clip_left = r->r.query_start;
     a.soft_clip_right = L - r->r.query_end;
@@ -1169,12 +1170,22 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
             });
             double h1 = now_ms();
             if (stats) stats->host_ms += h1 - h0;
-            std::vector<ExtRequest> req;
-            std::vector<std::pair<size_t, size_t>> back;
-            for (size_t cix = cb; cix < ce; ++cix) {
-                Chunk& ch = chunks[cix];
-                for (size_t x = 0; x < ch.pending.size(); ++x) {
-                    const Key& kk = ch.pending[x];
+            // requests of all chunks, chunk by chunk (built in parallel)
+            std::vector<uint64_t> roff(ce - cb + 1, 0);
+            for (size_t cix = cb; cix < ce; ++cix) roff[cix - cb + 1] = roff[cix - cb] + chunks[cix].pending.size();
+            const uint64_t nreq = roff[ce - cb];
+            bool all_done = true;
+            for (size_t cix = cb; cix < ce; ++cix) all_done &= chunks[cix].done();
+            if (nreq == 0) {
+                if (!all_done) throw std::runtime_error("replay stalled without pending extensions");
+                break;
+            }
+            std::vector<ExtRequest> req(nreq);
+            parallel_for(o.threads, ce - cb, [&](size_t x) {
+                const Chunk& ch = chunks[cb + x];
+                ExtRequest* out_r = req.data() + roff[x];
+                for (size_t y = 0; y < ch.pending.size(); ++y) {
+                    const Key& kk = ch.pending[y];
                     ExtRequest r{};
                     r.kind = kk.kind;
                     r.read_id = kk.read;
@@ -1187,43 +1198,52 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
                     } else {
                         r.win_start = kk.a; r.win_end = kk.b;
                     }
-                    req.push_back(r);
-                    back.push_back({cix, x});
+                    out_r[y] = r;
                 }
-            }
-            bool all_done = true;
-            for (size_t cix = cb; cix < ce; ++cix) all_done &= chunks[cix].done();
-            if (req.empty()) {
-                if (!all_done) throw std::runtime_error("replay stalled without pending extensions");
-                break;
-            }
-            if (stats) { stats->rounds++; stats->requests += req.size(); }
+            });
+            if (stats) { stats->rounds++; stats->requests += nreq; }
             std::vector<ExtResult> res;
             std::vector<uint32_t> cig;
             std::vector<uint64_t> coff;
             double g0 = now_ms();
-            eng.extend(req.data(), (uint32_t)req.size(), sc, res, cig, coff);
+            if (stats) stats->host_ms += g0 - h1;
+            eng.extend(req.data(), (uint32_t)nreq, sc, res, cig, coff);
             double g1 = now_ms();
             if (stats) stats->extend_ms += g1 - g0;
-            for (size_t x = 0; x < req.size(); ++x) {
-                Chunk& ch = chunks[back[x].first];
-                CachedResult cr;
-                cr.r = res[x];
-                cr.cigar.assign(cig.begin() + (ptrdiff_t)coff[x], cig.begin() + (ptrdiff_t)coff[x + 1]);
-                ch.cache.emplace(ch.pending[back[x].second], std::move(cr));
-            }
-            for (size_t cix = cb; cix < ce; ++cix) {
-                chunks[cix].pending.clear();
-                chunks[cix].pending_set.clear();
-            }
+            // results into the chunks' caches (each chunk owns its cache and CIGAR pool)
+            parallel_for(o.threads, ce - cb, [&](size_t x) {
+                Chunk& ch = chunks[cb + x];
+                const uint64_t a = roff[x], b = roff[x + 1];
+                ch.cache.reserve(ch.cache.size() + (b - a));
+                ch.cigar_pool.reserve(ch.cigar_pool.size() + (coff[b] - coff[a]));
+                for (uint64_t y = a; y < b; ++y) {
+                    CachedResult cr;
+                    cr.r = res[y];
+                    cr.cigar_off = ch.cigar_pool.size();
+                    ch.cigar_pool.insert(ch.cigar_pool.end(), cig.begin() + (ptrdiff_t)coff[y],
+                                         cig.begin() + (ptrdiff_t)coff[y + 1]);
+                    ch.cache.emplace(ch.pending[y - a], cr);
+                }
+                ch.pending.clear();
+                ch.pending_set.clear();
+            });
             if (stats) stats->host_ms += now_ms() - g1;
         }
-        for (size_t c = cb; c < ce; ++c) {
-            out += chunks[c].out;
-            std::string().swap(chunks[c].out);
-            chunks[c].cache.clear();
-            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += chunks[c].abundances[i];
+        // append the chunks' output in order (copied in parallel)
+        {
+            std::vector<uint64_t> pos(ce - cb + 1, out.size());
+            for (size_t c = cb; c < ce; ++c) pos[c - cb + 1] = pos[c - cb] + chunks[c].out.size();
+            out.resize(pos[ce - cb]);
+            parallel_for(o.threads, ce - cb, [&](size_t x) {
+                Chunk& ch = chunks[cb + x];
+                if (!ch.out.empty()) std::memcpy(&out[pos[x]], ch.out.data(), ch.out.size());
+                std::string().swap(ch.out);
+                decltype(ch.cache)().swap(ch.cache);
+                std::vector<uint32_t>().swap(ch.cigar_pool);
+            });
         }
+        for (size_t c = cb; c < ce; ++c)
+            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += chunks[c].abundances[i];
         cb = ce;
     }
     if (o.mode == 2) {
