@@ -60,13 +60,13 @@ Engine::Engine(int device) : device_(device) {
 }
 
 Engine::~Engine() {
-    hipSetDevice(device_);
+    (void)hipSetDevice(device_);  // destructor: errors are not reported
     for (void* p : buf_)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (void* p : {(void*)d_entries_, (void*)d_buckets_, (void*)d_words_, (void*)d_word_off_, (void*)d_len_,
                     (void*)d_log2_, (void*)d_same_diag_})
-        if (p) hipFree(p);
-    if (stream_) hipStreamDestroy(stream_);
+        if (p) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Engine::add_kernel(const char* name, double ms, uint64_t items) {
@@ -257,9 +257,9 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n);
     SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
-    for (auto& e : ev) hipEventDestroy(e);
-    hipEventDestroy(fe0);
-    hipEventDestroy(fe1);
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(fe0);
+    (void)hipEventDestroy(fe1);
 }
 
 namespace {
@@ -419,8 +419,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     times_.extend = ext_ms;
     SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipStreamSynchronize(stream_));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     // requests that ran out of scratch are re-run one at a time with a large slot
     std::vector<uint32_t> big;
     for (uint32_t i = 0; i < n; ++i)

@@ -158,7 +158,6 @@ struct KeyHash {
 };
 static_assert(sizeof(Key) == 40, "Key packing");
 
-struct Miss {};
 
 struct CachedResult {
     ExtResult r;
@@ -190,12 +189,15 @@ struct Ctx {
     uint64_t batch_first;  // first record index of the engine batch
     uint32_t k;
     Chunk* ch;
-    bool probe;
+    bool missed;   // a lookup of this record had no result yet (it is now pending)
+    bool abandon;  // the record stopped early: its state changes are discarded
     Rng* rng;
     InsertSize* isize;
 };
 
-// Look up (or register a miss for) an extension result.
+// Look up an extension result. A missing one is registered as pending and
+// reported as no alignment; the record then runs on as a probe that collects
+// its other requests, and is not committed in this pass.
 const CachedResult* lookup(Ctx& c, const Key& key) {
     auto it = c.ch->cache.find(key);
     if (it != c.ch->cache.end()) return &it->second;
@@ -203,8 +205,8 @@ const CachedResult* lookup(Ctx& c, const Key& key) {
         c.ch->pending_set.emplace(key, 1);
         c.ch->pending.push_back(key);
     }
-    if (c.probe) return nullptr;
-    throw Miss{};
+    c.missed = true;
+    return nullptr;
 }
 
 // src/mapper.rs:532-620 (device request kind EXT_SEED)
@@ -661,14 +663,14 @@ PairedResult extend_paired_seeds(Ctx& c, std::vector<Nam> nams[2], const uint32_
         bool c1 = m1.cons, c2 = m2.cons;
         det[0].inconsistent_nams += !c1;
         det[1].inconsistent_nams += !c2;
-        // both requests are registered before a miss stops the record
-        bool saved = c.probe;
-        c.probe = true;
+        // both requests are registered before a new miss stops the record
         size_t before = c.ch->pending.size();
         auto a1 = extend_seed(c, m1, rid[0], L[0], c1, true);
         auto a2 = extend_seed(c, m2, rid[1], L[1], c2, true);
-        c.probe = saved;
-        if (c.ch->pending.size() != before) throw Miss{};
+        if (c.ch->pending.size() != before) {
+            c.abandon = true;
+            return res;
+        }
         if (a1 && a2) {
             det[0].tried_alignment++;
             det[0].gapped += a1->gapped;
@@ -787,6 +789,7 @@ void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Al
     InsertSize& isz = *c.isize;
     float mu = isz.mu, sigma = isz.sigma;
     PairedResult pr = extend_paired_seeds(c, nams, rid, L, det, mu, sigma);
+    if (c.abandon) return;
     if (pr.proper) {
         bool ps = is_proper_pair(&pr.p1, &pr.p2, mu, sigma);
         if (ps && isz.sample_size < 400 && pr.p1.edit_distance + pr.p2.edit_distance < 3) {
@@ -1010,59 +1013,53 @@ void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
 }
 
 // One pass over a chunk: commit records until the first miss, then probe the rest.
+// Records that miss leave the state as their probe run made it, except records
+// that stop early, whose state changes are undone. (Control flow uses flags, not
+// exceptions: throwing from many threads serialises on the unwinder.)
 void run_chunk(Ctx base, Chunk& ch) {
     std::vector<Sam> recs;
     std::deque<Alignment> store;
     std::deque<std::array<Details, 2>> dstore;
     Ctx c = base;
     c.ch = &ch;
-    c.probe = false;
     Rng rng = ch.rng;
     InsertSize isz = ch.isize;
     bool probing = false;
     for (uint64_t r = ch.next; r < ch.count; ++r) {
         const uint64_t idx = ch.first + r;
-        Rng rng0 = rng;
-        InsertSize isz0 = isz;
+        const Rng rng0 = rng;
+        const InsertSize isz0 = isz;
         c.rng = &rng;
         c.isize = &isz;
-        c.probe = probing;
+        c.missed = false;
+        c.abandon = false;
         recs.clear();
         store.clear();
         dstore.clear();
-        try {
-            if (c.o.mode == 0) {
-                if (c.paired) {
-                    align_paired_end(c, idx, recs, store, dstore);
-                    if (c.o.only_mapped && !recs[0].is_mapped() && !recs[1].is_mapped()) recs.clear();
-                } else {
-                    align_single_end(c, idx, recs, store, dstore);
-                    if (c.o.only_mapped && !recs[0].is_mapped()) recs.clear();
-                }
-                if (!probing) {
-                    for (const Sam& s : recs) write_sam(ch.out, s, c);
-                }
+        if (c.o.mode == 0) {
+            if (c.paired) {
+                align_paired_end(c, idx, recs, store, dstore);
+                if (!c.abandon && c.o.only_mapped && !recs[0].is_mapped() && !recs[1].is_mapped()) recs.clear();
             } else {
-                std::string tmp;
-                std::vector<double> ab_probe;
-                if (probing) ab_probe.assign(ch.abundances.size(), 0.0);
-                map_only(c, idx, probing ? tmp : ch.out, probing ? ab_probe : ch.abundances);
+                align_single_end(c, idx, recs, store, dstore);
+                if (c.o.only_mapped && !recs[0].is_mapped()) recs.clear();
             }
-            if (!probing) {
-                ch.next = r + 1;
-                ch.rng = rng;
-                ch.isize = isz;
+            if (!probing && !c.missed && !c.abandon) {
+                for (const Sam& s : recs) write_sam(ch.out, s, c);
             }
-        } catch (const Miss&) {
-            // restore the state of this record, probe it for its other misses, and
-            // continue in probe mode
+        } else {
+            // mapping-only modes do no extension, so they never miss
+            map_only(c, idx, ch.out, ch.abundances);
+        }
+        if (c.abandon) {
             rng = rng0;
             isz = isz0;
-            if (!probing) {
-                probing = true;
-                --r;
-                continue;
-            }
+        }
+        if (c.missed || c.abandon) probing = true;
+        if (!probing) {
+            ch.next = r + 1;
+            ch.rng = rng;
+            ch.isize = isz;
         }
     }
 }
@@ -1156,7 +1153,7 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
             stats->nam_ms += t2 - t1;
             stats->nams += nb.nams.size();
         }
-        Ctx base{o, ref, nb, r1, r2, paired, bfirst, k, nullptr, false, nullptr, nullptr};
+        Ctx base{o, ref, nb, r1, r2, paired, bfirst, k, nullptr, false, false, nullptr, nullptr};
         for (size_t c = cb; c < ce; ++c) {
             chunks[c].abundances.assign(ref.names.size(), 0.0);
             chunks[c].out.reserve(chunks[c].count * (paired ? 1400 : 700));
