@@ -5,8 +5,8 @@ segments at 2% divergence; 1,000,000 pairs 2x150 (fragment Normal(300,30),
 0.5% substitutions, 0.05% indels) generated by strobealign_amd/synth.py. One
 step maps all pairs through the HIP library (seeding, lookups, chaining, NAMs,
 extension on the GPU; decision replay and SAM formatting on the host) with the
-reads already parsed in host memory and the index resident in HBM. Output SAM
-is produced in memory and discarded.
+reads parsed once and staged in HBM before timing and the index resident in
+HBM. The SAM text is produced per chunk in host memory and then dropped.
 
 Prints one JSON line (see the contract in the repository docs) with a roofline
 object for the dominant kernel (HIP-event timing on the engine's stream) and a
@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--data-dir", default=os.environ.get("SA_BENCH_DATA", "/tmp/strobealign_amd_bench"))
     ap.add_argument("--cpu-pairs", type=int, default=40_000, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -81,8 +82,12 @@ def main():
     ix = sa.Index(ctx, w["ref"], read_length=150, threads=threads)
     index_s = time.time() - t0
     reads = sa.Reads(w["reads1"], w["reads2"])
-    opts = sa.options(threads=threads)
-    print("index %.1fs, %d pairs loaded" % (index_s, len(reads)), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    reads.upload(ix)  # inputs resident in HBM before the timed region
+    upload_s = time.perf_counter() - t0
+    opts = sa.options(threads=threads, slots=a.slots)
+    print("index %.1fs, %d pairs loaded, staged on the device in %.1f ms" % (index_s, len(reads), 1000 * upload_s),
+          file=sys.stderr, flush=True)
     for i in range(a.warmup):
         _, st = ix.map_reads(reads, opts, keep_output=False)
         print("warmup %d: %.1f ms" % (i, st.total_ms), file=sys.stderr, flush=True)
@@ -92,14 +97,12 @@ def main():
     for i in range(a.steps):
         _, st = ix.map_reads(reads, opts, keep_output=False)
         stats.append(st)
-        print("step %d: %.1f ms (upload %.1f, nams %.1f, extend %.1f, host %.1f, rounds %d)" %
-              (i, st.total_ms, st.upload_ms, st.nam_ms, st.extend_ms, st.host_ms, st.rounds), file=sys.stderr,
-              flush=True)
+        print("step %d: %.1f ms (summed over batches: nams %.1f, extend %.1f, host %.1f; rounds %d)" %
+              (i, st.total_ms, st.nam_ms, st.extend_ms, st.host_ms, st.rounds), file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     k1 = ix.kernel_times()
     n_pairs = len(reads)
-    upload_s = sum(s.upload_ms for s in stats) / 1000.0
-    value = n_pairs * a.steps / (wall - upload_s)
+    value = n_pairs * a.steps / wall
     # dominant kernel over the timed steps
     delta = {k: (k1[k][0] - k0.get(k, (0, 0, 0))[0], k1[k][1] - k0.get(k, (0, 0, 0))[1],
                  k1[k][2] - k0.get(k, (0, 0, 0))[2]) for k in k1}
@@ -148,8 +151,8 @@ def main():
         "data": "synthetic (strobealign_amd/synth.py pe100m, seed 7)",
         "config": {"workload": "100 Mbp reference (20x5 Mbp, 30%% repeats), %d pairs 2x150, SAM output" % n_pairs,
                    "pairs": n_pairs, "read_len": 150, "host_threads": threads,
-                   "upload_ms_per_step": round(1000 * upload_s / a.steps, 2),
-                   "pairs_per_s_incl_pcie": round(n_pairs * a.steps / wall, 1),
+                   "read_upload_ms": round(1000 * upload_s, 2),
+                   "pairs_per_s_incl_pcie": round(n_pairs * a.steps / (wall + a.steps * upload_s), 1),
                    "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests},
         "roofline": roof,
         "cpu_baseline": cpu,

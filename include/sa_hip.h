@@ -97,6 +97,7 @@ typedef struct sa_map_options {
     uint64_t chunk_size;     /* --chunk-size (1000000) */
     int32_t threads;         /* host threads for the decision replay and output */
     uint64_t batch_bases;    /* bases per device batch (0: default) */
+    int32_t slots;           /* batches in flight on the device at once (0: default 2) */
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */
@@ -122,6 +123,9 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
 typedef struct sa_reads sa_reads;
 int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out);
 uint64_t sa_reads_count(const sa_reads* reads);
+/* Stages the read set in the index's device memory, so that sa_map_reads starts
+ * from reads already resident in HBM (no host-to-device copy per call). */
+int sa_reads_upload(sa_index* index, sa_reads* reads);
 void sa_reads_destroy(sa_reads* reads);
 
 /* Wall-clock breakdown of one sa_map_reads call (milliseconds). */

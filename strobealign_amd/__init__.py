@@ -36,7 +36,8 @@ class MapOptions(ctypes.Structure):
                 ("rg_id", ctypes.c_char_p), ("rg", ctypes.POINTER(ctypes.c_char_p)), ("n_rg", ctypes.c_int32),
                 ("max_secondary", ctypes.c_uint32), ("dropoff", ctypes.c_float),
                 ("rescue_distance", ctypes.c_uint32), ("max_tries", ctypes.c_uint32), ("mcs", ctypes.c_uint32),
-                ("chunk_size", ctypes.c_uint64), ("threads", ctypes.c_int32), ("batch_bases", ctypes.c_uint64)]
+                ("chunk_size", ctypes.c_uint64), ("threads", ctypes.c_int32), ("batch_bases", ctypes.c_uint64),
+                ("slots", ctypes.c_int32)]
 
 
 class QRand(ctypes.Structure):
@@ -60,7 +61,7 @@ EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa
            "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
            "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
-           "sa_reads_destroy", "sa_map_reads", "sa_free"]
+           "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_free"]
 
 
 class RunStats(ctypes.Structure):
@@ -81,6 +82,10 @@ class Reads:
     def __len__(self):
         return _lib.sa_reads_count(self._p)
 
+    def upload(self, index):
+        """Stages the reads in the index's device memory (sa_reads_upload)."""
+        _check(_lib.sa_reads_upload(index._p, self._p))
+
     def close(self):
         if self._p:
             _lib.sa_reads_destroy(self._p)
@@ -88,12 +93,12 @@ class Reads:
 
 
 def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000, use_ssw=False, max_secondary=0,
-            mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0):
+            mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0, slots=0):
     o = MapOptions()
     _lib.sa_map_options_default(ctypes.byref(o))
     o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
     o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
-    o.batch_bases = batch_bases
+    o.batch_bases, o.slots = batch_bases, slots
     if rg_id is not None:
         o.rg_id = rg_id.encode()
     o._rg_keep = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])

@@ -15,13 +15,22 @@ struct sa_ctx {
 
 struct sa_reads {
     std::vector<sa::Record> r1, r2;
+    // device-resident copy (sa_reads_upload)
+    int device = -1;
+    void* d_seq = nullptr;
+    sa::DeviceReads dr;
+    ~sa_reads() {
+        if (d_seq) sa::device_free(device, d_seq);
+    }
 };
 
 struct sa_index {
     int device;
     std::unique_ptr<sa::Engine> eng;
+    std::vector<std::unique_ptr<sa::Engine>> slots;  // more engines sharing eng's index
     sa::PackedRef ref;
     sa::SeedParams p;
+    ~sa_index() { slots.clear(); }  // before eng, whose index they share
 };
 
 namespace {
@@ -72,7 +81,17 @@ sa::Options options_of(const sa_map_options* o) {
     r.max_tries = o->max_tries; r.mcs = o->mcs; r.chunk_size = o->chunk_size;
     r.threads = o->threads > 0 ? o->threads : 1;
     if (o->batch_bases) r.batch_bases = o->batch_bases;
+    if (o->slots > 0) r.slots = o->slots;
     return r;
+}
+
+// The index's engine plus enough engines sharing its index for `n` slots.
+std::vector<sa::Engine*> engines(sa_index* ix, int n) {
+    std::vector<sa::Engine*> v{ix->eng.get()};
+    while ((int)ix->slots.size() + 1 < n)
+        ix->slots.push_back(std::make_unique<sa::Engine>(sa::Engine::ShareIndex{}, *ix->eng));
+    for (int i = 0; i + 1 < n; ++i) v.push_back(ix->slots[(size_t)i].get());
+    return v;
 }
 
 char* dup_out(const std::string& s, size_t* len) {
@@ -197,7 +216,7 @@ int sa_map_files(sa_index* index, const sa_map_options* opt, const char* reads1,
         sa::Options o = options_of(opt);
         std::string s;
         if (o.mode == 0) s = sa::sam_header(index->ref, o);
-        sa::map_records(*index->eng, index->ref, r1, r2, o, s, nullptr);
+        sa::map_records(engines(index, o.slots), index->ref, r1, r2, o, &s, nullptr, nullptr);
         *out = dup_out(s, out_len);
         return SA_OK;
     });
@@ -227,7 +246,7 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
         }
         sa::Options o = options_of(opt);
         std::string s;
-        sa::map_records(*index->eng, index->ref, r1, r2, o, s, nullptr);
+        sa::map_records(engines(index, o.slots), index->ref, r1, r2, o, &s, nullptr, nullptr);
         *out = dup_out(s, out_len);
         return SA_OK;
     });
@@ -248,6 +267,34 @@ int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out) {
 
 uint64_t sa_reads_count(const sa_reads* reads) { return reads ? reads->r1.size() : 0; }
 
+int sa_reads_upload(sa_index* index, sa_reads* reads) {
+    if (!index || !reads) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        const bool paired = !reads->r2.empty();
+        const uint64_t n = reads->r1.size();
+        std::vector<uint64_t> off(1, 0);
+        off.reserve((paired ? 2 : 1) * n + 1);
+        for (uint64_t i = 0; i < n; ++i) {
+            off.push_back(off.back() + reads->r1[i].seq.size());
+            if (paired) off.push_back(off.back() + reads->r2[i].seq.size());
+        }
+        std::string seq;
+        seq.resize(off.back() + 16);  // zero padding after the last base
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint64_t x = paired ? 2 * i : i;
+            std::memcpy(&seq[off[x]], reads->r1[i].seq.data(), reads->r1[i].seq.size());
+            if (paired) std::memcpy(&seq[off[x + 1]], reads->r2[i].seq.data(), reads->r2[i].seq.size());
+        }
+        if (reads->d_seq) sa::device_free(reads->device, reads->d_seq);
+        reads->d_seq = nullptr;
+        reads->d_seq = sa::device_upload(index->device, seq.data(), seq.size());
+        reads->device = index->device;
+        reads->dr.d_seq = (const uint8_t*)reads->d_seq;
+        reads->dr.off = std::move(off);
+        return SA_OK;
+    });
+}
+
 void sa_reads_destroy(sa_reads* reads) { delete reads; }
 
 int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* reads, char** out, size_t* out_len,
@@ -258,7 +305,8 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
         std::string s;
         if (o.mode == 0 && out) s = sa::sam_header(index->ref, o);
         sa::RunStats st;
-        sa::map_records(*index->eng, index->ref, reads->r1, reads->r2, o, s, &st);
+        const sa::DeviceReads* dr = reads->d_seq && reads->device == index->device ? &reads->dr : nullptr;
+        sa::map_records(engines(index, o.slots), index->ref, reads->r1, reads->r2, o, out ? &s : nullptr, &st, dr);
         if (stats) {
             stats->total_ms = st.total_ms; stats->upload_ms = st.upload_ms; stats->nam_ms = st.nam_ms;
             stats->extend_ms = st.extend_ms; stats->host_ms = st.host_ms; stats->records = st.records;
@@ -374,15 +422,32 @@ int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* quer
 
 int sa_index_stats(const sa_index* index, double* kernel_ms) {
     if (!index) return fail(SA_ERR_ARG, "index is NULL");
-    if (kernel_ms) *kernel_ms = index->eng->total_kernel_ms();
+    if (kernel_ms) {
+        *kernel_ms = index->eng->total_kernel_ms();
+        for (const auto& e : index->slots) *kernel_ms += e->total_kernel_ms();
+    }
     return SA_OK;
 }
 
 size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
     if (!index) return 0;
+    // summed over the index's engines
+    std::vector<std::pair<std::string, sa::Engine::KernelStat>> all = index->eng->kernel_stats();
+    for (const auto& e : index->slots)
+        for (const auto& k : e->kernel_stats()) {
+            bool found = false;
+            for (auto& a : all)
+                if (a.first == k.first) {
+                    a.second.ms += k.second.ms;
+                    a.second.launches += k.second.launches;
+                    a.second.items += k.second.items;
+                    found = true;
+                }
+            if (!found) all.push_back(k);
+        }
     std::string s = "{";
     bool first = true;
-    for (const auto& k : index->eng->kernel_stats()) {
+    for (const auto& k : all) {
         char tmp[256];
         std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu]", first ? "" : ", ", k.first.c_str(), k.second.ms,
                       (unsigned long long)k.second.launches, (unsigned long long)k.second.items);

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -17,7 +18,7 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: strobealign-hip [-t N] [-x|--aemb] [-r N] [-N N] [-U] [--eqx] [--ssw] [-C] [--details]\n"
                  "                       [--rg-id ID] [--rg TAG] [--chunk-size N] [--mcs MODE] [-o OUT]\n"
-                 "                       [--device D] [--batch-bases N] [--dump-nams PATH] [--time]\n"
+                 "                       [--device D] [--batch-bases N] [--slots N] [--dump-nams PATH] [--time]\n"
                  "                       ref.fa reads1 [reads2]\n");
 }
 
@@ -57,6 +58,7 @@ int main(int argc, char** argv) {
         else if (is("--rg")) o.rg_fields.push_back(val("--rg"));
         else if (is("--chunk-size")) o.chunk_size = std::stoull(val("--chunk-size"));
         else if (is("--batch-bases")) o.batch_bases = std::stoull(val("--batch-bases"));
+        else if (is("--slots")) o.slots = std::stoi(val("--slots"));
         else if (is("--device")) device = std::stoi(val("--device"));
         else if (is("--dump-nams")) dump_nams = val("--dump-nams");
         else if (is("--mcs")) {
@@ -114,7 +116,13 @@ int main(int argc, char** argv) {
         }
         std::string out;
         RunStats st;
-        map_records(eng, ref, r1, r2, o, out, &st);
+        std::vector<std::unique_ptr<Engine>> extra;
+        std::vector<Engine*> engs{&eng};
+        for (int i = 1; i < o.slots; ++i) {
+            extra.push_back(std::make_unique<Engine>(Engine::ShareIndex{}, eng));
+            engs.push_back(extra.back().get());
+        }
+        map_records(engs, ref, r1, r2, o, &out, &st, nullptr);
         double t3 = now_s();
         FILE* f = out_path.empty() ? stdout : std::fopen(out_path.c_str(), "wb");
         if (!f) throw std::runtime_error("cannot open " + out_path);
@@ -135,7 +143,20 @@ int main(int argc, char** argv) {
         if (time_it) {
             // per-kernel totals: milliseconds, launches, work items
             std::string k = "{";
-            for (const auto& ks : eng.kernel_stats()) {
+            std::vector<std::pair<std::string, Engine::KernelStat>> all;
+            for (Engine* e : engs)
+                for (const auto& ks : e->kernel_stats()) {
+                    bool found = false;
+                    for (auto& a : all)
+                        if (a.first == ks.first) {
+                            a.second.ms += ks.second.ms;
+                            a.second.launches += ks.second.launches;
+                            a.second.items += ks.second.items;
+                            found = true;
+                        }
+                    if (!found) all.push_back(ks);
+                }
+            for (const auto& ks : all) {
                 char b[160];
                 std::snprintf(b, sizeof b, "%s\"%s\": [%.3f, %llu, %llu]", k.size() > 1 ? ", " : "", ks.first.c_str(),
                               ks.second.ms, (unsigned long long)ks.second.launches,

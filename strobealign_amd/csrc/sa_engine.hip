@@ -49,6 +49,20 @@ enum Slot {
 };
 }  // namespace
 
+void* device_upload(int device, const void* src, uint64_t bytes) {
+    SA_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    SA_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+    if (bytes) SA_CHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    return p;
+}
+
+void device_free(int device, void* p) {
+    if (!p) return;
+    (void)hipSetDevice(device);
+    (void)hipFree(p);
+}
+
 Engine::Engine(int device) : device_(device) {
     int n = 0;
     SA_CHECK(hipGetDeviceCount(&n));
@@ -59,13 +73,30 @@ Engine::Engine(int device) : device_(device) {
     cap_.assign(S_COUNT, 0);
 }
 
+Engine::Engine(ShareIndex, const Engine& owner) : Engine(owner.device_) {
+    // shares the owner's read-only reference, index and log2 table
+    p_ = owner.p_;
+    n_entries_ = owner.n_entries_;
+    d_entries_ = owner.d_entries_;
+    d_buckets_ = owner.d_buckets_;
+    d_words_ = owner.d_words_;
+    d_word_off_ = owner.d_word_off_;
+    d_len_ = owner.d_len_;
+    n_contigs_ = owner.n_contigs_;
+    d_log2_ = owner.d_log2_;
+    owns_index_ = false;
+    SA_CHECK(hipMalloc(&d_same_diag_, 1024 * sizeof(float)));
+}
+
 Engine::~Engine() {
     (void)hipSetDevice(device_);  // destructor: errors are not reported
     for (void* p : buf_)
         if (p) (void)hipFree(p);
-    for (void* p : {(void*)d_entries_, (void*)d_buckets_, (void*)d_words_, (void*)d_word_off_, (void*)d_len_,
-                    (void*)d_log2_, (void*)d_same_diag_})
-        if (p) (void)hipFree(p);
+    if (owns_index_)
+        for (void* p : {(void*)d_entries_, (void*)d_buckets_, (void*)d_words_, (void*)d_word_off_, (void*)d_len_,
+                        (void*)d_log2_})
+            if (p) (void)hipFree(p);
+    if (d_same_diag_) (void)hipFree(d_same_diag_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -131,22 +162,29 @@ void Engine::upload_index(const RefEntry* entries, uint64_t n, const uint64_t* b
 
 void Engine::upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_reads) {
     SA_CHECK(hipSetDevice(device_));
+    const uint64_t nb = off[n_reads];
+    void* s = grow(S_SEQ, nb + 16);
+    if (nb) SA_CHECK(hipMemcpyAsync(s, seq, nb, hipMemcpyHostToDevice, stream_));
+    use_reads((const uint8_t*)s, off, n_reads);
+}
+
+void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_reads) {
+    SA_CHECK(hipSetDevice(device_));
     n_reads_ = n_reads;
     h_off_.assign(off, off + n_reads + 1);
     n_bases_ = h_off_[n_reads];
     max_len_ = 0;
     for (uint32_t i = 0; i < n_reads; ++i) max_len_ = std::max<uint32_t>(max_len_, (uint32_t)(off[i + 1] - off[i]));
     if (max_len_ >= 65536) throw std::runtime_error("reads longer than 65535 bp are not supported");
-    void* s = grow(S_SEQ, n_bases_ + 16);
+    d_seq_ = d_seq;
     void* o = grow(S_OFF, (uint64_t)(n_reads + 1) * 8);
-    if (n_bases_) SA_CHECK(hipMemcpyAsync(s, seq, n_bases_, hipMemcpyHostToDevice, stream_));
     SA_CHECK(hipMemcpyAsync(o, off, (uint64_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, stream_));
     SA_CHECK(hipStreamSynchronize(stream_));
 }
 
 void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) {
     const uint32_t n = n_reads_;
-    auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    auto* d_seq = d_seq_;
     auto* d_off = (const uint64_t*)buf_[S_OFF];
     auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
@@ -182,7 +220,7 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     for (auto& e : ev) SA_CHECK(hipEventCreate(&e));
     IndexView ix{d_entries_, d_buckets_, p_};
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
-    auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    auto* d_seq = d_seq_;
     auto* d_off = (const uint64_t*)buf_[S_OFF];
     auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
@@ -307,7 +345,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     SA_CHECK(hipMemcpyAsync(d_req, req.data(), (uint64_t)n * sizeof(ExtRequest), hipMemcpyHostToDevice, stream_));
     SA_CHECK(hipMemsetAsync(d_cnt, 0, 16, stream_));
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
-    const auto* d_seq = (const uint8_t*)buf_[S_SEQ];
+    const auto* d_seq = d_seq_;
     const auto* d_off = (const uint64_t*)buf_[S_OFF];
     const uint32_t* d_chain = (const uint32_t*)buf_[S_CHAIN];
     const uint32_t k = (uint32_t)p_.k;

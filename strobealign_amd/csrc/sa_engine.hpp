@@ -26,9 +26,17 @@ struct StageTimes {
     float seed = 0, hits = 0, chain = 0, finalize = 0, extend = 0, scans = 0;
 };
 
+// Plain device buffers outside an engine (a read set staged once for many calls).
+void* device_upload(int device, const void* src, uint64_t bytes);
+void device_free(int device, void* p);
+
 class Engine {
 public:
     explicit Engine(int device);
+    // A second engine (own stream and batch buffers) on the owner's device that
+    // shares its resident reference and index; the owner must outlive it.
+    struct ShareIndex {};
+    Engine(ShareIndex, const Engine& owner);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -42,6 +50,9 @@ public:
     // resident when `seq == nullptr`, see upload_reads). Keeps the reads and the chain
     // anchors on the device for the extension calls that follow.
     void upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_reads);
+    // Same for reads already in device memory: `d_seq` points at the first base of
+    // the batch, `off` (host) holds n_reads + 1 offsets relative to it.
+    void use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_reads);
     void find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out);
 
     // Fine-grained stages for parity probes (valid after upload_reads).
@@ -86,6 +97,8 @@ private:
     uint64_t n_bases_ = 0;
     uint32_t max_len_ = 0;
     std::vector<uint64_t> h_off_;
+    const uint8_t* d_seq_ = nullptr;
+    bool owns_index_ = true;
     // growable device buffers indexed by slot
     std::vector<void*> buf_;
     std::vector<uint64_t> cap_;

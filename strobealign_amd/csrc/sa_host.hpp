@@ -61,7 +61,8 @@ struct Options {
     uint32_t mcs = 0;
     uint64_t chunk_size = 1000000;
     int threads = 1;
-    uint64_t batch_bases = 512ull << 20;  // bases per device batch (about 90 GB of device buffers at most)
+    uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
+    int slots = 2;                        // engines working on batches concurrently
 };
 
 // src/io/sam.rs:177-201 (no @PG line)
@@ -77,5 +78,21 @@ struct RunStats {
 // as the final table.
 void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
                  const Options& o, std::string& out, RunStats* stats);
+
+// A read set already in device memory: the sequences of all records
+// concatenated (paired: r1 then r2 of each pair) at d_seq, and host offsets
+// (records + 1) into it.
+struct DeviceReads {
+    const uint8_t* d_seq = nullptr;
+    std::vector<uint64_t> off;
+};
+
+// Same over several engines that share one resident index: device batches of
+// whole chunks are dealt to them and run concurrently. `out` may be null (the
+// output is produced per chunk and dropped); `dreads` may be null (each batch
+// uploads its reads).
+void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, const std::vector<Record>& r1,
+                 const std::vector<Record>& r2, const Options& o, std::string* out, RunStats* stats,
+                 const DeviceReads* dreads);
 
 }  // namespace sa

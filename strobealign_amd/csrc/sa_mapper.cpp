@@ -18,6 +18,7 @@
 #include <cstring>
 #include <array>
 #include <deque>
+#include <exception>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -1085,8 +1086,10 @@ double now_ms() {
 
 }  // namespace
 
-void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
-                 const Options& o, std::string& out, RunStats* stats) {
+void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, const std::vector<Record>& r1,
+                 const std::vector<Record>& r2, const Options& o, std::string* out, RunStats* stats,
+                 const DeviceReads* dreads) {
+    if (engines.empty()) throw std::invalid_argument("map_records: no engine");
     const double t_start = now_ms();
     const bool paired = !r2.empty();
     const uint64_t n = paired ? std::min(r1.size(), r2.size()) : r1.size();
@@ -1112,16 +1115,9 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
             chunks.push_back(std::move(c));
         }
     }
-    std::vector<double> abundances(ref.names.size(), 0.0);
-    Scores sc{2, 8, 12, 1, 10, 500};
-    MapParams mp{o.mcs, o.rescue_distance};
-    ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
-    const uint32_t k = (uint32_t)eng.params().k;
     // device batches of whole chunks
-    size_t cb = 0;
-    std::vector<uint8_t> seq;
-    std::vector<uint64_t> off;
-    while (cb < chunks.size()) {
+    std::vector<std::pair<size_t, size_t>> batches;
+    for (size_t cb = 0; cb < chunks.size();) {
         size_t ce = cb;
         uint64_t bases = 0;
         while (ce < chunks.size() && (ce == cb || bases < o.batch_bases)) {
@@ -1129,36 +1125,58 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
                 bases += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
             ++ce;
         }
+        batches.push_back({cb, ce});
+        cb = ce;
+    }
+    const Scores sc{2, 8, 12, 1, 10, 500};
+    const MapParams mp{o.mcs, o.rescue_distance};
+    const ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+    const uint32_t k = (uint32_t)engines[0]->params().k;
+    std::vector<RunStats> bst(batches.size());
+    const bool keep = out != nullptr || o.mode == 2;
+    // One batch on one engine: reads to the device, NAMs, then replay passes and
+    // extension rounds until every chunk of the batch is committed.
+    auto run_batch = [&](Engine& eng, size_t b) {
+        const size_t cb = batches[b].first, ce = batches[b].second;
+        RunStats& st = bst[b];
         const uint64_t bfirst = chunks[cb].first;
         const uint64_t blast = chunks[ce - 1].first + chunks[ce - 1].count;
         double t0 = now_ms();
-        seq.clear();
-        off.assign(1, 0);
-        seq.reserve(bases);
-        for (uint64_t i = bfirst; i < blast; ++i) {
-            seq.insert(seq.end(), r1[i].seq.begin(), r1[i].seq.end());
-            off.push_back(seq.size());
-            if (paired) {
-                seq.insert(seq.end(), r2[i].seq.begin(), r2[i].seq.end());
+        if (dreads) {
+            // resident read set: offsets of this batch, relative to its first base
+            const uint64_t r0 = paired ? 2 * bfirst : bfirst, r1x = paired ? 2 * blast : blast;
+            std::vector<uint64_t> off(dreads->off.begin() + (ptrdiff_t)r0, dreads->off.begin() + (ptrdiff_t)r1x + 1);
+            const uint64_t base = off[0];
+            for (auto& v : off) v -= base;
+            eng.use_reads(dreads->d_seq + base, off.data(), (uint32_t)(off.size() - 1));
+        } else {
+            std::vector<uint8_t> seq;
+            std::vector<uint64_t> off(1, 0);
+            uint64_t bases = 0;
+            for (uint64_t i = bfirst; i < blast; ++i) bases += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
+            seq.reserve(bases);
+            for (uint64_t i = bfirst; i < blast; ++i) {
+                seq.insert(seq.end(), r1[i].seq.begin(), r1[i].seq.end());
                 off.push_back(seq.size());
+                if (paired) {
+                    seq.insert(seq.end(), r2[i].seq.begin(), r2[i].seq.end());
+                    off.push_back(seq.size());
+                }
             }
+            eng.upload_reads(seq.data(), off.data(), (uint32_t)(off.size() - 1));
         }
-        eng.upload_reads(seq.data(), off.data(), (uint32_t)(off.size() - 1));
         double t1 = now_ms();
         NamBatch nb;
         eng.find_nams(mp, cp, nb);
         double t2 = now_ms();
-        if (stats) {
-            stats->upload_ms += t1 - t0;
-            stats->nam_ms += t2 - t1;
-            stats->nams += nb.nams.size();
-        }
+        st.upload_ms += t1 - t0;
+        st.nam_ms += t2 - t1;
+        st.nams += nb.nams.size();
         Ctx base{o, ref, nb, r1, r2, paired, bfirst, k, nullptr, false, false, nullptr, nullptr};
         for (size_t c = cb; c < ce; ++c) {
             chunks[c].abundances.assign(ref.names.size(), 0.0);
             chunks[c].out.reserve(chunks[c].count * (paired ? 1400 : 700));
         }
-        // replay rounds
         for (int round = 0;; ++round) {
             double h0 = now_ms();
             parallel_for(o.threads, ce - cb, [&](size_t x) {
@@ -1166,7 +1184,7 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
                 if (!ch.done() || round == 0) run_chunk(base, ch);
             });
             double h1 = now_ms();
-            if (stats) stats->host_ms += h1 - h0;
+            st.host_ms += h1 - h0;
             // requests of all chunks, chunk by chunk (built in parallel)
             std::vector<uint64_t> roff(ce - cb + 1, 0);
             for (size_t cix = cb; cix < ce; ++cix) roff[cix - cb + 1] = roff[cix - cb] + chunks[cix].pending.size();
@@ -1198,22 +1216,23 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
                     out_r[y] = r;
                 }
             });
-            if (stats) { stats->rounds++; stats->requests += nreq; }
+            st.rounds++;
+            st.requests += nreq;
             std::vector<ExtResult> res;
             std::vector<uint32_t> cig;
             std::vector<uint64_t> coff;
             double g0 = now_ms();
-            if (stats) stats->host_ms += g0 - h1;
+            st.host_ms += g0 - h1;
             eng.extend(req.data(), (uint32_t)nreq, sc, res, cig, coff);
             double g1 = now_ms();
-            if (stats) stats->extend_ms += g1 - g0;
+            st.extend_ms += g1 - g0;
             // results into the chunks' caches (each chunk owns its cache and CIGAR pool)
             parallel_for(o.threads, ce - cb, [&](size_t x) {
                 Chunk& ch = chunks[cb + x];
-                const uint64_t a = roff[x], b = roff[x + 1];
-                ch.cache.reserve(ch.cache.size() + (b - a));
-                ch.cigar_pool.reserve(ch.cigar_pool.size() + (coff[b] - coff[a]));
-                for (uint64_t y = a; y < b; ++y) {
+                const uint64_t a = roff[x], e = roff[x + 1];
+                ch.cache.reserve(ch.cache.size() + (e - a));
+                ch.cigar_pool.reserve(ch.cigar_pool.size() + (coff[e] - coff[a]));
+                for (uint64_t y = a; y < e; ++y) {
                     CachedResult cr;
                     cr.r = res[y];
                     cr.cigar_off = ch.cigar_pool.size();
@@ -1224,37 +1243,79 @@ void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r
                 ch.pending.clear();
                 ch.pending_set.clear();
             });
-            if (stats) stats->host_ms += now_ms() - g1;
+            st.host_ms += now_ms() - g1;
         }
-        // append the chunks' output in order (copied in parallel)
-        {
-            std::vector<uint64_t> pos(ce - cb + 1, out.size());
-            for (size_t c = cb; c < ce; ++c) pos[c - cb + 1] = pos[c - cb] + chunks[c].out.size();
-            out.resize(pos[ce - cb]);
-            parallel_for(o.threads, ce - cb, [&](size_t x) {
-                Chunk& ch = chunks[cb + x];
-                if (!ch.out.empty()) std::memcpy(&out[pos[x]], ch.out.data(), ch.out.size());
-                std::string().swap(ch.out);
-                decltype(ch.cache)().swap(ch.cache);
-                std::vector<uint32_t>().swap(ch.cigar_pool);
+        // the batch's caches are no longer needed; output text stays until it is joined
+        parallel_for(o.threads, ce - cb, [&](size_t x) {
+            Chunk& ch = chunks[cb + x];
+            decltype(ch.cache)().swap(ch.cache);
+            std::vector<uint32_t>().swap(ch.cigar_pool);
+            if (!keep) std::string().swap(ch.out);
+        });
+    };
+    // batches are dealt to the engines; each engine works on its own stream, so one
+    // batch's device stages overlap another's host replay
+    if (engines.size() == 1 || batches.size() <= 1) {
+        for (size_t b = 0; b < batches.size(); ++b) run_batch(*engines[0], b);
+    } else {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> th;
+        std::vector<std::exception_ptr> err(engines.size());
+        for (size_t e = 0; e < std::min(engines.size(), batches.size()); ++e)
+            th.emplace_back([&, e] {
+                try {
+                    for (size_t b; (b = next.fetch_add(1)) < batches.size();) run_batch(*engines[e], b);
+                } catch (...) {
+                    err[e] = std::current_exception();
+                    next.store(batches.size());
+                }
             });
-        }
-        for (size_t c = cb; c < ce; ++c)
-            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += chunks[c].abundances[i];
-        cb = ce;
+        for (auto& t : th) t.join();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
+    }
+    // output in input order (copied in parallel)
+    if (out && o.mode != 2) {
+        std::vector<uint64_t> pos(chunks.size() + 1, out->size());
+        for (size_t c = 0; c < chunks.size(); ++c) pos[c + 1] = pos[c] + chunks[c].out.size();
+        out->resize(pos[chunks.size()]);
+        parallel_for(o.threads, chunks.size(), [&](size_t x) {
+            Chunk& ch = chunks[x];
+            if (!ch.out.empty()) std::memcpy(&(*out)[pos[x]], ch.out.data(), ch.out.size());
+            std::string().swap(ch.out);
+        });
     }
     if (o.mode == 2) {
-        char buf[64];
-        for (size_t i = 0; i < ref.names.size(); ++i) {
-            snprintf(buf, sizeof buf, "\t%.6f\n", abundances[i] / (double)ref.len[i]);
-            out += ref.names[i];
-            out += buf;
+        std::vector<double> abundances(ref.names.size(), 0.0);
+        for (const Chunk& c : chunks)
+            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += c.abundances[i];
+        if (out) {
+            char buf[64];
+            for (size_t i = 0; i < ref.names.size(); ++i) {
+                snprintf(buf, sizeof buf, "\t%.6f\n", abundances[i] / (double)ref.len[i]);
+                *out += ref.names[i];
+                *out += buf;
+            }
         }
     }
     if (stats) {
+        for (const RunStats& b : bst) {
+            stats->upload_ms += b.upload_ms;
+            stats->nam_ms += b.nam_ms;
+            stats->extend_ms += b.extend_ms;
+            stats->host_ms += b.host_ms;
+            stats->requests += b.requests;
+            stats->rounds += b.rounds;
+            stats->nams += b.nams;
+        }
         stats->records += n;
         stats->total_ms += now_ms() - t_start;
     }
+}
+
+void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
+                 const Options& o, std::string& out, RunStats* stats) {
+    map_records(std::vector<Engine*>{&eng}, ref, r1, r2, o, &out, stats, nullptr);
 }
 
 }  // namespace sa

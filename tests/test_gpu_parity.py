@@ -143,6 +143,26 @@ def test_paired_end_100m_vs_oracle(sa, ctx, tmp_path_factory):
     assert got == exp
 
 
+def test_resident_reads_and_concurrent_batches(sa, ctx, tmp_path_factory):
+    """The same SAM whether reads are uploaded per batch or staged once, and
+    whether batches run on one engine or on three concurrently."""
+    say("case: 100 Mbp PE, resident reads, small batches on 1 and 3 engines")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("pe_slots"))
+    w = synth.make("pe100m", d, 3000)
+    exp = oracle("-t", "8", "--no-PG", w["ref"], w["reads1"], w["reads2"])
+    with sa.Index(ctx, w["ref"], read_length=150, threads=8) as ix:
+        reads = sa.Reads(w["reads1"], w["reads2"])
+        for slots, staged in ((1, False), (3, False), (3, True)):
+            if staged:
+                reads.upload(ix)
+            out, st = ix.map_reads(reads, sa.options(threads=8, batch_bases=200_000, slots=slots))
+            assert st.rounds >= 2
+            assert out == exp, (slots, staged)
+        reads.close()
+
+
 def test_ssw_against_reference_vectors(ctx, orc):
     """Aligner::align on the device vs the oracle, over the queries/references of
     the vectors written by the reference's ssw.c (ACGT-only references)."""
