@@ -103,30 +103,29 @@ def main():
     k1 = ix.kernel_times()
     n_pairs = len(reads)
     value = n_pairs * a.steps / wall
-    # dominant kernel over the timed steps
-    delta = {k: (k1[k][0] - k0.get(k, (0, 0, 0))[0], k1[k][1] - k0.get(k, (0, 0, 0))[1],
-                 k1[k][2] - k0.get(k, (0, 0, 0))[2]) for k in k1}
+    # dominant kernel over the timed steps: HIP-event time on the engines' streams;
+    # algorithmic bytes are counted by the engine per launch
+    z = (0, 0, 0, 0)
+    delta = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(4)) for k in k1}
     dom = max(delta, key=lambda k: delta[k][0])
-    ms, launches, items = delta[dom]
+    ms, launches, items, nbytes = delta[dom]
     per_launch_ms = ms / max(1, launches)
-    # algorithmic bytes per launch ("Roofline accounting")
+    bytes_per_launch = nbytes / max(1, launches)
+    achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
     traffic = None
-    if dom == "k_extend":
-        # per request: query bases (1 B) + packed reference window (~1.2 L / 4 B) + request and
-        # result records (64 + 40 B) + CIGAR ops (4 B each, ~3 per request)
-        per_item = 150 + (180 / 4) + 64 + 40 + 12
-    elif dom == "k_hits":
-        per_item = 26 * (3 * 16 + 2 * 8) + 26 * 24 + 26 * 56
-    elif dom == "k_chain":
-        per_item = 60 * (16 + 16 + 16) + 26 * 56
-    else:
-        per_item = 150 * 1 + 52 * 24 + 30 * 16
-    bytes_per_launch = per_item * items / max(1, launches)
-    achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        if dom in tj.get("kernels", {}) and tj.get("workload") == "pe100m":
+            traffic = tj["kernels"][dom]["hbm_bytes_per_launch_scaled"]
+    copy_gbps = ctx.hbm_copy_gbps()
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
             "avg_launch_ms": round(per_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
-            "kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in delta.items()}}
+            "measured_copy_gbps": round(copy_gbps, 1),
+            "kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in delta.items()},
+            "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in delta.items() if v[3] and v[0]}}
     cpu = None
     if not a.no_cpu and a.cpu_pairs > 0:
         try:

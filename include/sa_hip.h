@@ -189,9 +189,15 @@ int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* quer
 int sa_index_stats(const sa_index* index, double* kernel_ms);
 
 /* Per-kernel totals since the index was created, as JSON text
- * {"k_seed": [milliseconds, launches, work_items], ...}, timed with HIP events on
- * the engine's stream. Returns the JSON length (truncated to cap-1 in buf). */
+ * {"k_seed": [milliseconds, launches, work_items, algorithmic_bytes], ...}, timed
+ * with HIP events on the engines' streams (algorithmic_bytes is 0 where the kernel
+ * does not count them). Returns the JSON length (truncated to cap-1 in buf). */
 size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap);
+
+/* Device copy bandwidth: copies `bytes` between two device buffers `iters` times
+ * with a plain vector-copy kernel (k_hbm_copy) and reports read+write GB/s. The
+ * HBM reference point of the roofline figures; no counterpart in the reference. */
+int sa_hbm_copy_probe(sa_ctx* ctx, uint64_t bytes, int iters, double* gbps);
 
 void sa_free(void* p);
 

@@ -61,7 +61,7 @@ EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa
            "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
            "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
-           "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_free"]
+           "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_free"]
 
 
 class RunStats(ctypes.Structure):
@@ -172,6 +172,12 @@ class Context:
         _lib.sa_free(cig)
         return res
 
+    def hbm_copy_gbps(self, nbytes=1 << 30, iters=20):
+        """Read+write GB/s of a plain device copy (sa_hbm_copy_probe)."""
+        v = ctypes.c_double()
+        _check(_lib.sa_hbm_copy_probe(self._p, ctypes.c_uint64(nbytes), iters, ctypes.byref(v)))
+        return v.value
+
     def __enter__(self):
         return self
 
@@ -273,7 +279,7 @@ class Index:
         return res
 
     def kernel_times(self):
-        """{kernel: (total_ms, launches, items)} timed with HIP events."""
+        """{kernel: (total_ms, launches, items, algorithmic_bytes)} timed with HIP events."""
         import json
         n = _lib.sa_index_kernel_times(self._p, None, ctypes.c_size_t(0))
         buf = ctypes.create_string_buffer(n + 1)

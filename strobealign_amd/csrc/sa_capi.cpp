@@ -441,6 +441,7 @@ size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
                     a.second.ms += k.second.ms;
                     a.second.launches += k.second.launches;
                     a.second.items += k.second.items;
+                    a.second.bytes += k.second.bytes;
                     found = true;
                 }
             if (!found) all.push_back(k);
@@ -449,8 +450,9 @@ size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
     bool first = true;
     for (const auto& k : all) {
         char tmp[256];
-        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu]", first ? "" : ", ", k.first.c_str(), k.second.ms,
-                      (unsigned long long)k.second.launches, (unsigned long long)k.second.items);
+        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu, %llu]", first ? "" : ", ", k.first.c_str(),
+                      k.second.ms, (unsigned long long)k.second.launches, (unsigned long long)k.second.items,
+                      (unsigned long long)k.second.bytes);
         s += tmp;
         first = false;
     }
@@ -461,6 +463,14 @@ size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
         buf[n] = 0;
     }
     return s.size();
+}
+
+int sa_hbm_copy_probe(sa_ctx* ctx, uint64_t bytes, int iters, double* gbps) {
+    if (!ctx || !gbps || iters <= 0 || bytes < 16) return fail(SA_ERR_ARG, "bad argument");
+    return guard([&] {
+        *gbps = sa::hbm_copy_probe(ctx->device, bytes, iters);
+        return SA_OK;
+    });
 }
 
 void sa_free(void* p) { std::free(p); }

@@ -45,7 +45,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_COUNT
 };
 }  // namespace
 
@@ -55,6 +55,36 @@ void* device_upload(int device, const void* src, uint64_t bytes) {
     SA_CHECK(hipMalloc(&p, bytes ? bytes : 16));
     if (bytes) SA_CHECK(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
     return p;
+}
+
+__global__ void __launch_bounds__(256) k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+double hbm_copy_probe(int device, uint64_t bytes, int iters) {
+    SA_CHECK(hipSetDevice(device));
+    const uint64_t n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    SA_CHECK(hipMalloc(&a, n * 16 + 16));
+    SA_CHECK(hipMalloc(&b, n * 16 + 16));
+    SA_CHECK(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    SA_CHECK(hipEventCreate(&e0));
+    SA_CHECK(hipEventCreate(&e1));
+    const uint32_t grid = 256 * 8 * 4;  // 8 XCDs x 32 CUs x 4 blocks of 256 per CU
+    k_hbm_copy<<<grid, 256>>>((const uint4*)a, (uint4*)b, n);  // warm-up
+    SA_CHECK(hipEventRecord(e0, nullptr));
+    for (int i = 0; i < iters; ++i) k_hbm_copy<<<grid, 256>>>((const uint4*)a, (uint4*)b, n);
+    SA_CHECK(hipEventRecord(e1, nullptr));
+    SA_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    SA_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ms > 0 ? 2.0 * (double)n * 16 * iters / (ms * 1e-3) / 1e9 : 0.0;
 }
 
 void device_free(int device, void* p) {
@@ -100,10 +130,16 @@ Engine::~Engine() {
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-void Engine::add_kernel(const char* name, double ms, uint64_t items) {
+void Engine::add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes) {
     for (auto& k : kstats_)
-        if (k.first == name) { k.second.ms += ms; k.second.launches += 1; k.second.items += items; return; }
-    kstats_.push_back({name, KernelStat{ms, 1, items}});
+        if (k.first == name) {
+            k.second.ms += ms;
+            k.second.launches += 1;
+            k.second.items += items;
+            k.second.bytes += bytes;
+            return;
+        }
+    kstats_.push_back({name, KernelStat{ms, 1, items, bytes}});
 }
 
 void* Engine::grow(int slot, uint64_t bytes) {
@@ -247,6 +283,7 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     b.used = (uint8_t*)grow(S_USED, A + 1);
     b.nams = (NamRec*)grow(S_NAMS_IN, (A + 1) * sizeof(NamRec));
     b.chain_anchors = (uint32_t*)grow(S_CHAIN, (4 * A + 4) * 4);
+    b.alg_bytes = (uint32_t*)grow(S_CBYTES, (2ull * n + 1) * 4);
     auto* d_namn = (uint32_t*)grow(S_NAMN, (2ull * n + 1) * 4);
     auto* d_araw = (uint32_t*)grow(S_ARAW, (2ull * n + 1) * 4);
     SA_CHECK(hipEventRecord(ev[3], stream_));
@@ -272,7 +309,8 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     if (N) SA_CHECK(hipMemcpyAsync(out.nams.data(), d_nams, N * sizeof(NamRec), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(out.nam_off.data(), d_outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
     std::vector<HitCounts> hc(2ull * n);
-    std::vector<uint32_t> qrn(2ull * n), namn(2ull * n), araw(2ull * n);
+    std::vector<uint32_t> qrn(2ull * n), namn(2ull * n), araw(2ull * n), cbytes(2ull * n);
+    SA_CHECK(hipMemcpyAsync(cbytes.data(), b.alg_bytes, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(hc.data(), d_hc, 2ull * n * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(qrn.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(namn.data(), d_namn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
@@ -290,9 +328,15 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         r.both_orientations = !better_orientation(hc[2 * i], hc[2 * i + 1]) && !better_orientation(hc[2 * i + 1], hc[2 * i]);
     }
     float t;
-    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n);
+    // algorithmic bytes: read bases in, query randstrobes out; hits/entries in, NAMs out
+    uint64_t seed_bytes = n_bases_, chain_bytes = 0;
+    for (uint64_t x = 0; x < 2ull * n; ++x) {
+        seed_bytes += sizeof(QRand) * (uint64_t)qrn[x];
+        chain_bytes += cbytes[x];
+    }
+    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n, seed_bytes);
     SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n);
-    SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n);
+    SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n, chain_bytes);
     SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -353,7 +397,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     hipEvent_t e0, e1;
     SA_CHECK(hipEventCreate(&e0));
     SA_CHECK(hipEventCreate(&e1));
-    auto timed = [&](const char* name, uint64_t items, auto&& launch) {
+    auto timed = [&](const char* name, uint64_t items, auto&& launch, uint64_t bytes = 0) {
         SA_CHECK(hipEventRecord(e0, stream_));
         launch();
         SA_CHECK(hipGetLastError());
@@ -361,16 +405,24 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         SA_CHECK(hipEventSynchronize(e1));
         float t = 0;
         SA_CHECK(hipEventElapsedTime(&t, e0, e1));
-        add_kernel(name, t, items);
+        add_kernel(name, t, items, bytes);
         total_ms_ += t;
         return t;
     };
     float ext_ms = 0;
     // 1. Hamming fast path, rescue prefilter, job lists
+    // algorithmic bytes: request in, result out, the query and the 2-bit reference
+    // it is compared with (read window, or the rescue window of the prefilter)
+    uint64_t prep_bytes = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t Lq = h_off_[req[i].read_id + 1] - h_off_[req[i].read_id];
+        const uint64_t wl = req[i].kind == EXT_SEED ? Lq : (uint64_t)(req[i].win_end - req[i].win_start);
+        prep_bytes += sizeof(ExtRequest) + sizeof(ExtResult) + Lq + (wl + 3) / 4;
+    }
     ext_ms += timed("k_ext_prep", n, [&] {
         k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res, d_slots, d_jobs,
                                                         d_ljobs, d_pjobs, d_cnt);
-    });
+    }, prep_bytes);
     uint32_t cnt[4] = {0, 0, 0, 0};
     SA_CHECK(hipMemcpy(cnt, d_cnt, 16, hipMemcpyDeviceToHost));
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];

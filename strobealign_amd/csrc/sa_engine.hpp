@@ -28,6 +28,9 @@ struct StageTimes {
 
 // Plain device buffers outside an engine (a read set staged once for many calls).
 void* device_upload(int device, const void* src, uint64_t bytes);
+// Copies `bytes` between two device buffers `iters` times with a plain vector-copy
+// kernel; returns the achieved read+write GB/s (the HBM reference of the roofline).
+double hbm_copy_probe(int device, uint64_t bytes, int iters);
 void device_free(int device, void* p);
 
 class Engine {
@@ -65,9 +68,10 @@ public:
 
     const StageTimes& times() const { return times_; }
     // Per-kernel totals since creation: name -> (milliseconds, launches, work items).
-    struct KernelStat { double ms = 0; uint64_t launches = 0, items = 0; };
+    // bytes: algorithmic bytes (the compulsory reads and writes of the work items), 0 if not counted
+    struct KernelStat { double ms = 0; uint64_t launches = 0, items = 0, bytes = 0; };
     const std::vector<std::pair<std::string, KernelStat>>& kernel_stats() const { return kstats_; }
-    void add_kernel(const char* name, double ms, uint64_t items);
+    void add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes = 0);
     const SeedParams& params() const { return p_; }
     uint64_t n_entries() const { return n_entries_; }
     ihipStream_t* stream() const { return stream_; }

@@ -42,6 +42,7 @@ struct ChainBufs {
     uint8_t* used;
     NamRec* nams;
     uint32_t* chain_anchors;
+    uint32_t* alg_bytes;  // per read orientation: hits + index entries read, NAMs + chain anchors written
 };
 
 SA_HD void item_chain(uint32_t t, IndexView ix, ChainParams cp, const uint64_t* off,
@@ -61,17 +62,22 @@ SA_HD void item_chain(uint32_t t, IndexView ix, ChainParams cp, const uint64_t* 
     else sel = true;
     nam_n[t] = 0;
     anchors_raw[t] = 0;
+    b.alg_bytes[t] = 0;
     if (!sel) return;
     uint64_t ro = off[i];
     uint32_t L = (uint32_t)(off[i + 1] - ro);
     uint64_t base = 2 * ro + (uint64_t)o * L;
     uint64_t ab = aoff[t];
     ChainScratch s{b.anchors + ab, b.dp + ab, b.pred + ab, b.cand + ab, b.used + ab, (uint32_t)acap[t]};
-    uint32_t na = hits_to_anchors(ix, hits + base, hc[t].n_hits, s.anchors);
+    uint32_t entries = 0, n_chain = 0;
+    uint32_t na = hits_to_anchors(ix, hits + base, hc[t].n_hits, s.anchors, &entries);
     anchors_raw[t] = na;
     na = sort_dedup_anchors(s.anchors, na);
-    nam_n[t] = chain_orientation(s, na, (uint32_t)ix.p.k, L, o == 1, cp, log2tab, same_diag, b.nams + ab,
-                                 b.chain_anchors + 2 * (2 * ab), 2 * ab);
+    const uint32_t nn = chain_orientation(s, na, (uint32_t)ix.p.k, L, o == 1, cp, log2tab, same_diag, b.nams + ab,
+                                          b.chain_anchors + 2 * (2 * ab), 2 * ab, &n_chain);
+    nam_n[t] = nn;
+    b.alg_bytes[t] = (uint32_t)sizeof(Hit) * hc[t].n_hits + (uint32_t)sizeof(RefEntry) * entries +
+                     (uint32_t)sizeof(NamRec) * nn + 8u * n_chain;
 }
 
 // Merge both orientations, number NAMs like src/chainer.rs:435-436, stable-sort by

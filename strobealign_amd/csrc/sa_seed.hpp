@@ -435,34 +435,49 @@ SA_HD void heap_sort(T* a, uint32_t n, Less less) {
     }
 }
 
-// Hits -> anchors (src/chainer.rs:304-384); returns count.
-SA_HD uint32_t hits_to_anchors(const IndexView& ix, const Hit* hits, uint32_t nh, Anchor* a) {
+// Anchors of one hit (src/chainer.rs:304-384): a full hit walks its run of
+// equal hashes and keeps the entries whose span length differs from the query
+// span by no more than the smallest difference so far (in table order), two
+// anchors each; a partial hit gives one anchor per entry. Writes to `a` unless it
+// is null; returns the number of anchors and adds the entries read to *entries.
+SA_HD uint32_t hit_anchors(const IndexView& ix, const Hit& h, Anchor* a, uint32_t* entries) {
+    if (h.run_start == NONE64) return 0;
     const uint32_t k = (uint32_t)ix.p.k;
+    *entries += h.run_len;
     uint32_t n = 0;
-    for (uint32_t i = 0; i < nh; ++i) {
-        const Hit& h = hits[i];
-        if (h.run_start == NONE64) continue;
-        if (h.flags & HIT_PARTIAL) {
-            for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-                const RefEntry& e = ix.e[j];
-                a[n++] = Anchor{((uint64_t)e.ref_index << 32) | e.position, h.query_start, 0};
-            }
-        } else {
-            uint32_t min_diff = 0xFFFFFFFFu;
-            uint32_t ql = h.query_end - h.query_start;
-            for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-                const RefEntry& e = ix.e[j];
-                uint32_t rs = e.position, re = rs + (uint32_t)(e.hash_offset & 0xFF) + k;
-                uint32_t rl = re - rs;
-                uint32_t diff = ql > rl ? ql - rl : rl - ql;
-                if (diff <= min_diff) {
-                    a[n++] = Anchor{((uint64_t)e.ref_index << 32) | rs, h.query_start, 0};
-                    a[n++] = Anchor{((uint64_t)e.ref_index << 32) | (re - k), h.query_end - k, 0};
-                    min_diff = diff;
+    if (h.flags & HIT_PARTIAL) {
+        for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
+            const RefEntry& e = ix.e[j];
+            if (a) a[n] = Anchor{((uint64_t)e.ref_index << 32) | e.position, h.query_start, 0};
+            ++n;
+        }
+    } else {
+        uint32_t min_diff = 0xFFFFFFFFu;
+        const uint32_t ql = h.query_end - h.query_start;
+        for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
+            const RefEntry& e = ix.e[j];
+            const uint32_t rs = e.position, re = rs + (uint32_t)(e.hash_offset & 0xFF) + k;
+            const uint32_t rl = re - rs;
+            const uint32_t diff = ql > rl ? ql - rl : rl - ql;
+            if (diff <= min_diff) {
+                if (a) {
+                    a[n] = Anchor{((uint64_t)e.ref_index << 32) | rs, h.query_start, 0};
+                    a[n + 1] = Anchor{((uint64_t)e.ref_index << 32) | (re - k), h.query_end - k, 0};
                 }
+                n += 2;
+                min_diff = diff;
             }
         }
     }
+    return n;
+}
+
+// Hits -> anchors for one read orientation; returns the count.
+SA_HD uint32_t hits_to_anchors(const IndexView& ix, const Hit* hits, uint32_t nh, Anchor* a,
+                                uint32_t* entries_read = nullptr) {
+    uint32_t n = 0, entries = 0;
+    for (uint32_t i = 0; i < nh; ++i) n += hit_anchors(ix, hits[i], a + n, &entries);
+    if (entries_read) *entries_read = entries;
     return n;
 }
 
@@ -492,7 +507,8 @@ struct ChainScratch {
 // orientation) and their anchors (last first). Returns the number of NAMs.
 SA_HD uint32_t chain_orientation(const ChainScratch& s, uint32_t n, uint32_t k, uint32_t read_len, bool is_revcomp,
                                  const ChainParams& cp, const float* log2tab, const float* same_diag, NamRec* nams,
-                                 uint32_t* chain_anchors, uint64_t chain_anchor_base) {
+                                 uint32_t* chain_anchors, uint64_t chain_anchor_base, uint32_t* n_chain_anchors = nullptr) {
+    if (n_chain_anchors) *n_chain_anchors = 0;
     if (n == 0) return 0;
     Anchor* A = s.anchors;
     uint64_t max_ref_gap = cp.max_ref_gap >= 0 ? (uint64_t)cp.max_ref_gap : read_len;
@@ -590,6 +606,7 @@ SA_HD uint32_t chain_orientation(const ChainScratch& s, uint32_t n, uint32_t k, 
         r.anchor_off = chain_anchor_base + start;
         nams[nn++] = r;
     }
+    if (n_chain_anchors) *n_chain_anchors = (uint32_t)ca;
     return nn;
 }
 
