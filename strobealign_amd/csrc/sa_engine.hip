@@ -222,7 +222,7 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
     const uint32_t n = n_reads_;
     auto* d_seq = d_seq_;
     auto* d_off = (const uint64_t*)buf_[S_OFF];
-    auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
+    auto* d_sy = (Syncmer*)grow(S_SY, (p_.w_max < 24 ? 1 : n_bases_ + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
     auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
     if (n) k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
@@ -258,7 +258,8 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
     auto* d_seq = d_seq_;
     auto* d_off = (const uint64_t*)buf_[S_OFF];
-    auto* d_sy = (Syncmer*)grow(S_SY, (n_bases_ + 1) * sizeof(Syncmer));
+    // the syncmer list is only materialised by the fallback for w_max >= 24
+    auto* d_sy = (Syncmer*)grow(S_SY, (p_.w_max < 24 ? 1 : n_bases_ + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
     auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
     auto* d_hits = (Hit*)grow(S_HITS, (2 * n_bases_ + 1) * sizeof(Hit));

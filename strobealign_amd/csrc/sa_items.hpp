@@ -13,7 +13,10 @@ SA_HD void item_seed(uint32_t i, const uint8_t* seq, const uint64_t* off,
     if (i >= n) return;
     uint64_t o = off[i];
     uint32_t L = (uint32_t)(off[i + 1] - o);
-    read_randstrobes(seq + o, L, p, sy + o, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1]);
+    if (p.w_max < 24)
+        read_randstrobes_stream<24>(seq + o, L, p, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1]);
+    else
+        read_randstrobes(seq + o, L, p, sy + o, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1]);
 }
 
 // ------------------------------------------------------------------ hits ---

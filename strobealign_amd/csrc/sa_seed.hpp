@@ -105,6 +105,53 @@ SA_HD void syncmer_scan(uint32_t len, const SeedParams& p, Code code, Emit emit)
     }
 }
 
+// syncmer_scan with the window length W = k-s+1 fixed at compile time, so the
+// window lives in registers (a shift register) instead of an indexed ring.
+template <int W, typename Code, typename Emit>
+SA_HD void syncmer_scan_w(uint32_t len, const SeedParams& p, Code code, Emit emit) {
+    const int k = p.k, s = p.s;
+    const uint32_t tm1 = (uint32_t)p.t - 1;
+    const uint64_t kmask = (k == 32) ? ~0ULL : ((1ULL << (2 * k)) - 1);
+    const uint64_t smask = (1ULL << (2 * s)) - 1;
+    const int kshift = (k - 1) * 2, sshift = (s - 1) * 2;
+    uint64_t hv[W];  // s-mer hashes of the window, oldest first
+SA_UNROLL
+    for (int j = 0; j < W; ++j) hv[j] = 0;
+    uint64_t xk0 = 0, xk1 = 0, xs0 = 0, xs1 = 0;
+    uint32_t l = 0, nq = 0;
+    for (uint32_t i = 0; i < len; ++i) {
+        const uint32_t c = code(i);
+        if (c < 4) {
+            xk0 = ((xk0 << 2) | c) & kmask;
+            xk1 = (xk1 >> 2) | ((uint64_t)(3 - c) << kshift);
+            xs0 = ((xs0 << 2) | c) & smask;
+            xs1 = (xs1 >> 2) | ((uint64_t)(3 - c) << sshift);
+            ++l;
+            if (l < (uint32_t)s) continue;
+            const uint64_t hs = xxh64(xs0 < xs1 ? xs0 : xs1);
+SA_UNROLL
+            for (int j = 0; j + 1 < W; ++j) hv[j] = hv[j + 1];
+            hv[W - 1] = hs;
+            if (nq < (uint32_t)W) ++nq;
+            if (nq < (uint32_t)W) continue;
+            uint64_t mn = hv[0], at = hv[0];
+SA_UNROLL
+            for (int j = 1; j < W; ++j) {
+                mn = hv[j] < mn ? hv[j] : mn;
+                at = (uint32_t)j == tm1 ? hv[j] : at;
+            }
+            if (at == mn) {
+                const uint64_t yk = xk0 < xk1 ? xk0 : xk1;
+                emit(xxh64(yk), xk0 <= xk1, i + 1 - (uint32_t)k);
+            }
+        } else {
+            l = 0;
+            nq = 0;
+            xk0 = xk1 = xs0 = xs1 = 0;
+        }
+    }
+}
+
 // src/seeding/strobes.rs:112-124 (bit 8, the strobe-2 orientation, is masked off)
 SA_HD uint64_t randstrobe_hash(uint64_t h1, uint64_t h2, uint64_t f1, uint64_t f2, const SeedParams& p) {
     return ((h1 & p.main_mask) | (h2 & ~p.fwd_main_mask) | (f1 << p.partial_pos) | (f2 << 8)) & (REF_HASH_MASK << 1);
@@ -132,6 +179,118 @@ SA_HD uint32_t pick_strobe2(uint32_t a, uint32_t n, const SeedParams& p, At at) 
         if (ones < min_val) { min_val = ones; best = a + i; }
     }
     return best;
+}
+
+// Query randstrobes of one read without a syncmer list (same output as
+// read_randstrobes): the last RING syncmers are kept in a register shift
+// register (slot j = the j-th most recent). When syncmer m arrives, the forward
+// randstrobe of syncmer m - w_max is complete (its candidates are slots
+// w_max - i), and the reverse-complement randstrobe whose first strobe is m can
+// be formed at once (its candidates are the earlier syncmers m - i, slot i;
+// src/seeding/mod.rs:53-57 reverses the list). After the last syncmer, w_max
+// sentinel steps (position "infinitely far", which ends a candidate scan like the
+// end of the list) flush the forward randstrobes of the tail. Reverse-complement
+// records are produced last first and reversed in place at the end.
+template <int RING>
+SA_HD void read_randstrobes_stream(const uint8_t* seq, uint32_t len, const SeedParams& p, QRand* out0, QRand* out1,
+                                   uint32_t* n0, uint32_t* n1) {
+    *n0 = 0;
+    *n1 = 0;
+    if (len < (uint32_t)p.w_max) return;
+    const uint32_t k = (uint32_t)p.k;
+    const uint32_t w_min = (uint32_t)p.w_min, w_max = (uint32_t)p.w_max, max_dist = (uint32_t)p.max_dist;
+    uint64_t rh[RING];
+    uint32_t rp[RING];  // position << 1 | is_forward; 0xFFFFFFFF = sentinel
+SA_UNROLL
+    for (int j = 0; j < RING; ++j) {
+        rh[j] = 0;
+        rp[j] = 0xFFFFFFFFu;
+    }
+    uint32_t m = 0, nf = 0;
+    auto fwd_strobe = [&]() {
+        // randstrobe of syncmer a = m - w_max (slot w_max): candidates a + i = slot w_max - i
+        uint64_t h1 = 0;
+        uint32_t p1 = 0;
+SA_UNROLL
+        for (int j = 0; j < RING; ++j)
+            if ((uint32_t)j == w_max) { h1 = rh[j]; p1 = rp[j]; }
+        const uint32_t max_pos = (p1 >> 1) + max_dist;
+        uint32_t best_slot = w_max, min_val = 0xFFFFFFFFu;
+        bool stop = false;
+SA_UNROLL
+        for (int j = RING - 1; j >= 0; --j) {
+            // slot j holds candidate i = w_max - j
+            if ((uint32_t)j < w_max && w_max - (uint32_t)j >= w_min && !stop) {
+                if (rp[j] == 0xFFFFFFFFu || (rp[j] >> 1) > max_pos) {
+                    stop = true;
+                } else {
+                    const uint32_t ones = (uint32_t)__builtin_popcountll((h1 ^ rh[j]) & p.q);
+                    if (ones < min_val) { min_val = ones; best_slot = (uint32_t)j; }
+                }
+            }
+        }
+        uint64_t h2 = 0;
+        uint32_t p2 = 0;
+SA_UNROLL
+        for (int j = 0; j < RING; ++j)
+            if ((uint32_t)j == best_slot) { h2 = rh[j]; p2 = rp[j]; }
+        const uint64_t f1 = p1 & 1, f2 = p2 & 1;
+        out0[nf++] = QRand{randstrobe_hash(h1, h2, f1, f2, p), randstrobe_hash(h2, h1, f1 ^ 1, f2 ^ 1, p), p1 >> 1,
+                           (p2 >> 1) + k};
+    };
+    auto push = [&](uint64_t h, uint32_t pf) {
+SA_UNROLL
+        for (int j = RING - 1; j > 0; --j) {
+            rh[j] = rh[j - 1];
+            rp[j] = rp[j - 1];
+        }
+        rh[0] = h;
+        rp[0] = pf;
+    };
+    auto on_syncmer = [&](uint64_t h, bool fwd, uint32_t pos) {
+        push(h, (pos << 1) | (fwd ? 1u : 0u));
+        // reverse-complement randstrobe with first strobe m: candidates m - i = slot i
+        const uint32_t max_gap = max_dist;  // pos(m) - pos(m - i) > max_dist ends the scan
+        uint32_t best_slot = 0, min_val = 0xFFFFFFFFu;
+        bool stop = false;
+SA_UNROLL
+        for (int j = 1; j < RING; ++j) {
+            if ((uint32_t)j >= w_min && (uint32_t)j <= w_max && (uint32_t)j <= m && !stop) {
+                if (pos - (rp[j] >> 1) > max_gap) {
+                    stop = true;
+                } else {
+                    const uint32_t ones = (uint32_t)__builtin_popcountll((h ^ rh[j]) & p.q);
+                    if (ones < min_val) { min_val = ones; best_slot = (uint32_t)j; }
+                }
+            }
+        }
+        uint64_t h2 = h;
+        uint32_t p2 = rp[0];
+SA_UNROLL
+        for (int j = 1; j < RING; ++j)
+            if ((uint32_t)j == best_slot) { h2 = rh[j]; p2 = rp[j]; }
+        const uint32_t pos1 = len - pos - k, pos2 = len - (p2 >> 1) - k;
+        const uint64_t f1 = (fwd ? 1u : 0u) ^ 1u, f2 = (p2 & 1) ^ 1u;
+        out1[m] = QRand{randstrobe_hash(h, h2, f1, f2, p), randstrobe_hash(h2, h, f1 ^ 1, f2 ^ 1, p), pos1, pos2 + k};
+        if (m >= w_max) fwd_strobe();
+        ++m;
+    };
+    if (p.k - p.s + 1 == 5) syncmer_scan_w<5>(len, p, [&](uint32_t i) { return nt2(seq[i]); }, on_syncmer);
+    else syncmer_scan(len, p, [&](uint32_t i) { return nt2(seq[i]); }, on_syncmer);
+    const uint32_t n = m;
+    // flush: the forward randstrobes of the last min(n, w_max) syncmers
+    for (uint32_t e = 0; e < w_max; ++e) {
+        push(0, 0xFFFFFFFFu);
+        if (m >= w_max && m - w_max < n) fwd_strobe();
+        ++m;
+    }
+    for (uint32_t x = 0; x < n / 2; ++x) {
+        const QRand t = out1[x];
+        out1[x] = out1[n - 1 - x];
+        out1[n - 1 - x] = t;
+    }
+    *n0 = n;
+    *n1 = n;
 }
 
 // Query randstrobes of one read (src/seeding/mod.rs:19-75). `sy` is scratch

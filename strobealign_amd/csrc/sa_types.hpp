@@ -7,8 +7,10 @@
 
 #if defined(__HIPCC__)
 #define SA_HD __host__ __device__ inline
+#define SA_UNROLL _Pragma("unroll")
 #else
 #define SA_HD inline
+#define SA_UNROLL
 #endif
 
 namespace sa {
