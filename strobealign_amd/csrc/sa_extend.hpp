@@ -37,18 +37,44 @@ struct CigarOut {
 };
 
 // Views of the query (read or its reverse complement) and of a reference window.
+// Both keep the last aligned 8-byte block (query) or packed 64-bit word
+// (reference) they read: consecutive accesses, the common pattern of the
+// aligners, then cost no memory access. The read buffers they point into are
+// padded by at least 8 bytes, so the aligned block read never leaves them.
 struct QueryView {
     const uint8_t* seq;
     uint32_t len;
     bool rc;
-    SA_HD uint8_t operator[](uint32_t i) const { return read_char(seq, len, rc, i); }
+    mutable uintptr_t cbi = ~(uintptr_t)0;  // address >> 3 of the cached block
+    mutable uint64_t cb = 0;
+    SA_HD uint8_t byte_at(uint32_t pos) const {
+        const uintptr_t addr = (uintptr_t)(seq + pos);
+        if ((addr >> 3) != cbi) {
+            cbi = addr >> 3;
+            cb = *(const uint64_t*)(addr & ~(uintptr_t)7);
+        }
+        return (uint8_t)(cb >> ((addr & 7) * 8));
+    }
+    SA_HD uint8_t operator[](uint32_t i) const { return rc ? comp_ascii(byte_at(len - 1 - i)) : byte_at(i); }
 };
 struct RefWin {
     const RefView* ref;
     uint32_t id;
     uint64_t start;  // absolute position of window index 0
     uint32_t len;
-    SA_HD uint8_t operator[](uint32_t i) const { return ref_char(*ref, id, start + i); }
+    mutable const uint64_t* base = nullptr;  // packed words of contig id
+    mutable uint64_t cwi = ~0ull;            // index of the cached word
+    mutable uint64_t cw = 0;
+    SA_HD uint32_t code(uint32_t i) const {
+        const uint64_t a = start + i;
+        if ((a >> 5) != cwi) {
+            if (!base) base = ref->words + ref->word_off[id];
+            cwi = a >> 5;
+            cw = base[cwi];
+        }
+        return (uint32_t)(cw >> ((a & 31) * 2)) & 3;
+    }
+    SA_HD uint8_t operator[](uint32_t i) const { return (uint8_t)((0x54474341u >> (8 * code(i))) & 0xFFu); }
 };
 
 // src/ssw.rs:8-22 translate: A/U -> 0, C 1, G 2, T 3, other 4
@@ -164,6 +190,18 @@ SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool rever
     return r;
 }
 
+// Clears n bytes with 16-byte stores where aligned.
+SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
+    uint64_t x = 0;
+    while (x < n && (((uintptr_t)(p + x)) & 15)) p[x++] = 0;
+    for (; x + 16 <= n; x += 16) {
+        uint64_t* q = (uint64_t*)(p + x);
+        q[0] = 0;
+        q[1] = 0;
+    }
+    for (; x < n; ++x) p[x] = 0;
+}
+
 // Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774).
 // Emits M/I/D ops in forward order through `emit(op, len)`. Returns 0 ok,
 // 1 traceback failure, 2 scratch too small.
@@ -188,7 +226,7 @@ SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score
         hc = eb + (width + 1);
         dir = (int8_t*)(hc + (width + 1));
         for (int32_t x = 0; x <= width; ++x) hb[x] = eb[x] = hc[x] = 0;
-        for (uint64_t x = 0; x < (uint64_t)width_d * readLen * 3 + 3; ++x) dir[x] = 0;
+        zero_bytes((uint8_t*)dir, (uint64_t)width_d * readLen * 3 + 3);
         for (int32_t i = 0; i < readLen; ++i) {
             int32_t beg = i - bw > 0 ? i - bw : 0, end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
             int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;

@@ -279,10 +279,33 @@ void put_u(std::string& s, uint64_t v) {
     do { b[n++] = (char)('0' + v % 10); v /= 10; } while (v);
     while (n) s += b[--n];
 }
-void put_i(std::string& s, int64_t v) {
-    if (v < 0) { s += '-'; put_u(s, (uint64_t)(-v)); }
-    else put_u(s, (uint64_t)v);
-}
+
+// Raw output cursor for SAM records: the caller reserves enough room.
+struct Out {
+    char* p;
+    void str(const std::string& x) { std::memcpy(p, x.data(), x.size()); p += x.size(); }
+    void chr(char c) { *p++ = c; }
+    void lit(const char* x, size_t n) { std::memcpy(p, x, n); p += n; }
+    void u(uint64_t v) {
+        char b[24];
+        int n = 0;
+        do { b[n++] = (char)('0' + v % 10); v /= 10; } while (v);
+        while (n) *p++ = b[--n];
+    }
+    void i(int64_t v) {
+        if (v < 0) { *p++ = '-'; u((uint64_t)(-v)); }
+        else u((uint64_t)v);
+    }
+};
+
+// comp_ascii as a table
+struct CompTable {
+    char t[256];
+    CompTable() {
+        for (int c = 0; c < 256; ++c) t[c] = (char)comp_ascii((uint8_t)c);
+    }
+};
+const CompTable kComp;
 
 // SAM record fields (src/io/sam.rs:21-43)
 struct Sam {
@@ -302,60 +325,75 @@ struct Sam {
     bool is_mapped() const { return (flags & UNMAP) == 0; }
 };
 
-void write_cigar(std::string& s, const Alignment& a, bool eqx) {
+void write_cigar(Out& s, const Alignment& a, bool eqx) {
     static const char L[] = "MIDNSHP=X";
     uint32_t pend_op = 99, pend_len = 0;
     auto push = [&](uint32_t op, uint32_t len) {
         if (len == 0) return;
         if (!eqx && (op == 7 || op == 8)) op = 0;
         if (op == pend_op) { pend_len += len; return; }
-        if (pend_len) { put_u(s, pend_len); s += L[pend_op]; }
+        if (pend_len) { s.u(pend_len); s.chr(L[pend_op]); }
         pend_op = op;
         pend_len = len;
     };
     push(4, a.soft_clip_left);
     for (uint32_t i = 0; i < a.n_cigar; ++i) push(a.cigar[i] & 0xF, a.cigar[i] >> 4);
     push(4, a.soft_clip_right);
-    if (pend_len) { put_u(s, pend_len); s += L[pend_op]; }
+    if (pend_len) { s.u(pend_len); s.chr(L[pend_op]); }
 }
 
 // src/io/sam.rs:53-138
-void write_sam(std::string& s, const Sam& r, const Ctx& c) {
-    s += r.rec->name; s += '\t';
-    put_u(s, r.flags); s += '\t';
-    if (r.rname >= 0) s += c.ref.names[r.rname]; else s += '*';
-    s += '\t';
-    put_u(s, r.pos >= 0 ? (uint64_t)r.pos + 1 : 0); s += '\t';
-    put_u(s, r.mapq); s += '\t';
-    if (r.al) write_cigar(s, *r.al, c.o.eqx); else s += '*';
-    s += '\t';
-    if (r.rnext_eq) s += '=';
-    else if (r.rnext >= 0) s += c.ref.names[r.rnext];
-    else s += '*';
-    s += '\t';
-    put_u(s, r.mate_pos >= 0 ? (uint64_t)r.mate_pos + 1 : 0); s += '\t';
-    put_i(s, r.has_tlen ? r.tlen : 0); s += '\t';
-    bool sec = (r.flags & SECONDARY) != 0;
+void write_sam(std::string& out, const Sam& r, const Ctx& c) {
     const std::string& seq = r.rec->seq;
-    if (sec || seq.empty()) s += '*';
-    else if (r.revseq) for (size_t i = seq.size(); i-- > 0;) s += (char)comp_ascii((uint8_t)seq[i]);
-    else s += seq;
-    s += '\t';
     const std::string& q = r.rec->qual;
-    if (sec || !r.rec->has_qual || q.empty()) s += '*';
-    else if (r.revseq) s.append(q.rbegin(), q.rend());
-    else s += q;
-    if (r.al) { s += "\tNM:i:"; put_u(s, r.al->edit_distance); s += "\tAS:i:"; put_u(s, r.al->score); }
+    // upper bound of the record's length
+    size_t bound = r.rec->name.size() + seq.size() + q.size() + r.rec->comment.size() + c.o.rg_id.size() + 400 +
+                   (r.al ? 12 * (r.al->n_cigar + 2) : 0);
+    if (r.rname >= 0) bound += c.ref.names[(size_t)r.rname].size();
+    if (r.rnext >= 0) bound += c.ref.names[(size_t)r.rnext].size();
+    const size_t old = out.size();
+    out.resize(old + bound);
+    Out s{&out[old]};
+    s.str(r.rec->name); s.chr('\t');
+    s.u(r.flags); s.chr('\t');
+    if (r.rname >= 0) s.str(c.ref.names[(size_t)r.rname]); else s.chr('*');
+    s.chr('\t');
+    s.u(r.pos >= 0 ? (uint64_t)r.pos + 1 : 0); s.chr('\t');
+    s.u(r.mapq); s.chr('\t');
+    if (r.al) write_cigar(s, *r.al, c.o.eqx); else s.chr('*');
+    s.chr('\t');
+    if (r.rnext_eq) s.chr('=');
+    else if (r.rnext >= 0) s.str(c.ref.names[(size_t)r.rnext]);
+    else s.chr('*');
+    s.chr('\t');
+    s.u(r.mate_pos >= 0 ? (uint64_t)r.mate_pos + 1 : 0); s.chr('\t');
+    s.i(r.has_tlen ? r.tlen : 0); s.chr('\t');
+    const bool sec = (r.flags & SECONDARY) != 0;
+    if (sec || seq.empty()) s.chr('*');
+    else if (r.revseq) {
+        const size_t n = seq.size();
+        for (size_t x = 0; x < n; ++x) s.p[x] = kComp.t[(uint8_t)seq[n - 1 - x]];
+        s.p += n;
+    } else s.str(seq);
+    s.chr('\t');
+    if (sec || !r.rec->has_qual || q.empty()) s.chr('*');
+    else if (r.revseq) {
+        const size_t n = q.size();
+        for (size_t x = 0; x < n; ++x) s.p[x] = q[n - 1 - x];
+        s.p += n;
+    } else s.str(q);
+    if (r.al) { s.lit("\tNM:i:", 6); s.u(r.al->edit_distance); s.lit("\tAS:i:", 6); s.u(r.al->score); }
     if (c.o.details && r.det) {
-        s += "\tna:i:"; put_u(s, r.det->nam.n_nams);
-        s += "\tal:i:"; put_u(s, r.det->tried_alignment);
-        s += "\tga:i:"; put_u(s, r.det->gapped);
-        s += "\tX0:i:"; put_u(s, r.det->best_alignments);
-        if (r.flags & PAIRED) { s += "\tmr:i:"; put_u(s, r.det->mate_rescue); }
+        s.lit("\tna:i:", 6); s.u(r.det->nam.n_nams);
+        s.lit("\tal:i:", 6); s.u(r.det->tried_alignment);
+        s.lit("\tga:i:", 6); s.u(r.det->gapped);
+        s.lit("\tX0:i:", 6); s.u(r.det->best_alignments);
+        if (r.flags & PAIRED) { s.lit("\tmr:i:", 6); s.u(r.det->mate_rescue); }
     }
-    if (c.o.has_rg) { s += "\tRG:Z:"; s += c.o.rg_id; }
-    if (c.o.fastq_comments && r.rec->has_comment) { s += '\t'; s += r.rec->comment; }
-    s += '\n';
+    if (c.o.has_rg) { s.lit("\tRG:Z:", 6); s.str(c.o.rg_id); }
+    if (c.o.fastq_comments && r.rec->has_comment) { s.chr('\t'); s.str(r.rec->comment); }
+    s.chr('\n');
+    out.resize((size_t)(s.p - out.data()));
 }
 
 // src/mapper.rs:115-243

@@ -790,7 +790,9 @@ struct RefView {
 SA_HD uint32_t ref_bits(const RefView& r, uint32_t id, uint64_t i) {
     return (uint32_t)(r.words[r.word_off[id] + (i >> 5)] >> ((i & 31) * 2)) & 3;
 }
-SA_HD uint8_t ref_char(const RefView& r, uint32_t id, uint64_t i) { return (uint8_t)"ACGT"[ref_bits(r, id, i)]; }
+SA_HD uint8_t ref_char(const RefView& r, uint32_t id, uint64_t i) {
+    return (uint8_t)((0x54474341u >> (8 * ref_bits(r, id, i))) & 0xFFu);  // "ACGT"
+}
 
 // Read character i of the read or of its reverse complement.
 SA_HD uint8_t read_char(const uint8_t* seq, uint32_t len, bool rc, uint32_t i) {
