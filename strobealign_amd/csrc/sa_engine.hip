@@ -98,7 +98,10 @@ Engine::Engine(int device) : device_(device) {
     SA_CHECK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw std::runtime_error("no HIP device " + std::to_string(device));
     SA_CHECK(hipSetDevice(device));
-    SA_CHECK(hipStreamCreate(&stream_));
+    // non-blocking: no implicit ordering with the null stream or other engines' streams
+    SA_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    // waits on this event sleep instead of spinning, leaving the host cores to the replay
+    SA_CHECK(hipEventCreateWithFlags(&sync_ev_, hipEventBlockingSync | hipEventDisableTiming));
     buf_.assign(S_COUNT, nullptr);
     cap_.assign(S_COUNT, 0);
 }
@@ -127,7 +130,13 @@ Engine::~Engine() {
                         (void*)d_log2_})
             if (p) (void)hipFree(p);
     if (d_same_diag_) (void)hipFree(d_same_diag_);
+    if (sync_ev_) (void)hipEventDestroy(sync_ev_);
     if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::sync() {
+    SA_CHECK(hipEventRecord(sync_ev_, stream_));
+    SA_CHECK(hipEventSynchronize(sync_ev_));
 }
 
 void Engine::add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes) {
@@ -162,7 +171,7 @@ void Engine::scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* t
     uint64_t last[2];
     SA_CHECK(hipMemcpyAsync(&last[0], in + n - 1, 8, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(&last[1], out + n - 1, 8, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
     *total = last[0] + last[1];
 }
 
@@ -215,7 +224,7 @@ void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_rea
     d_seq_ = d_seq;
     void* o = grow(S_OFF, (uint64_t)(n_reads + 1) * 8);
     SA_CHECK(hipMemcpyAsync(o, off, (uint64_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
 }
 
 void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) {
@@ -231,7 +240,7 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
     counts.resize(2ull * n);
     if (n_bases_) SA_CHECK(hipMemcpyAsync(qr.data(), d_qr, 2 * n_bases_ * sizeof(QRand), hipMemcpyDeviceToHost, stream_));
     if (n) SA_CHECK(hipMemcpyAsync(counts.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
 }
 
 void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
@@ -248,12 +257,13 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         for (size_t d = 0; d < lg.size(); ++d) lg[d] = std::log2((float)d + 1.0f);
         std::vector<float> sd(1024);
         for (uint32_t i = 0; i < 1024; ++i) sd[i] = chain_score(i, i, k, cp, lg.data());
-        SA_CHECK(hipMemcpy(d_same_diag_, sd.data(), sd.size() * 4, hipMemcpyHostToDevice));
+        SA_CHECK(hipMemcpyAsync(d_same_diag_, sd.data(), sd.size() * 4, hipMemcpyHostToDevice, stream_));
+        sync();
         tab_k_ = k;
         tab_cp_ = cp;
     }
     hipEvent_t ev[6];
-    for (auto& e : ev) SA_CHECK(hipEventCreate(&e));
+    for (auto& e : ev) SA_CHECK(hipEventCreateWithFlags(&e, hipEventBlockingSync));
     IndexView ix{d_entries_, d_buckets_, p_};
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
     auto* d_seq = d_seq_;
@@ -299,8 +309,8 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     scan_u64(d_tot, d_outoff, n, &N);
     auto* d_nams = (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec));
     hipEvent_t fe0, fe1;
-    SA_CHECK(hipEventCreate(&fe0));
-    SA_CHECK(hipEventCreate(&fe1));
+    SA_CHECK(hipEventCreateWithFlags(&fe0, hipEventBlockingSync));
+    SA_CHECK(hipEventCreateWithFlags(&fe1, hipEventBlockingSync));
     SA_CHECK(hipEventRecord(fe0, stream_));
     k_finalize<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, ref, k, d_namn, d_aoff, b.nams, d_outoff, d_nams);
     SA_CHECK(hipGetLastError());
@@ -316,7 +326,7 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     SA_CHECK(hipMemcpyAsync(qrn.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(namn.data(), d_namn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(araw.data(), d_araw, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
     out.nam_off[n] = N;
     out.info.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
@@ -396,8 +406,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     const uint32_t k = (uint32_t)p_.k;
     const uint64_t L = max_len_;
     hipEvent_t e0, e1;
-    SA_CHECK(hipEventCreate(&e0));
-    SA_CHECK(hipEventCreate(&e1));
+    SA_CHECK(hipEventCreateWithFlags(&e0, hipEventBlockingSync));
+    SA_CHECK(hipEventCreateWithFlags(&e1, hipEventBlockingSync));
     auto timed = [&](const char* name, uint64_t items, auto&& launch, uint64_t bytes = 0) {
         SA_CHECK(hipEventRecord(e0, stream_));
         launch();
@@ -425,7 +435,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                                                         d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
     uint32_t cnt[4] = {0, 0, 0, 0};
-    SA_CHECK(hipMemcpy(cnt, d_cnt, 16, hipMemcpyDeviceToHost));
+    SA_CHECK(hipMemcpyAsync(cnt, d_cnt, 16, hipMemcpyDeviceToHost, stream_));
+    sync();
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
     // thread-per-job kernels run in waves sized to the scratch budget
     const uint64_t budget = 16ull << 30;
@@ -447,7 +458,10 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         k_ssw_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, nssw, d_lists, d_bcnt, cap);
         SA_CHECK(hipGetLastError());
         uint32_t bc[SSW_BUCKETS];
-        auto read_counts = [&] { SA_CHECK(hipMemcpy(bc, d_bcnt, sizeof bc, hipMemcpyDeviceToHost)); };
+        auto read_counts = [&] {
+            SA_CHECK(hipMemcpyAsync(bc, d_bcnt, sizeof bc, hipMemcpyDeviceToHost, stream_));
+            sync();
+        };
         // launches every non-empty bucket of one (mode, pass) family
         auto run = [&](bool word, bool rev) {
             for (uint32_t seg = 1; seg <= (word ? 32u : 16u); ++seg) {
@@ -509,7 +523,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     }
     times_.extend = ext_ms;
     SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // requests that ran out of scratch are re-run one at a time with a large slot
@@ -524,7 +538,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
             SA_CHECK(hipGetLastError());
         }
         SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
-        SA_CHECK(hipStreamSynchronize(stream_));
+        sync();
     }
     for (uint32_t i = 0; i < n; ++i)
         if (res[i].status != 0) throw std::runtime_error("extension request failed (status)");
@@ -540,7 +554,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     cigars.resize(total);
     if (total) SA_CHECK(hipMemcpyAsync(cigars.data(), d_cout, total * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipStreamSynchronize(stream_));
+    sync();
     cigar_off[n] = total;
 }
 

@@ -9,6 +9,7 @@
 #include "sa_types.hpp"
 
 struct ihipStream_t;
+struct ihipEvent_t;
 
 namespace sa {
 
@@ -81,10 +82,12 @@ public:
 
 private:
     void* grow(int slot, uint64_t bytes);
+    void sync();  // waits for the engine's stream (blocking wait)
     void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total);
 
     int device_;
     ihipStream_t* stream_ = nullptr;
+    ihipEvent_t* sync_ev_ = nullptr;
     SeedParams p_{};
     uint64_t n_entries_ = 0;
     RefEntry* d_entries_ = nullptr;
