@@ -131,8 +131,22 @@ Engine::~Engine() {
             if (p) (void)hipFree(p);
     if (d_same_diag_) (void)hipFree(d_same_diag_);
     if (sync_ev_) (void)hipEventDestroy(sync_ev_);
+    for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
+
+hipEvent_t Engine::event() {
+    if (!event_pool_.empty()) {
+        hipEvent_t e = event_pool_.back();
+        event_pool_.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    SA_CHECK(hipEventCreate(&e));
+    return e;
+}
+
+void Engine::release_event(hipEvent_t e) { event_pool_.push_back(e); }
 
 void Engine::sync() {
     SA_CHECK(hipEventRecord(sync_ev_, stream_));
@@ -383,12 +397,16 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     if (n == 0) return;
     // CIGAR slots: at most 2*L+1 ops for a read of length L
     std::vector<ExtRequest> req(req_in, req_in + n);
-    uint64_t slots = 0;
+    uint64_t slots = 0, prep_bytes = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        uint32_t L = (uint32_t)(h_off_[req[i].read_id + 1] - h_off_[req[i].read_id]);
-        req[i].cigar_cap = 2 * L + 8;
+        const uint64_t Lq = h_off_[req[i].read_id + 1] - h_off_[req[i].read_id];
+        req[i].cigar_cap = 2 * (uint32_t)Lq + 8;
         req[i].cigar_off = slots;
         slots += req[i].cigar_cap;
+        // algorithmic bytes of the prep kernel: request in, result out, the query and
+        // the 2-bit reference it is compared with (read window, or the rescue window)
+        const uint64_t wl = req[i].kind == EXT_SEED ? Lq : (uint64_t)(req[i].win_end - req[i].win_start);
+        prep_bytes += sizeof(ExtRequest) + sizeof(ExtResult) + Lq + (wl + 3) / 4;
     }
     auto* d_req = (ExtRequest*)grow(S_REQ, (uint64_t)n * sizeof(ExtRequest));
     auto* d_res = (ExtResult*)grow(S_RES, (uint64_t)n * sizeof(ExtResult));
@@ -396,48 +414,48 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_jobs = (ExtJob*)grow(S_JOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_ljobs = (ExtJob*)grow(S_LJOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
-    auto* d_cnt = (uint32_t*)grow(S_XCNT, 16);
+    // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets
+    auto* d_cnt = (uint32_t*)grow(S_XCNT, (4 + SSW_BUCKETS) * 4);
+    uint32_t* d_bcnt = d_cnt + 4;
+    // SSW bucket lists sized for every request (the job count is known only on the device)
+    const uint32_t cap = n;
+    auto* d_lists = (uint32_t*)grow(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
+    auto* d_ends = (SswEnds*)grow(S_ENDS, (uint64_t)n * sizeof(SswEnds));
+    auto* d_fwd = (SswEnd*)grow(S_FWD, (uint64_t)n * sizeof(SswEnd));
     SA_CHECK(hipMemcpyAsync(d_req, req.data(), (uint64_t)n * sizeof(ExtRequest), hipMemcpyHostToDevice, stream_));
-    SA_CHECK(hipMemsetAsync(d_cnt, 0, 16, stream_));
+    SA_CHECK(hipMemsetAsync(d_cnt, 0, (4 + SSW_BUCKETS) * 4, stream_));
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
     const auto* d_seq = d_seq_;
     const auto* d_off = (const uint64_t*)buf_[S_OFF];
     const uint32_t* d_chain = (const uint32_t*)buf_[S_CHAIN];
     const uint32_t k = (uint32_t)p_.k;
     const uint64_t L = max_len_;
-    hipEvent_t e0, e1;
-    SA_CHECK(hipEventCreateWithFlags(&e0, hipEventBlockingSync));
-    SA_CHECK(hipEventCreateWithFlags(&e1, hipEventBlockingSync));
+    // kernel times are read once the work is done (no wait per kernel)
+    struct Timed { const char* name; uint64_t items, bytes; hipEvent_t a, b; };
+    std::vector<Timed> tl;
     auto timed = [&](const char* name, uint64_t items, auto&& launch, uint64_t bytes = 0) {
-        SA_CHECK(hipEventRecord(e0, stream_));
+        Timed x{name, items, bytes, event(), event()};
+        SA_CHECK(hipEventRecord(x.a, stream_));
         launch();
         SA_CHECK(hipGetLastError());
-        SA_CHECK(hipEventRecord(e1, stream_));
-        SA_CHECK(hipEventSynchronize(e1));
-        float t = 0;
-        SA_CHECK(hipEventElapsedTime(&t, e0, e1));
-        add_kernel(name, t, items, bytes);
-        total_ms_ += t;
-        return t;
+        SA_CHECK(hipEventRecord(x.b, stream_));
+        tl.push_back(x);
     };
-    float ext_ms = 0;
-    // 1. Hamming fast path, rescue prefilter, job lists
-    // algorithmic bytes: request in, result out, the query and the 2-bit reference
-    // it is compared with (read window, or the rescue window of the prefilter)
-    uint64_t prep_bytes = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint64_t Lq = h_off_[req[i].read_id + 1] - h_off_[req[i].read_id];
-        const uint64_t wl = req[i].kind == EXT_SEED ? Lq : (uint64_t)(req[i].win_end - req[i].win_start);
-        prep_bytes += sizeof(ExtRequest) + sizeof(ExtResult) + Lq + (wl + 3) / 4;
-    }
-    ext_ms += timed("k_ext_prep", n, [&] {
-        k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res, d_slots, d_jobs,
-                                                        d_ljobs, d_pjobs, d_cnt);
+    uint32_t cnt[4 + SSW_BUCKETS];
+    auto read_counts = [&] {
+        SA_CHECK(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, stream_));
+        sync();
+    };
+    // 1. Hamming fast path, rescue prefilter, job lists; SSW jobs into forward-byte buckets
+    timed("k_ext_prep", n, [&] {
+        k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res,
+                                                                      d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
-    uint32_t cnt[4] = {0, 0, 0, 0};
-    SA_CHECK(hipMemcpyAsync(cnt, d_cnt, 16, hipMemcpyDeviceToHost, stream_));
-    sync();
+    k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap);
+    SA_CHECK(hipGetLastError());
+    read_counts();
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
+    const uint32_t* bc = cnt + 4;
     // thread-per-job kernels run in waves sized to the scratch budget
     const uint64_t budget = 16ull << 30;
     auto waves = [&](uint32_t njobs, uint64_t per, auto&& launch) {
@@ -449,19 +467,6 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     // 2. SSW: score passes per lane group, bucketed by stripe count, then
     //    traceback per thread
     if (nssw) {
-        const uint32_t cap = nssw;
-        auto* d_ends = (SswEnds*)grow(S_ENDS, (uint64_t)nssw * sizeof(SswEnds));
-        auto* d_fwd = (SswEnd*)grow(S_FWD, (uint64_t)nssw * sizeof(SswEnd));
-        auto* d_lists = (uint32_t*)grow(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
-        auto* d_bcnt = (uint32_t*)grow(S_BCNT, SSW_BUCKETS * 4);
-        SA_CHECK(hipMemsetAsync(d_bcnt, 0, SSW_BUCKETS * 4, stream_));
-        k_ssw_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, nssw, d_lists, d_bcnt, cap);
-        SA_CHECK(hipGetLastError());
-        uint32_t bc[SSW_BUCKETS];
-        auto read_counts = [&] {
-            SA_CHECK(hipMemcpyAsync(bc, d_bcnt, sizeof bc, hipMemcpyDeviceToHost, stream_));
-            sync();
-        };
         // launches every non-empty bucket of one (mode, pass) family
         auto run = [&](bool word, bool rev) {
             for (uint32_t seg = 1; seg <= (word ? 32u : 16u); ++seg) {
@@ -477,22 +482,21 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 SA_CHECK(hipGetLastError());
             }
         };
-        read_counts();
-        ext_ms += timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); });
+        timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); });
         read_counts();
         uint64_t nrb = 0, nfw = 0;
         for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1)];
         for (uint32_t x = 0; x < 32; ++x) nfw += bc[ssw_bucket(true, false, x + 1)];
-        if (nrb) ext_ms += timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
+        if (nrb) timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
         if (nfw) {
-            ext_ms += timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });
+            timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });
             read_counts();
-            ext_ms += timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
+            timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
         }
         // banded traceback: band up to 64 without a retry
         uint64_t per = 3ull * (2 * 64 + 4) * 4 + 129ull * L * 3 + 8ull * (2 * L + 64) + 4096;
         per = (per + 255) & ~255ull;
-        ext_ms += timed("k_ssw_finish", nssw, [&] {
+        timed("k_ssw_finish", nssw, [&] {
             waves(nssw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
                 k_ssw_finish<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, cnt_,
                                                                     d_res, d_slots, scr, per, first);
@@ -503,7 +507,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     if (nlong) {
         uint64_t per = 16ull * ((L + 15) / 16 * 16) + 256 + 65536 + 8 * (2 * L + 64) + 129ull * L * 3;
         per = (per + 255) & ~255ull;
-        ext_ms += timed("k_ext_long", nlong, [&] {
+        timed("k_ext_long", nlong, [&] {
             waves(nlong, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
                 k_ext_long<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_ljobs, cnt_, d_res,
                                                                   d_slots, scr, per, first);
@@ -514,18 +518,38 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     if (npw) {
         uint64_t per = 16 * L + 256 + (L + 1) * (L + L / 10 + 2) + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
         per = (per + 255) & ~255ull;
-        ext_ms += timed("k_ext_pw", npw, [&] {
+        timed("k_ext_pw", npw, [&] {
             waves(npw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
                 k_ext_pw<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res,
                                                                 d_slots, d_chain, scr, per, first);
             });
         });
     }
-    times_.extend = ext_ms;
+    // 5. CIGAR offsets (found results only) and the results, in one wait
+    auto* d_ncig = (uint64_t*)grow(S_NCIG, (uint64_t)n * 8);
+    auto* d_coff = (uint64_t*)grow(S_CIGOFF, (uint64_t)(n + 1) * 8);
+    k_ncigar<<<blocks(n, 256), 256, 0, stream_>>>(d_res, d_ncig, n);
+    {
+        size_t tmp = 0;
+        SA_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, d_ncig, d_coff + 1, (int)n, stream_));
+        void* t = grow(S_SCANTMP, tmp);
+        SA_CHECK(hipcub::DeviceScan::InclusiveSum(t, tmp, d_ncig, d_coff + 1, (int)n, stream_));
+        SA_CHECK(hipMemsetAsync(d_coff, 0, 8, stream_));
+    }
     SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)(n + 1) * 8, hipMemcpyDeviceToHost, stream_));
     sync();
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    float ext_ms = 0;
+    for (Timed& x : tl) {
+        float t = 0;
+        SA_CHECK(hipEventElapsedTime(&t, x.a, x.b));
+        add_kernel(x.name, t, x.items, x.bytes);
+        total_ms_ += t;
+        ext_ms += t;
+        release_event(x.a);
+        release_event(x.b);
+    }
+    times_.extend = ext_ms;
     // requests that ran out of scratch are re-run one at a time with a large slot
     std::vector<uint32_t> big;
     for (uint32_t i = 0; i < n; ++i)
@@ -537,25 +561,25 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
             k_extend<<<1, 1, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, 1, d_res, d_slots, d_chain, d_big, bigs, i);
             SA_CHECK(hipGetLastError());
         }
+        k_ncigar<<<blocks(n, 256), 256, 0, stream_>>>(d_res, d_ncig, n);
+        size_t tmp = 0;
+        SA_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, tmp, d_ncig, d_coff + 1, (int)n, stream_));
+        void* t = grow(S_SCANTMP, tmp);
+        SA_CHECK(hipcub::DeviceScan::InclusiveSum(t, tmp, d_ncig, d_coff + 1, (int)n, stream_));
         SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
+        SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)(n + 1) * 8, hipMemcpyDeviceToHost, stream_));
         sync();
     }
     for (uint32_t i = 0; i < n; ++i)
         if (res[i].status != 0) throw std::runtime_error("extension request failed (status)");
-    // compact the CIGARs
-    auto* d_ncig = (uint64_t*)grow(S_NCIG, (uint64_t)n * 8);
-    auto* d_coff = (uint64_t*)grow(S_CIGOFF, (uint64_t)n * 8);
-    k_ncigar<<<blocks(n, 256), 256, 0, stream_>>>(d_res, d_ncig, n);
-    uint64_t total = 0;
-    scan_u64(d_ncig, d_coff, n, &total);
+    // 6. the CIGARs, compacted on the device
+    const uint64_t total = cigar_off[n];
     auto* d_cout = (uint32_t*)grow(S_CIGOUT, (total + 1) * 4);
     k_compact_cigars<<<blocks(n, 256), 256, 0, stream_>>>(d_req, d_res, d_slots, d_coff, d_cout, n);
     SA_CHECK(hipGetLastError());
     cigars.resize(total);
     if (total) SA_CHECK(hipMemcpyAsync(cigars.data(), d_cout, total * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
     sync();
-    cigar_off[n] = total;
 }
 
 }  // namespace sa

@@ -83,11 +83,14 @@ public:
 private:
     void* grow(int slot, uint64_t bytes);
     void sync();  // waits for the engine's stream (blocking wait)
+    ihipEvent_t* event();  // a timing event from the pool
+    void release_event(ihipEvent_t* e);
     void scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total);
 
     int device_;
     ihipStream_t* stream_ = nullptr;
     ihipEvent_t* sync_ev_ = nullptr;
+    std::vector<ihipEvent_t*> event_pool_;
     SeedParams p_{};
     uint64_t n_entries_ = 0;
     RefEntry* d_entries_ = nullptr;

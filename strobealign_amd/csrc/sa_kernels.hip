@@ -210,14 +210,15 @@ SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg) {
     return (rev ? (word ? 64u : 16u) : (word ? 32u : 0u)) + seg - 1;
 }
 
-// Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY).
+// Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY);
+// the job count is counters[0] on the device, the grid covers every request.
 __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__ off,
                                                    const ExtRequest* __restrict__ req,
-                                                   const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                                   const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ counters,
                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
                                                    uint32_t cap) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= njobs) return;
+    if (t >= counters[0]) return;
     const uint32_t rid = req[jobs[t].req].read_id;
     const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
     const uint32_t b = ssw_bucket(false, false, (L + 15) / 16);
