@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all cores)")
     ap.add_argument("--data-dir", default=os.environ.get("SA_BENCH_DATA", "/tmp/strobealign_amd_bench"))
-    ap.add_argument("--cpu-pairs", type=int, default=40_000, help="oracle sample size (0: skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=200_000, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     a = ap.parse_args()

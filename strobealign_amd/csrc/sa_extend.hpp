@@ -17,12 +17,19 @@ struct CigarOut {
     uint32_t* ops;
     uint32_t cap, n;
     bool overflow;
+    uint32_t last = 15;  // op of the last entry (15: none); entries past cap are only counted
+    SA_HD void reset() {
+        n = 0;
+        overflow = false;
+        last = 15;
+    }
     SA_HD void push(uint32_t op, uint32_t len) {
         if (len == 0) return;
-        if (n > 0 && (ops[n - 1] & 0xF) == op) {
-            ops[n - 1] += len << 4;
+        if (n > 0 && last == op) {
+            if (n <= cap) ops[n - 1] += len << 4;
             return;
         }
+        last = op;
         if (n >= cap) { overflow = true; ++n; return; }
         ops[n++] = (len << 4) | op;
     }
@@ -314,8 +321,7 @@ struct SswEnds {
 SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e, const Scores& sc, uint8_t* scratch,
                            uint64_t scratch_bytes, CigarOut& cig) {
     AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
-    cig.n = 0;
-    cig.overflow = false;
+    cig.reset();
     const int32_t score1 = e.score1, ref_end1 = e.ref_end1, read_end1 = e.read_end1;
     const int32_t ref_begin1 = e.ref_begin1, read_begin1 = e.read_begin1;
     if (ref_begin1 < 0 || read_begin1 < 0) return a;
@@ -333,7 +339,7 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
     if (front_room >= cig.cap) { a.status = 1; return a; }
     mainc.ops = cig.ops + front_room;
     mainc.cap = cig.cap - front_room;
-    mainc.n = 0;
+    mainc.reset();
     // emit main alignment into mainc
     {
         auto emit_main = [&](uint32_t op, uint32_t len) {
@@ -368,8 +374,7 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
     }
     bool front = fq == 0 && fscore + sc.end_bonus > score;
     // final CIGAR: [front ops][main ops][back ops]
-    cig.n = 0;
-    cig.overflow = false;
+    cig.reset();
     if (front && qs > 0) {
         for (uint32_t t = 0; t < qs; ++t) cig.push(q[t] == r[rs - qs + t] ? 7u : 8u, 1);
     }
@@ -416,8 +421,7 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
 SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Scores& sc, uint8_t* scratch,
                                   uint64_t scratch_bytes, CigarOut& cig) {
     AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
-    cig.n = 0;
-    cig.overflow = false;
+    cig.reset();
     if (r.len > 2000 || q.len == 0) return a;
     const int32_t qlen = (int32_t)q.len, rlen = (int32_t)r.len;
     uint64_t ssw_bytes = 4ull * 4 * (uint64_t)(((qlen + 15) / 16) * 16);
@@ -456,14 +460,64 @@ SA_HD AlignInfo hamming_align(const Q& q, const R& r, uint32_t n, const Scores& 
         if (score > best) { bs = start; best = score; be = i + 1; }
     }
     if (score + sc.end_bonus > best) { best = score + sc.end_bonus; be = n; bs = start; }
-    cig.n = 0;
-    cig.overflow = false;
+    cig.reset();
     uint32_t mm = 0;
     for (uint32_t i = bs; i < be; ++i) {
         bool eq = q[i] == r[i];
         mm += eq ? 0 : 1;
         cig.push(eq ? 7u : 8u, 1);
     }
+    AlignInfo a{true, (uint32_t)best, bs, be, bs, be, mm, cig.overflow ? 1 : 0};
+    return a;
+}
+
+// hamming_align over a mismatch bit mask (bit i set: position i mismatches),
+// visiting mismatches only: between two mismatches the score rises by `match`
+// per base, so the running maximum of a run of matches is reached at its last
+// base, and a mismatch can only lower the score (and reset it below zero).
+// Same result as hamming_align (src/aligner.rs:181-259) when match > 0; callers
+// use hamming_align otherwise.
+SA_HD AlignInfo hamming_align_bits(const uint64_t* m, uint32_t n, const Scores& sc, CigarOut& cig) {
+    uint32_t start = 0, bs = 0, be = 0;
+    int32_t score = sc.end_bonus, best = 0;
+    uint32_t prev = 0;  // first position after the previous mismatch
+    auto run_to = [&](uint32_t end) {  // matches at [prev, end)
+        if (end > prev) {
+            score += (int32_t)(end - prev) * sc.match;
+            if (score > best) { bs = start; best = score; be = end; }
+        }
+    };
+    for (uint32_t w = 0; w * 64 < n; ++w) {
+        uint64_t bits = m[w];
+        while (bits) {
+            const uint32_t x = w * 64 + (uint32_t)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            if (x >= n) break;
+            run_to(x);
+            score -= sc.mismatch;
+            if (score < 0) { start = x + 1; score = 0; }
+            if (score > best) { bs = start; best = score; be = x + 1; }  // only while the end bonus lasts
+            prev = x + 1;
+        }
+    }
+    run_to(n);
+    if (score + sc.end_bonus > best) { best = score + sc.end_bonus; be = n; bs = start; }
+    cig.reset();
+    uint32_t mm = 0, p = bs;
+    for (uint32_t w = bs / 64; w * 64 < be; ++w) {
+        uint64_t bits = m[w];
+        while (bits) {
+            const uint32_t x = w * 64 + (uint32_t)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            if (x < bs) continue;
+            if (x >= be) break;
+            if (x > p) cig.push(7u, x - p);
+            cig.push(8u, 1);
+            ++mm;
+            p = x + 1;
+        }
+    }
+    if (be > p) cig.push(7u, be - p);
     AlignInfo a{true, (uint32_t)best, bs, be, bs, be, mm, cig.overflow ? 1 : 0};
     return a;
 }
@@ -644,8 +698,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
                                  const Scores& sc, uint32_t k, uint8_t* scratch, uint64_t scratch_bytes,
                                  CigarOut& cig) {
     AlignInfo res{false, 0, 0, 0, 0, 0, 0, 0};
-    cig.n = 0;
-    cig.overflow = false;
+    cig.reset();
     if (na == 0) return res;
     const uint32_t ql = q.len;
     const uint32_t rl = win.len;

@@ -109,7 +109,8 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
                 if ((float)hd / (float)L < 0.05f) {
                     if (lane == 0) {
                         CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
-                        AlignInfo a = hamming_align(ZeroBits{}, m, L, sc, cig);
+                        AlignInfo a = sc.match > 0 ? hamming_align_bits(m.w, L, sc, cig)
+                                                   : hamming_align(ZeroBits{}, m, L, sc, cig);
                         out.status = (uint32_t)a.status;
                         if (a.found && a.status == 0) write_found(out, a, prs, false, cig.n);
                         res[t] = out;
