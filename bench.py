@@ -8,6 +8,12 @@ extension on the GPU; decision replay and SAM formatting on the host) with the
 reads parsed once and staged in HBM before timing and the index resident in
 HBM. The SAM text is produced per chunk in host memory and then dropped.
 
+Then the serving loop (sa_server_*) runs on the same resident index: requests
+of log-uniform [1, 50000] pairs cut from the same reads arrive Poisson at 200/s
+(400 requests), each timed from its submit call to its last SAM byte (p50/p99
+in "latency_ms"); the same requests submitted at once give the saturated
+throughput of the request loop.
+
 Prints one JSON line (see the contract in the repository docs) with a roofline
 object for the dominant kernel (HIP-event timing on the engine's stream) and a
 CPU baseline: the oracle (a C++ restatement of the reference) on the host's
@@ -15,9 +21,13 @@ cores over a bounded sample of the same reads.
 """
 import argparse
 import json
+import math
 import os
+import queue
+import random
 import subprocess
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -52,6 +62,63 @@ def cpu_baseline(ref, r1, r2, pairs, threads, workdir):
     return pairs / t["map_s"], t
 
 
+def request_plan(n_pairs, n_req, rate, seed):
+    """Poisson arrivals (rate per second) of requests of log-uniform [1, 50000]
+    pairs, cut consecutively from the read set (wrapping): (arrival_s, first, count)."""
+    rnd = random.Random(seed)
+    plan, pos, t = [], 0, 0.0
+    for _ in range(n_req):
+        t += rnd.expovariate(rate)
+        n = min(50_000, max(1, int(math.exp(rnd.uniform(0.0, math.log(50_000))))), n_pairs)
+        if pos + n > n_pairs:
+            pos = 0
+        plan.append((t, pos, n))
+        pos += n
+    return plan
+
+
+def serve(sa, ix, reads, threads, slots, plan, paced):
+    """Runs a request plan through the serving loop. paced: submit at the arrival
+    times; otherwise all at once (saturation). Returns latencies (ms), the wall
+    time from the first submission to the last completion, and server stats."""
+    srv = sa.Server(ix, sa.options(threads=threads, slots=slots))
+    lat = [None] * len(plan)
+    q = queue.Queue()
+
+    def waiter():
+        while True:
+            item = q.get()
+            if item is None:
+                return
+            k, t = item
+            lat[k] = srv.wait(t, keep_output=False)[1]
+    ws = [threading.Thread(target=waiter) for _ in range(8)]
+    for w in ws:
+        w.start()
+    t0 = time.perf_counter()
+    lag = 0.0
+    for k, (ta, a, n) in enumerate(plan):
+        if paced:
+            d = t0 + ta - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            lag = max(lag, -d)
+        q.put((k, srv.submit(reads, a, n)))
+    for _ in ws:
+        q.put(None)
+    for w in ws:
+        w.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return lat, wall, st, lag
+
+
+def pct(v, p):
+    v = sorted(v)
+    return v[min(len(v) - 1, int(math.ceil(p / 100.0 * len(v))) - 1)]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +130,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=200_000, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -126,6 +195,28 @@ def main():
             "measured_copy_gbps": round(copy_gbps, 1),
             "kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in delta.items()},
             "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in delta.items() if v[3] and v[0]}}
+    serving = None
+    if a.serve_requests > 0:
+        # request loop over the same resident index: Poisson arrivals, then the
+        # same requests submitted at once (saturation throughput)
+        plan = request_plan(len(reads), a.serve_requests, a.serve_rate, 11)
+        slots = a.slots or 2
+        lat, swall, sst, lag = serve(sa, ix, reads, threads, slots, plan, True)
+        pairs = sum(n for _, _, n in plan)
+        _, sat_wall, sat_st, _ = serve(sa, ix, reads, threads, slots, plan, False)
+        serving = {"requests": len(plan), "arrivals": "Poisson %.0f/s" % a.serve_rate,
+                   "pairs_per_request": "log-uniform [1, 50000] (mean %.0f)" % (pairs / len(plan)),
+                   "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+                   "mean_ms": round(sum(lat) / len(lat), 2), "max_ms": round(max(lat), 2),
+                   "offered_pairs_per_s": round(pairs / plan[-1][0], 1),
+                   "served_pairs_per_s": round(pairs / swall, 1),
+                   "saturated_pairs_per_s": round(pairs / sat_wall, 1),
+                   "batches": sst["batches"], "max_requests_per_batch": sst["max_requests_per_batch"],
+                   "mean_requests_per_batch": round(sst["batch_requests"] / max(1, sst["batches"]), 2),
+                   "arena_peak_bytes": sst["arena_peak"], "driver_max_lag_ms": round(1000 * lag, 2),
+                   "engines": slots}
+        print("serving: p50 %.1f ms, p99 %.1f ms, saturated %.0f pairs/s" %
+              (serving["p50_ms"], serving["p99_ms"], serving["saturated_pairs_per_s"]), file=sys.stderr, flush=True)
     cpu = None
     if not a.no_cpu and a.cpu_pairs > 0:
         try:
@@ -153,6 +244,8 @@ def main():
                    "read_upload_ms": round(1000 * upload_s, 2),
                    "pairs_per_s_incl_pcie": round(n_pairs * a.steps / (wall + a.steps * upload_s), 1),
                    "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests},
+        "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"]} if serving else None,
+        "serving": serving,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

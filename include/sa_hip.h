@@ -139,6 +139,42 @@ typedef struct sa_run_stats {
 int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* reads, char** out, size_t* out_len,
                  sa_run_stats* stats);
 
+/* Serving loop: requests (read sets) are submitted as they arrive and mapped
+ * with continuous batching. A request is admitted, in arrival order, once its
+ * reads fit in a device arena of arena_bytes (0: min(4 GiB, a quarter of the
+ * free device memory)); its reads stay in HBM until it completes. Each of the
+ * opt->slots engines (sharing the index, which must outlive the server) forms its
+ * next device batch from the oldest admitted chunks of any requests, up to
+ * opt->batch_bases bases (0: 8M), so requests that arrive while others run share
+ * batches and overlap on the device. Each request's output (SAM records without
+ * header, PAF, or the abundance table) equals sa_map_records on that request
+ * alone. Submit and wait may be called from any threads. This is the request loop
+ * the reference lacks: its unit of work is Mapper::map_chunk (src/main.rs:873-988),
+ * which each request runs as its own read set. */
+typedef struct sa_server sa_server;
+typedef struct sa_server_stats {
+    uint64_t submitted, completed, batches, batch_requests, max_requests_per_batch;
+    uint64_t arena_bytes, arena_peak;
+    double busy_ms; /* engine time in batches, summed over engines */
+} sa_server_stats;
+int sa_server_create(sa_index* index, const sa_map_options* opt, uint64_t arena_bytes, sa_server** out);
+/* Queues a request (arrays as for sa_map_records) and returns its ticket. */
+int sa_server_submit(sa_server* server, uint64_t n, const char* const* names1, const char* const* seqs1,
+                     const char* const* quals1, const char* const* names2, const char* const* seqs2,
+                     const char* const* quals2, uint64_t* ticket);
+/* Queues records [first, first + count) of a loaded read set as one request. */
+int sa_server_submit_reads(sa_server* server, const sa_reads* reads, uint64_t first, uint64_t count,
+                           uint64_t* ticket);
+/* Waits for a request; *out (sa_free) receives its output unless out is NULL,
+ * *latency_ms the time from the start of its submit call to its completion (all
+ * of its output formatted). A ticket is waited once. */
+int sa_server_wait(sa_server* server, uint64_t ticket, char** out, size_t* out_len, double* latency_ms);
+int sa_server_get_stats(const sa_server* server, sa_server_stats* st);
+/* As sa_index_kernel_times, over the server's engines. */
+size_t sa_server_kernel_times(const sa_server* server, char* buf, size_t cap);
+/* Stops the workers (requests not yet waited for are dropped) and frees the arena. */
+void sa_server_destroy(sa_server* server);
+
 /* Per-stage probes (parity tests, and a finer-grained drop-in for a caller that
  * keeps its own decision logic). Reads are concatenated ASCII with offsets[n+1]. */
 

@@ -61,7 +61,9 @@ EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa
            "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
            "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
-           "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_free"]
+           "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
+           "sa_server_destroy", "sa_server_submit", "sa_server_submit_reads", "sa_server_wait",
+           "sa_server_get_stats", "sa_server_kernel_times", "sa_free"]
 
 
 class RunStats(ctypes.Structure):
@@ -123,6 +125,8 @@ def load(path=LIB_PATH):
     lib.sa_index_kernel_times.restype = ctypes.c_size_t
     lib.sa_reads_count.restype = ctypes.c_uint64
     lib.sa_reads_destroy.restype = None
+    lib.sa_server_destroy.restype = None
+    lib.sa_server_kernel_times.restype = ctypes.c_size_t
     _lib = lib
     return lib
 
@@ -290,6 +294,84 @@ class Index:
         v = ctypes.c_double()
         _check(_lib.sa_index_stats(self._p, ctypes.byref(v)))
         return v.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "batches", "batch_requests",
+                                                "max_requests_per_batch", "arena_bytes", "arena_peak")] + \
+               [("busy_ms", ctypes.c_double)]
+
+
+class Server:
+    """Serving loop with continuous batching (sa_server_*): requests are read sets
+    submitted as they arrive; each one's output equals mapping it alone."""
+
+    def __init__(self, index, opts, arena_bytes=0):
+        self._p = ctypes.c_void_p()
+        self._index = index
+        _check(_lib.sa_server_create(index._p, ctypes.byref(opts), ctypes.c_uint64(arena_bytes), ctypes.byref(self._p)))
+
+    def submit(self, reads, first, count):
+        """Records [first, first + count) of a Reads set as one request; returns its ticket."""
+        t = ctypes.c_uint64()
+        _check(_lib.sa_server_submit_reads(self._p, reads._p, ctypes.c_uint64(first), ctypes.c_uint64(count),
+                                           ctypes.byref(t)))
+        return t.value
+
+    def submit_records(self, recs1, recs2=None):
+        """recs: lists of (name, seq, qual) bytes (qual may be None); returns the ticket."""
+        n = len(recs1)
+
+        def arrays(recs):
+            names = (ctypes.c_char_p * max(1, n))(*[r[0] for r in recs])
+            seqs = (ctypes.c_char_p * max(1, n))(*[r[1] for r in recs])
+            quals = None
+            if n and recs[0][2] is not None:
+                quals = (ctypes.c_char_p * n)(*[r[2] for r in recs])
+            return names, seqs, quals
+        a1 = arrays(recs1)
+        a2 = arrays(recs2) if recs2 is not None else (None, None, None)
+        t = ctypes.c_uint64()
+        _check(_lib.sa_server_submit(self._p, ctypes.c_uint64(n), a1[0], a1[1], a1[2], a2[0], a2[1], a2[2],
+                                     ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, keep_output=True):
+        """Blocks until the request is done; returns (output bytes or None, latency ms)."""
+        lat = ctypes.c_double()
+        if keep_output:
+            out = ctypes.c_void_p()
+            n = ctypes.c_size_t()
+            _check(_lib.sa_server_wait(self._p, ctypes.c_uint64(ticket), ctypes.byref(out), ctypes.byref(n),
+                                       ctypes.byref(lat)))
+            s = ctypes.string_at(out, n.value)
+            _lib.sa_free(out)
+            return s, lat.value
+        _check(_lib.sa_server_wait(self._p, ctypes.c_uint64(ticket), None, None, ctypes.byref(lat)))
+        return None, lat.value
+
+    def stats(self):
+        st = ServerStats()
+        _check(_lib.sa_server_get_stats(self._p, ctypes.byref(st)))
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}
+
+    def kernel_times(self):
+        import json
+        n = _lib.sa_server_kernel_times(self._p, None, ctypes.c_size_t(0))
+        buf = ctypes.create_string_buffer(n + 1)
+        _lib.sa_server_kernel_times(self._p, buf, ctypes.c_size_t(n + 1))
+        return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
+
+    def close(self):
+        if self._p:
+            _lib.sa_server_destroy(self._p)
+            self._p = None
 
     def __enter__(self):
         return self

@@ -33,6 +33,13 @@ struct sa_index {
     ~sa_index() { slots.clear(); }  // before eng, whose index they share
 };
 
+struct sa_server {
+    sa_index* index;
+    std::vector<std::unique_ptr<sa::Engine>> engines;  // share the index's resident index
+    std::unique_ptr<sa::Server> srv;
+    ~sa_server() { srv.reset(); }  // workers stop before their engines go
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -101,6 +108,52 @@ char* dup_out(const std::string& s, size_t* len) {
     p[s.size()] = 0;
     *len = s.size();
     return p;
+}
+// Per-kernel totals summed over engines, as JSON text (see sa_index_kernel_times).
+size_t kernel_times_json(const std::vector<const sa::Engine*>& engs, char* buf, size_t cap) {
+    std::vector<std::pair<std::string, sa::Engine::KernelStat>> all;
+    for (const sa::Engine* e : engs)
+        for (const auto& k : e->kernel_stats()) {
+            bool found = false;
+            for (auto& a : all)
+                if (a.first == k.first) {
+                    a.second.ms += k.second.ms;
+                    a.second.launches += k.second.launches;
+                    a.second.items += k.second.items;
+                    a.second.bytes += k.second.bytes;
+                    found = true;
+                }
+            if (!found) all.push_back(k);
+        }
+    std::string s = "{";
+    bool first = true;
+    for (const auto& k : all) {
+        char tmp[256];
+        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu, %llu]", first ? "" : ", ", k.first.c_str(),
+                      k.second.ms, (unsigned long long)k.second.launches, (unsigned long long)k.second.items,
+                      (unsigned long long)k.second.bytes);
+        s += tmp;
+        first = false;
+    }
+    s += "}";
+    if (buf && cap) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return s.size();
+}
+
+sa::Record make_record(const char* name, const char* seq, const char* qual) {
+    sa::Record r;
+    std::string h = name ? name : "";
+    size_t p = h.find_first_of(" \t");
+    if (p == std::string::npos) r.name = h;
+    else { r.name = h.substr(0, p); r.comment = h.substr(p + 1); r.has_comment = true; }
+    r.seq = seq;
+    for (char& c : r.seq) c = (char)toupper((unsigned char)c);
+    if (qual) r.qual = qual; else r.has_qual = false;
+    return r;
 }
 }  // namespace
 
@@ -228,21 +281,10 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
     if (!index || !names1 || !seqs1 || !out || !out_len) return fail(SA_ERR_ARG, "NULL argument");
     if ((names2 == nullptr) != (seqs2 == nullptr)) return fail(SA_ERR_ARG, "names2 and seqs2 must both be set");
     return guard([&] {
-        auto mk = [](const char* name, const char* seq, const char* qual) {
-            sa::Record r;
-            std::string h = name;
-            size_t p = h.find_first_of(" \t");
-            if (p == std::string::npos) r.name = h;
-            else { r.name = h.substr(0, p); r.comment = h.substr(p + 1); r.has_comment = true; }
-            r.seq = seq;
-            for (char& c : r.seq) c = (char)toupper((unsigned char)c);
-            if (qual) r.qual = qual; else r.has_qual = false;
-            return r;
-        };
         std::vector<sa::Record> r1, r2;
         for (uint64_t i = 0; i < n; ++i) {
-            r1.push_back(mk(names1[i], seqs1[i], quals1 ? quals1[i] : nullptr));
-            if (seqs2) r2.push_back(mk(names2[i], seqs2[i], quals2 ? quals2[i] : nullptr));
+            r1.push_back(make_record(names1[i], seqs1[i], quals1 ? quals1[i] : nullptr));
+            if (seqs2) r2.push_back(make_record(names2[i], seqs2[i], quals2 ? quals2[i] : nullptr));
         }
         sa::Options o = options_of(opt);
         std::string s;
@@ -318,6 +360,89 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
         }
         return SA_OK;
     });
+}
+
+int sa_server_create(sa_index* index, const sa_map_options* opt, uint64_t arena_bytes, sa_server** out) {
+    if (!index || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        sa::Options o = options_of(opt);
+        if (!opt || opt->batch_bases == 0) o.batch_bases = 8ull << 20;  // serving default: short batches
+        if (arena_bytes == 0) arena_bytes = std::min<uint64_t>(4ull << 30, sa::device_free_bytes(index->device) / 4);
+        auto s = std::make_unique<sa_server>();
+        s->index = index;
+        std::vector<sa::Engine*> v;
+        for (int i = 0; i < o.slots; ++i) {
+            s->engines.push_back(std::make_unique<sa::Engine>(sa::Engine::ShareIndex{}, *index->eng));
+            v.push_back(s->engines.back().get());
+        }
+        s->srv = std::make_unique<sa::Server>(v, index->ref, o, index->device, arena_bytes);
+        *out = s.release();
+        return SA_OK;
+    });
+}
+
+void sa_server_destroy(sa_server* server) { delete server; }
+
+int sa_server_submit(sa_server* server, uint64_t n, const char* const* names1, const char* const* seqs1,
+                     const char* const* quals1, const char* const* names2, const char* const* seqs2,
+                     const char* const* quals2, uint64_t* ticket) {
+    if (!server || !ticket || (n && (!names1 || !seqs1))) return fail(SA_ERR_ARG, "NULL argument");
+    if ((names2 == nullptr) != (seqs2 == nullptr)) return fail(SA_ERR_ARG, "names2 and seqs2 must both be set");
+    const double t_in = sa::steady_ms();
+    return guard([&] {
+        std::vector<sa::Record> r1, r2;
+        r1.reserve(n);
+        for (uint64_t i = 0; i < n; ++i) {
+            r1.push_back(make_record(names1[i], seqs1[i], quals1 ? quals1[i] : nullptr));
+            if (seqs2) r2.push_back(make_record(names2[i], seqs2[i], quals2 ? quals2[i] : nullptr));
+        }
+        *ticket = server->srv->submit(std::move(r1), std::move(r2), t_in);
+        return SA_OK;
+    });
+}
+
+int sa_server_submit_reads(sa_server* server, const sa_reads* reads, uint64_t first, uint64_t count,
+                           uint64_t* ticket) {
+    if (!server || !reads || !ticket) return fail(SA_ERR_ARG, "NULL argument");
+    if (first > reads->r1.size() || count > reads->r1.size() - first) return fail(SA_ERR_ARG, "range out of bounds");
+    const double t_in = sa::steady_ms();
+    return guard([&] {
+        const auto b = reads->r1.begin() + (ptrdiff_t)first;
+        std::vector<sa::Record> r1(b, b + (ptrdiff_t)count), r2;
+        if (!reads->r2.empty()) {
+            const auto b2 = reads->r2.begin() + (ptrdiff_t)first;
+            r2.assign(b2, b2 + (ptrdiff_t)count);
+        }
+        *ticket = server->srv->submit(std::move(r1), std::move(r2), t_in);
+        return SA_OK;
+    });
+}
+
+int sa_server_wait(sa_server* server, uint64_t ticket, char** out, size_t* out_len, double* latency_ms) {
+    if (!server) return fail(SA_ERR_ARG, "NULL argument");
+    if (out && !out_len) return fail(SA_ERR_ARG, "out_len is NULL");
+    return guard([&] {
+        std::string s, err;
+        sa::Server::RequestTimes t;
+        if (!server->srv->wait(ticket, s, t, err)) return fail(SA_ERR_INTERNAL, err);
+        if (latency_ms) *latency_ms = t.done_ms;
+        if (out) *out = dup_out(s, out_len);
+        return SA_OK;
+    });
+}
+
+int sa_server_get_stats(const sa_server* server, sa_server_stats* st) {
+    if (!server || !st) return fail(SA_ERR_ARG, "NULL argument");
+    const sa::Server::Stats s = server->srv->stats();
+    st->submitted = s.submitted;
+    st->completed = s.completed;
+    st->batches = s.batches;
+    st->batch_requests = s.batch_requests;
+    st->max_requests_per_batch = s.max_requests_per_batch;
+    st->arena_bytes = s.arena_bytes;
+    st->arena_peak = s.arena_peak;
+    st->busy_ms = s.busy_ms;
+    return SA_OK;
 }
 
 int sa_randstrobes_query_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
@@ -431,38 +556,16 @@ int sa_index_stats(const sa_index* index, double* kernel_ms) {
 
 size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap) {
     if (!index) return 0;
-    // summed over the index's engines
-    std::vector<std::pair<std::string, sa::Engine::KernelStat>> all = index->eng->kernel_stats();
-    for (const auto& e : index->slots)
-        for (const auto& k : e->kernel_stats()) {
-            bool found = false;
-            for (auto& a : all)
-                if (a.first == k.first) {
-                    a.second.ms += k.second.ms;
-                    a.second.launches += k.second.launches;
-                    a.second.items += k.second.items;
-                    a.second.bytes += k.second.bytes;
-                    found = true;
-                }
-            if (!found) all.push_back(k);
-        }
-    std::string s = "{";
-    bool first = true;
-    for (const auto& k : all) {
-        char tmp[256];
-        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu, %llu]", first ? "" : ", ", k.first.c_str(),
-                      k.second.ms, (unsigned long long)k.second.launches, (unsigned long long)k.second.items,
-                      (unsigned long long)k.second.bytes);
-        s += tmp;
-        first = false;
-    }
-    s += "}";
-    if (buf && cap) {
-        size_t n = std::min(cap - 1, s.size());
-        std::memcpy(buf, s.data(), n);
-        buf[n] = 0;
-    }
-    return s.size();
+    std::vector<const sa::Engine*> v{index->eng.get()};
+    for (const auto& e : index->slots) v.push_back(e.get());
+    return kernel_times_json(v, buf, cap);
+}
+
+size_t sa_server_kernel_times(const sa_server* server, char* buf, size_t cap) {
+    if (!server) return 0;
+    std::vector<const sa::Engine*> v;
+    for (const auto& e : server->engines) v.push_back(e.get());
+    return kernel_times_json(v, buf, cap);
 }
 
 int sa_hbm_copy_probe(sa_ctx* ctx, uint64_t bytes, int iters, double* gbps) {

@@ -57,6 +57,42 @@ void* device_upload(int device, const void* src, uint64_t bytes) {
     return p;
 }
 
+void* device_alloc(int device, uint64_t bytes) {
+    SA_CHECK(hipSetDevice(device));
+    void* p = nullptr;
+    SA_CHECK(hipMalloc(&p, bytes ? bytes : 16));
+    return p;
+}
+
+void device_copy_in(int device, void* dst, const void* src, uint64_t bytes) {
+    if (!bytes) return;
+    SA_CHECK(hipSetDevice(device));
+    // a stream of the calling thread's own, so that copies overlap the engines' work
+    struct ThreadStream {
+        hipStream_t st = nullptr;
+        int dev = -1;
+        ~ThreadStream() {
+            if (st) (void)hipStreamDestroy(st);
+        }
+    };
+    thread_local ThreadStream ts;
+    if (ts.dev != device) {
+        if (ts.st) (void)hipStreamDestroy(ts.st);
+        ts.st = nullptr;
+        SA_CHECK(hipStreamCreateWithFlags(&ts.st, hipStreamNonBlocking));
+        ts.dev = device;
+    }
+    SA_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ts.st));
+    SA_CHECK(hipStreamSynchronize(ts.st));
+}
+
+uint64_t device_free_bytes(int device) {
+    SA_CHECK(hipSetDevice(device));
+    size_t fr = 0, tot = 0;
+    SA_CHECK(hipMemGetInfo(&fr, &tot));
+    return fr;
+}
+
 __global__ void __launch_bounds__(256) k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
@@ -225,6 +261,31 @@ void Engine::upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_re
     void* s = grow(S_SEQ, nb + 16);
     if (nb) SA_CHECK(hipMemcpyAsync(s, seq, nb, hipMemcpyHostToDevice, stream_));
     use_reads((const uint8_t*)s, off, n_reads);
+}
+
+void Engine::gather_reads(const uint8_t* d_base, const uint64_t* seg, const uint64_t* seg_len, uint32_t n_segs,
+                          const uint64_t* off, uint32_t n_reads) {
+    SA_CHECK(hipSetDevice(device_));
+    // adjacent segments are copied (or used) as one
+    std::vector<std::pair<uint64_t, uint64_t>> runs;
+    for (uint32_t i = 0; i < n_segs; ++i) {
+        if (!runs.empty() && runs.back().first + runs.back().second == seg[i])
+            runs.back().second += seg_len[i];
+        else if (seg_len[i])
+            runs.push_back({seg[i], seg_len[i]});
+    }
+    if (runs.size() <= 1) {
+        use_reads(d_base + (runs.empty() ? 0 : runs[0].first), off, n_reads);
+        return;
+    }
+    uint8_t* s = (uint8_t*)grow(S_SEQ, off[n_reads] + 16);
+    uint64_t at = 0;
+    for (const auto& r : runs) {
+        SA_CHECK(hipMemcpyAsync(s + at, d_base + r.first, r.second, hipMemcpyDeviceToDevice, stream_));
+        at += r.second;
+    }
+    if (at != off[n_reads]) throw std::invalid_argument("gather_reads: segments do not cover the offsets");
+    use_reads(s, off, n_reads);
 }
 
 void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_reads) {

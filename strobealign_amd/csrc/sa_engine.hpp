@@ -29,6 +29,10 @@ struct StageTimes {
 
 // Plain device buffers outside an engine (a read set staged once for many calls).
 void* device_upload(int device, const void* src, uint64_t bytes);
+void* device_alloc(int device, uint64_t bytes);
+// Host -> device copy on a stream of the calling thread; returns when it is done.
+void device_copy_in(int device, void* dst, const void* src, uint64_t bytes);
+uint64_t device_free_bytes(int device);
 // Copies `bytes` between two device buffers `iters` times with a plain vector-copy
 // kernel; returns the achieved read+write GB/s (the HBM reference of the roofline).
 double hbm_copy_probe(int device, uint64_t bytes, int iters);
@@ -57,6 +61,11 @@ public:
     // Same for reads already in device memory: `d_seq` points at the first base of
     // the batch, `off` (host) holds n_reads + 1 offsets relative to it.
     void use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_reads);
+    // Same for reads in device memory as segments: segment i holds the reads of
+    // the batch's i-th part at d_base + seg[i] (seg_len[i] bytes); the parts are
+    // copied together (a device-to-device copy) unless they are already adjacent.
+    void gather_reads(const uint8_t* d_base, const uint64_t* seg, const uint64_t* seg_len, uint32_t n_segs,
+                      const uint64_t* off, uint32_t n_reads);
     void find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out);
 
     // Fine-grained stages for parity probes (valid after upload_reads).

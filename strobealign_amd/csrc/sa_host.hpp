@@ -3,6 +3,7 @@
 // PAF or abundance output with the reference's decision logic.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <optional>
 #include <string>
 #include <vector>
@@ -94,5 +95,40 @@ struct DeviceReads {
 void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, const std::vector<Record>& r1,
                  const std::vector<Record>& r2, const Options& o, std::string* out, RunStats* stats,
                  const DeviceReads* dreads);
+
+// Request loop with continuous batching over engines that share one resident
+// index. Each request is a read set mapped as map_records maps it alone (its own
+// chunks, RNG and insert-size state; output in record order). Requests are
+// admitted in arrival order while their reads fit in a device arena (their reads
+// stay there until the request completes); each engine forms its next device
+// batch from the oldest admitted chunks of any requests, so requests that arrive
+// while others run share batches and overlap on the device.
+// Milliseconds on the steady clock the server's timings use.
+double steady_ms();
+
+class Server {
+public:
+    Server(std::vector<Engine*> engines, const PackedRef& ref, const Options& o, int device, uint64_t arena_bytes);
+    ~Server();
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+    // Queues a request (paired when r2 is non-empty); returns its ticket. Its
+    // latency counts from t_arrival (steady_ms() clock; < 0: now).
+    uint64_t submit(std::vector<Record> r1, std::vector<Record> r2, double t_arrival = -1);
+    // Times since submission, milliseconds.
+    struct RequestTimes { double admit_ms = 0, done_ms = 0; };
+    // Blocks until the request is done; false (and `err`) if it failed.
+    bool wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err);
+    struct Stats {
+        uint64_t submitted = 0, completed = 0, batches = 0, batch_requests = 0, max_requests_per_batch = 0;
+        uint64_t arena_bytes = 0, arena_peak = 0;
+        double busy_ms = 0;  // summed over engines
+    };
+    Stats stats() const;
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 }  // namespace sa

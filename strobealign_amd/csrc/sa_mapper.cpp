@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,8 @@
 #include <deque>
 #include <exception>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <set>
 #include <stdexcept>
 #include <thread>
@@ -167,7 +170,10 @@ struct CachedResult {
 
 // Per-chunk state of the replay.
 struct Chunk {
-    uint64_t first, count;  // records (pairs) [first, first+count)
+    uint64_t first, count;  // records (pairs) [first, first+count) of the caller's read set
+    const Record* rec1 = nullptr;  // the chunk's first record (and its mate)
+    const Record* rec2 = nullptr;
+    uint64_t pos = 0;  // index of the chunk's first record (pair) within its device batch
     uint64_t next = 0;       // next record to commit (relative)
     Rng rng;
     InsertSize isize;
@@ -184,10 +190,10 @@ struct Ctx {
     const Options& o;
     const PackedRef& ref;
     const NamBatch& nb;
-    const std::vector<Record>& r1;
-    const std::vector<Record>& r2;
+    const Record* r1;  // records of the current chunk (indexed from its first record)
+    const Record* r2;
     bool paired;
-    uint64_t batch_first;  // first record index of the engine batch
+    uint64_t rel0;  // batch index of the chunk's first record
     uint32_t k;
     Chunk* ch;
     bool missed;   // a lookup of this record had no result yet (it is now pending)
@@ -493,7 +499,7 @@ void nams_of(const Ctx& c, uint64_t read_id, std::vector<Nam>& v) {
 void align_single_end(Ctx& c, uint64_t rec_idx, std::vector<Sam>& out, std::deque<Alignment>& store,
                       std::deque<std::array<Details, 2>>& dstore) {
     const Record& rec = c.r1[rec_idx];
-    const uint32_t read_id = (uint32_t)(rec_idx - c.batch_first);
+    const uint32_t read_id = (uint32_t)(c.rel0 + rec_idx);
     const uint32_t L = (uint32_t)rec.seq.size();
     std::vector<Nam> nams;
     nams_of(c, read_id, nams);
@@ -814,7 +820,7 @@ uint8_t alignment_quality(const std::vector<ScoredPair>& v) {
 void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Alignment>& store,
                       std::deque<std::array<Details, 2>>& dstore) {
     const Record* recs[2] = {&c.r1[idx], &c.r2[idx]};
-    const uint64_t rel = idx - c.batch_first;
+    const uint64_t rel = c.rel0 + idx;
     const uint32_t rid[2] = {(uint32_t)(2 * rel), (uint32_t)(2 * rel + 1)};
     const uint32_t L[2] = {(uint32_t)recs[0]->seq.size(), (uint32_t)recs[1]->seq.size()};
     dstore.emplace_back();
@@ -987,7 +993,7 @@ void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
     const PackedRef& ref = c.ref;
     if (!c.paired) {
         const Record& r1 = c.r1[idx];
-        uint32_t id = (uint32_t)(idx - c.batch_first);
+        uint32_t id = (uint32_t)(c.rel0 + idx);
         std::vector<Nam> nams;
         nams_of(c, id, nams);
         shuffle_best(nams, *c.rng);
@@ -1003,7 +1009,7 @@ void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
     }
     const Record& r1 = c.r1[idx];
     const Record& r2 = c.r2[idx];
-    uint64_t rel = idx - c.batch_first;
+    uint64_t rel = c.rel0 + idx;
     uint32_t id1 = (uint32_t)(2 * rel), id2 = (uint32_t)(2 * rel + 1);
     std::vector<Nam> n1, n2;
     nams_of(c, id1, n1);
@@ -1061,11 +1067,14 @@ void run_chunk(Ctx base, Chunk& ch) {
     std::deque<std::array<Details, 2>> dstore;
     Ctx c = base;
     c.ch = &ch;
+    c.r1 = ch.rec1;
+    c.r2 = ch.rec2;
+    c.rel0 = ch.pos;
     Rng rng = ch.rng;
     InsertSize isz = ch.isize;
     bool probing = false;
     for (uint64_t r = ch.next; r < ch.count; ++r) {
-        const uint64_t idx = ch.first + r;
+        const uint64_t idx = r;
         const Rng rng0 = rng;
         const InsertSize isz0 = isz;
         c.rng = &rng;
@@ -1122,6 +1131,198 @@ double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Splits records [0, n) into chunks (src/main.rs:591-604): records until the
+// cumulative bases exceed chunk_size, the crossing record included.
+std::vector<Chunk> make_chunks(const Record* r1, const Record* r2, uint64_t n, uint64_t chunk_size) {
+    std::vector<Chunk> chunks;
+    uint64_t nt = 0, first = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        nt += r1[i].seq.size() + (r2 ? r2[i].seq.size() : 0);
+        if (nt > chunk_size || i + 1 == n) {
+            Chunk c;
+            c.first = first;
+            c.count = i + 1 - first;
+            c.rec1 = r1 + first;
+            c.rec2 = r2 ? r2 + first : nullptr;
+            chunks.push_back(std::move(c));
+            first = i + 1;
+            nt = 0;
+        }
+    }
+    return chunks;
+}
+
+uint64_t chunk_bases(const Chunk& c) {
+    uint64_t b = 0;
+    for (uint64_t i = 0; i < c.count; ++i) b += c.rec1[i].seq.size() + (c.rec2 ? c.rec2[i].seq.size() : 0);
+    return b;
+}
+
+// Where the reads of a device batch come from: staged from the chunks' host
+// records (d_base null), or already in device memory, each chunk's reads
+// (records in order, paired: r1 then r2 of each pair) at d_base + seg[chunk].
+struct BatchReads {
+    const uint8_t* d_base = nullptr;
+    std::vector<uint64_t> seg;
+};
+
+const Scores kScores{2, 8, 12, 1, 10, 500};
+const ChainParams kChain{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+
+// One device batch of whole chunks (of one or several read sets) on one engine:
+// reads to the device, NAMs, then replay passes and extension rounds until every
+// chunk is committed. Each chunk keeps its own output, RNG and insert-size state,
+// so a chunk's result does not depend on the batch it ran in.
+void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired, Chunk* const* chs, size_t nch,
+               const BatchReads& br, RunStats& st, bool keep) {
+    double t0 = now_ms();
+    std::vector<uint64_t> off(1, 0), seg_len(nch, 0);
+    uint64_t nrec = 0;
+    for (size_t x = 0; x < nch; ++x) {
+        Chunk& ch = *chs[x];
+        ch.pos = nrec;
+        nrec += ch.count;
+        const uint64_t s0 = off.back();
+        for (uint64_t i = 0; i < ch.count; ++i) {
+            off.push_back(off.back() + ch.rec1[i].seq.size());
+            if (paired) off.push_back(off.back() + ch.rec2[i].seq.size());
+        }
+        seg_len[x] = off.back() - s0;
+    }
+    const uint32_t n_reads = (uint32_t)(off.size() - 1);
+    if (br.d_base) {
+        eng.gather_reads(br.d_base, br.seg.data(), seg_len.data(), (uint32_t)nch, off.data(), n_reads);
+    } else {
+        std::vector<uint8_t> seq(off.back());
+        uint64_t at = 0;
+        for (size_t x = 0; x < nch; ++x) {
+            const Chunk& ch = *chs[x];
+            for (uint64_t i = 0; i < ch.count; ++i) {
+                std::memcpy(seq.data() + at, ch.rec1[i].seq.data(), ch.rec1[i].seq.size());
+                at += ch.rec1[i].seq.size();
+                if (paired) {
+                    std::memcpy(seq.data() + at, ch.rec2[i].seq.data(), ch.rec2[i].seq.size());
+                    at += ch.rec2[i].seq.size();
+                }
+            }
+        }
+        eng.upload_reads(seq.data(), off.data(), n_reads);
+    }
+    double t1 = now_ms();
+    const MapParams mp{o.mcs, o.rescue_distance};
+    NamBatch nb;
+    eng.find_nams(mp, kChain, nb);
+    double t2 = now_ms();
+    st.upload_ms += t1 - t0;
+    st.nam_ms += t2 - t1;
+    st.nams += nb.nams.size();
+    const uint32_t k = (uint32_t)eng.params().k;
+    Ctx base{o, ref, nb, nullptr, nullptr, paired, 0, k, nullptr, false, false, nullptr, nullptr};
+    for (size_t x = 0; x < nch; ++x) {
+        chs[x]->abundances.assign(ref.names.size(), 0.0);
+        chs[x]->out.reserve(chs[x]->count * (paired ? 1400 : 700));
+    }
+    for (int round = 0;; ++round) {
+        double h0 = now_ms();
+        parallel_for(o.threads, nch, [&](size_t x) {
+            Chunk& ch = *chs[x];
+            if (!ch.done() || round == 0) run_chunk(base, ch);
+        });
+        double h1 = now_ms();
+        st.host_ms += h1 - h0;
+        // requests of all chunks, chunk by chunk (built in parallel)
+        std::vector<uint64_t> roff(nch + 1, 0);
+        for (size_t x = 0; x < nch; ++x) roff[x + 1] = roff[x] + chs[x]->pending.size();
+        const uint64_t nreq = roff[nch];
+        bool all_done = true;
+        for (size_t x = 0; x < nch; ++x) all_done &= chs[x]->done();
+        if (nreq == 0) {
+            if (!all_done) throw std::runtime_error("replay stalled without pending extensions");
+            break;
+        }
+        std::vector<ExtRequest> req(nreq);
+        parallel_for(o.threads, nch, [&](size_t x) {
+            const Chunk& ch = *chs[x];
+            ExtRequest* out_r = req.data() + roff[x];
+            for (size_t y = 0; y < ch.pending.size(); ++y) {
+                const Key& kk = ch.pending[y];
+                ExtRequest r{};
+                r.kind = kk.kind;
+                r.read_id = kk.read;
+                r.flags = kk.flags;
+                r.ref_id = kk.ref_id;
+                if (kk.kind == EXT_SEED) {
+                    r.ref_start = kk.a; r.ref_end = kk.b; r.query_start = kk.c; r.query_end = kk.d;
+                    r.anchor_off = kk.e & ((1ull << 48) - 1);
+                    r.n_anchors = (uint32_t)(kk.e >> 48);
+                } else {
+                    r.win_start = kk.a; r.win_end = kk.b;
+                }
+                out_r[y] = r;
+            }
+        });
+        st.rounds++;
+        st.requests += nreq;
+        std::vector<ExtResult> res;
+        std::vector<uint32_t> cig;
+        std::vector<uint64_t> coff;
+        double g0 = now_ms();
+        st.host_ms += g0 - h1;
+        eng.extend(req.data(), (uint32_t)nreq, kScores, res, cig, coff);
+        double g1 = now_ms();
+        st.extend_ms += g1 - g0;
+        // results into the chunks' caches (each chunk owns its cache and CIGAR pool)
+        parallel_for(o.threads, nch, [&](size_t x) {
+            Chunk& ch = *chs[x];
+            const uint64_t a = roff[x], e = roff[x + 1];
+            ch.cache.reserve(ch.cache.size() + (e - a));
+            ch.cigar_pool.reserve(ch.cigar_pool.size() + (coff[e] - coff[a]));
+            for (uint64_t y = a; y < e; ++y) {
+                CachedResult cr;
+                cr.r = res[y];
+                cr.cigar_off = ch.cigar_pool.size();
+                ch.cigar_pool.insert(ch.cigar_pool.end(), cig.begin() + (ptrdiff_t)coff[y],
+                                     cig.begin() + (ptrdiff_t)coff[y + 1]);
+                ch.cache.emplace(ch.pending[y - a], cr);
+            }
+            ch.pending.clear();
+            ch.pending_set.clear();
+        });
+        st.host_ms += now_ms() - g1;
+    }
+    // the batch's caches are no longer needed; output text stays until it is joined
+    parallel_for(o.threads, nch, [&](size_t x) {
+        Chunk& ch = *chs[x];
+        decltype(ch.cache)().swap(ch.cache);
+        std::vector<uint32_t>().swap(ch.cigar_pool);
+        if (!keep) std::string().swap(ch.out);
+    });
+}
+
+// --aemb table (src/main.rs output_abundances): per contig, the summed weights
+// divided by the contig length.
+void abundance_table(const PackedRef& ref, const std::vector<Chunk>& chunks, std::string& out) {
+    std::vector<double> abundances(ref.names.size(), 0.0);
+    for (const Chunk& c : chunks)
+        for (size_t i = 0; i < abundances.size() && i < c.abundances.size(); ++i) abundances[i] += c.abundances[i];
+    char buf[64];
+    for (size_t i = 0; i < ref.names.size(); ++i) {
+        snprintf(buf, sizeof buf, "\t%.6f\n", abundances[i] / (double)ref.len[i]);
+        out += ref.names[i];
+        out += buf;
+    }
+}
+
+void add_stats(RunStats& to, const RunStats& b) {
+    to.upload_ms += b.upload_ms;
+    to.nam_ms += b.nam_ms;
+    to.extend_ms += b.extend_ms;
+    to.host_ms += b.host_ms;
+    to.requests += b.requests;
+    to.rounds += b.rounds;
+    to.nams += b.nams;
+}
+
 }  // namespace
 
 void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, const std::vector<Record>& r1,
@@ -1131,170 +1332,33 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
     const double t_start = now_ms();
     const bool paired = !r2.empty();
     const uint64_t n = paired ? std::min(r1.size(), r2.size()) : r1.size();
-    // src/main.rs:591-604 chunking
-    std::vector<Chunk> chunks;
-    {
-        uint64_t nt = 0, first = 0;
-        for (uint64_t i = 0; i < n; ++i) {
-            nt += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
-            if (nt > o.chunk_size) {
-                Chunk c;
-                c.first = first;
-                c.count = i + 1 - first;
-                chunks.push_back(std::move(c));
-                first = i + 1;
-                nt = 0;
-            }
-        }
-        if (first < n) {
-            Chunk c;
-            c.first = first;
-            c.count = n - first;
-            chunks.push_back(std::move(c));
-        }
-    }
+    std::vector<Chunk> chunks = make_chunks(r1.data(), paired ? r2.data() : nullptr, n, o.chunk_size);
     // device batches of whole chunks
     std::vector<std::pair<size_t, size_t>> batches;
     for (size_t cb = 0; cb < chunks.size();) {
         size_t ce = cb;
         uint64_t bases = 0;
-        while (ce < chunks.size() && (ce == cb || bases < o.batch_bases)) {
-            for (uint64_t i = chunks[ce].first; i < chunks[ce].first + chunks[ce].count; ++i)
-                bases += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
-            ++ce;
-        }
+        while (ce < chunks.size() && (ce == cb || bases < o.batch_bases)) bases += chunk_bases(chunks[ce++]);
         batches.push_back({cb, ce});
         cb = ce;
     }
-    const Scores sc{2, 8, 12, 1, 10, 500};
-    const MapParams mp{o.mcs, o.rescue_distance};
-    const ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
-    const uint32_t k = (uint32_t)engines[0]->params().k;
     std::vector<RunStats> bst(batches.size());
     const bool keep = out != nullptr || o.mode == 2;
-    // One batch on one engine: reads to the device, NAMs, then replay passes and
-    // extension rounds until every chunk of the batch is committed.
-    auto run_batch = [&](Engine& eng, size_t b) {
+    auto one = [&](Engine& eng, size_t b) {
         const size_t cb = batches[b].first, ce = batches[b].second;
-        RunStats& st = bst[b];
-        const uint64_t bfirst = chunks[cb].first;
-        const uint64_t blast = chunks[ce - 1].first + chunks[ce - 1].count;
-        double t0 = now_ms();
-        if (dreads) {
-            // resident read set: offsets of this batch, relative to its first base
-            const uint64_t r0 = paired ? 2 * bfirst : bfirst, r1x = paired ? 2 * blast : blast;
-            std::vector<uint64_t> off(dreads->off.begin() + (ptrdiff_t)r0, dreads->off.begin() + (ptrdiff_t)r1x + 1);
-            const uint64_t base = off[0];
-            for (auto& v : off) v -= base;
-            eng.use_reads(dreads->d_seq + base, off.data(), (uint32_t)(off.size() - 1));
-        } else {
-            std::vector<uint8_t> seq;
-            std::vector<uint64_t> off(1, 0);
-            uint64_t bases = 0;
-            for (uint64_t i = bfirst; i < blast; ++i) bases += r1[i].seq.size() + (paired ? r2[i].seq.size() : 0);
-            seq.reserve(bases);
-            for (uint64_t i = bfirst; i < blast; ++i) {
-                seq.insert(seq.end(), r1[i].seq.begin(), r1[i].seq.end());
-                off.push_back(seq.size());
-                if (paired) {
-                    seq.insert(seq.end(), r2[i].seq.begin(), r2[i].seq.end());
-                    off.push_back(seq.size());
-                }
-            }
-            eng.upload_reads(seq.data(), off.data(), (uint32_t)(off.size() - 1));
-        }
-        double t1 = now_ms();
-        NamBatch nb;
-        eng.find_nams(mp, cp, nb);
-        double t2 = now_ms();
-        st.upload_ms += t1 - t0;
-        st.nam_ms += t2 - t1;
-        st.nams += nb.nams.size();
-        Ctx base{o, ref, nb, r1, r2, paired, bfirst, k, nullptr, false, false, nullptr, nullptr};
+        std::vector<Chunk*> chs;
+        BatchReads br;
         for (size_t c = cb; c < ce; ++c) {
-            chunks[c].abundances.assign(ref.names.size(), 0.0);
-            chunks[c].out.reserve(chunks[c].count * (paired ? 1400 : 700));
+            chs.push_back(&chunks[c]);
+            if (dreads) br.seg.push_back(dreads->off[paired ? 2 * chunks[c].first : chunks[c].first]);
         }
-        for (int round = 0;; ++round) {
-            double h0 = now_ms();
-            parallel_for(o.threads, ce - cb, [&](size_t x) {
-                Chunk& ch = chunks[cb + x];
-                if (!ch.done() || round == 0) run_chunk(base, ch);
-            });
-            double h1 = now_ms();
-            st.host_ms += h1 - h0;
-            // requests of all chunks, chunk by chunk (built in parallel)
-            std::vector<uint64_t> roff(ce - cb + 1, 0);
-            for (size_t cix = cb; cix < ce; ++cix) roff[cix - cb + 1] = roff[cix - cb] + chunks[cix].pending.size();
-            const uint64_t nreq = roff[ce - cb];
-            bool all_done = true;
-            for (size_t cix = cb; cix < ce; ++cix) all_done &= chunks[cix].done();
-            if (nreq == 0) {
-                if (!all_done) throw std::runtime_error("replay stalled without pending extensions");
-                break;
-            }
-            std::vector<ExtRequest> req(nreq);
-            parallel_for(o.threads, ce - cb, [&](size_t x) {
-                const Chunk& ch = chunks[cb + x];
-                ExtRequest* out_r = req.data() + roff[x];
-                for (size_t y = 0; y < ch.pending.size(); ++y) {
-                    const Key& kk = ch.pending[y];
-                    ExtRequest r{};
-                    r.kind = kk.kind;
-                    r.read_id = kk.read;
-                    r.flags = kk.flags;
-                    r.ref_id = kk.ref_id;
-                    if (kk.kind == EXT_SEED) {
-                        r.ref_start = kk.a; r.ref_end = kk.b; r.query_start = kk.c; r.query_end = kk.d;
-                        r.anchor_off = kk.e & ((1ull << 48) - 1);
-                        r.n_anchors = (uint32_t)(kk.e >> 48);
-                    } else {
-                        r.win_start = kk.a; r.win_end = kk.b;
-                    }
-                    out_r[y] = r;
-                }
-            });
-            st.rounds++;
-            st.requests += nreq;
-            std::vector<ExtResult> res;
-            std::vector<uint32_t> cig;
-            std::vector<uint64_t> coff;
-            double g0 = now_ms();
-            st.host_ms += g0 - h1;
-            eng.extend(req.data(), (uint32_t)nreq, sc, res, cig, coff);
-            double g1 = now_ms();
-            st.extend_ms += g1 - g0;
-            // results into the chunks' caches (each chunk owns its cache and CIGAR pool)
-            parallel_for(o.threads, ce - cb, [&](size_t x) {
-                Chunk& ch = chunks[cb + x];
-                const uint64_t a = roff[x], e = roff[x + 1];
-                ch.cache.reserve(ch.cache.size() + (e - a));
-                ch.cigar_pool.reserve(ch.cigar_pool.size() + (coff[e] - coff[a]));
-                for (uint64_t y = a; y < e; ++y) {
-                    CachedResult cr;
-                    cr.r = res[y];
-                    cr.cigar_off = ch.cigar_pool.size();
-                    ch.cigar_pool.insert(ch.cigar_pool.end(), cig.begin() + (ptrdiff_t)coff[y],
-                                         cig.begin() + (ptrdiff_t)coff[y + 1]);
-                    ch.cache.emplace(ch.pending[y - a], cr);
-                }
-                ch.pending.clear();
-                ch.pending_set.clear();
-            });
-            st.host_ms += now_ms() - g1;
-        }
-        // the batch's caches are no longer needed; output text stays until it is joined
-        parallel_for(o.threads, ce - cb, [&](size_t x) {
-            Chunk& ch = chunks[cb + x];
-            decltype(ch.cache)().swap(ch.cache);
-            std::vector<uint32_t>().swap(ch.cigar_pool);
-            if (!keep) std::string().swap(ch.out);
-        });
+        if (dreads) br.d_base = dreads->d_seq;
+        run_batch(eng, ref, o, paired, chs.data(), chs.size(), br, bst[b], keep);
     };
     // batches are dealt to the engines; each engine works on its own stream, so one
     // batch's device stages overlap another's host replay
     if (engines.size() == 1 || batches.size() <= 1) {
-        for (size_t b = 0; b < batches.size(); ++b) run_batch(*engines[0], b);
+        for (size_t b = 0; b < batches.size(); ++b) one(*engines[0], b);
     } else {
         std::atomic<size_t> next{0};
         std::vector<std::thread> th;
@@ -1302,7 +1366,7 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
         for (size_t e = 0; e < std::min(engines.size(), batches.size()); ++e)
             th.emplace_back([&, e] {
                 try {
-                    for (size_t b; (b = next.fetch_add(1)) < batches.size();) run_batch(*engines[e], b);
+                    for (size_t b; (b = next.fetch_add(1)) < batches.size();) one(*engines[e], b);
                 } catch (...) {
                     err[e] = std::current_exception();
                     next.store(batches.size());
@@ -1323,29 +1387,9 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
             std::string().swap(ch.out);
         });
     }
-    if (o.mode == 2) {
-        std::vector<double> abundances(ref.names.size(), 0.0);
-        for (const Chunk& c : chunks)
-            for (size_t i = 0; i < abundances.size(); ++i) abundances[i] += c.abundances[i];
-        if (out) {
-            char buf[64];
-            for (size_t i = 0; i < ref.names.size(); ++i) {
-                snprintf(buf, sizeof buf, "\t%.6f\n", abundances[i] / (double)ref.len[i]);
-                *out += ref.names[i];
-                *out += buf;
-            }
-        }
-    }
+    if (o.mode == 2 && out) abundance_table(ref, chunks, *out);
     if (stats) {
-        for (const RunStats& b : bst) {
-            stats->upload_ms += b.upload_ms;
-            stats->nam_ms += b.nam_ms;
-            stats->extend_ms += b.extend_ms;
-            stats->host_ms += b.host_ms;
-            stats->requests += b.requests;
-            stats->rounds += b.rounds;
-            stats->nams += b.nams;
-        }
+        for (const RunStats& b : bst) add_stats(*stats, b);
         stats->records += n;
         stats->total_ms += now_ms() - t_start;
     }
@@ -1354,6 +1398,300 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
 void map_records(Engine& eng, const PackedRef& ref, const std::vector<Record>& r1, const std::vector<Record>& r2,
                  const Options& o, std::string& out, RunStats* stats) {
     map_records(std::vector<Engine*>{&eng}, ref, r1, r2, o, &out, stats, nullptr);
+}
+
+// ------------------------------------------------------------- serving ---
+
+struct Server::Impl {
+    struct Req {
+        uint64_t id = 0;
+        std::vector<Record> r1, r2;
+        bool paired = false;
+        std::vector<Chunk> chunks;
+        std::vector<uint64_t> chunk_b;    // bases per chunk
+        std::vector<uint64_t> chunk_seg;  // arena offset of each chunk's reads
+        size_t next_chunk = 0;            // next chunk to hand to a batch
+        size_t left = 0;                  // chunks not finished
+        uint64_t bytes = 0;               // bases of all reads
+        uint64_t dev = 0, dev_size = 0;   // arena block
+        double t_submit = 0, t_admit = 0, t_done = 0;
+        bool done = false, failed = false;
+        std::string out, err;
+    };
+
+    std::vector<Engine*> engines;
+    const PackedRef& ref;
+    Options o;
+    int device;
+    uint8_t* arena = nullptr;
+    uint64_t arena_bytes = 0;
+    std::map<uint64_t, uint64_t> free_blocks;  // offset -> size
+    mutable std::mutex m;
+    std::condition_variable cv_admit, cv_work, cv_done;
+    std::deque<Req*> waiting;  // submitted, reads not yet on the device
+    std::deque<Req*> ready;    // admitted, with chunks not yet in a batch
+    std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    Server::Stats st{};
+    std::thread admitter;
+    std::vector<std::thread> workers;
+
+    Impl(std::vector<Engine*> e, const PackedRef& r, const Options& opt, int dev, uint64_t bytes)
+        : engines(std::move(e)), ref(r), o(opt), device(dev), arena_bytes(bytes) {}
+
+    static uint64_t round_up(uint64_t x) { return (x + 255) & ~(uint64_t)255; }
+
+    // first fit in the arena; false when no free block is large enough
+    bool alloc(uint64_t size, uint64_t& off) {
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < size) continue;
+            off = it->first;
+            const uint64_t rest = it->second - size;
+            free_blocks.erase(it);
+            if (rest) free_blocks[off + size] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t size) {
+        auto it = free_blocks.emplace(off, size).first;
+        auto nx = std::next(it);
+        if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_blocks.erase(nx);
+        }
+        if (it != free_blocks.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_blocks.erase(it);
+            }
+        }
+    }
+    uint64_t arena_used() const {
+        uint64_t f = 0;
+        for (const auto& b : free_blocks) f += b.second;
+        return arena_bytes - f;
+    }
+
+    // Admits requests in arrival order while their reads fit in the arena, and
+    // copies their reads to the device (outside the lock, on its own stream).
+    void admit_loop() {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            uint64_t off = 0;
+            cv_admit.wait(lk, [&] {
+                return stop || (!waiting.empty() && alloc(round_up(waiting.front()->bytes + 16), off));
+            });
+            if (stop) return;
+            Req* r = waiting.front();
+            waiting.pop_front();
+            r->dev = off;
+            r->dev_size = round_up(r->bytes + 16);
+            st.arena_peak = std::max(st.arena_peak, arena_used());
+            lk.unlock();
+            std::string err;
+            try {
+                std::vector<uint8_t> buf(r->bytes);
+                uint64_t at = 0;
+                for (size_t c = 0; c < r->chunks.size(); ++c) {
+                    const Chunk& ch = r->chunks[c];
+                    r->chunk_seg.push_back(r->dev + at);
+                    for (uint64_t i = 0; i < ch.count; ++i) {
+                        std::memcpy(buf.data() + at, ch.rec1[i].seq.data(), ch.rec1[i].seq.size());
+                        at += ch.rec1[i].seq.size();
+                        if (r->paired) {
+                            std::memcpy(buf.data() + at, ch.rec2[i].seq.data(), ch.rec2[i].seq.size());
+                            at += ch.rec2[i].seq.size();
+                        }
+                    }
+                }
+                device_copy_in(device, arena + r->dev, buf.data(), r->bytes);
+            } catch (const std::exception& e) {
+                err = e.what();
+            }
+            lk.lock();
+            r->t_admit = now_ms();
+            if (!err.empty()) {
+                r->failed = true;
+                r->err = err;
+                finish_locked(r);
+            } else {
+                ready.push_back(r);
+                cv_work.notify_all();
+            }
+        }
+    }
+
+    void finish_locked(Req* r) {
+        release(r->dev, r->dev_size);
+        r->dev_size = 0;
+        r->done = true;
+        r->t_done = now_ms();
+        st.completed++;
+        cv_done.notify_all();
+        cv_admit.notify_all();
+    }
+
+    // One engine: forms device batches from the oldest admitted chunks of any
+    // requests (same pairedness), up to batch_bases, and runs them.
+    void work_loop(size_t e) {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            cv_work.wait(lk, [&] { return stop || !ready.empty(); });
+            if (stop) return;
+            std::vector<std::pair<Req*, size_t>> take;
+            uint64_t bases = 0;
+            const bool paired = ready.front()->paired;
+            while (!ready.empty() && ready.front()->paired == paired && (take.empty() || bases < o.batch_bases)) {
+                Req* r = ready.front();
+                const size_t c = r->next_chunk++;
+                take.push_back({r, c});
+                bases += r->chunk_b[c];
+                if (r->next_chunk == r->chunks.size()) ready.pop_front();
+            }
+            size_t nreq = 0;
+            for (size_t i = 0; i < take.size(); ++i) nreq += i == 0 || take[i].first != take[i - 1].first;
+            st.batches++;
+            st.batch_requests += nreq;
+            st.max_requests_per_batch = std::max<uint64_t>(st.max_requests_per_batch, nreq);
+            lk.unlock();
+            std::vector<Chunk*> chs;
+            BatchReads br;
+            br.d_base = arena;
+            for (const auto& t : take) {
+                chs.push_back(&t.first->chunks[t.second]);
+                br.seg.push_back(t.first->chunk_seg[t.second]);
+            }
+            std::string err;
+            RunStats rs;
+            const double t0 = now_ms();
+            try {
+                run_batch(*engines[e], ref, o, paired, chs.data(), chs.size(), br, rs, true);
+            } catch (const std::exception& ex) {
+                err = ex.what();
+            }
+            const double t1 = now_ms();
+            // requests whose last chunk this was: output joined outside the lock
+            std::vector<Req*> fin;
+            lk.lock();
+            st.busy_ms += t1 - t0;
+            for (const auto& t : take) {
+                Req* r = t.first;
+                if (!err.empty() && !r->failed) {
+                    r->failed = true;
+                    r->err = err;
+                }
+                if (--r->left == 0) fin.push_back(r);
+            }
+            lk.unlock();
+            for (Req* r : fin) {
+                if (r->failed) continue;
+                if (o.mode == 2) {
+                    abundance_table(ref, r->chunks, r->out);
+                } else {
+                    uint64_t total = 0;
+                    for (const Chunk& ch : r->chunks) total += ch.out.size();
+                    r->out.reserve(total);
+                    for (Chunk& ch : r->chunks) {
+                        r->out += ch.out;
+                        std::string().swap(ch.out);
+                    }
+                }
+            }
+            lk.lock();
+            for (Req* r : fin) finish_locked(r);
+        }
+    }
+};
+
+Server::Server(std::vector<Engine*> engines, const PackedRef& ref, const Options& o, int device,
+               uint64_t arena_bytes)
+    : impl_(std::make_unique<Impl>(std::move(engines), ref, o, device, arena_bytes)) {
+    if (impl_->engines.empty()) throw std::invalid_argument("Server: no engine");
+    if (arena_bytes < 4096) throw std::invalid_argument("Server: arena smaller than 4 KiB");
+    impl_->arena = (uint8_t*)device_alloc(device, arena_bytes);
+    impl_->free_blocks[0] = arena_bytes;
+    impl_->st.arena_bytes = arena_bytes;
+    impl_->admitter = std::thread([this] { impl_->admit_loop(); });
+    for (size_t e = 0; e < impl_->engines.size(); ++e) impl_->workers.emplace_back([this, e] { impl_->work_loop(e); });
+}
+
+Server::~Server() {
+    {
+        std::lock_guard<std::mutex> lk(impl_->m);
+        impl_->stop = true;
+    }
+    impl_->cv_admit.notify_all();
+    impl_->cv_work.notify_all();
+    impl_->admitter.join();
+    for (auto& t : impl_->workers) t.join();
+    device_free(impl_->device, impl_->arena);
+}
+
+double steady_ms() { return now_ms(); }
+
+uint64_t Server::submit(std::vector<Record> r1, std::vector<Record> r2, double t_arrival) {
+    const double t_in = t_arrival >= 0 ? t_arrival : now_ms();
+    auto r = std::make_unique<Impl::Req>();
+    r->paired = !r2.empty();
+    if (r->paired && r1.size() != r2.size()) throw std::invalid_argument("submit: mates differ in count");
+    r->r1 = std::move(r1);
+    r->r2 = std::move(r2);
+    const uint64_t n = r->r1.size();
+    r->chunks = make_chunks(r->r1.data(), r->paired ? r->r2.data() : nullptr, n, impl_->o.chunk_size);
+    for (const Chunk& c : r->chunks) {
+        r->chunk_b.push_back(chunk_bases(c));
+        r->bytes += r->chunk_b.back();
+    }
+    if (Impl::round_up(r->bytes + 16) > impl_->arena_bytes)
+        throw std::invalid_argument("submit: request reads (" + std::to_string(r->bytes) +
+                                    " bases) exceed the device arena");
+    r->left = r->chunks.size();
+    std::lock_guard<std::mutex> lk(impl_->m);
+    r->id = impl_->next_id++;
+    r->t_submit = t_in;
+    Impl::Req* p = r.get();
+    impl_->reqs.emplace(p->id, std::move(r));
+    impl_->st.submitted++;
+    if (p->chunks.empty()) {
+        // no records: nothing to map (an abundance table still lists every contig)
+        if (impl_->o.mode == 2) abundance_table(impl_->ref, p->chunks, p->out);
+        p->t_admit = p->t_submit;
+        p->dev_size = 0;
+        p->done = true;
+        p->t_done = p->t_submit;
+        impl_->st.completed++;
+        impl_->cv_done.notify_all();
+    } else {
+        impl_->waiting.push_back(p);
+        impl_->cv_admit.notify_all();
+    }
+    return p->id;
+}
+
+bool Server::wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err) {
+    std::unique_lock<std::mutex> lk(impl_->m);
+    auto it = impl_->reqs.find(ticket);
+    if (it == impl_->reqs.end()) {
+        err = "unknown ticket " + std::to_string(ticket);
+        return false;
+    }
+    Impl::Req* r = it->second.get();
+    impl_->cv_done.wait(lk, [&] { return r->done; });
+    t.admit_ms = r->t_admit - r->t_submit;
+    t.done_ms = r->t_done - r->t_submit;
+    const bool ok = !r->failed;
+    if (ok) out.swap(r->out);
+    else err = r->err;
+    impl_->reqs.erase(it);
+    return ok;
+}
+
+Server::Stats Server::stats() const {
+    std::lock_guard<std::mutex> lk(impl_->m);
+    return impl_->st;
 }
 
 }  // namespace sa
