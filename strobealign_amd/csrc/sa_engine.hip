@@ -43,7 +43,7 @@ __global__ void k_compact_cigars(const ExtRequest* __restrict__ req, const ExtRe
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 enum Slot {
-    S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_DP, S_PRED, S_CAND, S_USED,
+    S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
     S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_COUNT
 };
@@ -363,6 +363,7 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     scan_u64(d_acap, d_aoff, 2ull * n, &A);
     ChainBufs b;
     b.anchors = (Anchor*)grow(S_ANCH, (A + 1) * sizeof(Anchor));
+    b.anchors_tmp = (Anchor*)grow(S_ANCH2, (A + 1) * sizeof(Anchor));
     b.dp = (float*)grow(S_DP, (A + 1) * 4);
     b.pred = (uint32_t*)grow(S_PRED, (A + 1) * 4);
     b.cand = (uint32_t*)grow(S_CAND, (A + 1) * 4);

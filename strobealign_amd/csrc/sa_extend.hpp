@@ -209,61 +209,137 @@ SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
     for (; x < n; ++x) p[x] = 0;
 }
 
+// One pass of the banded DP of ext/ssw/ssw.c:605-660 for band width bw <= BWMAX
+// with the three band rows (previous H, E carried in place, current H) in
+// registers: entry u of row i holds column j = x_i + u - 1, x_i = max(i - bw, 0),
+// so the previous row's entries for column j and j - 1 sit at u + (x_i - x_{i-1})
+// and one below, and every index is static once the cell loop is unrolled over u.
+// Writes the same direction bytes and returns the same maximum as the scratch form
+// (band_pass_mem), including its resets of the row entries at 0 and `edge`.
+template <int BWMAX, typename QC, typename RC>
+SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, int32_t bw, int8_t* dir) {
+    constexpr int WM = 2 * BWMAX + 3;  // largest row width
+    constexpr int UM = 2 * BWMAX + 1;  // largest number of cells in a row
+    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
+    const int32_t width = bw * 2 + 3, width_d = bw * 2 + 1;
+    int32_t hb[WM + 1], eb[WM + 1], hc[WM + 1];
+    SA_UNROLL
+    for (int x = 0; x <= WM; ++x) hb[x] = eb[x] = hc[x] = 0;
+    int32_t maxv = 0, xprev = 0;
+    for (int32_t i = 0; i < readLen; ++i) {
+        const int32_t xi = i - bw > 0 ? i - bw : 0;
+        const bool shift = xi != xprev;
+        xprev = xi;
+        const int32_t end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
+        const int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
+        hb[0] = eb[0] = hc[0] = 0;
+        SA_UNROLL
+        for (int x = 1; x < WM; ++x)
+            if (x == edge) hb[x] = eb[x] = 0;
+        const int32_t uend = end - xi + 1;
+        int8_t* dl = dir + (uint64_t)width_d * i * 3;
+        const int32_t qi = qc(i);
+        int32_t f = 0;
+        SA_UNROLL
+        for (int u = 1; u <= UM; ++u) {
+            if (u <= uend) {
+                const int32_t hbe = shift ? hb[u + 1] : hb[u];
+                const int32_t ebe = shift ? eb[u + 1] : eb[u];
+                const int32_t hbd = shift ? hb[u] : hb[u - 1];
+                int32_t t1 = i == 0 ? -gapO : hbe - gapO;
+                int32_t t2 = i == 0 ? -gapE : ebe - gapE;
+                const int32_t ev = t1 > t2 ? t1 : t2;
+                eb[u] = ev;
+                const int8_t de = t1 > t2 ? 3 : 2;
+                t1 = hc[u - 1] - gapO;
+                t2 = f - gapE;
+                f = t1 > t2 ? t1 : t2;
+                const int8_t df = t1 > t2 ? 5 : 4;
+                const int32_t e1 = ev > 0 ? ev : 0, f1 = f > 0 ? f : 0;
+                t1 = e1 > f1 ? e1 : f1;
+                t2 = hbd + ssw_mat(rc(xi + u - 1), qi, sc);
+                const int32_t h = t1 > t2 ? t1 : t2;
+                hc[u] = h;
+                maxv = h > maxv ? h : maxv;
+                dl[(u - 1) * 3] = de;
+                dl[(u - 1) * 3 + 1] = df;
+                dl[(u - 1) * 3 + 2] = t1 <= t2 ? (int8_t)1 : (e1 > f1 ? de : df);
+            }
+        }
+        SA_UNROLL
+        for (int u = 1; u <= UM; ++u)
+            if (u <= uend) hb[u] = hc[u];
+    }
+    return maxv;
+}
+
+// The same pass with the band rows in scratch memory (any band width).
+template <typename QC, typename RC>
+SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, int32_t bw,
+                            int32_t* hb, int32_t* eb, int32_t* hc, int8_t* dir) {
+    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
+    const int32_t width = bw * 2 + 3, width_d = bw * 2 + 1;
+    auto U = [&](int32_t i, int32_t j) { int32_t x = i - bw; x = x > 0 ? x : 0; return j - x + 1; };
+    auto D = [&](int32_t i, int32_t j, int32_t p) { int32_t x = i - bw; x = x > 0 ? x : 0; return (j - x) * 3 + p; };
+    int32_t maxv = 0;
+    for (int32_t x = 0; x <= width; ++x) hb[x] = eb[x] = hc[x] = 0;
+    for (int32_t i = 0; i < readLen; ++i) {
+        int32_t beg = i - bw > 0 ? i - bw : 0, end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
+        int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
+        int32_t f = 0;
+        hb[0] = eb[0] = hb[edge] = eb[edge] = hc[0] = 0;
+        int8_t* dl = dir + (uint64_t)width_d * i * 3;
+        int32_t u = 0;
+        const int32_t qi = qc(i);
+        for (int32_t j = beg; j <= end; ++j) {
+            u = U(i, j);
+            int32_t e = U(i - 1, j), b = U(i, j - 1), d = U(i - 1, j - 1);
+            int32_t de = D(i, j, 0), df = D(i, j, 1), dh = D(i, j, 2);
+            int32_t t1 = i == 0 ? -gapO : hb[e] - gapO;
+            int32_t t2 = i == 0 ? -gapE : eb[e] - gapE;
+            eb[u] = t1 > t2 ? t1 : t2;
+            dl[de] = t1 > t2 ? 3 : 2;
+            t1 = hc[b] - gapO;
+            t2 = f - gapE;
+            f = t1 > t2 ? t1 : t2;
+            dl[df] = t1 > t2 ? 5 : 4;
+            int32_t e1 = eb[u] > 0 ? eb[u] : 0, f1 = f > 0 ? f : 0;
+            t1 = e1 > f1 ? e1 : f1;
+            t2 = hb[d] + ssw_mat(rc(j), qi, sc);
+            hc[u] = t1 > t2 ? t1 : t2;
+            if (hc[u] > maxv) maxv = hc[u];
+            if (t1 <= t2) dl[dh] = 1;
+            else dl[dh] = e1 > f1 ? dl[de] : dl[df];
+        }
+        for (int32_t j = 1; j <= u; ++j) hb[j] = hc[j];
+    }
+    return maxv;
+}
+
 // Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774).
 // Emits M/I/D ops in forward order through `emit(op, len)`. Returns 0 ok,
 // 1 traceback failure, 2 scratch too small.
 template <typename QC, typename RC, typename Emit>
 SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score, const Scores& sc, int32_t bw,
                     uint8_t* scratch, uint64_t scratch_bytes, Emit emit) {
-    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
     int32_t len = refLen > readLen ? refLen : readLen;
     int32_t maxv = 0;
     int32_t width = 0, width_d = 0;
-    int32_t *hb = nullptr, *eb = nullptr, *hc = nullptr;
     int8_t* dir = nullptr;
-    auto U = [&](int32_t i, int32_t j) { int32_t x = i - bw; x = x > 0 ? x : 0; return j - x + 1; };
     auto D = [&](int32_t i, int32_t j, int32_t p) { int32_t x = i - bw; x = x > 0 ? x : 0; return (j - x) * 3 + p; };
     do {
         width = bw * 2 + 3;
         width_d = bw * 2 + 1;
         uint64_t need = 3ull * (uint64_t)(width + 1) * 4 + (uint64_t)width_d * (uint64_t)readLen * 3 + 3;
         if (need > scratch_bytes) return 2;
-        hb = (int32_t*)scratch;
-        eb = hb + (width + 1);
-        hc = eb + (width + 1);
+        int32_t* hb = (int32_t*)scratch;
+        int32_t* eb = hb + (width + 1);
+        int32_t* hc = eb + (width + 1);
         dir = (int8_t*)(hc + (width + 1));
-        for (int32_t x = 0; x <= width; ++x) hb[x] = eb[x] = hc[x] = 0;
         zero_bytes((uint8_t*)dir, (uint64_t)width_d * readLen * 3 + 3);
-        for (int32_t i = 0; i < readLen; ++i) {
-            int32_t beg = i - bw > 0 ? i - bw : 0, end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
-            int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
-            int32_t f = 0;
-            hb[0] = eb[0] = hb[edge] = eb[edge] = hc[0] = 0;
-            int8_t* dl = dir + (uint64_t)width_d * i * 3;
-            int32_t u = 0;
-            const int32_t qi = qc(i);
-            for (int32_t j = beg; j <= end; ++j) {
-                u = U(i, j);
-                int32_t e = U(i - 1, j), b = U(i, j - 1), d = U(i - 1, j - 1);
-                int32_t de = D(i, j, 0), df = D(i, j, 1), dh = D(i, j, 2);
-                int32_t t1 = i == 0 ? -gapO : hb[e] - gapO;
-                int32_t t2 = i == 0 ? -gapE : eb[e] - gapE;
-                eb[u] = t1 > t2 ? t1 : t2;
-                dl[de] = t1 > t2 ? 3 : 2;
-                t1 = hc[b] - gapO;
-                t2 = f - gapE;
-                f = t1 > t2 ? t1 : t2;
-                dl[df] = t1 > t2 ? 5 : 4;
-                int32_t e1 = eb[u] > 0 ? eb[u] : 0, f1 = f > 0 ? f : 0;
-                t1 = e1 > f1 ? e1 : f1;
-                t2 = hb[d] + ssw_mat(rc(j), qi, sc);
-                hc[u] = t1 > t2 ? t1 : t2;
-                if (hc[u] > maxv) maxv = hc[u];
-                if (t1 <= t2) dl[dh] = 1;
-                else dl[dh] = e1 > f1 ? dl[de] : dl[df];
-            }
-            for (int32_t j = 1; j <= u; ++j) hb[j] = hc[j];
-        }
+        if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, bw, dir);
+        else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, bw, dir);
+        else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, bw, hb, eb, hc, dir);
         bw *= 2;
     } while (maxv < score && bw <= len);
     bw /= 2;

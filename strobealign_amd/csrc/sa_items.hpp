@@ -39,6 +39,7 @@ SA_HD void item_hits(uint32_t t, IndexView ix, MapParams mp, const uint64_t* off
 // -------------------------------------------------------------- chaining ---
 struct ChainBufs {
     Anchor* anchors;
+    Anchor* anchors_tmp;  // second buffer of the anchor sort
     float* dp;
     uint32_t* pred;
     uint32_t* cand;
@@ -72,10 +73,11 @@ SA_HD void item_chain(uint32_t t, IndexView ix, ChainParams cp, const uint64_t* 
     uint64_t base = 2 * ro + (uint64_t)o * L;
     uint64_t ab = aoff[t];
     ChainScratch s{b.anchors + ab, b.dp + ab, b.pred + ab, b.cand + ab, b.used + ab, (uint32_t)acap[t]};
-    uint32_t entries = 0, n_chain = 0;
-    uint32_t na = hits_to_anchors(ix, hits + base, hc[t].n_hits, s.anchors, &entries);
-    anchors_raw[t] = na;
-    na = sort_dedup_anchors(s.anchors, na);
+    uint32_t entries = 0, n_chain = 0, ns = 0, ne = 0;
+    Anchor* tmp = b.anchors_tmp + ab;
+    hits_to_anchors(ix, hits + base, hc[t].n_hits, s.anchors, &ns, tmp, &ne, &entries);
+    anchors_raw[t] = ns + ne;
+    const uint32_t na = sort_dedup_anchors(s.anchors, ns, tmp, ne, &s.anchors);
     const uint32_t nn = chain_orientation(s, na, (uint32_t)ix.p.k, L, o == 1, cp, log2tab, same_diag, b.nams + ab,
                                           b.chain_anchors + 2 * (2 * ab), 2 * ab, &n_chain);
     nam_n[t] = nn;

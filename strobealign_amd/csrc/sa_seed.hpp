@@ -597,47 +597,43 @@ SA_HD void heap_sort(T* a, uint32_t n, Less less) {
 // Anchors of one hit (src/chainer.rs:304-384): a full hit walks its run of
 // equal hashes and keeps the entries whose span length differs from the query
 // span by no more than the smallest difference so far (in table order), two
-// anchors each; a partial hit gives one anchor per entry. Writes to `a` unless it
-// is null; returns the number of anchors and adds the entries read to *entries.
-SA_HD uint32_t hit_anchors(const IndexView& ix, const Hit& h, Anchor* a, uint32_t* entries) {
-    if (h.run_start == NONE64) return 0;
+// anchors each; a partial hit gives one anchor per entry. Start anchors go to
+// s[*ns], end anchors to e[*ne] (hits arrive in query order, so each sequence is
+// nearly sorted by reference position on its own); adds the entries read to
+// *entries.
+SA_HD void hit_anchors(const IndexView& ix, const Hit& h, Anchor* s, uint32_t* ns, Anchor* e, uint32_t* ne,
+                       uint32_t* entries) {
+    if (h.run_start == NONE64) return;
     const uint32_t k = (uint32_t)ix.p.k;
     *entries += h.run_len;
-    uint32_t n = 0;
     if (h.flags & HIT_PARTIAL) {
         for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-            const RefEntry& e = ix.e[j];
-            if (a) a[n] = Anchor{((uint64_t)e.ref_index << 32) | e.position, h.query_start, 0};
-            ++n;
+            const RefEntry& x = ix.e[j];
+            s[(*ns)++] = Anchor{((uint64_t)x.ref_index << 32) | x.position, h.query_start, 0};
         }
     } else {
         uint32_t min_diff = 0xFFFFFFFFu;
         const uint32_t ql = h.query_end - h.query_start;
         for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-            const RefEntry& e = ix.e[j];
-            const uint32_t rs = e.position, re = rs + (uint32_t)(e.hash_offset & 0xFF) + k;
+            const RefEntry& x = ix.e[j];
+            const uint32_t rs = x.position, re = rs + (uint32_t)(x.hash_offset & 0xFF) + k;
             const uint32_t rl = re - rs;
             const uint32_t diff = ql > rl ? ql - rl : rl - ql;
             if (diff <= min_diff) {
-                if (a) {
-                    a[n] = Anchor{((uint64_t)e.ref_index << 32) | rs, h.query_start, 0};
-                    a[n + 1] = Anchor{((uint64_t)e.ref_index << 32) | (re - k), h.query_end - k, 0};
-                }
-                n += 2;
+                s[(*ns)++] = Anchor{((uint64_t)x.ref_index << 32) | rs, h.query_start, 0};
+                e[(*ne)++] = Anchor{((uint64_t)x.ref_index << 32) | (re - k), h.query_end - k, 0};
                 min_diff = diff;
             }
         }
     }
-    return n;
 }
 
-// Hits -> anchors for one read orientation; returns the count.
-SA_HD uint32_t hits_to_anchors(const IndexView& ix, const Hit* hits, uint32_t nh, Anchor* a,
-                                uint32_t* entries_read = nullptr) {
-    uint32_t n = 0, entries = 0;
-    for (uint32_t i = 0; i < nh; ++i) n += hit_anchors(ix, hits[i], a + n, &entries);
-    if (entries_read) *entries_read = entries;
-    return n;
+// Hits -> anchors for one read orientation: start anchors in s, end anchors in e.
+SA_HD void hits_to_anchors(const IndexView& ix, const Hit* hits, uint32_t nh, Anchor* s, uint32_t* ns, Anchor* e,
+                           uint32_t* ne, uint32_t* entries) {
+    *ns = *ne = 0;
+    *entries = 0;
+    for (uint32_t i = 0; i < nh; ++i) hit_anchors(ix, hits[i], s, ns, e, ne, entries);
 }
 
 // src/chainer.rs:286-302; log2(dd+1) comes from a host-filled table (host libm).
@@ -769,13 +765,59 @@ SA_HD uint32_t chain_orientation(const ChainScratch& s, uint32_t n, uint32_t k, 
     return nn;
 }
 
-// Sort + dedup anchors in place (src/chainer.rs:228-229); returns the new count.
-SA_HD uint32_t sort_dedup_anchors(Anchor* a, uint32_t n) {
-    heap_sort(a, n, anchor_less);
+SA_HD bool anchors_ascending(const Anchor* a, uint32_t n) {
+    for (uint32_t i = 1; i < n; ++i)
+        if (anchor_less(a[i], a[i - 1])) return false;
+    return true;
+}
+
+// Sort + dedup anchors (src/chainer.rs:228-229): the order is total up to equal
+// anchors, which dedup removes, so any sort gives the reference's result. Input:
+// start anchors a[0, ns) and end anchors tmp[0, ne). When both are ascending (a
+// read with one placement per hit), they are merged back to front into a[0, ns +
+// ne), which never overwrites an unread start anchor. Otherwise a natural merge
+// sort: ascending runs are merged pairwise, ping-ponging between a and tmp, so
+// every pass reads and writes each array front to back. The deduplicated anchors
+// end in *out (a or tmp); returns their count.
+SA_HD uint32_t sort_dedup_anchors(Anchor* a, uint32_t ns, Anchor* tmp, uint32_t ne, Anchor** out) {
+    uint32_t n = ns + ne;
+    Anchor* src = a;
+    Anchor* dst = tmp;
+    if (anchors_ascending(a, ns) && anchors_ascending(tmp, ne)) {
+        uint32_t x = ns, y = ne, w = n;
+        while (y > 0) a[--w] = (x > 0 && anchor_less(tmp[y - 1], a[x - 1])) ? a[--x] : tmp[--y];
+    } else {
+        for (uint32_t i = 0; i < ne; ++i) a[ns + i] = tmp[i];
+        while (n > 1) {
+            uint32_t runs = 0;
+            for (uint32_t i = 0; i < n;) {
+                uint32_t m = i + 1;
+                while (m < n && !anchor_less(src[m], src[m - 1])) ++m;
+                if (m == n && runs == 0) break;  // a single run: sorted
+                uint32_t e = m;
+                if (m < n) {
+                    ++e;
+                    while (e < n && !anchor_less(src[e], src[e - 1])) ++e;
+                }
+                uint32_t x = i, y = m, w = i;
+                while (x < m && y < e) dst[w++] = anchor_less(src[y], src[x]) ? src[y++] : src[x++];
+                while (x < m) dst[w++] = src[x++];
+                while (y < e) dst[w++] = src[y++];
+                ++runs;
+                i = e;
+            }
+            if (runs == 0) break;
+            Anchor* t = src;
+            src = dst;
+            dst = t;
+            if (runs == 1) break;
+        }
+    }
+    *out = src;
     if (n == 0) return 0;
     uint32_t m = 1;
     for (uint32_t i = 1; i < n; ++i)
-        if (a[i].rk != a[m - 1].rk || a[i].q != a[m - 1].q) a[m++] = a[i];
+        if (src[i].rk != src[m - 1].rk || src[i].q != src[m - 1].q) src[m++] = src[i];
     return m;
 }
 
