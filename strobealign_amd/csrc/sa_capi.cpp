@@ -157,6 +157,11 @@ sa::Record make_record(const char* name, const char* seq, const char* qual) {
 }
 }  // namespace
 
+namespace sa {
+// error text for sa_last_error from entry points outside this file (the SSW shim)
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace sa
+
 extern "C" {
 
 size_t sa_last_error(char* buf, size_t cap) {

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -91,6 +92,58 @@ uint64_t device_free_bytes(int device) {
     size_t fr = 0, tot = 0;
     SA_CHECK(hipMemGetInfo(&fr, &tot));
     return fr;
+}
+
+void ssw_raw_device(int device, const int8_t* read, int32_t readLen, const int8_t* ref, int32_t refLen,
+                    const int8_t* mat, int32_t n, int32_t score_size, int32_t gapO, int32_t gapE, uint32_t flag,
+                    uint32_t filters, int32_t filterd, int32_t maskLen, SswRaw& out, std::vector<uint32_t>& cigar) {
+    // one device context for all shim calls of the process, used one call at a time
+    struct Ctx {
+        int dev = -1;
+        hipStream_t st = nullptr;
+        uint8_t* buf = nullptr;
+        uint64_t cap = 0;
+    };
+    static std::mutex mu;
+    static Ctx c;
+    std::lock_guard<std::mutex> lk(mu);
+    SA_CHECK(hipSetDevice(device));
+    if (c.dev != device) {
+        if (c.st) (void)hipStreamDestroy(c.st);
+        if (c.buf) (void)hipFree(c.buf);
+        c = Ctx{};
+        SA_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+        c.dev = device;
+    }
+    const uint64_t cig_cap = (uint64_t)readLen + (uint64_t)refLen + 4;
+    auto al = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+    const uint64_t o_read = 0, o_ref = al(o_read + (uint64_t)readLen + 1), o_mat = al(o_ref + (uint64_t)refLen + 1);
+    const uint64_t o_out = al(o_mat + (uint64_t)(n * n) + 1), o_cig = al(o_out + sizeof(SswRaw));
+    const uint64_t o_scr = al(o_cig + 4 * cig_cap);
+    const uint64_t scr = ssw_raw_scratch(readLen, refLen);
+    const uint64_t need = o_scr + scr;
+    if (need > c.cap) {
+        if (c.buf) SA_CHECK(hipFree(c.buf));
+        c.buf = nullptr;
+        c.cap = 0;
+        SA_CHECK(hipMalloc((void**)&c.buf, need));
+        c.cap = need;
+    }
+    SA_CHECK(hipMemcpyAsync(c.buf + o_read, read, (uint64_t)readLen, hipMemcpyHostToDevice, c.st));
+    SA_CHECK(hipMemcpyAsync(c.buf + o_ref, ref, (uint64_t)refLen, hipMemcpyHostToDevice, c.st));
+    SA_CHECK(hipMemcpyAsync(c.buf + o_mat, mat, (uint64_t)(n * n), hipMemcpyHostToDevice, c.st));
+    k_ssw_raw<<<1, 64, 0, c.st>>>((const int8_t*)(c.buf + o_read), readLen, (const int8_t*)(c.buf + o_ref), refLen,
+                                  (const int8_t*)(c.buf + o_mat), n, score_size, gapO, gapE, flag, filters, filterd,
+                                  maskLen, c.buf + o_scr, scr, (uint32_t*)(c.buf + o_cig), (uint32_t)cig_cap,
+                                  (SswRaw*)(c.buf + o_out));
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipMemcpyAsync(&out, c.buf + o_out, sizeof(SswRaw), hipMemcpyDeviceToHost, c.st));
+    SA_CHECK(hipStreamSynchronize(c.st));
+    cigar.assign((size_t)(out.status == 0 ? out.n_cigar : 0), 0u);
+    if (!cigar.empty()) {
+        SA_CHECK(hipMemcpyAsync(cigar.data(), c.buf + o_cig, cigar.size() * 4, hipMemcpyDeviceToHost, c.st));
+        SA_CHECK(hipStreamSynchronize(c.st));
+    }
 }
 
 __global__ void __launch_bounds__(256) k_hbm_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {

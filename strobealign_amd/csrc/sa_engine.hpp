@@ -33,6 +33,13 @@ void* device_alloc(int device, uint64_t bytes);
 // Host -> device copy on a stream of the calling thread; returns when it is done.
 void device_copy_in(int device, void* dst, const void* src, uint64_t bytes);
 uint64_t device_free_bytes(int device);
+
+struct SswRaw;
+// ssw_align of one pair on the device (the SSW link shim): codes, an n x n
+// matrix and ssw_align's arguments in; its result and M/I/D CIGAR out.
+void ssw_raw_device(int device, const int8_t* read, int32_t readLen, const int8_t* ref, int32_t refLen,
+                    const int8_t* mat, int32_t n, int32_t score_size, int32_t gapO, int32_t gapE, uint32_t flag,
+                    uint32_t filters, int32_t filterd, int32_t maskLen, SswRaw& out, std::vector<uint32_t>& cigar);
 // Copies `bytes` between two device buffers `iters` times with a plain vector-copy
 // kernel; returns the achieved read+write GB/s (the HBM reference of the roofline).
 double hbm_copy_probe(int device, uint64_t bytes, int iters);

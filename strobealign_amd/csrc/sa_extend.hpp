@@ -94,6 +94,19 @@ SA_HD int32_t ssw_code_comp(uint8_t c) {
 }
 SA_HD int32_t ssw_mat(int32_t a, int32_t b, const Scores& sc) { return (a == b && a < 4) ? sc.match : -sc.mismatch; }
 
+// Substitution scores of the SSW passes, called as score(reference code, query
+// code): the reference's 5x5 match/mismatch matrix (src/ssw.rs:139-149), or an
+// n x n matrix as ssw_init takes it (row = reference code, ext/ssw/ssw.c:179, 656).
+struct MatchMismatch {
+    int32_t match, mismatch;
+    SA_HD int32_t operator()(int32_t r, int32_t q) const { return (r == q && r < 4) ? match : -mismatch; }
+};
+struct MatrixScore {
+    const int8_t* m;
+    int32_t n;
+    SA_HD int32_t operator()(int32_t r, int32_t q) const { return m[r * n + q]; }
+};
+
 // ------------------------------------------------------------------ SSW ---
 struct SswEnd {
     int32_t score, ref, read;
@@ -104,11 +117,15 @@ struct SswEnd {
 // is `qcode(j)` for j < qlen; the reference index runs forward over [0, rlen)
 // or, for the reverse pass, backward from rlen-1 to 0. H/E/Hmax live in
 // `buf` (4 * segLen * LANES ints).
-template <int LANES, bool BYTE, typename QC, typename RC>
-SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool reverse, const Scores& sc, int32_t terminate,
-                      int32_t* buf) {
-    const int32_t bias = BYTE ? sc.mismatch : 0;
-    const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
+// `bias` is the byte form's offset (minus the smallest substitution score); when
+// `maxcol` is set it receives each processed column's maximum (0 for columns the
+// pass did not reach), from which ssw_align picks its second-best end.
+template <int LANES, bool BYTE, typename QC, typename RC, typename SM>
+SA_HD SswEnd ssw_pass_sm(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool reverse, int32_t gapO, int32_t gapE,
+                         int32_t byte_bias, const SM& score, int32_t terminate, int32_t* buf, int32_t* maxcol) {
+    const int32_t bias = BYTE ? byte_bias : 0;
+    if (maxcol)
+        for (int32_t i = 0; i < rlen; ++i) maxcol[i] = 0;
     const int32_t segLen = (qlen + LANES - 1) / LANES;
     const int32_t W = segLen * LANES;
     int32_t* HS = buf;
@@ -132,7 +149,7 @@ SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool rever
         for (int32_t j = 0; j < segLen; ++j) {
             for (int l = 0; l < LANES; ++l) {
                 int32_t qi = j + l * segLen;
-                int32_t prof = qi >= qlen ? 0 : ssw_mat(rc, qcode(qi), sc);
+                int32_t prof = qi >= qlen ? 0 : score(rc, qcode(qi));
                 int32_t h = BYTE ? subs(sat(vH[l] + prof + bias), bias) : sat(vH[l] + prof);
                 int32_t e = E[j * LANES + l];
                 h = h > e ? h : e;
@@ -182,6 +199,7 @@ SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool rever
         }
         int32_t colmax = 0;
         for (int l = 0; l < LANES; ++l) colmax = colmax > vMaxCol[l] ? colmax : vMaxCol[l];
+        if (maxcol) maxcol[i] = colmax;
         if (colmax == terminate) break;
     }
     for (int32_t idx = 0; idx < W; ++idx) {
@@ -195,6 +213,13 @@ SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool rever
     r.ref = end_ref;
     r.read = end_read;
     return r;
+}
+
+template <int LANES, bool BYTE, typename QC, typename RC>
+SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool reverse, const Scores& sc, int32_t terminate,
+                      int32_t* buf) {
+    return ssw_pass_sm<LANES, BYTE>(qcode, qlen, rcode, rlen, reverse, sc.gap_open, sc.gap_extend, sc.mismatch,
+                                    MatchMismatch{sc.match, sc.mismatch}, terminate, buf, nullptr);
 }
 
 // Clears n bytes with 16-byte stores where aligned.
@@ -216,8 +241,9 @@ SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
 // and one below, and every index is static once the cell loop is unrolled over u.
 // Writes the same direction bytes and returns the same maximum as the scratch form
 // (band_pass_mem), including its resets of the row entries at 0 and `edge`.
-template <int BWMAX, typename QC, typename RC>
-SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, int32_t bw, int8_t* dir) {
+template <int BWMAX, typename QC, typename RC, typename SM>
+SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, const SM& score,
+                            int32_t bw, int8_t* dir) {
     constexpr int WM = 2 * BWMAX + 3;  // largest row width
     constexpr int UM = 2 * BWMAX + 1;  // largest number of cells in a row
     const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
@@ -257,7 +283,7 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
                 const int8_t df = t1 > t2 ? 5 : 4;
                 const int32_t e1 = ev > 0 ? ev : 0, f1 = f > 0 ? f : 0;
                 t1 = e1 > f1 ? e1 : f1;
-                t2 = hbd + ssw_mat(rc(xi + u - 1), qi, sc);
+                t2 = hbd + score(rc(xi + u - 1), qi);
                 const int32_t h = t1 > t2 ? t1 : t2;
                 hc[u] = h;
                 maxv = h > maxv ? h : maxv;
@@ -274,9 +300,9 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
 }
 
 // The same pass with the band rows in scratch memory (any band width).
-template <typename QC, typename RC>
-SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, int32_t bw,
-                            int32_t* hb, int32_t* eb, int32_t* hc, int8_t* dir) {
+template <typename QC, typename RC, typename SM>
+SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, const SM& score,
+                            int32_t bw, int32_t* hb, int32_t* eb, int32_t* hc, int8_t* dir) {
     const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
     const int32_t width = bw * 2 + 3, width_d = bw * 2 + 1;
     auto U = [&](int32_t i, int32_t j) { int32_t x = i - bw; x = x > 0 ? x : 0; return j - x + 1; };
@@ -305,7 +331,7 @@ SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const
             dl[df] = t1 > t2 ? 5 : 4;
             int32_t e1 = eb[u] > 0 ? eb[u] : 0, f1 = f > 0 ? f : 0;
             t1 = e1 > f1 ? e1 : f1;
-            t2 = hb[d] + ssw_mat(rc(j), qi, sc);
+            t2 = hb[d] + score(rc(j), qi);
             hc[u] = t1 > t2 ? t1 : t2;
             if (hc[u] > maxv) maxv = hc[u];
             if (t1 <= t2) dl[dh] = 1;
@@ -319,9 +345,9 @@ SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const
 // Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774).
 // Emits M/I/D ops in forward order through `emit(op, len)`. Returns 0 ok,
 // 1 traceback failure, 2 scratch too small.
-template <typename QC, typename RC, typename Emit>
-SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score, const Scores& sc, int32_t bw,
-                    uint8_t* scratch, uint64_t scratch_bytes, Emit emit) {
+template <typename QC, typename RC, typename Emit, typename SM>
+SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score, const Scores& sc, const SM& sm,
+                    int32_t bw, uint8_t* scratch, uint64_t scratch_bytes, Emit emit) {
     int32_t len = refLen > readLen ? refLen : readLen;
     int32_t maxv = 0;
     int32_t width = 0, width_d = 0;
@@ -337,9 +363,9 @@ SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score
         int32_t* hc = eb + (width + 1);
         dir = (int8_t*)(hc + (width + 1));
         zero_bytes((uint8_t*)dir, (uint64_t)width_d * readLen * 3 + 3);
-        if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, bw, dir);
-        else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, bw, dir);
-        else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, bw, hb, eb, hc, dir);
+        if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dir);
+        else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, sm, bw, dir);
+        else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, sm, bw, hb, eb, hc, dir);
         bw *= 2;
     } while (maxv < score && bw <= len);
     bw /= 2;
@@ -433,8 +459,8 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
             }
         };
         int st = banded_sw([&](int32_t i) { return ssw_code(q[qs + (uint32_t)i]); },
-                           [&](int32_t j) { return ssw_code(r[rs + (uint32_t)j]); }, refL, readL, score1, sc, bw,
-                           scratch, scratch_bytes, emit_main);
+                           [&](int32_t j) { return ssw_code(r[rs + (uint32_t)j]); }, refL, readL, score1, sc,
+                           MatchMismatch{sc.match, sc.mismatch}, bw, scratch, scratch_bytes, emit_main);
         if (st == 1) return a;  // flag 1: no alignment
         if (st == 2) { a.status = 2; return a; }
     }
@@ -523,6 +549,115 @@ SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Sco
     e.read_begin1 = e.read_end1 - br.read;
     e.rev_short = e.score1 > br.score;
     return ssw_finish(q, r, e, sc, scratch + ssw_bytes, scratch_bytes - ssw_bytes, cig);
+}
+
+// Outcome of ssw_align (the s_align fields of ext/ssw/ssw.h; the CIGAR goes to
+// the caller's buffer).
+struct SswRaw {
+    int32_t score1, score2, ref_begin1, ref_end1, read_begin1, read_end1, ref_end2;
+    int32_t n_cigar;  // ops written
+    int32_t flag;     // s_align.flag: 1 traceback failed, 2 the reverse pass fell short
+    int32_t status;   // 0 result; 1 ssw_align returns NULL (no profile for the scores it needs);
+                      // 2 scratch too small; 3 CIGAR buffer too small
+};
+
+// Bytes of scratch ssw_align_raw needs at most (passes, column maxima, and the
+// banded DP at the widest band the doubling can reach).
+SA_HD uint64_t ssw_raw_scratch(int32_t readLen, int32_t refLen) {
+    const uint64_t W = (uint64_t)((readLen + 15) / 16) * 16;
+    const uint64_t len = (uint64_t)(readLen > refLen ? readLen : refLen);
+    const uint64_t bw = len + 1;  // the band doubling stops once the width exceeds len
+    const uint64_t band = 3 * (2 * bw + 4) * 4 + (2 * bw + 1) * (uint64_t)readLen * 3 + 3 + 8 * (uint64_t)(readLen + refLen + 8);
+    return 16 * W + 4 * (uint64_t)refLen + band + 64;
+}
+
+// ssw_align (ext/ssw/ssw.c:818-922) on translated codes with an n x n matrix:
+// the forward pass (byte lanes, or word lanes when score_size is 1, or when the
+// byte scores saturate and score_size is 2), the second-best end outside maskLen
+// columns of the best (ssw.c:357-380, 560-583), the reverse pass for the start, and the
+// banded traceback as M/I/D ops (len << 4 | op), each step gated by `flag`,
+// `filters` and `filterd` as there.
+SA_HD void ssw_align_raw(const int8_t* read, int32_t readLen, const int8_t* ref, int32_t refLen, const int8_t* mat,
+                         int32_t n, int32_t score_size, int32_t gapO, int32_t gapE, uint32_t flag, uint32_t filters,
+                         int32_t filterd, int32_t maskLen, uint8_t* scratch, uint64_t scratch_bytes, uint32_t* cigar,
+                         uint32_t cigar_cap, SswRaw& o) {
+    o = SswRaw{0, 0, -1, 0, -1, 0, 0, 0, 0, 0};
+    const bool has_byte = score_size == 0 || score_size == 2, has_word = score_size == 1 || score_size == 2;
+    if (!has_byte && !has_word) { o.status = 1; return; }
+    const uint64_t W = (uint64_t)((readLen + 15) / 16) * 16;
+    if (16 * W + 4 * (uint64_t)refLen + 64 > scratch_bytes) { o.status = 2; return; }
+    int32_t* buf = (int32_t*)scratch;
+    int32_t* maxcol = buf + 4 * W;
+    uint8_t* rest = (uint8_t*)(maxcol + refLen);
+    rest = (uint8_t*)(((uintptr_t)rest + 15) & ~(uintptr_t)15);
+    const uint64_t rest_bytes = scratch_bytes - (uint64_t)(rest - scratch);
+    int32_t bias = 0;
+    for (int32_t i = 0; i < n * n; ++i) bias = mat[i] < bias ? mat[i] : bias;
+    bias = -bias;
+    const MatrixScore sm{mat, n};
+    auto qf = [&](int32_t j) { return (int32_t)read[j]; };
+    auto rf = [&](int32_t i) { return (int32_t)ref[i]; };
+    SswEnd best;
+    bool word = false;
+    if (has_byte) {
+        best = ssw_pass_sm<16, true>(qf, readLen, rf, refLen, false, gapO, gapE, bias, sm, 255, buf, maxcol);
+        if (best.score == 255) {
+            if (!has_word) { o.status = 1; return; }
+            best = ssw_pass_sm<8, false>(qf, readLen, rf, refLen, false, gapO, gapE, 0, sm, 65535, buf, maxcol);
+            word = true;
+        }
+    } else {
+        best = ssw_pass_sm<8, false>(qf, readLen, rf, refLen, false, gapO, gapE, 0, sm, 65535, buf, maxcol);
+        word = true;
+    }
+    o.score1 = best.score;
+    o.ref_end1 = best.ref;
+    o.read_end1 = best.read;
+    if (maskLen >= 15) {
+        int32_t s2 = 0, r2 = 0;
+        int32_t edge = best.ref - maskLen > 0 ? best.ref - maskLen : 0;
+        for (int32_t i = 0; i < edge; ++i)
+            if (maxcol[i] > s2) { s2 = maxcol[i]; r2 = i; }
+        edge = best.ref + maskLen > refLen ? refLen : best.ref + maskLen;
+        // the byte pass scans from edge + 1, the word pass from edge (ssw.c:374 and 578)
+        for (int32_t i = word ? edge : edge + 1; i < refLen; ++i)
+            if (maxcol[i] > s2) { s2 = maxcol[i]; r2 = i; }
+        o.score2 = s2;
+        o.ref_end2 = r2;
+    } else {
+        o.score2 = 0;
+        o.ref_end2 = -1;
+    }
+    if (flag == 0 || (flag == 2 && (uint32_t)o.score1 < filters)) return;
+    const int32_t re1 = o.read_end1;
+    auto qrev = [&](int32_t j) { return (int32_t)read[re1 - j]; };
+    const SswEnd br = word ? ssw_pass_sm<8, false>(qrev, re1 + 1, rf, o.ref_end1 + 1, true, gapO, gapE, 0, sm,
+                                                   (uint16_t)o.score1, buf, nullptr)
+                           : ssw_pass_sm<16, true>(qrev, re1 + 1, rf, o.ref_end1 + 1, true, gapO, gapE, bias, sm,
+                                                   (uint8_t)o.score1, buf, nullptr);
+    o.ref_begin1 = br.ref;
+    o.read_begin1 = re1 - br.read;
+    if (o.score1 > br.score) o.flag = 2;
+    if ((7 & flag) == 0 || ((2 & flag) != 0 && (uint32_t)o.score1 < filters) ||
+        ((4 & flag) != 0 && (o.ref_end1 - o.ref_begin1 > filterd || o.read_end1 - o.read_begin1 > filterd)))
+        return;
+    const int32_t rL = o.ref_end1 - o.ref_begin1 + 1, qL = o.read_end1 - o.read_begin1 + 1;
+    const int32_t bw = (rL > qL ? rL - qL : qL - rL) + 1;
+    const int32_t qb = o.read_begin1, rb = o.ref_begin1;
+    uint32_t nc = 0;
+    bool over = false;
+    auto emit = [&](uint32_t op, uint32_t len) {
+        if (nc < cigar_cap) cigar[nc] = (len << 4) | op;
+        else over = true;
+        ++nc;
+    };
+    const Scores gaps{0, 0, gapO, gapE, 0, 0};
+    const int st = banded_sw([&](int32_t i) { return (int32_t)read[qb + i]; }, [&](int32_t j) { return (int32_t)ref[rb + j]; },
+                             rL, qL, o.score1, gaps, sm, bw, rest, rest_bytes, emit);
+    if (st == 2) { o.status = 2; return; }
+    if (st == 1) { o.flag = 1; return; }
+    if (over) { o.status = 3; return; }
+    o.n_cigar = (int32_t)nc;
 }
 
 // src/aligner.rs:181-259 (segment scores with the end bonus at both ends)

@@ -269,4 +269,20 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
     }
 }
 
+// ssw_align of one (query, reference) pair in one thread: the device side of the
+// SSW link shim (ssw_init / ssw_align), which serves single calls for conformance.
+__global__ void __launch_bounds__(64) k_ssw_raw(const int8_t* __restrict__ read, int32_t readLen,
+                                               const int8_t* __restrict__ ref, int32_t refLen,
+                                               const int8_t* __restrict__ mat, int32_t n, int32_t score_size,
+                                               int32_t gapO, int32_t gapE, uint32_t flag, uint32_t filters,
+                                               int32_t filterd, int32_t maskLen, uint8_t* __restrict__ scratch,
+                                               uint64_t scratch_bytes, uint32_t* __restrict__ cigar,
+                                               uint32_t cigar_cap, SswRaw* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    SswRaw o;
+    ssw_align_raw(read, readLen, ref, refLen, mat, n, score_size, gapO, gapE, flag, filters, filterd, maskLen, scratch,
+                  scratch_bytes, cigar, cigar_cap, o);
+    *out = o;
+}
+
 }  // namespace sa

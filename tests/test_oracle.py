@@ -332,6 +332,46 @@ def test_ssw_against_reference_vectors(orc):
     assert n_word > 10  # the i16 kernel is exercised
 
 
+def test_oracle_ssw_on_shim_vectors(orc):
+    """The oracle's SSW restatement against the wider ssw.c vectors of the link
+    shim (other matrices and gap penalties), where it covers the call: flag 0x0f
+    and score_size 2 (mask_len only moves score2, which the oracle does not report)."""
+    with open(os.path.join(GOLDEN, "ssw_shim_vectors.json")) as f:
+        vecs = json.load(f)
+    n = 0
+    for v in vecs:
+        q = [int(c) for c in v["q"]]
+        if v["flag"] != 0x0F or v["score_size"] != 2:
+            continue
+        r = bytes(int(c) for c in v["r"])
+        mat = bytes(x & 0xFF for x in v["mat"])
+        out = (ctypes.c_int64 * 7)()
+        cig = (ctypes.c_uint32 * 4096)()
+        orc.orc_ssw_raw(bytes(q), len(q), r, len(r), mat, v["n"], v["gap_open"], v["gap_extend"], out, cig,
+                        ctypes.c_int64(4096))
+        e = v["res"]
+        assert [out[i] for i in range(6)] == [e[0], e[2], e[3], e[4], e[5], e[7]], v
+        assert [cig[i] for i in range(out[6])] == e[8], v
+        n += 1
+    assert n >= 30
+
+
+def test_shim_vectors_are_self_consistent():
+    """Each ssw.c CIGAR spans exactly [read_begin1, read_end1] x [ref_begin1, ref_end1]."""
+    with open(os.path.join(GOLDEN, "ssw_shim_vectors.json")) as f:
+        vecs = json.load(f)
+    n = 0
+    for v in vecs:
+        e = v["res"]
+        if not e or not e[8]:
+            continue
+        qlen = sum(c >> 4 for c in e[8] if c & 0xF in (0, 1))
+        rlen = sum(c >> 4 for c in e[8] if c & 0xF in (0, 2))
+        assert (qlen, rlen) == (e[5] - e[4] + 1, e[3] - e[2] + 1), v
+        n += 1
+    assert n >= 200
+
+
 def ssw(orc, q, r):
     out = (ctypes.c_int64 * 6)()
     buf = ctypes.create_string_buffer(8192)
