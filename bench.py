@@ -56,8 +56,8 @@ def cpu_baseline(ref, r1, r2, pairs, threads, workdir):
                 if i >= 4 * pairs:
                     break
                 g.write(line)
-    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "-o", os.devnull, ref, s1, s2],
-                       check=True, capture_output=True, text=True)
+    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "--t1-chunks", "3", "-o", os.devnull, ref,
+                        s1, s2], check=True, capture_output=True, text=True)
     t = json.loads(p.stderr.strip().splitlines()[-1])
     return pairs / t["map_s"], t
 
@@ -119,6 +119,17 @@ def pct(v, p):
     return v[min(len(v) - 1, int(math.ceil(p / 100.0 * len(v))) - 1)]
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,20 +185,42 @@ def main():
     value = n_pairs * a.steps / wall
     # dominant kernel over the timed steps: HIP-event time on the engines' streams;
     # algorithmic bytes are counted by the engine per launch
-    z = (0, 0, 0, 0)
-    delta = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(4)) for k in k1}
+    z = (0, 0, 0, 0, 0)
+    delta = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
     dom = max(delta, key=lambda k: delta[k][0])
-    ms, launches, items, nbytes = delta[dom]
+    ms, launches, items, nbytes, _ = delta[dom]
     per_launch_ms = ms / max(1, launches)
     bytes_per_launch = nbytes / max(1, launches)
     achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
-    traffic = None
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # of this workload (copy-kernel scaled)
+    tj = {}
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
-        if dom in tj.get("kernels", {}) and tj.get("workload") == "pe100m":
-            traffic = tj["kernels"][dom]["hbm_bytes_per_launch_scaled"]
+    measured = tj.get("kernels", {}) if tj.get("workload") == "pe100m" else {}
+    traffic = measured[dom]["hbm_bytes_per_launch_scaled"] if dom in measured else None
+    # index lookups: useful bytes (counted per probe) against the sector traffic
+    lookup = None
+    if delta.get("k_hits", z)[0] > 0:
+        ms_, l_, _, b_, _ = delta["k_hits"]
+        avg = ms_ / max(1, l_)
+        lookup = {"kernel": "k_hits", "avg_launch_ms": round(avg, 3), "useful_bytes_per_launch": int(b_ / max(1, l_)),
+                  "useful_gbps": round(b_ / (ms_ / 1000.0) / 1e9, 2), "peak_gbps": HBM_PEAK_GBS}
+        if "k_hits" in measured:
+            tb = measured["k_hits"]["hbm_bytes_per_launch_scaled"]
+            lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1)})
+    # forward SSW pass: DP cells per second against the VALU integer peak
+    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
+    # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
+    extension = None
+    if delta.get("k_ssw_byte_fwd", z)[4] > 0 and delta["k_ssw_byte_fwd"][0] > 0:
+        ms_, _, _, _, cells = delta["k_ssw_byte_fwd"]
+        gcups = cells / (ms_ / 1000.0) / 1e9
+        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / a.steps),
+                     "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
+                     "frac": round(gcups * 10 / 39300.0, 4)}
     copy_gbps = ctx.hbm_copy_gbps()
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
@@ -223,7 +256,12 @@ def main():
             v, t = cpu_baseline(w["ref"], w["reads1"], w["reads2"], min(a.cpu_pairs, n_pairs), cores, a.data_dir)
             cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
                    "sample": "first %d pairs of the same workload, oracle -t %d, index build excluded (%.1f s map)"
-                             % (min(a.cpu_pairs, n_pairs), cores, t["map_s"])}
+                             % (min(a.cpu_pairs, n_pairs), cores, t["map_s"]),
+                   "label": "CPU restatement of reference v0.18.0-alpha (oracle/), not the upstream binary",
+                   "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                   "single_thread_value": round(t["t1_records"] / t["t1_s"], 1) if t.get("t1_s") else None,
+                   "single_thread_sample": "first %d pairs on one thread" % t.get("t1_records", 0),
+                   "stage_seconds_summed_over_threads": t.get("stages_s")}
         except Exception as e:  # reported, never fatal
             cpu = {"value": None, "unit": "read pairs/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
     out = {
@@ -247,6 +285,8 @@ def main():
         "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"]} if serving else None,
         "serving": serving,
         "roofline": roof,
+        "lookup": lookup,
+        "extension": extension,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -225,9 +225,11 @@ int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* quer
 int sa_index_stats(const sa_index* index, double* kernel_ms);
 
 /* Per-kernel totals since the index was created, as JSON text
- * {"k_seed": [milliseconds, launches, work_items, algorithmic_bytes], ...}, timed
- * with HIP events on the engines' streams (algorithmic_bytes is 0 where the kernel
- * does not count them). Returns the JSON length (truncated to cap-1 in buf). */
+ * {"k_seed": [milliseconds, launches, work_items, algorithmic_bytes, dp_cells], ...},
+ * timed with HIP events on the engines' streams (algorithmic_bytes is 0 where the
+ * kernel does not count them; dp_cells, query x reference cells of the forward SSW
+ * pass, is counted for k_ssw_byte_fwd only). Returns the JSON length (truncated to
+ * cap-1 in buf). */
 size_t sa_index_kernel_times(const sa_index* index, char* buf, size_t cap);
 
 /* Device copy bandwidth: copies `bytes` between two device buffers `iters` times

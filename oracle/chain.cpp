@@ -270,10 +270,16 @@ std::vector<Nam> get_nams_by_chaining(const uint8_t* seq, size_t len, const Inde
                                       const ChainingParams& cp, size_t rescue_distance, Mcs mcs,
                                       NamDetails& details) {
     std::vector<QueryRandstrobe> qr[2];
-    randstrobes_query(seq, len, index.params, qr);
+    {
+        StageTimer st(stage_times().randstrobes);
+        randstrobes_query(seq, len, index.params, qr);
+    }
     std::vector<Hit> hits[2];
     HitsDetails hd[2];
-    for (int o = 0; o < 2; ++o) hits[o] = find_hits(qr[o], index, mcs, index.filter_cutoff, rescue_distance, hd[o]);
+    {
+        StageTimer st(stage_times().find_hits);
+        for (int o = 0; o < 2; ++o) hits[o] = find_hits(qr[o], index, mcs, index.filter_cutoff, rescue_distance, hd[o]);
+    }
     auto better = [](const HitsDetails& a, const HitsDetails& b) {
         size_t t = a.full_found + a.full_filtered, ot = b.full_found + b.full_filtered;
         return t >= ot * 2 && t > ot + 5;
@@ -295,7 +301,12 @@ std::vector<Nam> get_nams_by_chaining(const uint8_t* seq, size_t len, const Inde
     std::vector<Nam> chains;
     size_t n_anchors = 0;
     for (int o : orient) {
-        std::vector<Anchor> a = hits_to_anchors(hits[o], index);
+        std::vector<Anchor> a;
+        {
+            StageTimer st(stage_times().find_hits);
+            a = hits_to_anchors(hits[o], index);
+        }
+        StageTimer st(stage_times().chaining);
         n_anchors += a.size();
         std::sort(a.begin(), a.end());
         a.erase(std::unique(a.begin(), a.end()), a.end());
@@ -361,6 +372,7 @@ uint64_t Rng::gen_mod_u64(uint64_t n) {
 
 // src/nam.rs:168-188, src/shuffle.rs:7-25 (fastrand shuffle: swap(i, usize(..=i)) for i in 1..n)
 void sort_nams(std::vector<Nam>& nams, Rng& rng) {
+    StageTimer st(stage_times().sort_nams);
     std::stable_sort(nams.begin(), nams.end(), [](const Nam& a, const Nam& b) { return a.score > b.score; });
     if (nams.empty()) return;
     float best = nams[0].score;
@@ -368,6 +380,11 @@ void sort_nams(std::vector<Nam>& nams, Rng& rng) {
     while (n_best < nams.size() && nams[n_best].score == best) ++n_best;
     if (n_best > 1)
         for (size_t i = 1; i < n_best; ++i) std::swap(nams[i], nams[rng.gen_mod_u64(i + 1)]);
+}
+
+StageTimes& stage_times() {
+    thread_local StageTimes t;
+    return t;
 }
 
 }  // namespace orc

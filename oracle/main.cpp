@@ -26,6 +26,7 @@ int main(int argc, char** argv) {
     size_t threads = 1, chunk_size = 1000000, read_length = 0;
     int mode = 0;
     bool time_it = false;
+    size_t t1_chunks = 0;  // with --time: also map the first N chunks again on one thread
     MappingParams mp;
     SamOpts so;
     std::vector<std::string> rg_fields;
@@ -52,6 +53,7 @@ int main(int argc, char** argv) {
         else if (a == "--details") so.details = true;
         else if (a == "--no-PG") {}
         else if (a == "--time") time_it = true;
+        else if (is("--t1-chunks")) t1_chunks = std::stoul(val("--t1-chunks"));
         else if (is("--rg-id")) { so.has_rg = true; so.rg_id = val("--rg-id"); }
         else if (is("--rg")) rg_fields.push_back(val("--rg"));
         else if (is("--chunk-size")) chunk_size = std::stoul(val("--chunk-size"));
@@ -131,6 +133,7 @@ int main(int argc, char** argv) {
     }
     std::vector<std::string> outs(chunks.size());
     std::vector<std::vector<double>> abund(threads, std::vector<double>(refs.size(), 0.0));
+    std::vector<orc::StageTimes> stages(threads);
     std::atomic<size_t> next{0};
     auto t2 = std::chrono::steady_clock::now();
     auto worker = [&](size_t tid) {
@@ -142,11 +145,27 @@ int main(int argc, char** argv) {
             mm.map_chunk(chunks[c], outs[c]);
         }
         abund[tid] = mm.abundances;
+        stages[tid] = orc::stage_times();
     };
     std::vector<std::thread> th;
     for (size_t t = 0; t < threads; ++t) th.emplace_back(worker, t);
     for (auto& t : th) t.join();
     auto t3 = std::chrono::steady_clock::now();
+    // single-thread rate on the leading chunks (same index, output dropped)
+    double t1_s = 0;
+    size_t t1_records = 0;
+    if (time_it && t1_chunks) {
+        Mapper mm = m;
+        mm.abundances.assign(refs.size(), 0.0);
+        std::string sink;
+        auto a = std::chrono::steady_clock::now();
+        for (size_t c = 0; c < chunks.size() && c < t1_chunks; ++c) {
+            sink.clear();
+            mm.map_chunk(chunks[c], sink);
+            t1_records += chunks[c].size();
+        }
+        t1_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+    }
 
     FILE* out = out_path.empty() ? stdout : std::fopen(out_path.c_str(), "wb");
     if (!out) { std::fprintf(stderr, "error: cannot open %s\n", out_path.c_str()); return 1; }
@@ -166,7 +185,17 @@ int main(int argc, char** argv) {
         double ti = std::chrono::duration<double>(t1 - t0).count();
         double tm = std::chrono::duration<double>(t3 - t2).count();
         size_t n = r2.empty() ? r1.size() : std::min(r1.size(), r2.size());
-        std::fprintf(stderr, "{\"index_s\": %.4f, \"map_s\": %.4f, \"records\": %zu, \"threads\": %zu}\n", ti, tm, n, threads);
+        orc::StageTimes s;
+        for (const auto& x : stages) {
+            s.randstrobes += x.randstrobes; s.find_hits += x.find_hits; s.chaining += x.chaining;
+            s.sort_nams += x.sort_nams; s.extend += x.extend;
+        }
+        std::fprintf(stderr,
+                     "{\"index_s\": %.4f, \"map_s\": %.4f, \"records\": %zu, \"threads\": %zu, \"stages_s\": "
+                     "{\"randstrobes\": %.4f, \"find_hits\": %.4f, \"chaining\": %.4f, \"sort_nams\": %.4f, "
+                     "\"extend\": %.4f}, \"t1_s\": %.4f, \"t1_records\": %zu}\n",
+                     ti, tm, n, threads, s.randstrobes, s.find_hits, s.chaining, s.sort_nams, s.extend, t1_s,
+                     t1_records);
     }
     return 0;
 }

@@ -168,6 +168,7 @@ bool reverse_nam_if_needed(Nam& nam, const Read& read, const Ctx& c) {
 
 // src/mapper.rs:532-620
 std::optional<Alignment> extend_seed(Nam& nam, const Read& read, bool consistent, bool use_ssw, const Ctx& c) {
+    StageTimer st(stage_times().extend);
     const std::string& query = nam.is_revcomp ? read.rc : *read.seq;
     const PackedSeq& refseq = c.ref(nam.ref_id);
     const Scores& sc = c.m.sc;
@@ -416,6 +417,7 @@ bool has_shared_substring(const std::string& rs, const std::string& ref, size_t 
 
 // src/mapper.rs:1094-1153
 std::optional<Alignment> rescue_align(const Nam& mate, const Read& read, float mu, float sigma, const Ctx& c) {
+    StageTimer st(stage_times().extend);
     size_t L = read.len();
     size_t k = c.k();
     const std::string* rtmp;

@@ -10,6 +10,7 @@
 // in the test suite; functions whose behaviour the reference's own fixtures
 // and known-answer tests cannot pin say "parity unpinned" next to them.
 #pragma once
+#include <chrono>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -17,6 +18,21 @@
 #include <optional>
 
 namespace orc {
+
+// Per-thread stage clocks in seconds, the timers of src/main.rs:821-846:
+// randstrobes, find_hits (with the anchor expansion, as chainer.rs:222-224 counts
+// it), chaining (anchor sort, collinear chaining, chain extraction), sort_nams, and
+// extend (extend_seed and rescue_align).
+struct StageTimes {
+    double randstrobes = 0, find_hits = 0, chaining = 0, sort_nams = 0, extend = 0;
+};
+StageTimes& stage_times();  // the calling thread's
+struct StageTimer {
+    double& acc;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit StageTimer(double& a) : acc(a) {}
+    ~StageTimer() { acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
 
 // ---------------------------------------------------------------- seeding ---
 // src/seeding/hash.rs:11-27

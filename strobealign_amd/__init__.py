@@ -283,7 +283,7 @@ class Index:
         return res
 
     def kernel_times(self):
-        """{kernel: (total_ms, launches, items, algorithmic_bytes)} timed with HIP events."""
+        """{kernel: (total_ms, launches, items, algorithmic_bytes, dp_cells)} timed with HIP events."""
         import json
         n = _lib.sa_index_kernel_times(self._p, None, ctypes.c_size_t(0))
         buf = ctypes.create_string_buffer(n + 1)

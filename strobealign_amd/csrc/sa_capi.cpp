@@ -121,6 +121,7 @@ size_t kernel_times_json(const std::vector<const sa::Engine*>& engs, char* buf, 
                     a.second.launches += k.second.launches;
                     a.second.items += k.second.items;
                     a.second.bytes += k.second.bytes;
+                    a.second.cells += k.second.cells;
                     found = true;
                 }
             if (!found) all.push_back(k);
@@ -129,9 +130,10 @@ size_t kernel_times_json(const std::vector<const sa::Engine*>& engs, char* buf, 
     bool first = true;
     for (const auto& k : all) {
         char tmp[256];
-        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu, %llu]", first ? "" : ", ", k.first.c_str(),
-                      k.second.ms, (unsigned long long)k.second.launches, (unsigned long long)k.second.items,
-                      (unsigned long long)k.second.bytes);
+        std::snprintf(tmp, sizeof tmp, "%s\"%s\": [%.6f, %llu, %llu, %llu, %llu]", first ? "" : ", ",
+                      k.first.c_str(), k.second.ms, (unsigned long long)k.second.launches,
+                      (unsigned long long)k.second.items, (unsigned long long)k.second.bytes,
+                      (unsigned long long)k.second.cells);
         s += tmp;
         first = false;
     }

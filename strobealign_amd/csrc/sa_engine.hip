@@ -46,7 +46,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_COUNT
 };
 }  // namespace
 
@@ -242,16 +242,17 @@ void Engine::sync() {
     SA_CHECK(hipEventSynchronize(sync_ev_));
 }
 
-void Engine::add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes) {
+void Engine::add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes, uint64_t cells) {
     for (auto& k : kstats_)
         if (k.first == name) {
             k.second.ms += ms;
             k.second.launches += 1;
             k.second.items += items;
             k.second.bytes += bytes;
+            k.second.cells += cells;
             return;
         }
-    kstats_.push_back({name, KernelStat{ms, 1, items, bytes}});
+    kstats_.push_back({name, KernelStat{ms, 1, items, bytes, cells}});
 }
 
 void* Engine::grow(int slot, uint64_t bytes) {
@@ -409,7 +410,9 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
     SA_CHECK(hipGetLastError());
     SA_CHECK(hipEventRecord(ev[1], stream_));
-    k_hits<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, mp, d_off, n, d_qr, d_qrn, d_hits, d_cnt, d_hc, d_acap);
+    auto* d_hbytes = (uint32_t*)grow(S_HBYTES, (2ull * n + 1) * 4);
+    k_hits<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, mp, d_off, n, d_qr, d_qrn, d_hits, d_cnt, d_hc, d_acap,
+                                                       d_hbytes);
     SA_CHECK(hipGetLastError());
     SA_CHECK(hipEventRecord(ev[2], stream_));
     uint64_t A = 0;
@@ -455,6 +458,8 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     SA_CHECK(hipMemcpyAsync(qrn.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(namn.data(), d_namn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(araw.data(), d_araw, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    std::vector<uint32_t> hbytes(2ull * n);
+    SA_CHECK(hipMemcpyAsync(hbytes.data(), d_hbytes, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
     sync();
     out.nam_off[n] = N;
     out.info.resize(n);
@@ -468,14 +473,17 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         r.both_orientations = !better_orientation(hc[2 * i], hc[2 * i + 1]) && !better_orientation(hc[2 * i + 1], hc[2 * i]);
     }
     float t;
-    // algorithmic bytes: read bases in, query randstrobes out; hits/entries in, NAMs out
-    uint64_t seed_bytes = n_bases_, chain_bytes = 0;
+    // algorithmic bytes: read bases in, query randstrobes out (seed); randstrobes,
+    // bucket_starts words and index entries in, hits out (lookups); hits/entries in,
+    // NAMs out (chaining)
+    uint64_t seed_bytes = n_bases_, chain_bytes = 0, hit_bytes = 0;
     for (uint64_t x = 0; x < 2ull * n; ++x) {
         seed_bytes += sizeof(QRand) * (uint64_t)qrn[x];
         chain_bytes += cbytes[x];
+        hit_bytes += hbytes[x];
     }
     SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n, seed_bytes);
-    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n);
+    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n, hit_bytes);
     SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n, chain_bytes);
     SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
@@ -529,8 +537,11 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_jobs = (ExtJob*)grow(S_JOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_ljobs = (ExtJob*)grow(S_LJOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
-    // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets
-    auto* d_cnt = (uint32_t*)grow(S_XCNT, (4 + SSW_BUCKETS) * 4);
+    // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets,
+    // then (64-bit, at word 4 + SSW_BUCKETS) the DP cells of the forward SSW passes
+    constexpr uint32_t NCNT = 4 + SSW_BUCKETS + 2;
+    static_assert((4 + SSW_BUCKETS) % 2 == 0, "cell counter alignment");
+    auto* d_cnt = (uint32_t*)grow(S_XCNT, NCNT * 4);
     uint32_t* d_bcnt = d_cnt + 4;
     // SSW bucket lists sized for every request (the job count is known only on the device)
     const uint32_t cap = n;
@@ -538,7 +549,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_ends = (SswEnds*)grow(S_ENDS, (uint64_t)n * sizeof(SswEnds));
     auto* d_fwd = (SswEnd*)grow(S_FWD, (uint64_t)n * sizeof(SswEnd));
     SA_CHECK(hipMemcpyAsync(d_req, req.data(), (uint64_t)n * sizeof(ExtRequest), hipMemcpyHostToDevice, stream_));
-    SA_CHECK(hipMemsetAsync(d_cnt, 0, (4 + SSW_BUCKETS) * 4, stream_));
+    SA_CHECK(hipMemsetAsync(d_cnt, 0, NCNT * 4, stream_));
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
     const auto* d_seq = d_seq_;
     const auto* d_off = (const uint64_t*)buf_[S_OFF];
@@ -546,17 +557,17 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     const uint32_t k = (uint32_t)p_.k;
     const uint64_t L = max_len_;
     // kernel times are read once the work is done (no wait per kernel)
-    struct Timed { const char* name; uint64_t items, bytes; hipEvent_t a, b; };
+    struct Timed { const char* name; uint64_t items, bytes, cells; hipEvent_t a, b; };
     std::vector<Timed> tl;
-    auto timed = [&](const char* name, uint64_t items, auto&& launch, uint64_t bytes = 0) {
-        Timed x{name, items, bytes, event(), event()};
+    auto timed = [&](const char* name, uint64_t items, auto&& launch, uint64_t bytes = 0, uint64_t cells = 0) {
+        Timed x{name, items, bytes, cells, event(), event()};
         SA_CHECK(hipEventRecord(x.a, stream_));
         launch();
         SA_CHECK(hipGetLastError());
         SA_CHECK(hipEventRecord(x.b, stream_));
         tl.push_back(x);
     };
-    uint32_t cnt[4 + SSW_BUCKETS];
+    uint32_t cnt[NCNT];
     auto read_counts = [&] {
         SA_CHECK(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, stream_));
         sync();
@@ -566,10 +577,13 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res,
                                                                       d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
-    k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap);
+    k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
+                                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
     SA_CHECK(hipGetLastError());
     read_counts();
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
+    uint64_t fwd_cells = 0;
+    std::memcpy(&fwd_cells, cnt + 4 + SSW_BUCKETS, 8);
     const uint32_t* bc = cnt + 4;
     // thread-per-job kernels run in waves sized to the scratch budget
     const uint64_t budget = 16ull << 30;
@@ -597,7 +611,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 SA_CHECK(hipGetLastError());
             }
         };
-        timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); });
+        timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); }, 0, fwd_cells);
         read_counts();
         uint64_t nrb = 0, nfw = 0;
         for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1)];
@@ -658,7 +672,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     for (Timed& x : tl) {
         float t = 0;
         SA_CHECK(hipEventElapsedTime(&t, x.a, x.b));
-        add_kernel(x.name, t, x.items, x.bytes);
+        add_kernel(x.name, t, x.items, x.bytes, x.cells);
         total_ms_ += t;
         ext_ms += t;
         release_event(x.a);

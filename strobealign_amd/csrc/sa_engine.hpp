@@ -85,10 +85,11 @@ public:
 
     const StageTimes& times() const { return times_; }
     // Per-kernel totals since creation: name -> (milliseconds, launches, work items).
-    // bytes: algorithmic bytes (the compulsory reads and writes of the work items), 0 if not counted
-    struct KernelStat { double ms = 0; uint64_t launches = 0, items = 0, bytes = 0; };
+    // bytes: algorithmic bytes (the compulsory reads and writes of the work items), 0 if not counted;
+    // cells: dynamic-programming cells (query x reference) of the forward SSW pass, 0 elsewhere
+    struct KernelStat { double ms = 0; uint64_t launches = 0, items = 0, bytes = 0, cells = 0; };
     const std::vector<std::pair<std::string, KernelStat>>& kernel_stats() const { return kstats_; }
-    void add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes = 0);
+    void add_kernel(const char* name, double ms, uint64_t items, uint64_t bytes = 0, uint64_t cells = 0);
     const SeedParams& params() const { return p_; }
     uint64_t n_entries() const { return n_entries_; }
     ihipStream_t* stream() const { return stream_; }

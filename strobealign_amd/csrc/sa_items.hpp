@@ -24,7 +24,7 @@ SA_HD void item_hits(uint32_t t, IndexView ix, MapParams mp, const uint64_t* off
                                              uint32_t n, const QRand* qr,
                                              const uint32_t* qr_n, Hit* hits,
                                              uint64_t* cnt, HitCounts* hc,
-                                             uint64_t* acap) {
+                                             uint64_t* acap, uint32_t* lookup_bytes) {
     if (t >= 2 * n) return;
     uint32_t i = t >> 1, o = t & 1;
     uint64_t ro = off[i];
@@ -34,6 +34,9 @@ SA_HD void item_hits(uint32_t t, IndexView ix, MapParams mp, const uint64_t* off
     uint32_t nh = find_hits(ix, qr + base, qr_n[t], mp, hits + base, cnt + base, d);
     hc[t] = d;
     acap[t] = hit_runs(ix, hits + base, nh);
+    // query randstrobes in, bucket_starts words and index entries read, hits out
+    lookup_bytes[t] = (uint32_t)sizeof(QRand) * qr_n[t] + 8u * ix.bucket_reads + (uint32_t)sizeof(RefEntry) * ix.entry_reads +
+                      (uint32_t)sizeof(Hit) * nh;
 }
 
 // -------------------------------------------------------------- chaining ---

@@ -19,8 +19,8 @@ __global__ void __launch_bounds__(256) k_hits(IndexView ix, MapParams mp, const 
                                              uint32_t n, const QRand* __restrict__ qr,
                                              const uint32_t* __restrict__ qr_n, Hit* __restrict__ hits,
                                              uint64_t* __restrict__ cnt, HitCounts* __restrict__ hc,
-                                             uint64_t* __restrict__ acap) {
-    item_hits(blockIdx.x * blockDim.x + threadIdx.x, ix, mp, off, n, qr, qr_n, hits, cnt, hc, acap);
+                                             uint64_t* __restrict__ acap, uint32_t* __restrict__ lookup_bytes) {
+    item_hits(blockIdx.x * blockDim.x + threadIdx.x, ix, mp, off, n, qr, qr_n, hits, cnt, hc, acap, lookup_bytes);
 }
 
 __global__ void __launch_bounds__(256) k_chain(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
@@ -212,18 +212,26 @@ SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg) {
 }
 
 // Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY);
-// the job count is counters[0] on the device, the grid covers every request.
+// the job count is counters[0] on the device, the grid covers every request. Also
+// sums the jobs' forward DP cells (query length x window length) into *cells.
 __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__ off,
                                                    const ExtRequest* __restrict__ req,
                                                    const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ counters,
                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
-                                                   uint32_t cap) {
+                                                   uint32_t cap, unsigned long long* __restrict__ cells) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= counters[0]) return;
-    const uint32_t rid = req[jobs[t].req].read_id;
-    const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
-    const uint32_t b = ssw_bucket(false, false, (L + 15) / 16);
-    lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
+    uint64_t c = 0;
+    if (t < counters[0]) {
+        const uint32_t rid = req[jobs[t].req].read_id;
+        const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
+        const uint32_t b = ssw_bucket(false, false, (L + 15) / 16);
+        lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
+        c = (uint64_t)L * jobs[t].wlen;
+    }
+    // one atomic per wave
+    SA_UNROLL
+    for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+    if ((threadIdx.x & 63u) == 0 && c) atomicAdd(cells, (unsigned long long)c);
 }
 
 // One striped score pass (ext/ssw/ssw.c:197-588) per group of G lanes: the

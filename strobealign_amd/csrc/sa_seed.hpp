@@ -333,9 +333,14 @@ struct IndexView {
     const RefEntry* e;
     const uint64_t* bs;  // bucket_starts, 2^bits + 1 entries
     SeedParams p;
+    // reads made through this copy (the lookup stage's algorithmic bytes)
+    mutable uint32_t bucket_reads = 0, entry_reads = 0;
 };
 
-SA_HD uint64_t ehash(const IndexView& ix, uint64_t i) { return ix.e[i].hash_offset & REF_HASH_MASK; }
+SA_HD uint64_t ehash(const IndexView& ix, uint64_t i) {
+    ++ix.entry_reads;
+    return ix.e[i].hash_offset & REF_HASH_MASK;
+}
 
 // Lower bound of `masked` among entries [lo, hi) under `mask` (src/partition.rs:27-86).
 SA_HD uint64_t lower_bound_masked(const IndexView& ix, uint64_t lo, uint64_t hi, uint64_t mask, uint64_t masked) {
@@ -361,6 +366,7 @@ SA_HD uint64_t get_masked_from(const IndexView& ix, uint64_t hash, uint64_t mask
     uint64_t top = hash >> (64 - ix.p.bits);
     uint64_t ps = start == NONE64 ? ix.bs[top] : start;
     uint64_t pe = ix.bs[top + 1];
+    ix.bucket_reads += start == NONE64 ? 2 : 1;
     if (ps >= pe) return NONE64;
     if (pe - ps < 4) {
         for (uint64_t i = ps; i < pe; ++i) {
@@ -381,6 +387,7 @@ SA_HD uint64_t get_count(const IndexView& ix, uint64_t position, uint64_t mask) 
     uint64_t mk = key & mask;
     uint64_t top = key >> (64 - ix.p.bits);
     uint64_t pe = ix.bs[top + 1];
+    ++ix.bucket_reads;
     if (pe - position < 8) {
         uint64_t count = 1;
         for (uint64_t i = position + 1; i < pe; ++i) {

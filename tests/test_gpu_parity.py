@@ -143,6 +143,29 @@ def test_paired_end_100m_vs_oracle(sa, ctx, tmp_path_factory):
     assert got == exp
 
 
+def test_single_end_variable_length_vs_oracle(sa, ctx, tmp_path_factory):
+    """Single-end reads of 50-500 bp (15% from repeat copies) on the 100 Mbp
+    reference: the r250 profile from the estimated read length, piecewise and
+    --ssw extension (queries above 256 bp take the long-query SSW kernel), and -x."""
+    say("case: 100 Mbp, 3000 SE reads of 50-500 bp (SAM, --ssw, -x)")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("sevar"))
+    w = synth.make("se_var", d, 3000)
+    with open(w["reads1"], "rb") as f:
+        lines = f.read().split(b"\n")
+    lens = [len(lines[i]) for i in range(1, len(lines), 4)]
+    assert min(lens) <= 60 and max(lens) >= 480
+    est = sum(lens[:500]) // len(lens[:500])  # src/main.rs:351 estimate over the first 500 records
+    with sa.Index(ctx, w["ref"], read_length=est, threads=8) as ix:
+        p, _, _ = ix.params()
+        assert (p.k, p.s, p.w_min, p.w_max) == (22, 18, 6, 16)  # r250 (src/seeding/parameters.rs)
+        for flags, kw in (((), {}), (("--ssw",), {"use_ssw": True}), (("-x",), {"mode": 1})):
+            exp = oracle("-t", "8", "--no-PG", *flags, w["ref"], w["reads1"])
+            got = ix.map_files(w["reads1"], threads=8, **kw)
+            assert got == exp, flags
+
+
 def test_resident_reads_and_concurrent_batches(sa, ctx, tmp_path_factory):
     """The same SAM whether reads are uploaded per batch or staged once, and
     whether batches run on one engine or on three concurrently."""
