@@ -1430,6 +1430,7 @@ struct Server::Impl {
     std::condition_variable cv_admit, cv_work, cv_done;
     std::deque<Req*> waiting;  // submitted, reads not yet on the device
     std::deque<Req*> ready;    // admitted, with chunks not yet in a batch
+    uint64_t ready_bases = 0;  // bases of the chunks in `ready` not yet in a batch
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
@@ -1519,6 +1520,7 @@ struct Server::Impl {
                 finish_locked(r);
             } else {
                 ready.push_back(r);
+                ready_bases += r->bytes;
                 cv_work.notify_all();
             }
         }
@@ -1535,7 +1537,9 @@ struct Server::Impl {
     }
 
     // One engine: forms device batches from the oldest admitted chunks of any
-    // requests (same pairedness), up to batch_bases, and runs them.
+    // requests (same pairedness) and runs them. A batch takes batch_bases when the
+    // queue is short (latency) and up to 8x that when a backlog waits (each
+    // batch's fixed cost of passes and waits is then spread over more reads).
     void work_loop(size_t e) {
         std::unique_lock<std::mutex> lk(m);
         for (;;) {
@@ -1544,11 +1548,14 @@ struct Server::Impl {
             std::vector<std::pair<Req*, size_t>> take;
             uint64_t bases = 0;
             const bool paired = ready.front()->paired;
-            while (!ready.empty() && ready.front()->paired == paired && (take.empty() || bases < o.batch_bases)) {
+            const uint64_t share = ready_bases / engines.size();
+            const uint64_t limit = std::min<uint64_t>(std::max<uint64_t>(share, o.batch_bases), 8 * o.batch_bases);
+            while (!ready.empty() && ready.front()->paired == paired && (take.empty() || bases < limit)) {
                 Req* r = ready.front();
                 const size_t c = r->next_chunk++;
                 take.push_back({r, c});
                 bases += r->chunk_b[c];
+                ready_bases -= r->chunk_b[c];
                 if (r->next_chunk == r->chunks.size()) ready.pop_front();
             }
             size_t nreq = 0;
