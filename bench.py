@@ -141,6 +141,7 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=200_000, help="oracle sample size (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
+    ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     a = ap.parse_args()
@@ -165,7 +166,7 @@ def main():
     t0 = time.perf_counter()
     reads.upload(ix)  # inputs resident in HBM before the timed region
     upload_s = time.perf_counter() - t0
-    opts = sa.options(threads=threads, slots=a.slots)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
     print("index %.1fs, %d pairs loaded, staged on the device in %.1f ms" % (index_s, len(reads), 1000 * upload_s),
           file=sys.stderr, flush=True)
     for i in range(a.warmup):

@@ -495,18 +495,18 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
 namespace {
 // Launches k_ssw_pass<G, BYTE, seg> for a run-time stripe count 1 <= seg <= MAX.
 template <int G, bool BYTE, int S, int MAX>
-void launch_ssw_pass(uint32_t seg, uint32_t count, hipStream_t st, const uint8_t* seq, const uint64_t* off,
-                     RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, const uint32_t* list,
-                     uint32_t rev, SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap) {
+void launch_ssw_pass(uint32_t seg, const SswLaunchLists& ll, hipStream_t st, const uint8_t* seq, const uint64_t* off,
+                     RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, uint32_t rev,
+                     SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap) {
     if constexpr (S > MAX) {
         throw std::logic_error("SSW stripe count out of range");
     } else if (seg != S) {
-        launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, count, st, seq, off, ref, sc, req, jobs, list, rev, fwd, ends,
-                                             lists, counts, cap);
+        launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, ll, st, seq, off, ref, sc, req, jobs, rev, fwd, ends, lists, counts,
+                                             cap);
     } else {
-        k_ssw_pass<G, BYTE, S><<<blocks((uint64_t)count * G, 256), 256, 0, st>>>(seq, off, ref, sc, req, jobs, list,
-                                                                                 count, rev, fwd, ends, lists,
-                                                                                 counts, cap);
+        const uint64_t count = ll.start[SSW_WCLASSES];
+        k_ssw_pass<G, BYTE, S><<<blocks(count * G, 256), 256, 0, st>>>(seq, off, ref, sc, req, jobs, ll, rev, fwd,
+                                                                       ends, lists, counts, cap);
     }
 }
 }  // namespace
@@ -599,14 +599,24 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         // launches every non-empty bucket of one (mode, pass) family
         auto run = [&](bool word, bool rev) {
             for (uint32_t seg = 1; seg <= (word ? 32u : 16u); ++seg) {
-                const uint32_t b = ssw_bucket(word, rev, seg), c = bc[b];
-                if (!c) continue;
-                const uint32_t* list = d_lists + (uint64_t)b * cap;
+                // one launch per stripe count: its window classes, longest first
+                SswLaunchLists ll{};
+                uint32_t at = 0;
+                for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+                    const uint32_t wc = SSW_WCLASSES - 1 - x;
+                    const bool used = !rev || wc == 0;
+                    const uint32_t b = ssw_bucket(word, rev, seg, used ? wc : 0);
+                    ll.list[x] = d_lists + (uint64_t)b * cap;
+                    ll.start[x] = at;
+                    at += used ? bc[b] : 0;
+                }
+                ll.start[SSW_WCLASSES] = at;
+                if (!at) continue;
                 if (word)
-                    launch_ssw_pass<8, false, 1, 32>(seg, c, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, list, rev,
+                    launch_ssw_pass<8, false, 1, 32>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
                                                      d_fwd, d_ends, d_lists, d_bcnt, cap);
                 else
-                    launch_ssw_pass<16, true, 1, 16>(seg, c, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, list, rev,
+                    launch_ssw_pass<16, true, 1, 16>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
                                                      d_fwd, d_ends, d_lists, d_bcnt, cap);
                 SA_CHECK(hipGetLastError());
             }
@@ -614,8 +624,9 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); }, 0, fwd_cells);
         read_counts();
         uint64_t nrb = 0, nfw = 0;
-        for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1)];
-        for (uint32_t x = 0; x < 32; ++x) nfw += bc[ssw_bucket(true, false, x + 1)];
+        for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1, 0)];
+        for (uint32_t x = 0; x < 32; ++x)
+            for (uint32_t wc = 0; wc < SSW_WCLASSES; ++wc) nfw += bc[ssw_bucket(true, false, x + 1, wc)];
         if (nrb) timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
         if (nfw) {
             timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });

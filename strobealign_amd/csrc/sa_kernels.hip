@@ -204,11 +204,18 @@ __global__ void __launch_bounds__(128) k_ssw_finish(const uint8_t* __restrict__ 
 }
 
 // SSW score-pass buckets: jobs are listed by (pass, stripe count) so that each
-// launch of k_ssw_pass runs one compile-time stripe count.
-constexpr uint32_t SSW_BUCKETS = 96;
-// byte forward 0..15, byte reverse 16..31, word forward 32..63, word reverse 64..95
-SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg) {
-    return (rev ? (word ? 64u : 16u) : (word ? 32u : 0u)) + seg - 1;
+// launch of k_ssw_pass runs one compile-time stripe count, and the forward passes
+// also by reference-window class, since a lane group runs as many columns as its
+// window is long and a wave as long as its longest group (seed windows are
+// ~1.2 L, rescue windows up to 2000).
+constexpr uint32_t SSW_WCLASSES = 4;
+SA_HD uint32_t ssw_wclass(uint32_t wlen) { return wlen < 256 ? 0u : wlen < 512 ? 1u : wlen < 1024 ? 2u : 3u; }
+// byte forward 0..63 (16 stripe counts x 4 classes), byte reverse 64..79,
+// word forward 80..207 (32 x 4), word reverse 208..239
+constexpr uint32_t SSW_BUCKETS = 240;
+SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg, uint32_t wc) {
+    if (rev) return (word ? 208u : 64u) + seg - 1;
+    return (word ? 80u : 0u) + (seg - 1) * SSW_WCLASSES + wc;
 }
 
 // Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY);
@@ -224,7 +231,7 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
     if (t < counters[0]) {
         const uint32_t rid = req[jobs[t].req].read_id;
         const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
-        const uint32_t b = ssw_bucket(false, false, (L + 15) / 16);
+        const uint32_t b = ssw_bucket(false, false, (L + 15) / 16, ssw_wclass(jobs[t].wlen));
         lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
         c = (uint64_t)L * jobs[t].wlen;
     }
@@ -239,17 +246,28 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
 // (ext/ssw/ssw.c:818-888). A saturated forward byte pass sends the job to the
 // word buckets; a finished forward pass sends it to the reverse bucket of its
 // stripe count; the reverse pass writes the job's SswEnds.
+// The job lists of one launch: the window classes of one stripe count, longest
+// windows first, so that each wave's groups have similar column counts and the
+// long ones start early.
+struct SswLaunchLists {
+    const uint32_t* list[SSW_WCLASSES];
+    uint32_t start[SSW_WCLASSES + 1];  // slot ranges of the lists
+};
+
 template <int G, bool BYTE, int SEG>
 __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                                  RefView ref, Scores sc, const ExtRequest* __restrict__ req,
-                                                 const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ list,
-                                                 uint32_t count, uint32_t rev, SswEnd* __restrict__ fwd,
+                                                 const ExtJob* __restrict__ jobs, SswLaunchLists ll,
+                                                 uint32_t rev, SswEnd* __restrict__ fwd,
                                                  SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
                                                  uint32_t* __restrict__ counts, uint32_t cap) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t slot = tid / G, l = tid % G;
-    if (slot >= count) return;
-    const uint32_t job = list[slot];
+    if (slot >= ll.start[SSW_WCLASSES]) return;
+    uint32_t c = 0;
+    SA_UNROLL
+    for (uint32_t x = 1; x < SSW_WCLASSES; ++x) c += slot >= ll.start[x] ? 1u : 0u;
+    const uint32_t job = ll.list[c][slot - ll.start[c]];
     const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
@@ -263,10 +281,10 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
         if (l == 0) {
             uint32_t b;
             if (BYTE && f.score == 255) {
-                b = ssw_bucket(true, false, (uint32_t)(L + 7) / 8);
+                b = ssw_bucket(true, false, (uint32_t)(L + 7) / 8, ssw_wclass(j.wlen));
             } else {
                 fwd[job] = f;
-                b = ssw_bucket(!BYTE, true, (uint32_t)(f.read + 1 + G - 1) / G);
+                b = ssw_bucket(!BYTE, true, (uint32_t)(f.read + 1 + G - 1) / G, 0);
             }
             lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = job;
         }
