@@ -20,6 +20,7 @@
 #include <array>
 #include <deque>
 #include <exception>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -162,6 +163,74 @@ struct KeyHash {
 };
 static_assert(sizeof(Key) == 40, "Key packing");
 
+// Open-addressing table keyed by Key (linear probing, power-of-two capacity, load
+// at most 1/2): the per-chunk extension cache and pending set take ~10^4 inserts
+// per chunk and round, which a node-per-entry map pays for with an allocation each.
+template <typename V>
+class KeyTable {
+public:
+    V* find(const Key& k) {
+        if (n_ == 0) return nullptr;
+        for (size_t i = KeyHash{}(k) & mask_;; i = (i + 1) & mask_) {
+            if (!used_[i]) return nullptr;
+            if (slots_[i].k == k) return &slots_[i].v;
+        }
+    }
+    size_t count(const Key& k) { return find(k) ? 1 : 0; }
+    // inserts unless present; true if inserted
+    bool emplace(const Key& k, const V& v) {
+        if (2 * (n_ + 1) > cap_) grow(2 * (n_ + 1));
+        for (size_t i = KeyHash{}(k) & mask_;; i = (i + 1) & mask_) {
+            if (!used_[i]) {
+                used_[i] = 1;
+                slots_[i].k = k;
+                slots_[i].v = v;
+                ++n_;
+                return true;
+            }
+            if (slots_[i].k == k) return false;
+        }
+    }
+    void reserve(size_t n) {
+        if (2 * n > cap_) grow(2 * n);
+    }
+    void clear() {
+        if (n_) std::memset(used_.get(), 0, cap_);
+        n_ = 0;
+    }
+    void swap(KeyTable& o) {
+        slots_.swap(o.slots_);
+        used_.swap(o.used_);
+        std::swap(n_, o.n_);
+        std::swap(cap_, o.cap_);
+        std::swap(mask_, o.mask_);
+    }
+    size_t size() const { return n_; }
+
+private:
+    struct Slot {
+        Key k;
+        V v;
+    };
+    void grow(size_t want) {
+        size_t c = 16;
+        while (c < want) c <<= 1;
+        std::unique_ptr<Slot[]> os(new Slot[c]);  // left uninitialised: `used_` says which are live
+        std::unique_ptr<uint8_t[]> ou(new uint8_t[c]());
+        os.swap(slots_);
+        ou.swap(used_);
+        const size_t oc = cap_;
+        cap_ = c;
+        mask_ = c - 1;
+        n_ = 0;
+        for (size_t i = 0; i < oc; ++i)
+            if (ou[i]) emplace(os[i].k, os[i].v);
+    }
+    std::unique_ptr<Slot[]> slots_;
+    std::unique_ptr<uint8_t[]> used_;
+    size_t n_ = 0, cap_ = 0, mask_ = 0;
+};
+
 
 struct CachedResult {
     ExtResult r;
@@ -179,10 +248,10 @@ struct Chunk {
     InsertSize isize;
     std::string out;
     std::vector<double> abundances;
-    std::unordered_map<Key, CachedResult, KeyHash> cache;
+    KeyTable<CachedResult> cache;
     std::vector<uint32_t> cigar_pool;  // CIGARs of the cached results (grows only between passes)
     std::vector<Key> pending;
-    std::unordered_map<Key, char, KeyHash> pending_set;
+    KeyTable<char> pending_set;
     bool done() const { return next >= count; }
 };
 
@@ -206,12 +275,8 @@ struct Ctx {
 // reported as no alignment; the record then runs on as a probe that collects
 // its other requests, and is not committed in this pass.
 const CachedResult* lookup(Ctx& c, const Key& key) {
-    auto it = c.ch->cache.find(key);
-    if (it != c.ch->cache.end()) return &it->second;
-    if (!c.ch->pending_set.count(key)) {
-        c.ch->pending_set.emplace(key, 1);
-        c.ch->pending.push_back(key);
-    }
+    if (CachedResult* hit = c.ch->cache.find(key)) return hit;
+    if (c.ch->pending_set.emplace(key, 1)) c.ch->pending.push_back(key);
     c.missed = true;
     return nullptr;
 }
@@ -1112,19 +1177,85 @@ void run_chunk(Ctx base, Chunk& ch) {
     }
 }
 
+// Worker threads shared by every parallel_for of the process (the engines' host
+// passes run concurrently and draw from one pool instead of each starting its own
+// threads per call). A job takes at most `threads` - 1 helpers besides its caller.
+class HostPool {
+public:
+    struct Job {
+        std::function<void(size_t)> f;
+        size_t n = 0;
+        int helpers = 0;  // helpers still allowed to join
+        std::atomic<size_t> next{0}, done{0};
+        std::mutex m;
+        std::condition_variable cv;
+        void run() {
+            size_t k = 0;
+            for (size_t i; (i = next.fetch_add(1)) < n; ++k) f(i);
+            if (k && done.fetch_add(k) + k == n) {
+                std::lock_guard<std::mutex> lk(m);
+                cv.notify_all();
+            }
+        }
+    };
+    static HostPool& get() {
+        static HostPool p;
+        return p;
+    }
+    void submit(const std::shared_ptr<Job>& j) {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            q_.push_back(j);
+        }
+        cv_.notify_all();
+    }
+
+private:
+    HostPool() {
+        const unsigned n = std::max(2u, std::thread::hardware_concurrency());
+        for (unsigned t = 0; t < n; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void loop() {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+            if (stop_) return;
+            std::shared_ptr<Job> j = q_.front();
+            if (--j->helpers <= 0 || j->next.load() >= j->n) q_.pop_front();
+            lk.unlock();
+            j->run();
+            lk.lock();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<std::shared_ptr<HostPool::Job>> q_;
+    std::vector<std::thread> th_;
+    bool stop_ = false;
+};
+
 template <typename F>
 void parallel_for(int threads, size_t n, F f) {
     if (threads <= 1 || n <= 1) {
         for (size_t i = 0; i < n; ++i) f(i);
         return;
     }
-    std::vector<std::thread> th;
-    std::atomic<size_t> next{0};
-    for (int t = 0; t < std::min<int>(threads, (int)n); ++t)
-        th.emplace_back([&] {
-            for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-        });
-    for (auto& t : th) t.join();
+    auto j = std::make_shared<HostPool::Job>();
+    j->f = [&f](size_t i) { f(i); };
+    j->n = n;
+    j->helpers = std::min<int>(threads, (int)n) - 1;
+    HostPool::get().submit(j);
+    j->run();
+    std::unique_lock<std::mutex> lk(j->m);
+    j->cv.wait(lk, [&] { return j->done.load() == j->n; });
 }
 
 double now_ms() {
