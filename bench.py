@@ -234,7 +234,7 @@ def main():
         # request loop over the same resident index: Poisson arrivals, then the
         # same requests submitted at once (saturation throughput)
         plan = request_plan(len(reads), a.serve_requests, a.serve_rate, 11)
-        slots = a.slots or 3
+        slots = a.slots or 4
         lat, swall, sst, lag = serve(sa, ix, reads, threads, slots, plan, True)
         pairs = sum(n for _, _, n in plan)
         _, sat_wall, sat_st, _ = serve(sa, ix, reads, threads, slots, plan, False)

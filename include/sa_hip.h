@@ -97,7 +97,7 @@ typedef struct sa_map_options {
     uint64_t chunk_size;     /* --chunk-size (1000000) */
     int32_t threads;         /* host threads for the decision replay and output */
     uint64_t batch_bases;    /* bases per device batch (0: default) */
-    int32_t slots;           /* batches in flight on the device at once (0: default 3) */
+    int32_t slots;           /* batches in flight on the device at once (0: default 4) */
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */

@@ -63,7 +63,7 @@ struct Options {
     uint64_t chunk_size = 1000000;
     int threads = 1;
     uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
-    int slots = 3;                        // engines working on batches concurrently
+    int slots = 4;                        // engines working on batches concurrently
 };
 
 // src/io/sam.rs:177-201 (no @PG line)

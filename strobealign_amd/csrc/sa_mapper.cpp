@@ -1464,12 +1464,21 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
     const bool paired = !r2.empty();
     const uint64_t n = paired ? std::min(r1.size(), r2.size()) : r1.size();
     std::vector<Chunk> chunks = make_chunks(r1.data(), paired ? r2.data() : nullptr, n, o.chunk_size);
-    // device batches of whole chunks
+    // device batches of whole chunks: at most batch_bases each, and as many as a
+    // multiple of the engine count, so that the engines finish their last batches
+    // together instead of one engine running the remainder alone
+    std::vector<uint64_t> cbases(chunks.size());
+    uint64_t total_bases = 0;
+    for (size_t c = 0; c < chunks.size(); ++c) total_bases += cbases[c] = chunk_bases(chunks[c]);
+    const uint64_t ne = engines.size();
+    uint64_t nb = std::max<uint64_t>(1, (total_bases + o.batch_bases - 1) / std::max<uint64_t>(1, o.batch_bases));
+    nb = (nb + ne - 1) / ne * ne;
+    const uint64_t target = std::max<uint64_t>(1, (total_bases + nb - 1) / nb);
     std::vector<std::pair<size_t, size_t>> batches;
     for (size_t cb = 0; cb < chunks.size();) {
         size_t ce = cb;
         uint64_t bases = 0;
-        while (ce < chunks.size() && (ce == cb || bases < o.batch_bases)) bases += chunk_bases(chunks[ce++]);
+        while (ce < chunks.size() && (ce == cb || bases < target)) bases += cbases[ce++];
         batches.push_back({cb, ce});
         cb = ce;
     }
