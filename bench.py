@@ -97,13 +97,23 @@ def serve(sa, ix, reads, threads, slots, plan, paced):
         w.start()
     t0 = time.perf_counter()
     lag = 0.0
-    for k, (ta, a, n) in enumerate(plan):
-        if paced:
+    if paced:
+        for k, (ta, a, n) in enumerate(plan):
             d = t0 + ta - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             lag = max(lag, -d)
-        q.put((k, srv.submit(reads, a, n)))
+            q.put((k, srv.submit(reads, a, n)))
+    else:
+        # a backlog from several clients at once: four submitting threads
+        def client(ks):
+            for k in ks:
+                q.put((k, srv.submit(reads, plan[k][1], plan[k][2])))
+        cs = [threading.Thread(target=client, args=(list(range(c, len(plan), 4)),)) for c in range(4)]
+        for c in cs:
+            c.start()
+        for c in cs:
+            c.join()
     for _ in ws:
         q.put(None)
     for w in ws:
@@ -245,6 +255,7 @@ def main():
                    "offered_pairs_per_s": round(pairs / plan[-1][0], 1),
                    "served_pairs_per_s": round(pairs / swall, 1),
                    "saturated_pairs_per_s": round(pairs / sat_wall, 1),
+                   "saturated_clients": 4,
                    "batches": sst["batches"], "max_requests_per_batch": sst["max_requests_per_batch"],
                    "mean_requests_per_batch": round(sst["batch_requests"] / max(1, sst["batches"]), 2),
                    "arena_peak_bytes": sst["arena_peak"], "driver_max_lag_ms": round(1000 * lag, 2),

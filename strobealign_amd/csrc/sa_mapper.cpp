@@ -1549,7 +1549,8 @@ struct Server::Impl {
         bool paired = false;
         std::vector<Chunk> chunks;
         std::vector<uint64_t> chunk_b;    // bases per chunk
-        std::vector<uint64_t> chunk_seg;  // arena offset of each chunk's reads
+        std::vector<uint64_t> chunk_seg;  // offset of each chunk's reads in `packed`, then in the arena
+        std::vector<uint8_t> packed;      // the reads' bases, packed by the submitting thread
         size_t next_chunk = 0;            // next chunk to hand to a batch
         size_t left = 0;                  // chunks not finished
         uint64_t bytes = 0;               // bases of all reads
@@ -1634,21 +1635,9 @@ struct Server::Impl {
             lk.unlock();
             std::string err;
             try {
-                std::vector<uint8_t> buf(r->bytes);
-                uint64_t at = 0;
-                for (size_t c = 0; c < r->chunks.size(); ++c) {
-                    const Chunk& ch = r->chunks[c];
-                    r->chunk_seg.push_back(r->dev + at);
-                    for (uint64_t i = 0; i < ch.count; ++i) {
-                        std::memcpy(buf.data() + at, ch.rec1[i].seq.data(), ch.rec1[i].seq.size());
-                        at += ch.rec1[i].seq.size();
-                        if (r->paired) {
-                            std::memcpy(buf.data() + at, ch.rec2[i].seq.data(), ch.rec2[i].seq.size());
-                            at += ch.rec2[i].seq.size();
-                        }
-                    }
-                }
-                device_copy_in(device, arena + r->dev, buf.data(), r->bytes);
+                device_copy_in(device, arena + r->dev, r->packed.data(), r->bytes);
+                for (uint64_t& sgo : r->chunk_seg) sgo += r->dev;
+                std::vector<uint8_t>().swap(r->packed);
             } catch (const std::exception& e) {
                 err = e.what();
             }
@@ -1791,6 +1780,22 @@ uint64_t Server::submit(std::vector<Record> r1, std::vector<Record> r2, double t
     for (const Chunk& c : r->chunks) {
         r->chunk_b.push_back(chunk_bases(c));
         r->bytes += r->chunk_b.back();
+    }
+    // the reads packed here, by the submitting thread, so that admission is one copy
+    r->packed.resize(r->bytes);
+    {
+        uint64_t at = 0;
+        for (const Chunk& ch : r->chunks) {
+            r->chunk_seg.push_back(at);
+            for (uint64_t i = 0; i < ch.count; ++i) {
+                std::memcpy(r->packed.data() + at, ch.rec1[i].seq.data(), ch.rec1[i].seq.size());
+                at += ch.rec1[i].seq.size();
+                if (r->paired) {
+                    std::memcpy(r->packed.data() + at, ch.rec2[i].seq.data(), ch.rec2[i].seq.size());
+                    at += ch.rec2[i].seq.size();
+                }
+            }
+        }
     }
     if (Impl::round_up(r->bytes + 16) > impl_->arena_bytes)
         throw std::invalid_argument("submit: request reads (" + std::to_string(r->bytes) +
