@@ -152,6 +152,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
+    ap.add_argument("--single-steps", type=int, default=2, help="timed steps of the one-engine comparison (0: skip)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="use the library's default schedule instead of sa_autotune")
+    ap.add_argument("--tune-pairs", type=int, default=0, help="pairs the autotuner times (0: the whole workload)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     a = ap.parse_args()
@@ -177,6 +181,14 @@ def main():
     reads.upload(ix)  # inputs resident in HBM before the timed region
     upload_s = time.perf_counter() - t0
     opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
+    trials = None
+    if a.autotune and not a.slots and not a.batch_bases:
+        # schedule (engines x batch size) chosen by timing the workload itself
+        # (sa_autotune; outputs are identical for every schedule)
+        t0 = time.perf_counter()
+        trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs)
+        print("autotune: %d engines, %d Mbases per batch (%d trials, %.1f s)" %
+              (opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
     print("index %.1fs, %d pairs loaded, staged on the device in %.1f ms" % (index_s, len(reads), 1000 * upload_s),
           file=sys.stderr, flush=True)
     for i in range(a.warmup):
@@ -194,6 +206,17 @@ def main():
     k1 = ix.kernel_times()
     n_pairs = len(reads)
     value = n_pairs * a.steps / wall
+    # the plain per-batch drop-in beside it: one engine, one batch at a time (as a
+    # caller handing batches to sa_map_reads one after another would run it)
+    single = None
+    if a.single_steps > 0:
+        o1 = sa.options(threads=threads, slots=1, batch_bases=a.batch_bases)
+        ix.map_reads(reads, o1, keep_output=False)
+        t1 = time.perf_counter()
+        for _ in range(a.single_steps):
+            ix.map_reads(reads, o1, keep_output=False)
+        single = n_pairs * a.single_steps / (time.perf_counter() - t1)
+        print("one engine: %.0f pairs/s" % single, file=sys.stderr, flush=True)
     # dominant kernel over the timed steps: HIP-event time on the engines' streams;
     # algorithmic bytes are counted by the engine per launch
     z = (0, 0, 0, 0, 0)
@@ -293,7 +316,11 @@ def main():
                    "pairs": n_pairs, "read_len": 150, "host_threads": threads,
                    "read_upload_ms": round(1000 * upload_s, 2),
                    "pairs_per_s_incl_pcie": round(n_pairs * a.steps / (wall + a.steps * upload_s), 1),
-                   "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests},
+                   "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests,
+                   "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
+                   "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed",
+                   "plain_one_engine_pairs_per_s": round(single, 1) if single else None,
+                   "batched_over_plain": round(value / single, 3) if single else None},
         "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"]} if serving else None,
         "serving": serving,
         "roofline": roof,

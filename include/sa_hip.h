@@ -139,6 +139,16 @@ typedef struct sa_run_stats {
 int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* reads, char** out, size_t* out_len,
                  sa_run_stats* stats);
 
+/* Schedule autotuner: maps the first sample_pairs records of `reads` (0: all;
+ * resident if uploaded, output dropped) with each engine count in {1, 2, 3, 4, 6}
+ * and batch size in {16, 32, 64} Mbases, and writes the fastest pair into
+ * opt->slots and opt->batch_bases. The outcome never depends on the schedule (the
+ * output is identical for every choice); only the speed does. report (may be
+ * NULL) receives the trials as JSON [{"slots", "batch_bases", "ms"}, ...]. No
+ * counterpart in the reference, which has no device schedule. */
+int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, sa_map_options* opt, char* report,
+                size_t report_cap);
+
 /* Serving loop: requests (read sets) are submitted as they arrive and mapped
  * with continuous batching. A request is admitted, in arrival order, once its
  * reads fit in a device arena of arena_bytes (0: min(4 GiB, a quarter of the

@@ -63,7 +63,7 @@ EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa
            "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
            "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
            "sa_server_destroy", "sa_server_submit", "sa_server_submit_reads", "sa_server_wait",
-           "sa_server_get_stats", "sa_server_kernel_times", "sa_free"]
+           "sa_server_get_stats", "sa_server_kernel_times", "sa_autotune", "sa_free"]
 
 
 class RunStats(ctypes.Structure):
@@ -228,6 +228,15 @@ class Index:
         s = ctypes.string_at(out, n.value)
         _lib.sa_free(out)
         return s
+
+    def autotune(self, reads, opts, sample_pairs=0):
+        """Chooses opts.slots / opts.batch_bases by timing the read set's leading
+        sample_pairs records (sa_autotune); returns the trials."""
+        import json
+        buf = ctypes.create_string_buffer(8192)
+        _check(_lib.sa_autotune(self._p, reads._p, ctypes.c_uint64(sample_pairs), ctypes.byref(opts), buf,
+                                ctypes.c_size_t(8192)))
+        return json.loads(buf.value.decode())
 
     def map_reads(self, reads, opts, keep_output=True):
         """Maps a Reads set; returns (output bytes or None, RunStats)."""

@@ -369,6 +369,67 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
     });
 }
 
+int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, sa_map_options* opt, char* report,
+                size_t report_cap) {
+    if (!index || !reads || !opt) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        const bool paired = !reads->r2.empty();
+        const uint64_t n = std::min<uint64_t>(reads->r1.size(), sample_pairs ? sample_pairs : reads->r1.size());
+        // the sample: the leading records, resident when the read set is
+        std::vector<sa::Record> s1, s2;
+        const bool whole = n == reads->r1.size();
+        if (!whole) {
+            s1.assign(reads->r1.begin(), reads->r1.begin() + (ptrdiff_t)n);
+            if (paired) s2.assign(reads->r2.begin(), reads->r2.begin() + (ptrdiff_t)n);
+        }
+        const std::vector<sa::Record>& r1 = whole ? reads->r1 : s1;
+        const std::vector<sa::Record>& r2 = whole ? reads->r2 : s2;
+        sa::DeviceReads dr;
+        const sa::DeviceReads* pdr = nullptr;
+        if (reads->d_seq && reads->device == index->device) {
+            dr.d_seq = reads->dr.d_seq;
+            dr.off.assign(reads->dr.off.begin(), reads->dr.off.begin() + (ptrdiff_t)((paired ? 2 : 1) * n + 1));
+            pdr = &dr;
+        }
+        const int slot_set[] = {1, 2, 3, 4, 6};
+        const uint64_t batch_set[] = {16ull << 20, 32ull << 20, 64ull << 20};
+        double best = 1e300;
+        int best_slots = 0;
+        uint64_t best_batch = 0;
+        std::string rep = "[";
+        for (int sl : slot_set)
+            for (uint64_t bb : batch_set) {
+                sa::Options o = options_of(opt);
+                o.slots = sl;
+                o.batch_bases = bb;
+                double ms = 0;
+                for (int rep_i = 0; rep_i < 2; ++rep_i) {  // the first run grows the buffers
+                    sa::RunStats st;
+                    sa::map_records(engines(index, sl), index->ref, r1, r2, o, nullptr, &st, pdr);
+                    ms = st.total_ms;
+                }
+                char tmp[128];
+                std::snprintf(tmp, sizeof tmp, "%s{\"slots\": %d, \"batch_bases\": %llu, \"ms\": %.2f}",
+                              rep.size() > 1 ? ", " : "", sl, (unsigned long long)bb, ms);
+                rep += tmp;
+                if (ms < best) {
+                    best = ms;
+                    best_slots = sl;
+                    best_batch = bb;
+                }
+            }
+        rep += "]";
+        opt->slots = best_slots;
+        opt->batch_bases = best_batch;
+        if (report && report_cap) {
+            const size_t m = std::min(report_cap - 1, rep.size());
+            std::memcpy(report, rep.data(), m);
+            report[m] = 0;
+        }
+        return SA_OK;
+    });
+}
+
 int sa_server_create(sa_index* index, const sa_map_options* opt, uint64_t arena_bytes, sa_server** out) {
     if (!index || !out) return fail(SA_ERR_ARG, "NULL argument");
     return guard([&] {

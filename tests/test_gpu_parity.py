@@ -186,6 +186,28 @@ def test_resident_reads_and_concurrent_batches(sa, ctx, tmp_path_factory):
         reads.close()
 
 
+def test_autotuned_schedule_gives_the_same_sam(sa, ctx, tmp_path_factory):
+    """sa_autotune picks an (engines, batch size) schedule from its candidate set;
+    mapping with it gives the oracle's SAM like every schedule does."""
+    say("case: autotuned schedule, 100 Mbp PE 3000 pairs")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("tune"))
+    w = synth.make("pe100m", d, 3000)
+    exp = oracle("-t", "8", "--no-PG", w["ref"], w["reads1"], w["reads2"])
+    with sa.Index(ctx, w["ref"], read_length=150, threads=8) as ix:
+        reads = sa.Reads(w["reads1"], w["reads2"])
+        reads.upload(ix)
+        opts = sa.options(threads=8)
+        trials = ix.autotune(reads, opts)
+        assert len(trials) == 15 and all(t["ms"] > 0 for t in trials)
+        assert opts.slots in (1, 2, 3, 4, 6) and opts.batch_bases in (16 << 20, 32 << 20, 64 << 20)
+        assert min(trials, key=lambda t: t["ms"])["slots"] == opts.slots
+        out, _ = ix.map_reads(reads, opts)
+        reads.close()
+    assert out == exp
+
+
 def test_ssw_against_reference_vectors(ctx, orc):
     """Aligner::align on the device vs the oracle, over the queries/references of
     the vectors written by the reference's ssw.c (ACGT-only references)."""
