@@ -211,18 +211,32 @@ def main():
     single = None
     if a.single_steps > 0:
         o1 = sa.options(threads=threads, slots=1, batch_bases=a.batch_bases)
-        ix.map_reads(reads, o1, keep_output=False)
+        ix.map_reads(reads, o1, keep_output=False)  # grows the engine's buffers to this schedule
+        k1b = ix.kernel_times()
         t1 = time.perf_counter()
         for _ in range(a.single_steps):
             ix.map_reads(reads, o1, keep_output=False)
         single = n_pairs * a.single_steps / (time.perf_counter() - t1)
         print("one engine: %.0f pairs/s" % single, file=sys.stderr, flush=True)
+    k2 = ix.kernel_times()
     # dominant kernel over the timed steps: HIP-event time on the engines' streams;
     # algorithmic bytes are counted by the engine per launch
     z = (0, 0, 0, 0, 0)
     delta = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
-    dom = max(delta, key=lambda k: delta[k][0])
-    ms, launches, items, nbytes, _ = delta[dom]
+    # Per-kernel figures come from the one-engine pass when it ran: there each kernel
+    # has the device to itself, so its HIP-event time is its own duration (with
+    # several engines, events also count the time a kernel shares the device or
+    # waits behind another engine's kernels).
+    kd = delta
+    kd_steps = a.steps
+    kd_source = "the %d-engine timed steps" % (opts.slots or 4)
+    if single:
+        kd = {k: tuple(k2[k][x] - k1b.get(k, z)[x] for x in range(5)) for k in k2}
+        kd_steps = a.single_steps
+        kd_source = "the one-engine pass (%d steps, %d Mbases per batch)" % (a.single_steps,
+                                                                           (a.batch_bases or (64 << 20)) >> 20)
+    dom = max(kd, key=lambda k: kd[k][0])
+    ms, launches, items, nbytes, _ = kd[dom]
     per_launch_ms = ms / max(1, launches)
     bytes_per_launch = nbytes / max(1, launches)
     achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
@@ -235,33 +249,46 @@ def main():
             tj = json.load(f)
     measured = tj.get("kernels", {}) if tj.get("workload") == "pe100m" else {}
     traffic = measured[dom]["hbm_bytes_per_launch_scaled"] if dom in measured else None
+    # the profile's batch size may differ from this run's schedule: per-launch
+    # traffic then scales with the bases per batch (noted in the line)
+    prof_batch = tj.get("schedule", {}).get("batch_bases")
+    traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE, same schedule"
+    run_batch = (a.batch_bases or (64 << 20)) if single else (opts.batch_bases or (64 << 20))
+    if traffic is not None and prof_batch and prof_batch != run_batch:
+        traffic = int(traffic * run_batch / prof_batch)
+        traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE at %d Mbases per batch, scaled to %d" % (
+            prof_batch >> 20, run_batch >> 20)
     # index lookups: useful bytes (counted per probe) against the sector traffic
     lookup = None
-    if delta.get("k_hits", z)[0] > 0:
-        ms_, l_, _, b_, _ = delta["k_hits"]
+    if kd.get("k_hits", z)[0] > 0:
+        ms_, l_, _, b_, _ = kd["k_hits"]
         avg = ms_ / max(1, l_)
         lookup = {"kernel": "k_hits", "avg_launch_ms": round(avg, 3), "useful_bytes_per_launch": int(b_ / max(1, l_)),
                   "useful_gbps": round(b_ / (ms_ / 1000.0) / 1e9, 2), "peak_gbps": HBM_PEAK_GBS}
         if "k_hits" in measured:
             tb = measured["k_hits"]["hbm_bytes_per_launch_scaled"]
+            if prof_batch and prof_batch != run_batch:
+                tb = int(tb * run_batch / prof_batch)
             lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1)})
     # forward SSW pass: DP cells per second against the VALU integer peak
     # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
     # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
     extension = None
-    if delta.get("k_ssw_byte_fwd", z)[4] > 0 and delta["k_ssw_byte_fwd"][0] > 0:
-        ms_, _, _, _, cells = delta["k_ssw_byte_fwd"]
+    if kd.get("k_ssw_byte_fwd", z)[4] > 0 and kd["k_ssw_byte_fwd"][0] > 0:
+        ms_, _, _, _, cells = kd["k_ssw_byte_fwd"]
         gcups = cells / (ms_ / 1000.0) / 1e9
-        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / a.steps),
+        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / kd_steps),
                      "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
                      "frac": round(gcups * 10 / 39300.0, 4)}
     copy_gbps = ctx.hbm_copy_gbps()
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_note,
+            "kernel": dom,
             "avg_launch_ms": round(per_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
             "measured_copy_gbps": round(copy_gbps, 1),
-            "kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in delta.items()},
-            "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in delta.items() if v[3] and v[0]}}
+            "kernel_ms_per_step": {k: round(v[0] / kd_steps, 2) for k, v in kd.items()},
+            "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in kd.items() if v[3] and v[0]},
+            "kernel_times_from": kd_source}
     serving = None
     if a.serve_requests > 0:
         # request loop over the same resident index: Poisson arrivals, then the
