@@ -43,8 +43,9 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(ref, r1, r2, pairs, threads, workdir):
-    """Times the oracle on the first `pairs` pairs (index build excluded)."""
+def cpu_baseline(ref, r1, r2, pairs, threads, workdir, out=os.devnull, t1_chunks=3):
+    """Times the oracle on the first `pairs` pairs (index build excluded); its
+    SAM goes to `out`. Returns (pairs/s, oracle timings, sample files)."""
     from strobealign_amd import synth  # noqa: F401
     oracle = os.path.join(ROOT, "oracle", "build", "sa_oracle")
     if not os.path.exists(oracle):
@@ -56,10 +57,10 @@ def cpu_baseline(ref, r1, r2, pairs, threads, workdir):
                 if i >= 4 * pairs:
                     break
                 g.write(line)
-    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "--t1-chunks", "3", "-o", os.devnull, ref,
-                        s1, s2], check=True, capture_output=True, text=True)
+    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "--t1-chunks", str(t1_chunks), "-o", out,
+                        ref, s1, s2], check=True, capture_output=True, text=True)
     t = json.loads(p.stderr.strip().splitlines()[-1])
-    return pairs / t["map_s"], t
+    return pairs / t["map_s"], t, (s1, s2)
 
 
 def request_plan(n_pairs, n_req, rate, seed):
@@ -140,15 +141,186 @@ def cpu_model():
     return "unknown"
 
 
+def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
+    """Roofline of the dominant kernel, lookup and extension figures from
+    per-kernel (ms, launches, items, bytes, cells) deltas over kd_steps steps."""
+    z = (0, 0, 0, 0, 0)
+    dom = max(kd, key=lambda k: kd[k][0])
+    ms, launches, items, nbytes, _ = kd[dom]
+    per_launch_ms = ms / max(1, launches)
+    bytes_per_launch = nbytes / max(1, launches)
+    achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    # of this workload (copy-kernel scaled)
+    tj = {}
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+    measured = tj.get("kernels", {}) if tj.get("workload") == workload else {}
+    traffic = measured[dom]["hbm_bytes_per_launch_scaled"] if dom in measured else None
+    # the profile's batch size may differ from this run's schedule: per-launch
+    # traffic then scales with the bases per batch (noted in the line)
+    prof_batch = tj.get("schedule", {}).get("batch_bases")
+    traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE, same schedule"
+    if traffic is not None and prof_batch and prof_batch != run_batch:
+        traffic = int(traffic * run_batch / prof_batch)
+        traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE at %d Mbases per batch, scaled to %d" % (
+            prof_batch >> 20, run_batch >> 20)
+    # index lookups: useful bytes (counted per probe) against the sector traffic
+    lookup = None
+    if kd.get("k_hits", z)[0] > 0:
+        ms_, l_, _, b_, _ = kd["k_hits"]
+        avg = ms_ / max(1, l_)
+        lookup = {"kernel": "k_hits", "avg_launch_ms": round(avg, 3), "useful_bytes_per_launch": int(b_ / max(1, l_)),
+                  "useful_gbps": round(b_ / (ms_ / 1000.0) / 1e9, 2), "peak_gbps": HBM_PEAK_GBS}
+        if "k_hits" in measured:
+            tb = measured["k_hits"]["hbm_bytes_per_launch_scaled"]
+            if prof_batch and prof_batch != run_batch:
+                tb = int(tb * run_batch / prof_batch)
+            lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1)})
+    # forward SSW pass: DP cells per second against the VALU integer peak
+    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
+    # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
+    extension = None
+    if kd.get("k_ssw_byte_fwd", z)[4] > 0 and kd["k_ssw_byte_fwd"][0] > 0:
+        ms_, _, _, _, cells = kd["k_ssw_byte_fwd"]
+        gcups = cells / (ms_ / 1000.0) / 1e9
+        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / kd_steps),
+                     "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
+                     "frac": round(gcups * 10 / 39300.0, 4)}
+    copy_gbps = ctx.hbm_copy_gbps()
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_note,
+            "kernel": dom,
+            "avg_launch_ms": round(per_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
+            "measured_copy_gbps": round(copy_gbps, 1),
+            "kernel_ms_per_step": {k: round(v[0] / kd_steps, 2) for k, v in kd.items()},
+            "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in kd.items() if v[3] and v[0]},
+            "kernel_times_from": kd_source}
+    return roof, lookup, extension
+
+
+def bench_human(a, sa, synth, cores, threads):
+    """Human-scale workload: 3.1 Gbp reference (24 contigs, seed 11, 45% of it
+    copies of 300 bp-50 kbp segments at 0-15% divergence) with its full index
+    resident in HBM (bits 28, about 620M 16-byte entries); 2x250 pairs at 3%
+    substitutions and 0.3% indels, mapped in alignment mode (SAM) and in
+    mapping-only mode (-x, PAF). The oracle leg maps a sample of the same pairs
+    on the host cores and its SAM is compared with the library's on that sample."""
+    t0 = time.time()
+    w = synth.make_human(a.data_dir, a.pairs)
+    gen_s = time.time() - t0
+    print("data ready in %.1fs: %s" % (gen_s, w), file=sys.stderr, flush=True)
+    ctx = sa.Context(0)
+    t0 = time.time()
+    ix = sa.Index(ctx, w["ref"], read_length=250, threads=threads)
+    index_s = time.time() - t0
+    params, n_entries, n_contigs = ix.params()
+    print("index %.1fs: %d entries, bits %d, %d contigs" % (index_s, n_entries, params.bits, n_contigs),
+          file=sys.stderr, flush=True)
+    reads = sa.Reads(w["reads1"], w["reads2"])
+    t0 = time.perf_counter()
+    reads.upload(ix)
+    upload_s = time.perf_counter() - t0
+    n_pairs = len(reads)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
+    trials = None
+    if a.autotune and not a.slots and not a.batch_bases:
+        t0 = time.perf_counter()
+        trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs or min(n_pairs, 250_000))
+        print("autotune: %d engines, %d Mbases per batch (%d trials, %.1f s)" %
+              (opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    res = {}
+    for name, mode in (("sam", 0), ("paf", 1)):
+        o = sa.options(mode=mode, threads=threads, slots=opts.slots, batch_bases=opts.batch_bases)
+        for i in range(a.warmup):
+            _, st = ix.map_reads(reads, o, keep_output=False)
+            print("%s warmup %d: %.1f ms" % (name, i, st.total_ms), file=sys.stderr, flush=True)
+        k0 = ix.kernel_times()
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            _, st = ix.map_reads(reads, o, keep_output=False)
+            print("%s step %d: %.1f ms (rounds %d, extension requests %d)" %
+                  (name, i, st.total_ms, st.rounds, st.requests), file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        k1 = ix.kernel_times()
+        z = (0, 0, 0, 0, 0)
+        kd = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
+        res[name] = (n_pairs * a.steps / wall, wall, st, kd)
+    value, wall, st, kd = res["sam"]
+    roof, lookup, extension = kernel_report(ctx, kd, a.steps, "the %d-engine SAM steps" % (opts.slots or 4),
+                                            opts.batch_bases or (64 << 20), "human")
+    cpu = None
+    parity = None
+    if not a.no_cpu and a.cpu_pairs != 0:
+        sample = min(a.cpu_pairs or 20_000, n_pairs)
+        osam = os.path.join(a.data_dir, "oracle_sample.sam")
+        try:
+            v, t, (s1, s2) = cpu_baseline(w["ref"], w["reads1"], w["reads2"], sample, cores, a.data_dir, out=osam,
+                                          t1_chunks=1)
+            cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
+                   "sample": "first %d pairs of the same workload, oracle -t %d, SAM; index build excluded "
+                             "(%.0f s, one thread)" % (sample, cores, t["index_s"]),
+                   "label": "CPU restatement of reference v0.18.0-alpha (oracle/), not the upstream binary",
+                   "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                   "single_thread_value": round(t["t1_records"] / t["t1_s"], 1) if t.get("t1_s") else None,
+                   "stage_seconds_summed_over_threads": t.get("stages_s")}
+            mine = ix.map_files(s1, s2, threads=threads)
+            with open(osam, "rb") as f:
+                theirs = f.read()
+            parity = {"pairs": sample, "identical_sam": mine == theirs, "bytes": len(theirs)}
+            print("oracle sample: %.0f pairs/s; SAM identical: %s" % (v, parity["identical_sam"]), file=sys.stderr,
+                  flush=True)
+        except Exception as e:  # reported, never fatal
+            cpu = {"value": None, "unit": "read pairs/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
+    out = {
+        "metric": "read pairs/sec at 2x250 on 3.1 Gbp ref (SAM; -x PAF beside it)",
+        "value": round(value, 1),
+        "unit": "read pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1000.0 * wall / a.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (strobealign_amd/synth.py human, seed 11)",
+        "config": {"workload": "3.1 Gbp reference (24 contigs, 45%% repeats), %d pairs 2x250, 3%% subs, "
+                               "0.3%% indels" % n_pairs,
+                   "pairs": n_pairs, "read_len": 250, "host_threads": threads,
+                   "index_entries": n_entries, "index_bits": params.bits, "index_build_s": round(index_s, 1),
+                   "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
+                   "paf_pairs_per_s": round(res["paf"][0], 1),
+                   "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
+                   "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
+                   "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed"},
+        "roofline": roof,
+        "lookup": lookup,
+        "extension": extension,
+        "paf_kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in res["paf"][3].items() if v[0]},
+        "cpu_baseline": cpu,
+        "parity_sample": parity,
+    }
+    print(json.dumps(out), flush=True)
+    reads.close()
+    ix.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["pe100m", "human"], default="pe100m",
+                    help="pe100m: 100 Mbp, 2x150 (the headline line); human: 3.1 Gbp, 2x250, SAM and -x")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all cores)")
     ap.add_argument("--data-dir", default=os.environ.get("SA_BENCH_DATA", "/tmp/strobealign_amd_bench"))
-    ap.add_argument("--cpu-pairs", type=int, default=200_000, help="oracle sample size (0: skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=None,
+                    help="oracle sample size (0: skip; default 200,000, human 20,000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
@@ -168,6 +340,10 @@ def main():
 
     cores = host_cores()
     threads = a.threads or cores
+    if a.workload == "human":
+        return bench_human(a, sa, synth, cores, threads)
+    if a.cpu_pairs is None:
+        a.cpu_pairs = 200_000
     t0 = time.time()
     w = synth.make("pe100m", a.data_dir, a.pairs)
     gen_s = time.time() - t0
@@ -235,60 +411,8 @@ def main():
         kd_steps = a.single_steps
         kd_source = "the one-engine pass (%d steps, %d Mbases per batch)" % (a.single_steps,
                                                                            (a.batch_bases or (64 << 20)) >> 20)
-    dom = max(kd, key=lambda k: kd[k][0])
-    ms, launches, items, nbytes, _ = kd[dom]
-    per_launch_ms = ms / max(1, launches)
-    bytes_per_launch = nbytes / max(1, launches)
-    achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    # of this workload (copy-kernel scaled)
-    tj = {}
-    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tj = json.load(f)
-    measured = tj.get("kernels", {}) if tj.get("workload") == "pe100m" else {}
-    traffic = measured[dom]["hbm_bytes_per_launch_scaled"] if dom in measured else None
-    # the profile's batch size may differ from this run's schedule: per-launch
-    # traffic then scales with the bases per batch (noted in the line)
-    prof_batch = tj.get("schedule", {}).get("batch_bases")
-    traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE, same schedule"
     run_batch = (a.batch_bases or (64 << 20)) if single else (opts.batch_bases or (64 << 20))
-    if traffic is not None and prof_batch and prof_batch != run_batch:
-        traffic = int(traffic * run_batch / prof_batch)
-        traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE at %d Mbases per batch, scaled to %d" % (
-            prof_batch >> 20, run_batch >> 20)
-    # index lookups: useful bytes (counted per probe) against the sector traffic
-    lookup = None
-    if kd.get("k_hits", z)[0] > 0:
-        ms_, l_, _, b_, _ = kd["k_hits"]
-        avg = ms_ / max(1, l_)
-        lookup = {"kernel": "k_hits", "avg_launch_ms": round(avg, 3), "useful_bytes_per_launch": int(b_ / max(1, l_)),
-                  "useful_gbps": round(b_ / (ms_ / 1000.0) / 1e9, 2), "peak_gbps": HBM_PEAK_GBS}
-        if "k_hits" in measured:
-            tb = measured["k_hits"]["hbm_bytes_per_launch_scaled"]
-            if prof_batch and prof_batch != run_batch:
-                tb = int(tb * run_batch / prof_batch)
-            lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1)})
-    # forward SSW pass: DP cells per second against the VALU integer peak
-    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
-    # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
-    extension = None
-    if kd.get("k_ssw_byte_fwd", z)[4] > 0 and kd["k_ssw_byte_fwd"][0] > 0:
-        ms_, _, _, _, cells = kd["k_ssw_byte_fwd"]
-        gcups = cells / (ms_ / 1000.0) / 1e9
-        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / kd_steps),
-                     "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
-                     "frac": round(gcups * 10 / 39300.0, 4)}
-    copy_gbps = ctx.hbm_copy_gbps()
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_note,
-            "kernel": dom,
-            "avg_launch_ms": round(per_launch_ms, 3), "bytes_per_launch": int(bytes_per_launch),
-            "measured_copy_gbps": round(copy_gbps, 1),
-            "kernel_ms_per_step": {k: round(v[0] / kd_steps, 2) for k, v in kd.items()},
-            "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in kd.items() if v[3] and v[0]},
-            "kernel_times_from": kd_source}
+    roof, lookup, extension = kernel_report(ctx, kd, kd_steps, kd_source, run_batch, "pe100m")
     serving = None
     if a.serve_requests > 0:
         # request loop over the same resident index: Poisson arrivals, then the
@@ -315,7 +439,7 @@ def main():
     cpu = None
     if not a.no_cpu and a.cpu_pairs > 0:
         try:
-            v, t = cpu_baseline(w["ref"], w["reads1"], w["reads2"], min(a.cpu_pairs, n_pairs), cores, a.data_dir)
+            v, t, _ = cpu_baseline(w["ref"], w["reads1"], w["reads2"], min(a.cpu_pairs, n_pairs), cores, a.data_dir)
             cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
                    "sample": "first %d pairs of the same workload, oracle -t %d, index build excluded (%.1f s map)"
                              % (min(a.cpu_pairs, n_pairs), cores, t["map_s"]),

@@ -94,6 +94,46 @@ SA_HD int32_t ssw_code_comp(uint8_t c) {
 }
 SA_HD int32_t ssw_mat(int32_t a, int32_t b, const Scores& sc) { return (a == b && a < 4) ? sc.match : -sc.mismatch; }
 
+// Maximum-sum segment summary: total, best prefix, best suffix and best
+// segment (each of the last three at least 0, the empty segment).
+struct SegSum {
+    int32_t tot = 0, pre = 0, suf = 0, best = 0;
+    SA_HD void push(int32_t v) {
+        tot += v;
+        pre = pre > tot ? pre : tot;
+        suf = suf + v > 0 ? suf + v : 0;
+        best = best > suf ? best : suf;
+    }
+    // this segment followed by b
+    SA_HD void join(const SegSum& b) {
+        const int32_t p = tot + b.pre > pre ? tot + b.pre : pre;
+        const int32_t s = b.tot + suf > b.suf ? b.tot + suf : b.suf;
+        int32_t m = best > b.best ? best : b.best;
+        m = m > suf + b.pre ? m : suf + b.pre;
+        tot += b.tot;
+        pre = p;
+        suf = s;
+        best = m;
+    }
+};
+
+// Substitution score of query position x against window position x + off under
+// the SSW passes' match/mismatch scores (reference codes are 0-3; a query
+// wildcard, code 4, scores a mismatch).
+template <typename Q>
+SA_HD int32_t diagonal_cell(const Q& q, const RefWin& w, uint32_t x, const Scores& sc) {
+    return ssw_code(q[x]) == (int32_t)w.code(x) ? sc.match : -sc.mismatch;
+}
+
+// Best ungapped segment of the query on one diagonal of the window (Kadane): a
+// lower bound of the local alignment score the SSW passes find.
+template <typename Q>
+SA_HD int32_t diagonal_segment_score(const Q& q, const RefWin& w, uint32_t n, const Scores& sc) {
+    SegSum s;
+    for (uint32_t x = 0; x < n; ++x) s.push(diagonal_cell(q, w, x, sc));
+    return s.best;
+}
+
 // Substitution scores of the SSW passes, called as score(reference code, query
 // code): the reference's 5x5 match/mismatch matrix (src/ssw.rs:139-149), or an
 // n x n matrix as ssw_init takes it (row = reference code, ext/ssw/ssw.c:179, 656).

@@ -6,6 +6,7 @@
 #include <memory>
 #include <optional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sa_engine.hpp"
@@ -32,9 +33,25 @@ SeedParams seed_params_for_read_length(uint64_t read_length);
 // src/indexer.rs:294-300
 int pick_bits(const PackedRef& ref, const SeedParams& p);
 
+// Allocator whose resize() leaves new elements uninitialised: the index
+// entries are written right after, in parallel, which also spreads the first
+// touch of the pages over the threads.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <typename U>
+    void construct(U* p) noexcept { ::new ((void*)p) U; }
+    template <typename U, typename... A>
+    void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+};
+
 struct HostIndex {
     SeedParams p;
-    std::vector<RefEntry> entries;
+    std::vector<RefEntry, NoInitAlloc<RefEntry>> entries;
     std::vector<uint64_t> bucket_starts;
 };
 // src/indexer.rs:18-170 (host construction with multiple threads)

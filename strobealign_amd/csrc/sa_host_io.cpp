@@ -1,5 +1,8 @@
 // Reference/read input and host-side index construction for the drop-in.
 #include <algorithm>
+#include <array>
+#include <cstdio>
+#include <memory>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -43,50 +46,83 @@ void parallel_for(int threads, size_t n, F f) {
 }
 }  // namespace
 
-// src/io/fasta.rs:116-165 and src/packed_seq.rs:56-92 (wildcards -> position hash)
+// src/io/fasta.rs:116-165 and src/packed_seq.rs:56-92 (wildcards -> position hash).
+// The file is read in blocks; each line is split at '\n', trailing whitespace
+// is stripped and the bases are packed 2 bits each, 32 to a word.
 PackedRef read_fasta(const std::string& path) {
-    std::ifstream f(path, std::ios::binary);
+    std::FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) throw std::runtime_error("cannot open " + path);
+    std::unique_ptr<std::FILE, int (*)(std::FILE*)> close(f, &std::fclose);
+    static const auto code = [] {
+        std::array<uint8_t, 256> t;
+        t.fill(4);
+        t['A'] = t['a'] = 0;
+        t['C'] = t['c'] = 1;
+        t['G'] = t['g'] = 2;
+        t['T'] = t['t'] = t['U'] = t['u'] = 3;
+        return t;
+    }();
     PackedRef r;
-    std::string line;
-    uint64_t cur_len = 0;
-    bool have = false;
     r.word_off.push_back(0);
-    auto finish = [&]() {
+    uint64_t cur_len = 0, word = 0;
+    bool have = false;
+    auto flush_word = [&] {
+        if (cur_len & 31) r.words.push_back(word);
+        word = 0;
+    };
+    auto finish = [&] {
         if (!have) return;
+        flush_word();
         r.len.push_back(cur_len);
         r.word_off.push_back(r.words.size());
     };
-    while (std::getline(f, line)) {
-        if (!line.empty() && line[0] == '>') {
+    auto line = [&](const char* p, size_t n) {
+        if (n && p[0] == '>') {
             finish();
             std::string name, comment;
             bool hc;
-            split_header(line.substr(1), name, comment, hc);
+            split_header(std::string(p + 1, n - 1), name, comment, hc);
             r.names.push_back(name);
             cur_len = 0;
+            word = 0;
             have = true;
-        } else if (have) {
-            rstrip(line);
-            for (unsigned char c : line) {
-                uint64_t b;
-                switch (c) {
-                    case 'A': case 'a': b = 0; break;
-                    case 'C': case 'c': b = 1; break;
-                    case 'G': case 'g': b = 2; break;
-                    case 'T': case 't': case 'U': case 'u': b = 3; break;
-                    default: {
-                        uint64_t h = cur_len * 0x9e3779b97f4a7c15ULL;
-                        b = (h ^ (h >> 30)) & 3;
-                    }
-                }
-                uint64_t bit = (cur_len & 31) * 2;
-                if (bit == 0) r.words.push_back(b);
-                else r.words.back() |= b << bit;
-                ++cur_len;
+            return;
+        }
+        if (!have) return;
+        while (n && (p[n - 1] == ' ' || p[n - 1] == '\t' || p[n - 1] == '\r' || p[n - 1] == '\f' || p[n - 1] == '\v'))
+            --n;
+        for (size_t i = 0; i < n; ++i) {
+            uint64_t b = code[(unsigned char)p[i]];
+            if (b == 4) {
+                uint64_t h = cur_len * 0x9e3779b97f4a7c15ULL;
+                b = (h ^ (h >> 30)) & 3;
+            }
+            word |= b << ((cur_len & 31) * 2);
+            if ((++cur_len & 31) == 0) {
+                r.words.push_back(word);
+                word = 0;
             }
         }
+    };
+    std::vector<char> buf(std::size_t(64) << 20);
+    size_t kept = 0;  // bytes of an unfinished line at the front of buf
+    for (;;) {
+        if (kept == buf.size()) buf.resize(buf.size() * 2);
+        size_t got = std::fread(buf.data() + kept, 1, buf.size() - kept, f);
+        size_t end = kept + got;
+        size_t at = 0;
+        for (const char* nl; (nl = (const char*)std::memchr(buf.data() + at, '\n', end - at));) {
+            line(buf.data() + at, (size_t)(nl - buf.data()) - at);
+            at = (size_t)(nl - buf.data()) + 1;
+        }
+        if (got == 0) {
+            if (at < end) line(buf.data() + at, end - at);
+            break;
+        }
+        std::memmove(buf.data(), buf.data() + at, end - at);
+        kept = end - at;
     }
+    if (std::ferror(f)) throw std::runtime_error("cannot read " + path);
     finish();
     if (r.names.empty()) throw std::runtime_error("No sequences found in the reference FASTA");
     return r;
@@ -154,31 +190,41 @@ void build_index(const PackedRef& ref, SeedParams p, int bits, double filter_fra
             v[a] = RefEntry{(h & REF_HASH_MASK) | (uint64_t)((p2 - p1) & 0xFF), p1, (uint32_t)c};
         }
     });
-    std::vector<RefEntry>& rs = out.entries;
+    // Sort: scatter every contig's entries into 4096 partitions by the top 12
+    // hash bits (the leading part of the sort key), then sort the partitions
+    // independently. The key (hash_offset, position, ref_index) is a total order.
+    constexpr int PB = 12;
+    constexpr size_t NP = size_t(1) << PB;
+    auto part = [](const RefEntry& e) { return (size_t)(e.hash_offset >> (64 - PB)); };
+    std::vector<std::vector<uint64_t>> cnt(nc, std::vector<uint64_t>(NP, 0));
+    parallel_for(threads, nc, [&](size_t c) {
+        for (const RefEntry& e : per[c]) ++cnt[c][part(e)];
+    });
+    std::vector<uint64_t> pstart(NP + 1, 0);
     size_t total = 0;
-    for (auto& v : per) total += v.size();
+    for (size_t q = 0; q < NP; ++q) {
+        pstart[q] = total;
+        for (uint32_t c = 0; c < nc; ++c) {
+            uint64_t n = cnt[c][q];
+            cnt[c][q] = total;  // this contig's first slot in partition q
+            total += n;
+        }
+    }
+    pstart[NP] = total;
+    auto& rs = out.entries;
     rs.clear();
-    rs.reserve(total);
-    for (auto& v : per) { rs.insert(rs.end(), v.begin(), v.end()); std::vector<RefEntry>().swap(v); }
+    rs.resize(total);
+    parallel_for(threads, nc, [&](size_t c) {
+        std::vector<uint64_t>& at = cnt[c];
+        for (const RefEntry& e : per[c]) rs[at[part(e)]++] = e;
+        std::vector<RefEntry>().swap(per[c]);
+    });
     auto less = [](const RefEntry& a, const RefEntry& b) {
         if (a.hash_offset != b.hash_offset) return a.hash_offset < b.hash_offset;
         if (a.position != b.position) return a.position < b.position;
         return a.ref_index < b.ref_index;
     };
-    // parallel sort: sort T slices, then merge pairwise (the key is a total order)
-    int T = std::max(1, std::min(threads, 64));
-    std::vector<size_t> bounds;
-    for (int t = 0; t <= T; ++t) bounds.push_back(total * (size_t)t / (size_t)T);
-    parallel_for(T, (size_t)T, [&](size_t t) { std::sort(rs.begin() + bounds[t], rs.begin() + bounds[t + 1], less); });
-    for (size_t width = 1; width < (size_t)T; width *= 2) {
-        std::vector<size_t> starts;
-        for (size_t t = 0; t + width < (size_t)T; t += 2 * width) starts.push_back(t);
-        parallel_for(T, starts.size(), [&](size_t x) {
-            size_t t = starts[x];
-            size_t lo = bounds[t], mid = bounds[t + width], hi = bounds[std::min((size_t)T, t + 2 * width)];
-            std::inplace_merge(rs.begin() + lo, rs.begin() + mid, rs.begin() + hi, less);
-        });
-    }
+    parallel_for(threads, NP, [&](size_t q) { std::sort(rs.begin() + pstart[q], rs.begin() + pstart[q + 1], less); });
     // bucket directory exactly as src/indexer.rs:77-129 builds it
     std::vector<uint64_t>& bs = out.bucket_starts;
     bs.clear();

@@ -158,9 +158,9 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
     const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
     QueryView q{seq + ro, L, (r.flags & 4) != 0};
     const uint64_t rlen = ref.len[r.ref_id];
-    auto push_ssw = [&](uint64_t ws, uint64_t we) {
+    auto push_ssw = [&](uint64_t ws, uint64_t we, uint32_t jflags) {
         if (we - ws > 2000 || L == 0) return;  // Aligner::align returns None (src/aligner.rs:83-86)
-        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), 0};
+        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), jflags};
         if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000 || sc.match > 127 || sc.mismatch > 127)
             long_jobs[SA_ATOMIC_ADD(&counters[1], 1u)] = j;
         else ssw_jobs[SA_ATOMIC_ADD(&counters[0], 1u)] = j;
@@ -187,7 +187,11 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
         if (use_ssw) {
             uint64_t ws = prs > padding ? prs - padding : 0;
             uint64_t we = pre + padding < rlen ? pre + padding : rlen;
-            push_ssw(ws, we);
+            // the NAM's diagonal (query position 0 at window position prs - ws)
+            const uint32_t o = (uint32_t)(prs - ws), wl = (uint32_t)(we - ws);
+            const uint32_t n = wl > o ? (L < wl - o ? L : wl - o) : 0;
+            const int32_t lb = diagonal_segment_score(q, RefWin{&ref, r.ref_id, prs, n}, n, sc);
+            push_ssw(ws, we, lb >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u);
         } else {
             uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
             uint64_t de = (uint64_t)r.ref_end + L + padding;
@@ -208,7 +212,7 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
                 if (x == sub) shared = true;
             }
         }
-        if (shared) push_ssw(ws, we);
+        if (shared) push_ssw(ws, we, 0u);
     }
     res[t] = out;
 }

@@ -82,9 +82,9 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
     QueryView q{seq + ro, L, (r.flags & 4) != 0};
     const uint64_t rlen = ref.len[r.ref_id];
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    auto push_ssw = [&](uint64_t ws, uint64_t we) {
+    auto push_ssw = [&](uint64_t ws, uint64_t we, uint32_t jflags) {
         if (we - ws > 2000 || L == 0) return;  // Aligner::align returns None (src/aligner.rs:83-86)
-        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), 0};
+        ExtJob j{t, r.ref_id, ws, (uint32_t)(we - ws), jflags};
         if (L > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * L > 30000 || sc.match > 127 || sc.mismatch > 127)
             long_jobs[atomicAdd(&counters[1], 1u)] = j;
         else
@@ -134,12 +134,35 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
                 if (__ballot(done)) return;
             }
         }
+        const uint32_t padding = L / 10;
+        const uint64_t ws = prs > padding ? prs - padding : 0;
+        const uint64_t we = pre + padding < rlen ? pre + padding : rlen;
+        uint32_t jflags = 0;
+        if (use_ssw) {
+            // lower bound of the SSW score: best ungapped segment on the NAM's
+            // diagonal (query position 0 at window position prs - ws), one
+            // contiguous slice of positions per lane, joined across the wave
+            const uint32_t o = (uint32_t)(prs - ws), wl = (uint32_t)(we - ws);
+            const uint32_t n = wl > o ? min(L, wl - o) : 0;
+            const RefWin w{&ref, r.ref_id, prs, n};
+            const uint32_t per = (n + 63) / 64;
+            const uint32_t x0 = min(n, lane * per), x1 = min(n, x0 + per);
+            SegSum s;
+            for (uint32_t x = x0; x < x1; ++x) s.push(diagonal_cell(q, w, x, sc));
+            SA_UNROLL
+            for (int d = 1; d < 64; d <<= 1) {
+                SegSum b;
+                b.tot = __shfl_down(s.tot, d);
+                b.pre = __shfl_down(s.pre, d);
+                b.suf = __shfl_down(s.suf, d);
+                b.best = __shfl_down(s.best, d);
+                s.join(b);
+            }
+            jflags = s.best >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;  // lane 0 holds the whole diagonal
+        }
         if (lane == 0) {
-            const uint32_t padding = L / 10;
             if (use_ssw) {
-                const uint64_t ws = prs > padding ? prs - padding : 0;
-                const uint64_t we = pre + padding < rlen ? pre + padding : rlen;
-                push_ssw(ws, we);
+                push_ssw(ws, we, jflags);
             } else {
                 const uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
                 uint64_t de = (uint64_t)r.ref_end + L + padding;
@@ -168,7 +191,7 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
             }
         }
         if (lane == 0) {
-            if (shared) push_ssw(ws, we);
+            if (shared) push_ssw(ws, we, 0u);
             res[t] = out;
         }
     }
@@ -231,9 +254,10 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
     if (t < counters[0]) {
         const uint32_t rid = req[jobs[t].req].read_id;
         const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
-        const uint32_t b = ssw_bucket(false, false, (L + 15) / 16, ssw_wclass(jobs[t].wlen));
+        const bool word = (jobs[t].flags & SSW_WORD_FIRST) != 0;
+        const uint32_t b = ssw_bucket(word, false, word ? (L + 7) / 8 : (L + 15) / 16, ssw_wclass(jobs[t].wlen));
         lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
-        c = (uint64_t)L * jobs[t].wlen;
+        c = word ? 0 : (uint64_t)L * jobs[t].wlen;  // cells of the byte forward pass
     }
     // one atomic per wave
     SA_UNROLL

@@ -123,8 +123,12 @@ struct ExtJob {
     uint32_t ref_id;
     uint64_t ws;       // window start (absolute)
     uint32_t wlen;     // window length
-    uint32_t pad;
+    uint32_t flags;    // SSW_WORD_FIRST: the byte pass is known to saturate
 };
+// An SSW job whose local score is at least 255 - bias (the byte pass's
+// saturation point) starts with the 16-bit pass: the byte pass would only find
+// that it saturates.
+constexpr uint32_t SSW_WORD_FIRST = 1;
 
 struct ExtResult {
     uint32_t found;        // 0: None

@@ -9,6 +9,10 @@ Workloads (all randomness from numpy's PCG64 with the stated seeds):
             clipped at >= 150, FR orientation, 0.5% substitutions, 0.05% indels.
   se_var  : same reference; single-end reads of uniform length 50-500 bp,
             15% starting inside copied repeat segments.
+  human   : 3.1 Gbp in 24 contigs (seed 11), about 45% of the bases overwritten
+            by copies of 300 bp-50 kbp segments at a divergence drawn uniformly
+            from [0, 15%] per segment; paired-end 2x250 reads, fragment length
+            round(Normal(550, 50)), 3% substitutions, 0.3% indels.
 Reads are generated from both strands with equal probability (single-end) or
 as FR pairs (paired-end). Indel events insert or delete 1-3 bases (50/50,
 uniform inserted bases); reads are cut or extended to keep their length.
@@ -22,7 +26,9 @@ ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
 COMP = np.array([3, 2, 1, 0], dtype=np.uint8)
 
 
-def random_reference(rng, n_contigs, contig_len, repeat_fraction=0.0, seg_min=1000, seg_max=10000, divergence=0.02):
+def random_reference(rng, n_contigs, contig_len, repeat_fraction=0.0, seg_min=1000, seg_max=10000, divergence=0.02,
+                     max_divergence=None):
+    """max_divergence: draw each segment's divergence uniformly from [divergence, max_divergence]."""
     contigs = [rng.integers(0, 4, contig_len, dtype=np.uint8) for _ in range(n_contigs)]
     segments = []  # (contig, start, length) of copied destinations
     target = int(repeat_fraction * n_contigs * contig_len)
@@ -32,7 +38,8 @@ def random_reference(rng, n_contigs, contig_len, repeat_fraction=0.0, seg_min=10
         sc, dc = int(rng.integers(0, n_contigs)), int(rng.integers(0, n_contigs))
         ss, ds = int(rng.integers(0, contig_len - L)), int(rng.integers(0, contig_len - L))
         seg = contigs[sc][ss:ss + L].copy()
-        mut = rng.random(L) < divergence
+        d = divergence if max_divergence is None else float(rng.uniform(divergence, max_divergence))
+        mut = rng.random(L) < d
         seg[mut] = (seg[mut] + rng.integers(1, 4, int(mut.sum()), dtype=np.uint8)) % 4
         contigs[dc][ds:ds + L] = seg
         segments.append((dc, ds, L))
@@ -44,9 +51,15 @@ def write_fasta(path, names, contigs, width=80):
     with open(path, "wb") as f:
         for name, c in zip(names, contigs):
             f.write(b">" + name.encode() + b"\n")
-            s = ACGT[c].tobytes()
-            for i in range(0, len(s), width):
-                f.write(s[i:i + width] + b"\n")
+            s = ACGT[c]
+            full = len(s) // width
+            if full:  # whole lines at once: (full, width + 1) with a newline column
+                lines = np.empty((full, width + 1), dtype=np.uint8)
+                lines[:, :width] = s[:full * width].reshape(full, width)
+                lines[:, width] = 10
+                f.write(lines.tobytes())
+            if len(s) % width:
+                f.write(s[full * width:].tobytes() + b"\n")
 
 
 def mutate(rng, template, length, sub_rate, indel_rate):
@@ -202,13 +215,29 @@ def make(workload, out_dir, n_reads=None):
     raise ValueError(workload)
 
 
+def make_human(out_dir, n_pairs=1_000_000):
+    """The 3.1 Gbp workload (see the module docstring); files are cached in out_dir."""
+    os.makedirs(out_dir, exist_ok=True)
+    ref = os.path.join(out_dir, "human.fa")
+    r1, r2 = os.path.join(out_dir, "hpe_%d_1.fq" % n_pairs), os.path.join(out_dir, "hpe_%d_2.fq" % n_pairs)
+    if os.path.exists(ref) and os.path.exists(r1) and os.path.exists(r2):
+        return {"ref": ref, "reads1": r1, "reads2": r2}
+    rng = np.random.default_rng(11)
+    contigs, _ = random_reference(rng, 24, 3_100_000_000 // 24, repeat_fraction=0.45, seg_min=300, seg_max=50_000,
+                                  divergence=0.0, max_divergence=0.15)
+    if not os.path.exists(ref):
+        write_fasta(ref, ["chr%d" % (i + 1) for i in range(24)], contigs)
+    paired_end(np.random.default_rng(110), contigs, n_pairs, 250, 0.03, 0.003, r1, r2, mean=550, sd=50, prefix="h")
+    return {"ref": ref, "reads1": r1, "reads2": r2}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=["small", "pe100m", "se_var"])
+    ap.add_argument("workload", choices=["small", "pe100m", "se_var", "human"])
     ap.add_argument("out_dir")
     ap.add_argument("-n", type=int, default=None, help="number of reads / pairs")
     a = ap.parse_args()
-    print(make(a.workload, a.out_dir, a.n))
+    print(make_human(a.out_dir, a.n or 1_000_000) if a.workload == "human" else make(a.workload, a.out_dir, a.n))
 
 
 if __name__ == "__main__":
