@@ -43,22 +43,27 @@ def host_cores():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(ref, r1, r2, pairs, threads, workdir, out=os.devnull, t1_chunks=3):
-    """Times the oracle on the first `pairs` pairs (index build excluded); its
-    SAM goes to `out`. Returns (pairs/s, oracle timings, sample files)."""
-    from strobealign_amd import synth  # noqa: F401
+def cpu_baseline(ref, r1, r2, pairs, threads, workdir, out=os.devnull, t1_chunks=3, read_length=0):
+    """Times the oracle on the first `pairs` records (pairs, or reads when r2 is
+    None; index build excluded); its SAM goes to `out`. Returns (records/s,
+    oracle timings, sample files)."""
     oracle = os.path.join(ROOT, "oracle", "build", "sa_oracle")
     if not os.path.exists(oracle):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    s1, s2 = os.path.join(workdir, "cpu_1.fq"), os.path.join(workdir, "cpu_2.fq")
+    s1 = os.path.join(workdir, "cpu_1.fq")
+    s2 = os.path.join(workdir, "cpu_2.fq") if r2 else None
     for src, dst in ((r1, s1), (r2, s2)):
+        if not src:
+            continue
         with open(src, "rb") as f, open(dst, "wb") as g:
             for i, line in enumerate(f):
                 if i >= 4 * pairs:
                     break
                 g.write(line)
-    p = subprocess.run([oracle, "-t", str(threads), "--no-PG", "--time", "--t1-chunks", str(t1_chunks), "-o", out,
-                        ref, s1, s2], check=True, capture_output=True, text=True)
+    cmd = [oracle, "-t", str(threads), "--no-PG", "--time", "--t1-chunks", str(t1_chunks), "-o", out]
+    if read_length:
+        cmd += ["-r", str(read_length)]
+    p = subprocess.run(cmd + [ref, s1] + ([s2] if s2 else []), check=True, capture_output=True, text=True)
     t = json.loads(p.stderr.strip().splitlines()[-1])
     return pairs / t["map_s"], t, (s1, s2)
 
@@ -201,45 +206,75 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
     return roof, lookup, extension
 
 
-def bench_human(a, sa, synth, cores, threads):
-    """Human-scale workload: 3.1 Gbp reference (24 contigs, seed 11, 45% of it
-    copies of 300 bp-50 kbp segments at 0-15% divergence) with its full index
-    resident in HBM (bits 28, about 620M 16-byte entries); 2x250 pairs at 3%
-    substitutions and 0.3% indels, mapped in alignment mode (SAM) and in
-    mapping-only mode (-x, PAF). The oracle leg maps a sample of the same pairs
-    on the host cores and its SAM is compared with the library's on that sample."""
+def fastq_mean_length(path, n=500):
+    """Mean length of the first n records (the reference's read-length estimate,
+    src/main.rs:1001-1012)."""
+    tot = k = 0
+    with open(path, "rb") as f:
+        for i, line in enumerate(f):
+            if i % 4 == 1:
+                tot += len(line.rstrip(b"\r\n"))
+                k += 1
+                if k == n:
+                    break
+    return tot // max(1, k)
+
+
+def bench_workload(a, sa, synth, cores, threads):
+    """The other workloads, each on one resident index:
+
+    human  -- 3.1 Gbp reference (24 contigs, seed 11, 45% of it copies of
+              300 bp-50 kbp segments at 0-15% divergence), full index in HBM
+              (bits 28, about 620M 16-byte entries); 2x250 pairs at 3%
+              substitutions and 0.3% indels, mapped in alignment mode (SAM) and
+              in mapping-only mode (-x, PAF).
+    se_var -- the 100 Mbp reference; 2M single-end reads of 50-500 bp, 15% of
+              them started inside copied repeats; batched (autotuned engines)
+              against the plain one-engine path.
+    The oracle leg maps a sample of the same reads on the host cores and its SAM
+    is compared with the library's on that sample."""
+    human = a.workload == "human"
     t0 = time.time()
-    w = synth.make_human(a.data_dir, a.pairs)
+    if human:
+        w = synth.make_human(a.data_dir, a.pairs)
+    else:
+        w = synth.make("se_var", a.data_dir, a.pairs if a.pairs != 1_000_000 else 2_000_000)
     gen_s = time.time() - t0
     print("data ready in %.1fs: %s" % (gen_s, w), file=sys.stderr, flush=True)
+    read_len = 250 if human else fastq_mean_length(w["reads1"])
     ctx = sa.Context(0)
     t0 = time.time()
-    ix = sa.Index(ctx, w["ref"], read_length=250, threads=threads)
+    ix = sa.Index(ctx, w["ref"], read_length=read_len, threads=threads)
     index_s = time.time() - t0
     params, n_entries, n_contigs = ix.params()
-    print("index %.1fs: %d entries, bits %d, %d contigs" % (index_s, n_entries, params.bits, n_contigs),
-          file=sys.stderr, flush=True)
-    reads = sa.Reads(w["reads1"], w["reads2"])
+    print("index %.1fs: %d entries, bits %d, %d contigs, read length %d" %
+          (index_s, n_entries, params.bits, n_contigs, read_len), file=sys.stderr, flush=True)
+    reads = sa.Reads(w["reads1"], w.get("reads2"))
     t0 = time.perf_counter()
     reads.upload(ix)
     upload_s = time.perf_counter() - t0
-    n_pairs = len(reads)
+    n_units = len(reads)
+    unit = "read pairs/s" if human else "reads/s"
     opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
         t0 = time.perf_counter()
-        trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs or min(n_pairs, 250_000))
+        trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs or min(n_units, 500_000))
         print("autotune: %d engines, %d Mbases per batch (%d trials, %.1f s)" %
               (opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    runs = [("sam", 0, opts.slots), ("paf", 1, opts.slots)] if human else [("sam", 0, opts.slots)]
+    if not human and a.single_steps > 0:
+        runs.append(("plain", 0, 1))
     res = {}
-    for name, mode in (("sam", 0), ("paf", 1)):
-        o = sa.options(mode=mode, threads=threads, slots=opts.slots, batch_bases=opts.batch_bases)
-        for i in range(a.warmup):
+    for name, mode, slots in runs:
+        o = sa.options(mode=mode, threads=threads, slots=slots, batch_bases=a.batch_bases if slots == 1 else opts.batch_bases)
+        steps = a.steps if name != "plain" else a.single_steps
+        for i in range(a.warmup if name != "plain" else 1):
             _, st = ix.map_reads(reads, o, keep_output=False)
             print("%s warmup %d: %.1f ms" % (name, i, st.total_ms), file=sys.stderr, flush=True)
         k0 = ix.kernel_times()
         t0 = time.perf_counter()
-        for i in range(a.steps):
+        for i in range(steps):
             _, st = ix.map_reads(reads, o, keep_output=False)
             print("%s step %d: %.1f ms (rounds %d, extension requests %d)" %
                   (name, i, st.total_ms, st.rounds, st.requests), file=sys.stderr, flush=True)
@@ -247,21 +282,26 @@ def bench_human(a, sa, synth, cores, threads):
         k1 = ix.kernel_times()
         z = (0, 0, 0, 0, 0)
         kd = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
-        res[name] = (n_pairs * a.steps / wall, wall, st, kd)
-    value, wall, st, kd = res["sam"]
-    roof, lookup, extension = kernel_report(ctx, kd, a.steps, "the %d-engine SAM steps" % (opts.slots or 4),
-                                            opts.batch_bases or (64 << 20), "human")
+        res[name] = (n_units * steps / wall, wall, st, kd, steps)
+    value, wall, st, kd, _ = res["sam"]
+    if "plain" in res:  # per-kernel figures from the one-engine pass (kernels alone on the device)
+        _, _, _, kdp, sp = res["plain"]
+        roof, lookup, extension = kernel_report(ctx, kdp, sp, "the one-engine pass (%d steps)" % sp,
+                                                a.batch_bases or (64 << 20), a.workload)
+    else:
+        roof, lookup, extension = kernel_report(ctx, kd, a.steps, "the %d-engine SAM steps" % (opts.slots or 4),
+                                                opts.batch_bases or (64 << 20), a.workload)
     cpu = None
     parity = None
     if not a.no_cpu and a.cpu_pairs != 0:
-        sample = min(a.cpu_pairs or 20_000, n_pairs)
+        sample = min(a.cpu_pairs or 20_000, n_units)
         osam = os.path.join(a.data_dir, "oracle_sample.sam")
         try:
-            v, t, (s1, s2) = cpu_baseline(w["ref"], w["reads1"], w["reads2"], sample, cores, a.data_dir, out=osam,
-                                          t1_chunks=1)
-            cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
-                   "sample": "first %d pairs of the same workload, oracle -t %d, SAM; index build excluded "
-                             "(%.0f s, one thread)" % (sample, cores, t["index_s"]),
+            v, t, (s1, s2) = cpu_baseline(w["ref"], w["reads1"], w.get("reads2"), sample, cores, a.data_dir,
+                                          out=osam, t1_chunks=1, read_length=read_len)
+            cpu = {"value": round(v, 1), "unit": unit, "cores": cores, "kind": "port",
+                   "sample": "first %d %s of the same workload, oracle -t %d, SAM; index build excluded "
+                             "(%.0f s, one thread)" % (sample, "pairs" if human else "reads", cores, t["index_s"]),
                    "label": "CPU restatement of reference v0.18.0-alpha (oracle/), not the upstream binary",
                    "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                    "single_thread_value": round(t["t1_records"] / t["t1_s"], 1) if t.get("t1_s") else None,
@@ -269,15 +309,34 @@ def bench_human(a, sa, synth, cores, threads):
             mine = ix.map_files(s1, s2, threads=threads)
             with open(osam, "rb") as f:
                 theirs = f.read()
-            parity = {"pairs": sample, "identical_sam": mine == theirs, "bytes": len(theirs)}
-            print("oracle sample: %.0f pairs/s; SAM identical: %s" % (v, parity["identical_sam"]), file=sys.stderr,
+            parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs)}
+            print("oracle sample: %.0f %s; SAM identical: %s" % (v, unit, parity["identical_sam"]), file=sys.stderr,
                   flush=True)
         except Exception as e:  # reported, never fatal
-            cpu = {"value": None, "unit": "read pairs/s", "cores": cores, "kind": "port", "sample": "failed: %s" % e}
+            cpu = {"value": None, "unit": unit, "cores": cores, "kind": "port", "sample": "failed: %s" % e}
+    if human:
+        metric = "read pairs/sec at 2x250 on 3.1 Gbp ref (SAM; -x PAF beside it)"
+        desc = "3.1 Gbp reference (24 contigs, 45%% repeats), %d pairs 2x250, 3%% subs, 0.3%% indels" % n_units
+        data = "synthetic (strobealign_amd/synth.py human, seed 11)"
+    else:
+        metric = "reads/sec, single-end 50-500 bp on 100 Mbp ref (batched vs plain per-batch)"
+        desc = "100 Mbp reference (20x5 Mbp, 30%% repeats), %d SE reads of 50-500 bp, 15%% from repeats" % n_units
+        data = "synthetic (strobealign_amd/synth.py se_var, seed 71)"
+    config = {"workload": desc, "records": n_units, "read_len": read_len, "host_threads": threads,
+              "index_entries": n_entries, "index_bits": params.bits, "index_build_s": round(index_s, 1),
+              "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
+              "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
+              "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
+              "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed"}
+    if human:
+        config["paf_pairs_per_s"] = round(res["paf"][0], 1)
+    if "plain" in res:
+        config["plain_one_engine_per_s"] = round(res["plain"][0], 1)
+        config["batched_over_plain"] = round(value / res["plain"][0], 3)
     out = {
-        "metric": "read pairs/sec at 2x250 on 3.1 Gbp ref (SAM; -x PAF beside it)",
+        "metric": metric,
         "value": round(value, 1),
-        "unit": "read pairs/s",
+        "unit": unit,
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
@@ -286,23 +345,16 @@ def bench_human(a, sa, synth, cores, threads):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (strobealign_amd/synth.py human, seed 11)",
-        "config": {"workload": "3.1 Gbp reference (24 contigs, 45%% repeats), %d pairs 2x250, 3%% subs, "
-                               "0.3%% indels" % n_pairs,
-                   "pairs": n_pairs, "read_len": 250, "host_threads": threads,
-                   "index_entries": n_entries, "index_bits": params.bits, "index_build_s": round(index_s, 1),
-                   "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
-                   "paf_pairs_per_s": round(res["paf"][0], 1),
-                   "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
-                   "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-                   "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed"},
+        "data": data,
+        "config": config,
         "roofline": roof,
         "lookup": lookup,
         "extension": extension,
-        "paf_kernel_ms_per_step": {k: round(v[0] / a.steps, 2) for k, v in res["paf"][3].items() if v[0]},
         "cpu_baseline": cpu,
         "parity_sample": parity,
     }
+    if human:
+        out["paf_kernel_ms_per_step"] = {k: round(v[0] / a.steps, 2) for k, v in res["paf"][3].items() if v[0]}
     print(json.dumps(out), flush=True)
     reads.close()
     ix.close()
@@ -312,8 +364,9 @@ def bench_human(a, sa, synth, cores, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["pe100m", "human"], default="pe100m",
-                    help="pe100m: 100 Mbp, 2x150 (the headline line); human: 3.1 Gbp, 2x250, SAM and -x")
+    ap.add_argument("--workload", choices=["pe100m", "human", "se_var"], default="pe100m",
+                    help="pe100m: 100 Mbp, 2x150 (the headline line); human: 3.1 Gbp, 2x250, SAM and -x; "
+                         "se_var: 100 Mbp, 2M single-end 50-500 bp, batched vs plain")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--pairs", type=int, default=1_000_000)
@@ -340,8 +393,8 @@ def main():
 
     cores = host_cores()
     threads = a.threads or cores
-    if a.workload == "human":
-        return bench_human(a, sa, synth, cores, threads)
+    if a.workload != "pe100m":
+        return bench_workload(a, sa, synth, cores, threads)
     if a.cpu_pairs is None:
         a.cpu_pairs = 200_000
     t0 = time.time()

@@ -46,7 +46,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_COUNT
 };
 }  // namespace
 
@@ -539,7 +539,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
     // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets,
     // then (64-bit, at word 4 + SSW_BUCKETS) the DP cells of the forward SSW passes
-    constexpr uint32_t NCNT = 4 + SSW_BUCKETS + 2;
+    constexpr uint32_t NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;
     static_assert((4 + SSW_BUCKETS) % 2 == 0, "cell counter alignment");
     auto* d_cnt = (uint32_t*)grow(S_XCNT, NCNT * 4);
     uint32_t* d_bcnt = d_cnt + 4;
@@ -636,10 +636,16 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         // banded traceback: band up to 64 without a retry
         uint64_t per = 3ull * (2 * 64 + 4) * 4 + 129ull * L * 3 + 8ull * (2 * L + 64) + 4096;
         per = (per + 255) & ~255ull;
+        // jobs grouped by the band width their traceback starts with
+        auto* d_fin = (uint32_t*)grow(S_FINLISTS, (uint64_t)FIN_CLASSES * nssw * 4);
+        uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
+        k_fin_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_ends, nssw, d_fin, d_fcnt, nssw);
+        SA_CHECK(hipGetLastError());
         timed("k_ssw_finish", nssw, [&] {
             waves(nssw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
                 k_ssw_finish<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, cnt_,
-                                                                    d_res, d_slots, scr, per, first);
+                                                                    d_res, d_slots, scr, per, first, d_fin, d_fcnt,
+                                                                    nssw);
             });
         });
     }

@@ -279,8 +279,11 @@ SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
 // registers: entry u of row i holds column j = x_i + u - 1, x_i = max(i - bw, 0),
 // so the previous row's entries for column j and j - 1 sit at u + (x_i - x_{i-1})
 // and one below, and every index is static once the cell loop is unrolled over u.
-// Writes the same direction bytes and returns the same maximum as the scratch form
-// (band_pass_mem), including its resets of the row entries at 0 and `edge`.
+// The reference codes of the row's columns x_i .. x_i + 2*BWMAX are kept in
+// registers too and slide by one when x_i advances, so each row reads one new
+// reference base. Writes the same direction bytes and returns the same maximum as
+// the scratch form (band_pass_mem), including its resets of the row entries at 0
+// and `edge`.
 template <int BWMAX, typename QC, typename RC, typename SM>
 SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, const SM& score,
                             int32_t bw, int8_t* dir) {
@@ -291,11 +294,19 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
     int32_t hb[WM + 1], eb[WM + 1], hc[WM + 1];
     SA_UNROLL
     for (int x = 0; x <= WM; ++x) hb[x] = eb[x] = hc[x] = 0;
+    int32_t rw[UM];  // rw[u - 1]: reference code of column x_i + u - 1 (0 past the end)
+    SA_UNROLL
+    for (int u = 0; u < UM; ++u) rw[u] = u < refLen ? rc(u) : 0;
     int32_t maxv = 0, xprev = 0;
     for (int32_t i = 0; i < readLen; ++i) {
         const int32_t xi = i - bw > 0 ? i - bw : 0;
         const bool shift = xi != xprev;
         xprev = xi;
+        if (shift) {
+            SA_UNROLL
+            for (int u = 0; u < UM - 1; ++u) rw[u] = rw[u + 1];
+            rw[UM - 1] = xi + UM - 1 < refLen ? rc(xi + UM - 1) : 0;
+        }
         const int32_t end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
         const int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
         hb[0] = eb[0] = hc[0] = 0;
@@ -323,7 +334,7 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
                 const int8_t df = t1 > t2 ? 5 : 4;
                 const int32_t e1 = ev > 0 ? ev : 0, f1 = f > 0 ? f : 0;
                 t1 = e1 > f1 ? e1 : f1;
-                t2 = hbd + score(rc(xi + u - 1), qi);
+                t2 = hbd + score(rw[u - 1], qi);
                 const int32_t h = t1 > t2 ? t1 : t2;
                 hc[u] = h;
                 maxv = h > maxv ? h : maxv;
@@ -403,7 +414,9 @@ SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score
         int32_t* hc = eb + (width + 1);
         dir = (int8_t*)(hc + (width + 1));
         zero_bytes((uint8_t*)dir, (uint64_t)width_d * readLen * 3 + 3);
-        if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dir);
+        if (bw <= 1) maxv = band_pass_reg<1>(qc, rc, refLen, readLen, sc, sm, bw, dir);
+        else if (bw <= 2) maxv = band_pass_reg<2>(qc, rc, refLen, readLen, sc, sm, bw, dir);
+        else if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dir);
         else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, sm, bw, dir);
         else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, sm, bw, hb, eb, hc, dir);
         bw *= 2;
@@ -498,8 +511,9 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
                 mainc.push(2, len); ri += len; edits += len;
             }
         };
+        // reference bases are 2-bit codes, which are their SSW codes
         int st = banded_sw([&](int32_t i) { return ssw_code(q[qs + (uint32_t)i]); },
-                           [&](int32_t j) { return ssw_code(r[rs + (uint32_t)j]); }, refL, readL, score1, sc,
+                           [&](int32_t j) { return (int32_t)r.code(rs + (uint32_t)j); }, refL, readL, score1, sc,
                            MatchMismatch{sc.match, sc.mismatch}, bw, scratch, scratch_bytes, emit_main);
         if (st == 1) return a;  // flag 1: no alignment
         if (st == 2) { a.status = 2; return a; }

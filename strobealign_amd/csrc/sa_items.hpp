@@ -265,11 +265,35 @@ SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
 }
 
 // Traceback, CIGAR and end bonus of an SSW job whose score passes are done.
+// Traceback order: SSW jobs grouped by the band width their banded traceback
+// starts with (|aligned reference length - aligned query length| + 1: 1, 2,
+// 3-4, 5-8, more; last the jobs that stop before it), so that the lanes of a
+// wave run the same band-pass variant.
+constexpr uint32_t FIN_CLASSES = 6;
+SA_HD uint32_t fin_class(const SswEnds& e) {
+    if (e.ref_begin1 < 0 || e.read_begin1 < 0 || e.rev_short) return FIN_CLASSES - 1;
+    const int32_t refL = e.ref_end1 - e.ref_begin1 + 1, readL = e.read_end1 - e.read_begin1 + 1;
+    const int32_t bw = (refL > readL ? refL - readL : readL - refL) + 1;
+    return bw <= 1 ? 0u : bw <= 2 ? 1u : bw <= 4 ? 2u : bw <= 8 ? 3u : 4u;
+}
+// lists: FIN_CLASSES lists of `cap` entries; counts: their lengths (zeroed before)
+SA_HD void item_fin_bucket(uint32_t t, const SswEnds* ends, uint32_t njobs, uint32_t* lists, uint32_t* counts,
+                           uint32_t cap) {
+    if (t >= njobs) return;
+    const uint32_t c = fin_class(ends[t]);
+    lists[(uint64_t)c * cap + SA_ATOMIC_ADD(&counts[c], 1u)] = t;
+}
+
+// Thread t runs the job at position first + t of the class lists taken in order.
 SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
                            const ExtRequest* req, const ExtJob* jobs, const SswEnds* ends, uint32_t njobs,
-                           ExtResult* res, uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
+                           ExtResult* res, uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first,
+                           const uint32_t* lists, const uint32_t* counts, uint32_t cap) {
     if (t >= njobs) return;
-    const ExtJob j = jobs[first + t];
+    uint32_t pos = first + t, c = 0;
+    while (c + 1 < FIN_CLASSES && pos >= counts[c]) pos -= counts[c++];
+    const uint32_t job = lists[(uint64_t)c * cap + pos];
+    const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
     const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
@@ -277,7 +301,7 @@ SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, 
     CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     RefWin w{&ref, r.ref_id, j.ws, j.wlen};
-    AlignInfo a = ssw_finish(q, w, ends[first + t], sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
+    AlignInfo a = ssw_finish(q, w, ends[job], sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
     out.status = (uint32_t)a.status;
     if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
     res[j.req] = out;

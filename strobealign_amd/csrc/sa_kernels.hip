@@ -221,9 +221,17 @@ __global__ void __launch_bounds__(128) k_ssw_finish(const uint8_t* __restrict__ 
                                                    const ExtJob* __restrict__ jobs, const SswEnds* __restrict__ ends,
                                                    uint32_t njobs, ExtResult* __restrict__ res,
                                                    uint32_t* __restrict__ cigars, uint8_t* __restrict__ scratch,
-                                                   uint64_t scratch_per, uint32_t first) {
+                                                   uint64_t scratch_per, uint32_t first,
+                                                   const uint32_t* __restrict__ lists,
+                                                   const uint32_t* __restrict__ counts, uint32_t cap) {
     item_ssw_finish(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, req, jobs, ends, njobs, res, cigars,
-                    scratch, scratch_per, first);
+                    scratch, scratch_per, first, lists, counts, cap);
+}
+
+__global__ void __launch_bounds__(256) k_fin_bucket(const SswEnds* __restrict__ ends, uint32_t njobs,
+                                                   uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
+                                                   uint32_t cap) {
+    item_fin_bucket(blockIdx.x * blockDim.x + threadIdx.x, ends, njobs, lists, counts, cap);
 }
 
 // SSW score-pass buckets: jobs are listed by (pass, stripe count) so that each
