@@ -46,7 +46,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_PWROWS, S_PWDIRS, S_COUNT
 };
 }  // namespace
 
@@ -662,13 +662,26 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     }
     // 4. piecewise extension, one thread per job
     if (npw) {
-        uint64_t per = 16 * L + 256 + (L + 1) * (L + L / 10 + 2) + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
+        // per thread: chain anchors, traceback ops (the DP itself uses the blocks below)
+        uint64_t per = 16 * L + 256 + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
         per = (per + 255) & ~255ull;
+        // DP rows and direction bytes of the end extensions and gaps, one
+        // lane-interleaved block of each per 64 threads (a DP with more rows or
+        // columns falls back to one thread with a large scratch)
+        const uint32_t rcap = (uint32_t)(L + L / 10 + 3), drows = (uint32_t)(L + 2);
+        const uint64_t pw_per = per + (uint64_t)rcap * sizeof(DpHF) + (uint64_t)rcap * drows;
+        const uint64_t slice = std::max<uint64_t>(1, std::min<uint64_t>(npw, budget / pw_per));
+        const uint64_t slice_threads = (slice + 127) / 128 * 128;
+        auto* scr = (uint8_t*)grow(S_SCRATCH, slice * per);
+        auto* rows = (DpHF*)grow(S_PWROWS, slice_threads * rcap * sizeof(DpHF));
+        auto* dirs = (uint8_t*)grow(S_PWDIRS, slice_threads * rcap * drows);
         timed("k_ext_pw", npw, [&] {
-            waves(npw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
+            for (uint64_t first = 0; first < npw; first += slice) {
+                const uint32_t cnt_ = (uint32_t)std::min<uint64_t>(slice, npw - first);
                 k_ext_pw<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res,
-                                                                d_slots, d_chain, scr, per, first);
-            });
+                                                                d_slots, d_chain, scr, per, (uint32_t)first, rows,
+                                                                dirs, rcap, drows);
+            }
         });
     }
     // 5. CIGAR offsets (found results only) and the results, in one wait

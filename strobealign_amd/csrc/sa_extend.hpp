@@ -800,57 +800,84 @@ struct DpOut {
     int status;
 };
 
+// H and F of one DP column, kept for the row being computed (updated in place).
+struct alignas(8) DpHF {
+    int32_t h, f;
+};
+// Where the DP keeps its row and its direction bytes: row element j at
+// p[j * stride] for j < cap, direction byte of cell (i, j) at
+// dir[(i * cap + j) * stride] for i < dir_rows. The piecewise kernel gives each
+// wave one block of each with its 64 lanes interleaved (stride 64), so lanes at
+// the same cell touch consecutive addresses; without blocks both are placed in
+// the scratch (the direction bytes first, row pitch n + 1).
+struct DpRows {
+    DpHF* p = nullptr;
+    uint32_t stride = 1, cap = 0;
+    uint8_t* dir = nullptr;
+    uint32_t dir_rows = 0;
+};
+
 template <typename Q, typename R>
 SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scores& sc, bool local_end,
-                      uint8_t* scratch, uint64_t scratch_bytes, uint8_t* tb, uint32_t tb_cap) {
+                      uint8_t* scratch, uint64_t scratch_bytes, uint8_t* tb, uint32_t tb_cap,
+                      DpRows rows = DpRows{}) {
     DpOut o{0, 0, 0, 0, 0};
     const int32_t go = sc.gap_open, ge = sc.gap_extend;
     const int32_t NEG = -(1 << 28);
     uint64_t W = (uint64_t)n + 1;
-    uint64_t need = (uint64_t)(m + 1) * W + 3 * W * 4 + 16;
-    if (need > scratch_bytes) { o.status = 2; return o; }
-    uint8_t* dirs = scratch;  // bit0 H==E, bit1 H==F, bit2 E opened, bit3 F opened
-    int32_t* Hp = (int32_t*)(scratch + (((uint64_t)(m + 1) * W + 15) & ~15ull));
-    int32_t* Hc = Hp + W;
-    int32_t* F = Hc + W;
+    const uint64_t dir_bytes = ((uint64_t)(m + 1) * W + 15) & ~15ull;
+    if (!rows.p) {
+        if (dir_bytes + W * sizeof(DpHF) + 16 > scratch_bytes) { o.status = 2; return o; }
+        rows = DpRows{(DpHF*)(scratch + dir_bytes), 1, (uint32_t)W, scratch, m + 1};
+    } else if (W > rows.cap || m + 1 > rows.dir_rows) {
+        o.status = 2;
+        return o;
+    }
+    // direction bytes: bit0 H==E, bit1 H==F, bit2 E opened, bit3 F opened
+    const uint64_t st = rows.stride, pitch = rows.cap;
+    uint8_t* const dirs = rows.dir;
+    auto dir_at = [&](uint64_t i, uint64_t j) -> uint8_t& { return dirs[(i * pitch + j) * st]; };
+    DpHF* row = rows.p;
     int32_t best = 0;
     uint32_t bi = 0, bj = 0;
     // row 0
-    Hp[0] = 0;
-    F[0] = NEG;
-    dirs[0] = 0;
+    row[0] = DpHF{0, NEG};
+    dir_at(0, 0) = 0;
     for (uint32_t j = 1; j <= n; ++j) {
-        Hp[j] = -(go + (int32_t)(j - 1) * ge);
-        F[j] = NEG;
-        dirs[j] = (uint8_t)(1 | (j == 1 ? 4 : 0));
-        if (local_end && Hp[j] > best) { best = Hp[j]; bi = 0; bj = j; }
+        const int32_t h0 = -(go + (int32_t)(j - 1) * ge);
+        row[j * st] = DpHF{h0, NEG};
+        dir_at(0, j) = (uint8_t)(1 | (j == 1 ? 4 : 0));
+        if (local_end && h0 > best) { best = h0; bi = 0; bj = j; }
     }
+    // row i overwrites row i - 1 in place: `diag` keeps the previous row's H of
+    // column j - 1, `left` the current row's
     for (uint32_t i = 1; i <= m; ++i) {
-        uint8_t* dr = dirs + (uint64_t)i * W;
+        uint8_t* dr = &dir_at(i, 0);
         int32_t Fi0 = -(go + (int32_t)(i - 1) * ge);
-        Hc[0] = Fi0;
-        F[0] = Fi0;
+        int32_t diag = row[0].h;
+        row[0] = DpHF{Fi0, Fi0};
         dr[0] = (uint8_t)(2 | (i == 1 ? 8 : 0));
-        if (local_end && Hc[0] > best) { best = Hc[0]; bi = i; bj = 0; }
-        int32_t E = NEG;
+        if (local_end && Fi0 > best) { best = Fi0; bi = i; bj = 0; }
+        int32_t E = NEG, left = Fi0;
         const uint8_t qc = q[i - 1];
         const int32_t bonus = (local_end && i == m) ? sc.end_bonus : 0;
         for (uint32_t j = 1; j <= n; ++j) {
-            int32_t eo = Hc[j - 1] - go, ee = E - ge;
+            const DpHF prev = row[j * st];
+            int32_t eo = left - go, ee = E - ge;
             E = eo > ee ? eo : ee;
-            int32_t fo = Hp[j] - go, fe = F[j] - ge;
+            int32_t fo = prev.h - go, fe = prev.f - ge;
             int32_t f = fo > fe ? fo : fe;
-            F[j] = f;
-            int32_t h = Hp[j - 1] + (qc == r[j - 1] ? sc.match : -sc.mismatch) + bonus;
+            int32_t h = diag + (qc == r[j - 1] ? sc.match : -sc.mismatch) + bonus;
             int32_t hh = h > E ? h : E;
             hh = hh > f ? hh : f;
-            Hc[j] = hh;
-            dr[j] = (uint8_t)((hh == E ? 1 : 0) | (hh == f ? 2 : 0) | (E == eo ? 4 : 0) | (f == fo ? 8 : 0));
+            row[j * st] = DpHF{hh, f};
+            dr[j * st] = (uint8_t)((hh == E ? 1 : 0) | (hh == f ? 2 : 0) | (E == eo ? 4 : 0) | (f == fo ? 8 : 0));
             if (local_end && hh > best) { best = hh; bi = i; bj = j; }
+            diag = prev.h;
+            left = hh;
         }
-        int32_t* t = Hp; Hp = Hc; Hc = t;
     }
-    if (!local_end) { best = Hp[n]; bi = m; bj = n; }
+    if (!local_end) { best = row[(uint64_t)n * st].h; bi = m; bj = n; }
     o.score = best;
     o.bi = bi;
     o.bj = bj;
@@ -858,7 +885,7 @@ SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scor
     uint32_t i = bi, j = bj, nt = 0;
     int state = 0;
     while (i > 0 || j > 0) {
-        uint8_t d = dirs[(uint64_t)i * W + j];
+        uint8_t d = dir_at(i, j);
         if (state == 0) {
             if (j > 0 && (d & 1)) state = 1;
             else if (i > 0 && (d & 2)) state = 2;
@@ -961,7 +988,7 @@ SA_HD void emit_tb(const uint8_t* tb, uint32_t ntb, bool reversed_space, CigarOu
 // reference window (its start is the decode start).
 SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t* anch, uint32_t na, uint32_t padding,
                                  const Scores& sc, uint32_t k, uint8_t* scratch, uint64_t scratch_bytes,
-                                 CigarOut& cig) {
+                                 CigarOut& cig, DpRows rows = DpRows{}) {
     AlignInfo res{false, 0, 0, 0, 0, 0, 0, 0};
     cig.reset();
     if (na == 0) return res;
@@ -988,7 +1015,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             SubView<RefWin> rv{&win, rs};
             RevView<SubView<QueryView>> qr{&qv, qlen};
             RevView<SubView<RefWin>> rr{&rv, rlen};
-            DpOut d = affine_dp(qr, qlen, rr, rlen, sc, true, dps, dps_bytes, tb, tb_cap);
+            DpOut d = affine_dp(qr, qlen, rr, rlen, sc, true, dps, dps_bytes, tb, tb_cap, rows);
             if (d.status) { res.status = d.status; return res; }
             if (d.score == 0) { query_start = fq; ref_start = fr; }
             else {
@@ -1029,7 +1056,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             if (!done) {
                 SubView<QueryView> qv{&q, pqe};
                 SubView<RefWin> rv{&win, pre};
-                DpOut d = affine_dp(qv, (uint32_t)qd, rv, (uint32_t)rd, sc, false, dps, dps_bytes, tb, tb_cap);
+                DpOut d = affine_dp(qv, (uint32_t)qd, rv, (uint32_t)rd, sc, false, dps, dps_bytes, tb, tb_cap, rows);
                 if (d.status) { res.status = d.status; return res; }
                 score += d.score;
                 emit_tb(tb, d.ntb, false, cig);
@@ -1064,7 +1091,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             uint32_t re = rl < lre + qlen + padding ? rl : lre + qlen + padding;
             SubView<QueryView> qv{&q, lqe};
             SubView<RefWin> rv{&win, lre};
-            DpOut d = affine_dp(qv, qlen, rv, re - lre, sc, true, dps, dps_bytes, tb, tb_cap);
+            DpOut d = affine_dp(qv, qlen, rv, re - lre, sc, true, dps, dps_bytes, tb, tb_cap, rows);
             if (d.status) { res.status = d.status; return res; }
             if (d.score == 0) { query_end = lqe; ref_end = lre; }
             else {

@@ -218,9 +218,14 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
 }
 
 // Piecewise extension job (thread per job).
+// rows, dirs: per 64 consecutive threads one block of row_cap DP columns and
+// one of dir_rows x row_cap direction bytes, lanes interleaved (thread t's
+// column j at block t / 64, index j * 64 + t % 64; cell (i, j) at
+// (i * row_cap + j) * 64 + t % 64).
 SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc, uint32_t k,
                        const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res, uint32_t* cigars,
-                       const uint32_t* chain_anchors, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
+                       const uint32_t* chain_anchors, uint8_t* scratch, uint64_t scratch_per, uint32_t first,
+                       DpHF* rows, uint8_t* dirs, uint32_t row_cap, uint32_t dir_rows) {
     if (t >= njobs) return;
     const ExtJob j = jobs[first + t];
     const ExtRequest r = req[j.req];
@@ -238,7 +243,10 @@ SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefV
     } else {
         for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
         uint32_t na = remove_spurious_anchors(an, r.n_anchors);
-        AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig);
+        const uint64_t wave = t / 64, lane = t % 64;
+        const DpRows dr{rows + wave * 64 * row_cap + lane, 64, row_cap, dirs + wave * 64 * row_cap * dir_rows + lane,
+                        dir_rows};
+        AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig, dr);
         out.status = (uint32_t)a.status;
         if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
     }
