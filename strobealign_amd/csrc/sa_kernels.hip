@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(128) k_extend(const uint8_t* __restrict__ seq,
 // Mismatch bits as a sequence view for hamming_align: q[i] == r[i] exactly when
 // bit i of the mask is clear.
 struct MismatchBits {
-    uint64_t w[4];
+    uint64_t w[8];
     __device__ uint8_t operator[](uint32_t i) const { return (uint8_t)((w[i >> 6] >> (i & 63)) & 1u); }
 };
 struct ZeroBits {
@@ -97,11 +97,11 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
         if (pre > rlen) pre = rlen;
         if (prs + L == pre && consistent) {
             RefWin w{&ref, r.ref_id, prs, L};
-            if (L <= 256) {
+            if (L <= 512) {
                 MismatchBits m;
                 uint32_t hd = 0;
 #pragma unroll
-                for (uint32_t c = 0; c < 4; ++c) {
+                for (uint32_t c = 0; c < 8; ++c) {
                     const uint32_t pos = c * 64 + lane;
                     m.w[c] = __ballot(pos < L && q[pos] != w[pos]);
                     hd += (uint32_t)__popcll(m.w[c]);
