@@ -669,11 +669,11 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         // lane-interleaved block of each per 64 threads (a DP with more rows or
         // columns falls back to one thread with a large scratch)
         const uint32_t rcap = (uint32_t)(L + L / 10 + 3), drows = (uint32_t)(L + 2);
-        const uint64_t pw_per = per + (uint64_t)rcap * sizeof(DpHF) + (uint64_t)rcap * drows;
+        const uint64_t pw_per = per + (uint64_t)rcap * sizeof(DpCol) + (uint64_t)rcap * drows;
         const uint64_t slice = std::max<uint64_t>(1, std::min<uint64_t>(npw, budget / pw_per));
         const uint64_t slice_threads = (slice + 127) / 128 * 128;
         auto* scr = (uint8_t*)grow(S_SCRATCH, slice * per);
-        auto* rows = (DpHF*)grow(S_PWROWS, slice_threads * rcap * sizeof(DpHF));
+        auto* rows = (DpCol*)grow(S_PWROWS, slice_threads * rcap * sizeof(DpCol));
         auto* dirs = (uint8_t*)grow(S_PWDIRS, slice_threads * rcap * drows);
         timed("k_ext_pw", npw, [&] {
             for (uint64_t first = 0; first < npw; first += slice) {

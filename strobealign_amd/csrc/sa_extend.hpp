@@ -800,9 +800,10 @@ struct DpOut {
     int status;
 };
 
-// H and F of one DP column, kept for the row being computed (updated in place).
-struct alignas(8) DpHF {
-    int32_t h, f;
+// One DP column as the row being computed sees it: H and F (updated in place)
+// and the column's reference base, read once per DP.
+struct alignas(16) DpCol {
+    int32_t h, f, r, pad;
 };
 // Where the DP keeps its row and its direction bytes: row element j at
 // p[j * stride] for j < cap, direction byte of cell (i, j) at
@@ -811,7 +812,7 @@ struct alignas(8) DpHF {
 // the same cell touch consecutive addresses; without blocks both are placed in
 // the scratch (the direction bytes first, row pitch n + 1).
 struct DpRows {
-    DpHF* p = nullptr;
+    DpCol* p = nullptr;
     uint32_t stride = 1, cap = 0;
     uint8_t* dir = nullptr;
     uint32_t dir_rows = 0;
@@ -827,8 +828,8 @@ SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scor
     uint64_t W = (uint64_t)n + 1;
     const uint64_t dir_bytes = ((uint64_t)(m + 1) * W + 15) & ~15ull;
     if (!rows.p) {
-        if (dir_bytes + W * sizeof(DpHF) + 16 > scratch_bytes) { o.status = 2; return o; }
-        rows = DpRows{(DpHF*)(scratch + dir_bytes), 1, (uint32_t)W, scratch, m + 1};
+        if (dir_bytes + W * sizeof(DpCol) + 16 > scratch_bytes) { o.status = 2; return o; }
+        rows = DpRows{(DpCol*)(scratch + dir_bytes), 1, (uint32_t)W, scratch, m + 1};
     } else if (W > rows.cap || m + 1 > rows.dir_rows) {
         o.status = 2;
         return o;
@@ -837,15 +838,15 @@ SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scor
     const uint64_t st = rows.stride, pitch = rows.cap;
     uint8_t* const dirs = rows.dir;
     auto dir_at = [&](uint64_t i, uint64_t j) -> uint8_t& { return dirs[(i * pitch + j) * st]; };
-    DpHF* row = rows.p;
+    DpCol* row = rows.p;
     int32_t best = 0;
     uint32_t bi = 0, bj = 0;
     // row 0
-    row[0] = DpHF{0, NEG};
+    row[0] = DpCol{0, NEG, 0, 0};
     dir_at(0, 0) = 0;
     for (uint32_t j = 1; j <= n; ++j) {
         const int32_t h0 = -(go + (int32_t)(j - 1) * ge);
-        row[j * st] = DpHF{h0, NEG};
+        row[j * st] = DpCol{h0, NEG, (int32_t)r[j - 1], 0};
         dir_at(0, j) = (uint8_t)(1 | (j == 1 ? 4 : 0));
         if (local_end && h0 > best) { best = h0; bi = 0; bj = j; }
     }
@@ -855,22 +856,24 @@ SA_HD DpOut affine_dp(const Q& q, uint32_t m, const R& r, uint32_t n, const Scor
         uint8_t* dr = &dir_at(i, 0);
         int32_t Fi0 = -(go + (int32_t)(i - 1) * ge);
         int32_t diag = row[0].h;
-        row[0] = DpHF{Fi0, Fi0};
+        row[0].h = Fi0;
+        row[0].f = Fi0;
         dr[0] = (uint8_t)(2 | (i == 1 ? 8 : 0));
         if (local_end && Fi0 > best) { best = Fi0; bi = i; bj = 0; }
         int32_t E = NEG, left = Fi0;
         const uint8_t qc = q[i - 1];
         const int32_t bonus = (local_end && i == m) ? sc.end_bonus : 0;
         for (uint32_t j = 1; j <= n; ++j) {
-            const DpHF prev = row[j * st];
+            const DpCol prev = row[j * st];
             int32_t eo = left - go, ee = E - ge;
             E = eo > ee ? eo : ee;
             int32_t fo = prev.h - go, fe = prev.f - ge;
             int32_t f = fo > fe ? fo : fe;
-            int32_t h = diag + (qc == r[j - 1] ? sc.match : -sc.mismatch) + bonus;
+            int32_t h = diag + ((int32_t)qc == prev.r ? sc.match : -sc.mismatch) + bonus;
             int32_t hh = h > E ? h : E;
             hh = hh > f ? hh : f;
-            row[j * st] = DpHF{hh, f};
+            row[j * st].h = hh;
+            row[j * st].f = f;
             dr[j * st] = (uint8_t)((hh == E ? 1 : 0) | (hh == f ? 2 : 0) | (E == eo ? 4 : 0) | (f == fo ? 8 : 0));
             if (local_end && hh > best) { best = hh; bi = i; bj = j; }
             diag = prev.h;

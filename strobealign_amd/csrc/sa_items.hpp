@@ -225,7 +225,7 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
 SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc, uint32_t k,
                        const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res, uint32_t* cigars,
                        const uint32_t* chain_anchors, uint8_t* scratch, uint64_t scratch_per, uint32_t first,
-                       DpHF* rows, uint8_t* dirs, uint32_t row_cap, uint32_t dir_rows) {
+                       DpCol* rows, uint8_t* dirs, uint32_t row_cap, uint32_t dir_rows) {
     if (t >= njobs) return;
     const ExtJob j = jobs[first + t];
     const ExtRequest r = req[j.req];

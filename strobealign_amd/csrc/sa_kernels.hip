@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(128) k_ext_pw(const uint8_t* __restrict__ seq,
                                                ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
                                                const uint32_t* __restrict__ chain_anchors,
                                                uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first,
-                                               DpHF* __restrict__ rows, uint8_t* __restrict__ dirs,
+                                               DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
                                                uint32_t row_cap, uint32_t dir_rows) {
     item_ext_pw(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, jobs, njobs, res, cigars,
                 chain_anchors, scratch, scratch_per, first, rows, dirs, row_cap, dir_rows);

@@ -202,7 +202,11 @@ def test_autotuned_schedule_gives_the_same_sam(sa, ctx, tmp_path_factory):
         trials = ix.autotune(reads, opts)
         assert len(trials) == 15 and all(t["ms"] > 0 for t in trials)
         assert opts.slots in (1, 2, 3, 4, 6) and opts.batch_bases in (16 << 20, 32 << 20, 64 << 20)
-        assert min(trials, key=lambda t: t["ms"])["slots"] == opts.slots
+        # the chosen schedule is a fastest trial (times are reported to 0.01 ms,
+        # so several trials may show the minimum)
+        fastest = min(t["ms"] for t in trials)
+        assert any(t["ms"] == fastest and t["slots"] == opts.slots and t["batch_bases"] == opts.batch_bases
+                   for t in trials)
         out, _ = ix.map_reads(reads, opts)
         reads.close()
     assert out == exp
