@@ -244,7 +244,7 @@ def bench_workload(a, sa, synth, cores, threads):
     read_len = 250 if human else fastq_mean_length(w["reads1"])
     ctx = sa.Context(0)
     t0 = time.time()
-    ix = sa.Index(ctx, w["ref"], read_length=read_len, threads=threads)
+    ix = sa.Index(ctx, w["ref"], read_length=read_len, threads=threads, device_build=not a.host_index)
     index_s = time.time() - t0
     params, n_entries, n_contigs = ix.params()
     print("index %.1fs: %d entries, bits %d, %d contigs, read length %d" %
@@ -324,6 +324,7 @@ def bench_workload(a, sa, synth, cores, threads):
         data = "synthetic (strobealign_amd/synth.py se_var, seed 71)"
     config = {"workload": desc, "records": n_units, "read_len": read_len, "host_threads": threads,
               "index_entries": n_entries, "index_bits": params.bits, "index_build_s": round(index_s, 1),
+              "index_build": "host, %d threads" % threads if a.host_index else "device (FASTA read on the host)",
               "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
               "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
               "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
@@ -375,6 +376,8 @@ def main():
     ap.add_argument("--cpu-pairs", type=int, default=None,
                     help="oracle sample size (0: skip; default 200,000, human 20,000)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-index", action="store_true",
+                    help="human/se_var: build the index on the host instead of the device")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
     ap.add_argument("--single-steps", type=int, default=2, help="timed steps of the one-engine comparison (0: skip)")

@@ -76,6 +76,14 @@ int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* en
                     const uint64_t* contig_word_offsets, const uint64_t* contig_lengths,
                     const char* const* contig_names, uint32_t n_contigs, sa_index** out);
 void sa_index_destroy(sa_index* index);
+/* sa_index_build with make_index (src/indexer.rs:18-170) run on the device:
+   syncmers per 4096-position chunk, randstrobes per syncmer, two stable radix
+   sorts, bucket directory and filter cutoff from device passes. Same entries,
+   bucket starts and parameters as the host build; up to 2^31 - 1 entries. */
+int sa_index_build_device(sa_ctx* ctx, const char* fasta_path, uint64_t read_length, int bits, sa_index** out);
+/* Copies an index back to the host: n_entries 16-byte entries (RefRandstrobe
+   layout, src/index.rs:24-30) and 2^bits + 1 bucket starts (src/index.rs:93-97). */
+int sa_index_download(const sa_index* index, void* entries, uint64_t* bucket_starts);
 int sa_index_info(const sa_index* index, sa_seeding_params* params, uint64_t* n_entries, uint32_t* n_contigs);
 
 /* Mapping options: the flags of src/main.rs:61-283 that the mapping path uses. */

@@ -58,7 +58,7 @@ class Alignment(ctypes.Structure):
 
 
 EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
-           "sa_index_build", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
+           "sa_index_build", "sa_index_build_device", "sa_index_download", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
            "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
            "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
@@ -192,10 +192,15 @@ class Context:
 class Index:
     """Device-resident reference + randstrobe index (make_index, src/indexer.rs:18-170)."""
 
-    def __init__(self, ctx, fasta, read_length, bits=0, threads=4):
+    def __init__(self, ctx, fasta, read_length, bits=0, threads=4, device_build=False):
+        """device_build: run make_index on the device (sa_index_build_device)."""
         self._p = ctypes.c_void_p()
-        _check(_lib.sa_index_build(ctx._p, fasta.encode(), ctypes.c_uint64(read_length), bits, threads,
-                                   ctypes.byref(self._p)))
+        if device_build:
+            _check(_lib.sa_index_build_device(ctx._p, fasta.encode(), ctypes.c_uint64(read_length), bits,
+                                              ctypes.byref(self._p)))
+        else:
+            _check(_lib.sa_index_build(ctx._p, fasta.encode(), ctypes.c_uint64(read_length), bits, threads,
+                                       ctypes.byref(self._p)))
 
     def close(self):
         if self._p:
@@ -208,6 +213,14 @@ class Index:
         c = ctypes.c_uint32()
         _check(_lib.sa_index_info(self._p, ctypes.byref(p), ctypes.byref(n), ctypes.byref(c)))
         return p, n.value, c.value
+
+    def download(self):
+        """(entries as bytes, 16 per entry; bucket starts as a list) copied from the device."""
+        p, n, _ = self.params()
+        ent = ctypes.create_string_buffer(max(1, 16 * n))
+        bs = (ctypes.c_uint64 * ((1 << p.bits) + 1))()
+        _check(_lib.sa_index_download(self._p, ent, bs))
+        return ent.raw[:16 * n], list(bs)
 
     def map_files(self, reads1, reads2=None, mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
                   use_ssw=False, max_secondary=0, mcs=0, details=False, fastq_comments=False, only_mapped=False,

@@ -219,6 +219,36 @@ int sa_index_build(sa_ctx* ctx, const char* fasta_path, uint64_t read_length, in
     });
 }
 
+int sa_index_build_device(sa_ctx* ctx, const char* fasta_path, uint64_t read_length, int bits, sa_index** out) {
+    if (!ctx || !fasta_path || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        auto ix = std::make_unique<sa_index>();
+        ix->device = ctx->device;
+        try {
+            ix->ref = sa::read_fasta(fasta_path);
+        } catch (const std::exception& e) {
+            return fail(SA_ERR_IO, e.what());
+        }
+        sa::SeedParams sp = sa::seed_params_for_read_length(read_length);
+        if (bits <= 0) bits = sa::pick_bits(ix->ref, sp);
+        if (bits > 32) return fail(SA_ERR_ARG, "bits must be in 1..32");
+        ix->eng = std::make_unique<sa::Engine>(ctx->device);
+        ix->eng->upload_reference(ix->ref.words.data(), ix->ref.words.size(), ix->ref.word_off.data(),
+                                  ix->ref.len.data(), (uint32_t)ix->ref.names.size());
+        ix->p = ix->eng->build_index(ix->ref.len.data(), (uint32_t)ix->ref.names.size(), sp, bits, 0.0002);
+        *out = ix.release();
+        return SA_OK;
+    });
+}
+
+int sa_index_download(const sa_index* index, void* entries, uint64_t* bucket_starts) {
+    if (!index || (!entries && index->p.n_entries) || !bucket_starts) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        index->eng->download_index((sa::RefEntry*)entries, bucket_starts);
+        return SA_OK;
+    });
+}
+
 int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* entries, uint64_t n,
                     const uint64_t* bucket_starts, const uint64_t* packed_ref_words, uint64_t n_words,
                     const uint64_t* contig_word_offsets, const uint64_t* contig_lengths,

@@ -301,12 +301,159 @@ void Engine::upload_index(const RefEntry* entries, uint64_t n, const uint64_t* b
     SA_CHECK(hipMalloc(&d_buckets_, nb * 8));
     if (n) SA_CHECK(hipMemcpy(d_entries_, entries, n * sizeof(RefEntry), hipMemcpyHostToDevice));
     SA_CHECK(hipMemcpy(d_buckets_, bucket_starts, nb * 8, hipMemcpyHostToDevice));
+    setup_chain_tables();
+}
+
+void Engine::setup_chain_tables() {
+    if (d_log2_) return;
     // log2(d + 1) for the chaining gap penalty, from the host libm (src/chainer.rs:294-298)
     std::vector<float> lg(65536);
     for (size_t d = 0; d < lg.size(); ++d) lg[d] = std::log2((float)d + 1.0f);
     SA_CHECK(hipMalloc(&d_log2_, lg.size() * sizeof(float)));
     SA_CHECK(hipMemcpy(d_log2_, lg.data(), lg.size() * sizeof(float), hipMemcpyHostToDevice));
     SA_CHECK(hipMalloc(&d_same_diag_, 1024 * sizeof(float)));
+}
+
+namespace {
+// filter cutoff of src/indexer.rs:109-129 from the histogram of repeated-hash run
+// lengths (slot 0: distinct hashes, slot c: runs of c entries, last slot: longer):
+// the index_cutoff-th largest run length clamped to [30, 100] only needs the
+// lengths up to 101.
+uint64_t cutoff_from_hist(const unsigned long long* h, double filter_fraction) {
+    const uint64_t unique = h[0];
+    uint64_t total = 0;
+    for (uint32_t c = 2; c < IX_HIST; ++c) total += h[c];
+    uint64_t idx = (uint64_t)((double)unique * filter_fraction), fc = 30;
+    if (idx < total) {
+        for (uint32_t c = IX_HIST - 1; c >= 2; --c) {
+            if (idx < h[c]) { fc = c; break; }
+            idx -= h[c];
+        }
+    } else if (total > 0) {
+        for (uint32_t c = 2; c < IX_HIST; ++c)
+            if (h[c]) { fc = c; break; }
+    }
+    return std::min<uint64_t>(std::max<uint64_t>(fc, 30), 100);
+}
+}  // namespace
+
+SeedParams Engine::build_index(const uint64_t* len, uint32_t n_contigs, SeedParams p, int bits,
+                               double filter_fraction) {
+    SA_CHECK(hipSetDevice(device_));
+    p.bits = bits;
+    // chunks of syncmer start positions per contig
+    constexpr uint32_t C = 4096;
+    std::vector<IxChunk> ch;
+    std::vector<uint64_t> first_chunk(n_contigs + 1);
+    for (uint32_t c = 0; c < n_contigs; ++c) {
+        first_chunk[c] = ch.size();
+        const uint64_t L = len[c];
+        if (L < (uint64_t)p.w_max) continue;
+        if (L >= (1ull << 31)) throw std::runtime_error("contigs longer than 2^31 bp are not supported");
+        for (uint64_t s = 0; s < L; s += C) ch.push_back(IxChunk{c, (uint32_t)s, (uint32_t)std::min<uint64_t>(L, s + C), 0});
+    }
+    first_chunk[n_contigs] = ch.size();
+    const uint32_t nch = (uint32_t)ch.size();
+    IxChunk* d_ch = nullptr;
+    uint64_t *d_cnt = nullptr, *d_off = nullptr, *d_syoff = nullptr;
+    SA_CHECK(hipMalloc(&d_ch, std::max<uint64_t>(nch, 1) * sizeof(IxChunk)));
+    SA_CHECK(hipMalloc(&d_cnt, (uint64_t)(nch + 1) * 8));
+    SA_CHECK(hipMalloc(&d_off, (uint64_t)(nch + 1) * 8));
+    SA_CHECK(hipMalloc(&d_syoff, (uint64_t)(n_contigs + 1) * 8));
+    if (nch) SA_CHECK(hipMemcpyAsync(d_ch, ch.data(), nch * sizeof(IxChunk), hipMemcpyHostToDevice, stream_));
+    k_ix_syncmers<true><<<blocks(nch, 256), 256, 0, stream_>>>(d_words_, d_word_off_, d_len_, d_ch, nch, p, d_cnt,
+                                                                nullptr, nullptr);
+    SA_CHECK(hipGetLastError());
+    uint64_t n = 0;
+    scan_u64(d_cnt, d_off, nch, &n);
+    if (n >= (1ull << 31)) throw std::runtime_error("more than 2^31 index entries: build the index on the host");
+    std::vector<uint64_t> off(nch);
+    if (nch) SA_CHECK(hipMemcpy(off.data(), d_off, nch * 8, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> syoff(n_contigs + 1);
+    for (uint32_t c = 0; c <= n_contigs; ++c) syoff[c] = first_chunk[c] < nch ? off[first_chunk[c]] : n;
+    SA_CHECK(hipMemcpy(d_syoff, syoff.data(), (uint64_t)(n_contigs + 1) * 8, hipMemcpyHostToDevice));
+    Syncmer* d_sy = nullptr;
+    RefEntry* d_e = nullptr;
+    SA_CHECK(hipMalloc(&d_sy, std::max<uint64_t>(n, 1) * sizeof(Syncmer)));
+    k_ix_syncmers<false><<<blocks(nch, 256), 256, 0, stream_>>>(d_words_, d_word_off_, d_len_, d_ch, nch, p, nullptr,
+                                                                 d_off, d_sy);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipMalloc(&d_e, std::max<uint64_t>(n, 1) * sizeof(RefEntry)));
+    k_ix_randstrobes<<<blocks(n, 256), 256, 0, stream_>>>(d_sy, n, d_syoff, n_contigs, p, d_e);
+    SA_CHECK(hipGetLastError());
+    sync();
+    SA_CHECK(hipFree(d_sy));
+    SA_CHECK(hipFree(d_ch));
+    SA_CHECK(hipFree(d_cnt));
+    SA_CHECK(hipFree(d_off));
+    SA_CHECK(hipFree(d_syoff));
+    // order (hash_offset, position, contig): stable by (position, contig), then
+    // stable by hash_offset
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    uint32_t *i0 = nullptr, *i1 = nullptr;
+    const uint64_t nn = std::max<uint64_t>(n, 1);
+    SA_CHECK(hipMalloc(&k0, nn * 8));
+    SA_CHECK(hipMalloc(&k1, nn * 8));
+    SA_CHECK(hipMalloc(&i0, nn * 4));
+    SA_CHECK(hipMalloc(&i1, nn * 4));
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    if (n) {
+        k_ix_poskey<<<blocks(n, 256), 256, 0, stream_>>>(d_e, n, k0, i0);
+        SA_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, stream_));
+        SA_CHECK(hipMalloc(&tmp, tmp_bytes));
+        SA_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i0, i1, (int)n, 0, 64, stream_));
+        k_ix_hashkey<<<blocks(n, 256), 256, 0, stream_>>>(d_e, i1, n, k0);
+        SA_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, i1, i0, (int)n, 0, 64, stream_));
+    }
+    p.n_entries = n;
+    p_ = p;
+    n_entries_ = n;
+    const uint64_t nb = (1ull << bits) + 1;
+    if (d_entries_) SA_CHECK(hipFree(d_entries_));
+    if (d_buckets_) SA_CHECK(hipFree(d_buckets_));
+    SA_CHECK(hipMalloc(&d_entries_, nn * sizeof(RefEntry)));
+    SA_CHECK(hipMalloc(&d_buckets_, nb * 8));
+    if (n) k_ix_gather<<<blocks(n, 256), 256, 0, stream_>>>(d_e, i0, n, d_entries_);
+    sync();
+    SA_CHECK(hipFree(d_e));
+    SA_CHECK(hipFree(k0));
+    SA_CHECK(hipFree(k1));
+    SA_CHECK(hipFree(i0));
+    SA_CHECK(hipFree(i1));
+    if (tmp) SA_CHECK(hipFree(tmp));
+    // bucket directory
+    uint64_t* d_first = nullptr;
+    unsigned long long* d_hist = nullptr;
+    SA_CHECK(hipMalloc(&d_first, 8));
+    SA_CHECK(hipMalloc(&d_hist, IX_HIST * 8));
+    k_ix_fill<<<blocks(nb, 256), 256, 0, stream_>>>(d_buckets_, nb, n);
+    k_ix_fill<<<1, 1, 0, stream_>>>(d_first, 1, n);
+    SA_CHECK(hipMemsetAsync(d_hist, 0, IX_HIST * 8, stream_));
+    if (n) {
+        k_ix_buckets<<<blocks(n, 256), 256, 0, stream_>>>(d_entries_, n, bits, d_buckets_, d_first);
+        RefEntry e0;
+        SA_CHECK(hipMemcpyAsync(&e0, d_entries_, sizeof e0, hipMemcpyDeviceToHost, stream_));
+        sync();
+        const uint64_t top0 = (e0.hash_offset & REF_HASH_MASK) >> (64 - bits);
+        k_ix_buckets_head<<<blocks(top0 + 1, 256), 256, 0, stream_>>>(d_entries_, n, bits, d_buckets_, d_first);
+        k_ix_runs<<<blocks(n, 256), 256, 0, stream_>>>(d_entries_, n, d_hist);
+    }
+    SA_CHECK(hipGetLastError());
+    unsigned long long hist[IX_HIST];
+    SA_CHECK(hipMemcpyAsync(hist, d_hist, sizeof hist, hipMemcpyDeviceToHost, stream_));
+    sync();
+    SA_CHECK(hipFree(d_first));
+    SA_CHECK(hipFree(d_hist));
+    p_.filter_cutoff = cutoff_from_hist(hist, filter_fraction);
+    setup_chain_tables();
+    return p_;
+}
+
+void Engine::download_index(RefEntry* entries, uint64_t* bucket_starts) const {
+    SA_CHECK(hipSetDevice(device_));
+    if (n_entries_) SA_CHECK(hipMemcpy(entries, d_entries_, n_entries_ * sizeof(RefEntry), hipMemcpyDeviceToHost));
+    SA_CHECK(hipMemcpy(bucket_starts, d_buckets_, ((1ull << p_.bits) + 1) * 8, hipMemcpyDeviceToHost));
 }
 
 void Engine::upload_reads(const uint8_t* seq, const uint64_t* off, uint32_t n_reads) {

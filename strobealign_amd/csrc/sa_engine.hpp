@@ -60,6 +60,13 @@ public:
     void upload_reference(const uint64_t* words, uint64_t n_words, const uint64_t* word_off, const uint64_t* len,
                           uint32_t n_contigs);
     void upload_index(const RefEntry* entries, uint64_t n, const uint64_t* bucket_starts, const SeedParams& p);
+    // make_index (src/indexer.rs:18-170) on the device from the uploaded reference:
+    // the same entries, bucket directory and filter cutoff as the host build
+    // (contig lengths on the host; p.bits set to `bits`). Returns the parameters.
+    SeedParams build_index(const uint64_t* len, uint32_t n_contigs, SeedParams p, int bits, double filter_fraction);
+    // copies the device index back (n_entries entries, 2^bits + 1 bucket starts)
+    void download_index(RefEntry* entries, uint64_t* bucket_starts) const;
+    void setup_chain_tables();
 
     // Seeding -> hits -> chaining -> NAMs for reads held in host memory (or already
     // resident when `seq == nullptr`, see upload_reads). Keeps the reads and the chain

@@ -124,8 +124,10 @@ SA_HD void item_finalize(uint32_t i, const uint8_t* seq, const uint64_t* off,
 // ------------------------------------------------------------- extension ---
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SA_ATOMIC_ADD(p, v) atomicAdd((p), (v))
+#define SA_ATOMIC_MIN(p, v) atomicMin((p), (v))
 #else
 #define SA_ATOMIC_ADD(p, v) ((*(p) += (v)) - (v))
+#define SA_ATOMIC_MIN(p, v) (*(p) = *(p) < (v) ? *(p) : (v))
 #endif
 
 // Longest query the lane-parallel SSW kernel handles; longer ones use the scalar form.
@@ -420,6 +422,116 @@ SA_HD void item_extend(uint32_t t, const uint8_t* seq, const uint64_t* off,
         }
     }
     res[ri] = out;
+}
+
+
+// ---------------------------------------------------------- index build ---
+// src/indexer.rs:18-170 on the device. The reference is cut into chunks of
+// positions; a chunk's thread scans its bases plus the k - 1 that follow, which
+// gives the syncmers starting in the chunk exactly as a scan of the whole contig
+// does (a syncmer decision depends only on the k bases of its k-mer, and a
+// packed reference has no wildcard restarts).
+struct IxChunk {
+    uint32_t contig, start, end;  // syncmer start positions [start, end) of the contig
+    uint32_t pad;
+};
+template <bool COUNT>
+SA_HD void item_ix_syncmers(uint32_t t, const uint64_t* words, const uint64_t* word_off, const uint64_t* len,
+                            const IxChunk* chunks, uint32_t nchunks, SeedParams p, uint64_t* counts,
+                            const uint64_t* offsets, Syncmer* out) {
+    if (t >= nchunks) return;
+    const IxChunk ch = chunks[t];
+    const uint64_t* w = words + word_off[ch.contig];
+    const uint32_t L = (uint32_t)len[ch.contig];
+    const uint32_t stop = ch.end + (uint32_t)p.k - 1 < L ? ch.end + (uint32_t)p.k - 1 : L;
+    const uint32_t base = ch.start;
+    uint64_t n = 0;
+    Syncmer* o = COUNT ? nullptr : out + offsets[t];
+    auto code = [&](uint32_t i) { const uint32_t a = base + i; return (uint32_t)(w[a >> 5] >> ((a & 31) * 2)) & 3u; };
+    auto emit = [&](uint64_t h, bool fwd, uint32_t pos) {
+        if (!COUNT) o[n] = Syncmer{h, ((base + pos) << 1) | (fwd ? 1u : 0u), 0};
+        ++n;
+    };
+    if (p.k - p.s + 1 == 5) syncmer_scan_w<5>(stop - base, p, code, emit);
+    else syncmer_scan(stop - base, p, code, emit);
+    if (COUNT) counts[t] = n;
+}
+
+// Randstrobe of syncmer a (src/seeding/strobes.rs:145-176) as the index entry
+// (src/indexer.rs:143-160); sy_off: first syncmer of each contig (n_contigs + 1).
+SA_HD void item_ix_randstrobes(uint32_t a, const Syncmer* sy, uint64_t n, const uint64_t* sy_off, uint32_t n_contigs,
+                               SeedParams p, RefEntry* out) {
+    if (a >= n) return;
+    uint32_t lo = 0, hi = n_contigs;  // the contig c with sy_off[c] <= a < sy_off[c + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (sy_off[mid] <= a) lo = mid; else hi = mid;
+    }
+    const uint32_t c = lo;
+    const uint64_t cs = sy_off[c];
+    const uint32_t m = (uint32_t)(sy_off[c + 1] - cs);
+    const uint32_t la = (uint32_t)(a - cs);
+    const uint32_t b = pick_strobe2(la, m, p, [&](uint32_t i) { return sy[cs + i]; });
+    const Syncmer s1 = sy[a], s2 = sy[cs + b];
+    const uint64_t h = randstrobe_hash(s1.hash, s2.hash, s1.pos_fwd & 1, s2.pos_fwd & 1, p);
+    const uint32_t p1 = s1.pos_fwd >> 1, p2 = s2.pos_fwd >> 1;
+    out[a] = RefEntry{(h & REF_HASH_MASK) | (uint64_t)((p2 - p1) & 0xFF), p1, c};
+}
+
+// Sort keys: the secondary key (position, then contig) and the hash key of the
+// entry a permutation points at.
+SA_HD void item_ix_poskey(uint32_t i, const RefEntry* e, uint64_t n, uint64_t* key, uint32_t* idx) {
+    if (i >= n) return;
+    key[i] = ((uint64_t)e[i].position << 32) | e[i].ref_index;
+    idx[i] = i;
+}
+SA_HD void item_ix_hashkey(uint32_t i, const RefEntry* e, const uint32_t* perm, uint64_t n, uint64_t* key) {
+    if (i >= n) return;
+    key[i] = e[perm[i]].hash_offset;
+}
+SA_HD void item_ix_gather(uint32_t i, const RefEntry* e, const uint32_t* perm, uint64_t n, RefEntry* out) {
+    if (i >= n) return;
+    out[i] = e[perm[i]];
+}
+
+// Bucket directory (src/indexer.rs:77-129 as the host build writes it): slot t
+// holds the first entry, after entry 0, that starts a new hash with top bits
+// >= t; slot 0 is 0; slots past the last hash hold n. `bs` is filled with n
+// before. Entry pos starting a new hash writes slots (top(pos - 1), top(pos)].
+SA_HD void item_ix_buckets(uint32_t i, const RefEntry* e, uint64_t n, int bits, uint64_t* bs, uint64_t* first_new) {
+    const uint64_t pos = i;
+    if (pos == 0 || pos >= n) return;
+    const uint64_t prev = e[pos - 1].hash_offset & REF_HASH_MASK, cur = e[pos].hash_offset & REF_HASH_MASK;
+    if (prev == cur) return;
+    const uint64_t t0 = prev >> (64 - bits), t1 = cur >> (64 - bits);
+    for (uint64_t t = t0 + 1; t <= t1; ++t) bs[t] = pos;
+    if (t0 == (e[0].hash_offset & REF_HASH_MASK) >> (64 - bits)) SA_ATOMIC_MIN(first_new, pos);
+}
+// Slots 1 .. top(entry 0) take the first new hash (or n), as the host build's
+// first push does.
+SA_HD void item_ix_buckets_head(uint32_t t, const RefEntry* e, uint64_t n, int bits, uint64_t* bs,
+                                const uint64_t* first_new) {
+    if (n == 0) return;
+    const uint64_t top0 = (e[0].hash_offset & REF_HASH_MASK) >> (64 - bits);
+    if (t == 0) bs[0] = 0;
+    else if (t <= top0) bs[t] = *first_new < n ? *first_new : n;
+}
+
+SA_HD void item_ix_fill(uint32_t i, uint64_t* a, uint64_t n, uint64_t v) {
+    if (i < n) a[i] = v;
+}
+
+// Per hash run starting at entry i: one distinct hash, and its length (at most
+// IX_HIST - 1, longer runs in the last slot) in the histogram of repeated hashes.
+constexpr uint32_t IX_HIST = 102;
+SA_HD void item_ix_runs(uint32_t i, const RefEntry* e, uint64_t n, unsigned long long* hist) {
+    if (i >= n) return;
+    const uint64_t h = e[i].hash_offset & REF_HASH_MASK;
+    if (i > 0 && (e[i - 1].hash_offset & REF_HASH_MASK) == h) return;
+    uint64_t len = 1;
+    while (len < IX_HIST - 1 && i + len < n && (e[i + len].hash_offset & REF_HASH_MASK) == h) ++len;
+    SA_ATOMIC_ADD(&hist[0], 1ull);  // slot 0: distinct hashes
+    if (len >= 2) SA_ATOMIC_ADD(&hist[len], 1ull);
 }
 
 }  // namespace sa

@@ -345,4 +345,50 @@ __global__ void __launch_bounds__(64) k_ssw_raw(const int8_t* __restrict__ read,
     *out = o;
 }
 
+
+// ------------------------------------------------- index construction ---
+template <bool COUNT>
+__global__ void __launch_bounds__(256) k_ix_syncmers(const uint64_t* __restrict__ words,
+                                                    const uint64_t* __restrict__ word_off,
+                                                    const uint64_t* __restrict__ len,
+                                                    const IxChunk* __restrict__ chunks, uint32_t nchunks, SeedParams p,
+                                                    uint64_t* __restrict__ counts, const uint64_t* __restrict__ offsets,
+                                                    Syncmer* __restrict__ out) {
+    item_ix_syncmers<COUNT>(blockIdx.x * blockDim.x + threadIdx.x, words, word_off, len, chunks, nchunks, p, counts,
+                            offsets, out);
+}
+__global__ void __launch_bounds__(256) k_ix_randstrobes(const Syncmer* __restrict__ sy, uint64_t n,
+                                                       const uint64_t* __restrict__ sy_off, uint32_t n_contigs,
+                                                       SeedParams p, RefEntry* __restrict__ out) {
+    item_ix_randstrobes(blockIdx.x * blockDim.x + threadIdx.x, sy, n, sy_off, n_contigs, p, out);
+}
+__global__ void __launch_bounds__(256) k_ix_poskey(const RefEntry* __restrict__ e, uint64_t n, uint64_t* __restrict__ key,
+                                                  uint32_t* __restrict__ idx) {
+    item_ix_poskey(blockIdx.x * blockDim.x + threadIdx.x, e, n, key, idx);
+}
+__global__ void __launch_bounds__(256) k_ix_hashkey(const RefEntry* __restrict__ e, const uint32_t* __restrict__ perm,
+                                                   uint64_t n, uint64_t* __restrict__ key) {
+    item_ix_hashkey(blockIdx.x * blockDim.x + threadIdx.x, e, perm, n, key);
+}
+__global__ void __launch_bounds__(256) k_ix_gather(const RefEntry* __restrict__ e, const uint32_t* __restrict__ perm,
+                                                  uint64_t n, RefEntry* __restrict__ out) {
+    item_ix_gather(blockIdx.x * blockDim.x + threadIdx.x, e, perm, n, out);
+}
+__global__ void __launch_bounds__(256) k_ix_fill(uint64_t* __restrict__ a, uint64_t n, uint64_t v) {
+    item_ix_fill(blockIdx.x * blockDim.x + threadIdx.x, a, n, v);
+}
+__global__ void __launch_bounds__(256) k_ix_buckets(const RefEntry* __restrict__ e, uint64_t n, int bits,
+                                                   uint64_t* __restrict__ bs, uint64_t* __restrict__ first_new) {
+    item_ix_buckets(blockIdx.x * blockDim.x + threadIdx.x, e, n, bits, bs, first_new);
+}
+__global__ void __launch_bounds__(256) k_ix_buckets_head(const RefEntry* __restrict__ e, uint64_t n, int bits,
+                                                        uint64_t* __restrict__ bs,
+                                                        const uint64_t* __restrict__ first_new) {
+    item_ix_buckets_head(blockIdx.x * blockDim.x + threadIdx.x, e, n, bits, bs, first_new);
+}
+__global__ void __launch_bounds__(256) k_ix_runs(const RefEntry* __restrict__ e, uint64_t n,
+                                                unsigned long long* __restrict__ hist) {
+    item_ix_runs(blockIdx.x * blockDim.x + threadIdx.x, e, n, hist);
+}
+
 }  // namespace sa

@@ -143,6 +143,56 @@ def test_paired_end_100m_vs_oracle(sa, ctx, tmp_path_factory):
     assert got == exp
 
 
+def _repeat_fasta(path, copies, seg_len, divergence, seed):
+    """Two contigs: `copies` diverged copies of one segment, then random sequence."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    rng = np.random.default_rng(seed)
+    seg = rng.integers(0, 4, seg_len, dtype=np.uint8)
+    parts = []
+    for _ in range(copies):
+        s = seg.copy()
+        m = rng.random(seg_len) < divergence
+        s[m] = (s[m] + rng.integers(1, 4, int(m.sum()), dtype=np.uint8)) % 4
+        parts += [s, rng.integers(0, 4, int(rng.integers(0, 500)), dtype=np.uint8)]
+    synth.write_fasta(path, ["rep", "rnd", "tiny"], [np.concatenate(parts), rng.integers(0, 4, 300_000, dtype=np.uint8),
+                                                     rng.integers(0, 4, 10, dtype=np.uint8)])
+
+
+def test_device_index_build_matches_host(sa, ctx, small, tmp_path_factory):
+    """make_index on the device (sa_index_build_device) gives the host build's
+    entries, bucket starts and filter cutoff: references with no, moderate and
+    heavy repeats (cutoffs 30, 45-60 and 100), read-length profiles r50/r150/r250,
+    explicit bucket bits (first hash above slot 0), and the 100 Mbp reference,
+    whose device-built index then maps like the oracle."""
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    d = str(tmp_path_factory.mktemp("ixdev"))
+    rep45, rep400 = os.path.join(d, "rep45.fa"), os.path.join(d, "rep400.fa")
+    _repeat_fasta(rep45, 45, 20_000, 0.0, 1)
+    _repeat_fasta(rep400, 400, 5_000, 0.001, 2)
+    w = synth.make("pe100m", d, 3000)
+    cases = [(PHIX, 150, 0), (PHIX, 150, 20), (small["ref"], 50, 0), (small["ref"], 250, 8), (rep45, 150, 0),
+             (rep400, 250, 0), (w["ref"], 150, 0)]
+    cutoffs = set()
+    for ref, rl, bits in cases:
+        say("case: device index build, %s r%d bits %d" % (os.path.basename(ref), rl, bits))
+        with sa.Index(ctx, ref, read_length=rl, bits=bits, threads=8) as a, \
+                sa.Index(ctx, ref, read_length=rl, bits=bits, device_build=True) as b:
+            pa, na, ca = a.params()
+            pb, nb, cb = b.params()
+            assert (na, ca, pa.bits, pa.filter_cutoff) == (nb, cb, pb.bits, pb.filter_cutoff)
+            ea, ba = a.download()
+            eb, bb = b.download()
+            assert ea == eb and ba == bb
+            cutoffs.add(pb.filter_cutoff)
+    assert 30 in cutoffs and 100 in cutoffs and len(cutoffs) >= 3
+    exp = oracle("-t", "8", "--no-PG", w["ref"], w["reads1"], w["reads2"])
+    with sa.Index(ctx, w["ref"], read_length=150, device_build=True) as ix:
+        assert ix.map_files(w["reads1"], w["reads2"], threads=8) == exp
+
+
 def test_single_end_variable_length_vs_oracle(sa, ctx, tmp_path_factory):
     """Single-end reads of 50-500 bp (15% from repeat copies) on the 100 Mbp
     reference: the r250 profile from the estimated read length, piecewise and
