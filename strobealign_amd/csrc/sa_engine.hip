@@ -577,11 +577,7 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     auto* d_namn = (uint32_t*)grow(S_NAMN, (2ull * n + 1) * 4);
     auto* d_araw = (uint32_t*)grow(S_ARAW, (2ull * n + 1) * 4);
     SA_CHECK(hipEventRecord(ev[3], stream_));
-    // 32 KB of LDS per 256-thread block, unused by the kernel, caps residency at
-    // 5 blocks (20 waves) per CU: fewer threads' chaining arrays compete for the
-    // caches (one engine, bench workload: 36.8 -> 34.7 ms per step; 12 waves: 38.1)
-    constexpr uint32_t CHAIN_LDS_CAP = 32768;
-    k_chain<<<blocks(2ull * n, 256), 256, CHAIN_LDS_CAP, stream_>>>(ix, cp, d_off, n, d_hits, d_hc, d_acap, d_aoff, b, d_namn,
+    k_chain<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, cp, d_off, n, d_hits, d_hc, d_acap, d_aoff, b, d_namn,
                                                          d_araw, d_log2_, d_same_diag_);
     SA_CHECK(hipGetLastError());
     SA_CHECK(hipEventRecord(ev[4], stream_));
