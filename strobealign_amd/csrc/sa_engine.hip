@@ -219,6 +219,8 @@ Engine::~Engine() {
                         (void*)d_log2_})
             if (p) (void)hipFree(p);
     if (d_same_diag_) (void)hipFree(d_same_diag_);
+    for (void* h : h_stage_)
+        if (h) (void)hipHostFree(h);
     if (sync_ev_) (void)hipEventDestroy(sync_ev_);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -264,6 +266,16 @@ void* Engine::grow(int slot, uint64_t bytes) {
         cap_[slot] = want;
     }
     return buf_[slot];
+}
+
+void* Engine::host_stage(int slot, uint64_t bytes) {
+    if (h_stage_cap_[slot] < bytes) {
+        if (h_stage_[slot]) SA_CHECK(hipHostFree(h_stage_[slot]));
+        const uint64_t want = bytes + bytes / 4 + 4096;
+        SA_CHECK(hipHostMalloc(&h_stage_[slot], want, hipHostMallocDefault));
+        h_stage_cap_[slot] = want;
+    }
+    return h_stage_[slot];
 }
 
 void Engine::scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* total) {
@@ -522,8 +534,9 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
 void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
     SA_CHECK(hipSetDevice(device_));
     const uint32_t n = n_reads_;
-    out.nams.clear();
-    out.nam_off.assign(1, 0);
+    out.nams = HostView<NamRec>{};
+    out.nam_off = HostView<uint64_t>{(uint64_t*)host_stage(0, 8), 1};
+    out.nam_off[0] = 0;
     out.info.clear();
     if (n == 0) return;
     // same-diagonal score table (src/chainer.rs:63-74), host f32 arithmetic
@@ -594,19 +607,29 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     k_finalize<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, ref, k, d_namn, d_aoff, b.nams, d_outoff, d_nams);
     SA_CHECK(hipGetLastError());
     SA_CHECK(hipEventRecord(fe1, stream_));
-    out.nams.resize(N);
-    out.nam_off.resize(n + 1);
+    // everything the host reads back goes to pinned staging memory in one block
+    const uint64_t n2 = 2ull * n;
+    const uint64_t o_off = ((uint64_t)N * sizeof(NamRec) + 15) & ~15ull;
+    const uint64_t o_hc = o_off + (((uint64_t)(n + 1) * 8 + 15) & ~15ull);
+    const uint64_t o_u32 = o_hc + ((n2 * sizeof(HitCounts) + 15) & ~15ull);
+    const uint64_t u32_bytes = (n2 * 4 + 15) & ~15ull;
+    auto* hs = (uint8_t*)host_stage(0, o_u32 + 5 * u32_bytes);
+    out.nams = HostView<NamRec>{(NamRec*)hs, N};
+    out.nam_off = HostView<uint64_t>{(uint64_t*)(hs + o_off), (size_t)n + 1};
+    const HitCounts* hc = (const HitCounts*)(hs + o_hc);
+    uint32_t* qrn = (uint32_t*)(hs + o_u32);
+    uint32_t* namn = (uint32_t*)(hs + o_u32 + u32_bytes);
+    uint32_t* araw = (uint32_t*)(hs + o_u32 + 2 * u32_bytes);
+    uint32_t* cbytes = (uint32_t*)(hs + o_u32 + 3 * u32_bytes);
+    uint32_t* hbytes = (uint32_t*)(hs + o_u32 + 4 * u32_bytes);
     if (N) SA_CHECK(hipMemcpyAsync(out.nams.data(), d_nams, N * sizeof(NamRec), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(out.nam_off.data(), d_outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
-    std::vector<HitCounts> hc(2ull * n);
-    std::vector<uint32_t> qrn(2ull * n), namn(2ull * n), araw(2ull * n), cbytes(2ull * n);
-    SA_CHECK(hipMemcpyAsync(cbytes.data(), b.alg_bytes, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(hc.data(), d_hc, 2ull * n * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(qrn.data(), d_qrn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(namn.data(), d_namn, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(araw.data(), d_araw, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
-    std::vector<uint32_t> hbytes(2ull * n);
-    SA_CHECK(hipMemcpyAsync(hbytes.data(), d_hbytes, 2ull * n * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(cbytes, b.alg_bytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync((void*)hc, d_hc, n2 * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(qrn, d_qrn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(namn, d_namn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(araw, d_araw, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(hbytes, d_hbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
     sync();
     out.nam_off[n] = N;
     out.info.resize(n);

@@ -16,9 +16,24 @@ namespace sa {
 struct Scores;
 
 // Host copy of what the chainer/NAM stage returns for one batch of reads.
+// A host array owned elsewhere (the engine's pinned staging memory).
+template <typename T>
+struct HostView {
+    T* p = nullptr;
+    size_t n = 0;
+    T& operator[](size_t i) const { return p[i]; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    T* data() const { return p; }
+    T* begin() const { return p; }
+    T* end() const { return p + n; }
+};
+
+// NAMs of a batch. nams and nam_off point into the engine's pinned staging
+// memory and stay valid until that engine's next find_nams.
 struct NamBatch {
-    std::vector<NamRec> nams;       // per read, stable-sorted by score (pre-shuffle)
-    std::vector<uint64_t> nam_off;  // n_reads + 1
+    HostView<NamRec> nams;          // per read, stable-sorted by score (pre-shuffle)
+    HostView<uint64_t> nam_off;     // n_reads + 1
     std::vector<ReadNamInfo> info;  // n_reads
 };
 
@@ -67,6 +82,8 @@ public:
     // copies the device index back (n_entries entries, 2^bits + 1 bucket starts)
     void download_index(RefEntry* entries, uint64_t* bucket_starts) const;
     void setup_chain_tables();
+    // pinned host memory of staging slot `slot` (kept, grown on demand)
+    void* host_stage(int slot, uint64_t bytes);
 
     // Seeding -> hits -> chaining -> NAMs for reads held in host memory (or already
     // resident when `seq == nullptr`, see upload_reads). Keeps the reads and the chain
@@ -136,6 +153,8 @@ private:
     // growable device buffers indexed by slot
     std::vector<void*> buf_;
     std::vector<uint64_t> cap_;
+    void* h_stage_[2] = {nullptr, nullptr};
+    uint64_t h_stage_cap_[2] = {0, 0};
     StageTimes times_;
     std::vector<std::pair<std::string, KernelStat>> kstats_;
     double total_ms_ = 0;
