@@ -104,17 +104,17 @@ def serve(sa, ix, reads, threads, slots, plan, paced):
     t0 = time.perf_counter()
     lag = 0.0
     if paced:
-        for k, (ta, a, n) in enumerate(plan):
+        for k, (ta, a, n, *_) in enumerate(plan):
             d = t0 + ta - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             lag = max(lag, -d)
-            q.put((k, srv.submit(reads, a, n)))
+            q.put((k, srv.submit(plan[k][3] if len(plan[k]) > 3 else reads, a, n)))
     else:
         # a backlog from several clients at once: four submitting threads
         def client(ks):
             for k in ks:
-                q.put((k, srv.submit(reads, plan[k][1], plan[k][2])))
+                q.put((k, srv.submit(plan[k][3] if len(plan[k]) > 3 else reads, plan[k][1], plan[k][2])))
         cs = [threading.Thread(target=client, args=(list(range(c, len(plan), 4)),)) for c in range(4)]
         for c in cs:
             c.start()
@@ -128,6 +128,44 @@ def serve(sa, ix, reads, threads, slots, plan, paced):
     st = srv.stats()
     srv.close()
     return lat, wall, st, lag
+
+
+def serve_varied(a, sa, synth, ctx, threads, slots):
+    """Requests as the serving configuration describes them: Poisson arrivals,
+    log-uniform [1, 50000] pairs, one read length per request drawn uniformly
+    from [100, 300] (rounded to the 25 bp steps of the pre-generated read sets),
+    on a 100 Mbp index built for read length 250. Returns the serving object."""
+    lengths = list(range(100, 301, 25))
+    w = synth.make_lengths(os.path.join(a.data_dir, "lengths"), lengths, 50_000)
+    ix = sa.Index(ctx, w["ref"], read_length=250, device_build=True)
+    rs = {L: sa.Reads(*w["sets"][L]) for L in lengths}
+    for r in rs.values():
+        r.upload(ix)
+    rnd = random.Random(13)
+    plan, t, pos = [], 0.0, {L: 0 for L in lengths}
+    for _ in range(a.serve_varied):
+        t += rnd.expovariate(a.serve_rate)
+        n = min(50_000, max(1, int(math.exp(rnd.uniform(0.0, math.log(50_000))))))
+        L = lengths[min(len(lengths) - 1, int(round((rnd.uniform(100, 300) - 100) / 25)))]
+        if pos[L] + n > 50_000:
+            pos[L] = 0
+        plan.append((t, pos[L], n, rs[L]))
+        pos[L] += n
+    lat, wall, st, lag = serve(sa, ix, None, threads, slots, plan, True)
+    _, sat_wall, _, _ = serve(sa, ix, None, threads, slots, plan, False)
+    pairs = sum(p[2] for p in plan)
+    out = {"requests": len(plan), "arrivals": "Poisson %.0f/s" % a.serve_rate,
+           "read_lengths": "uniform [100, 300] per request, in 25 bp steps", "index_read_length": 250,
+           "pairs_per_request": "log-uniform [1, 50000] (mean %.0f)" % (pairs / len(plan)),
+           "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2), "max_ms": round(max(lat), 2),
+           "served_pairs_per_s": round(pairs / wall, 1), "saturated_pairs_per_s": round(pairs / sat_wall, 1),
+           "batches": st["batches"], "driver_max_lag_ms": round(1000 * lag, 2)}
+    for r in rs.values():
+        r.close()
+    ix.close()
+    print("serving (100-300 bp): p50 %.1f ms, p99 %.1f ms, saturated %.0f pairs/s" %
+          (out["p50_ms"], out["p99_ms"], out["saturated_pairs_per_s"]), file=sys.stderr, flush=True)
+    return out
 
 
 def pct(v, p):
@@ -386,6 +424,8 @@ def main():
     ap.add_argument("--tune-pairs", type=int, default=0, help="pairs the autotuner times (0: the whole workload)")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-varied", type=int, default=0,
+                    help="requests of a serving run with 100-300 bp reads on an r250 index (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -492,6 +532,7 @@ def main():
                    "engines": slots}
         print("serving: p50 %.1f ms, p99 %.1f ms, saturated %.0f pairs/s" %
               (serving["p50_ms"], serving["p99_ms"], serving["saturated_pairs_per_s"]), file=sys.stderr, flush=True)
+    serving_varied = serve_varied(a, sa, synth, ctx, threads, a.slots or 4) if a.serve_varied > 0 else None
     cpu = None
     if not a.no_cpu and a.cpu_pairs > 0:
         try:
@@ -530,6 +571,7 @@ def main():
                    "batched_over_plain": round(value / single, 3) if single else None},
         "latency_ms": {"p50": serving["p50_ms"], "p99": serving["p99_ms"]} if serving else None,
         "serving": serving,
+        "serving_varied_lengths": serving_varied,
         "roofline": roof,
         "lookup": lookup,
         "extension": extension,

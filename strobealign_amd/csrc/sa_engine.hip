@@ -768,7 +768,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     if (nssw) {
         // launches every non-empty bucket of one (mode, pass) family
         auto run = [&](bool word, bool rev) {
-            for (uint32_t seg = 1; seg <= (word ? 32u : 16u); ++seg) {
+            for (uint32_t seg = 1; seg <= (word ? SSW_WORD_SEGS : SSW_BYTE_SEGS); ++seg) {
                 // one launch per stripe count: its window classes, longest first
                 SswLaunchLists ll{};
                 uint32_t at = 0;
@@ -783,10 +783,10 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 ll.start[SSW_WCLASSES] = at;
                 if (!at) continue;
                 if (word)
-                    launch_ssw_pass<8, false, 1, 32>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
+                    launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
                                                      d_fwd, d_ends, d_lists, d_bcnt, cap);
                 else
-                    launch_ssw_pass<16, true, 1, 16>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
+                    launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
                                                      d_fwd, d_ends, d_lists, d_bcnt, cap);
                 SA_CHECK(hipGetLastError());
             }
@@ -794,8 +794,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); }, 0, fwd_cells);
         read_counts();
         uint64_t nrb = 0, nfw = 0;
-        for (uint32_t x = 0; x < 16; ++x) nrb += bc[ssw_bucket(false, true, x + 1, 0)];
-        for (uint32_t x = 0; x < 32; ++x)
+        for (uint32_t x = 0; x < SSW_BYTE_SEGS; ++x) nrb += bc[ssw_bucket(false, true, x + 1, 0)];
+        for (uint32_t x = 0; x < SSW_WORD_SEGS; ++x)
             for (uint32_t wc = 0; wc < SSW_WCLASSES; ++wc) nfw += bc[ssw_bucket(true, false, x + 1, wc)];
         if (nrb) timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
         if (nfw) {

@@ -131,7 +131,7 @@ SA_HD void item_finalize(uint32_t i, const uint8_t* seq, const uint64_t* off,
 #endif
 
 // Longest query the lane-parallel SSW kernel handles; longer ones use the scalar form.
-constexpr uint32_t SSW_GROUP_MAX_QUERY = 256;
+constexpr uint32_t SSW_GROUP_MAX_QUERY = 512;
 
 SA_HD void write_found(ExtResult& out, const AlignInfo& a, uint64_t ref_base, bool gapped, uint32_t n_cigar) {
     out.found = 1;

@@ -243,12 +243,18 @@ __global__ void __launch_bounds__(256) k_fin_bucket(const SswEnds* __restrict__ 
 // ~1.2 L, rescue windows up to 2000).
 constexpr uint32_t SSW_WCLASSES = 4;
 SA_HD uint32_t ssw_wclass(uint32_t wlen) { return wlen < 256 ? 0u : wlen < 512 ? 1u : wlen < 1024 ? 2u : 3u; }
-// byte forward 0..63 (16 stripe counts x 4 classes), byte reverse 64..79,
-// word forward 80..207 (32 x 4), word reverse 208..239
-constexpr uint32_t SSW_BUCKETS = 240;
+// Stripe counts of queries up to SSW_GROUP_MAX_QUERY (512): 32 of 16 byte lanes,
+// 64 of 8 word lanes. Buckets: byte forward 0..127 (32 stripe counts x 4
+// classes), byte reverse 128..159, word forward 160..415 (64 x 4), word reverse
+// 416..479.
+constexpr uint32_t SSW_BYTE_SEGS = 32, SSW_WORD_SEGS = 64;
+constexpr uint32_t SSW_BYTE_REV0 = SSW_BYTE_SEGS * SSW_WCLASSES;
+constexpr uint32_t SSW_WORD_FWD0 = SSW_BYTE_REV0 + SSW_BYTE_SEGS;
+constexpr uint32_t SSW_WORD_REV0 = SSW_WORD_FWD0 + SSW_WORD_SEGS * SSW_WCLASSES;
+constexpr uint32_t SSW_BUCKETS = SSW_WORD_REV0 + SSW_WORD_SEGS;
 SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg, uint32_t wc) {
-    if (rev) return (word ? 208u : 64u) + seg - 1;
-    return (word ? 80u : 0u) + (seg - 1) * SSW_WCLASSES + wc;
+    if (rev) return (word ? SSW_WORD_REV0 : SSW_BYTE_REV0) + seg - 1;
+    return (word ? SSW_WORD_FWD0 : 0u) + (seg - 1) * SSW_WCLASSES + wc;
 }
 
 // Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY);

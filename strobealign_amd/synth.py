@@ -215,6 +215,26 @@ def make(workload, out_dir, n_reads=None):
     raise ValueError(workload)
 
 
+def make_lengths(out_dir, lengths, n_pairs):
+    """Paired-end read sets of several read lengths on the pe100m reference
+    (fragment Normal(max(300, 2L), 30), 0.5% substitutions, 0.05% indels);
+    returns {"ref": path, "sets": {L: (reads1, reads2)}}."""
+    os.makedirs(out_dir, exist_ok=True)
+    ref = os.path.join(out_dir, "ref.fa")
+    sets = {L: (os.path.join(out_dir, "len%d_%d_1.fq" % (L, n_pairs)), os.path.join(out_dir, "len%d_%d_2.fq" % (L, n_pairs)))
+            for L in lengths}
+    if os.path.exists(ref) and all(os.path.exists(x) and os.path.exists(y) for x, y in sets.values()):
+        return {"ref": ref, "sets": sets}
+    contigs, _ = random_reference(np.random.default_rng(7), 20, 5_000_000, repeat_fraction=0.30)
+    if not os.path.exists(ref):
+        write_fasta(ref, ["ctg%02d" % (i + 1) for i in range(20)], contigs)
+    for L, (r1, r2) in sets.items():
+        if not (os.path.exists(r1) and os.path.exists(r2)):
+            paired_end(np.random.default_rng(1000 + L), contigs, n_pairs, L, 0.005, 0.0005, r1, r2,
+                       mean=max(300, 2 * L), sd=30, prefix="l%d_" % L)
+    return {"ref": ref, "sets": sets}
+
+
 def make_human(out_dir, n_pairs=1_000_000):
     """The 3.1 Gbp workload (see the module docstring); files are cached in out_dir."""
     os.makedirs(out_dir, exist_ok=True)
