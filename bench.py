@@ -495,7 +495,9 @@ def main():
         print("one engine: %.0f pairs/s" % single, file=sys.stderr, flush=True)
     k2 = ix.kernel_times()
     # dominant kernel over the timed steps: HIP-event time on the engines' streams;
-    # algorithmic bytes are counted by the engine per launch
+    # algorithmic bytes are counted by the engine per launch (bytes each stage must
+    # read and write: read bases and randstrobes, bucket words and entries probed,
+    # hits, NAMs and chain anchors)
     z = (0, 0, 0, 0, 0)
     delta = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
     # Per-kernel figures come from the one-engine pass when it ran: there each kernel

@@ -114,7 +114,7 @@ def load(path=LIB_PATH):
     if _lib is not None:
         return _lib
     if not os.path.exists(path):
-        raise SaError("libsa_hip.so not built (run __graft_entry__.build() or make -C strobealign_amd/csrc)")
+        raise SaError("libsa_hip.so not built (run make -C strobealign_amd/csrc)")
     lib = ctypes.CDLL(path)
     lib.sa_last_error.restype = ctypes.c_size_t
     lib.sa_version.restype = ctypes.c_char_p
