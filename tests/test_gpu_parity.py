@@ -143,6 +143,19 @@ def test_paired_end_100m_vs_oracle(sa, ctx, tmp_path_factory):
     assert got == exp
 
 
+def test_paired_end_300bp_vs_oracle(sa, ctx, tmp_path_factory):
+    """2x300 pairs (queries of 257-512 bp take the widest SSW lane groups: up to
+    19 byte / 38 word stripes) on the 100 Mbp repetitive reference, SAM and -x."""
+    say("case: 100 Mbp repetitive reference, 3000 pairs 2x300")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    w = synth.make_lengths(str(tmp_path_factory.mktemp("pe300")), [300], 3000)
+    r1, r2 = w["sets"][300]
+    with sa.Index(ctx, w["ref"], read_length=300, threads=8) as ix:
+        assert ix.map_files(r1, r2, threads=8) == oracle("-t", "8", "--no-PG", w["ref"], r1, r2)
+        assert ix.map_files(r1, r2, mode=1, threads=8) == oracle("-t", "8", "-x", w["ref"], r1, r2)
+
+
 def _repeat_fasta(path, copies, seg_len, divergence, seed):
     """Two contigs: `copies` diverged copies of one segment, then random sequence."""
     import numpy as np
