@@ -335,7 +335,8 @@ def bench_workload(a, sa, synth, cores, threads):
     cpu = None
     parity = None
     if not a.no_cpu and a.cpu_pairs != 0:
-        sample = min(a.cpu_pairs or 20_000, n_units)
+        # enough records to keep all host threads busy (one chunk per thread and more)
+        sample = min(a.cpu_pairs or (100_000 if human else 200_000), n_units)
         osam = os.path.join(a.data_dir, "oracle_sample.sam")
         try:
             v, t, (s1, s2) = cpu_baseline(w["ref"], w["reads1"], w.get("reads2"), sample, cores, a.data_dir,
@@ -415,7 +416,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads (default: all cores)")
     ap.add_argument("--data-dir", default=os.environ.get("SA_BENCH_DATA", "/tmp/strobealign_amd_bench"))
     ap.add_argument("--cpu-pairs", type=int, default=None,
-                    help="oracle sample size (0: skip; default 200,000, human 20,000)")
+                    help="oracle sample size (0: skip; default 200,000, human 100,000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-index", action="store_true",
                     help="human/se_var: build the index on the host instead of the device")
