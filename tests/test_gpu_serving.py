@@ -141,3 +141,30 @@ def test_admission_waits_for_arena(sa, pe):
         reads.close()
     assert mixed == alone
     assert st["arena_peak"] <= 200_000
+
+
+def test_requests_of_several_read_lengths(sa, tmp_path):
+    """The serving configuration's request shape: each request one read length
+    (100 to 300 bp, including queries past 256 bp) on an index built for read
+    length 250; every request's output equals mapping its records alone."""
+    say("case: serving, requests of 100-300 bp pairs on an r250 index vs mapped alone")
+    sys.path.insert(0, ROOT)
+    from strobealign_amd import synth
+    lengths = [100, 175, 250, 275, 300]
+    w = synth.make_lengths(str(tmp_path / "len"), lengths, 400)
+    recs = {L: (fastq_records(w["sets"][L][0]), fastq_records(w["sets"][L][1])) for L in lengths}
+    reqs = [(L, a, n) for L, a, n in ((300, 0, 400), (100, 10, 57), (275, 0, 200), (250, 100, 300), (175, 0, 1),
+                                      (300, 50, 133), (100, 0, 400))]
+    with sa.Context(0) as ctx, sa.Index(ctx, w["ref"], read_length=250, device_build=True) as ix:
+        expected = []
+        for k, (L, a, n) in enumerate(reqs):
+            p1, p2 = str(tmp_path / ("q%d_1.fq" % k)), str(tmp_path / ("q%d_2.fq" % k))
+            write_fastq(p1, recs[L][0][a:a + n])
+            write_fastq(p2, recs[L][1][a:a + n])
+            expected.append(no_header(ix.map_files(p1, p2, threads=8, chunk_size=20_000)))
+        opts = sa.options(threads=8, chunk_size=20_000, batch_bases=200_000, slots=3)
+        with sa.Server(ix, opts) as srv:
+            tickets = [srv.submit_records(recs[L][0][a:a + n], recs[L][1][a:a + n]) for L, a, n in reqs]
+            outs = [srv.wait(t)[0] for t in tickets]
+    for k in range(len(reqs)):
+        assert outs[k] == expected[k], reqs[k]
