@@ -205,16 +205,34 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
         uint32_t ws = r.win_start, we = r.win_end;
         RefWin w{&ref, r.ref_id, ws, we - ws};
         uint32_t sub = 2 * k / 3, step = k / 3;
-        bool shared = (r.flags & 8) != 0;
+        bool shared = (r.flags & 8) != 0, found = false;
+        uint32_t hit_i = 0, hit_p = 0;
         uint32_t lim = L > sub ? L - sub : 0;
         for (uint32_t i0 = 0; i0 < lim && !shared; i0 += step) {
             for (uint32_t p = 0; p + sub <= w.len && !shared; ++p) {
                 uint32_t x = 0;
                 while (x < sub && w[p + x] == q[i0 + x]) ++x;
-                if (x == sub) shared = true;
+                if (x == sub) {
+                    shared = found = true;
+                    hit_i = i0;
+                    hit_p = p;
+                }
             }
         }
-        if (shared) push_ssw(ws, we, 0u);
+        // best ungapped segment on the diagonal of the first shared substring
+        uint32_t jflags = 0;
+        if (found) {
+            const int64_t d = (int64_t)hit_p - (int64_t)hit_i;
+            const int64_t x0 = d < 0 ? -d : 0;
+            const int64_t x1 = (int64_t)L < (int64_t)(we - ws) - d ? (int64_t)L : (int64_t)(we - ws) - d;
+            const uint32_t n = x1 > x0 ? (uint32_t)(x1 - x0) : 0;
+            const RefWin wd{&ref, r.ref_id, (uint64_t)((int64_t)ws + d + x0), n};
+            SegSum s;
+            for (uint32_t j = 0; j < n; ++j)
+                s.push(ssw_code(q[(uint32_t)x0 + j]) == (int32_t)wd.code(j) ? sc.match : -sc.mismatch);
+            jflags = s.best >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;
+        }
+        if (shared) push_ssw(ws, we, jflags);
     }
     res[t] = out;
 }

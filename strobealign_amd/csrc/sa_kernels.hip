@@ -177,6 +177,8 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
         RefWin w{&ref, r.ref_id, ws, we - ws};
         const uint32_t sub = 2 * k / 3, step = k / 3;
         bool shared = (r.flags & 8) != 0;
+        bool found = false;
+        uint32_t hit_i = 0, hit_p = 0;  // the first shared substring: query and window offsets
         const uint32_t lim = L > sub ? L - sub : 0;
         for (uint32_t i0 = 0; i0 < lim && !shared; i0 += step) {
             for (uint32_t p0 = 0; p0 + sub <= w.len && !shared; p0 += 64) {
@@ -187,11 +189,41 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
                     while (x < sub && w[p + x] == q[i0 + x]) ++x;
                     hit = x == sub;
                 }
-                shared = __ballot(hit) != 0;
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    shared = found = true;
+                    hit_i = i0;
+                    hit_p = p0 + (uint32_t)__builtin_ctzll(hm);
+                }
             }
         }
+        // lower bound of the SSW score: best ungapped segment on the diagonal of
+        // that substring, as for seed jobs
+        uint32_t jflags = 0;
+        if (found) {
+            const int64_t d = (int64_t)hit_p - (int64_t)hit_i;  // window offset of query position 0
+            const int64_t x0 = d < 0 ? -d : 0;
+            const int64_t x1 = (int64_t)L < (int64_t)(we - ws) - d ? (int64_t)L : (int64_t)(we - ws) - d;
+            const uint32_t n = x1 > x0 ? (uint32_t)(x1 - x0) : 0;
+            const RefWin wd{&ref, r.ref_id, (uint64_t)((int64_t)ws + d + x0), n};
+            const uint32_t per = (n + 63) / 64;
+            const uint32_t j0 = min(n, lane * per), j1 = min(n, j0 + per);
+            SegSum s;
+            for (uint32_t j = j0; j < j1; ++j)
+                s.push(ssw_code(q[(uint32_t)x0 + j]) == (int32_t)wd.code(j) ? sc.match : -sc.mismatch);
+            SA_UNROLL
+            for (int dd = 1; dd < 64; dd <<= 1) {
+                SegSum b;
+                b.tot = __shfl_down(s.tot, dd);
+                b.pre = __shfl_down(s.pre, dd);
+                b.suf = __shfl_down(s.suf, dd);
+                b.best = __shfl_down(s.best, dd);
+                s.join(b);
+            }
+            jflags = s.best >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;  // lane 0 holds the whole diagonal
+        }
         if (lane == 0) {
-            if (shared) push_ssw(ws, we, 0u);
+            if (shared) push_ssw(ws, we, jflags);
             res[t] = out;
         }
     }
