@@ -117,21 +117,51 @@ struct SegSum {
     }
 };
 
-// Substitution score of query position x against window position x + off under
-// the SSW passes' match/mismatch scores (reference codes are 0-3; a query
-// wildcard, code 4, scores a mismatch).
-template <typename Q>
-SA_HD int32_t diagonal_cell(const Q& q, const RefWin& w, uint32_t x, const Scores& sc) {
-    return ssw_code(q[x]) == (int32_t)w.code(x) ? sc.match : -sc.mismatch;
-}
+// Gap lengths up to which two_diagonal_score joins its two diagonals.
+constexpr int64_t TWO_DIAGONAL_MAX_GAP = 4096;
 
-// Best ungapped segment of the query on one diagonal of the window (Kadane): a
-// lower bound of the local alignment score the SSW passes find.
+// Best score of an alignment of the query against the window [0, w.len) made of
+// a segment on diagonal d1 (query position x against window position x + d1),
+// one gap and a segment on diagonal d2 further down the query, or of a segment
+// on either diagonal alone: a lower bound of the local score the SSW passes find
+// (a gap of g positions costs gap_open + (g - 1) * gap_extend there; a cell off
+// the window ends every segment through it). One pass: the d1 scan trails the
+// d2 scan by the query offset at which the segment after the gap starts
+// (1 when d2 > d1, a reference gap; 1 + g when d2 < d1, a query gap).
 template <typename Q>
-SA_HD int32_t diagonal_segment_score(const Q& q, const RefWin& w, uint32_t n, const Scores& sc) {
-    SegSum s;
-    for (uint32_t x = 0; x < n; ++x) s.push(diagonal_cell(q, w, x, sc));
-    return s.best;
+SA_HD int32_t two_diagonal_score(const Q& q, const RefWin& w, uint32_t L, int64_t d1, int64_t d2,
+                                 const Scores& sc) {
+    constexpr int32_t NEG = -(1 << 28);
+    const int64_t g = d1 > d2 ? d1 - d2 : d2 - d1;
+    const bool join = g > 0 && g <= TWO_DIAGONAL_MAX_GAP;
+    const uint32_t off = !join ? 0u : d2 > d1 ? 1u : 1u + (uint32_t)g;
+    const int32_t cost = join ? sc.gap_open + (int32_t)(g - 1) * sc.gap_extend : 0;
+    const RefWin w1 = w, w2 = w;  // one cached reference word per diagonal
+    auto cell = [&](const RefWin& wd, uint32_t x, int64_t d, bool& ok) -> int32_t {
+        const int64_t p = (int64_t)x + d;
+        ok = p >= 0 && p < (int64_t)wd.len;
+        return !ok ? 0 : ssw_code(q[x]) == (int32_t)wd.code((uint32_t)p) ? sc.match : -sc.mismatch;
+    };
+    int32_t best = 0, a = NEG, s2 = NEG;
+    const uint32_t zend = L + off;
+    for (uint32_t z = 0; z < zend; ++z) {
+        int32_t ax = NEG;  // best segment on d1 ending at z - off
+        bool ok;
+        if (z >= off && z - off < L) {
+            const int32_t c = cell(w1, z - off, d1, ok);
+            a = ok ? (a > 0 ? a : 0) + c : NEG;
+            best = best > a ? best : a;
+            ax = a;
+        }
+        if (z < L) {
+            const int32_t c = cell(w2, z, d2, ok);
+            int32_t from = s2 > 0 ? s2 : 0;
+            if (join && ax > NEG / 2 && ax - cost > from) from = ax - cost;
+            s2 = ok ? from + c : NEG;
+            best = best > s2 ? best : s2;
+        }
+    }
+    return best;
 }
 
 // Substitution scores of the SSW passes, called as score(reference code, query

@@ -189,11 +189,15 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
         if (use_ssw) {
             uint64_t ws = prs > padding ? prs - padding : 0;
             uint64_t we = pre + padding < rlen ? pre + padding : rlen;
-            // the NAM's diagonal (query position 0 at window position prs - ws)
-            const uint32_t o = (uint32_t)(prs - ws), wl = (uint32_t)(we - ws);
-            const uint32_t n = wl > o ? (L < wl - o ? L : wl - o) : 0;
-            const int32_t lb = diagonal_segment_score(q, RefWin{&ref, r.ref_id, prs, n}, n, sc);
-            push_ssw(ws, we, lb >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u);
+            // the NAM's start and end diagonals, in window coordinates
+            uint32_t jflags = 0;
+            if (L <= SSW_GROUP_MAX_QUERY && (uint64_t)sc.match * L <= 30000 && sc.match <= 127 && sc.mismatch <= 127) {
+                const int64_t d1 = (int64_t)prs - (int64_t)ws;
+                const int64_t d2 = (int64_t)r.ref_end - (int64_t)r.query_end - (int64_t)ws;
+                const RefWin w{&ref, r.ref_id, ws, (uint32_t)(we - ws)};
+                jflags = two_diagonal_score(q, w, L, d1, d2, sc) >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;
+            }
+            push_ssw(ws, we, jflags);
         } else {
             uint64_t ds = r.ref_start > (uint64_t)L + padding ? r.ref_start - ((uint64_t)L + padding) : 0;
             uint64_t de = (uint64_t)r.ref_end + L + padding;

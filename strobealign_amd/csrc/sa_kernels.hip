@@ -62,6 +62,96 @@ struct ZeroBits {
     __device__ uint8_t operator[](uint32_t) const { return 0; }
 };
 
+__device__ inline SegSum shfl_seg(const SegSum& s, int d, bool up) {
+    SegSum b;
+    b.tot = up ? __shfl_up(s.tot, d) : __shfl_down(s.tot, d);
+    b.pre = up ? __shfl_up(s.pre, d) : __shfl_down(s.pre, d);
+    b.suf = up ? __shfl_up(s.suf, d) : __shfl_down(s.suf, d);
+    b.best = up ? __shfl_up(s.best, d) : __shfl_down(s.best, d);
+    return b;
+}
+
+// two_diagonal_score for one wave (every lane returns the score), queries of at
+// most 512 bases whose cell scores sum to at most 30000 in magnitude. Query
+// positions are dealt to lanes in contiguous slices of at most 8; max-sum-segment
+// summaries scanned across the wave give each lane the best segment ending
+// before its slice on d1 and the best prefix after it on d2, and the segment
+// after a gap is read from the lane that holds its first position.
+template <typename Q>
+__device__ int32_t two_diagonal_score_wave(const Q& q, const RefWin& w, uint32_t L, int64_t d1, int64_t d2,
+                                           const Scores& sc, uint32_t lane) {
+    constexpr int32_t OUT = -(1 << 20);  // a cell off the window: below any sum of the other cells
+    const uint32_t per = (L + 63) / 64;
+    const uint32_t x0 = min(L, lane * per), n = min(L, x0 + per) - x0;
+    const RefWin w1 = w, w2 = w;
+    int32_t c1[8], c2[8];
+    SegSum s1, s2;
+    SA_UNROLL
+    for (uint32_t k = 0; k < 8; ++k) {
+        c1[k] = c2[k] = OUT;
+        if (k < n) {
+            const int32_t qc = ssw_code(q[x0 + k]);
+            const int64_t p1 = (int64_t)(x0 + k) + d1, p2 = (int64_t)(x0 + k) + d2;
+            if (p1 >= 0 && p1 < (int64_t)w.len) c1[k] = qc == (int32_t)w1.code((uint32_t)p1) ? sc.match : -sc.mismatch;
+            if (p2 >= 0 && p2 < (int64_t)w.len) c2[k] = qc == (int32_t)w2.code((uint32_t)p2) ? sc.match : -sc.mismatch;
+            s1.push(c1[k]);
+            s2.push(c2[k]);
+        }
+    }
+    // inclusive scans: d1 from lane 0 up, d2 from lane 63 down
+    SA_UNROLL
+    for (int d = 1; d < 64; d <<= 1) {
+        SegSum l1 = shfl_seg(s1, d, true);
+        const SegSum r2 = shfl_seg(s2, d, false);
+        if (lane >= (uint32_t)d) {
+            l1.join(s1);
+            s1 = l1;
+        }
+        if (lane + d < 64) s2.join(r2);
+    }
+    const SegSum e1 = shfl_seg(s1, 1, true), e2 = shfl_seg(s2, 1, false);
+    int32_t suf = lane >= 1 ? e1.suf : 0, pre = lane + 1 < 64 ? e2.pre : 0;
+    int32_t best = 0, a[8], b[8];
+    SA_UNROLL
+    for (uint32_t k = 0; k < 8; ++k) {  // a[k]: best segment on d1 ending at x0 + k
+        a[k] = OUT;
+        if (k < n) {
+            a[k] = suf + c1[k];
+            suf = max(a[k], 0);
+            best = max(best, a[k]);
+        }
+    }
+    SA_UNROLL
+    for (int k = 7; k >= 0; --k) {  // b[k]: best segment on d2 starting at x0 + k
+        b[k] = OUT;
+        if ((uint32_t)k < n) {
+            b[k] = c2[k] + pre;
+            pre = max(b[k], 0);
+            best = max(best, b[k]);
+        }
+    }
+    const int64_t g = d1 > d2 ? d1 - d2 : d2 - d1;
+    if (g > 0 && g <= TWO_DIAGONAL_MAX_GAP) {
+        const uint32_t off = d2 > d1 ? 1u : 1u + (uint32_t)g;
+        const int32_t cost = sc.gap_open + (int32_t)(g - 1) * sc.gap_extend;
+        const uint32_t q0 = off / per, r0 = off % per;
+        SA_UNROLL
+        for (uint32_t k = 0; k < 8; ++k) {
+            if (k >= per) break;
+            // position x0 + k + off: index (r0 + k) mod per of lane lane + q0 + (r0 + k) / per
+            const uint32_t i = r0 + k, carry = i >= per ? 1u : 0u, idx = i - carry * per;
+            int32_t v = b[0];
+            SA_UNROLL
+            for (uint32_t j = 1; j < 8; ++j) v = j == idx ? b[j] : v;
+            v = __shfl(v, (int)((lane + q0 + carry) & 63u));
+            if (k < n && x0 + k + off < L) best = max(best, a[k] + v - cost);
+        }
+    }
+    SA_UNROLL
+    for (int d = 32; d >= 1; d >>= 1) best = max(best, __shfl_xor(best, d));
+    return best;
+}
+
 // First stage of extension, one wave per request (item_ext_prep is the same
 // logic in one thread): the ungapped case of extend_seed (src/mapper.rs:567-588)
 // compares the read with the reference in 64-base ballots, and the rescue
@@ -138,27 +228,14 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
         const uint64_t ws = prs > padding ? prs - padding : 0;
         const uint64_t we = pre + padding < rlen ? pre + padding : rlen;
         uint32_t jflags = 0;
-        if (use_ssw) {
-            // lower bound of the SSW score: best ungapped segment on the NAM's
-            // diagonal (query position 0 at window position prs - ws), one
-            // contiguous slice of positions per lane, joined across the wave
-            const uint32_t o = (uint32_t)(prs - ws), wl = (uint32_t)(we - ws);
-            const uint32_t n = wl > o ? min(L, wl - o) : 0;
-            const RefWin w{&ref, r.ref_id, prs, n};
-            const uint32_t per = (n + 63) / 64;
-            const uint32_t x0 = min(n, lane * per), x1 = min(n, x0 + per);
-            SegSum s;
-            for (uint32_t x = x0; x < x1; ++x) s.push(diagonal_cell(q, w, x, sc));
-            SA_UNROLL
-            for (int d = 1; d < 64; d <<= 1) {
-                SegSum b;
-                b.tot = __shfl_down(s.tot, d);
-                b.pre = __shfl_down(s.pre, d);
-                b.suf = __shfl_down(s.suf, d);
-                b.best = __shfl_down(s.best, d);
-                s.join(b);
-            }
-            jflags = s.best >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;  // lane 0 holds the whole diagonal
+        if (use_ssw && L <= SSW_GROUP_MAX_QUERY && (uint64_t)sc.match * L <= 30000 && sc.match <= 127 &&
+            sc.mismatch <= 127) {
+            // lower bound of the SSW score from the NAM's start and end
+            // diagonals, in window coordinates (only lane-group jobs use it)
+            const int64_t d1 = (int64_t)prs - (int64_t)ws;
+            const int64_t d2 = (int64_t)r.ref_end - (int64_t)r.query_end - (int64_t)ws;
+            const RefWin w{&ref, r.ref_id, ws, (uint32_t)(we - ws)};
+            jflags = two_diagonal_score_wave(q, w, L, d1, d2, sc, lane) >= 255 - sc.mismatch ? SSW_WORD_FIRST : 0u;
         }
         if (lane == 0) {
             if (use_ssw) {
