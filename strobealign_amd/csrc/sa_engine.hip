@@ -768,17 +768,36 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     if (nssw) {
         // launches every non-empty bucket of one (mode, pass) family
         auto run = [&](bool word, bool rev) {
-            for (uint32_t seg = 1; seg <= (word ? SSW_WORD_SEGS : SSW_BYTE_SEGS); ++seg) {
-                // one launch per stripe count: its window classes, longest first
+            const uint32_t nseg = word ? SSW_WORD_SEGS : SSW_BYTE_SEGS;
+            uint32_t below = nseg;  // reverse passes: stripe counts below this one are still to run
+            for (uint32_t step = 1; step <= nseg; ++step) {
                 SswLaunchLists ll{};
-                uint32_t at = 0;
-                for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
-                    const uint32_t wc = SSW_WCLASSES - 1 - x;
-                    const bool used = !rev || wc == 0;
-                    const uint32_t b = ssw_bucket(word, rev, seg, used ? wc : 0);
-                    ll.list[x] = d_lists + (uint64_t)b * cap;
-                    ll.start[x] = at;
-                    at += used ? bc[b] : 0;
+                uint32_t at = 0, seg = step;
+                if (!rev) {
+                    // one launch per stripe count: its window classes, longest first
+                    for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+                        const uint32_t b = ssw_bucket(word, false, seg, SSW_WCLASSES - 1 - x);
+                        ll.list[x] = d_lists + (uint64_t)b * cap;
+                        ll.start[x] = at;
+                        at += bc[b];
+                    }
+                } else {
+                    // reverse jobs have no window classes: the jobs of the largest
+                    // stripe count left and of up to SSW_WCLASSES - 1 smaller ones
+                    // share a launch of that stripe count (a pass with more stripes
+                    // than a query needs finds the same ends), so that the few
+                    // long rescue jobs of small counts do not run one launch each
+                    while (below >= 1 && bc[ssw_bucket(word, true, below, 0)] == 0) --below;
+                    if (below == 0) break;
+                    seg = below;
+                    for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+                        const bool used = below >= 1;
+                        const uint32_t b = ssw_bucket(word, true, used ? below : seg, 0);
+                        ll.list[x] = d_lists + (uint64_t)b * cap;
+                        ll.start[x] = at;
+                        at += used ? bc[b] : 0;
+                        if (used) --below;
+                    }
                 }
                 ll.start[SSW_WCLASSES] = at;
                 if (!at) continue;
