@@ -106,6 +106,11 @@ typedef struct sa_map_options {
     int32_t threads;         /* host threads for the decision replay and output */
     uint64_t batch_bases;    /* bases per device batch (0: default) */
     int32_t slots;           /* batches in flight on the device at once (0: default 4) */
+    /* Alignment scores (src/aligner.rs:11-30; -A/-B/-O/-E/-L and --xdrop of
+     * src/main.rs:246-272): match score, mismatch / gap-open / gap-extend
+     * penalties (0..127 each), soft-clipping end bonus, piecewise x-drop. */
+    int32_t match_score, mismatch_penalty, gap_open_penalty, gap_extend_penalty;
+    int32_t end_bonus, xdrop;
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */
@@ -185,12 +190,15 @@ int sa_server_submit_reads(sa_server* server, const sa_reads* reads, uint64_t fi
                            uint64_t* ticket);
 /* Waits for a request; *out (sa_free) receives its output unless out is NULL,
  * *latency_ms the time from the start of its submit call to its completion (all
- * of its output formatted). A ticket is waited once. */
+ * of its output formatted). A ticket is waited once: a second wait on it, from
+ * any thread, returns SA_ERR_ARG. */
 int sa_server_wait(sa_server* server, uint64_t ticket, char** out, size_t* out_len, double* latency_ms);
 int sa_server_get_stats(const sa_server* server, sa_server_stats* st);
 /* As sa_index_kernel_times, over the server's engines. */
 size_t sa_server_kernel_times(const sa_server* server, char* buf, size_t cap);
-/* Stops the workers (requests not yet waited for are dropped) and frees the arena. */
+/* Stops the workers and frees the arena. Requests not yet completed fail: a
+ * thread blocked in sa_server_wait returns an error before this returns, and
+ * requests nobody waits for are dropped. */
 void sa_server_destroy(sa_server* server);
 
 /* Per-stage probes (parity tests, and a finer-grained drop-in for a caller that

@@ -19,7 +19,9 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: sa_oracle [-t N] [-x|--aemb] [-r N] [-N N] [-U] [--eqx] [--ssw] [-C] [--details]\n"
                  "                 [--rg-id ID] [--rg TAG] [--chunk-size N] [--mcs MODE] [-o OUT]\n"
-                 "                 [--dump-index PATH] [--time] ref.fa reads1 [reads2]\n");
+                 "                 [-A N] [-B N] [-O N] [-E N] [-L N] [--xdrop N]\n"
+                 "                 [--dump-index PATH] [--paf-out PATH] [--time] ref.fa reads1 [reads2]\n"
+                 "  --paf-out PATH: also map in -x mode with the same index and write the PAF to PATH\n");
 }
 
 int main(int argc, char** argv) {
@@ -29,8 +31,10 @@ int main(int argc, char** argv) {
     size_t t1_chunks = 0;  // with --time: also map the first N chunks again on one thread
     MappingParams mp;
     SamOpts so;
+    Scores sc;  // src/main.rs:246-272 (-A/-B/-O/-E/-L, --xdrop)
+    int xdrop = 500;
     std::vector<std::string> rg_fields;
-    std::string out_path, dump_index, dump_nams;
+    std::string out_path, dump_index, dump_nams, paf_out;
     std::vector<std::string> pos;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -60,9 +64,16 @@ int main(int argc, char** argv) {
         else if (is("--mcs")) {
             std::string m = val("--mcs");
             mp.mcs = m == "rescue" ? Mcs::Rescue : m == "off" ? Mcs::Off : m == "first-strobe" ? Mcs::FirstStrobe : Mcs::Always;
-        } else if (a == "-o") out_path = val("-o");
+        } else if (a == "-A") sc.match = std::stoi(val("-A"));
+        else if (a == "-B") sc.mismatch = std::stoi(val("-B"));
+        else if (a == "-O") sc.gap_open = std::stoi(val("-O"));
+        else if (a == "-E") sc.gap_extend = std::stoi(val("-E"));
+        else if (a == "-L") sc.end_bonus = std::stoi(val("-L"));
+        else if (is("--xdrop")) xdrop = std::stoi(val("--xdrop"));
+        else if (a == "-o") out_path = val("-o");
         else if (is("--dump-index")) dump_index = val("--dump-index");
         else if (is("--dump-nams")) dump_nams = val("--dump-nams");
+        else if (is("--paf-out")) paf_out = val("--paf-out");
         else if (!a.empty() && a[0] == '-' && a.size() > 1) { usage(); return 2; }
         else pos.push_back(a);
     }
@@ -114,6 +125,8 @@ int main(int argc, char** argv) {
     m.refs = &refs;
     m.mp = mp;
     m.so = so;
+    m.sc = sc;
+    m.xdrop = xdrop;
     m.mode = mode;
     m.abundances.assign(refs.size(), 0.0);
 
@@ -181,6 +194,28 @@ int main(int argc, char** argv) {
         for (size_t i = 0; i < refs.size(); ++i) std::fprintf(out, "%s\t%.6f\n", refs[i].name.c_str(), ab[i] / (double)refs[i].seq.len);
     }
     if (out != stdout) std::fclose(out);
+    if (!paf_out.empty()) {
+        // the same chunks once more in mapping-only mode (fresh per-chunk state)
+        Mapper mp_paf = m;
+        mp_paf.mode = 1;
+        std::vector<std::string> pafs(chunks.size());
+        std::atomic<size_t> nxt{0};
+        std::vector<std::thread> pth;
+        for (size_t t = 0; t < threads; ++t)
+            pth.emplace_back([&] {
+                Mapper mm = mp_paf;
+                for (;;) {
+                    size_t c = nxt.fetch_add(1);
+                    if (c >= chunks.size()) break;
+                    mm.map_chunk(chunks[c], pafs[c]);
+                }
+            });
+        for (auto& t : pth) t.join();
+        FILE* pf = std::fopen(paf_out.c_str(), "wb");
+        if (!pf) { std::fprintf(stderr, "error: cannot open %s\n", paf_out.c_str()); return 1; }
+        for (const std::string& x : pafs) std::fwrite(x.data(), 1, x.size(), pf);
+        std::fclose(pf);
+    }
     if (time_it) {
         double ti = std::chrono::duration<double>(t1 - t0).count();
         double tm = std::chrono::duration<double>(t3 - t2).count();

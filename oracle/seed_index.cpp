@@ -2,6 +2,8 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"
 
+#include <parallel/algorithm>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -392,6 +394,11 @@ Index make_index(const std::vector<RefSeq>& refs, const SeedingParams& p, int bi
     idx.params = p;
     idx.bits = bits;
     std::vector<RefRandstrobe>& rs = idx.randstrobes;
+    // contigs are independent (src/indexer.rs:65-69 generates them in parallel
+    // too); the sort key (hash_offset, position, ref_index) is unique per entry,
+    // so the sorted table does not depend on the order entries were produced in
+    std::vector<std::vector<RefRandstrobe>> per(refs.size());
+#pragma omp parallel for schedule(dynamic, 1)
     for (size_t ri = 0; ri < refs.size(); ++ri) {
         const PackedSeq& seq = refs[ri].seq;
         if (seq.len < (size_t)p.randstrobe.w_max) continue;
@@ -400,10 +407,17 @@ Index make_index(const std::vector<RefSeq>& refs, const SeedingParams& p, int bi
         std::vector<Syncmer> sy = syncmers_codes(codes.data(), seq.len, p.syncmer);
         for (const Randstrobe& r : randstrobes_from_syncmers(sy, p.randstrobe)) {
             uint64_t off = r.strobe2_pos - r.strobe1_pos;
-            rs.push_back({(r.hash & 0xFFFFFFFFFFFFFF00ULL) | (off & 0xFF), (uint32_t)r.strobe1_pos, (uint32_t)ri});
+            per[ri].push_back({(r.hash & 0xFFFFFFFFFFFFFF00ULL) | (off & 0xFF), (uint32_t)r.strobe1_pos, (uint32_t)ri});
         }
     }
-    std::sort(rs.begin(), rs.end(), [](const RefRandstrobe& a, const RefRandstrobe& b) {
+    size_t total = 0;
+    for (const auto& v : per) total += v.size();
+    rs.reserve(total);
+    for (auto& v : per) {
+        rs.insert(rs.end(), v.begin(), v.end());
+        std::vector<RefRandstrobe>().swap(v);
+    }
+    __gnu_parallel::sort(rs.begin(), rs.end(), [](const RefRandstrobe& a, const RefRandstrobe& b) {
         if (a.hash_offset != b.hash_offset) return a.hash_offset < b.hash_offset;
         if (a.position != b.position) return a.position < b.position;
         return a.ref_index < b.ref_index;

@@ -37,7 +37,9 @@ class MapOptions(ctypes.Structure):
                 ("max_secondary", ctypes.c_uint32), ("dropoff", ctypes.c_float),
                 ("rescue_distance", ctypes.c_uint32), ("max_tries", ctypes.c_uint32), ("mcs", ctypes.c_uint32),
                 ("chunk_size", ctypes.c_uint64), ("threads", ctypes.c_int32), ("batch_bases", ctypes.c_uint64),
-                ("slots", ctypes.c_int32)]
+                ("slots", ctypes.c_int32), ("match_score", ctypes.c_int32), ("mismatch_penalty", ctypes.c_int32),
+                ("gap_open_penalty", ctypes.c_int32), ("gap_extend_penalty", ctypes.c_int32),
+                ("end_bonus", ctypes.c_int32), ("xdrop", ctypes.c_int32)]
 
 
 class QRand(ctypes.Structure):
@@ -95,9 +97,16 @@ class Reads:
 
 
 def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000, use_ssw=False, max_secondary=0,
-            mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0, slots=0):
+            mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0, slots=0, scores=None,
+            xdrop=None):
+    """Mapping options; scores = (match, mismatch, gap_open, gap_extend, end_bonus)
+    as -A/-B/-O/-E/-L (defaults 2, 8, 12, 1, 10)."""
     o = MapOptions()
     _lib.sa_map_options_default(ctypes.byref(o))
+    if scores is not None:
+        o.match_score, o.mismatch_penalty, o.gap_open_penalty, o.gap_extend_penalty, o.end_bonus = scores
+    if xdrop is not None:
+        o.xdrop = xdrop
     o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
     o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
     o.batch_bases, o.slots = batch_bases, slots
@@ -224,16 +233,10 @@ class Index:
 
     def map_files(self, reads1, reads2=None, mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
                   use_ssw=False, max_secondary=0, mcs=0, details=False, fastq_comments=False, only_mapped=False,
-                  batch_bases=0):
-        o = MapOptions()
-        _lib.sa_map_options_default(ctypes.byref(o))
-        o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
-        o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
-        o.batch_bases = batch_bases
-        if rg_id is not None:
-            o.rg_id = rg_id.encode()
-        rgs = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])
-        o.rg, o.n_rg = rgs, len(rg)
+                  batch_bases=0, scores=None, xdrop=None):
+        o = options(mode=mode, eqx=eqx, rg_id=rg_id, rg=rg, threads=threads, chunk_size=chunk_size, use_ssw=use_ssw,
+                    max_secondary=max_secondary, mcs=mcs, details=details, fastq_comments=fastq_comments,
+                    only_mapped=only_mapped, batch_bases=batch_bases, scores=scores, xdrop=xdrop)
         out = ctypes.c_void_p()
         n = ctypes.c_size_t()
         _check(_lib.sa_map_files(self._p, ctypes.byref(o), reads1.encode(),

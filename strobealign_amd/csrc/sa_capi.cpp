@@ -55,6 +55,8 @@ int guard(F f) {
         return f();
     } catch (const std::bad_alloc&) {
         return fail(SA_ERR_INTERNAL, "out of memory");
+    } catch (const std::invalid_argument& e) {
+        return fail(SA_ERR_ARG, e.what());
     } catch (const std::exception& e) {
         std::string m = e.what();
         return fail(m.rfind("HIP error", 0) == 0 ? SA_ERR_DEVICE : SA_ERR_INTERNAL, m);
@@ -89,6 +91,13 @@ sa::Options options_of(const sa_map_options* o) {
     r.threads = o->threads > 0 ? o->threads : 1;
     if (o->batch_bases) r.batch_bases = o->batch_bases;
     if (o->slots > 0) r.slots = o->slots;
+    r.scores = sa::Scores{o->match_score, o->mismatch_penalty, o->gap_open_penalty, o->gap_extend_penalty,
+                          o->end_bonus, o->xdrop};
+    // the reference takes u8 scores and penalties (src/aligner.rs:11-19) and
+    // negates them into i8 matrix entries (src/ssw.rs:143-144)
+    for (int32_t v : {o->match_score, o->mismatch_penalty, o->gap_open_penalty, o->gap_extend_penalty})
+        if (v < 0 || v > 127) throw std::invalid_argument("scores and penalties must be in 0..127");
+    if (o->end_bonus < 0) throw std::invalid_argument("end_bonus must be >= 0");
     return r;
 }
 
@@ -293,6 +302,12 @@ void sa_map_options_default(sa_map_options* o) {
     o->max_tries = 20;
     o->chunk_size = 1000000;
     o->threads = 1;
+    o->match_score = 2;
+    o->mismatch_penalty = 8;
+    o->gap_open_penalty = 12;
+    o->gap_extend_penalty = 1;
+    o->end_bonus = 10;
+    o->xdrop = 500;
 }
 
 int sa_map_files(sa_index* index, const sa_map_options* opt, const char* reads1, const char* reads2, char** out,
@@ -522,7 +537,8 @@ int sa_server_wait(sa_server* server, uint64_t ticket, char** out, size_t* out_l
     return guard([&] {
         std::string s, err;
         sa::Server::RequestTimes t;
-        if (!server->srv->wait(ticket, s, t, err)) return fail(SA_ERR_INTERNAL, err);
+        const sa::Server::WaitStatus w = server->srv->wait(ticket, s, t, err);
+        if (w != sa::Server::WAIT_OK) return fail(w == sa::Server::WAIT_BAD_TICKET ? SA_ERR_ARG : SA_ERR_INTERNAL, err);
         if (latency_ms) *latency_ms = t.done_ms;
         if (out) *out = dup_out(s, out_len);
         return SA_OK;

@@ -18,6 +18,7 @@ static void usage() {
     std::fprintf(stderr,
                  "usage: strobealign-hip [-t N] [-x|--aemb] [-r N] [-N N] [-U] [--eqx] [--ssw] [-C] [--details]\n"
                  "                       [--rg-id ID] [--rg TAG] [--chunk-size N] [--mcs MODE] [-o OUT]\n"
+                 "                       [-A N] [-B N] [-O N] [-E N] [-L N] [--xdrop N]\n"
                  "                       [--device D] [--batch-bases N] [--slots N] [--dump-nams PATH] [--time]\n"
                  "                       ref.fa reads1 [reads2]\n");
 }
@@ -64,7 +65,13 @@ int main(int argc, char** argv) {
         else if (is("--mcs")) {
             std::string m = val("--mcs");
             o.mcs = m == "rescue" ? 1 : m == "off" ? 2 : m == "first-strobe" ? 3 : 0;
-        } else if (a == "-o") out_path = val("-o");
+        } else if (a == "-A") o.scores.match = std::stoi(val("-A"));
+        else if (a == "-B") o.scores.mismatch = std::stoi(val("-B"));
+        else if (a == "-O") o.scores.gap_open = std::stoi(val("-O"));
+        else if (a == "-E") o.scores.gap_extend = std::stoi(val("-E"));
+        else if (a == "-L") o.scores.end_bonus = std::stoi(val("-L"));
+        else if (is("--xdrop")) o.scores.xdrop = std::stoi(val("--xdrop"));
+        else if (a == "-o") out_path = val("-o");
         else if (!a.empty() && a[0] == '-' && a.size() > 1) { usage(); return 2; }
         else pos.push_back(a);
     }

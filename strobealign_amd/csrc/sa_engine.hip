@@ -784,14 +784,18 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 } else {
                     // reverse jobs have no window classes: the jobs of the largest
                     // stripe count left and of up to SSW_WCLASSES - 1 smaller ones
-                    // share a launch of that stripe count (a pass with more stripes
-                    // than a query needs finds the same ends), so that the few
-                    // long rescue jobs of small counts do not run one launch each
+                    // share a launch of that stripe count, so that the few long
+                    // rescue jobs of small counts do not run one launch each. A
+                    // pass with more stripes than a query needs finds the same
+                    // ends only while gap_open > gap_extend (with O <= E the lazy-F
+                    // loop's early exit depends on the striping), so other scores
+                    // run one launch per stripe count.
                     while (below >= 1 && bc[ssw_bucket(word, true, below, 0)] == 0) --below;
                     if (below == 0) break;
                     seg = below;
+                    const uint32_t merge = sc.gap_open > sc.gap_extend ? SSW_WCLASSES : 1;
                     for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
-                        const bool used = below >= 1;
+                        const bool used = below >= 1 && x < merge;
                         const uint32_t b = ssw_bucket(word, true, used ? below : seg, 0);
                         ll.list[x] = d_lists + (uint64_t)b * cap;
                         ll.start[x] = at;

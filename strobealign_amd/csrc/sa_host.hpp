@@ -81,6 +81,8 @@ struct Options {
     int threads = 1;
     uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
     int slots = 4;                        // engines working on batches concurrently
+    // -A/-B/-O/-E/-L and --xdrop (src/main.rs:246-272; defaults src/aligner.rs:20-30)
+    Scores scores{2, 8, 12, 1, 10, 500};
 };
 
 // src/io/sam.rs:177-201 (no @PG line)
@@ -134,8 +136,11 @@ public:
     uint64_t submit(std::vector<Record> r1, std::vector<Record> r2, double t_arrival = -1);
     // Times since submission, milliseconds.
     struct RequestTimes { double admit_ms = 0, done_ms = 0; };
-    // Blocks until the request is done; false (and `err`) if it failed.
-    bool wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err);
+    // Blocks until the request is done. Returns WAIT_OK, WAIT_BAD_TICKET (unknown,
+    // or already held by another wait) or WAIT_FAILED (the request failed or the
+    // server stopped first), with the reason in `err`.
+    enum WaitStatus { WAIT_OK = 0, WAIT_BAD_TICKET = 1, WAIT_FAILED = 2 };
+    WaitStatus wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err);
     struct Stats {
         uint64_t submitted = 0, completed = 0, batches = 0, batch_requests = 0, max_requests_per_batch = 0;
         uint64_t arena_bytes = 0, arena_peak = 0;

@@ -191,7 +191,8 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
             uint64_t we = pre + padding < rlen ? pre + padding : rlen;
             // the NAM's start and end diagonals, in window coordinates
             uint32_t jflags = 0;
-            if (L <= SSW_GROUP_MAX_QUERY && (uint64_t)sc.match * L <= 30000 && sc.match <= 127 && sc.mismatch <= 127) {
+            if (L <= SSW_GROUP_MAX_QUERY && (uint64_t)sc.match * L <= 30000 && sc.match <= 127 && sc.mismatch <= 127 &&
+                sc.gap_open > sc.gap_extend) {
                 const int64_t d1 = (int64_t)prs - (int64_t)ws;
                 const int64_t d2 = (int64_t)r.ref_end - (int64_t)r.query_end - (int64_t)ws;
                 const RefWin w{&ref, r.ref_id, ws, (uint32_t)(we - ws)};

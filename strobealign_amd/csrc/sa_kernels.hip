@@ -229,7 +229,8 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
         const uint64_t we = pre + padding < rlen ? pre + padding : rlen;
         uint32_t jflags = 0;
         if (use_ssw && L <= SSW_GROUP_MAX_QUERY && (uint64_t)sc.match * L <= 30000 && sc.match <= 127 &&
-            sc.mismatch <= 127) {
+            sc.mismatch <= 127 &&
+            sc.gap_open > sc.gap_extend) {
             // lower bound of the SSW score from the NAM's start and end
             // diagonals, in window coordinates (only lane-group jobs use it)
             const int64_t d1 = (int64_t)prs - (int64_t)ws;

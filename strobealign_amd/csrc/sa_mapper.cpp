@@ -619,7 +619,7 @@ void align_single_end(Ctx& c, uint64_t rec_idx, std::vector<Sam>& out, std::dequ
         for (size_t i = 0; i < alignments.size() && i < c.o.max_secondary; ++i) {
             const Alignment& a = alignments[i];
             uint32_t diff = a.score > best_score ? a.score - best_score : 0;
-            if (diff > 2 * 8 + 12) break;  // saturating_sub never exceeds: kept for parity
+            if (diff > 2 * (uint32_t)c.o.scores.mismatch + (uint32_t)c.o.scores.gap_open) break;  // saturating_sub never exceeds
             store.push_back(a);
             out.push_back(make_mapped(store.back(), rec, mapq, true, &det));
         }
@@ -938,7 +938,7 @@ void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Al
         out.push_back(b);
         return;
     }
-    const double dropoff = 2.0 * 8 + 12;
+    const double dropoff = (double)(uint8_t)(2 * c.o.scores.mismatch + c.o.scores.gap_open);  // u8 sum, src/mapper.rs:746
     size_t limit = c.o.max_secondary == 0 ? 1 : c.o.max_secondary;
     bool secondary = false;
     double smax = v[0].score;
@@ -1297,7 +1297,6 @@ struct BatchReads {
     std::vector<uint64_t> seg;
 };
 
-const Scores kScores{2, 8, 12, 1, 10, 500};
 const ChainParams kChain{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
 
 // One device batch of whole chunks (of one or several read sets) on one engine:
@@ -1399,7 +1398,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         std::vector<uint64_t> coff;
         double g0 = now_ms();
         st.host_ms += g0 - h1;
-        eng.extend(req.data(), (uint32_t)nreq, kScores, res, cig, coff);
+        eng.extend(req.data(), (uint32_t)nreq, o.scores, res, cig, coff);
         double g1 = now_ms();
         st.extend_ms += g1 - g0;
         // results into the chunks' caches (each chunk owns its cache and CIGAR pool)
@@ -1557,6 +1556,7 @@ struct Server::Impl {
         uint64_t dev = 0, dev_size = 0;   // arena block
         double t_submit = 0, t_admit = 0, t_done = 0;
         bool done = false, failed = false;
+        bool claimed = false;  // a wait call holds the ticket
         std::string out, err;
     };
 
@@ -1575,6 +1575,7 @@ struct Server::Impl {
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
+    int waiters = 0;  // threads inside Server::wait
     Server::Stats st{};
     std::thread admitter;
     std::vector<std::thread> workers;
@@ -1763,6 +1764,19 @@ Server::~Server() {
     impl_->cv_work.notify_all();
     impl_->admitter.join();
     for (auto& t : impl_->workers) t.join();
+    {
+        // the workers are gone: every request still unfinished fails, and the
+        // threads blocked in wait return before the server's state goes away
+        std::unique_lock<std::mutex> lk(impl_->m);
+        for (auto& kv : impl_->reqs) {
+            Impl::Req* r = kv.second.get();
+            if (r->done) continue;
+            r->failed = r->done = true;
+            r->err = "server destroyed before the request completed";
+        }
+        impl_->cv_done.notify_all();
+        impl_->cv_done.wait(lk, [&] { return impl_->waiters == 0; });
+    }
     device_free(impl_->device, impl_->arena);
 }
 
@@ -1823,22 +1837,30 @@ uint64_t Server::submit(std::vector<Record> r1, std::vector<Record> r2, double t
     return p->id;
 }
 
-bool Server::wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err) {
+Server::WaitStatus Server::wait(uint64_t ticket, std::string& out, RequestTimes& t, std::string& err) {
     std::unique_lock<std::mutex> lk(impl_->m);
     auto it = impl_->reqs.find(ticket);
     if (it == impl_->reqs.end()) {
         err = "unknown ticket " + std::to_string(ticket);
-        return false;
+        return WAIT_BAD_TICKET;
     }
     Impl::Req* r = it->second.get();
+    if (r->claimed) {
+        err = "ticket " + std::to_string(ticket) + " is already being waited for";
+        return WAIT_BAD_TICKET;
+    }
+    r->claimed = true;
+    impl_->waiters++;
     impl_->cv_done.wait(lk, [&] { return r->done; });
     t.admit_ms = r->t_admit - r->t_submit;
     t.done_ms = r->t_done - r->t_submit;
     const bool ok = !r->failed;
     if (ok) out.swap(r->out);
     else err = r->err;
-    impl_->reqs.erase(it);
-    return ok;
+    impl_->reqs.erase(ticket);  // by key: submits during the wait may have rehashed the table
+    impl_->waiters--;
+    if (impl_->stop) impl_->cv_done.notify_all();  // the destructor waits for the last waiter
+    return ok ? WAIT_OK : WAIT_FAILED;
 }
 
 Server::Stats Server::stats() const {

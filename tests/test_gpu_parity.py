@@ -94,14 +94,17 @@ def test_randstrobes_match_oracle(phix, orc):
 
 @pytest.fixture(scope="module")
 def small(tmp_path_factory):
+    """Configuration 1 at its full size: seed 42, 1 Mbp, 10,000 SE 150 bp reads."""
     sys.path.insert(0, ROOT)
     from strobealign_amd import synth
     d = str(tmp_path_factory.mktemp("small"))
-    return synth.make("small", d, 3000)
+    return synth.make("small", d, 10_000)
 
 
 def test_small_single_end_vs_oracle(sa, ctx, small):
-    say("case: 1 Mbp, 3000 SE 150 bp reads with indels (piecewise path)")
+    """Configuration 1 (1 Mbp, 10,000 single-end 150 bp reads, 1% substitutions,
+    0.1% indels): SAM byte for byte, piecewise (default) and --ssw extension."""
+    say("case: 1 Mbp, 10000 SE 150 bp reads with indels (piecewise path)")
     exp = oracle("--no-PG", "--eqx", small["ref"], small["reads1"])
     with sa.Index(ctx, small["ref"], read_length=150) as ix:
         got = ix.map_files(small["reads1"], eqx=True)
