@@ -234,6 +234,26 @@ typedef struct sa_nam {
 int sa_find_nams_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, uint32_t mcs,
                        uint32_t rescue_distance, sa_nam** nams, uint64_t* nam_offsets);
 
+/* Size of the per-device scratch array behind sa_find_nams_batch_noalloc. */
+#define SA_NOALLOC_SCRATCH_BYTES (128ull << 20)
+
+/* sa_find_nams_batch for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc. It has no
+ * counterpart in the reference. Inputs and outputs are device memory the caller
+ * owns: d_seqs (ASCII bases) with n + 1 offsets d_offsets; NAMs go to d_nams
+ * (room for nam_cap records), read i's at [d_nam_offsets[i], d_nam_offsets[i+1])
+ * (n + 1 entries); *n_nams receives the total. Work runs on `stream` (a
+ * hipStream_t; NULL: the null stream), and the call returns once the outputs are
+ * written (it waits twice on the stream to size the stages). Every intermediate
+ * buffer comes from one static device array of SA_NOALLOC_SCRATCH_BYTES per
+ * device; no hipMalloc, hipFree, pinned-memory or other allocation is made. A
+ * batch that needs more scratch or more NAM room than given fails with SA_ERR_ARG
+ * (split it). Calls on one device share that array: calls that overlap in time on
+ * one device race and must be serialised by the caller. */
+int sa_find_nams_batch_noalloc(sa_index* index, const uint8_t* d_seqs, const uint64_t* d_offsets, uint32_t n,
+                               uint32_t mcs, uint32_t rescue_distance, sa_nam* d_nams, uint64_t nam_cap,
+                               uint64_t* d_nam_offsets, uint64_t* n_nams, void* stream);
+
 /* Aligner::align (src/aligner.rs:80-151: SSW of ext/ssw/ssw.c plus the end-bonus
  * extension) of n (query, reference) pairs. Queries/references are ASCII,
  * concatenated with offsets. Each result: found, score, query_start, query_end,

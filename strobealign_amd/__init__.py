@@ -62,7 +62,7 @@ class Alignment(ctypes.Structure):
 EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
            "sa_index_build", "sa_index_build_device", "sa_index_download", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
-           "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
+           "sa_find_nams_batch_noalloc", "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
            "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
            "sa_server_destroy", "sa_server_submit", "sa_server_submit_reads", "sa_server_wait",
            "sa_server_get_stats", "sa_server_kernel_times", "sa_autotune", "sa_free"]
@@ -138,6 +138,13 @@ def load(path=LIB_PATH):
     lib.sa_server_kernel_times.restype = ctypes.c_size_t
     _lib = lib
     return lib
+
+
+def last_error():
+    """Text of the library's last error on this thread."""
+    buf = ctypes.create_string_buffer(4096)
+    _lib.sa_last_error(buf, ctypes.c_size_t(4096))
+    return buf.value.decode(errors="replace")
 
 
 def _check(rc):
@@ -291,6 +298,18 @@ class Index:
                         [(rc[o + x].hash, rc[o + x].hash_revcomp, rc[o + x].start, rc[o + x].end)
                          for x in range(counts[2 * i + 1])]))
         return res
+
+    def nams_raw(self, seqs, mcs=0, rescue_distance=100):
+        """sa_find_nams_batch: (the NAM records as bytes, 48 per record; the n + 1 offsets)."""
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        p = ctypes.POINTER(Nam)()
+        noff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_find_nams_batch(self._p, data, off, ctypes.c_uint32(n), mcs, rescue_distance,
+                                       ctypes.byref(p), noff))
+        raw = ctypes.string_at(p, 48 * noff[n])
+        _lib.sa_free(p)
+        return raw, list(noff)
 
     def nams(self, seqs, mcs=0, rescue_distance=100):
         data, off = self._pack(seqs)

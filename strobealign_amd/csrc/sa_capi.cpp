@@ -598,6 +598,23 @@ int sa_find_nams_batch(sa_index* index, const uint8_t* seqs, const uint64_t* off
     });
 }
 
+int sa_find_nams_batch_noalloc(sa_index* index, const uint8_t* d_seqs, const uint64_t* d_offsets, uint32_t n,
+                               uint32_t mcs, uint32_t rescue_distance, sa_nam* d_nams, uint64_t nam_cap,
+                               uint64_t* d_nam_offsets, uint64_t* n_nams, void* stream) {
+    if (!index || (!d_seqs && n) || !d_offsets || (!d_nams && nam_cap) || !d_nam_offsets || !n_nams)
+        return fail(SA_ERR_ARG, "NULL argument");
+    if (mcs > 3) return fail(SA_ERR_ARG, "mcs must be 0..3");
+    static_assert(sa::NOALLOC_SCRATCH_BYTES == SA_NOALLOC_SCRATCH_BYTES, "scratch size");
+    return guard([&] {
+        uint8_t* scr = sa::noalloc_scratch(index->device);
+        sa::MapParams mp{mcs, rescue_distance};
+        sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+        index->eng->find_nams_noalloc(d_seqs, d_offsets, n, mp, cp, (sa::NamRec*)d_nams, nam_cap, d_nam_offsets,
+                                      n_nams, (ihipStream_t*)stream, scr, sa::NOALLOC_SCRATCH_BYTES);
+        return SA_OK;
+    });
+}
+
 int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
                        const uint64_t* ref_offsets, uint32_t n, sa_alignment* out, uint32_t** cigars,
                        uint64_t* cigar_offsets) {

@@ -20,7 +20,34 @@
                                      __FILE__ + ":" + std::to_string(__LINE__) + " (" #x ")");       \
     } while (0)
 
+// Scratch of the entry point that allocates nothing (sa_find_nams_batch_noalloc):
+// each device that loads this code object holds one copy until the process ends.
+// Host code never passes `scratch` itself to a kernel (in host code the name is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress returns
+// for the current device.
+static __device__ __align__(256) uint8_t scratch[sa::NOALLOC_SCRATCH_BYTES];
+
 namespace sa {
+
+namespace {
+// device address of `scratch` per device, looked up once per device
+std::mutex g_scratch_mu;
+constexpr int MAX_DEVICES = 64;
+uint8_t* g_scratch_ptr[MAX_DEVICES] = {};
+}  // namespace
+
+uint8_t* noalloc_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) throw std::invalid_argument("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        SA_CHECK(hipSetDevice(device));
+        void* ptr = nullptr;
+        SA_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw std::runtime_error("HIP error: no device address for the no-allocation scratch");
+        g_scratch_ptr[device] = (uint8_t*)ptr;
+    }
+    return g_scratch_ptr[device];
+}
 
 namespace {
 __global__ void k_pair_sum(const uint32_t* __restrict__ a, uint64_t* __restrict__ out, uint32_t n) {
@@ -531,6 +558,89 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
     sync();
 }
 
+namespace {
+// Device buffers and counts of one seeding -> NAM pass.
+struct NamPass {
+    uint32_t* qrn = nullptr;     // 2n: randstrobes per read orientation
+    HitCounts* hc = nullptr;     // 2n
+    uint32_t* namn = nullptr;    // 2n: NAMs per read orientation
+    uint32_t* araw = nullptr;    // 2n: anchors before dedup
+    uint32_t* cbytes = nullptr;  // 2n: chaining algorithmic bytes
+    uint32_t* hbytes = nullptr;  // 2n: lookup algorithmic bytes
+    uint64_t* outoff = nullptr;  // n: first NAM of each read
+    NamRec* nams = nullptr;      // n_nams
+    uint64_t n_nams = 0;
+};
+
+// Seeding -> lookups -> chaining -> NAMs for the n reads at d_seq with device
+// offsets d_off (n_bases bases), on stream st. take(slot, bytes) supplies every
+// device buffer; scan(in, out, n) writes an exclusive prefix sum and returns its
+// total (the two host waits of the pass: anchor capacity, NAM count); nams_out(N)
+// supplies the NAM output once N is known. ev (7 events, or null) brackets the
+// stages: [0,1] seed, [1,2] hits, [3,4] chain, [5,6] finalize.
+template <typename Take, typename Scan, typename NamsOut>
+void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, const float* same_diag,
+              const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, uint64_t n_bases, const MapParams& mp,
+              const ChainParams& cp, hipStream_t st, Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev,
+              NamPass& o) {
+    const SeedParams& p = ix.p;
+    const uint32_t k = (uint32_t)p.k;
+    // the syncmer list is only materialised by the fallback for w_max >= 24
+    auto* d_sy = (Syncmer*)take(S_SY, (p.w_max < 24 ? 1 : n_bases + 1) * sizeof(Syncmer));
+    auto* d_qr = (QRand*)take(S_QR, (2 * n_bases + 1) * sizeof(QRand));
+    o.qrn = (uint32_t*)take(S_QRN, (2ull * n + 1) * 4);
+    auto* d_hits = (Hit*)take(S_HITS, (2 * n_bases + 1) * sizeof(Hit));
+    auto* d_cnt = (uint64_t*)take(S_CNT, (2 * n_bases + 1) * 8);
+    o.hc = (HitCounts*)take(S_HC, (2ull * n + 1) * sizeof(HitCounts));
+    auto* d_acap = (uint64_t*)take(S_ACAP, (2ull * n + 1) * 8);
+    auto* d_aoff = (uint64_t*)take(S_AOFF, (2ull * n + 1) * 8);
+    o.hbytes = (uint32_t*)take(S_HBYTES, (2ull * n + 1) * 4);
+    if (ev) SA_CHECK(hipEventRecord(ev[0], st));
+    k_seed<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, p, d_sy, d_qr, o.qrn);
+    SA_CHECK(hipGetLastError());
+    if (ev) SA_CHECK(hipEventRecord(ev[1], st));
+    k_hits<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, d_hits, d_cnt, o.hc, d_acap,
+                                                  o.hbytes);
+    SA_CHECK(hipGetLastError());
+    if (ev) SA_CHECK(hipEventRecord(ev[2], st));
+    const uint64_t A = scan(d_acap, d_aoff, 2ull * n);
+    ChainBufs b;
+    b.anchors = (Anchor*)take(S_ANCH, (A + 1) * sizeof(Anchor));
+    b.anchors_tmp = (Anchor*)take(S_ANCH2, (A + 1) * sizeof(Anchor));
+    b.dp = (float*)take(S_DP, (A + 1) * 4);
+    b.pred = (uint32_t*)take(S_PRED, (A + 1) * 4);
+    b.cand = (uint32_t*)take(S_CAND, (A + 1) * 4);
+    b.used = (uint8_t*)take(S_USED, A + 1);
+    b.nams = (NamRec*)take(S_NAMS_IN, (A + 1) * sizeof(NamRec));
+    b.chain_anchors = (uint32_t*)take(S_CHAIN, (4 * A + 4) * 4);
+    b.alg_bytes = o.cbytes = (uint32_t*)take(S_CBYTES, (2ull * n + 1) * 4);
+    o.namn = (uint32_t*)take(S_NAMN, (2ull * n + 1) * 4);
+    o.araw = (uint32_t*)take(S_ARAW, (2ull * n + 1) * 4);
+    if (ev) SA_CHECK(hipEventRecord(ev[3], st));
+    k_chain<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b, o.namn,
+                                                   o.araw, log2tab, same_diag);
+    SA_CHECK(hipGetLastError());
+    if (ev) SA_CHECK(hipEventRecord(ev[4], st));
+    auto* d_tot = (uint64_t*)take(S_TOT, (uint64_t)(n + 1) * 8);
+    o.outoff = (uint64_t*)take(S_OUTOFF, (uint64_t)(n + 1) * 8);
+    k_pair_sum<<<blocks(n, 256), 256, 0, st>>>(o.namn, d_tot, n);
+    SA_CHECK(hipGetLastError());
+    o.n_nams = scan(d_tot, o.outoff, n);
+    o.nams = nams_out(o.n_nams);
+    if (ev) SA_CHECK(hipEventRecord(ev[5], st));
+    k_finalize<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, ref, k, o.namn, d_aoff, b.nams, o.outoff, o.nams);
+    SA_CHECK(hipGetLastError());
+    if (ev) SA_CHECK(hipEventRecord(ev[6], st));
+}
+
+// same-diagonal chaining scores (src/chainer.rs:63-74) in host f32 arithmetic
+void same_diag_table(uint32_t k, const ChainParams& cp, float* sd) {
+    float lg[1025];
+    for (uint32_t d = 0; d < 1025; ++d) lg[d] = std::log2((float)d + 1.0f);
+    for (uint32_t i = 0; i < 1024; ++i) sd[i] = chain_score(i, i, k, cp, lg);
+}
+}  // namespace
+
 void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
     SA_CHECK(hipSetDevice(device_));
     const uint32_t n = n_reads_;
@@ -539,74 +649,30 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     out.nam_off[0] = 0;
     out.info.clear();
     if (n == 0) return;
-    // same-diagonal score table (src/chainer.rs:63-74), host f32 arithmetic
     const uint32_t k = (uint32_t)p_.k;
     if (tab_k_ != k || std::memcmp(&tab_cp_, &cp, sizeof cp) != 0) {
-        std::vector<float> lg(1025);
-        for (size_t d = 0; d < lg.size(); ++d) lg[d] = std::log2((float)d + 1.0f);
-        std::vector<float> sd(1024);
-        for (uint32_t i = 0; i < 1024; ++i) sd[i] = chain_score(i, i, k, cp, lg.data());
-        SA_CHECK(hipMemcpyAsync(d_same_diag_, sd.data(), sd.size() * 4, hipMemcpyHostToDevice, stream_));
+        float sd[1024];
+        same_diag_table(k, cp, sd);
+        SA_CHECK(hipMemcpyAsync(d_same_diag_, sd, sizeof sd, hipMemcpyHostToDevice, stream_));
         sync();
         tab_k_ = k;
         tab_cp_ = cp;
     }
-    hipEvent_t ev[6];
-    for (auto& e : ev) SA_CHECK(hipEventCreateWithFlags(&e, hipEventBlockingSync));
+    hipEvent_t ev[7];
+    for (auto& e : ev) e = event();
     IndexView ix{d_entries_, d_buckets_, p_};
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
-    auto* d_seq = d_seq_;
-    auto* d_off = (const uint64_t*)buf_[S_OFF];
-    // the syncmer list is only materialised by the fallback for w_max >= 24
-    auto* d_sy = (Syncmer*)grow(S_SY, (p_.w_max < 24 ? 1 : n_bases_ + 1) * sizeof(Syncmer));
-    auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
-    auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
-    auto* d_hits = (Hit*)grow(S_HITS, (2 * n_bases_ + 1) * sizeof(Hit));
-    auto* d_cnt = (uint64_t*)grow(S_CNT, (2 * n_bases_ + 1) * 8);
-    auto* d_hc = (HitCounts*)grow(S_HC, (2ull * n + 1) * sizeof(HitCounts));
-    auto* d_acap = (uint64_t*)grow(S_ACAP, (2ull * n + 1) * 8);
-    auto* d_aoff = (uint64_t*)grow(S_AOFF, (2ull * n + 1) * 8);
-    SA_CHECK(hipEventRecord(ev[0], stream_));
-    k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
-    SA_CHECK(hipGetLastError());
-    SA_CHECK(hipEventRecord(ev[1], stream_));
-    auto* d_hbytes = (uint32_t*)grow(S_HBYTES, (2ull * n + 1) * 4);
-    k_hits<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, mp, d_off, n, d_qr, d_qrn, d_hits, d_cnt, d_hc, d_acap,
-                                                       d_hbytes);
-    SA_CHECK(hipGetLastError());
-    SA_CHECK(hipEventRecord(ev[2], stream_));
-    uint64_t A = 0;
-    scan_u64(d_acap, d_aoff, 2ull * n, &A);
-    ChainBufs b;
-    b.anchors = (Anchor*)grow(S_ANCH, (A + 1) * sizeof(Anchor));
-    b.anchors_tmp = (Anchor*)grow(S_ANCH2, (A + 1) * sizeof(Anchor));
-    b.dp = (float*)grow(S_DP, (A + 1) * 4);
-    b.pred = (uint32_t*)grow(S_PRED, (A + 1) * 4);
-    b.cand = (uint32_t*)grow(S_CAND, (A + 1) * 4);
-    b.used = (uint8_t*)grow(S_USED, A + 1);
-    b.nams = (NamRec*)grow(S_NAMS_IN, (A + 1) * sizeof(NamRec));
-    b.chain_anchors = (uint32_t*)grow(S_CHAIN, (4 * A + 4) * 4);
-    b.alg_bytes = (uint32_t*)grow(S_CBYTES, (2ull * n + 1) * 4);
-    auto* d_namn = (uint32_t*)grow(S_NAMN, (2ull * n + 1) * 4);
-    auto* d_araw = (uint32_t*)grow(S_ARAW, (2ull * n + 1) * 4);
-    SA_CHECK(hipEventRecord(ev[3], stream_));
-    k_chain<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, cp, d_off, n, d_hits, d_hc, d_acap, d_aoff, b, d_namn,
-                                                         d_araw, d_log2_, d_same_diag_);
-    SA_CHECK(hipGetLastError());
-    SA_CHECK(hipEventRecord(ev[4], stream_));
-    auto* d_tot = (uint64_t*)grow(S_TOT, (uint64_t)(n + 1) * 8);
-    auto* d_outoff = (uint64_t*)grow(S_OUTOFF, (uint64_t)(n + 1) * 8);
-    k_pair_sum<<<blocks(n, 256), 256, 0, stream_>>>(d_namn, d_tot, n);
-    uint64_t N = 0;
-    scan_u64(d_tot, d_outoff, n, &N);
-    auto* d_nams = (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec));
-    hipEvent_t fe0, fe1;
-    SA_CHECK(hipEventCreateWithFlags(&fe0, hipEventBlockingSync));
-    SA_CHECK(hipEventCreateWithFlags(&fe1, hipEventBlockingSync));
-    SA_CHECK(hipEventRecord(fe0, stream_));
-    k_finalize<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, ref, k, d_namn, d_aoff, b.nams, d_outoff, d_nams);
-    SA_CHECK(hipGetLastError());
-    SA_CHECK(hipEventRecord(fe1, stream_));
+    NamPass o;
+    nam_pass(
+        ix, ref, d_log2_, d_same_diag_, d_seq_, (const uint64_t*)buf_[S_OFF], n, n_bases_, mp, cp, stream_,
+        [&](int slot, uint64_t bytes) { return grow(slot, bytes); },
+        [&](const uint64_t* in, uint64_t* outp, uint64_t m) {
+            uint64_t t = 0;
+            scan_u64(in, outp, m, &t);
+            return t;
+        },
+        [&](uint64_t N) { return (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec)); }, ev, o);
+    const uint64_t N = o.n_nams;
     // everything the host reads back goes to pinned staging memory in one block
     const uint64_t n2 = 2ull * n;
     const uint64_t o_off = ((uint64_t)N * sizeof(NamRec) + 15) & ~15ull;
@@ -622,14 +688,14 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     uint32_t* araw = (uint32_t*)(hs + o_u32 + 2 * u32_bytes);
     uint32_t* cbytes = (uint32_t*)(hs + o_u32 + 3 * u32_bytes);
     uint32_t* hbytes = (uint32_t*)(hs + o_u32 + 4 * u32_bytes);
-    if (N) SA_CHECK(hipMemcpyAsync(out.nams.data(), d_nams, N * sizeof(NamRec), hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(out.nam_off.data(), d_outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(cbytes, b.alg_bytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync((void*)hc, d_hc, n2 * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(qrn, d_qrn, n2 * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(namn, d_namn, n2 * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(araw, d_araw, n2 * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(hbytes, d_hbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    if (N) SA_CHECK(hipMemcpyAsync(out.nams.data(), o.nams, N * sizeof(NamRec), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(out.nam_off.data(), o.outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(cbytes, o.cbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync((void*)hc, o.hc, n2 * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(qrn, o.qrn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(namn, o.namn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(araw, o.araw, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(hbytes, o.hbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
     sync();
     out.nam_off[n] = N;
     out.info.resize(n);
@@ -655,11 +721,107 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n, seed_bytes);
     SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n, hit_bytes);
     SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n, chain_bytes);
-    SA_CHECK(hipEventElapsedTime(&t, fe0, fe1)); times_.finalize = t; add_kernel("k_finalize", t, n);
+    SA_CHECK(hipEventElapsedTime(&t, ev[5], ev[6])); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(fe0);
-    (void)hipEventDestroy(fe1);
+    for (auto& e : ev) release_event(e);
+}
+
+namespace {
+// n_bases (= off[n], the extent the per-base buffers cover) and the longest read
+// of a batch whose offsets are in device memory: one block, written unconditionally.
+__global__ void __launch_bounds__(256) k_batch_shape(const uint64_t* __restrict__ off, uint32_t n,
+                                                     uint64_t* __restrict__ out) {
+    __shared__ uint64_t m[256];
+    uint64_t mx = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        const uint64_t L = off[i + 1] - off[i];
+        mx = L > mx ? L : mx;
+    }
+    m[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w && m[threadIdx.x + w] > m[threadIdx.x]) m[threadIdx.x] = m[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = off[n];
+        out[1] = m[0];
+    }
+}
+__global__ void k_set_u64(uint64_t* __restrict__ p, uint64_t v) { *p = v; }
+
+// Raised when a batch does not fit the scratch array or the caller's NAM buffer.
+struct TooLarge : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+}  // namespace
+
+void Engine::find_nams_noalloc(const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const MapParams& mp,
+                               const ChainParams& cp, NamRec* d_nams, uint64_t nam_cap, uint64_t* d_nam_off,
+                               uint64_t* n_nams, hipStream_t st, uint8_t* scratch, uint64_t scratch_bytes) {
+    SA_CHECK(hipSetDevice(device_));
+    *n_nams = 0;
+    if (n == 0) {
+        k_set_u64<<<1, 1, 0, st>>>(d_nam_off, 0);
+        SA_CHECK(hipGetLastError());
+        SA_CHECK(hipStreamSynchronize(st));
+        return;
+    }
+    uint64_t used = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t at = used;
+        used = (used + (bytes ? bytes : 16) + 255) & ~(uint64_t)255;
+        if (used > scratch_bytes)
+            throw TooLarge("batch needs more than the " + std::to_string(scratch_bytes >> 20) +
+                           " MiB scratch of the no-allocation entry point (at least " + std::to_string(used) +
+                           " bytes); split it into smaller batches");
+        return (void*)(scratch + at);
+    };
+    // batch shape from the device offsets
+    auto* shape = (uint64_t*)take(16);
+    k_batch_shape<<<1, 256, 0, st>>>(d_off, n, shape);
+    SA_CHECK(hipGetLastError());
+    uint64_t h_shape[2] = {0, 0};
+    SA_CHECK(hipMemcpyAsync(h_shape, shape, 16, hipMemcpyDeviceToHost, st));
+    SA_CHECK(hipStreamSynchronize(st));
+    const uint64_t n_bases = h_shape[0];
+    if (h_shape[1] >= 65536) throw std::invalid_argument("reads longer than 65535 bp are not supported");
+    const uint32_t k = (uint32_t)p_.k;
+    if (tab_k_ != k || std::memcmp(&tab_cp_, &cp, sizeof cp) != 0) {
+        float sd[1024];
+        same_diag_table(k, cp, sd);
+        SA_CHECK(hipMemcpyAsync(d_same_diag_, sd, sizeof sd, hipMemcpyHostToDevice, st));
+        SA_CHECK(hipStreamSynchronize(st));
+        tab_k_ = k;
+        tab_cp_ = cp;
+    }
+    IndexView ix{d_entries_, d_buckets_, p_};
+    RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    NamPass o;
+    nam_pass(
+        ix, ref, d_log2_, d_same_diag_, d_seq, d_off, n, n_bases, mp, cp, st,
+        [&](int slot, uint64_t bytes) { return slot == S_OUTOFF ? (void*)d_nam_off : take(bytes); },
+        [&](const uint64_t* in, uint64_t* outp, uint64_t m) {
+            size_t tmp = 0;
+            SA_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, outp, (int)m, st));
+            void* t = take(tmp);
+            SA_CHECK(hipcub::DeviceScan::ExclusiveSum(t, tmp, in, outp, (int)m, st));
+            uint64_t last[2];
+            SA_CHECK(hipMemcpyAsync(&last[0], in + m - 1, 8, hipMemcpyDeviceToHost, st));
+            SA_CHECK(hipMemcpyAsync(&last[1], outp + m - 1, 8, hipMemcpyDeviceToHost, st));
+            SA_CHECK(hipStreamSynchronize(st));
+            return last[0] + last[1];
+        },
+        [&](uint64_t N) {
+            if (N > nam_cap)
+                throw TooLarge("batch has " + std::to_string(N) + " NAMs, the output holds " + std::to_string(nam_cap));
+            return d_nams;
+        },
+        nullptr, o);
+    k_set_u64<<<1, 1, 0, st>>>(d_nam_off + n, o.n_nams);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipStreamSynchronize(st));
+    *n_nams = o.n_nams;
 }
 
 namespace {

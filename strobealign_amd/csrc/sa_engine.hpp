@@ -60,6 +60,12 @@ void ssw_raw_device(int device, const int8_t* read, int32_t readLen, const int8_
 double hbm_copy_probe(int device, uint64_t bytes, int iters);
 void device_free(int device, void* p);
 
+// Size of the static device array that backs the entry point that allocates
+// nothing (Engine::find_nams_noalloc), and that array's address on `device`
+// (looked up with hipGetSymbolAddress on the first call per device).
+constexpr uint64_t NOALLOC_SCRATCH_BYTES = 128ull << 20;
+uint8_t* noalloc_scratch(int device);
+
 class Engine {
 public:
     explicit Engine(int device);
@@ -98,6 +104,16 @@ public:
     void gather_reads(const uint8_t* d_base, const uint64_t* seg, const uint64_t* seg_len, uint32_t n_segs,
                       const uint64_t* off, uint32_t n_reads);
     void find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out);
+
+    // find_nams for reads and outputs in device memory, on the caller's stream,
+    // with every intermediate buffer carved from `scratch` (scratch_bytes): no
+    // device or pinned allocation, no timing events. NAMs go to d_nams (capacity
+    // nam_cap records), read i's at [d_nam_off[i], d_nam_off[i + 1]). Throws
+    // std::invalid_argument when the batch needs more scratch or NAM capacity.
+    // Returns once the outputs are written.
+    void find_nams_noalloc(const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const MapParams& mp,
+                           const ChainParams& cp, NamRec* d_nams, uint64_t nam_cap, uint64_t* d_nam_off,
+                           uint64_t* n_nams, ihipStream_t* st, uint8_t* scratch, uint64_t scratch_bytes);
 
     // Fine-grained stages for parity probes (valid after upload_reads).
     void randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts);
