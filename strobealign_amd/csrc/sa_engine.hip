@@ -73,7 +73,7 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_PWROWS, S_PWDIRS, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_PWROWS, S_PWDIRS, S_TIERS, S_COUNT
 };
 }  // namespace
 
@@ -559,6 +559,9 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
 }
 
 namespace {
+// anchor capacities of the LDS chaining tiers
+constexpr uint32_t CHAIN_CAP1 = 128, CHAIN_CAP2 = 1024;
+
 // Device buffers and counts of one seeding -> NAM pass.
 struct NamPass {
     uint32_t* qrn = nullptr;     // 2n: randstrobes per read orientation
@@ -616,9 +619,26 @@ void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, con
     b.alg_bytes = o.cbytes = (uint32_t*)take(S_CBYTES, (2ull * n + 1) * 4);
     o.namn = (uint32_t*)take(S_NAMN, (2ull * n + 1) * 4);
     o.araw = (uint32_t*)take(S_ARAW, (2ull * n + 1) * 4);
+    // chaining tiers by anchor capacity: a 16-lane group per orientation up to
+    // 128 anchors, a wave per orientation up to 1024 (both in LDS), a thread per
+    // orientation over global memory beyond
+    ChainTiers tiers;
+    auto* d_tiers = (uint32_t*)take(S_TIERS, (4ull * n + 4) * 4);
+    tiers.count = d_tiers;
+    tiers.list[0] = d_tiers + 4;
+    tiers.list[1] = d_tiers + 4 + 2ull * n;
+    tiers.cap1 = CHAIN_CAP1;
+    tiers.cap2 = CHAIN_CAP2;
     if (ev) SA_CHECK(hipEventRecord(ev[3], st));
-    k_chain<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b, o.namn,
-                                                   o.araw, log2tab, same_diag);
+    k_zero_u32<<<1, 64, 0, st>>>(tiers.count, 4);
+    k_chain_t1<16, CHAIN_CAP1><<<blocks(2ull * n, 4), 64, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b,
+                                                                   o.namn, o.araw, log2tab, same_diag, tiers);
+    SA_CHECK(hipGetLastError());
+    k_chain_t2<64, CHAIN_CAP2><<<1024, 64, 0, st>>>(ix, cp, d_off, d_hits, o.hc, d_aoff, b, o.namn, o.araw, log2tab,
+                                                    same_diag, tiers);
+    SA_CHECK(hipGetLastError());
+    k_chain<<<256, 256, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b, o.namn, o.araw, log2tab,
+                                 same_diag, tiers);
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[4], st));
     auto* d_tot = (uint64_t*)take(S_TOT, (uint64_t)(n + 1) * 8);

@@ -4,6 +4,7 @@
 // SSW score passes run one alignment per lane group (sa_ssw_group.hpp).
 #include <hip/hip_runtime.h>
 
+#include "sa_chain_group.hpp"
 #include "sa_items.hpp"
 #include "sa_ssw_group.hpp"
 
@@ -23,15 +24,76 @@ __global__ void __launch_bounds__(256) k_hits(IndexView ix, MapParams mp, const 
     item_hits(blockIdx.x * blockDim.x + threadIdx.x, ix, mp, off, n, qr, qr_n, hits, cnt, hc, acap, lookup_bytes);
 }
 
+// Chaining, first tier: one read orientation per group of G lanes with its
+// anchors in LDS (sa_chain_group.hpp), for anchor capacities up to tiers.cap1
+// (CAP); orientations not chained (src/chainer.rs:206-216) get empty results,
+// larger ones are listed for the later tiers.
+template <int G, int CAP>
+__global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
+                                                 uint32_t n, const Hit* __restrict__ hits,
+                                                 const HitCounts* __restrict__ hc, const uint64_t* __restrict__ acap,
+                                                 const uint64_t* __restrict__ aoff, ChainBufs b,
+                                                 uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
+                                                 const float* __restrict__ log2tab,
+                                                 const float* __restrict__ same_diag, ChainTiers tiers) {
+    __shared__ ChainLds<CAP> lds[64 / G];
+    const uint32_t g = threadIdx.x / G;
+    const Grp<G> gp{threadIdx.x % G, g * G};
+    const uint32_t t = blockIdx.x * (64 / G) + g;
+    if (t >= 2 * n) return;
+    if (!orientation_chained(hc, t)) {
+        if (gp.r == 0) {
+            nam_n[t] = 0;
+            anchors_raw[t] = 0;
+            b.alg_bytes[t] = 0;
+        }
+        return;
+    }
+    const uint64_t cap = acap[t];
+    if (cap > tiers.cap1) {
+        if (gp.r == 0) {
+            const uint32_t tier = cap > tiers.cap2 ? 1 : 0;
+            tiers.list[tier][atomicAdd(&tiers.count[tier], 1u)] = t;
+        }
+        return;
+    }
+    chain_grp<G, CAP>(gp, lds[g], t, ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
+}
+
+// Chaining, second tier: the listed orientations with capacities up to
+// tiers.cap2 (CAP), a group each, grid-stride over the list.
+template <int G, int CAP>
+__global__ void __launch_bounds__(64) k_chain_t2(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
+                                                 const Hit* __restrict__ hits, const HitCounts* __restrict__ hc,
+                                                 const uint64_t* __restrict__ aoff, ChainBufs b,
+                                                 uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
+                                                 const float* __restrict__ log2tab,
+                                                 const float* __restrict__ same_diag, ChainTiers tiers) {
+    __shared__ ChainLds<CAP> lds[64 / G];
+    const uint32_t g = threadIdx.x / G;
+    const Grp<G> gp{threadIdx.x % G, g * G};
+    const uint32_t cnt = tiers.count[0];
+    for (uint32_t x = blockIdx.x * (64 / G) + g; x < cnt; x += gridDim.x * (64 / G))
+        chain_grp<G, CAP>(gp, lds[g], tiers.list[0][x], ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab,
+                          same_diag);
+}
+
+// Chaining, last tier: the listed orientations above tiers.cap2, a thread each
+// with its work arrays in global memory (item_chain), grid-stride over the list.
 __global__ void __launch_bounds__(256) k_chain(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
                                               uint32_t n, const Hit* __restrict__ hits,
                                               const HitCounts* __restrict__ hc, const uint64_t* __restrict__ acap,
                                               const uint64_t* __restrict__ aoff, ChainBufs b,
                                               uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
                                               const float* __restrict__ log2tab,
-                                              const float* __restrict__ same_diag) {
-    item_chain(blockIdx.x * blockDim.x + threadIdx.x, ix, cp, off, n, hits, hc, acap, aoff, b, nam_n, anchors_raw,
-               log2tab, same_diag);
+                                              const float* __restrict__ same_diag, ChainTiers tiers) {
+    const uint32_t cnt = tiers.count[1];
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < cnt; x += gridDim.x * blockDim.x)
+        item_chain(tiers.list[1][x], ix, cp, off, n, hits, hc, acap, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
+}
+
+__global__ void __launch_bounds__(64) k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += 64) p[i] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_finalize(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
