@@ -76,6 +76,19 @@ int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* en
                     const uint64_t* contig_word_offsets, const uint64_t* contig_lengths,
                     const char* const* contig_names, uint32_t n_contigs, sa_index** out);
 void sa_index_destroy(sa_index* index);
+/* Writes the index as a .sti file (StrobemerIndex::write, src/index.rs:374-409):
+ * "STI\1", format version 7, an 8-byte reserved chunk (length, contents), then
+ * filter cutoff, bits, profile number (of read_length's profile), k, s, w_min,
+ * w_max, q and max_dist as u32, main_hash_mask as u64, the 16-byte entries and
+ * the bucket starts each after its u64 length; native byte order. */
+int sa_index_save_sti(const sa_index* index, uint64_t read_length, const char* path);
+/* Loads a .sti file (read_index, src/index.rs:412-495) with its FASTA reference:
+ * the header is checked against the parameters for read_length and `bits`
+ * (<= 0: pick_bits of the reference) as the reference checks it, and the entries
+ * and bucket starts stream from the file through pinned staging memory straight
+ * into device memory (no host copy of the index). */
+int sa_index_load_sti(sa_ctx* ctx, const char* fasta_path, const char* sti_path, uint64_t read_length, int bits,
+                      sa_index** out);
 /* sa_index_build with make_index (src/indexer.rs:18-170) run on the device:
    syncmers per 4096-position chunk, randstrobes per syncmer, two stable radix
    sorts, bucket directory and filter cutoff from device passes. Same entries,
@@ -254,6 +267,60 @@ int sa_find_nams_batch_noalloc(sa_index* index, const uint8_t* d_seqs, const uin
                                uint32_t mcs, uint32_t rescue_distance, sa_nam* d_nams, uint64_t nam_cap,
                                uint64_t* d_nam_offsets, uint64_t* n_nams, void* stream);
 
+/* One NAM's extension in sa_map_batch: what extend_seed (src/mapper.rs:532-620)
+ * returns for it (Alignment, src/mapper.rs:58-77: reference start and span,
+ * query_start / query_end give the soft clips). The CIGAR (BAM-encoded ops with
+ * =/X) is at [cigar_offset, cigar_offset + n_cigar) of the batch's CIGAR array. */
+typedef struct sa_nam_extension {
+    uint32_t found, gapped, score, ref_start, length, query_start, query_end, edit_distance;
+    uint64_t cigar_offset;
+    uint32_t n_cigar, extended; /* extended: 0 when this NAM was not extended (found then 0) */
+} sa_nam_extension;
+
+/* Batch drop-in for a caller that keeps its own decision loop (the reference's
+ * align_single_end_read, src/mapper.rs:349-528, with its RNG and dropoff rules):
+ * for n single-end reads, the NAMs of sa_find_nams_batch (stable score order,
+ * before the per-chunk shuffle) and, for the first `speculative` NAMs of each
+ * read that are consistent (Nam::is_consistent, src/nam.rs:49-66), the result of
+ * extend_seed with opt's scores and --ssw choice; ext[x] belongs to nams[x].
+ * A NAM outside that set is not extended (its entry has extended = 0) and the
+ * caller aligns it itself. Work runs on `stream` (a hipStream_t; NULL: the
+ * library's own): it is ordered after the work already queued there, and the
+ * call returns once its results are copied out. *nams, *ext and *cigars are
+ * released with sa_free. */
+int sa_map_batch(sa_index* index, const sa_map_options* opt, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
+                 uint32_t speculative, void* stream, sa_nam** nams, uint64_t* nam_offsets, sa_nam_extension** ext,
+                 uint32_t** cigars);
+
+/* Hit (src/hit.rs:19-28) and HitsDetails (src/hit.rs:32-52). */
+typedef struct sa_hit {
+    uint64_t query_start, query_end;
+    uint64_t position; /* index entry: forward position for full hits, undirected for partial */
+    uint64_t hash, hash_revcomp;
+    uint32_t is_partial, is_filtered;
+} sa_hit;
+typedef struct sa_hits_details {
+    uint64_t full_not_found, full_filtered, full_found;
+    uint64_t partial_not_found, partial_filtered, partial_found;
+    uint64_t rescued;
+} sa_hits_details;
+
+/* find_hits (src/hit.rs:270-333, with global and local rescue) for both
+ * orientations of n reads (randstrobes_query first, src/seeding/mod.rs:19-75).
+ * Orientation o (0 forward, 1 reverse complement) of read i has its hits at
+ * [hit_offsets[2i+o], hit_offsets[2i+o+1]) of *hits (released with sa_free;
+ * hit_offsets has 2n + 1 entries) and its details in details[2i+o]. */
+int sa_find_hits_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, uint32_t mcs,
+                       uint32_t rescue_distance, sa_hit** hits, uint64_t* hit_offsets, sa_hits_details* details);
+
+/* The chaining half of Chainer::get_chains (src/chainer.rs:182-267): orientation
+ * choice, hits_to_anchors, sort + dedup, collinear_chaining, extract_chains, then
+ * the stable score sort and consistency flags as sa_find_nams_batch returns them,
+ * from hits and details laid out as sa_find_hits_batch returns them (an
+ * orientation has at most as many hits as its read has bases). */
+int sa_chain_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, const sa_hit* hits,
+                   const uint64_t* hit_offsets, const sa_hits_details* details, sa_nam** nams, uint64_t* nam_offsets);
+
 /* Aligner::align (src/aligner.rs:80-151: SSW of ext/ssw/ssw.c plus the end-bonus
  * extension) of n (query, reference) pairs. Queries/references are ASCII,
  * concatenated with offsets. Each result: found, score, query_start, query_end,
@@ -265,6 +332,40 @@ typedef struct sa_alignment {
 int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
                        const uint64_t* ref_offsets, uint32_t n, sa_alignment* out, uint32_t** cigars,
                        uint64_t* cigar_offsets);
+
+/* Alignment scores (src/aligner.rs:11-30) plus the piecewise x-drop (src/main.rs:271). */
+typedef struct sa_scores {
+    int32_t match_score, mismatch_penalty, gap_open_penalty, gap_extend_penalty, end_bonus, xdrop;
+} sa_scores;
+
+/* Kinds of sa_extend_batch jobs: the aligner's entry points. */
+#define SA_EXT_SSW 0       /* Aligner::align (src/aligner.rs:80-151): SSW + end-bonus extension */
+#define SA_EXT_HAMMING 1   /* hamming_align (src/aligner.rs:223-259); not found when lengths differ */
+#define SA_EXT_PIECEWISE 2 /* Aligner::align_piecewise (src/aligner.rs:153-163) = extend_piecewise
+                              (src/piecewisealigner.rs:396-510) with the job's anchors, no pruning */
+#define SA_EXT_GLOBAL 3    /* PiecewiseAligner::global_alignment (src/piecewisealigner.rs:84-120) */
+#define SA_EXT_XDROP 4     /* xdrop_alignment, GlobalStartLocalEnd (src/piecewisealigner.rs:161-229) */
+#define SA_EXT_XDROP_REV 5 /* xdrop_alignment, LocalStartGlobalEnd */
+
+typedef struct sa_extend_job {
+    uint32_t kind;          /* SA_EXT_* */
+    uint32_t padding;       /* SA_EXT_PIECEWISE: end padding (src/mapper.rs:581 passes read length / 10) */
+    uint64_t anchor_offset; /* SA_EXT_PIECEWISE: first of its n_anchors pairs in `anchors` */
+    uint32_t n_anchors;
+    uint32_t reserved;
+} sa_extend_job;
+
+/* Job i aligns query i (queries/query_offsets, ASCII) against reference window i
+ * (refs/ref_offsets, ACGT, as refseq.decode returns it) with its kind's entry
+ * point, on the device (the kernels of the mapping path). Piecewise anchors are
+ * (ref_start, query_start) pairs in window coordinates, last anchor first, each
+ * spanning k bases (the index's k). Results as sa_align_ssw_batch; coordinates
+ * are window-relative; for SA_EXT_GLOBAL / SA_EXT_XDROP* `found` is set and
+ * `score` holds the block aligner's i32 score (two's complement). */
+int sa_extend_batch(sa_ctx* ctx, const sa_scores* scores, uint32_t k, const sa_extend_job* jobs, uint32_t n,
+                    const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
+                    const uint64_t* ref_offsets, const uint32_t* anchors, uint64_t n_anchor_pairs, sa_alignment* out,
+                    uint32_t** cigars, uint64_t* cigar_offsets);
 
 /* Kernel time (ms) spent by the index's engine since creation, and the number
  * of device launches; for measurement. */

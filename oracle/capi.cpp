@@ -3,6 +3,8 @@
 #include "oracle.hpp"
 
 #include <cstring>
+#include <string>
+#include <vector>
 
 using namespace orc;
 
@@ -149,6 +151,88 @@ int64_t orc_remove_spurious_anchors(uint64_t* pairs, int64_t n) {
 void orc_rng_u64(uint64_t seed, int64_t n, uint64_t* out) {
     Rng r(seed);
     for (int64_t i = 0; i < n; ++i) out[i] = r.gen_u64();
+}
+
+}  // extern "C"
+
+namespace {
+// The index built from `fasta` for `read_length`, kept for later calls with the
+// same arguments.
+struct CachedIndex {
+    std::string path;
+    uint64_t rl = 0;
+    std::vector<RefSeq> refs;
+    Index index;
+};
+CachedIndex& cached_index(const char* fasta, uint64_t read_length) {
+    static CachedIndex c;
+    if (c.path != fasta || c.rl != read_length) {
+        SeedingParams sp = seeding_params_for_read_length(read_length);
+        c.refs = read_fasta(fasta);
+        c.index = make_index(c.refs, sp, pick_bits(c.refs, sp.syncmer), 0.0002);
+        c.path = fasta;
+        c.rl = read_length;
+    }
+    return c;
+}
+}  // namespace
+
+extern "C" {
+
+// find_hits (src/hit.rs:270-333) of both orientations of one read against the
+// index built from `fasta` for `read_length`. Orientation o writes counts[o] hits
+// of 7 words (query_start, query_end, position, hash, hash_revcomp, is_partial,
+// is_filtered) at out + o * cap * 7 and its 7 detail counters at details + o * 7.
+// Returns -1 if an orientation has more than cap hits.
+int64_t orc_find_hits(const char* fasta, uint64_t read_length, const uint8_t* seq, uint64_t len, int mcs,
+                      uint64_t rescue_distance, uint64_t cap, uint64_t* out, uint64_t* counts, uint64_t* details) {
+    const Index& index = cached_index(fasta, read_length).index;
+    std::vector<QueryRandstrobe> qr[2];
+    randstrobes_query(seq, len, index.params, qr);
+    for (int o = 0; o < 2; ++o) {
+        HitsDetails d;
+        std::vector<Hit> hits = find_hits(qr[o], index, (Mcs)mcs, index.filter_cutoff, rescue_distance, d);
+        if (hits.size() > cap) return -1;
+        counts[o] = hits.size();
+        uint64_t* w = out + (uint64_t)o * cap * 7;
+        for (const Hit& h : hits) {
+            *w++ = h.query_start; *w++ = h.query_end; *w++ = h.position; *w++ = h.hash; *w++ = h.hash_revcomp;
+            *w++ = h.is_partial; *w++ = h.is_filtered;
+        }
+        uint64_t* dd = details + o * 7;
+        dd[0] = d.full_not_found; dd[1] = d.full_filtered; dd[2] = d.full_found; dd[3] = d.partial_not_found;
+        dd[4] = d.partial_filtered; dd[5] = d.partial_found; dd[6] = d.rescued;
+    }
+    return 0;
+}
+
+// speculative_extensions of one single-end read (index as orc_find_hits, default
+// scores unless sc is given): per extension 10 words (nam_index, found, gapped,
+// score, ref_start, length, query_start, query_end, edit_distance, cigar length)
+// at out + 10 i and its CIGAR text in cigars, each after the previous one. Returns
+// the number of extensions, or -1 if cap or cigar_cap is too small.
+int64_t orc_speculative_extensions(const char* fasta, uint64_t read_length, const char* seq, int use_ssw,
+                                   uint64_t spec, const int* sc, uint64_t cap, uint64_t* out, char* cigars,
+                                   uint64_t cigar_cap) {
+    CachedIndex& ci = cached_index(fasta, read_length);
+    Mapper m;
+    m.index = &ci.index;
+    m.refs = &ci.refs;
+    m.mp.use_ssw = use_ssw != 0;
+    if (sc) m.sc = scores_of(sc);
+    std::vector<SpecExtension> ext = speculative_extensions(m, seq, spec);
+    if (ext.size() > cap) return -1;
+    uint64_t at = 0;
+    for (size_t i = 0; i < ext.size(); ++i) {
+        const SpecExtension& e = ext[i];
+        uint64_t* w = out + 10 * i;
+        w[0] = e.nam_index; w[1] = e.found; w[2] = e.gapped; w[3] = e.score; w[4] = e.ref_start; w[5] = e.length;
+        w[6] = e.query_start; w[7] = e.query_end; w[8] = e.edit_distance; w[9] = e.cigar.size();
+        if (at + e.cigar.size() > cigar_cap) return -1;
+        std::memcpy(cigars + at, e.cigar.data(), e.cigar.size());
+        at += e.cigar.size();
+    }
+    return (int64_t)ext.size();
 }
 
 }  // extern "C"

@@ -2,6 +2,9 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"
 
+#include <atomic>
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 
@@ -330,15 +333,33 @@ std::vector<Nam> get_nams_by_chaining(const uint8_t* seq, size_t len, const Inde
     return chains;
 }
 
+// ORC_RNG_VARIANT (test infrastructure: plants a mistake in the generator to
+// measure what the golden files pin): 0 fastrand as restated; 1 another wyrand
+// increment; 2 modulo in place of Lemire's bounded sampling; 3 the shuffle
+// drawing from [0, i) instead of [0, i]. Bounded draws with n >= 2 are counted.
+int rng_variant() {
+    static const int v = [] {
+        const char* e = std::getenv("ORC_RNG_VARIANT");
+        return e ? std::atoi(e) : 0;
+    }();
+    return v;
+}
+std::atomic<uint64_t>& rng_draws() {
+    static std::atomic<uint64_t> n{0};
+    return n;
+}
+
 // fastrand 2.3.0 (wyrand v4.2 constants) -- parity unpinned, see oracle.hpp.
 uint64_t Rng::gen_u64() {
-    s += 0x2d358dccaa6c78a5ULL;
+    s += rng_variant() == 1 ? 0x9e3779b97f4a7c15ULL : 0x2d358dccaa6c78a5ULL;
     unsigned __int128 t = (unsigned __int128)s * (unsigned __int128)(s ^ 0x8bb84b93962eacc9ULL);
     return (uint64_t)t ^ (uint64_t)(t >> 64);
 }
 
 // Lemire's nearly divisionless bounded sampling, 32-bit variant.
 uint32_t Rng::gen_mod_u32(uint32_t n) {
+    if (n >= 2) rng_draws()++;
+    if (rng_variant() == 2) return (uint32_t)gen_u64() % n;
     uint32_t r = (uint32_t)gen_u64();
     uint64_t m = (uint64_t)r * n;
     uint32_t hi = (uint32_t)(m >> 32), lo = (uint32_t)m;
@@ -355,6 +376,8 @@ uint32_t Rng::gen_mod_u32(uint32_t n) {
 }
 
 uint64_t Rng::gen_mod_u64(uint64_t n) {
+    if (n >= 2) rng_draws()++;
+    if (rng_variant() == 2) return gen_u64() % n;
     uint64_t r = gen_u64();
     unsigned __int128 m = (unsigned __int128)r * n;
     uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
@@ -379,7 +402,8 @@ void sort_nams(std::vector<Nam>& nams, Rng& rng) {
     size_t n_best = 0;
     while (n_best < nams.size() && nams[n_best].score == best) ++n_best;
     if (n_best > 1)
-        for (size_t i = 1; i < n_best; ++i) std::swap(nams[i], nams[rng.gen_mod_u64(i + 1)]);
+        for (size_t i = 1; i < n_best; ++i)
+            std::swap(nams[i], nams[rng.gen_mod_u64(rng_variant() == 3 ? i : i + 1)]);
 }
 
 StageTimes& stage_times() {

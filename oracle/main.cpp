@@ -28,6 +28,7 @@ int main(int argc, char** argv) {
     size_t threads = 1, chunk_size = 1000000, read_length = 0;
     int mode = 0;
     bool time_it = false;
+    bool rng_stats = false;  // print the number of bounded random draws to stderr
     size_t t1_chunks = 0;  // with --time: also map the first N chunks again on one thread
     MappingParams mp;
     SamOpts so;
@@ -57,6 +58,7 @@ int main(int argc, char** argv) {
         else if (a == "--details") so.details = true;
         else if (a == "--no-PG") {}
         else if (a == "--time") time_it = true;
+        else if (a == "--rng-stats") rng_stats = true;
         else if (is("--t1-chunks")) t1_chunks = std::stoul(val("--t1-chunks"));
         else if (is("--rg-id")) { so.has_rg = true; so.rg_id = val("--rg-id"); }
         else if (is("--rg")) rg_fields.push_back(val("--rg"));
@@ -232,5 +234,6 @@ int main(int argc, char** argv) {
                      ti, tm, n, threads, s.randstrobes, s.find_hits, s.chaining, s.sort_nams, s.extend, t1_s,
                      t1_records);
     }
+    if (rng_stats) std::fprintf(stderr, "rng_draws %llu\n", (unsigned long long)orc::rng_draws().load());
     return 0;
 }

@@ -894,6 +894,39 @@ std::vector<NamPairOwned> get_nam_pairs(std::vector<Nam>& n1, std::vector<Nam>& 
 
 }  // namespace
 
+std::vector<SpecExtension> speculative_extensions(const Mapper& m, const std::string& seq, size_t spec) {
+    NamDetails nd;
+    std::vector<Nam> nams = get_nams_by_chaining((const uint8_t*)seq.data(), seq.size(), *m.index, m.cp,
+                                                 m.mp.rescue_distance, m.mp.mcs, nd);
+    std::stable_sort(nams.begin(), nams.end(), [](const Nam& a, const Nam& b) { return a.score > b.score; });
+    Rng rng(0);
+    InsertSize isize;
+    Ctx c{m, rng, isize};
+    Read read(seq);
+    std::vector<SpecExtension> out;
+    for (size_t i = 0, done = 0; i < nams.size() && done < spec; ++i) {
+        Nam nam = nams[i];
+        if (!is_consistent(nam, read, c)) continue;
+        ++done;
+        SpecExtension e;
+        e.nam_index = i;
+        auto al = extend_seed(nam, read, true, m.mp.use_ssw, c);
+        if (al) {
+            e.found = true;
+            e.score = al->score;
+            e.ref_start = al->ref_start;
+            e.length = al->length;
+            e.query_start = al->soft_clip_left;
+            e.query_end = seq.size() - al->soft_clip_right;
+            e.edit_distance = al->edit_distance;
+            e.gapped = al->gapped;
+            e.cigar = al->cigar.str();
+        }
+        out.push_back(e);
+    }
+    return out;
+}
+
 // src/main.rs:873-988
 void Mapper::map_chunk(const std::vector<std::pair<Record, std::optional<Record>>>& chunk, std::string& out) {
     Rng rng(0);

@@ -11,6 +11,7 @@
 // and known-answer tests cannot pin say "parity unpinned" next to them.
 #pragma once
 #include <chrono>
+#include <atomic>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -191,6 +192,8 @@ struct Rng {
 
 // src/nam.rs:168-188 + src/shuffle.rs:7-25
 void sort_nams(std::vector<Nam>& nams, Rng& rng);
+// bounded draws with n >= 2 made so far by every Rng of the process
+std::atomic<uint64_t>& rng_draws();
 
 // ---------------------------------------------------------------- cigar ---
 // src/cigar.rs:4-243. op codes: 0 M 1 I 2 D 3 N 4 S 5 H 6 P 7 = 8 X
@@ -284,6 +287,19 @@ struct Mapper {
     // src/main.rs:873-988: one chunk with fresh Rng(0) and insert-size state
     void map_chunk(const std::vector<std::pair<Record, std::optional<Record>>>& chunk, std::string& out);
 };
+
+// Test hook for the batch drop-in: the NAMs of one single-end read after the
+// stable score sort (src/nam.rs:170, no shuffle) and extend_seed
+// (src/mapper.rs:532-620) of the first `spec` of them that are consistent, in
+// NAM order (the try loop's order without its stopping rules).
+struct SpecExtension {
+    size_t nam_index = 0;
+    bool found = false, gapped = false;
+    uint32_t score = 0;
+    size_t ref_start = 0, length = 0, query_start = 0, query_end = 0, edit_distance = 0;
+    std::string cigar;
+};
+std::vector<SpecExtension> speculative_extensions(const Mapper& m, const std::string& seq, size_t spec);
 
 // src/io/sam.rs:177-201
 std::string sam_header(const std::vector<RefSeq>& refs, const SamOpts& so,

@@ -42,6 +42,33 @@ class MapOptions(ctypes.Structure):
                 ("end_bonus", ctypes.c_int32), ("xdrop", ctypes.c_int32)]
 
 
+class Scores(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("match_score", "mismatch_penalty", "gap_open_penalty",
+                                              "gap_extend_penalty", "end_bonus", "xdrop")]
+
+
+class ExtendJob(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("padding", ctypes.c_uint32), ("anchor_offset", ctypes.c_uint64),
+                ("n_anchors", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class NamExtension(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("found", "gapped", "score", "ref_start", "length", "query_start",
+                                              "query_end", "edit_distance")] + \
+               [("cigar_offset", ctypes.c_uint64), ("n_cigar", ctypes.c_uint32), ("extended", ctypes.c_uint32)]
+
+
+class HitRec(ctypes.Structure):
+    _fields_ = [("query_start", ctypes.c_uint64), ("query_end", ctypes.c_uint64), ("position", ctypes.c_uint64),
+                ("hash", ctypes.c_uint64), ("hash_revcomp", ctypes.c_uint64), ("is_partial", ctypes.c_uint32),
+                ("is_filtered", ctypes.c_uint32)]
+
+
+class HitsDetails(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("full_not_found", "full_filtered", "full_found", "partial_not_found",
+                                               "partial_filtered", "partial_found", "rescued")]
+
+
 class QRand(ctypes.Structure):
     _fields_ = [("hash", ctypes.c_uint64), ("hash_revcomp", ctypes.c_uint64), ("start", ctypes.c_uint32),
                 ("end", ctypes.c_uint32)]
@@ -60,9 +87,9 @@ class Alignment(ctypes.Structure):
 
 
 EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
-           "sa_index_build", "sa_index_build_device", "sa_index_download", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
+           "sa_index_build", "sa_index_build_device", "sa_index_save_sti", "sa_index_load_sti", "sa_index_download", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
-           "sa_find_nams_batch_noalloc", "sa_align_ssw_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
+           "sa_find_nams_batch_noalloc", "sa_align_ssw_batch", "sa_extend_batch", "sa_find_hits_batch", "sa_chain_batch", "sa_map_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
            "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
            "sa_server_destroy", "sa_server_submit", "sa_server_submit_reads", "sa_server_wait",
            "sa_server_get_stats", "sa_server_kernel_times", "sa_autotune", "sa_free"]
@@ -192,6 +219,43 @@ class Context:
         _lib.sa_free(cig)
         return res
 
+    # sa_extend_batch job kinds (include/sa_hip.h)
+    EXT_SSW, EXT_HAMMING, EXT_PIECEWISE, EXT_GLOBAL, EXT_XDROP, EXT_XDROP_REV = range(6)
+
+    def extend(self, jobs, scores=(2, 8, 12, 1, 10), xdrop=500, k=20):
+        """sa_extend_batch for [(kind, query, ref, anchors, padding)] (anchors: a list
+        of (ref_start, query_start) pairs, last anchor first, window coordinates);
+        returns [(found, score, qs, qe, rs, re, edit_distance, cigar_ops)] with the
+        score as a signed int."""
+        n = len(jobs)
+        q = b"".join(j[1] for j in jobs)
+        r = b"".join(j[2] for j in jobs)
+        qo, ro, flat = [0], [0], []
+        js = (ExtendJob * max(1, n))()
+        for i, (kind, a, b, anchors, padding) in enumerate(jobs):
+            qo.append(qo[-1] + len(a))
+            ro.append(ro[-1] + len(b))
+            js[i].kind, js[i].padding = kind, padding
+            js[i].anchor_offset, js[i].n_anchors = len(flat) // 2, len(anchors or ())
+            for rs, qs in anchors or ():
+                flat += [rs, qs]
+        sc = Scores(*scores, xdrop)
+        an = (ctypes.c_uint32 * max(1, len(flat)))(*flat)
+        out = (Alignment * max(1, n))()
+        cig = ctypes.POINTER(ctypes.c_uint32)()
+        coff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_extend_batch(self._p, ctypes.byref(sc), ctypes.c_uint32(k), js, ctypes.c_uint32(n), q,
+                                    (ctypes.c_uint64 * (n + 1))(*qo), r, (ctypes.c_uint64 * (n + 1))(*ro), an,
+                                    ctypes.c_uint64(len(flat) // 2), out, ctypes.byref(cig), coff))
+        res = []
+        for i in range(n):
+            a = out[i]
+            ops = [cig[x] for x in range(coff[i], coff[i + 1])]
+            score = a.score - (1 << 32) if a.score >= (1 << 31) else a.score
+            res.append((a.found, score, a.query_start, a.query_end, a.ref_start, a.ref_end, a.edit_distance, ops))
+        _lib.sa_free(cig)
+        return res
+
     def hbm_copy_gbps(self, nbytes=1 << 30, iters=20):
         """Read+write GB/s of a plain device copy (sa_hbm_copy_probe)."""
         v = ctypes.c_double()
@@ -208,10 +272,24 @@ class Context:
 class Index:
     """Device-resident reference + randstrobe index (make_index, src/indexer.rs:18-170)."""
 
-    def __init__(self, ctx, fasta, read_length, bits=0, threads=4, device_build=False):
-        """device_build: run make_index on the device (sa_index_build_device)."""
+    def __init__(self, ctx, fasta, read_length, bits=0, threads=4, device_build=False, sti=None, upload=None):
+        """device_build: run make_index on the device (sa_index_build_device);
+        sti: load this .sti file instead of building (sa_index_load_sti);
+        upload: (params, entries bytes, bucket starts, packed words, contig word
+        offsets, contig lengths, contig names) for sa_index_upload."""
         self._p = ctypes.c_void_p()
-        if device_build:
+        if upload is not None:
+            prm, ent, bs, words, woff, lens, names = upload
+            nm = (ctypes.c_char_p * max(1, len(names)))(*[x.encode() for x in names])
+            _check(_lib.sa_index_upload(ctx._p, ctypes.byref(prm), ent, ctypes.c_uint64(len(ent) // 16),
+                                        (ctypes.c_uint64 * len(bs))(*bs), (ctypes.c_uint64 * max(1, len(words)))(*words),
+                                        ctypes.c_uint64(len(words)), (ctypes.c_uint64 * len(woff))(*woff),
+                                        (ctypes.c_uint64 * max(1, len(lens)))(*lens), nm, ctypes.c_uint32(len(lens)),
+                                        ctypes.byref(self._p)))
+        elif sti is not None:
+            _check(_lib.sa_index_load_sti(ctx._p, fasta.encode(), sti.encode(), ctypes.c_uint64(read_length), bits,
+                                          ctypes.byref(self._p)))
+        elif device_build:
             _check(_lib.sa_index_build_device(ctx._p, fasta.encode(), ctypes.c_uint64(read_length), bits,
                                               ctypes.byref(self._p)))
         else:
@@ -229,6 +307,10 @@ class Index:
         c = ctypes.c_uint32()
         _check(_lib.sa_index_info(self._p, ctypes.byref(p), ctypes.byref(n), ctypes.byref(c)))
         return p, n.value, c.value
+
+    def save_sti(self, path, read_length):
+        """Writes the index as a .sti file (sa_index_save_sti)."""
+        _check(_lib.sa_index_save_sti(self._p, ctypes.c_uint64(read_length), path.encode()))
 
     def download(self):
         """(entries as bytes, 16 per entry; bucket starts as a list) copied from the device."""
@@ -298,6 +380,74 @@ class Index:
                         [(rc[o + x].hash, rc[o + x].hash_revcomp, rc[o + x].start, rc[o + x].end)
                          for x in range(counts[2 * i + 1])]))
         return res
+
+    def map_batch(self, seqs, opts, speculative=3, stream=None):
+        """sa_map_batch: per read, its NAMs (as nams() gives them) and for each NAM
+        None (not extended) or (found, gapped, score, ref_start, length,
+        query_start, query_end, edit_distance, cigar_ops)."""
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        p = ctypes.POINTER(Nam)()
+        e = ctypes.POINTER(NamExtension)()
+        c = ctypes.POINTER(ctypes.c_uint32)()
+        noff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_map_batch(self._p, ctypes.byref(opts), data, off, ctypes.c_uint32(n),
+                                 ctypes.c_uint32(speculative), ctypes.c_void_p(stream), ctypes.byref(p), noff,
+                                 ctypes.byref(e), ctypes.byref(c)))
+        nams, exts = [], []
+        for i in range(n):
+            nams.append([(p[x].nam_id, p[x].ref_id, p[x].ref_start, p[x].ref_end, p[x].query_start, p[x].query_end,
+                          p[x].matching_bases, p[x].n_anchors, p[x].score, p[x].flags & 1)
+                         for x in range(noff[i], noff[i + 1])])
+            row = []
+            for x in range(noff[i], noff[i + 1]):
+                a = e[x]
+                if not a.extended:
+                    row.append(None)
+                    continue
+                ops = [c[y] for y in range(a.cigar_offset, a.cigar_offset + a.n_cigar)] if a.found else []
+                row.append((a.found, a.gapped, a.score, a.ref_start, a.length, a.query_start, a.query_end,
+                            a.edit_distance, ops))
+            exts.append(row)
+        for q in (p, e, c):
+            _lib.sa_free(q)
+        return nams, exts
+
+    def find_hits(self, seqs, mcs=0, rescue_distance=100):
+        """sa_find_hits_batch: per read orientation (2 per read, forward first) a
+        list of hits (query_start, query_end, position, hash, hash_revcomp,
+        is_partial, is_filtered) and the details tuple."""
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        p = ctypes.POINTER(HitRec)()
+        hoff = (ctypes.c_uint64 * (2 * n + 1))()
+        det = (HitsDetails * max(1, 2 * n))()
+        _check(_lib.sa_find_hits_batch(self._p, data, off, ctypes.c_uint32(n), mcs, rescue_distance, ctypes.byref(p),
+                                       hoff, det))
+        hits = [[(p[x].query_start, p[x].query_end, p[x].position, p[x].hash, p[x].hash_revcomp, p[x].is_partial,
+                  p[x].is_filtered) for x in range(hoff[t], hoff[t + 1])] for t in range(2 * n)]
+        dets = [tuple(getattr(det[t], f) for f, _ in HitsDetails._fields_) for t in range(2 * n)]
+        _lib.sa_free(p)
+        return hits, dets
+
+    def chain_raw(self, seqs, hits, dets):
+        """sa_chain_batch over hits and details as find_hits returns them: (NAM
+        records as bytes, the n + 1 offsets), like nams_raw."""
+        data, off = self._pack(seqs)
+        n = len(seqs)
+        flat = [h for t in hits for h in t]
+        arr = (HitRec * max(1, len(flat)))(*[HitRec(*h) for h in flat])
+        hoff = [0]
+        for t in hits:
+            hoff.append(hoff[-1] + len(t))
+        det = (HitsDetails * max(1, 2 * n))(*[HitsDetails(*d) for d in dets])
+        p = ctypes.POINTER(Nam)()
+        noff = (ctypes.c_uint64 * (n + 1))()
+        _check(_lib.sa_chain_batch(self._p, data, off, ctypes.c_uint32(n), arr, (ctypes.c_uint64 * (2 * n + 1))(*hoff),
+                                   det, ctypes.byref(p), noff))
+        raw = ctypes.string_at(p, 48 * noff[n])
+        _lib.sa_free(p)
+        return raw, list(noff)
 
     def nams_raw(self, seqs, mcs=0, rescue_distance=100):
         """sa_find_nams_batch: (the NAM records as bytes, 48 per record; the n + 1 offsets)."""

@@ -284,6 +284,100 @@ int sa_index_upload(sa_ctx* ctx, const sa_seeding_params* params, const void* en
     });
 }
 
+int sa_index_save_sti(const sa_index* index, uint64_t read_length, const char* path) {
+    if (!index || !path) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        const sa::SeedParams& p = index->p;
+        const uint64_t n = p.n_entries, nb = (1ull << p.bits) + 1;
+        std::vector<sa::RefEntry> e(std::max<uint64_t>(n, 1));
+        std::vector<uint64_t> b(nb);
+        index->eng->download_index(e.data(), b.data());
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return fail(SA_ERR_IO, std::string("cannot create ") + path);
+        bool ok = true;
+        auto put = [&](const void* x, size_t bytes) { ok = ok && std::fwrite(x, 1, bytes, f) == bytes; };
+        auto u32 = [&](uint32_t v) { put(&v, 4); };
+        auto u64 = [&](uint64_t v) { put(&v, 8); };
+        put("STI\x01", 4);
+        u32(7);  // STI_FILE_FORMAT_VERSION
+        u64(8);  // reserved chunk: its length, then its contents
+        u64(0);
+        u32((uint32_t)p.filter_cutoff);
+        u32((uint32_t)p.bits);
+        u32(sa::profile_for_read_length(read_length));
+        u32((uint32_t)p.k);
+        u32((uint32_t)p.s);
+        u32((uint32_t)p.w_min);
+        u32((uint32_t)p.w_max);
+        u32((uint32_t)p.q);
+        u32((uint32_t)p.max_dist);
+        u64(p.main_mask);
+        u64(n);
+        put(e.data(), n * sizeof(sa::RefEntry));
+        u64(nb);
+        put(b.data(), nb * 8);
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) return fail(SA_ERR_IO, std::string("cannot write ") + path);
+        return SA_OK;
+    });
+}
+
+int sa_index_load_sti(sa_ctx* ctx, const char* fasta_path, const char* sti_path, uint64_t read_length, int bits,
+                      sa_index** out) {
+    if (!ctx || !fasta_path || !sti_path || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        auto ix = std::make_unique<sa_index>();
+        ix->device = ctx->device;
+        try {
+            ix->ref = sa::read_fasta(fasta_path);
+        } catch (const std::exception& e) {
+            return fail(SA_ERR_IO, e.what());
+        }
+        sa::SeedParams sp = sa::seed_params_for_read_length(read_length);
+        if (bits <= 0) bits = sa::pick_bits(ix->ref, sp);
+        std::unique_ptr<FILE, int (*)(FILE*)> f(std::fopen(sti_path, "rb"), &std::fclose);
+        if (!f) return fail(SA_ERR_IO, std::string("cannot open ") + sti_path);
+        auto read = [&](void* dst, uint64_t bytes) {
+            if (std::fread(dst, 1, bytes, f.get()) != bytes)
+                throw std::runtime_error(std::string("When reading the .sti (index) file: unexpected end of ") +
+                                         sti_path);
+        };
+        auto u32 = [&] { uint32_t v; read(&v, 4); return v; };
+        auto u64 = [&] { uint64_t v; read(&v, 8); return v; };
+        // src/index.rs:412-495
+        char magic[4];
+        read(magic, 4);
+        if (std::memcmp(magic, "STI\x01", 4) != 0)
+            return fail(SA_ERR_IO, "Signature at the beginning of the .sti file is incorrect.");
+        const uint32_t ver = u32();
+        if (ver != 7)
+            return fail(SA_ERR_IO, "The .sti file format version is " + std::to_string(ver) +
+                                       ", but this version of strobealign can only read files that have version 7");
+        std::vector<uint8_t> reserved(u64());
+        if (!reserved.empty()) read(reserved.data(), reserved.size());
+        const uint32_t filter_cutoff = u32();
+        const uint32_t sti_bits = u32();
+        const uint32_t profile = u32();
+        const uint32_t k = u32(), s = u32(), w_min = u32(), w_max = u32(), q = u32(), max_dist = u32();
+        const uint64_t main_mask = u64();
+        if (profile > 7) return fail(SA_ERR_IO, "The .sti (index) file uses an unknown profile");
+        if ((int)sti_bits != bits || profile != sa::profile_for_read_length(read_length) || (int)k != sp.k ||
+            (int)s != sp.s || (int)w_min != sp.w_min || (int)w_max != sp.w_max || q != sp.q ||
+            (int)max_dist != sp.max_dist || main_mask != sp.main_mask)
+            return fail(SA_ERR_ARG, "Index parameters in .sti file and those specified on command line differ");
+        sp.bits = bits;
+        sp.filter_cutoff = filter_cutoff;
+        const uint64_t n = u64();
+        ix->eng = std::make_unique<sa::Engine>(ctx->device);
+        ix->eng->upload_reference(ix->ref.words.data(), ix->ref.words.size(), ix->ref.word_off.data(),
+                                  ix->ref.len.data(), (uint32_t)ix->ref.names.size());
+        ix->eng->upload_index_streamed(n, sp, read);
+        ix->p = ix->eng->params();
+        *out = ix.release();
+        return SA_OK;
+    });
+}
+
 void sa_index_destroy(sa_index* index) { delete index; }
 
 int sa_index_info(const sa_index* index, sa_seeding_params* params, uint64_t* n_entries, uint32_t* n_contigs) {
@@ -598,6 +692,169 @@ int sa_find_nams_batch(sa_index* index, const uint8_t* seqs, const uint64_t* off
     });
 }
 
+int sa_map_batch(sa_index* index, const sa_map_options* opt, const uint8_t* seqs, const uint64_t* offsets, uint32_t n,
+                 uint32_t speculative, void* stream, sa_nam** nams, uint64_t* nam_offsets, sa_nam_extension** ext,
+                 uint32_t** cigars) {
+    if (!index || (!seqs && n) || !offsets || !nams || !nam_offsets || !ext || !cigars)
+        return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        const sa::Options o = options_of(opt);
+        sa::Engine& eng = *index->eng;
+        // the caller's stream for the duration of the call
+        struct StreamScope {
+            sa::Engine& e;
+            ihipStream_t* prev;
+            ~StreamScope() { e.set_stream(prev); }
+        } scope{eng, eng.set_stream((ihipStream_t*)stream)};
+        eng.upload_reads(seqs, offsets, n);
+        sa::NamBatch nb;
+        sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+        eng.find_nams(sa::MapParams{o.mcs, o.rescue_distance}, cp, nb);
+        const uint64_t N = nb.nams.size();
+        sa_nam* pn = (sa_nam*)std::malloc(std::max<uint64_t>(1, N) * sizeof(sa_nam));
+        sa_nam_extension* pe = (sa_nam_extension*)std::calloc(std::max<uint64_t>(1, N), sizeof(sa_nam_extension));
+        if (!pn || !pe) {
+            std::free(pn);
+            std::free(pe);
+            throw std::bad_alloc();
+        }
+        if (N) std::memcpy(pn, nb.nams.data(), N * sizeof(sa_nam));
+        for (uint32_t i = 0; i <= n; ++i) nam_offsets[i] = nb.nam_off[i];
+        // extend_seed requests (the mapping path's EXT_SEED) for the first
+        // `speculative` consistent NAMs of each read
+        std::vector<sa::ExtRequest> req;
+        std::vector<uint64_t> which;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t done = 0;
+            for (uint64_t x = nb.nam_off[i]; x < nb.nam_off[i + 1] && done < speculative; ++x) {
+                const sa::NamRec& r = nb.nams[x];
+                if (!(r.flags & 2)) continue;  // not consistent as it is
+                ++done;
+                sa::ExtRequest q;
+                std::memset(&q, 0, sizeof q);
+                q.kind = sa::EXT_SEED;
+                q.read_id = i;
+                q.flags = (o.use_ssw ? 1u : 0u) | 2u | ((r.flags & 1) ? 4u : 0u);
+                q.ref_id = r.ref_id;
+                q.ref_start = r.ref_start;
+                q.ref_end = r.ref_end;
+                q.query_start = r.query_start;
+                q.query_end = r.query_end;
+                q.anchor_off = r.anchor_off;
+                q.n_anchors = r.n_anchors;
+                req.push_back(q);
+                which.push_back(x);
+            }
+        }
+        std::vector<sa::ExtResult> res;
+        std::vector<uint32_t> cig;
+        std::vector<uint64_t> coff;
+        eng.extend(req.data(), (uint32_t)req.size(), o.scores, res, cig, coff);
+        for (size_t y = 0; y < req.size(); ++y) {
+            const sa::ExtResult& a = res[y];
+            sa_nam_extension& e = pe[which[y]];
+            e.extended = 1;
+            e.found = a.found;
+            if (!a.found) continue;
+            e.gapped = a.gapped;
+            e.score = a.score;
+            e.ref_start = a.ref_start;
+            e.length = a.length;
+            e.query_start = a.query_start;
+            e.query_end = a.query_end;
+            e.edit_distance = a.edit_distance;
+            e.cigar_offset = coff[y];
+            e.n_cigar = a.n_cigar;
+        }
+        uint32_t* pc = (uint32_t*)std::malloc(std::max<size_t>(1, cig.size()) * 4);
+        if (!pc) {
+            std::free(pn);
+            std::free(pe);
+            throw std::bad_alloc();
+        }
+        if (!cig.empty()) std::memcpy(pc, cig.data(), cig.size() * 4);
+        *nams = pn;
+        *ext = pe;
+        *cigars = pc;
+        return SA_OK;
+    });
+}
+
+int sa_find_hits_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, uint32_t mcs,
+                       uint32_t rescue_distance, sa_hit** hits, uint64_t* hit_offsets, sa_hits_details* details) {
+    if (!index || (!seqs && n) || !offsets || !hits || !hit_offsets || !details) return fail(SA_ERR_ARG, "NULL argument");
+    if (mcs > 3) return fail(SA_ERR_ARG, "mcs must be 0..3");
+    return guard([&] {
+        index->eng->upload_reads(seqs, offsets, n);
+        std::vector<sa::Hit> h;
+        std::vector<sa::HitCounts> hc;
+        index->eng->find_hits(sa::MapParams{mcs, rescue_distance}, h, hc);
+        uint64_t total = 0;
+        for (uint32_t t = 0; t < 2 * n; ++t) total += hc[t].n_hits;
+        sa_hit* p = (sa_hit*)std::malloc(std::max<uint64_t>(1, total) * sizeof(sa_hit));
+        if (!p) throw std::bad_alloc();
+        uint64_t at = 0;
+        for (uint32_t t = 0; t < 2 * n; ++t) {
+            const uint64_t ro = offsets[t / 2] - offsets[0], L = offsets[t / 2 + 1] - offsets[t / 2];
+            const sa::Hit* src = h.data() + 2 * ro + (t & 1) * L;
+            hit_offsets[t] = at;
+            for (uint32_t x = 0; x < hc[t].n_hits; ++x) {
+                const sa::Hit& a = src[x];
+                p[at++] = sa_hit{a.query_start, a.query_end, a.position, a.hash, a.hash_rc,
+                                 (a.flags & sa::HIT_PARTIAL) ? 1u : 0u, (a.flags & sa::HIT_FILTERED) ? 1u : 0u};
+            }
+            const sa::HitCounts& c = hc[t];
+            details[t] = sa_hits_details{c.full_not_found, c.full_filtered, c.full_found, c.partial_not_found,
+                                         c.partial_filtered, c.partial_found, c.rescued};
+        }
+        hit_offsets[2 * n] = at;
+        *hits = p;
+        return SA_OK;
+    });
+}
+
+int sa_chain_batch(sa_index* index, const uint8_t* seqs, const uint64_t* offsets, uint32_t n, const sa_hit* hits,
+                   const uint64_t* hit_offsets, const sa_hits_details* details, sa_nam** nams, uint64_t* nam_offsets) {
+    if (!index || (!seqs && n) || !offsets || (!hits && n) || !hit_offsets || !details || !nams || !nam_offsets)
+        return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        // device layout: orientation t of read i at 2 * off[i] + t % 2 * L
+        const uint64_t nb = offsets[n] - offsets[0];
+        std::vector<sa::Hit> h(2 * nb + 1);
+        std::vector<sa::HitCounts> hc(2ull * n);
+        for (uint32_t t = 0; t < 2 * n; ++t) {
+            const uint64_t ro = offsets[t / 2] - offsets[0], L = offsets[t / 2 + 1] - offsets[t / 2];
+            const uint64_t cnt = hit_offsets[t + 1] - hit_offsets[t];
+            if (hit_offsets[t + 1] < hit_offsets[t] || cnt > L)
+                throw std::invalid_argument("read orientation " + std::to_string(t) +
+                                            " has more hits than its read has bases");
+            for (uint64_t x = 0; x < cnt; ++x) {
+                const sa_hit& a = hits[hit_offsets[t] + x];
+                sa::Hit& d = h[2 * ro + (t & 1) * L + x];
+                d = sa::Hit{a.hash, a.hash_revcomp, a.position, sa::NONE64, 0, (uint32_t)a.query_start,
+                            (uint32_t)a.query_end,
+                            (a.is_partial ? sa::HIT_PARTIAL : 0u) | (a.is_filtered ? sa::HIT_FILTERED : 0u)};
+            }
+            const sa_hits_details& c = details[t];
+            hc[t] = sa::HitCounts{(uint32_t)c.full_not_found, (uint32_t)c.full_filtered, (uint32_t)c.full_found,
+                                  (uint32_t)c.partial_not_found, (uint32_t)c.partial_filtered,
+                                  (uint32_t)c.partial_found, (uint32_t)c.rescued, (uint32_t)cnt};
+        }
+        std::vector<uint64_t> off(offsets, offsets + n + 1);
+        for (auto& x : off) x -= offsets[0];
+        index->eng->upload_reads(seqs + offsets[0], off.data(), n);
+        sa::NamBatch nb_;
+        sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
+        index->eng->chain_hits(h.data(), hc.data(), cp, nb_);
+        sa_nam* p = (sa_nam*)std::malloc(std::max<size_t>(1, nb_.nams.size()) * sizeof(sa_nam));
+        if (!p) throw std::bad_alloc();
+        if (!nb_.nams.empty()) std::memcpy(p, nb_.nams.data(), nb_.nams.size() * sizeof(sa_nam));
+        for (uint32_t i = 0; i <= n; ++i) nam_offsets[i] = nb_.nam_off[i];
+        *nams = p;
+        return SA_OK;
+    });
+}
+
 int sa_find_nams_batch_noalloc(sa_index* index, const uint8_t* d_seqs, const uint64_t* d_offsets, uint32_t n,
                                uint32_t mcs, uint32_t rescue_distance, sa_nam* d_nams, uint64_t nam_cap,
                                uint64_t* d_nam_offsets, uint64_t* n_nams, void* stream) {
@@ -615,64 +872,121 @@ int sa_find_nams_batch_noalloc(sa_index* index, const uint8_t* d_seqs, const uin
     });
 }
 
+namespace {
+// Runs n extension jobs of the aligner's entry points on the device: query i
+// against reference window i (ACGT; each window becomes its own contig of a
+// reference made for the call), through the same kernels as the mapping path.
+int extend_jobs(sa_ctx* ctx, const sa::Scores& sc, uint32_t k, const sa_extend_job* jobs, uint32_t n,
+                const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
+                const uint64_t* ref_offsets, const uint32_t* anchors, uint64_t n_anchor_pairs, sa_alignment* out,
+                uint32_t** cigars, uint64_t* cigar_offsets) {
+    std::vector<uint64_t> words, woff{0}, len;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t L = ref_offsets[i + 1] - ref_offsets[i];
+        for (uint64_t x = 0; x < L; ++x) {
+            const uint32_t c = sa::nt2(refs[ref_offsets[i] + x]);
+            if (c > 3) throw std::invalid_argument("reference windows must be ACGT");
+            if ((x & 31) == 0) words.push_back(0);
+            words.back() |= (uint64_t)c << ((x & 31) * 2);
+        }
+        woff.push_back(words.size());
+        len.push_back(L);
+    }
+    sa::Engine eng(ctx->device);
+    eng.upload_reference(words.data(), words.size(), woff.data(), len.data(), n);
+    sa::SeedParams sp = sa::seed_params_for_read_length(150);
+    sp.k = (int32_t)k;
+    uint64_t bs[3] = {0, 0, 0};
+    sp.bits = 1;
+    eng.upload_index(nullptr, 0, bs, sp);
+    eng.upload_reads(queries, query_offsets, n);
+    eng.set_chain_anchors(anchors, n_anchor_pairs);
+    std::vector<sa::ExtRequest> req(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const sa_extend_job& j = jobs[i];
+        sa::ExtRequest& r = req[i];
+        std::memset(&r, 0, sizeof r);
+        r.read_id = i;
+        r.ref_id = i;
+        r.win_start = 0;
+        r.win_end = (uint32_t)len[i];
+        switch (j.kind) {
+            case SA_EXT_SSW:
+                r.kind = sa::EXT_RESCUE;
+                r.flags = 8;  // plain alignment: no shared-substring precheck, query as given
+                break;
+            case SA_EXT_HAMMING: r.kind = sa::EXT_HAMMING; break;
+            case SA_EXT_PIECEWISE:
+                if (j.anchor_offset + j.n_anchors > n_anchor_pairs || j.n_anchors == 0)
+                    throw std::invalid_argument("piecewise job anchors outside the anchor array");
+                r.kind = sa::EXT_ALIGN_PW;
+                r.anchor_off = j.anchor_offset;
+                r.n_anchors = j.n_anchors;
+                r.ref_start = j.padding;
+                break;
+            case SA_EXT_GLOBAL: r.kind = sa::EXT_PW_GLOBAL; break;
+            case SA_EXT_XDROP: r.kind = sa::EXT_PW_XDROP; break;
+            case SA_EXT_XDROP_REV:
+                r.kind = sa::EXT_PW_XDROP;
+                r.flags = 1;
+                break;
+            default: throw std::invalid_argument("unknown extension job kind " + std::to_string(j.kind));
+        }
+    }
+    std::vector<sa::ExtResult> res;
+    std::vector<uint32_t> cig;
+    std::vector<uint64_t> coff;
+    eng.extend(req.data(), n, sc, res, cig, coff);
+    for (uint32_t i = 0; i < n; ++i) {
+        out[i].found = res[i].found;
+        out[i].score = res[i].score;
+        out[i].query_start = res[i].query_start;
+        out[i].query_end = res[i].query_end;
+        out[i].ref_start = res[i].ref_start;
+        out[i].ref_end = res[i].ref_start + res[i].length;
+        out[i].edit_distance = res[i].edit_distance;
+        out[i].n_cigar = res[i].found ? res[i].n_cigar : 0;
+        cigar_offsets[i] = coff[i];
+    }
+    cigar_offsets[n] = coff[n];
+    uint32_t* p = (uint32_t*)std::malloc(std::max<size_t>(1, cig.size()) * 4);
+    if (!p) throw std::bad_alloc();
+    if (!cig.empty()) std::memcpy(p, cig.data(), cig.size() * 4);
+    *cigars = p;
+    return SA_OK;
+}
+}  // namespace
+
+int sa_extend_batch(sa_ctx* ctx, const sa_scores* scores, uint32_t k, const sa_extend_job* jobs, uint32_t n,
+                    const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
+                    const uint64_t* ref_offsets, const uint32_t* anchors, uint64_t n_anchor_pairs, sa_alignment* out,
+                    uint32_t** cigars, uint64_t* cigar_offsets) {
+    if (!ctx || !scores || (!jobs && n) || !query_offsets || !ref_offsets || !out || !cigars || !cigar_offsets ||
+        (!anchors && n_anchor_pairs))
+        return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        const sa::Scores sc{scores->match_score, scores->mismatch_penalty, scores->gap_open_penalty,
+                            scores->gap_extend_penalty, scores->end_bonus, scores->xdrop};
+        for (int32_t v : {sc.match, sc.mismatch, sc.gap_open, sc.gap_extend})
+            if (v < 0 || v > 127) throw std::invalid_argument("scores and penalties must be in 0..127");
+        if (k == 0 || k > 32) throw std::invalid_argument("k must be in 1..32");
+        return extend_jobs(ctx, sc, k, jobs, n, queries, query_offsets, refs, ref_offsets, anchors, n_anchor_pairs,
+                           out, cigars, cigar_offsets);
+    });
+}
+
 int sa_align_ssw_batch(sa_ctx* ctx, const uint8_t* queries, const uint64_t* query_offsets, const uint8_t* refs,
                        const uint64_t* ref_offsets, uint32_t n, sa_alignment* out, uint32_t** cigars,
                        uint64_t* cigar_offsets) {
     if (!ctx || !query_offsets || !ref_offsets || !out || !cigars || !cigar_offsets) return fail(SA_ERR_ARG, "NULL argument");
     return guard([&] {
-        // references are packed as one contig per pair (ACGT only)
-        std::vector<uint64_t> words, woff{0}, len;
-        for (uint32_t i = 0; i < n; ++i) {
-            uint64_t L = ref_offsets[i + 1] - ref_offsets[i];
-            for (uint64_t x = 0; x < L; ++x) {
-                uint32_t c = sa::nt2(refs[ref_offsets[i] + x]);
-                if (c > 3) { throw std::invalid_argument("reference windows must be ACGT"); }
-                if ((x & 31) == 0) words.push_back(0);
-                words.back() |= (uint64_t)c << ((x & 31) * 2);
-            }
-            woff.push_back(words.size());
-            len.push_back(L);
+        std::vector<sa_extend_job> jobs(n);
+        for (auto& j : jobs) {
+            std::memset(&j, 0, sizeof j);
+            j.kind = SA_EXT_SSW;
         }
-        sa::Engine eng(ctx->device);
-        eng.upload_reference(words.data(), words.size(), woff.data(), len.data(), n);
-        sa::SeedParams sp = sa::seed_params_for_read_length(150);
-        uint64_t bs[3] = {0, 0, 0};
-        sp.bits = 1;
-        eng.upload_index(nullptr, 0, bs, sp);
-        eng.upload_reads(queries, query_offsets, n);
-        std::vector<sa::ExtRequest> req(n);
-        for (uint32_t i = 0; i < n; ++i) {
-            sa::ExtRequest& r = req[i];
-            std::memset(&r, 0, sizeof r);
-            r.kind = sa::EXT_RESCUE;
-            r.read_id = i;
-            r.flags = 8;  // plain alignment: no shared-substring precheck, query as given
-            r.ref_id = i;
-            r.win_start = 0;
-            r.win_end = (uint32_t)len[i];
-        }
-        std::vector<sa::ExtResult> res;
-        std::vector<uint32_t> cig;
-        std::vector<uint64_t> coff;
-        sa::Scores sc{2, 8, 12, 1, 10, 500};
-        eng.extend(req.data(), n, sc, res, cig, coff);
-        for (uint32_t i = 0; i < n; ++i) {
-            out[i].found = res[i].found;
-            out[i].score = res[i].score;
-            out[i].query_start = res[i].query_start;
-            out[i].query_end = res[i].query_end;
-            out[i].ref_start = res[i].ref_start;
-            out[i].ref_end = res[i].ref_start + res[i].length;
-            out[i].edit_distance = res[i].edit_distance;
-            out[i].n_cigar = res[i].found ? res[i].n_cigar : 0;
-            cigar_offsets[i] = coff[i];
-        }
-        cigar_offsets[n] = coff[n];
-        uint32_t* p = (uint32_t*)std::malloc(std::max<size_t>(1, cig.size()) * 4);
-        if (!p) throw std::bad_alloc();
-        if (!cig.empty()) std::memcpy(p, cig.data(), cig.size() * 4);
-        *cigars = p;
-        return SA_OK;
+        return extend_jobs(ctx, sa::Scores{2, 8, 12, 1, 10, 500}, 20, jobs.data(), n, queries, query_offsets, refs,
+                           ref_offsets, nullptr, 0, out, cigars, cigar_offsets);
     });
 }
 

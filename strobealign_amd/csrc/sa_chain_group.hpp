@@ -29,17 +29,27 @@
 
 namespace sa {
 
-// Orientations deferred by the first chaining tier, per later tier.
+// Orientations deferred by the first chaining tier, per later tier: list[x]
+// (count[x] entries) holds those for tier x + 2, whose anchor capacities are
+// (cap[x], cap[x + 1]]; the last list goes to the thread-per-orientation form.
+constexpr int CHAIN_TIERS = 4;
 struct ChainTiers {
-    uint32_t* list[2];   // tier 2 (wave, LDS) and tier 3 (thread, global memory)
-    uint32_t* count;     // count[0], count[1]
-    uint64_t cap1, cap2; // anchor capacities handled by tiers 1 and 2
+    uint32_t* list[CHAIN_TIERS - 1];
+    uint32_t* count;
+    uint64_t cap[CHAIN_TIERS - 1];  // anchor capacities handled by the LDS tiers 1..3
 };
 
+
+
+// An anchor in LDS: (ref_id, ref_start, query_start) and its chaining score,
+// read and written as one 16-byte access.
+struct alignas(16) AnchorL {
+    uint32_t ref, rs, q;
+    float dp;
+};
 template <int CAP>
 struct ChainLds {
-    uint32_t ref[CAP], rs[CAP], q[CAP];
-    float dp[CAP];
+    AnchorL a[CAP];
     uint16_t pred[CAP], cand[CAP], order[CAP];
     uint8_t used[CAP];
 };
@@ -51,8 +61,22 @@ __device__ __forceinline__ void lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// DPP move of a 32-bit value (lanes whose source lies outside the 16-lane row
+// keep `old`)
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float old, float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
 // Collective operations of one group of G consecutive lanes (every lane of the
-// group takes part; control flow around them is uniform within the group).
+// group takes part; control flow around them is uniform within the group). A
+// 16-lane group is a DPP row: its scans and reductions are row shifts and
+// butterflies in the VALU; a 64-lane group uses cross-lane permutes.
 template <int G>
 struct Grp {
     uint32_t r;     // lane within the group
@@ -63,25 +87,84 @@ struct Grp {
     }
     __device__ __forceinline__ uint32_t excl_sum(uint32_t v) const {
         uint32_t x = v;
-        for (int d = 1; d < G; d <<= 1) {
-            const uint32_t y = __shfl_up(x, d, G);
-            if (r >= (uint32_t)d) x += y;
+        if constexpr (G == 8) {
+            // row shifts cross into the other half-row: lanes below d take 0
+            uint32_t y = dpp_u<0x111>(0, x);
+            x += r >= 1 ? y : 0u;
+            y = dpp_u<0x112>(0, x);
+            x += r >= 2 ? y : 0u;
+            y = dpp_u<0x114>(0, x);
+            x += r >= 4 ? y : 0u;
+        } else if constexpr (G == 16) {
+            x += dpp_u<0x111>(0, x);  // row_shr:1
+            x += dpp_u<0x112>(0, x);  // row_shr:2
+            x += dpp_u<0x114>(0, x);  // row_shr:4
+            x += dpp_u<0x118>(0, x);  // row_shr:8
+        } else {
+            for (int d = 1; d < G; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, G);
+                if (r >= (uint32_t)d) x += y;
+            }
         }
         return x - v;
     }
     __device__ __forceinline__ uint32_t sum(uint32_t v) const {
-        for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+        if constexpr (G == 8) {
+            v += dpp_u<0xB1>(0, v);   // quad_perm [1,0,3,2]
+            v += dpp_u<0x4E>(0, v);   // quad_perm [2,3,0,1]
+            v += dpp_u<0x141>(0, v);  // row_half_mirror
+        } else if constexpr (G == 16) {
+            v += dpp_u<0xB1>(0, v);   // quad_perm [1,0,3,2]
+            v += dpp_u<0x4E>(0, v);   // quad_perm [2,3,0,1]
+            v += dpp_u<0x141>(0, v);  // row_half_mirror
+            v += dpp_u<0x140>(0, v);  // row_mirror
+        } else {
+            for (int d = G / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, G);
+        }
         return v;
     }
     // maximum over the lanes before this one (-inf for the first)
     __device__ __forceinline__ float excl_max(float v) const {
+        const float NEG = -__builtin_huge_valf();
         float x = v;
-        for (int d = 1; d < G; d <<= 1) {
-            const float y = __shfl_up(x, d, G);
-            if (r >= (uint32_t)d) x = fmaxf(x, y);
+        if constexpr (G == 8) {
+            float y = dpp_f<0x111>(NEG, x);
+            x = fmaxf(x, r >= 1 ? y : NEG);
+            y = dpp_f<0x112>(NEG, x);
+            x = fmaxf(x, r >= 2 ? y : NEG);
+            y = dpp_f<0x114>(NEG, x);
+            x = fmaxf(x, r >= 4 ? y : NEG);
+            y = dpp_f<0x111>(NEG, x);
+            return r >= 1 ? y : NEG;
+        } else if constexpr (G == 16) {
+            x = fmaxf(x, dpp_f<0x111>(NEG, x));
+            x = fmaxf(x, dpp_f<0x112>(NEG, x));
+            x = fmaxf(x, dpp_f<0x114>(NEG, x));
+            x = fmaxf(x, dpp_f<0x118>(NEG, x));
+            return dpp_f<0x111>(NEG, x);
+        } else {
+            for (int d = 1; d < G; d <<= 1) {
+                const float y = __shfl_up(x, d, G);
+                if (r >= (uint32_t)d) x = fmaxf(x, y);
+            }
+            const float e = __shfl_up(x, 1, G);
+            return r == 0 ? NEG : e;
         }
-        const float e = __shfl_up(x, 1, G);
-        return r == 0 ? -__builtin_huge_valf() : e;
+    }
+    __device__ __forceinline__ float max(float v) const {
+        if constexpr (G == 8) {
+            v = fmaxf(v, dpp_f<0xB1>(v, v));
+            v = fmaxf(v, dpp_f<0x4E>(v, v));
+            v = fmaxf(v, dpp_f<0x141>(v, v));
+        } else if constexpr (G == 16) {
+            v = fmaxf(v, dpp_f<0xB1>(v, v));
+            v = fmaxf(v, dpp_f<0x4E>(v, v));
+            v = fmaxf(v, dpp_f<0x141>(v, v));
+            v = fmaxf(v, dpp_f<0x140>(v, v));
+        } else {
+            for (int d = G / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, G));
+        }
+        return v;
     }
     template <typename T>
     __device__ __forceinline__ T bcast(T v, uint32_t src) const {
@@ -93,23 +176,19 @@ SA_HD bool key_less(uint32_t ar, uint32_t as, uint32_t aq, uint32_t br, uint32_t
     return ar < br || (ar == br && (as < bs || (as == bs && aq < bq)));
 }
 
-// Anchors of a hit (src/chainer.rs:304-384) into the group's arrays at `pos`, or
+// Anchors of a hit (src/chainer.rs:304-384) into the group's array at `pos`, or
 // only counted when `write` is false.
 template <int CAP>
 __device__ uint32_t grp_hit_anchors(const IndexView& ix, const Hit& h, bool write, uint32_t pos, ChainLds<CAP>& L) {
     const uint32_t k = (uint32_t)ix.p.k;
     uint32_t c = 0;
     if (h.flags & HIT_PARTIAL) {
-        for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-            if (write) {
+        if (write)
+            for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
                 const RefEntry x = ix.e[j];
-                L.ref[pos + c] = x.ref_index;
-                L.rs[pos + c] = x.position;
-                L.q[pos + c] = h.query_start;
+                L.a[pos + c++] = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
             }
-            ++c;
-        }
-        return c;
+        return h.run_len;
     }
     uint32_t min_diff = 0xFFFFFFFFu;
     const uint32_t ql = h.query_end - h.query_start;
@@ -119,18 +198,18 @@ __device__ uint32_t grp_hit_anchors(const IndexView& ix, const Hit& h, bool writ
         const uint32_t diff = ql > rl ? ql - rl : rl - ql;
         if (diff <= min_diff) {
             if (write) {
-                L.ref[pos + c] = x.ref_index;
-                L.rs[pos + c] = x.position;
-                L.q[pos + c] = h.query_start;
-                L.ref[pos + c + 1] = x.ref_index;
-                L.rs[pos + c + 1] = x.position + rl - k;
-                L.q[pos + c + 1] = h.query_end - k;
+                L.a[pos + c] = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
+                L.a[pos + c + 1] = AnchorL{x.ref_index, x.position + rl - k, h.query_end - k, 0.0f};
             }
             c += 2;
             min_diff = diff;
         }
     }
     return c;
+}
+
+__device__ __forceinline__ bool anchor_lt(const AnchorL& a, const AnchorL& b) {
+    return a.ref < b.ref || (a.ref == b.ref && (a.rs < b.rs || (a.rs == b.rs && a.q < b.q)));
 }
 
 // One orientation t on one group: same outputs as item_chain.
@@ -155,9 +234,12 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         const uint32_t h = h0 + r;
         Hit hh;
         uint32_t c = 0;
-        const bool has = h < nh && H[h].run_start != NONE64;
-        if (has) {
+        bool has = false;
+        if (h < nh) {
             hh = H[h];
+            has = hh.run_start != NONE64;
+        }
+        if (has) {
             entries += hh.run_len;
             c = grp_hit_anchors<CAP>(ix, hh, false, 0, L);
         }
@@ -167,16 +249,11 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
     }
     entries = gp.sum(entries);
     if (na > CAP) na = CAP;  // capacity bound: cannot be exceeded (2 per entry at most)
-    lds_fence();
 
     // 2. bitonic sort, padded to a power of two with keys above every anchor
     uint32_t P = 1;
     while (P < na) P <<= 1;
-    for (uint32_t x = na + r; x < P; x += G) {
-        L.ref[x] = 0xFFFFFFFFu;
-        L.rs[x] = 0xFFFFFFFFu;
-        L.q[x] = 0xFFFFFFFFu;
-    }
+    for (uint32_t x = na + r; x < P; x += G) L.a[x] = AnchorL{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f};
     lds_fence();
     for (uint32_t k2 = 2; k2 <= P; k2 <<= 1) {
         for (uint32_t j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
@@ -184,11 +261,10 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
                 const uint32_t i1 = ((x & ~(j2 - 1)) << 1) | (x & (j2 - 1));
                 const uint32_t i2 = i1 + j2;
                 const bool up = (i1 & k2) == 0;
-                const uint32_t ar = L.ref[i1], as = L.rs[i1], aq = L.q[i1];
-                const uint32_t br = L.ref[i2], bs = L.rs[i2], bq = L.q[i2];
-                if (key_less(br, bs, bq, ar, as, aq) == up) {
-                    L.ref[i1] = br; L.rs[i1] = bs; L.q[i1] = bq;
-                    L.ref[i2] = ar; L.rs[i2] = as; L.q[i2] = aq;
+                const AnchorL A = L.a[i1], B = L.a[i2];
+                if (anchor_lt(B, A) == up) {
+                    L.a[i1] = B;
+                    L.a[i2] = A;
                 }
             }
             lds_fence();
@@ -197,36 +273,33 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
 
     // 3. dedup (equal anchors are adjacent): compaction in place
     uint32_t m = 0;
-    uint32_t pr = 0xFFFFFFFFu, ps = 0xFFFFFFFFu, pq = 0xFFFFFFFFu;  // last anchor of the previous chunk
+    AnchorL prev{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f};  // last anchor of the previous chunk
     for (uint32_t x0 = 0; x0 < na; x0 += G) {
         const uint32_t x = x0 + r;
-        uint32_t er = 0, es = 0, eq = 0;
-        if (x < na) { er = L.ref[x]; es = L.rs[x]; eq = L.q[x]; }
-        uint32_t qr_ = __shfl_up(er, 1, G), qs_ = __shfl_up(es, 1, G), qq_ = __shfl_up(eq, 1, G);
-        if (r == 0) { qr_ = pr; qs_ = ps; qq_ = pq; }
-        const bool keep = x < na && (x == 0 || er != qr_ || es != qs_ || eq != qq_);
-        pr = gp.bcast(er, G - 1);
-        ps = gp.bcast(es, G - 1);
-        pq = gp.bcast(eq, G - 1);
+        AnchorL e{0, 0, 0, 0.0f};
+        if (x < na) e = L.a[x];
+        uint32_t pr = __shfl_up(e.ref, 1, G), ps = __shfl_up(e.rs, 1, G), pq = __shfl_up(e.q, 1, G);
+        if (r == 0) { pr = prev.ref; ps = prev.rs; pq = prev.q; }
+        const bool keep = x < na && (x == 0 || e.ref != pr || e.rs != ps || e.q != pq);
+        prev.ref = gp.bcast(e.ref, G - 1);
+        prev.rs = gp.bcast(e.rs, G - 1);
+        prev.q = gp.bcast(e.q, G - 1);
         const uint32_t pos = m + gp.excl_sum(keep ? 1u : 0u);
         lds_fence();
-        if (keep) { L.ref[pos] = er; L.rs[pos] = es; L.q[pos] = eq; }
+        if (keep) L.a[pos] = e;
         m += gp.sum(keep ? 1u : 0u);
         lds_fence();
     }
 
     // 4. collinear chaining DP (src/chainer.rs:85-180)
     const uint64_t max_ref_gap = cp.max_ref_gap >= 0 ? (uint64_t)cp.max_ref_gap : L_read;
-    auto score_of = [&](uint32_t dq, uint32_t dr) {
-        if (dq == dr && dq < 1024) return same_diag[dq];
-        return chain_score(dq, dr, k, cp, log2tab);
-    };
+    const RatioTest rt = make_ratio_test(cp.max_diagonal_ratio);
     const float NEG = -__builtin_huge_valf();
     float best = 0.0f;
-    uint32_t best_i = NONE32, b_ref = 0, b_rs = 0, b_q = 0;
-    float b_dp = 0.0f;
+    uint32_t best_i = NONE32;
+    AnchorL bA{0, 0, 0, 0.0f};
     for (uint32_t i = 0; i < m; ++i) {
-        const uint32_t ai_ref = L.ref[i], ai_rs = L.rs[i], ai_q = L.q[i];
+        const AnchorL ai = L.a[i];
         float dpi = (float)k;
         uint32_t pi = NONE32;
         const uint32_t lookup_end = i > cp.max_lookback ? i - cp.max_lookback : 0;
@@ -236,24 +309,19 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             bool brk = false, cand = false, diag = false;
             float ns = NEG;
             if (l < span) {
-                const uint32_t j = i - 1 - l;
-                if (L.ref[j] != ai_ref) {
-                    brk = true;
-                } else {
-                    const uint32_t aj_q = L.q[j];
-                    if (ai_q >= aj_q) {
-                        const uint32_t dq = ai_q - aj_q, dr = ai_rs - L.rs[j];
-                        if (dr >= max_ref_gap) {
-                            brk = true;
-                        } else {
-                            const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
-                            const float ratio = (float)mx / (float)mn;
-                            if (!(ratio > cp.max_diagonal_ratio)) {
-                                cand = true;
-                                ns = L.dp[j] + score_of(dq, dr);
-                                diag = dq == dr;
-                            }
-                        }
+                const AnchorL aj = L.a[i - 1 - l];
+                const uint32_t dq = ai.q - aj.q, dr = ai.rs - aj.rs;
+                const bool qok = ai.q >= aj.q;
+                // the scalar loop: ref_id change breaks; a later query start is
+                // skipped; otherwise dr >= max_ref_gap breaks
+                brk = aj.ref != ai.ref || (qok && dr >= max_ref_gap);
+                if (!brk && qok) {
+                    const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
+                    diag = dq == dr;
+                    if (!ratio_exceeds(mx, mn, rt)) {
+                        cand = true;
+                        const float sc = diag && dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab);
+                        ns = aj.dp + sc;
                     }
                 }
             }
@@ -261,25 +329,28 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             const uint64_t bm = gp.ballot(brk);
             const uint32_t fb = bm ? (uint32_t)__builtin_ctzll(bm) : (uint32_t)G;
             cand = cand && r < fb;
-            // taken where ns >= the running best (k or an earlier candidate)
+            // taken where ns >= the running best (k or an earlier candidate);
+            // after a taken same-diagonal predecessor the loop stops
             const float before = fmaxf(dpi, gp.excl_max(cand ? ns : NEG));
             const bool taken = cand && ns >= before;
-            const uint64_t sm = gp.ballot(taken && diag);  // taken on the same diagonal: the loop stops
+            const uint64_t sm = gp.ballot(taken && diag);
             const uint32_t fs = sm ? (uint32_t)__builtin_ctzll(sm) : (uint32_t)G;
             const uint64_t tm = gp.ballot(taken && r <= fs);
             if (tm) {
+                // the last taken value is the running maximum up to the stop
                 const uint32_t last = 63u - (uint32_t)__builtin_clzll(tm);
-                dpi = gp.bcast(ns, last);
+                dpi = gp.max(cand && r <= fs ? ns : NEG);
                 pi = i - 1 - (c0 + last);
             }
             if (sm || bm) break;
         }
-        if (best_i != NONE32 && b_ref == ai_ref && ai_q > b_q) {
-            const uint32_t dq = ai_q - b_q, dr = ai_rs - b_rs;
+        if (best_i != NONE32 && bA.ref == ai.ref && ai.q > bA.q) {
+            const uint32_t dq = ai.q - bA.q, dr = ai.rs - bA.rs;
             const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
-            const float ratio = (float)mx / (float)mn;
-            if (dr < max_ref_gap && dr > 0 && ratio <= cp.max_diagonal_ratio) {
-                const float ns = b_dp + score_of(dq, dr);
+            // ratio <= R: dq >= 1 and dr >= 1 here, so the quotient is a number
+            if (dr < max_ref_gap && dr > 0 && !ratio_exceeds(mx, mn, rt)) {
+                const float sc = dq == dr && dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab);
+                const float ns = bA.dp + sc;
                 if (ns > dpi) {
                     dpi = ns;
                     pi = best_i;
@@ -287,16 +358,13 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             }
         }
         if (r == 0) {
-            L.dp[i] = dpi;
+            L.a[i].dp = dpi;
             L.pred[i] = pi == NONE32 ? NONE16 : (uint16_t)pi;
         }
         if (dpi > best) {
             best = dpi;
             best_i = i;
-            b_ref = ai_ref;
-            b_rs = ai_rs;
-            b_q = ai_q;
-            b_dp = dpi;
+            bA = AnchorL{ai.ref, ai.rs, ai.q, dpi};
         }
         lds_fence();
     }
@@ -307,7 +375,7 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
     uint32_t nc = 0;
     for (uint32_t x0 = 0; x0 < m; x0 += G) {
         const uint32_t x = x0 + r;
-        const bool c = x < m && L.dp[x] >= valid;
+        const bool c = x < m && L.a[x].dp >= valid;
         const uint32_t pos = nc + gp.excl_sum(c ? 1u : 0u);
         if (c) L.cand[pos] = (uint16_t)x;
         if (x < m) L.used[x] = 0;
@@ -315,14 +383,13 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
     }
     lds_fence();
     for (uint32_t c = r; c < nc; c += G) {
-        const uint32_t x = L.cand[c];
-        const float dx = L.dp[x];
+        const float dx = L.a[L.cand[c]].dp;
         uint32_t rank = 0;
         for (uint32_t d = 0; d < nc; ++d) {
-            const float dy = L.dp[L.cand[d]];
+            const float dy = L.a[L.cand[d]].dp;
             rank += (dy > dx || (dy == dx && d < c)) ? 1u : 0u;
         }
-        L.order[rank] = (uint16_t)x;
+        L.order[rank] = L.cand[c];
     }
     lds_fence();
     if (r == 0) {
@@ -333,27 +400,29 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         for (uint32_t c = 0; c < nc; ++c) {
             const uint32_t i = L.order[c];
             if (L.used[i]) continue;
+            const AnchorL ai = L.a[i];
             uint32_t j = i;
+            AnchorL aj = ai;
             bool overlaps = false;
             const uint64_t start = ca;
-            cha[2 * ca] = L.rs[i];
-            cha[2 * ca + 1] = L.q[i];
+            cha[2 * ca] = ai.rs;
+            cha[2 * ca + 1] = ai.q;
             ++ca;
-            uint32_t matching = k, cov = L.rs[i];
+            uint32_t matching = k, cov = ai.rs;
             while (L.pred[j] != NONE16) {
                 j = L.pred[j];
                 if (L.used[j]) {
                     overlaps = true;
                     break;
                 }
-                const uint32_t rj = L.rs[j];
-                cha[2 * ca] = rj;
-                cha[2 * ca + 1] = L.q[j];
+                aj = L.a[j];
+                cha[2 * ca] = aj.rs;
+                cha[2 * ca + 1] = aj.q;
                 ++ca;
                 L.used[j] = 1;
-                const uint32_t gap = cov > rj ? cov - rj : 0;
+                const uint32_t gap = cov > aj.rs ? cov - aj.rs : 0;
                 matching += gap < k ? gap : k;
-                cov = rj;
+                cov = aj.rs;
             }
             if (overlaps) {
                 ca = start;
@@ -361,14 +430,14 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             }
             NamRec rec;
             rec.nam_id = nn;
-            rec.query_start = L.q[j];
-            rec.query_end = L.q[i] + k;
-            rec.ref_start = L.rs[j];
-            rec.ref_end = L.rs[i] + k;
+            rec.query_start = aj.q;
+            rec.query_end = ai.q + k;
+            rec.ref_start = aj.rs;
+            rec.ref_end = ai.rs + k;
             rec.matching_bases = matching;
-            rec.ref_id = L.ref[i];
+            rec.ref_id = ai.ref;
             rec.n_anchors = (uint32_t)(ca - start);
-            rec.score = L.dp[i] + (float)rec.n_anchors * cp.matches_weight;
+            rec.score = ai.dp + (float)rec.n_anchors * cp.matches_weight;
             rec.flags = o ? 1u : 0u;
             rec.anchor_off = 2 * ab + start;
             nams[nn++] = rec;

@@ -4,6 +4,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
+#include <functional>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -216,6 +217,7 @@ Engine::Engine(int device) : device_(device) {
     SA_CHECK(hipSetDevice(device));
     // non-blocking: no implicit ordering with the null stream or other engines' streams
     SA_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    own_stream_ = stream_;
     // waits on this event sleep instead of spinning, leaving the host cores to the replay
     SA_CHECK(hipEventCreateWithFlags(&sync_ev_, hipEventBlockingSync | hipEventDisableTiming));
     buf_.assign(S_COUNT, nullptr);
@@ -250,7 +252,13 @@ Engine::~Engine() {
         if (h) (void)hipHostFree(h);
     if (sync_ev_) (void)hipEventDestroy(sync_ev_);
     for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
-    if (stream_) (void)hipStreamDestroy(stream_);
+    if (own_stream_) (void)hipStreamDestroy(own_stream_);
+}
+
+hipStream_t Engine::set_stream(hipStream_t st) {
+    hipStream_t prev = stream_;
+    stream_ = st ? st : own_stream_;
+    return prev;
 }
 
 hipEvent_t Engine::event() {
@@ -340,6 +348,51 @@ void Engine::upload_index(const RefEntry* entries, uint64_t n, const uint64_t* b
     SA_CHECK(hipMalloc(&d_buckets_, nb * 8));
     if (n) SA_CHECK(hipMemcpy(d_entries_, entries, n * sizeof(RefEntry), hipMemcpyHostToDevice));
     SA_CHECK(hipMemcpy(d_buckets_, bucket_starts, nb * 8, hipMemcpyHostToDevice));
+    setup_chain_tables();
+}
+
+void Engine::upload_index_streamed(uint64_t n, const SeedParams& p, const std::function<void(void*, uint64_t)>& read) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint64_t nb = (1ull << p.bits) + 1;
+    SA_CHECK(hipMalloc(&d_entries_, std::max<uint64_t>(n, 1) * sizeof(RefEntry)));
+    SA_CHECK(hipMalloc(&d_buckets_, nb * 8));
+    // two pinned staging buffers: one fills from the reader while the other's
+    // copy to the device runs
+    const uint64_t chunk = 64ull << 20;
+    uint8_t* stage[2] = {(uint8_t*)host_stage(0, chunk), (uint8_t*)host_stage(1, chunk)};
+    hipEvent_t done[2] = {event(), event()};
+    SA_CHECK(hipEventRecord(done[0], stream_));
+    SA_CHECK(hipEventRecord(done[1], stream_));
+    uint64_t turn = 0;
+    auto stream_in = [&](uint8_t* dst, uint64_t total) {
+        for (uint64_t at = 0; at < total; at += chunk, ++turn) {
+            const uint64_t b = std::min(chunk, total - at);
+            const int x = (int)(turn & 1);
+            SA_CHECK(hipEventSynchronize(done[x]));
+            read(stage[x], b);
+            SA_CHECK(hipMemcpyAsync(dst + at, stage[x], b, hipMemcpyHostToDevice, stream_));
+            SA_CHECK(hipEventRecord(done[x], stream_));
+        }
+    };
+    try {
+        stream_in((uint8_t*)d_entries_, n * sizeof(RefEntry));
+        uint64_t nb_file = 0;
+        read(&nb_file, 8);
+        if (nb_file != nb)
+            throw std::invalid_argument("The randstrobe starts vector has an unexpected size in the .sti file");
+        stream_in((uint8_t*)d_buckets_, nb * 8);
+    } catch (...) {
+        sync();  // no copy out of the staging buffers may outlive this call
+        release_event(done[0]);
+        release_event(done[1]);
+        throw;
+    }
+    sync();
+    release_event(done[0]);
+    release_event(done[1]);
+    p_ = p;
+    p_.n_entries = n;
+    n_entries_ = n;
     setup_chain_tables();
 }
 
@@ -542,6 +595,13 @@ void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_rea
     sync();
 }
 
+void Engine::set_chain_anchors(const uint32_t* pairs, uint64_t n_pairs) {
+    SA_CHECK(hipSetDevice(device_));
+    void* d = grow(S_CHAIN, n_pairs * 8 + 8);
+    if (n_pairs) SA_CHECK(hipMemcpyAsync(d, pairs, n_pairs * 8, hipMemcpyHostToDevice, stream_));
+    sync();
+}
+
 void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) {
     const uint32_t n = n_reads_;
     auto* d_seq = d_seq_;
@@ -560,53 +620,71 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
 
 namespace {
 // anchor capacities of the LDS chaining tiers
-constexpr uint32_t CHAIN_CAP1 = 128, CHAIN_CAP2 = 1024;
+constexpr uint32_t CHAIN_CAP1 = 64, CHAIN_CAP2 = 256, CHAIN_CAP3 = 1024;
+
+}  // namespace
 
 // Device buffers and counts of one seeding -> NAM pass.
 struct NamPass {
-    uint32_t* qrn = nullptr;     // 2n: randstrobes per read orientation
+    uint32_t* qrn = nullptr;     // 2n: randstrobes per read orientation (null when hits were handed in)
     HitCounts* hc = nullptr;     // 2n
     uint32_t* namn = nullptr;    // 2n: NAMs per read orientation
     uint32_t* araw = nullptr;    // 2n: anchors before dedup
     uint32_t* cbytes = nullptr;  // 2n: chaining algorithmic bytes
-    uint32_t* hbytes = nullptr;  // 2n: lookup algorithmic bytes
+    uint32_t* hbytes = nullptr;  // 2n: lookup algorithmic bytes (null when hits were handed in)
     uint64_t* outoff = nullptr;  // n: first NAM of each read
     NamRec* nams = nullptr;      // n_nams
     uint64_t n_nams = 0;
 };
 
-// Seeding -> lookups -> chaining -> NAMs for the n reads at d_seq with device
-// offsets d_off (n_bases bases), on stream st. take(slot, bytes) supplies every
-// device buffer; scan(in, out, n) writes an exclusive prefix sum and returns its
-// total (the two host waits of the pass: anchor capacity, NAM count); nams_out(N)
-// supplies the NAM output once N is known. ev (7 events, or null) brackets the
-// stages: [0,1] seed, [1,2] hits, [3,4] chain, [5,6] finalize.
-template <typename Take, typename Scan, typename NamsOut>
-void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, const float* same_diag,
-              const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, uint64_t n_bases, const MapParams& mp,
-              const ChainParams& cp, hipStream_t st, Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev,
-              NamPass& o) {
+namespace {
+
+// Device buffers of the lookup stage: hits of read orientation t at
+// 2 * off[t / 2] + (t % 2) * L (L the read's length), hc[t].n_hits of them.
+struct HitPass {
+    Hit* hits = nullptr;
+    HitCounts* hc = nullptr;
+    uint64_t* acap = nullptr;  // 2n: anchor capacity per read orientation
+};
+
+// Seeding and lookups (k_seed, k_hits) for the n reads at d_seq with device
+// offsets d_off (n_bases bases), on stream st; take(slot, bytes) supplies every
+// device buffer. ev (or null) gets [0, 1] seed, [1, 2] hits.
+template <typename Take>
+void hit_pass(const IndexView& ix, const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, uint64_t n_bases,
+              const MapParams& mp, hipStream_t st, Take take, hipEvent_t* ev, HitPass& h, NamPass& o) {
     const SeedParams& p = ix.p;
-    const uint32_t k = (uint32_t)p.k;
     // the syncmer list is only materialised by the fallback for w_max >= 24
     auto* d_sy = (Syncmer*)take(S_SY, (p.w_max < 24 ? 1 : n_bases + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)take(S_QR, (2 * n_bases + 1) * sizeof(QRand));
     o.qrn = (uint32_t*)take(S_QRN, (2ull * n + 1) * 4);
-    auto* d_hits = (Hit*)take(S_HITS, (2 * n_bases + 1) * sizeof(Hit));
+    h.hits = (Hit*)take(S_HITS, (2 * n_bases + 1) * sizeof(Hit));
     auto* d_cnt = (uint64_t*)take(S_CNT, (2 * n_bases + 1) * 8);
-    o.hc = (HitCounts*)take(S_HC, (2ull * n + 1) * sizeof(HitCounts));
-    auto* d_acap = (uint64_t*)take(S_ACAP, (2ull * n + 1) * 8);
-    auto* d_aoff = (uint64_t*)take(S_AOFF, (2ull * n + 1) * 8);
+    o.hc = h.hc = (HitCounts*)take(S_HC, (2ull * n + 1) * sizeof(HitCounts));
+    h.acap = (uint64_t*)take(S_ACAP, (2ull * n + 1) * 8);
     o.hbytes = (uint32_t*)take(S_HBYTES, (2ull * n + 1) * 4);
     if (ev) SA_CHECK(hipEventRecord(ev[0], st));
     k_seed<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, p, d_sy, d_qr, o.qrn);
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[1], st));
-    k_hits<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, d_hits, d_cnt, o.hc, d_acap,
+    k_hits<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc, h.acap,
                                                   o.hbytes);
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[2], st));
-    const uint64_t A = scan(d_acap, d_aoff, 2ull * n);
+}
+
+// Chaining -> NAMs from the hits of a lookup stage (or hits handed in): anchor
+// capacities scanned into per-orientation offsets (scan(in, out, n) writes an
+// exclusive prefix sum and returns its total: a host wait), the chaining tiers,
+// then NAM counts scanned (the second host wait) and nams_out(N) supplies the
+// NAM output for k_finalize. ev (or null) gets [3, 4] chain, [5, 6] finalize.
+template <typename Take, typename Scan, typename NamsOut>
+void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, const float* same_diag,
+                const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const ChainParams& cp, hipStream_t st,
+                Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev, const HitPass& h, NamPass& o) {
+    const uint32_t k = (uint32_t)ix.p.k;
+    auto* d_aoff = (uint64_t*)take(S_AOFF, (2ull * n + 1) * 8);
+    const uint64_t A = scan(h.acap, d_aoff, 2ull * n);
     ChainBufs b;
     b.anchors = (Anchor*)take(S_ANCH, (A + 1) * sizeof(Anchor));
     b.anchors_tmp = (Anchor*)take(S_ANCH2, (A + 1) * sizeof(Anchor));
@@ -619,25 +697,28 @@ void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, con
     b.alg_bytes = o.cbytes = (uint32_t*)take(S_CBYTES, (2ull * n + 1) * 4);
     o.namn = (uint32_t*)take(S_NAMN, (2ull * n + 1) * 4);
     o.araw = (uint32_t*)take(S_ARAW, (2ull * n + 1) * 4);
-    // chaining tiers by anchor capacity: a 16-lane group per orientation up to
-    // 128 anchors, a wave per orientation up to 1024 (both in LDS), a thread per
-    // orientation over global memory beyond
+    // chaining tiers by anchor capacity, all in LDS: an 8-lane group per
+    // orientation up to 64 anchors, a 16-lane group up to 256, a wave up to 1024;
+    // a thread per orientation over global memory beyond
     ChainTiers tiers;
-    auto* d_tiers = (uint32_t*)take(S_TIERS, (4ull * n + 4) * 4);
+    auto* d_tiers = (uint32_t*)take(S_TIERS, (6ull * n + 4) * 4);
     tiers.count = d_tiers;
-    tiers.list[0] = d_tiers + 4;
-    tiers.list[1] = d_tiers + 4 + 2ull * n;
-    tiers.cap1 = CHAIN_CAP1;
-    tiers.cap2 = CHAIN_CAP2;
+    for (int x = 0; x < CHAIN_TIERS - 1; ++x) tiers.list[x] = d_tiers + 4 + (uint64_t)x * 2 * n;
+    tiers.cap[0] = CHAIN_CAP1;
+    tiers.cap[1] = CHAIN_CAP2;
+    tiers.cap[2] = CHAIN_CAP3;
     if (ev) SA_CHECK(hipEventRecord(ev[3], st));
     k_zero_u32<<<1, 64, 0, st>>>(tiers.count, 4);
-    k_chain_t1<16, CHAIN_CAP1><<<blocks(2ull * n, 4), 64, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b,
-                                                                   o.namn, o.araw, log2tab, same_diag, tiers);
+    k_chain_t1<8, CHAIN_CAP1><<<blocks(2ull * n, 8), 64, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b,
+                                                                  o.namn, o.araw, log2tab, same_diag, tiers);
     SA_CHECK(hipGetLastError());
-    k_chain_t2<64, CHAIN_CAP2><<<1024, 64, 0, st>>>(ix, cp, d_off, d_hits, o.hc, d_aoff, b, o.namn, o.araw, log2tab,
-                                                    same_diag, tiers);
+    k_chain_tn<16, CHAIN_CAP2><<<2048, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
+                                                    same_diag, tiers, 0);
     SA_CHECK(hipGetLastError());
-    k_chain<<<256, 256, 0, st>>>(ix, cp, d_off, n, d_hits, o.hc, d_acap, d_aoff, b, o.namn, o.araw, log2tab,
+    k_chain_tn<64, CHAIN_CAP3><<<1024, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
+                                                    same_diag, tiers, 1);
+    SA_CHECK(hipGetLastError());
+    k_chain<<<256, 256, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b, o.namn, o.araw, log2tab,
                                  same_diag, tiers);
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[4], st));
@@ -653,6 +734,17 @@ void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, con
     if (ev) SA_CHECK(hipEventRecord(ev[6], st));
 }
 
+// Seeding -> lookups -> chaining -> NAMs: hit_pass then chain_pass.
+template <typename Take, typename Scan, typename NamsOut>
+void nam_pass(const IndexView& ix, const RefView& ref, const float* log2tab, const float* same_diag,
+              const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, uint64_t n_bases, const MapParams& mp,
+              const ChainParams& cp, hipStream_t st, Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev,
+              NamPass& o) {
+    HitPass h;
+    hit_pass(ix, d_seq, d_off, n, n_bases, mp, st, take, ev, h, o);
+    chain_pass(ix, ref, log2tab, same_diag, d_seq, d_off, n, cp, st, take, scan, nams_out, ev, h, o);
+}
+
 // same-diagonal chaining scores (src/chainer.rs:63-74) in host f32 arithmetic
 void same_diag_table(uint32_t k, const ChainParams& cp, float* sd) {
     float lg[1025];
@@ -661,14 +753,7 @@ void same_diag_table(uint32_t k, const ChainParams& cp, float* sd) {
 }
 }  // namespace
 
-void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
-    SA_CHECK(hipSetDevice(device_));
-    const uint32_t n = n_reads_;
-    out.nams = HostView<NamRec>{};
-    out.nam_off = HostView<uint64_t>{(uint64_t*)host_stage(0, 8), 1};
-    out.nam_off[0] = 0;
-    out.info.clear();
-    if (n == 0) return;
+void Engine::chain_tables(const ChainParams& cp) {
     const uint32_t k = (uint32_t)p_.k;
     if (tab_k_ != k || std::memcmp(&tab_cp_, &cp, sizeof cp) != 0) {
         float sd[1024];
@@ -678,6 +763,17 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         tab_k_ = k;
         tab_cp_ = cp;
     }
+}
+
+void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint32_t n = n_reads_;
+    out.nams = HostView<NamRec>{};
+    out.nam_off = HostView<uint64_t>{(uint64_t*)host_stage(0, 8), 1};
+    out.nam_off[0] = 0;
+    out.info.clear();
+    if (n == 0) return;
+    chain_tables(cp);
     hipEvent_t ev[7];
     for (auto& e : ev) e = event();
     IndexView ix{d_entries_, d_buckets_, p_};
@@ -692,6 +788,66 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
             return t;
         },
         [&](uint64_t N) { return (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec)); }, ev, o);
+    finish_nams(n, o, ev, out);
+}
+
+void Engine::find_hits(const MapParams& mp, std::vector<Hit>& hits, std::vector<HitCounts>& hc) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint32_t n = n_reads_;
+    hits.clear();
+    hc.assign(2ull * n, HitCounts{});
+    if (n == 0) return;
+    IndexView ix{d_entries_, d_buckets_, p_};
+    HitPass h;
+    NamPass o;
+    hit_pass(ix, d_seq_, (const uint64_t*)buf_[S_OFF], n, n_bases_, mp, stream_,
+             [&](int slot, uint64_t bytes) { return grow(slot, bytes); }, nullptr, h, o);
+    hits.resize(2 * n_bases_);
+    if (n_bases_) SA_CHECK(hipMemcpyAsync(hits.data(), h.hits, 2 * n_bases_ * sizeof(Hit), hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(hc.data(), h.hc, 2ull * n * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
+    sync();
+}
+
+void Engine::chain_hits(const Hit* hits, const HitCounts* hc, const ChainParams& cp, NamBatch& out) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint32_t n = n_reads_;
+    out.nams = HostView<NamRec>{};
+    out.nam_off = HostView<uint64_t>{(uint64_t*)host_stage(0, 8), 1};
+    out.nam_off[0] = 0;
+    out.info.clear();
+    if (n == 0) return;
+    chain_tables(cp);
+    IndexView ix{d_entries_, d_buckets_, p_};
+    RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    const auto* d_off = (const uint64_t*)buf_[S_OFF];
+    HitPass h;
+    h.hits = (Hit*)grow(S_HITS, (2 * n_bases_ + 1) * sizeof(Hit));
+    h.hc = (HitCounts*)grow(S_HC, (2ull * n + 1) * sizeof(HitCounts));
+    h.acap = (uint64_t*)grow(S_ACAP, (2ull * n + 1) * 8);
+    if (n_bases_) SA_CHECK(hipMemcpyAsync(h.hits, hits, 2 * n_bases_ * sizeof(Hit), hipMemcpyHostToDevice, stream_));
+    SA_CHECK(hipMemcpyAsync(h.hc, hc, 2ull * n * sizeof(HitCounts), hipMemcpyHostToDevice, stream_));
+    k_hit_runs<<<blocks(2ull * n, 256), 256, 0, stream_>>>(ix, d_off, n, h.hits, h.hc, h.acap);
+    SA_CHECK(hipGetLastError());
+    hipEvent_t ev[7];
+    for (auto& e : ev) e = event();
+    SA_CHECK(hipEventRecord(ev[0], stream_));
+    SA_CHECK(hipEventRecord(ev[1], stream_));
+    SA_CHECK(hipEventRecord(ev[2], stream_));
+    NamPass o;
+    o.hc = h.hc;
+    chain_pass(
+        ix, ref, d_log2_, d_same_diag_, d_seq_, d_off, n, cp, stream_,
+        [&](int slot, uint64_t bytes) { return grow(slot, bytes); },
+        [&](const uint64_t* in, uint64_t* outp, uint64_t m) {
+            uint64_t t = 0;
+            scan_u64(in, outp, m, &t);
+            return t;
+        },
+        [&](uint64_t N) { return (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec)); }, ev, h, o);
+    finish_nams(n, o, ev, out);
+}
+
+void Engine::finish_nams(uint32_t n, const NamPass& o, hipEvent_t* ev, NamBatch& out) {
     const uint64_t N = o.n_nams;
     // everything the host reads back goes to pinned staging memory in one block
     const uint64_t n2 = 2ull * n;
@@ -712,10 +868,12 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
     SA_CHECK(hipMemcpyAsync(out.nam_off.data(), o.outoff, (uint64_t)n * 8, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(cbytes, o.cbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync((void*)hc, o.hc, n2 * sizeof(HitCounts), hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(qrn, o.qrn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    if (o.qrn) SA_CHECK(hipMemcpyAsync(qrn, o.qrn, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    else std::memset(qrn, 0, n2 * 4);
     SA_CHECK(hipMemcpyAsync(namn, o.namn, n2 * 4, hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(araw, o.araw, n2 * 4, hipMemcpyDeviceToHost, stream_));
-    SA_CHECK(hipMemcpyAsync(hbytes, o.hbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    if (o.hbytes) SA_CHECK(hipMemcpyAsync(hbytes, o.hbytes, n2 * 4, hipMemcpyDeviceToHost, stream_));
+    else std::memset(hbytes, 0, n2 * 4);
     sync();
     out.nam_off[n] = N;
     out.info.resize(n);
@@ -738,12 +896,16 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         chain_bytes += cbytes[x];
         hit_bytes += hbytes[x];
     }
-    SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n, seed_bytes);
-    SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n, hit_bytes);
+    if (o.qrn) {
+        SA_CHECK(hipEventElapsedTime(&t, ev[0], ev[1])); times_.seed = t; add_kernel("k_seed", t, n, seed_bytes);
+        SA_CHECK(hipEventElapsedTime(&t, ev[1], ev[2])); times_.hits = t; add_kernel("k_hits", t, 2ull * n, hit_bytes);
+    } else {
+        times_.seed = times_.hits = 0;
+    }
     SA_CHECK(hipEventElapsedTime(&t, ev[3], ev[4])); times_.chain = t; add_kernel("k_chain", t, 2ull * n, chain_bytes);
     SA_CHECK(hipEventElapsedTime(&t, ev[5], ev[6])); times_.finalize = t; add_kernel("k_finalize", t, n);
     total_ms_ += times_.seed + times_.hits + times_.chain + times_.finalize;
-    for (auto& e : ev) release_event(e);
+    for (int x = 0; x < 7; ++x) release_event(ev[x]);
 }
 
 namespace {
@@ -875,7 +1037,11 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     uint64_t slots = 0, prep_bytes = 0;
     for (uint32_t i = 0; i < n; ++i) {
         const uint64_t Lq = h_off_[req[i].read_id + 1] - h_off_[req[i].read_id];
-        req[i].cigar_cap = 2 * (uint32_t)Lq + 8;
+        // the aligner's entry points on explicit windows may need one op per query
+        // and window position
+        req[i].cigar_cap = req[i].kind >= EXT_HAMMING
+                               ? (uint32_t)Lq + (req[i].win_end - req[i].win_start) + 8
+                               : 2 * (uint32_t)Lq + 8;
         req[i].cigar_off = slots;
         slots += req[i].cigar_cap;
         // algorithmic bytes of the prep kernel: request in, result out, the query and

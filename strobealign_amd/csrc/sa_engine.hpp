@@ -3,6 +3,7 @@
 // layer (sa_mapper.cpp, the C ABI) never touches device pointers directly.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,7 @@ struct ihipEvent_t;
 namespace sa {
 
 struct Scores;
+struct NamPass;
 
 // Host copy of what the chainer/NAM stage returns for one batch of reads.
 // A host array owned elsewhere (the engine's pinned staging memory).
@@ -81,6 +83,11 @@ public:
     void upload_reference(const uint64_t* words, uint64_t n_words, const uint64_t* word_off, const uint64_t* len,
                           uint32_t n_contigs);
     void upload_index(const RefEntry* entries, uint64_t n, const uint64_t* bucket_starts, const SeedParams& p);
+    // The same from a byte stream in the .sti layout after the parameters
+    // (src/index.rs:401-402): n 16-byte entries, then the bucket-starts length
+    // (checked against p.bits) and the bucket starts. read(dst, bytes) fills host
+    // memory; the bytes go through pinned staging memory straight to the device.
+    void upload_index_streamed(uint64_t n, const SeedParams& p, const std::function<void(void*, uint64_t)>& read);
     // make_index (src/indexer.rs:18-170) on the device from the uploaded reference:
     // the same entries, bucket directory and filter cutoff as the host build
     // (contig lengths on the host; p.bits set to `bits`). Returns the parameters.
@@ -104,6 +111,12 @@ public:
     void gather_reads(const uint8_t* d_base, const uint64_t* seg, const uint64_t* seg_len, uint32_t n_segs,
                       const uint64_t* off, uint32_t n_reads);
     void find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out);
+    // The two halves of find_nams on their own (per-stage drop-ins): the hits of
+    // every read orientation (orientation t of read i at 2 * off[i] + t % 2 * L,
+    // hc[2i + t % 2].n_hits of them; find_hits, src/hit.rs:270-333), and NAMs from
+    // hits in that layout (src/chainer.rs:182-267 after find_hits).
+    void find_hits(const MapParams& mp, std::vector<Hit>& hits, std::vector<HitCounts>& hc);
+    void chain_hits(const Hit* hits, const HitCounts* hc, const ChainParams& cp, NamBatch& out);
 
     // find_nams for reads and outputs in device memory, on the caller's stream,
     // with every intermediate buffer carved from `scratch` (scratch_bytes): no
@@ -114,6 +127,11 @@ public:
     void find_nams_noalloc(const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const MapParams& mp,
                            const ChainParams& cp, NamRec* d_nams, uint64_t nam_cap, uint64_t* d_nam_off,
                            uint64_t* n_nams, ihipStream_t* st, uint8_t* scratch, uint64_t scratch_bytes);
+
+    // Replaces the chain-anchor buffer (what find_nams leaves for piecewise
+    // extension) with `n_pairs` (ref_start, query_start) pairs, for EXT_ALIGN_PW
+    // requests that bring their own anchors.
+    void set_chain_anchors(const uint32_t* pairs, uint64_t n_pairs);
 
     // Fine-grained stages for parity probes (valid after upload_reads).
     void randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts);
@@ -133,11 +151,16 @@ public:
     const SeedParams& params() const { return p_; }
     uint64_t n_entries() const { return n_entries_; }
     ihipStream_t* stream() const { return stream_; }
+    // Runs the engine's work on `st` (null: its own stream) from now on; returns
+    // the stream it used before. The caller keeps `st` alive while it is set.
+    ihipStream_t* set_stream(ihipStream_t* st);
     // Kernel time accumulated over all find_nams/extend calls, milliseconds.
     double total_kernel_ms() const { return total_ms_; }
     uint64_t total_lookup_bytes() const { return lookup_bytes_; }
 
 private:
+    void chain_tables(const ChainParams& cp);
+    void finish_nams(uint32_t n, const NamPass& o, ihipEvent_t** ev, NamBatch& out);
     void* grow(int slot, uint64_t bytes);
     void sync();  // waits for the engine's stream (blocking wait)
     ihipEvent_t* event();  // a timing event from the pool
@@ -146,6 +169,7 @@ private:
 
     int device_;
     ihipStream_t* stream_ = nullptr;
+    ihipStream_t* own_stream_ = nullptr;
     ihipEvent_t* sync_ev_ = nullptr;
     std::vector<ihipEvent_t*> event_pool_;
     SeedParams p_{};

@@ -32,6 +32,9 @@ PackedRef read_fasta(const std::string& path);
 SeedParams seed_params_for_read_length(uint64_t read_length);
 // src/indexer.rs:294-300
 int pick_bits(const PackedRef& ref, const SeedParams& p);
+// Profile number of the read-length profile (src/seeding/parameters.rs:10-19,
+// Noisy = 0, ReadLength50 = 1, ..., ReadLength400 = 7), as .sti files store it.
+uint32_t profile_for_read_length(uint64_t read_length);
 
 // Allocator whose resize() leaves new elements uninitialised: the index
 // entries are written right after, in parallel, which also spreads the first

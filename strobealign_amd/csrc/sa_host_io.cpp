@@ -157,6 +157,13 @@ SeedParams seed_params_for_read_length(uint64_t read_length) {
     return p;
 }
 
+uint32_t profile_for_read_length(uint64_t read_length) {
+    static const uint64_t thr[] = {70, 90, 110, 135, 175, 375};
+    uint32_t i = 0;
+    while (i < 6 && read_length > thr[i]) ++i;
+    return i + 1;
+}
+
 int pick_bits(const PackedRef& ref, const SeedParams& p) {
     uint64_t total = 0;
     for (uint64_t l : ref.len) total += l;

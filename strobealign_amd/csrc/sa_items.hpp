@@ -145,6 +145,69 @@ SA_HD void write_found(ExtResult& out, const AlignInfo& a, uint64_t ref_base, bo
     out.n_cigar = n_cigar;
 }
 
+// The piecewise aligner's entry points on an explicit window (EXT_ALIGN_PW,
+// EXT_PW_GLOBAL, EXT_PW_XDROP; see ExtKind). Coordinates are window-relative.
+SA_HD AlignInfo pw_probe(const ExtRequest& r, const QueryView& q, const RefWin& w, const Scores& sc, uint32_t k,
+                         const uint32_t* chain_anchors, uint8_t* scr, uint64_t scratch_per, CigarOut& cig,
+                         DpRows rows = DpRows{}) {
+    AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
+    cig.reset();
+    if (r.kind == EXT_ALIGN_PW) {
+        uint32_t* an = (uint32_t*)scr;
+        const uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
+        if (an_bytes >= scratch_per) {
+            a.status = 2;
+            return a;
+        }
+        for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
+        return extend_piecewise(q, w, an, r.n_anchors, r.ref_start, sc, k, scr + an_bytes, scratch_per - an_bytes,
+                                cig, rows);
+    }
+    const uint32_t ql = q.len, rl = w.len;
+    const bool rev = r.kind == EXT_PW_XDROP && (r.flags & 1) != 0;
+    const bool local_end = r.kind == EXT_PW_XDROP;
+    if (local_end && (ql == 0 || rl == 0)) {
+        a.found = true;
+        return a;
+    }
+    const uint32_t tb_cap = ql + rl + 16;
+    const uint64_t tb_bytes = ((uint64_t)tb_cap + 15) & ~15ull;
+    if (tb_bytes >= scratch_per) {
+        a.status = 2;
+        return a;
+    }
+    uint8_t* tb = scr;
+    DpOut d;
+    if (rev) {
+        RevView<QueryView> qr{&q, ql};
+        RevView<RefWin> rr{&w, rl};
+        d = affine_dp(qr, ql, rr, rl, sc, true, scr + tb_bytes, scratch_per - tb_bytes, tb, tb_cap, rows);
+    } else {
+        d = affine_dp(q, ql, w, rl, sc, local_end, scr + tb_bytes, scratch_per - tb_bytes, tb, tb_cap, rows);
+    }
+    if (d.status) {
+        a.status = d.status;
+        return a;
+    }
+    emit_tb(tb, d.ntb, rev, cig);
+    a.found = true;
+    a.score = (uint32_t)d.score;
+    if (rev) {
+        a.query_start = ql - d.bi;
+        a.query_end = ql;
+        a.ref_start = rl - d.bj;
+        a.ref_end = rl;
+    } else {
+        a.query_start = 0;
+        a.query_end = d.bi;
+        a.ref_start = 0;
+        a.ref_end = d.bj;
+    }
+    a.edit_distance = cig.edit_distance();
+    if (cig.overflow) a.status = 1;
+    return a;
+}
+
 // First stage of extend_seed (src/mapper.rs:532-620) and rescue_align
 // (src/mapper.rs:1094-1153): the ungapped (Hamming) case and the rescue
 // substring prefilter are finished here; gapped alignments become jobs for the
@@ -205,6 +268,16 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
             if (de > rlen) de = rlen;
             pw_jobs[SA_ATOMIC_ADD(&counters[2], 1u)] = ExtJob{t, r.ref_id, ds, (uint32_t)(de - ds), 0};
         }
+    } else if (r.kind == EXT_HAMMING) {
+        const RefWin w{&ref, r.ref_id, r.win_start, r.win_end - r.win_start};
+        if (w.len == L) {
+            CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+            AlignInfo a = hamming_align(q, w, L, sc, cig);
+            out.status = (uint32_t)a.status;
+            if (a.found && a.status == 0) write_found(out, a, r.win_start, false, cig.n);
+        }
+    } else if (r.kind >= EXT_ALIGN_PW) {
+        pw_jobs[SA_ATOMIC_ADD(&counters[2], 1u)] = ExtJob{t, r.ref_id, r.win_start, r.win_end - r.win_start, 0};
     } else {
         // has_shared_substring (src/mapper.rs:1156-1170) unless bit3 asks for a plain alignment
         uint32_t ws = r.win_start, we = r.win_end;
@@ -261,14 +334,20 @@ SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefV
     CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     RefWin w{&ref, r.ref_id, j.ws, j.wlen};
+    const uint64_t wave = t / 64, lane = t % 64;
     uint32_t* an = (uint32_t*)scr;
     uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
-    if (an_bytes >= scratch_per) {
+    if (r.kind != EXT_SEED) {
+        const DpRows dr{rows + wave * 64 * row_cap + lane, 64, row_cap, dirs + wave * 64 * row_cap * dir_rows + lane,
+                        dir_rows};
+        AlignInfo a = pw_probe(r, q, w, sc, k, chain_anchors, scr, scratch_per, cig, dr);
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    } else if (an_bytes >= scratch_per) {
         out.status = 2;
     } else {
         for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
         uint32_t na = remove_spurious_anchors(an, r.n_anchors);
-        const uint64_t wave = t / 64, lane = t % 64;
         const DpRows dr{rows + wave * 64 * row_cap + lane, 64, row_cap, dirs + wave * 64 * row_cap * dir_rows + lane,
                         dir_rows};
         AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig, dr);
@@ -414,6 +493,18 @@ SA_HD void item_extend(uint32_t t, const uint8_t* seq, const uint64_t* off,
             out.edit_distance = a.edit_distance;
             out.n_cigar = cig.n;
         }
+    } else if (r.kind == EXT_HAMMING) {
+        const RefWin w{&ref, r.ref_id, r.win_start, r.win_end - r.win_start};
+        if (w.len == L) {
+            AlignInfo a = hamming_align(q, w, L, sc, cig);
+            out.status = (uint32_t)a.status;
+            if (a.found && a.status == 0) write_found(out, a, r.win_start, false, cig.n);
+        }
+    } else if (r.kind >= EXT_ALIGN_PW) {
+        const RefWin w{&ref, r.ref_id, r.win_start, r.win_end - r.win_start};
+        AlignInfo a = pw_probe(r, q, w, sc, k, chain_anchors, scr, scratch_per, cig);
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) write_found(out, a, r.win_start, true, cig.n);
     } else {
         // rescue: shared-substring test, then SSW over the window
         uint32_t ws = r.win_start, we = r.win_end;

@@ -25,7 +25,7 @@ __global__ void __launch_bounds__(256) k_hits(IndexView ix, MapParams mp, const 
 }
 
 // Chaining, first tier: one read orientation per group of G lanes with its
-// anchors in LDS (sa_chain_group.hpp), for anchor capacities up to tiers.cap1
+// anchors in LDS (sa_chain_group.hpp), for anchor capacities up to tiers.cap[0]
 // (CAP); orientations not chained (src/chainer.rs:206-216) get empty results,
 // larger ones are listed for the later tiers.
 template <int G, int CAP>
@@ -50,9 +50,9 @@ __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, c
         return;
     }
     const uint64_t cap = acap[t];
-    if (cap > tiers.cap1) {
+    if (cap > tiers.cap[0]) {
         if (gp.r == 0) {
-            const uint32_t tier = cap > tiers.cap2 ? 1 : 0;
+            const uint32_t tier = cap > tiers.cap[2] ? 2u : cap > tiers.cap[1] ? 1u : 0u;
             tiers.list[tier][atomicAdd(&tiers.count[tier], 1u)] = t;
         }
         return;
@@ -60,25 +60,26 @@ __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, c
     chain_grp<G, CAP>(gp, lds[g], t, ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
 }
 
-// Chaining, second tier: the listed orientations with capacities up to
-// tiers.cap2 (CAP), a group each, grid-stride over the list.
+// Chaining, later LDS tiers: the orientations of list `which`, a group of G
+// lanes each, grid-stride over the list.
 template <int G, int CAP>
-__global__ void __launch_bounds__(64) k_chain_t2(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
+__global__ void __launch_bounds__(64) k_chain_tn(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
                                                  const Hit* __restrict__ hits, const HitCounts* __restrict__ hc,
                                                  const uint64_t* __restrict__ aoff, ChainBufs b,
                                                  uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
                                                  const float* __restrict__ log2tab,
-                                                 const float* __restrict__ same_diag, ChainTiers tiers) {
+                                                 const float* __restrict__ same_diag, ChainTiers tiers,
+                                                 uint32_t which) {
     __shared__ ChainLds<CAP> lds[64 / G];
     const uint32_t g = threadIdx.x / G;
     const Grp<G> gp{threadIdx.x % G, g * G};
-    const uint32_t cnt = tiers.count[0];
+    const uint32_t cnt = tiers.count[which];
     for (uint32_t x = blockIdx.x * (64 / G) + g; x < cnt; x += gridDim.x * (64 / G))
-        chain_grp<G, CAP>(gp, lds[g], tiers.list[0][x], ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab,
-                          same_diag);
+        chain_grp<G, CAP>(gp, lds[g], tiers.list[which][x], ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw,
+                          log2tab, same_diag);
 }
 
-// Chaining, last tier: the listed orientations above tiers.cap2, a thread each
+// Chaining, last tier: the listed orientations above the LDS tiers, a thread each
 // with its work arrays in global memory (item_chain), grid-stride over the list.
 __global__ void __launch_bounds__(256) k_chain(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
                                               uint32_t n, const Hit* __restrict__ hits,
@@ -87,9 +88,22 @@ __global__ void __launch_bounds__(256) k_chain(IndexView ix, ChainParams cp, con
                                               uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
                                               const float* __restrict__ log2tab,
                                               const float* __restrict__ same_diag, ChainTiers tiers) {
-    const uint32_t cnt = tiers.count[1];
+    const uint32_t cnt = tiers.count[CHAIN_TIERS - 2];
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < cnt; x += gridDim.x * blockDim.x)
-        item_chain(tiers.list[1][x], ix, cp, off, n, hits, hc, acap, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
+        item_chain(tiers.list[CHAIN_TIERS - 2][x], ix, cp, off, n, hits, hc, acap, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
+}
+
+// Anchor runs and capacities of hits handed in from outside (the chaining
+// stage on its own): hit_runs per read orientation, as k_hits ends.
+__global__ void __launch_bounds__(256) k_hit_runs(IndexView ix, const uint64_t* __restrict__ off, uint32_t n,
+                                                  Hit* __restrict__ hits, const HitCounts* __restrict__ hc,
+                                                  uint64_t* __restrict__ acap) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * n) return;
+    const uint32_t i = t >> 1, o = t & 1;
+    const uint64_t ro = off[i];
+    const uint32_t L = (uint32_t)(off[i + 1] - ro);
+    acap[t] = hit_runs(ix, hits + 2 * ro + (uint64_t)o * L, hc[t].n_hits);
 }
 
 __global__ void __launch_bounds__(64) k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
@@ -309,6 +323,24 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
                 if (de > rlen) de = rlen;
                 pw_jobs[atomicAdd(&counters[2], 1u)] = ExtJob{t, r.ref_id, ds, (uint32_t)(de - ds), 0};
             }
+            res[t] = out;
+        }
+    } else if (r.kind == EXT_HAMMING) {
+        // hamming_align (src/aligner.rs:223-259) of the whole query against the window
+        if (lane == 0) {
+            const RefWin w{&ref, r.ref_id, r.win_start, r.win_end - r.win_start};
+            if (w.len == L) {
+                CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+                AlignInfo a = hamming_align(q, w, L, sc, cig);
+                out.status = (uint32_t)a.status;
+                if (a.found && a.status == 0) write_found(out, a, r.win_start, false, cig.n);
+            }
+            res[t] = out;
+        }
+    } else if (r.kind >= EXT_ALIGN_PW) {
+        // the piecewise aligner's entry points: a job for the piecewise kernel
+        if (lane == 0) {
+            pw_jobs[atomicAdd(&counters[2], 1u)] = ExtJob{t, r.ref_id, r.win_start, r.win_end - r.win_start, 0};
             res[t] = out;
         }
     } else {

@@ -643,6 +643,44 @@ SA_HD void hits_to_anchors(const IndexView& ix, const Hit* hits, uint32_t nh, An
     for (uint32_t i = 0; i < nh; ++i) hit_anchors(ix, hits[i], s, ns, e, ne, entries);
 }
 
+// ratio = (float)mx / (float)mn > R, the diagonal-ratio test of
+// src/chainer.rs:126-130, without the division: with R = S * 2^e (S its 24-bit
+// significand) the correctly rounded quotient exceeds R exactly when mx / mn
+// exceeds the midpoint (2S + 1) * 2^(e-1) between R and the next float, or
+// equals it and the tie rounds up (S odd); that is mx * 2^(1-e) > or >= mn *
+// (2S + 1). Exact while mx < 2^24 (its float conversion exact) and the products
+// fit in 64 bits; the division is used otherwise.
+struct RatioTest {
+    uint64_t mul_mn;   // 2S + 1
+    uint32_t shift;    // 1 - e
+    uint32_t tie_up;   // S odd
+    uint32_t exact;    // this form applies to R at all (R finite, positive, 1 - e in 0..39)
+    float R;
+};
+SA_HD RatioTest make_ratio_test(float R) {
+    RatioTest t{0, 0, 0, 0, R};
+    if (!(R > 0.0f) || R > 3.0e38f) return t;
+    uint32_t bits;
+    __builtin_memcpy(&bits, &R, 4);
+    const int32_t ex = (int32_t)((bits >> 23) & 0xFF);
+    if (ex == 0) return t;  // subnormal
+    const uint64_t S = (bits & 0x7FFFFF) | 0x800000;
+    const int32_t e = ex - 127 - 23;
+    if (1 - e < 0 || 1 - e > 39) return t;
+    t.mul_mn = 2 * S + 1;
+    t.shift = (uint32_t)(1 - e);
+    t.tie_up = (uint32_t)(S & 1);
+    t.exact = 1;
+    return t;
+}
+SA_HD bool ratio_exceeds(uint32_t mx, uint32_t mn, const RatioTest& t) {
+    if (t.exact && mx < (1u << 24)) {
+        const uint64_t lhs = (uint64_t)mx << t.shift, rhs = (uint64_t)mn * t.mul_mn;
+        return t.tie_up ? lhs >= rhs && mx > 0 : lhs > rhs;
+    }
+    return (float)mx / (float)mn > t.R;
+}
+
 // src/chainer.rs:286-302; log2(dd+1) comes from a host-filled table (host libm).
 SA_HD float chain_score(uint32_t dq, uint32_t dr, uint32_t k, const ChainParams& cp, const float* log2tab) {
     uint32_t dd = dr > dq ? dr - dq : dq - dr;

@@ -98,7 +98,27 @@ struct ReadNamInfo {
 };
 
 // Extension request / result (src/mapper.rs:532-620 extend_seed, 1094-1153 rescue_align).
-enum ExtKind : uint32_t { EXT_SEED = 0, EXT_RESCUE = 1 };
+// The mapping path issues EXT_SEED and EXT_RESCUE. The other kinds are the
+// aligner's own entry points on an explicit reference window [win_start,
+// win_end) of contig ref_id (the per-stage drop-in sa_extend_batch):
+//   EXT_HAMMING   hamming_align (src/aligner.rs:223-259), None when the lengths differ;
+//   EXT_ALIGN_PW  Aligner::align_piecewise (src/aligner.rs:153-163): extend_piecewise
+//                 with the caller's anchors (anchor_off, n_anchors: (ref_start,
+//                 query_start) pairs, last anchor first, contig coordinates), no
+//                 pruning; ref_start holds the padding;
+//   EXT_PW_GLOBAL global_alignment (src/piecewisealigner.rs:84-120);
+//   EXT_PW_XDROP  xdrop_alignment (src/piecewisealigner.rs:161-229), GlobalStartLocalEnd,
+//                 or LocalStartGlobalEnd when flags bit0 is set.
+// The last two report the i32 score (score field, two's complement) with the
+// end coordinates of the block aligner, found set.
+enum ExtKind : uint32_t {
+    EXT_SEED = 0,
+    EXT_RESCUE = 1,
+    EXT_HAMMING = 2,
+    EXT_ALIGN_PW = 3,
+    EXT_PW_GLOBAL = 4,
+    EXT_PW_XDROP = 5
+};
 
 struct ExtRequest {
     uint32_t kind;

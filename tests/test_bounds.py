@@ -23,3 +23,15 @@ def test_two_diagonal_bound(tmp_path):
     p = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
     assert p.stdout.startswith("ok ")
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_diagonal_ratio_without_division(tmp_path):
+    """The chaining kernels' division-free diagonal-ratio test equals the
+    reference's f32 division and comparison (src/chainer.rs:126-130)."""
+    exe = str(tmp_path / "ratio_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "strobealign_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "ratio_check.cpp"), "-o", exe], check=True)
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert p.stdout.startswith("ok ")

@@ -407,7 +407,7 @@ def pw(orc, q, r, mode, eb=10, xdrop=100):
     return list(out), buf.value.decode()
 
 
-@pytest.mark.parametrize("q,r,score,qe,re_,cigar", [
+GLOBAL_KATS = [
     # src/piecewisealigner.rs:725-933
     (b"AAATTT", b"AAATTT", 12, 6, 6, "6="),
     (b"AAA", b"TTT", -24, 3, 3, "3X"),
@@ -420,14 +420,17 @@ def pw(orc, q, r, mode, eb=10, xdrop=100):
     (b"AAATTTAA", b"AAATTT", -1, 8, 6, "6=2I"),
     (b"ATATATATAT", b"AAAAAAAAAA", -30, 10, 10, "1=1X1=1X1=1X1=1X1=1X"),
     (b"AAACTTAAACCTT", b"AAAATTGAAATT", -13, 13, 12, "3=1X2=1D3=2I2="),
-])
+]
+
+
+@pytest.mark.parametrize("q,r,score,qe,re_,cigar", GLOBAL_KATS)
 def test_global_alignment_kats(orc, q, r, score, qe, re_, cigar):
     out, cig = pw(orc, q, r, 0)
     assert out == [score, 0, qe, 0, re_]
     assert cig == cigar
 
 
-@pytest.mark.parametrize("q,r,mode,eb,exp,cigar", [
+XDROP_KATS = [
     # src/piecewisealigner.rs:935-1215
     (b"AAATTT", b"AAATTT", 1, 10, [22, 0, 6, 0, 6], "6="),
     (b"AAATAA", b"AAAAAA", 1, 10, [12, 0, 6, 0, 6], "3=1X2="),
@@ -441,7 +444,10 @@ def test_global_alignment_kats(orc, q, r, score, qe, re_, cigar):
     (b"TTAAAAAAAA", b"AAAAAAAAAA", 2, 50, [50, 0, 10, 0, 10], "2X8="),
     (b"TTTCCAAAAA", b"TTTAAAAA", 2, 10, [13, 0, 10, 0, 8], "3=2I5="),
     (b"TTTAAAAA", b"TTTCCAAAAA", 2, 10, [13, 0, 8, 0, 10], "3=2D5="),
-])
+]
+
+
+@pytest.mark.parametrize("q,r,mode,eb,exp,cigar", XDROP_KATS)
 def test_xdrop_alignment_kats(orc, q, r, mode, eb, exp, cigar):
     out, cig = pw(orc, q, r, mode, eb)
     assert out == exp
@@ -476,14 +482,20 @@ def ext_pw(orc, q, r, anchors, padding=5, k=5):
     return (list(out), buf.value.decode()) if ok else None
 
 
+# src/piecewisealigner.rs:1521-1711: (query, window, anchors (ref_start,
+# query_start) last first, expected [score, qs, qe, rs, re, edit distance] and
+# CIGAR, or None) with padding 5 and k = 5
+EXTEND_PIECEWISE_KATS = [
+    (b"A" * 50, b"A" * 70, [(40, 30), (30, 20), (20, 10), (10, 0)], ([120, 0, 50, 10, 60, 0], "50=")),
+    (b"A" * 50, b"T" * 50, [(30, 20), (20, 20), (10, 10)], None),
+    (b"A" * 50, b"A" * 81, [(40, 30), (37, 27), (35, 25), (30, 20), (25, 20), (20, 15), (10, 5)],
+     ([50 * 2 + 20 - 12 - 4, 0, 50, 5, 60, 5], "25=5D25=")),
+    (b"CTTTTAAAAATTTTAAAAATGGTTTCAAAAATTCCTAAAAATTTTTCCCCC", b"TTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAA",
+     [(35, 36), (15, 14), (5, 5)],
+     ([(13 + 6 + 3 + 7 + 11) * 2 + 10 - 12 * 2 - 1 - 8 * 4, 0, 46, 0, 45, 7], "1X13=1D6=2I3=1X7=2X11=")),
+]
+
+
 def test_extend_piecewise_kats(orc):
-    # src/piecewisealigner.rs:1521-1711
-    q = b"A" * 50
-    assert ext_pw(orc, q, b"A" * 70, [(40, 30), (30, 20), (20, 10), (10, 0)]) == ([120, 0, 50, 10, 60, 0], "50=")
-    assert ext_pw(orc, q, b"T" * 50, [(30, 20), (20, 20), (10, 10)]) is None
-    out = ext_pw(orc, q, b"A" * 81, [(40, 30), (37, 27), (35, 25), (30, 20), (25, 20), (20, 15), (10, 5)])
-    assert out == ([50 * 2 + 20 - 12 - 4, 0, 50, 5, 60, 5], "25=5D25=")
-    q = b"CTTTTAAAAATTTTAAAAATGGTTTCAAAAATTCCTAAAAATTTTTCCCCC"
-    r = b"TTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAATTTTTAAAAA"
-    out = ext_pw(orc, q, r, [(35, 36), (15, 14), (5, 5)])
-    assert out == ([(13 + 6 + 3 + 7 + 11) * 2 + 10 - 12 * 2 - 1 - 8 * 4, 0, 46, 0, 45, 7], "1X13=1D6=2I3=1X7=2X11=")
+    for q, r, anchors, exp in EXTEND_PIECEWISE_KATS:
+        assert ext_pw(orc, q, r, anchors) == exp
