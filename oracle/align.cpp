@@ -2,6 +2,8 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"
 
+#include <emmintrin.h>
+
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -104,96 +106,109 @@ AlignmentInfo hamming_align_global(const uint8_t* q, const uint8_t* r, size_t n,
 }
 
 // ------------------------------------------------------------------ SSW ---
-// Striped Smith-Waterman as in ext/ssw/ssw.c:197-588. The vector registers are
-// emulated lane by lane: LANES=16 unsigned bytes with bias (the u8 kernel) or
-// LANES=8 signed 16-bit words (the i16 kernel). Saturating add/sub semantics of
-// the SSE2 instructions are reproduced, so results (incl. the lazy-F loop that
-// does not revisit E) match the striped layout exactly.
+// Striped Smith-Waterman as in ext/ssw/ssw.c:197-588.
 namespace {
 struct EndPos { int score; int32_t ref; int32_t read; };
 
-template <int LANES, bool BYTE>
-struct Striped {
-    static int sat(int v) { return BYTE ? std::min(255, std::max(0, v)) : std::min(32767, std::max(-32768, v)); }
-    static int subs_u(int a, int b) { int v = a - b; return v < 0 ? 0 : v; }  // subs_epu8 / subs_epu16 on non-negative values
+// The striped passes on SSE2 vectors: 16 unsigned bytes with bias (BYTE) or 8
+// signed 16-bit words, one vector per stripe segment, so the CPU baseline runs
+// at the speed of the reference's own SIMD kernels. Saturating arithmetic, the
+// lazy-F loop that does not revisit E and stops once F cannot raise any H of the
+// next stripe, the column maximum, the best column's H kept for the end position
+// on the read: the results of ext/ssw/ssw.c, whose scores depend on the striping.
+template <bool BYTE>
+struct StripedSse {
+    static __m128i adds(__m128i a, __m128i b) { return BYTE ? _mm_adds_epu8(a, b) : _mm_adds_epi16(a, b); }
+    static __m128i subs(__m128i a, __m128i b) { return BYTE ? _mm_subs_epu8(a, b) : _mm_subs_epu16(a, b); }
+    static __m128i vmax(__m128i a, __m128i b) { return BYTE ? _mm_max_epu8(a, b) : _mm_max_epi16(a, b); }
+    static __m128i set1(int v) { return BYTE ? _mm_set1_epi8((char)v) : _mm_set1_epi16((short)v); }
+    static __m128i shift_up(__m128i v) { return BYTE ? _mm_slli_si128(v, 1) : _mm_slli_si128(v, 2); }
+    // all lanes of a <= b (unsigned bytes or non-negative words): a - b saturates to 0
+    static bool all_le(__m128i a, __m128i b) {
+        const __m128i d = subs(a, b);
+        return _mm_movemask_epi8(BYTE ? _mm_cmpeq_epi8(d, _mm_setzero_si128())
+                                      : _mm_cmpeq_epi16(d, _mm_setzero_si128())) == 0xFFFF;
+    }
+    static int hmax(__m128i v) {
+        alignas(16) uint16_t w[8];
+        alignas(16) uint8_t b[16];
+        int m = 0;
+        if (BYTE) {
+            _mm_store_si128((__m128i*)b, v);
+            for (int l = 0; l < 16; ++l) m = std::max<int>(m, b[l]);
+        } else {
+            _mm_store_si128((__m128i*)w, v);
+            for (int l = 0; l < 8; ++l) m = std::max<int>(m, (int16_t)w[l]);
+        }
+        return m;
+    }
+    static constexpr int LANES = BYTE ? 16 : 8;
 
-    // Profile: prof[nt][seg][lane]
-    static std::vector<int> profile(const int8_t* read, int len, const int8_t* mat, int n, int bias) {
-        int segLen = (len + LANES - 1) / LANES;
-        std::vector<int> p((size_t)n * segLen * LANES);
+    static std::vector<__m128i> profile(const int8_t* read, int len, const int8_t* mat, int n, int bias) {
+        const int segLen = (len + LANES - 1) / LANES;
+        std::vector<__m128i> p((size_t)n * segLen);
         for (int nt = 0; nt < n; ++nt)
-            for (int i = 0; i < segLen; ++i)
+            for (int i = 0; i < segLen; ++i) {
+                alignas(16) int16_t w[8];
+                alignas(16) uint8_t b[16];
                 for (int l = 0; l < LANES; ++l) {
-                    int j = i + l * segLen;
-                    int v = j >= len ? 0 : mat[nt * n + read[j]];
-                    p[((size_t)nt * segLen + i) * LANES + l] = BYTE ? v + bias : v;
+                    const int j = i + l * segLen;
+                    const int v = j >= len ? 0 : mat[nt * n + read[j]];
+                    if (BYTE) b[l] = (uint8_t)(v + bias);
+                    else w[l] = (int16_t)v;
                 }
+                p[(size_t)nt * segLen + i] = BYTE ? _mm_load_si128((const __m128i*)b) : _mm_load_si128((const __m128i*)w);
+            }
         return p;
     }
 
     static EndPos run(const int8_t* ref, bool reverse, int refLen, int readLen, int gapO, int gapE,
-                      const std::vector<int>& prof, int terminate, int bias) {
-        int segLen = (readLen + LANES - 1) / LANES;
+                      const std::vector<__m128i>& prof, int terminate, int bias) {
+        const int segLen = (readLen + LANES - 1) / LANES;
         int maxv = 0;
         int32_t end_read = readLen - 1;
         int32_t end_ref = BYTE ? -1 : 0;
-        std::vector<int> HS((size_t)segLen * LANES, 0), HL((size_t)segLen * LANES, 0),
-            E((size_t)segLen * LANES, 0), Hmax((size_t)segLen * LANES, 0);
-        int vMaxScore[LANES] = {0}, vMaxMark[LANES] = {0};
-        int begin = reverse ? refLen - 1 : 0, end = reverse ? -1 : refLen, step = reverse ? -1 : 1;
+        const __m128i zero = _mm_setzero_si128(), vGapO = set1(gapO), vGapE = set1(gapE), vBias = set1(bias);
+        std::vector<__m128i> HS(segLen, zero), HL(segLen, zero), E(segLen, zero), Hmax(segLen, zero);
+        __m128i vMaxScore = zero, vMaxMark = zero;
+        const int begin = reverse ? refLen - 1 : 0, end = reverse ? -1 : refLen, step = reverse ? -1 : 1;
         for (int i = begin; i != end; i += step) {
-            int vF[LANES] = {0}, vMaxCol[LANES] = {0}, vH[LANES];
-            // shift previous column's last segment up by one lane
-            for (int l = LANES - 1; l > 0; --l) vH[l] = HS[(size_t)(segLen - 1) * LANES + l - 1];
-            vH[0] = 0;
-            const int* vP = &prof[(size_t)ref[i] * segLen * LANES];
+            __m128i vF = zero, vMaxCol = zero;
+            __m128i vH = shift_up(HS[segLen - 1]);
+            const __m128i* vP = &prof[(size_t)ref[i] * segLen];
             std::swap(HS, HL);
             for (int j = 0; j < segLen; ++j) {
-                int* hs = &HS[(size_t)j * LANES];
-                int* ej = &E[(size_t)j * LANES];
-                const int* hl = &HL[(size_t)j * LANES];
-                const int* pj = vP + (size_t)j * LANES;
-                for (int l = 0; l < LANES; ++l) {
-                    int h = BYTE ? subs_u(sat(vH[l] + pj[l]), bias) : sat(vH[l] + pj[l]);
-                    int e = ej[l];
-                    h = std::max(h, e);
-                    h = std::max(h, vF[l]);
-                    vMaxCol[l] = std::max(vMaxCol[l], h);
-                    hs[l] = h;
-                    int hg = subs_u(h, gapO);
-                    e = std::max(subs_u(e, gapE), hg);
-                    ej[l] = e;
-                    vF[l] = std::max(subs_u(vF[l], gapE), hg);
-                    vH[l] = hl[l];
-                }
+                vH = BYTE ? subs(adds(vH, vP[j]), vBias) : adds(vH, vP[j]);
+                __m128i e = E[j];
+                vH = vmax(vH, e);
+                vH = vmax(vH, vF);
+                vMaxCol = vmax(vMaxCol, vH);
+                HS[j] = vH;
+                vH = subs(vH, vGapO);
+                e = vmax(subs(e, vGapE), vH);
+                E[j] = e;
+                vF = vmax(subs(vF, vGapE), vH);
+                vH = HL[j];
             }
-            // lazy-F: propagate F across lane boundaries without touching E
+            // lazy F: at most LANES passes, F carried one lane up per pass, E left
+            // as it is; stops at the first segment where F, after the extension,
+            // cannot exceed that segment's updated H minus the opening
             bool done = false;
             for (int k = 0; k < LANES && !done; ++k) {
-                for (int l = LANES - 1; l > 0; --l) vF[l] = vF[l - 1];
-                vF[0] = 0;
+                vF = shift_up(vF);
                 for (int j = 0; j < segLen; ++j) {
-                    int* hs = &HS[(size_t)j * LANES];
-                    bool all_le = true;
-                    for (int l = 0; l < LANES; ++l) {
-                        int h = std::max(hs[l], vF[l]);
-                        vMaxCol[l] = std::max(vMaxCol[l], h);
-                        hs[l] = h;
-                        int hg = subs_u(h, gapO);
-                        vF[l] = subs_u(vF[l], gapE);
-                        if (vF[l] > hg) all_le = false;
-                    }
-                    if (all_le) { done = true; break; }
+                    vH = vmax(HS[j], vF);
+                    vMaxCol = vmax(vMaxCol, vH);
+                    HS[j] = vH;
+                    vH = subs(vH, vGapO);
+                    vF = subs(vF, vGapE);
+                    if (all_le(vF, vH)) { done = true; break; }
                 }
             }
-            bool changed = false;
-            for (int l = 0; l < LANES; ++l) {
-                vMaxScore[l] = std::max(vMaxScore[l], vMaxCol[l]);
-                if (vMaxMark[l] != vMaxScore[l]) changed = true;
-            }
-            if (changed) {
-                int temp = 0;
-                for (int l = 0; l < LANES; ++l) { vMaxMark[l] = vMaxScore[l]; temp = std::max(temp, vMaxScore[l]); }
+            vMaxScore = vmax(vMaxScore, vMaxCol);
+            if (!all_le(vMaxScore, vMaxMark) || !all_le(vMaxMark, vMaxScore)) {
+                vMaxMark = vMaxScore;
+                const int temp = hmax(vMaxScore);
                 if (temp > maxv) {
                     maxv = temp;
                     if (BYTE && maxv + bias >= 255) break;
@@ -201,14 +216,19 @@ struct Striped {
                     Hmax = HS;
                 }
             }
-            int colmax = 0;
-            for (int l = 0; l < LANES; ++l) colmax = std::max(colmax, vMaxCol[l]);
-            if (colmax == terminate) break;
+            if (hmax(vMaxCol) == terminate) break;
         }
-        for (int idx = 0; idx < segLen * LANES; ++idx) {
-            if (Hmax[idx] == maxv) {
-                int32_t t = idx / LANES + (idx % LANES) * segLen;
-                if (t < end_read) end_read = t;
+        for (int s = 0; s < segLen; ++s) {
+            alignas(16) int16_t w[8];
+            alignas(16) uint8_t b[16];
+            if (BYTE) _mm_store_si128((__m128i*)b, Hmax[s]);
+            else _mm_store_si128((__m128i*)w, Hmax[s]);
+            for (int l = 0; l < LANES; ++l) {
+                const int v = BYTE ? b[l] : w[l];
+                if (v == maxv) {
+                    const int32_t t = s + l * segLen;
+                    if (t < end_read) end_read = t;
+                }
             }
         }
         EndPos r;
@@ -305,8 +325,8 @@ SswResult ssw_align(const int8_t* read, int readLen, const int8_t* ref, int refL
     int bias = 0;
     for (int i = 0; i < n * n; ++i) bias = std::min<int>(bias, mat[i]);
     bias = -bias;
-    using B = Striped<16, true>;
-    using W = Striped<8, false>;
+    using B = StripedSse<true>;
+    using W = StripedSse<false>;
     EndPos best = B::run(ref, false, refLen, readLen, gapO, gapE, B::profile(read, readLen, mat, n, bias), 255, bias);
     bool word = false;
     if (best.score == 255) {

@@ -304,7 +304,25 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         uint32_t pi = NONE32;
         const uint32_t lookup_end = i > cp.max_lookback ? i - cp.max_lookback : 0;
         const uint32_t span = i - lookup_end;
-        for (uint32_t c0 = 0; c0 < span; c0 += G) {
+        // The scalar loop's first predecessor j = i - 1 settles most steps (the
+        // group decides it uniformly, without the lane scan): a break there ends
+        // the loop with nothing taken; taken on the same diagonal, the loop stops.
+        bool settled = span == 0;
+        if (!settled) {
+            const AnchorL aj = L.a[i - 1];
+            const uint32_t dq = ai.q - aj.q, dr = ai.rs - aj.rs;
+            if (aj.ref != ai.ref || (ai.q >= aj.q && dr >= max_ref_gap)) {
+                settled = true;
+            } else if (ai.q >= aj.q && dq == dr && !ratio_exceeds(dq, dr, rt)) {
+                const float ns = aj.dp + (dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab));
+                if (ns >= dpi) {
+                    dpi = ns;
+                    pi = i - 1;
+                    settled = true;
+                }
+            }
+        }
+        for (uint32_t c0 = 0; c0 < span && !settled; c0 += G) {
             const uint32_t l = c0 + r;
             bool brk = false, cand = false, diag = false;
             float ns = NEG;
@@ -344,7 +362,9 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             }
             if (sm || bm) break;
         }
-        if (best_i != NONE32 && bA.ref == ai.ref && ai.q > bA.q) {
+        // the best-index re-check; when the loop took the best anchor itself it
+        // recomputes the same value, which cannot be strictly larger
+        if (best_i != NONE32 && best_i != pi && bA.ref == ai.ref && ai.q > bA.q) {
             const uint32_t dq = ai.q - bA.q, dr = ai.rs - bA.rs;
             const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
             // ratio <= R: dq >= 1 and dr >= 1 here, so the quotient is a number

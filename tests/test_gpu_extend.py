@@ -185,3 +185,28 @@ def _ext_pw_scores(orc, q, r, anchors, scores):
                                   ctypes.c_int64(len(anchors)), ctypes.c_uint64(5), sc, ctypes.c_uint64(5), 500, out,
                                   buf, ctypes.c_uint64(4096))
     return (list(out), buf.value.decode()) if ok else None
+
+
+def test_block_aligner_small_cases_vs_oracle(ctx, orc):
+    """global and x-drop alignment on the device over the small cases whose
+    oracle scores the brute-force checks in test_oracle.py establish (every
+    alignment enumerated): same score, ends and CIGAR."""
+    import itertools
+    say("case: block-aligner small cases on the device vs oracle")
+    rng = random.Random(5)
+    cases = [(bytes(a), bytes(b)) for m in range(1, 6) for n in range(1, 6)
+             for a in itertools.product(b"AC", repeat=m) for b in [rng.choice(list(itertools.product(b"AC", repeat=n)))]]
+    cases += [(bytes(rng.choice(b"ACGT") for _ in range(rng.randint(1, 7))),
+               bytes(rng.choice(b"ACGT") for _ in range(rng.randint(1, 7)))) for _ in range(300)]
+    kinds = ((ctx.EXT_GLOBAL, 0), (ctx.EXT_XDROP, 1), (ctx.EXT_XDROP_REV, 2))
+    for scores in ((2, 8, 12, 1, 10), (3, 2, 3, 1, 0), (1, 4, 6, 1, 5)):
+        jobs = [(kind, q, r, None, 0) for q, r in cases for kind, _ in kinds]
+        got = ctx.extend(jobs, scores=scores)
+        at = 0
+        for q, r in cases:
+            for _, mode in kinds:
+                g = got[at]
+                at += 1
+                out, cig = _pw_scores(orc, q, r, mode, scores)
+                assert (g[0], g[1], g[2], g[3], g[4], g[5], cigar(g[7])) == (1, out[0], out[1], out[2], out[3], out[4],
+                                                                             cig), (q, r, mode, scores)

@@ -60,3 +60,31 @@ def test_ssw_shim_matches_reference_ssw_c():
     assert not bad, bad[:3]
     assert sum(1 for v in vecs if v["res"] is None) > 10  # NULL returns covered
     assert sum(1 for v in vecs if v["res"] and v["res"][0] >= 255) > 50  # word fallback covered
+
+
+def test_ssw_shim_under_user_scores():
+    """The device SSW under the -A/-B/-O/-E matrices and gaps a user can pick
+    (ssw.c vectors of tests/golden/gen_ssw_score_vectors.py): linear gaps and
+    small openings, where the lazy-F loop decides scores, and the 16-bit pass."""
+    print("case: SSW link shim, 500 ssw.c vectors under user scores", flush=True)
+    lib = shim()
+    with open(os.path.join(GOLDEN, "ssw_score_vectors.json")) as f:
+        vecs = json.load(f)
+    bad = []
+    for i, v in enumerate(vecs):
+        m, x, go, ge = v["sc"]
+        q = bytes(int(c) for c in v["q"])
+        r = bytes(int(c) for c in v["r"])
+        mat = bytes(m if (a == b and a < 4) else (-x) & 0xFF for a in range(5) for b in range(5))
+        p = lib.ssw_init(q, len(q), mat, 5, 2)
+        assert p
+        a = lib.ssw_align(p, r, len(r), go, ge, 0x0F, 0, 2**31 - 1, max(len(q) // 2, 15))
+        assert a
+        A = a.contents
+        got = [A.score1, A.ref_begin1, A.ref_end1, A.read_begin1, A.read_end1]
+        cig = [A.cigar[k] for k in range(A.cigarLen)] if A.cigar else []
+        lib.align_destroy(a)
+        lib.init_destroy(p)
+        if (got, cig) != (v["res"], v["cigar"]):
+            bad.append((i, v["sc"], got, v["res"]))
+    assert not bad, bad[:3]
