@@ -201,6 +201,129 @@ def serve_varied(a, sa, synth, ctx, threads, slots):
     return out
 
 
+# Counter profiles -> per-launch traffic and VALU figures (bench.py --summarize-pmc).
+# Engine stages (the names the engine times) and the HIP kernels each launches
+# once per batch: a stage's traffic per launch is the sum over its kernels.
+STAGE_KERNELS = {"k_hits": ("k_hits", "k_hits_grp"), "k_chain": ("k_zero_u32", "k_chain_t1", "k_chain_tn", "k_chain"),
+                 "k_seed": ("k_seed",), "k_finalize": ("k_finalize",)}
+COPY_BYTES = 1 << 30  # k_hbm_copy reads and writes 1 GiB per dispatch
+VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each
+
+
+def _short(kernel_name):
+    return kernel_name.split("(")[0].replace("void ", "").replace("sa::", "").strip()
+
+
+def summarize_pmc(dbs, workload, batch_bases):
+    """Per-kernel counter averages per dispatch from rocprofv3 --pmc databases
+    (one pass per database), the stage traffic per launch, and the text table."""
+    import sqlite3
+    per = {}  # kernel -> counter -> {dispatch: value}
+    dur = {}  # kernel -> {dispatch: ns}
+    for db in dbs:
+        c = sqlite3.connect(db)
+        for kn, did, cn, val, st, en in c.execute(
+                "select kernel_name, dispatch_id, counter_name, value, start, end from counters_collection"):
+            k = _short(kn)
+            d = per.setdefault(k, {}).setdefault(cn, {})
+            d[did] = d.get(did, 0.0) + float(val)
+            dur.setdefault(k, {})[(db, did)] = en - st
+        c.close()
+
+    def avg(k, cn):
+        # a counter's per-dispatch mean, collected as the plain counter (summed
+        # over its dimensions here) or as its _sum form
+        d = per.get(k, {}).get(cn) or per.get(k, {}).get(cn[:-4] if cn.endswith("_sum") else cn + "_sum")
+        return sum(d.values()) / len(d) if d else None
+
+    def ndisp(k):
+        return max((len(v) for v in per.get(k, {}).values()), default=0)
+
+    def rd_bytes(k):  # every TCC/EA read request by size
+        a, b, c_ = avg(k, "TCC_EA0_RDREQ_32B_sum"), avg(k, "TCC_EA0_RDREQ_64B_sum"), avg(k, "TCC_EA0_RDREQ_128B_sum")
+        return None if a is None else 32 * a + 64 * b + 128 * c_
+
+    def dram_rd(k):  # read requests destined for DRAM, in 32-byte units
+        v = avg(k, "TCC_EA0_RDREQ_DRAM_32B_sum")
+        return None if v is None else 32 * v
+
+    def dram_wr(k):
+        v = avg(k, "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum")
+        return None if v is None else 32 * v
+
+    def wr_bytes(k):
+        a, b = avg(k, "TCC_EA0_WRREQ_sum"), avg(k, "TCC_EA0_WRREQ_64B_sum")
+        return None if a is None else 32 * (a - b) + 64 * b
+
+    cp = "k_hbm_copy"
+    f_rd = COPY_BYTES / dram_rd(cp) if dram_rd(cp) else None
+    f_wr = COPY_BYTES / dram_wr(cp) if dram_wr(cp) else None
+    f_fetch = COPY_BYTES / 1024 / avg(cp, "FETCH_SIZE") if avg(cp, "FETCH_SIZE") else None
+    f_write = COPY_BYTES / 1024 / avg(cp, "WRITE_SIZE") if avg(cp, "WRITE_SIZE") else None
+    out = {"workload": workload, "unit": "bytes per launch", "schedule": {"engines": 1, "batch_bases": batch_bases},
+           "copy_kernel": {"bytes_read": COPY_BYTES, "bytes_written": COPY_BYTES,
+                           "dram_read_bytes": dram_rd(cp), "dram_write_bytes": dram_wr(cp),
+                           "fetch_size_kib": avg(cp, "FETCH_SIZE"), "write_size_kib": avg(cp, "WRITE_SIZE"),
+                           "dram_read_scale": f_rd, "dram_write_scale": f_wr,
+                           "fetch_size_scale": f_fetch, "write_size_scale": f_write},
+           "kernels": {}, "stages": {}}
+    rows = []
+    for k in sorted(per, key=lambda x: -(dram_rd(x) or 0) * ndisp(x)):
+        e = {"dispatches": ndisp(k)}
+        if dram_rd(k) is not None and f_rd and f_wr and dram_wr(k) is not None:
+            e.update(dram_read_bytes=dram_rd(k), dram_write_bytes=dram_wr(k), ea_read_bytes=rd_bytes(k),
+                     ea_write_bytes=wr_bytes(k), hbm_bytes=int(dram_rd(k) * f_rd + dram_wr(k) * f_wr))
+        if avg(k, "FETCH_SIZE") is not None and f_fetch:
+            e["fetch_size_bytes_scaled"] = int(avg(k, "FETCH_SIZE") * 1024 * f_fetch)
+        if avg(k, "WRITE_SIZE") is not None and f_write:
+            e["write_size_bytes_scaled"] = int(avg(k, "WRITE_SIZE") * 1024 * f_write)
+        vi = avg(k, "SQ_INSTS_VALU_sum")
+        if vi is not None:
+            ns = sum(dur[k].values()) / len(dur[k])
+            e.update(valu_insts=vi, valu_issue_frac=vi / (ns * 1e-9 * VALU_WAVE_INSTR_PEAK),
+                     salu_insts=avg(k, "SQ_INSTS_SALU_sum"),
+                     valu_int32_insts=avg(k, "SQ_INSTS_VALU_INT32_sum"))
+            act, thr = avg(k, "SQ_ACTIVE_INST_VALU_sum"), avg(k, "SQ_THREAD_CYCLES_VALU_sum")
+            if act:
+                e["valu_active_lane_pct"] = 100.0 * thr / (act * 64) if thr is not None else None
+        out["kernels"][k] = e
+        rows.append((k, e))
+    for stage, ks in STAGE_KERNELS.items():
+        members = [k for k in per if k.split("<")[0] in ks]
+        main = max(members, key=ndisp, default=None)
+        if not main or any("hbm_bytes" not in out["kernels"][k] for k in members):
+            continue
+        launches = ndisp(main)
+        tot = lambda f: sum(out["kernels"][k][f] * out["kernels"][k]["dispatches"] for k in members) / launches
+        st = {"launches": launches, "kernels": sorted(members), "hbm_bytes_per_launch": int(tot("hbm_bytes")),
+              "dram_read_bytes": int(tot("dram_read_bytes")), "dram_write_bytes": int(tot("dram_write_bytes"))}
+        if all("fetch_size_bytes_scaled" in out["kernels"][k] and "write_size_bytes_scaled" in out["kernels"][k]
+               for k in members):
+            st["fetch_write_size_bytes_scaled"] = int(tot("fetch_size_bytes_scaled") + tot("write_size_bytes_scaled"))
+        out["stages"][stage] = st
+    lines = ["%-28s %6s %14s %14s %12s %12s %9s %7s" % ("kernel", "disp", "dram_rd_MB", "dram_wr_MB", "hbm_MB",
+                                                        "fetchws_MB", "valu_iss", "lanes%")]
+    for k, e in rows:
+        f = lambda x, s=1e6: ("%.1f" % (e[x] / s)) if e.get(x) is not None else "-"
+        lines.append("%-28s %6d %14s %14s %12s %12s %9s %7s" % (
+            k[:28], e["dispatches"], f("dram_read_bytes"), f("dram_write_bytes"), f("hbm_bytes"),
+            ("%.1f" % ((e["fetch_size_bytes_scaled"] + e["write_size_bytes_scaled"]) / 1e6))
+            if "fetch_size_bytes_scaled" in e and "write_size_bytes_scaled" in e else "-",
+            ("%.3f" % e["valu_issue_frac"]) if e.get("valu_issue_frac") is not None else "-",
+            ("%.1f" % e["valu_active_lane_pct"]) if e.get("valu_active_lane_pct") is not None else "-"))
+    return out, "\n".join(lines)
+
+
+def schedule_note(trials):
+    """How the schedule was chosen: the autotuner's trials, or fixed."""
+    if not trials:
+        return "fixed"
+    n_lanes = sum(1 for t in trials if "lookup_lanes" in t)
+    n_sched = len(trials) - n_lanes
+    lanes = "lookup stage form over %d lane counts, then " % n_lanes if n_lanes else ""
+    return "autotuned: %s(engines, batch) over %d pairs" % (lanes, n_sched)
+
+
 def pct(v, p):
     v = sorted(v)
     return v[min(len(v) - 1, int(math.ceil(p / 100.0 * len(v))) - 1)]
@@ -226,24 +349,27 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
     per_launch_ms = ms / max(1, launches)
     bytes_per_launch = nbytes / max(1, launches)
     achieved = bytes_per_launch / (per_launch_ms / 1000.0) / 1e9 if per_launch_ms > 0 else 0.0
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    # of this workload (copy-kernel scaled)
+    # HBM bytes per launch from the committed counter profile of this workload's
+    # one-engine run (rocprofv3 --pmc, TCC read/write requests destined for DRAM,
+    # copy-kernel calibrated; bench.py --summarize-pmc): carried from the profile,
+    # not measured in this run
     tj = {}
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
-    measured = tj.get("kernels", {}) if tj.get("workload") == workload else {}
-    traffic = measured[dom]["hbm_bytes_per_launch_scaled"] if dom in measured else None
+    measured = tj.get("stages", {}) if tj.get("workload") == workload else {}
+    traffic = measured[dom]["hbm_bytes_per_launch"] if dom in measured else None
     # the profile's batch size may differ from this run's schedule: per-launch
     # traffic then scales with the bases per batch (noted in the line)
     prof_batch = tj.get("schedule", {}).get("batch_bases")
-    traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE, same schedule"
+    traffic_note = ("carried from profiles/pmc_traffic.json (rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B and "
+                    "TCC_EA0_WRREQ_WRITE_DRAM_32B of the same one-engine command, k_hbm_copy calibrated), "
+                    "not measured in this run")
     if traffic is not None and prof_batch and prof_batch != run_batch:
         traffic = int(traffic * run_batch / prof_batch)
-        traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE at %d Mbases per batch, scaled to %d" % (
-            prof_batch >> 20, run_batch >> 20)
-    # index lookups: useful bytes (counted per probe) against the sector traffic
+        traffic_note += "; profiled at %d Mbases per batch, scaled to %d" % (prof_batch >> 20, run_batch >> 20)
+    # index lookups: useful bytes (counted per probe) against the DRAM-side traffic
     lookup = None
     if kd.get("k_hits", z)[0] > 0:
         ms_, l_, _, b_, _ = kd["k_hits"]
@@ -251,10 +377,11 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
         lookup = {"kernel": "k_hits", "avg_launch_ms": round(avg, 3), "useful_bytes_per_launch": int(b_ / max(1, l_)),
                   "useful_gbps": round(b_ / (ms_ / 1000.0) / 1e9, 2), "peak_gbps": HBM_PEAK_GBS}
         if "k_hits" in measured:
-            tb = measured["k_hits"]["hbm_bytes_per_launch_scaled"]
+            tb = measured["k_hits"]["hbm_bytes_per_launch"]
             if prof_batch and prof_batch != run_batch:
                 tb = int(tb * run_batch / prof_batch)
-            lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1)})
+            lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1),
+                           "traffic_source": "carried from profiles/pmc_traffic.json"})
     # forward SSW pass: DP cells per second against the VALU integer peak
     # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
     # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
@@ -326,13 +453,14 @@ def bench_workload(a, sa, synth, cores, threads):
     upload_s = time.perf_counter() - t0
     n_units = len(reads)
     unit = "read pairs/s" if human else "reads/s"
-    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
         t0 = time.perf_counter()
         trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs or min(n_units, 500_000))
-        print("autotune: %d engines, %d Mbases per batch (%d trials, %.1f s)" %
-              (opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
+        print("autotune: %d lookup lanes, %d engines, %d Mbases per batch (%d trials, %.1f s)" %
+              (opts.lookup_lanes, opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
     runs = [("sam", 0, opts.slots), ("paf", 1, opts.slots)] if human else [("sam", 0, opts.slots)]
     if not human and a.single_steps > 0:
         runs.append(("plain", 0, 1))
@@ -400,7 +528,7 @@ def bench_workload(a, sa, synth, cores, threads):
               "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
               "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
               "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-              "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed"}
+              "lookup_lanes": opts.lookup_lanes or 16, "schedule": schedule_note(trials)}
     if human:
         config["paf_pairs_per_s"] = round(res["paf"][0], 1)
     if "plain" in res:
@@ -452,6 +580,8 @@ def main():
                     help="human/se_var: build the index on the host instead of the device")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
+    ap.add_argument("--lookup-lanes", type=int, default=0, choices=[0, 1, 8, 16, 64],
+                    help="lanes per read orientation in the lookup stage (0: autotuned, else library default)")
     ap.add_argument("--single-steps", type=int, default=2, help="timed steps of the one-engine comparison (0: skip)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use the library's default schedule instead of sa_autotune")
@@ -460,7 +590,16 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-varied", type=int, default=400,
                     help="requests of a serving run with 100-300 bp reads on an r250 index (0: skip)")
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="DB",
+                    help="write profiles/pmc_traffic.json from rocprofv3 --pmc databases of this workload's "
+                         "one-engine run (one counter pass each) and print the per-kernel table; no GPU use")
     a = ap.parse_args()
+    if a.summarize_pmc:
+        res, table = summarize_pmc(a.summarize_pmc, a.workload, a.batch_bases or (64 << 20))
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json"), "w") as f:
+            json.dump(res, f, indent=1)
+        print(table)
+        sys.exit(0)
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -486,15 +625,16 @@ def main():
     t0 = time.perf_counter()
     reads.upload(ix)  # inputs resident in HBM before the timed region
     upload_s = time.perf_counter() - t0
-    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
         # schedule (engines x batch size) chosen by timing the workload itself
         # (sa_autotune; outputs are identical for every schedule)
         t0 = time.perf_counter()
         trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs)
-        print("autotune: %d engines, %d Mbases per batch (%d trials, %.1f s)" %
-              (opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
+        print("autotune: %d lookup lanes, %d engines, %d Mbases per batch (%d trials, %.1f s)" %
+              (opts.lookup_lanes, opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
     print("index %.1fs, %d pairs loaded, staged on the device in %.1f ms" % (index_s, len(reads), 1000 * upload_s),
           file=sys.stderr, flush=True)
     for i in range(a.warmup):
@@ -616,7 +756,8 @@ def main():
                    "pairs_per_s_incl_pcie": round(n_pairs * a.steps / (wall + a.steps * upload_s), 1),
                    "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests,
                    "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-                   "schedule": "autotuned over %d (engines, batch) pairs" % len(trials) if trials else "fixed",
+                   "lookup_lanes": opts.lookup_lanes or 16,
+                   "schedule": schedule_note(trials),
                    "plain_one_engine_pairs_per_s": round(single, 1) if single else None,
                    "batched_over_plain": round(value / single, 3) if single else None},
         # request latency of the serving configuration as specified (one read length

@@ -124,6 +124,10 @@ typedef struct sa_map_options {
      * penalties (0..127 each), soft-clipping end bonus, piecewise x-drop. */
     int32_t match_score, mismatch_penalty, gap_open_penalty, gap_extend_penalty;
     int32_t end_bonus, xdrop;
+    /* Lanes per read orientation in the lookup stage (find_hits): 1 (one thread
+     * makes the orientation's lookups in turn) or 8, 16, 64 (a lane per
+     * randstrobe); 0: default 16. The hits never depend on it, only the speed. */
+    int32_t lookup_lanes;
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */
@@ -166,12 +170,16 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
                  sa_run_stats* stats);
 
 /* Schedule autotuner: maps the first sample_pairs records of `reads` (0: all;
- * resident if uploaded, output dropped) with each engine count in {1, 2, 3, 4, 6}
- * and batch size in {16, 32, 64} Mbases, and writes the fastest pair into
- * opt->slots and opt->batch_bases. The outcome never depends on the schedule (the
- * output is identical for every choice); only the speed does. report (may be
- * NULL) receives the trials as JSON [{"slots", "batch_bases", "ms"}, ...]. No
- * counterpart in the reference, which has no device schedule. */
+ * resident if uploaded, output dropped). Unless opt->lookup_lanes is set, it
+ * first times the lookup stage on one engine with 1, 8, 16 and 64 lanes per
+ * read orientation and keeps the fastest; then it maps the sample with each
+ * engine count in {1, 2, 3, 4, 6} and batch size in {16, 32, 64} Mbases, and
+ * writes the fastest choices into opt->lookup_lanes, opt->slots and
+ * opt->batch_bases. The outcome never depends on the schedule (the output is
+ * identical for every choice); only the speed does. report (may be NULL)
+ * receives the trials as JSON [{"lookup_lanes", "lookup_ms"}, ...,
+ * {"slots", "batch_bases", "ms"}, ...]. No counterpart in the reference, which
+ * has no device schedule. */
 int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, sa_map_options* opt, char* report,
                 size_t report_cap);
 

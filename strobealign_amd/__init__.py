@@ -39,7 +39,7 @@ class MapOptions(ctypes.Structure):
                 ("chunk_size", ctypes.c_uint64), ("threads", ctypes.c_int32), ("batch_bases", ctypes.c_uint64),
                 ("slots", ctypes.c_int32), ("match_score", ctypes.c_int32), ("mismatch_penalty", ctypes.c_int32),
                 ("gap_open_penalty", ctypes.c_int32), ("gap_extend_penalty", ctypes.c_int32),
-                ("end_bonus", ctypes.c_int32), ("xdrop", ctypes.c_int32)]
+                ("end_bonus", ctypes.c_int32), ("xdrop", ctypes.c_int32), ("lookup_lanes", ctypes.c_int32)]
 
 
 class Scores(ctypes.Structure):
@@ -125,9 +125,10 @@ class Reads:
 
 def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000, use_ssw=False, max_secondary=0,
             mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0, slots=0, scores=None,
-            xdrop=None):
+            xdrop=None, lookup_lanes=0):
     """Mapping options; scores = (match, mismatch, gap_open, gap_extend, end_bonus)
-    as -A/-B/-O/-E/-L (defaults 2, 8, 12, 1, 10)."""
+    as -A/-B/-O/-E/-L (defaults 2, 8, 12, 1, 10); lookup_lanes: lanes per read
+    orientation in the lookup stage (0: default)."""
     o = MapOptions()
     _lib.sa_map_options_default(ctypes.byref(o))
     if scores is not None:
@@ -136,7 +137,7 @@ def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
         o.xdrop = xdrop
     o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
     o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
-    o.batch_bases, o.slots = batch_bases, slots
+    o.batch_bases, o.slots, o.lookup_lanes = batch_bases, slots, lookup_lanes
     if rg_id is not None:
         o.rg_id = rg_id.encode()
     o._rg_keep = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])
@@ -322,10 +323,11 @@ class Index:
 
     def map_files(self, reads1, reads2=None, mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
                   use_ssw=False, max_secondary=0, mcs=0, details=False, fastq_comments=False, only_mapped=False,
-                  batch_bases=0, scores=None, xdrop=None):
+                  batch_bases=0, scores=None, xdrop=None, lookup_lanes=0):
         o = options(mode=mode, eqx=eqx, rg_id=rg_id, rg=rg, threads=threads, chunk_size=chunk_size, use_ssw=use_ssw,
                     max_secondary=max_secondary, mcs=mcs, details=details, fastq_comments=fastq_comments,
-                    only_mapped=only_mapped, batch_bases=batch_bases, scores=scores, xdrop=xdrop)
+                    only_mapped=only_mapped, batch_bases=batch_bases, scores=scores, xdrop=xdrop,
+                    lookup_lanes=lookup_lanes)
         out = ctypes.c_void_p()
         n = ctypes.c_size_t()
         _check(_lib.sa_map_files(self._p, ctypes.byref(o), reads1.encode(),
@@ -335,8 +337,8 @@ class Index:
         return s
 
     def autotune(self, reads, opts, sample_pairs=0):
-        """Chooses opts.slots / opts.batch_bases by timing the read set's leading
-        sample_pairs records (sa_autotune); returns the trials."""
+        """Chooses opts.lookup_lanes / opts.slots / opts.batch_bases by timing the
+        read set's leading sample_pairs records (sa_autotune); returns the trials."""
         import json
         buf = ctypes.create_string_buffer(8192)
         _check(_lib.sa_autotune(self._p, reads._p, ctypes.c_uint64(sample_pairs), ctypes.byref(opts), buf,

@@ -98,6 +98,11 @@ sa::Options options_of(const sa_map_options* o) {
     for (int32_t v : {o->match_score, o->mismatch_penalty, o->gap_open_penalty, o->gap_extend_penalty})
         if (v < 0 || v > 127) throw std::invalid_argument("scores and penalties must be in 0..127");
     if (o->end_bonus < 0) throw std::invalid_argument("end_bonus must be >= 0");
+    if (o->lookup_lanes) {
+        if (o->lookup_lanes != 1 && o->lookup_lanes != 8 && o->lookup_lanes != 16 && o->lookup_lanes != 64)
+            throw std::invalid_argument("lookup_lanes must be 0, 1, 8, 16 or 64");
+        r.lookup_lanes = (uint32_t)o->lookup_lanes;
+    }
     return r;
 }
 
@@ -530,17 +535,48 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
             dr.off.assign(reads->dr.off.begin(), reads->dr.off.begin() + (ptrdiff_t)((paired ? 2 : 1) * n + 1));
             pdr = &dr;
         }
+        std::string rep = "[";
+        // the lookup stage's form first (unless the caller fixed it): the sample
+        // on one engine with each lane count, timed by the lookup kernel's events
+        uint32_t lanes = options_of(opt).lookup_lanes;
+        if (opt->lookup_lanes == 0) {
+            double best_k = 1e300;
+            for (uint32_t ln : {1u, 8u, 16u, 64u}) {
+                sa::Options o = options_of(opt);
+                o.slots = 1;
+                o.lookup_lanes = ln;
+                const std::vector<sa::Engine*> eng = engines(index, 1);
+                auto lookup_ms = [&] {
+                    double ms = 0;
+                    for (const auto& k : eng[0]->kernel_stats())
+                        if (k.first == "k_hits") ms += k.second.ms;
+                    return ms;
+                };
+                const double before = lookup_ms();
+                sa::RunStats st;
+                sa::map_records(eng, index->ref, r1, r2, o, nullptr, &st, pdr);
+                const double ms = lookup_ms() - before;
+                char tmp[128];
+                std::snprintf(tmp, sizeof tmp, "%s{\"lookup_lanes\": %u, \"lookup_ms\": %.3f}",
+                              rep.size() > 1 ? ", " : "", ln, ms);
+                rep += tmp;
+                if (ms < best_k) {
+                    best_k = ms;
+                    lanes = ln;
+                }
+            }
+        }
         const int slot_set[] = {1, 2, 3, 4, 6};
         const uint64_t batch_set[] = {16ull << 20, 32ull << 20, 64ull << 20};
         double best = 1e300;
         int best_slots = 0;
         uint64_t best_batch = 0;
-        std::string rep = "[";
         for (int sl : slot_set)
             for (uint64_t bb : batch_set) {
                 sa::Options o = options_of(opt);
                 o.slots = sl;
                 o.batch_bases = bb;
+                o.lookup_lanes = lanes;
                 double ms = 0;
                 for (int rep_i = 0; rep_i < 2; ++rep_i) {  // the first run grows the buffers
                     sa::RunStats st;
@@ -560,6 +596,7 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
         rep += "]";
         opt->slots = best_slots;
         opt->batch_bases = best_batch;
+        opt->lookup_lanes = (int32_t)lanes;
         if (report && report_cap) {
             const size_t m = std::min(report_cap - 1, rep.size());
             std::memcpy(report, rep.data(), m);
@@ -709,7 +746,7 @@ int sa_map_batch(sa_index* index, const sa_map_options* opt, const uint8_t* seqs
         eng.upload_reads(seqs, offsets, n);
         sa::NamBatch nb;
         sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
-        eng.find_nams(sa::MapParams{o.mcs, o.rescue_distance}, cp, nb);
+        eng.find_nams(sa::MapParams{o.mcs, o.rescue_distance, o.lookup_lanes}, cp, nb);
         const uint64_t N = nb.nams.size();
         sa_nam* pn = (sa_nam*)std::malloc(std::max<uint64_t>(1, N) * sizeof(sa_nam));
         sa_nam_extension* pe = (sa_nam_extension*)std::calloc(std::max<uint64_t>(1, N), sizeof(sa_nam_extension));

@@ -667,8 +667,26 @@ void hit_pass(const IndexView& ix, const uint8_t* d_seq, const uint64_t* d_off, 
     k_seed<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, p, d_sy, d_qr, o.qrn);
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[1], st));
-    k_hits<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc, h.acap,
-                                                  o.hbytes);
+    switch (mp.lookup_lanes) {
+    case 1:
+        k_hits<<<blocks(2ull * n, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc, h.acap,
+                                                      o.hbytes);
+        break;
+    case 8:
+        k_hits_grp<8><<<blocks(2ull * n * 8, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc,
+                                                                 h.acap, o.hbytes);
+        break;
+    case 16:
+        k_hits_grp<16><<<blocks(2ull * n * 16, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc,
+                                                                   h.acap, o.hbytes);
+        break;
+    case 64:
+        k_hits_grp<64><<<blocks(2ull * n * 64, 256), 256, 0, st>>>(ix, mp, d_off, n, d_qr, o.qrn, h.hits, d_cnt, h.hc,
+                                                                   h.acap, o.hbytes);
+        break;
+    default:
+        throw std::invalid_argument("lookup lanes must be 1, 8, 16 or 64");
+    }
     SA_CHECK(hipGetLastError());
     if (ev) SA_CHECK(hipEventRecord(ev[2], st));
 }
@@ -1107,7 +1125,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     const uint64_t budget = 16ull << 30;
     auto waves = [&](uint32_t njobs, uint64_t per, auto&& launch) {
         uint64_t wave = std::max<uint64_t>(1, std::min<uint64_t>(njobs, budget / per));
-        auto* d_scr = (uint8_t*)grow(S_SCRATCH, wave * per);
+        // whole 64-job blocks: a kernel may interleave the scratch of a wave's jobs
+        auto* d_scr = (uint8_t*)grow(S_SCRATCH, (wave + 63) / 64 * 64 * per);
         for (uint64_t first = 0; first < njobs; first += wave)
             launch((uint32_t)std::min<uint64_t>(wave, njobs - first), (uint32_t)first, d_scr);
     };
@@ -1174,8 +1193,9 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
             read_counts();
             timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
         }
-        // banded traceback: band up to 64 without a retry
-        uint64_t per = 3ull * (2 * 64 + 4) * 4 + 129ull * L * 3 + 8ull * (2 * L + 64) + 4096;
+        // banded traceback: band up to 64 without a retry (words: three band rows,
+        // 17 words of packed direction cells per query row, the traceback ops)
+        uint64_t per = 4ull * (3 * (2 * 64 + 4) + 17ull * L + 2 * (2ull * L + 64) + 64);
         per = (per + 255) & ~255ull;
         // jobs grouped by the band width their traceback starts with
         auto* d_fin = (uint32_t*)grow(S_FINLISTS, (uint64_t)FIN_CLASSES * nssw * 4);

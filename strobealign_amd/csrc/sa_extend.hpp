@@ -292,17 +292,61 @@ SA_HD SswEnd ssw_pass(QC qcode, int32_t qlen, RC rcode, int32_t rlen, bool rever
                                     MatchMismatch{sc.match, sc.mismatch}, terminate, buf, nullptr);
 }
 
-// Clears n bytes with 16-byte stores where aligned.
-SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
-    uint64_t x = 0;
-    while (x < n && (((uintptr_t)(p + x)) & 15)) p[x++] = 0;
-    for (; x + 16 <= n; x += 16) {
-        uint64_t* q = (uint64_t*)(p + x);
-        q[0] = 0;
-        q[1] = 0;
-    }
-    for (; x < n; ++x) p[x] = 0;
+// A lane's working memory in 32-bit words: word k at p[k * stride]. With
+// stride 64 the 64 lanes of a wave interleave their words, so that lanes at the
+// same k touch one 256-byte block (one job per lane, coalesced); stride 1 is
+// plain contiguous memory.
+struct LaneMem {
+    uint32_t* p;
+    uint32_t stride;
+    uint64_t words;
+    SA_HD uint32_t& operator[](uint64_t k) const { return p[k * stride]; }
+};
+
+SA_HD LaneMem contiguous_mem(uint8_t* scratch, uint64_t bytes) {
+    return LaneMem{(uint32_t*)scratch, 1, bytes / 4};
 }
+
+// Direction codes of the banded DP (ext/ssw/ssw.c:605-660), three per cell:
+// E (2 or 3), F (4 or 5) and H (1, or the cell's E or F code). ssw.c keeps them
+// in a byte array, cell j of row i at byte 3 * (width_d * i + j - x_i) + p.
+// Here a cell is 4 bits (0: never written, else 1 + [E == 3] + 2 [F == 5] +
+// 4 kind, kind 0 H = 1, 1 H = E, 2 H = F), 8 cells per word, a band row in
+// ceil(width_d / 8) words; a traceback read outside a row's cells finds the
+// cell and code that ssw.c's byte index names, so reads that leave the band
+// resolve as they do there.
+SA_HD uint32_t dir_cell(int32_t de, int32_t df, bool diag, bool e_over_f) {
+    const uint32_t kind = diag ? 0u : (e_over_f ? 1u : 2u);
+    return 1u + (de == 3 ? 1u : 0u) + (df == 5 ? 2u : 0u) + 4u * kind;
+}
+
+struct DirView {
+    LaneMem m;
+    uint64_t base;    // first word of the rows
+    int32_t width_d;  // cells per band row
+    int32_t wd;       // words per band row
+    int32_t rows;
+    // code `part` of cell `cell` of row `row`: ssw.c's byte
+    // 3 * (width_d * row + cell) + part, which lies in another row when cell is
+    // outside [0, width_d)
+    SA_HD int32_t code(int32_t row, int32_t cell, int32_t part) const {
+        if (cell < 0 || cell >= width_d) {
+            const int64_t byte = 3ll * ((int64_t)width_d * row + cell) + part;
+            if (byte < 0) return 0;
+            const int64_t rb = 3ll * width_d;
+            row = (int32_t)(byte / rb);
+            const int32_t rem = (int32_t)(byte % rb);
+            cell = rem / 3;
+            part = rem % 3;
+        }
+        if (row >= rows) return 0;
+        const uint32_t n = (m[base + (uint64_t)row * wd + (uint32_t)cell / 8] >> (4 * (cell % 8))) & 15u;
+        if (n == 0) return 0;
+        const uint32_t v = n - 1;
+        const int32_t de = 2 + (int32_t)(v & 1), df = 4 + (int32_t)((v >> 1) & 1), kind = (int32_t)(v >> 2);
+        return part == 0 ? de : part == 1 ? df : (kind == 0 ? 1 : kind == 1 ? de : df);
+    }
+};
 
 // One pass of the banded DP of ext/ssw/ssw.c:605-660 for band width bw <= BWMAX
 // with the three band rows (previous H, E carried in place, current H) in
@@ -311,16 +355,18 @@ SA_HD void zero_bytes(uint8_t* p, uint64_t n) {
 // and one below, and every index is static once the cell loop is unrolled over u.
 // The reference codes of the row's columns x_i .. x_i + 2*BWMAX are kept in
 // registers too and slide by one when x_i advances, so each row reads one new
-// reference base. Writes the same direction bytes and returns the same maximum as
-// the scratch form (band_pass_mem), including its resets of the row entries at 0
-// and `edge`.
+// reference base. Each row's direction cells are packed in registers and stored
+// as whole words (cells past the row's end as 0, as ssw.c's cleared array reads
+// there). Returns the same maximum as the scratch form (band_pass_mem),
+// including its resets of the row entries at 0 and `edge`.
 template <int BWMAX, typename QC, typename RC, typename SM>
 SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, const SM& score,
-                            int32_t bw, int8_t* dir) {
+                            int32_t bw, const DirView& dv) {
     constexpr int WM = 2 * BWMAX + 3;  // largest row width
     constexpr int UM = 2 * BWMAX + 1;  // largest number of cells in a row
+    constexpr int WDM = (UM + 7) / 8;  // largest number of words in a row
     const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
-    const int32_t width = bw * 2 + 3, width_d = bw * 2 + 1;
+    const int32_t width = bw * 2 + 3;
     int32_t hb[WM + 1], eb[WM + 1], hc[WM + 1];
     SA_UNROLL
     for (int x = 0; x <= WM; ++x) hb[x] = eb[x] = hc[x] = 0;
@@ -344,9 +390,11 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
         for (int x = 1; x < WM; ++x)
             if (x == edge) hb[x] = eb[x] = 0;
         const int32_t uend = end - xi + 1;
-        int8_t* dl = dir + (uint64_t)width_d * i * 3;
         const int32_t qi = qc(i);
         int32_t f = 0;
+        uint32_t wv[WDM];
+        SA_UNROLL
+        for (int w = 0; w < WDM; ++w) wv[w] = 0;
         SA_UNROLL
         for (int u = 1; u <= UM; ++u) {
             if (u <= uend) {
@@ -357,22 +405,24 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
                 int32_t t2 = i == 0 ? -gapE : ebe - gapE;
                 const int32_t ev = t1 > t2 ? t1 : t2;
                 eb[u] = ev;
-                const int8_t de = t1 > t2 ? 3 : 2;
+                const int32_t de = t1 > t2 ? 3 : 2;
                 t1 = hc[u - 1] - gapO;
                 t2 = f - gapE;
                 f = t1 > t2 ? t1 : t2;
-                const int8_t df = t1 > t2 ? 5 : 4;
+                const int32_t df = t1 > t2 ? 5 : 4;
                 const int32_t e1 = ev > 0 ? ev : 0, f1 = f > 0 ? f : 0;
                 t1 = e1 > f1 ? e1 : f1;
                 t2 = hbd + score(rw[u - 1], qi);
                 const int32_t h = t1 > t2 ? t1 : t2;
                 hc[u] = h;
                 maxv = h > maxv ? h : maxv;
-                dl[(u - 1) * 3] = de;
-                dl[(u - 1) * 3 + 1] = df;
-                dl[(u - 1) * 3 + 2] = t1 <= t2 ? (int8_t)1 : (e1 > f1 ? de : df);
+                wv[(u - 1) / 8] |= dir_cell(de, df, t1 <= t2, e1 > f1) << (4 * ((u - 1) % 8));
             }
         }
+        const uint64_t rowp = dv.base + (uint64_t)i * dv.wd;
+        SA_UNROLL
+        for (int w = 0; w < WDM; ++w)
+            if (w < dv.wd) dv.m[rowp + w] = wv[w];
         SA_UNROLL
         for (int u = 1; u <= UM; ++u)
             if (u <= uend) hb[u] = hc[u];
@@ -380,90 +430,99 @@ SA_HD int32_t band_pass_reg(QC qc, RC rc, int32_t refLen, int32_t readLen, const
     return maxv;
 }
 
-// The same pass with the band rows in scratch memory (any band width).
+// The same pass with the band rows in the lane's memory (any band width).
 template <typename QC, typename RC, typename SM>
 SA_HD int32_t band_pass_mem(QC qc, RC rc, int32_t refLen, int32_t readLen, const Scores& sc, const SM& score,
-                            int32_t bw, int32_t* hb, int32_t* eb, int32_t* hc, int8_t* dir) {
+                            int32_t bw, LaneMem m, uint64_t hb, uint64_t eb, uint64_t hc, const DirView& dv) {
     const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
-    const int32_t width = bw * 2 + 3, width_d = bw * 2 + 1;
+    const int32_t width = bw * 2 + 3;
+    auto H = [&](uint64_t a, int32_t x) -> uint32_t& { return m[a + (uint32_t)x]; };
     auto U = [&](int32_t i, int32_t j) { int32_t x = i - bw; x = x > 0 ? x : 0; return j - x + 1; };
-    auto D = [&](int32_t i, int32_t j, int32_t p) { int32_t x = i - bw; x = x > 0 ? x : 0; return (j - x) * 3 + p; };
     int32_t maxv = 0;
-    for (int32_t x = 0; x <= width; ++x) hb[x] = eb[x] = hc[x] = 0;
+    for (int32_t x = 0; x <= width; ++x) H(hb, x) = H(eb, x) = H(hc, x) = 0;
     for (int32_t i = 0; i < readLen; ++i) {
         int32_t beg = i - bw > 0 ? i - bw : 0, end = refLen - 1 < i + bw ? refLen - 1 : i + bw;
         int32_t edge = end + 1 < width - 1 ? end + 1 : width - 1;
         int32_t f = 0;
-        hb[0] = eb[0] = hb[edge] = eb[edge] = hc[0] = 0;
-        int8_t* dl = dir + (uint64_t)width_d * i * 3;
+        H(hb, 0) = H(eb, 0) = H(hb, edge) = H(eb, edge) = H(hc, 0) = 0;
         int32_t u = 0;
         const int32_t qi = qc(i);
+        const uint64_t rowp = dv.base + (uint64_t)i * dv.wd;
+        uint32_t word = 0;
+        int32_t wi = 0;  // words of the row stored
         for (int32_t j = beg; j <= end; ++j) {
             u = U(i, j);
             int32_t e = U(i - 1, j), b = U(i, j - 1), d = U(i - 1, j - 1);
-            int32_t de = D(i, j, 0), df = D(i, j, 1), dh = D(i, j, 2);
-            int32_t t1 = i == 0 ? -gapO : hb[e] - gapO;
-            int32_t t2 = i == 0 ? -gapE : eb[e] - gapE;
-            eb[u] = t1 > t2 ? t1 : t2;
-            dl[de] = t1 > t2 ? 3 : 2;
-            t1 = hc[b] - gapO;
+            int32_t t1 = i == 0 ? -gapO : (int32_t)H(hb, e) - gapO;
+            int32_t t2 = i == 0 ? -gapE : (int32_t)H(eb, e) - gapE;
+            const int32_t ev = t1 > t2 ? t1 : t2;
+            H(eb, u) = (uint32_t)ev;
+            const int32_t de = t1 > t2 ? 3 : 2;
+            t1 = (int32_t)H(hc, b) - gapO;
             t2 = f - gapE;
             f = t1 > t2 ? t1 : t2;
-            dl[df] = t1 > t2 ? 5 : 4;
-            int32_t e1 = eb[u] > 0 ? eb[u] : 0, f1 = f > 0 ? f : 0;
+            const int32_t df = t1 > t2 ? 5 : 4;
+            int32_t e1 = ev > 0 ? ev : 0, f1 = f > 0 ? f : 0;
             t1 = e1 > f1 ? e1 : f1;
-            t2 = hb[d] + score(rc(j), qi);
-            hc[u] = t1 > t2 ? t1 : t2;
-            if (hc[u] > maxv) maxv = hc[u];
-            if (t1 <= t2) dl[dh] = 1;
-            else dl[dh] = e1 > f1 ? dl[de] : dl[df];
+            t2 = (int32_t)H(hb, d) + score(rc(j), qi);
+            const int32_t h = t1 > t2 ? t1 : t2;
+            H(hc, u) = (uint32_t)h;
+            if (h > maxv) maxv = h;
+            word |= dir_cell(de, df, t1 <= t2, e1 > f1) << (4 * ((u - 1) % 8));
+            if ((u - 1) % 8 == 7) {
+                dv.m[rowp + (uint32_t)wi++] = word;
+                word = 0;
+            }
         }
-        for (int32_t j = 1; j <= u; ++j) hb[j] = hc[j];
+        // the partial word and the rest of the row: cells never written
+        for (; wi < dv.wd; ++wi) {
+            dv.m[rowp + (uint32_t)wi] = word;
+            word = 0;
+        }
+        for (int32_t j = 1; j <= u; ++j) H(hb, j) = H(hc, j);
     }
     return maxv;
 }
 
-// Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774).
-// Emits M/I/D ops in forward order through `emit(op, len)`. Returns 0 ok,
-// 1 traceback failure, 2 scratch too small.
+// Banded global alignment with band doubling and traceback (ext/ssw/ssw.c:590-774)
+// in the lane's memory m. Emits M/I/D ops in forward order through
+// `emit(op, len)`. Returns 0 ok, 1 traceback failure, 2 memory too small.
 template <typename QC, typename RC, typename Emit, typename SM>
 SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score, const Scores& sc, const SM& sm,
-                    int32_t bw, uint8_t* scratch, uint64_t scratch_bytes, Emit emit) {
+                    int32_t bw, LaneMem m, Emit emit) {
     int32_t len = refLen > readLen ? refLen : readLen;
     int32_t maxv = 0;
     int32_t width = 0, width_d = 0;
-    int8_t* dir = nullptr;
-    auto D = [&](int32_t i, int32_t j, int32_t p) { int32_t x = i - bw; x = x > 0 ? x : 0; return (j - x) * 3 + p; };
+    DirView dv{m, 0, 0, 0, readLen};
+    uint64_t used = 0;
     do {
         width = bw * 2 + 3;
         width_d = bw * 2 + 1;
-        uint64_t need = 3ull * (uint64_t)(width + 1) * 4 + (uint64_t)width_d * (uint64_t)readLen * 3 + 3;
-        if (need > scratch_bytes) return 2;
-        int32_t* hb = (int32_t*)scratch;
-        int32_t* eb = hb + (width + 1);
-        int32_t* hc = eb + (width + 1);
-        dir = (int8_t*)(hc + (width + 1));
-        zero_bytes((uint8_t*)dir, (uint64_t)width_d * readLen * 3 + 3);
-        if (bw <= 1) maxv = band_pass_reg<1>(qc, rc, refLen, readLen, sc, sm, bw, dir);
-        else if (bw <= 2) maxv = band_pass_reg<2>(qc, rc, refLen, readLen, sc, sm, bw, dir);
-        else if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dir);
-        else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, sm, bw, dir);
-        else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, sm, bw, hb, eb, hc, dir);
+        // words: the three band rows of the memory form, then the direction rows
+        const uint64_t rows_at = 3ull * (uint64_t)(width + 1);
+        dv = DirView{m, rows_at, width_d, (width_d + 7) / 8, readLen};
+        used = rows_at + (uint64_t)dv.wd * (uint64_t)readLen;
+        if (used > m.words) return 2;
+        if (bw <= 1) maxv = band_pass_reg<1>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else if (bw <= 2) maxv = band_pass_reg<2>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, sm, bw, m, 0, (uint64_t)(width + 1),
+                                  2ull * (uint64_t)(width + 1), dv);
         bw *= 2;
     } while (maxv < score && bw <= len);
     bw /= 2;
-    // traceback (ops collected backwards into the tail of scratch, emitted forwards)
-    uint64_t used = 3ull * (uint64_t)(width + 1) * 4 + (uint64_t)width_d * (uint64_t)readLen * 3 + 3;
-    uint64_t room = (scratch_bytes - used) / 8;
-    uint32_t* st = (uint32_t*)(scratch + ((used + 7) & ~7ull));
-    if (room < 4) return 2;
-    room -= 1;
-    uint32_t nst = 0;
+    // traceback (ops collected backwards after the direction rows, emitted forwards)
+    if (m.words - used < 4) return 2;
+    const uint64_t room = m.words - used - 1;
+    const uint64_t st = used;
+    uint64_t nst = 0;
     int32_t i = readLen - 1, j = refLen - 1, e = 0, t2 = 2, row = readLen - 1;
     char op = 'M', prev = 'M';
     auto code = [](char o) { return o == 'M' ? 0u : (o == 'I' ? 1u : 2u); };
     while (i >= 0 && j > 0) {
-        int8_t v = dir[(uint64_t)width_d * row * 3 + D(i, j, t2)];
+        const int32_t x = i - bw > 0 ? i - bw : 0;
+        const int32_t v = dv.code(row, j - x, t2);
         switch (v) {
             case 1: --i; --j; t2 = 2; --row; op = 'M'; break;
             case 2: --i; t2 = 0; --row; op = 'I'; break;
@@ -475,15 +534,18 @@ SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score
         if (op == prev) ++e;
         else {
             if (nst >= room) return 2;
-            st[nst++] = ((uint32_t)e << 4) | code(prev);
+            m[st + nst++] = ((uint32_t)e << 4) | code(prev);
             prev = op;
             e = 1;
         }
     }
     if (nst + 2 >= room) return 2;
-    if (op == 'M') st[nst++] = ((uint32_t)(e + 1) << 4) | code(op);
-    else { st[nst++] = ((uint32_t)e << 4) | code(op); st[nst++] = (1u << 4) | 0u; }
-    for (uint32_t x = nst; x-- > 0;) emit(st[x] & 0xF, st[x] >> 4);
+    if (op == 'M') m[st + nst++] = ((uint32_t)(e + 1) << 4) | code(op);
+    else { m[st + nst++] = ((uint32_t)e << 4) | code(op); m[st + nst++] = (1u << 4) | 0u; }
+    for (uint64_t x = nst; x-- > 0;) {
+        const uint32_t v = m[st + x];
+        emit(v & 0xF, v >> 4);
+    }
     return 0;
 }
 
@@ -503,8 +565,8 @@ struct SswEnds {
 
 // Banded traceback, with_eqx CIGAR and the end-bonus extensions (ext/ssw/ssw.c:
 // 895-915 then src/ssw.rs:181-188 and src/aligner.rs:90-148), given the passes' result.
-SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e, const Scores& sc, uint8_t* scratch,
-                           uint64_t scratch_bytes, CigarOut& cig) {
+SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e, const Scores& sc, LaneMem m,
+                           CigarOut& cig) {
     AlignInfo a{false, 0, 0, 0, 0, 0, 0, 0};
     cig.reset();
     const int32_t score1 = e.score1, ref_end1 = e.ref_end1, read_end1 = e.read_end1;
@@ -544,7 +606,7 @@ SA_HD AlignInfo ssw_finish(const QueryView& q, const RefWin& r, const SswEnds& e
         // reference bases are 2-bit codes, which are their SSW codes
         int st = banded_sw([&](int32_t i) { return ssw_code(q[qs + (uint32_t)i]); },
                            [&](int32_t j) { return (int32_t)r.code(rs + (uint32_t)j); }, refL, readL, score1, sc,
-                           MatchMismatch{sc.match, sc.mismatch}, bw, scratch, scratch_bytes, emit_main);
+                           MatchMismatch{sc.match, sc.mismatch}, bw, m, emit_main);
         if (st == 1) return a;  // flag 1: no alignment
         if (st == 2) { a.status = 2; return a; }
     }
@@ -632,7 +694,7 @@ SA_HD AlignInfo ssw_aligner_align(const QueryView& q, const RefWin& r, const Sco
     e.ref_begin1 = br.ref;
     e.read_begin1 = e.read_end1 - br.read;
     e.rev_short = e.score1 > br.score;
-    return ssw_finish(q, r, e, sc, scratch + ssw_bytes, scratch_bytes - ssw_bytes, cig);
+    return ssw_finish(q, r, e, sc, contiguous_mem(scratch + ssw_bytes, scratch_bytes - ssw_bytes), cig);
 }
 
 // Outcome of ssw_align (the s_align fields of ext/ssw/ssw.h; the CIGAR goes to
@@ -737,7 +799,7 @@ SA_HD void ssw_align_raw(const int8_t* read, int32_t readLen, const int8_t* ref,
     };
     const Scores gaps{0, 0, gapO, gapE, 0, 0};
     const int st = banded_sw([&](int32_t i) { return (int32_t)read[qb + i]; }, [&](int32_t j) { return (int32_t)ref[rb + j]; },
-                             rL, qL, o.score1, gaps, sm, bw, rest, rest_bytes, emit);
+                             rL, qL, o.score1, gaps, sm, bw, contiguous_mem(rest, rest_bytes), emit);
     if (st == 2) { o.status = 2; return; }
     if (st == 1) { o.flag = 1; return; }
     if (over) { o.status = 3; return; }

@@ -84,6 +84,7 @@ struct Options {
     int threads = 1;
     uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
     int slots = 4;                        // engines working on batches concurrently
+    uint32_t lookup_lanes = 16;           // lanes per read orientation in the lookup stage (1, 8, 16, 64)
     // -A/-B/-O/-E/-L and --xdrop (src/main.rs:246-272; defaults src/aligner.rs:20-30)
     Scores scores{2, 8, 12, 1, 10, 500};
 };

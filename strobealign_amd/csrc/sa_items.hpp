@@ -413,7 +413,10 @@ SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, 
     CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     RefWin w{&ref, r.ref_id, j.ws, j.wlen};
-    AlignInfo a = ssw_finish(q, w, ends[job], sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
+    // the 64 jobs of a wave interleave their scratch words (the launch's scratch
+    // covers whole waves of scratch_per bytes per job)
+    const LaneMem m{(uint32_t*)(scratch + (uint64_t)(t / 64) * 64 * scratch_per) + t % 64, 64, scratch_per / 4};
+    AlignInfo a = ssw_finish(q, w, ends[job], sc, m, cig);
     out.status = (uint32_t)a.status;
     if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
     res[j.req] = out;

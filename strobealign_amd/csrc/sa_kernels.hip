@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sa_chain_group.hpp"
+#include "sa_hits_group.hpp"
 #include "sa_items.hpp"
 #include "sa_ssw_group.hpp"
 
@@ -23,6 +24,43 @@ __global__ void __launch_bounds__(256) k_hits(IndexView ix, MapParams mp, const 
                                              uint64_t* __restrict__ acap, uint32_t* __restrict__ lookup_bytes) {
     item_hits(blockIdx.x * blockDim.x + threadIdx.x, ix, mp, off, n, qr, qr_n, hits, cnt, hc, acap, lookup_bytes);
 }
+
+// find_hits on lane groups (sa_hits_group.hpp): one read orientation per G
+// lanes, same outputs as k_hits.
+template <int G>
+__global__ void __launch_bounds__(256) k_hits_grp(IndexView ix, MapParams mp, const uint64_t* __restrict__ off,
+                                                  uint32_t n, const QRand* __restrict__ qr,
+                                                  const uint32_t* __restrict__ qr_n, Hit* __restrict__ hits,
+                                                  uint64_t* __restrict__ cnt, HitCounts* __restrict__ hc,
+                                                  uint64_t* __restrict__ acap, uint32_t* __restrict__ lookup_bytes) {
+    const uint32_t lane = threadIdx.x % 64;
+    const Grp<G> gp{lane % G, lane / G * G};
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (t >= 2 * n) return;
+    const uint32_t i = t >> 1, o = t & 1;
+    const uint64_t ro = off[i];
+    const uint32_t L = (uint32_t)(off[i + 1] - ro);
+    const uint64_t base = 2 * ro + (uint64_t)o * L;
+    IndexView lx = ix;
+    HitCounts d;
+    uint64_t cap = 0;
+    const uint32_t nq = qr_n[t];
+    const uint32_t nh = find_hits_grp<G>(gp, lx, qr + base, nq, mp, hits + base, cnt + base, d, cap);
+    const uint32_t br = gp.sum(lx.bucket_reads), er = gp.sum(lx.entry_reads);
+    if (gp.r == 0) {
+        hc[t] = d;
+        acap[t] = cap;
+        // query randstrobes in, bucket_starts words and index entries read, hits out
+        lookup_bytes[t] = (uint32_t)sizeof(QRand) * nq + 8u * br + (uint32_t)sizeof(RefEntry) * er +
+                          (uint32_t)sizeof(Hit) * nh;
+    }
+}
+template __global__ void k_hits_grp<8>(IndexView, MapParams, const uint64_t*, uint32_t, const QRand*, const uint32_t*,
+                                       Hit*, uint64_t*, HitCounts*, uint64_t*, uint32_t*);
+template __global__ void k_hits_grp<16>(IndexView, MapParams, const uint64_t*, uint32_t, const QRand*,
+                                        const uint32_t*, Hit*, uint64_t*, HitCounts*, uint64_t*, uint32_t*);
+template __global__ void k_hits_grp<64>(IndexView, MapParams, const uint64_t*, uint32_t, const QRand*,
+                                        const uint32_t*, Hit*, uint64_t*, HitCounts*, uint64_t*, uint32_t*);
 
 // Chaining, first tier: one read orientation per group of G lanes with its
 // anchors in LDS (sa_chain_group.hpp), for anchor capacities up to tiers.cap[0]

@@ -1339,7 +1339,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         eng.upload_reads(seq.data(), off.data(), n_reads);
     }
     double t1 = now_ms();
-    const MapParams mp{o.mcs, o.rescue_distance};
+    const MapParams mp{o.mcs, o.rescue_distance, o.lookup_lanes};
     NamBatch nb;
     eng.find_nams(mp, kChain, nb);
     double t2 = now_ms();

@@ -75,6 +75,10 @@ struct ChainParams {
 struct MapParams {
     uint32_t mcs;  // 0 Always, 1 Rescue, 2 Off, 3 FirstStrobe (src/mcsstrategy.rs:4-19)
     uint32_t rescue_distance;
+    // lanes per read orientation in the lookup stage: 1 (a thread runs the
+    // orientation's lookups in turn) or 8, 16, 64 (a lane per randstrobe); the
+    // hits are the same either way
+    uint32_t lookup_lanes = 16;
 };
 
 // NAM (src/nam.rs:18-31). Chain anchors are stored separately as

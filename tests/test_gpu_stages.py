@@ -119,6 +119,29 @@ def test_find_hits_vs_oracle(sa, ctx, orc, workloads, mcs):
 
 
 @pytest.mark.parametrize("mcs", [0, 1, 2, 3])
+def test_lookup_lanes_give_the_same_nams(sa, ctx, workloads, mcs):
+    """The lookup stage's forms (one thread per read orientation making its
+    lookups in turn, or 8, 16 or 64 lanes with a lane per randstrobe, rescue on
+    the group) give the same NAM lists, f32 score bits included; the hits of
+    the 16-lane default are checked against the oracle above."""
+    for fasta, seqs in workloads:
+        say("case: lookup lanes 1/8/16/64, mcs=%s on %s" % (MCS[mcs], fasta))
+        with sa.Index(ctx, fasta, read_length=150, threads=8) as ix:
+            res = {ln: ix.map_batch(seqs, sa.options(mcs=mcs, lookup_lanes=ln), speculative=0)[0]
+                   for ln in (1, 8, 16, 64)}
+        for ln in (1, 8, 64):
+            assert res[ln] == res[16], ln
+        assert sum(len(x) for x in res[16]) > len(seqs) // 2
+
+
+def test_lookup_lanes_out_of_range(sa, ctx, workloads):
+    fasta, seqs = workloads[0]
+    with sa.Index(ctx, fasta, read_length=150) as ix:
+        with pytest.raises(sa.SaError, match="lookup_lanes"):
+            ix.map_batch(seqs[:4], sa.options(lookup_lanes=32), speculative=0)
+
+
+@pytest.mark.parametrize("mcs", [0, 1, 2, 3])
 def test_chain_stage_equals_nam_stage(sa, ctx, workloads, mcs):
     """sa_chain_batch over sa_find_hits_batch's output gives the NAM records of
     sa_find_nams_batch, byte for byte."""
