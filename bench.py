@@ -621,10 +621,26 @@ def main():
     t0 = time.time()
     ix = sa.Index(ctx, w["ref"], read_length=150, threads=threads)
     index_s = time.time() - t0
+    t0 = time.perf_counter()
     reads = sa.Reads(w["reads1"], w["reads2"])
+    host_parse_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     reads.upload(ix)  # inputs resident in HBM before the timed region
     upload_s = time.perf_counter() - t0
+    # the same files parsed on the device (sa_reads_load_device): file read,
+    # copy in, newline scan, record checks, staging; records must be equal
+    t0 = time.perf_counter()
+    dreads = sa.Reads(w["reads1"], w["reads2"], device=ix)
+    dev_ingest_s = time.perf_counter() - t0
+    ingest = {"host_parse_ms": round(1000 * host_parse_s, 1), "host_upload_ms": round(1000 * upload_s, 1),
+              "device_ingest_wall_ms": round(1000 * dev_ingest_s, 1),
+              "device_parse_ms": round(dreads.ingest_ms, 2),
+              "records_equal_sampled": all(dreads.record(i, m) == reads.record(i, m)
+                                           for i in range(0, len(reads), max(1, len(reads) // 1000)) for m in (0, 1))}
+    dreads.close()
+    print("ingest: host parse %.0f ms + upload %.0f ms; device ingest %.0f ms wall (%.1f ms on the device)" %
+          (1000 * host_parse_s, 1000 * upload_s, 1000 * dev_ingest_s, ingest["device_parse_ms"]),
+          file=sys.stderr, flush=True)
     opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
@@ -754,6 +770,8 @@ def main():
                    "pairs": n_pairs, "read_len": 150, "host_threads": threads,
                    "read_upload_ms": round(1000 * upload_s, 2),
                    "pairs_per_s_incl_pcie": round(n_pairs * a.steps / (wall + a.steps * upload_s), 1),
+                   "pairs_per_s_incl_device_ingest": round(n_pairs * a.steps / (wall + a.steps * dev_ingest_s), 1),
+                   "ingest": ingest,
                    "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests,
                    "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
                    "lookup_lanes": opts.lookup_lanes or 16,

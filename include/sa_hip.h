@@ -153,6 +153,23 @@ int sa_map_records(sa_index* index, const sa_map_options* opt, uint64_t n, const
 typedef struct sa_reads sa_reads;
 int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out);
 uint64_t sa_reads_count(const sa_reads* reads);
+/* Fields of record i of a read set (mate 0: reads1, 1: reads2), valid while the
+ * set lives: name (without "/1" or "/2", whose number goes to *end, else 0),
+ * comment (NULL when the header has none), uppercase sequence, qualities (NULL
+ * for FASTA input). The SequenceRecord of src/io/record.rs. */
+int sa_reads_record(const sa_reads* reads, uint64_t i, int mate, const char** name, const char** comment,
+                    const char** seq, const char** qual, int* end);
+/* Same read set parsed on the index's device (FASTQ; FASTA read files are
+ * parsed on the host): the files' bytes are copied in once, newlines found,
+ * records located and checked, and the uppercase sequences staged in HBM as
+ * sa_reads_upload leaves them; the host only gets back each record's
+ * location (name, comment, sequence and quality offsets) to keep the fields
+ * for output. Files ending in ".gz" are decompressed on the host first
+ * (src/io/xopen.rs:9-20). Errors are those of sa_reads_load. *ingest_ms (may be
+ * NULL) receives the device time from the copy of the files in to the last
+ * kernel. Replaces src/io/fastq.rs:22-95 for the device path. */
+int sa_reads_load_device(sa_index* index, const char* reads1, const char* reads2, sa_reads** out,
+                         double* ingest_ms);
 /* Stages the read set in the index's device memory, so that sa_map_reads starts
  * from reads already resident in HBM (no host-to-device copy per call). */
 int sa_reads_upload(sa_index* index, sa_reads* reads);

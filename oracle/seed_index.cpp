@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <stdexcept>
 
 namespace orc {
@@ -207,21 +208,20 @@ std::string PackedSeq::decode(size_t start, size_t end) const {
     return s;
 }
 
+// u8::is_ascii_whitespace, which trim_ascii_end uses (no vertical tab)
+static bool ascii_ws(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\f'; }
+
+static void trim_end(std::string& s) {
+    while (!s.empty() && ascii_ws(s.back())) s.pop_back();
+}
+
 // src/io/mod.rs:32-38 split_header
 static void split_header(const std::string& h0, std::string& name, std::string& comment, bool& has_comment) {
     std::string h = h0;
-    while (!h.empty() && (h.back() == ' ' || h.back() == '\t' || h.back() == '\n' || h.back() == '\r' ||
-                          h.back() == '\f' || h.back() == '\v'))
-        h.pop_back();
+    trim_end(h);
     size_t p = h.find_first_of(" \t");
     if (p == std::string::npos) { name = h; comment.clear(); has_comment = false; }
     else { name = h.substr(0, p); comment = h.substr(p + 1); has_comment = true; }
-}
-
-static void trim_end(std::string& s) {
-    while (!s.empty() && (s.back() == ' ' || s.back() == '\t' || s.back() == '\n' || s.back() == '\r' ||
-                          s.back() == '\f' || s.back() == '\v'))
-        s.pop_back();
 }
 
 // src/io/fasta.rs:116-165 (no gzip support in the oracle)
@@ -246,17 +246,33 @@ std::vector<RefSeq> read_fasta(const std::string& path) {
     return refs;
 }
 
-// src/io/fastq.rs:22-95 and src/io/fasta.rs:67-114 (reads given as FASTA)
+// BufRead::read_line / read_until(b'\n') over a file held in memory: the next
+// line including its '\n' (if it has one); empty at the end of the file.
+struct LineReader {
+    std::string data;
+    size_t at = 0;
+    std::string next() {
+        if (at >= data.size()) return std::string();
+        size_t e = data.find('\n', at);
+        e = e == std::string::npos ? data.size() : e + 1;
+        std::string l = data.substr(at, e - at);
+        at = e;
+        return l;
+    }
+};
+
+// src/io/fastq.rs:22-95 and src/io/fasta.rs:67-114 (reads given as FASTA;
+// src/io/reads.rs:121-135 picks it when the first byte is '>')
 bool read_reads(const std::string& path, std::vector<Record>& out, std::string& err) {
     std::ifstream f(path, std::ios::binary);
     if (!f) { err = "cannot open " + path; return false; }
-    std::string line;
-    int first = f.peek();
-    if (first == '>') {
+    LineReader rd;
+    rd.data.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    if (!rd.data.empty() && rd.data[0] == '>') {
         Record cur;
         bool have = false;
-        while (std::getline(f, line)) {
-            if (!line.empty() && line[0] == '>') {
+        for (std::string line = rd.next(); !line.empty(); line = rd.next()) {
+            if (line[0] == '>') {
                 if (have) out.push_back(cur);
                 cur = Record();
                 split_header(line.substr(1), cur.name, cur.comment, cur.has_comment);
@@ -264,27 +280,32 @@ bool read_reads(const std::string& path, std::vector<Record>& out, std::string& 
                 have = true;
             } else if (have) {
                 trim_end(line);
-                for (char& c : line) c = (char)toupper((unsigned char)c);
+                for (char& c : line) c = (c >= 'a' && c <= 'z') ? (char)(c - 32) : c;
                 cur.seq += line;
             }
         }
         if (have) out.push_back(cur);
         return true;
     }
-    while (std::getline(f, line)) {
-        if (line.empty() || line[0] != '@') { err = "Record must start with '@'"; return false; }
+    // sequence and quality lines lose their last byte (the '\n', or the last
+    // character of an unterminated last line), then one '\r'
+    auto content = [](std::string l) {
+        if (!l.empty()) l.pop_back();
+        if (!l.empty() && l.back() == '\r') l.pop_back();
+        return l;
+    };
+    for (std::string line = rd.next(); !line.empty(); line = rd.next()) {
+        if (line[0] != '@') { err = "Record must start with '@'"; return false; }
         Record r;
         split_header(line.substr(1), r.name, r.comment, r.has_comment);
+        if (r.name.empty()) { err = "Record identifier is empty"; return false; }
         if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/1") == 0) { r.name.resize(r.name.size() - 2); r.end = 1; }
         else if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/2") == 0) { r.name.resize(r.name.size() - 2); r.end = 2; }
-        std::string seq, plus, qual;
-        std::getline(f, seq);
-        if (!seq.empty() && seq.back() == '\r') seq.pop_back();
-        std::getline(f, plus);
-        std::getline(f, qual);
-        if (!qual.empty() && qual.back() == '\r') qual.pop_back();
+        std::string seq = content(rd.next());
+        rd.next();  // the '+' line
+        std::string qual = content(rd.next());
         if (seq.size() != qual.size()) { err = "sequence/quality length mismatch in " + r.name; return false; }
-        for (char& c : seq) c = (char)toupper((unsigned char)c);
+        for (char& c : seq) c = (c >= 'a' && c <= 'z') ? (char)(c - 32) : c;
         r.seq = seq;
         r.qual = qual;
         out.push_back(r);

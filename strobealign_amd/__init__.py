@@ -89,7 +89,7 @@ class Alignment(ctypes.Structure):
 EXPORTS = ["sa_last_error", "sa_version", "sa_ctx_create", "sa_ctx_destroy", "sa_seeding_params_for_read_length",
            "sa_index_build", "sa_index_build_device", "sa_index_save_sti", "sa_index_load_sti", "sa_index_download", "sa_index_upload", "sa_index_destroy", "sa_index_info", "sa_map_options_default",
            "sa_map_files", "sa_map_records", "sa_randstrobes_query_batch", "sa_find_nams_batch",
-           "sa_find_nams_batch_noalloc", "sa_align_ssw_batch", "sa_extend_batch", "sa_find_hits_batch", "sa_chain_batch", "sa_map_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_count",
+           "sa_find_nams_batch_noalloc", "sa_align_ssw_batch", "sa_extend_batch", "sa_find_hits_batch", "sa_chain_batch", "sa_map_batch", "sa_index_stats", "sa_index_kernel_times", "sa_reads_load", "sa_reads_load_device", "sa_reads_count", "sa_reads_record",
            "sa_reads_destroy", "sa_reads_upload", "sa_map_reads", "sa_hbm_copy_probe", "sa_server_create",
            "sa_server_destroy", "sa_server_submit", "sa_server_submit_reads", "sa_server_wait",
            "sa_server_get_stats", "sa_server_kernel_times", "sa_autotune", "sa_free"]
@@ -102,16 +102,33 @@ class RunStats(ctypes.Structure):
 
 
 class Reads:
-    """A read set parsed once (sa_reads_load)."""
+    """A read set parsed once (sa_reads_load), or parsed on the device of an
+    index and left staged there (device=index: sa_reads_load_device; ingest_ms
+    is then the device time of the parse)."""
 
-    def __init__(self, reads1, reads2=None):
+    def __init__(self, reads1, reads2=None, device=None):
         load()
         self._p = ctypes.c_void_p()
-        _check(_lib.sa_reads_load(reads1.encode(), reads2.encode() if reads2 else None, ctypes.byref(self._p)))
+        self.ingest_ms = None
+        if device is not None:
+            ms = ctypes.c_double()
+            _check(_lib.sa_reads_load_device(device._p, reads1.encode(), reads2.encode() if reads2 else None,
+                                             ctypes.byref(self._p), ctypes.byref(ms)))
+            self.ingest_ms = ms.value
+        else:
+            _check(_lib.sa_reads_load(reads1.encode(), reads2.encode() if reads2 else None, ctypes.byref(self._p)))
         self.paired = reads2 is not None
 
     def __len__(self):
         return _lib.sa_reads_count(self._p)
+
+    def record(self, i, mate=0):
+        """(name, comment or None, sequence, qualities or None, end) of record i."""
+        f = [ctypes.c_char_p() for _ in range(4)]
+        end = ctypes.c_int()
+        _check(_lib.sa_reads_record(self._p, ctypes.c_uint64(i), mate, *[ctypes.byref(x) for x in f],
+                                    ctypes.byref(end)))
+        return tuple(x.value for x in f) + (end.value,)
 
     def upload(self, index):
         """Stages the reads in the index's device memory (sa_reads_upload)."""

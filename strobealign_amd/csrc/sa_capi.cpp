@@ -8,6 +8,7 @@
 
 #include "sa_hip.h"
 #include "sa_host.hpp"
+#include "sa_ingest.hpp"
 
 struct sa_ctx {
     int device;
@@ -459,6 +460,67 @@ int sa_reads_load(const char* reads1, const char* reads2, sa_reads** out) {
 }
 
 uint64_t sa_reads_count(const sa_reads* reads) { return reads ? reads->r1.size() : 0; }
+
+int sa_reads_record(const sa_reads* reads, uint64_t i, int mate, const char** name, const char** comment,
+                    const char** seq, const char** qual, int* end) {
+    if (!reads || !name || !comment || !seq || !qual || !end) return fail(SA_ERR_ARG, "NULL argument");
+    const std::vector<sa::Record>& v = mate ? reads->r2 : reads->r1;
+    if ((mate != 0 && mate != 1) || i >= v.size()) return fail(SA_ERR_ARG, "no such record");
+    const sa::Record& r = v[i];
+    *name = r.name.c_str();
+    *comment = r.has_comment ? r.comment.c_str() : nullptr;
+    *seq = r.seq.c_str();
+    *qual = r.has_qual ? r.qual.c_str() : nullptr;
+    *end = r.end;
+    return SA_OK;
+}
+
+int sa_reads_load_device(sa_index* index, const char* reads1, const char* reads2, sa_reads** out, double* ingest_ms) {
+    if (!index || !reads1 || !out) return fail(SA_ERR_ARG, "NULL argument");
+    return guard([&] {
+        auto r = std::make_unique<sa_reads>();
+        std::string err, raw1, raw2;
+        if (!sa::read_file_bytes(reads1, raw1, err)) return fail(SA_ERR_IO, err);
+        if (reads2 && !sa::read_file_bytes(reads2, raw2, err)) return fail(SA_ERR_IO, err);
+        const int threads = 16;
+        if (ingest_ms) *ingest_ms = 0;
+        const bool fasta = (!raw1.empty() && raw1[0] == '>') || (reads2 && !raw2.empty() && raw2[0] == '>');
+        if (fasta) {
+            // FASTA read files: parsed on the host, then staged
+            if (!sa::parse_records(raw1, r->r1, err, threads)) return fail(SA_ERR_IO, err);
+            if (reads2 && !sa::parse_records(raw2, r->r2, err, threads)) return fail(SA_ERR_IO, err);
+            if (reads2 && r->r1.size() != r->r2.size()) return fail(SA_ERR_IO, "reads1 and reads2 differ in length");
+            sa_reads* p = r.get();
+            const int rc = sa_reads_upload(index, p);
+            if (rc != SA_OK) return rc;
+            *out = r.release();
+            return SA_OK;
+        }
+        std::vector<sa::FqLoc> l1, l2;
+        void* d_seq = nullptr;
+        std::vector<uint64_t> off;
+        const double ms = sa::ingest_fastq(index->device, raw1, reads2 ? &raw2 : nullptr, l1, l2, &d_seq, off);
+        struct Own {
+            int dev;
+            void* p;
+            ~Own() {
+                if (p) sa::device_free(dev, p);
+            }
+        } own{index->device, d_seq};
+        if (!sa::records_from_locs(raw1, l1, r->r1, err, threads)) return fail(SA_ERR_IO, err);
+        if (reads2 && !sa::records_from_locs(raw2, l2, r->r2, err, threads)) return fail(SA_ERR_IO, err);
+        if (reads2 && r->r1.size() != r->r2.size()) return fail(SA_ERR_IO, "reads1 and reads2 differ in length");
+        if (!d_seq) return fail(SA_ERR_INTERNAL, "device parse staged no sequences");
+        r->device = index->device;
+        r->d_seq = d_seq;
+        own.p = nullptr;
+        r->dr.d_seq = (const uint8_t*)d_seq;
+        r->dr.off = std::move(off);
+        if (ingest_ms) *ingest_ms = ms;
+        *out = r.release();
+        return SA_OK;
+    });
+}
 
 int sa_reads_upload(sa_index* index, sa_reads* reads) {
     if (!index || !reads) return fail(SA_ERR_ARG, "NULL argument");

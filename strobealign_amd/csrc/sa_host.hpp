@@ -67,6 +67,15 @@ struct Record {
     int end = 0;
 };
 bool read_records(const std::string& path, std::vector<Record>& out, std::string& err);
+// The whole file (gunzipped when the name ends in ".gz", src/io/xopen.rs:9-20).
+bool read_file_bytes(const std::string& path, std::string& out, std::string& err);
+// FASTQ or FASTA read records of a file held in memory.
+bool parse_records(const std::string& raw, std::vector<Record>& out, std::string& err, int threads);
+struct FqLoc;
+// Records from their locations in the file (as the device parser finds them);
+// fails with the first record's error.
+bool records_from_locs(const std::string& raw, const std::vector<FqLoc>& locs, std::vector<Record>& out,
+                       std::string& err, int threads);
 uint64_t estimate_read_length(const std::vector<Record>& recs);
 
 struct Options {

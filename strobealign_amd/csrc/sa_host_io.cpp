@@ -10,15 +10,16 @@
 #include <stdexcept>
 #include <thread>
 
+#include <zlib.h>
+
 #include "sa_host.hpp"
+#include "sa_ingest.hpp"
 
 namespace sa {
 
 namespace {
 void rstrip(std::string& s) {
-    while (!s.empty() && (s.back() == ' ' || s.back() == '\t' || s.back() == '\n' || s.back() == '\r' ||
-                          s.back() == '\f' || s.back() == '\v'))
-        s.pop_back();
+    while (!s.empty() && ascii_ws((uint8_t)s.back())) s.pop_back();
 }
 
 // src/io/mod.rs:32-38
@@ -89,8 +90,7 @@ PackedRef read_fasta(const std::string& path) {
             return;
         }
         if (!have) return;
-        while (n && (p[n - 1] == ' ' || p[n - 1] == '\t' || p[n - 1] == '\r' || p[n - 1] == '\f' || p[n - 1] == '\v'))
-            --n;
+        while (n && ascii_ws((uint8_t)p[n - 1])) --n;
         for (size_t i = 0; i < n; ++i) {
             uint64_t b = code[(unsigned char)p[i]];
             if (b == 4) {
@@ -261,48 +261,137 @@ void build_index(const PackedRef& ref, SeedParams p, int bits, double filter_fra
     out.p = p;
 }
 
-// src/io/fastq.rs:22-95 and FASTA reads (src/io/reads.rs:121-135)
-bool read_records(const std::string& path, std::vector<Record>& out, std::string& err) {
-    std::ifstream f(path, std::ios::binary);
+// The whole file; gzip-compressed (also several members one after another)
+// when the name ends in ".gz" (src/io/xopen.rs:9-20).
+bool read_file_bytes(const std::string& path, std::string& out, std::string& err) {
+    out.clear();
+    if (path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0) {
+        gzFile g = gzopen(path.c_str(), "rb");
+        if (!g) { err = "cannot open " + path; return false; }
+        gzbuffer(g, 1 << 20);
+        std::vector<char> buf(1 << 22);
+        for (;;) {
+            const int k = gzread(g, buf.data(), (unsigned)buf.size());
+            if (k < 0) {
+                int code = 0;
+                err = std::string("gzip error in ") + path + ": " + gzerror(g, &code);
+                gzclose(g);
+                return false;
+            }
+            if (k == 0) break;
+            out.append(buf.data(), (size_t)k);
+        }
+        gzclose(g);
+        return true;
+    }
+    std::FILE* f = std::fopen(path.c_str(), "rb");
     if (!f) { err = "cannot open " + path; return false; }
-    std::string line;
-    if (f.peek() == '>') {
+    std::fseek(f, 0, SEEK_END);
+    const long n = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (n > 0) {
+        out.resize((size_t)n);
+        const size_t got = std::fread(&out[0], 1, (size_t)n, f);
+        out.resize(got);
+    }
+    std::fclose(f);
+    return true;
+}
+
+std::string fq_error(const FqLoc& r, const std::string& name) {
+    switch (r.err) {
+        case FQ_NO_AT: return "FASTQ record must start with '@'";
+        case FQ_EMPTY_NAME: return "FASTQ record identifier is empty";
+        case FQ_LENGTHS: return "sequence and quality lengths differ in record " + name;
+        default: return "";
+    }
+}
+
+// Records from their locations in the file's bytes (uppercase sequences);
+// fails with the first record's error, if any.
+bool records_from_locs(const std::string& raw, const std::vector<FqLoc>& locs, std::vector<Record>& out,
+                       std::string& err, int threads) {
+    for (const FqLoc& r : locs)
+        if (r.err) {
+            err = fq_error(r, raw.substr(r.name_off, r.name_len));
+            return false;
+        }
+    const size_t base = out.size();
+    out.resize(base + locs.size());
+    const uint8_t* p = (const uint8_t*)raw.data();
+    parallel_for(threads, (locs.size() + 4095) / 4096, [&](size_t blk) {
+        const size_t e = std::min(locs.size(), (blk + 1) * 4096);
+        for (size_t i = blk * 4096; i < e; ++i) {
+            const FqLoc& l = locs[i];
+            Record& r = out[base + i];
+            r.name.assign(raw, l.name_off, l.name_len);
+            r.has_comment = l.has_comment != 0;
+            if (r.has_comment) r.comment.assign(raw, l.comment_off, l.comment_len);
+            r.seq.resize(l.seq_len);
+            for (uint32_t x = 0; x < l.seq_len; ++x) r.seq[x] = (char)ascii_upper(p[l.seq_off + x]);
+            r.qual.assign(raw, l.qual_off, l.qual_len);
+            r.has_qual = true;
+            r.end = l.end;
+        }
+    });
+    return true;
+}
+
+// FASTQ records (src/io/fastq.rs:22-95, fq_record) or FASTA reads
+// (src/io/fasta.rs:60-113; src/io/reads.rs:121-135 picks FASTA when the first
+// byte is '>') of a file held in memory.
+bool parse_records(const std::string& raw, std::vector<Record>& out, std::string& err, int threads) {
+    const uint8_t* p = (const uint8_t*)raw.data();
+    const uint64_t n = raw.size();
+    if (n > 0 && p[0] == '>') {
         Record cur;
         bool have = false;
-        while (std::getline(f, line)) {
-            if (!line.empty() && line[0] == '>') {
+        uint64_t s = 0;
+        while (s < n) {
+            const void* q = std::memchr(p + s, '\n', n - s);
+            const uint64_t e = q ? (uint64_t)((const uint8_t*)q - p) + 1 : n;
+            if (p[s] == '>') {
                 if (have) out.push_back(std::move(cur));
                 cur = Record();
-                split_header(line.substr(1), cur.name, cur.comment, cur.has_comment);
+                split_header(std::string((const char*)p + s + 1, e - s - 1), cur.name, cur.comment, cur.has_comment);
                 cur.has_qual = false;
                 have = true;
             } else if (have) {
-                rstrip(line);
-                for (char& c : line) c = (char)toupper((unsigned char)c);
-                cur.seq += line;
+                uint64_t t = e;
+                while (t > s && ascii_ws(p[t - 1])) --t;
+                for (uint64_t x = s; x < t; ++x) cur.seq.push_back((char)ascii_upper(p[x]));
             }
+            s = e;
         }
         if (have) out.push_back(std::move(cur));
         return true;
     }
-    while (std::getline(f, line)) {
-        if (line.empty() || line[0] != '@') { err = "FASTQ record must start with '@'"; return false; }
-        Record r;
-        split_header(line.substr(1), r.name, r.comment, r.has_comment);
-        if (r.name.empty()) { err = "FASTQ record identifier is empty"; return false; }
-        if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/1") == 0) { r.name.resize(r.name.size() - 2); r.end = 1; }
-        else if (r.name.size() >= 2 && r.name.compare(r.name.size() - 2, 2, "/2") == 0) { r.name.resize(r.name.size() - 2); r.end = 2; }
-        std::string plus;
-        std::getline(f, r.seq);
-        if (!r.seq.empty() && r.seq.back() == '\r') r.seq.pop_back();
-        std::getline(f, plus);
-        std::getline(f, r.qual);
-        if (!r.qual.empty() && r.qual.back() == '\r') r.qual.pop_back();
-        if (r.seq.size() != r.qual.size()) { err = "sequence and quality lengths differ in record " + r.name; return false; }
-        for (char& c : r.seq) c = (char)toupper((unsigned char)c);
-        out.push_back(std::move(r));
+    // newline table, then each record from its four lines
+    std::vector<uint64_t> nl;
+    nl.reserve(n / 64 + 4);
+    for (uint64_t s = 0; s < n;) {
+        const void* q = std::memchr(p + s, '\n', n - s);
+        if (!q) break;
+        nl.push_back((uint64_t)((const uint8_t*)q - p));
+        s = nl.back() + 1;
     }
-    return true;
+    const uint64_t nrec = fq_records(nl.size(), n, n > 0 && p[n - 1] == '\n');
+    std::vector<FqLoc> locs(nrec);
+    parallel_for(threads, (nrec + 4095) / 4096, [&](size_t blk) {
+        const uint64_t e = std::min<uint64_t>(nrec, (blk + 1) * 4096);
+        for (uint64_t r = blk * 4096; r < e; ++r) {
+            FqLine l[4];
+            for (int j = 0; j < 4; ++j) l[j] = fq_line(nl.data(), nl.size(), n, 4 * r + (uint64_t)j);
+            locs[r] = fq_record(p, l);
+        }
+    });
+    return records_from_locs(raw, locs, out, err, threads);
+}
+
+bool read_records(const std::string& path, std::vector<Record>& out, std::string& err) {
+    std::string raw;
+    if (!read_file_bytes(path, raw, err)) return false;
+    return parse_records(raw, out, err, (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
 }
 
 // src/main.rs:1001-1012
