@@ -507,6 +507,8 @@ SA_HD int banded_sw(QC qc, RC rc, int32_t refLen, int32_t readLen, int32_t score
         else if (bw <= 2) maxv = band_pass_reg<2>(qc, rc, refLen, readLen, sc, sm, bw, dv);
         else if (bw <= 4) maxv = band_pass_reg<4>(qc, rc, refLen, readLen, sc, sm, bw, dv);
         else if (bw <= 8) maxv = band_pass_reg<8>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else if (bw <= 16) maxv = band_pass_reg<16>(qc, rc, refLen, readLen, sc, sm, bw, dv);
+        else if (bw <= 32) maxv = band_pass_reg<32>(qc, rc, refLen, readLen, sc, sm, bw, dv);
         else maxv = band_pass_mem(qc, rc, refLen, readLen, sc, sm, bw, m, 0, (uint64_t)(width + 1),
                                   2ull * (uint64_t)(width + 1), dv);
         bw *= 2;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -186,10 +187,36 @@ double ingest_fastq(int device, const std::string& r1, const std::string* r2, st
     } eg{e0, e1};
     SA_ICHECK(hipEventRecord(e0, st));
     // the files' bytes, padded to whole 16-byte vectors
+    // the files' bytes through two pinned staging buffers: the copy of one
+    // chunk into pinned memory overlaps the transfer of the previous one
+    constexpr uint64_t CHUNK = 32ull << 20;
+    struct Pinned {
+        void* p[2] = {nullptr, nullptr};
+        hipEvent_t done[2] = {nullptr, nullptr};
+        ~Pinned() {
+            for (int i = 0; i < 2; ++i) {
+                if (done[i]) (void)hipEventDestroy(done[i]);
+                if (p[i]) (void)hipHostFree(p[i]);
+            }
+        }
+    } pin;
+    for (int i = 0; i < 2; ++i) {
+        SA_ICHECK(hipHostMalloc(&pin.p[i], CHUNK, hipHostMallocDefault));
+        SA_ICHECK(hipEventCreateWithFlags(&pin.done[i], hipEventDisableTiming));
+    }
+    uint64_t nchunk = 0;
     auto upload = [&](const std::string& s) {
         auto b = std::make_unique<DevBuf>((s.size() + 31) / 16 * 16);
-        SA_ICHECK(hipMemsetAsync(b->p, 0, (s.size() + 31) / 16 * 16, st));
-        if (!s.empty()) SA_ICHECK(hipMemcpyAsync(b->p, s.data(), s.size(), hipMemcpyHostToDevice, st));
+        SA_ICHECK(hipMemsetAsync(b->as<uint8_t>() + s.size() / 16 * 16, 0, (s.size() + 31) / 16 * 16 - s.size() / 16 * 16,
+                                 st));
+        for (uint64_t at = 0; at < s.size(); at += CHUNK, ++nchunk) {
+            const int x = (int)(nchunk & 1);
+            const uint64_t len = std::min<uint64_t>(CHUNK, s.size() - at);
+            SA_ICHECK(hipEventSynchronize(pin.done[x]));  // the buffer's previous transfer is done
+            std::memcpy(pin.p[x], s.data() + at, len);
+            SA_ICHECK(hipMemcpyAsync(b->as<uint8_t>() + at, pin.p[x], len, hipMemcpyHostToDevice, st));
+            SA_ICHECK(hipEventRecord(pin.done[x], st));
+        }
         return b;
     };
     auto raw1 = upload(r1);

@@ -381,12 +381,14 @@ SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
 // starts with (|aligned reference length - aligned query length| + 1: 1, 2,
 // 3-4, 5-8, more; last the jobs that stop before it), so that the lanes of a
 // wave run the same band-pass variant.
-constexpr uint32_t FIN_CLASSES = 6;
+// Classes by the band width the traceback starts with, widest first (their
+// jobs take longest, so their waves start first), jobs without a traceback last.
+constexpr uint32_t FIN_CLASSES = 8;
 SA_HD uint32_t fin_class(const SswEnds& e) {
     if (e.ref_begin1 < 0 || e.read_begin1 < 0 || e.rev_short) return FIN_CLASSES - 1;
     const int32_t refL = e.ref_end1 - e.ref_begin1 + 1, readL = e.read_end1 - e.read_begin1 + 1;
     const int32_t bw = (refL > readL ? refL - readL : readL - refL) + 1;
-    return bw <= 1 ? 0u : bw <= 2 ? 1u : bw <= 4 ? 2u : bw <= 8 ? 3u : 4u;
+    return bw > 32 ? 0u : bw > 16 ? 1u : bw > 8 ? 2u : bw > 4 ? 3u : bw > 2 ? 4u : bw > 1 ? 5u : 6u;
 }
 // lists: FIN_CLASSES lists of `cap` entries; counts: their lengths (zeroed before)
 SA_HD void item_fin_bucket(uint32_t t, const SswEnds* ends, uint32_t njobs, uint32_t* lists, uint32_t* counts,
