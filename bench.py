@@ -392,6 +392,17 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
         extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / kd_steps),
                      "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
                      "frac": round(gcups * 10 / 39300.0, 4)}
+        # VALU issue of the score passes by their counters (SQ_INSTS_VALU over
+        # the kernel's duration), from the committed profile of this workload
+        if tj.get("workload") == workload:
+            iss = {}
+            for kname, e in tj.get("kernels", {}).items():
+                if kname.startswith("k_ssw_pass<") and e.get("valu_issue_frac") is not None:
+                    form = "byte" if ", true," in kname else "word"
+                    if e["dispatches"] > iss.get(form, (0, 0))[0]:
+                        iss[form] = (e["dispatches"], round(e["valu_issue_frac"], 3))
+            if iss:
+                extension["valu_issue_frac_from_profile"] = {k: v[1] for k, v in iss.items()}
     copy_gbps = ctx.hbm_copy_gbps()
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_note,
