@@ -190,7 +190,7 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
  * resident if uploaded, output dropped). Unless opt->lookup_lanes is set, it
  * first times the lookup stage on one engine with 1, 8, 16 and 64 lanes per
  * read orientation and keeps the fastest; then it maps the sample with each
- * engine count in {1, 2, 3, 4, 6} and batch size in {16, 32, 64} Mbases, and
+ * engine count in {1, 2, 3, 4, 6, 8} and batch size in {16, 32, 64} Mbases, and
  * writes the fastest choices into opt->lookup_lanes, opt->slots and
  * opt->batch_bases. The outcome never depends on the schedule (the output is
  * identical for every choice); only the speed does. report (may be NULL)

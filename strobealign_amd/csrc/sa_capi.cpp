@@ -628,7 +628,7 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
                 }
             }
         }
-        const int slot_set[] = {1, 2, 3, 4, 6};
+        const int slot_set[] = {1, 2, 3, 4, 6, 8};
         const uint64_t batch_set[] = {16ull << 20, 32ull << 20, 64ull << 20};
         double best = 1e300;
         int best_slots = 0;

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <array>
+#include <immintrin.h>
 #include <deque>
 #include <exception>
 #include <functional>
@@ -237,6 +238,48 @@ struct CachedResult {
     uint64_t cigar_off;  // into Chunk::cigar_pool
 };
 
+// Chunk output buffers kept for reuse: a fresh multi-megabyte string per chunk
+// and batch costs a page fault per 4 KiB the records are written into (about
+// 180,000 per million pairs of SAM); a recycled one is already mapped.
+class OutPool {
+public:
+    std::string take(size_t cap) {
+        std::string s;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            size_t best = free_.size();
+            for (size_t i = 0; i < free_.size(); ++i)
+                if (free_[i].capacity() >= cap && (best == free_.size() || free_[i].capacity() < free_[best].capacity()))
+                    best = i;
+            if (best == free_.size() && !free_.empty()) best = free_.size() - 1;
+            if (best < free_.size()) {
+                s.swap(free_[best]);
+                held_ -= s.capacity();
+                free_.erase(free_.begin() + (ptrdiff_t)best);
+            }
+        }
+        s.clear();
+        s.reserve(cap);
+        return s;
+    }
+    void give(std::string& s) {
+        std::string t;
+        t.swap(s);
+        t.clear();
+        std::lock_guard<std::mutex> lk(mu_);
+        if (held_ + t.capacity() > kMaxHeld || free_.size() >= 256) return;  // freed when t goes
+        held_ += t.capacity();
+        free_.push_back(std::move(t));
+    }
+
+private:
+    static constexpr size_t kMaxHeld = 4ull << 30;
+    std::mutex mu_;
+    std::vector<std::string> free_;
+    size_t held_ = 0;
+};
+OutPool g_out_pool;
+
 // Per-chunk state of the replay.
 struct Chunk {
     uint64_t first, count;  // records (pairs) [first, first+count) of the caller's read set
@@ -351,23 +394,17 @@ void put_u(std::string& s, uint64_t v) {
     while (n) s += b[--n];
 }
 
-// Raw output cursor for SAM records: the caller reserves enough room.
-struct Out {
-    char* p;
-    void str(const std::string& x) { std::memcpy(p, x.data(), x.size()); p += x.size(); }
-    void chr(char c) { *p++ = c; }
-    void lit(const char* x, size_t n) { std::memcpy(p, x, n); p += n; }
-    void u(uint64_t v) {
-        char b[24];
-        int n = 0;
-        do { b[n++] = (char)('0' + v % 10); v /= 10; } while (v);
-        while (n) *p++ = b[--n];
-    }
-    void i(int64_t v) {
-        if (v < 0) { *p++ = '-'; u((uint64_t)(-v)); }
-        else u((uint64_t)v);
+// Decimal digits of 0..99, two characters each.
+struct DigitPairs {
+    char d[200];
+    DigitPairs() {
+        for (int i = 0; i < 100; ++i) {
+            d[2 * i] = (char)('0' + i / 10);
+            d[2 * i + 1] = (char)('0' + i % 10);
+        }
     }
 };
+const DigitPairs kPairs;
 
 // comp_ascii as a table
 struct CompTable {
@@ -377,6 +414,85 @@ struct CompTable {
     }
 };
 const CompTable kComp;
+
+// Reverse complement of n bytes (comp_ascii of each, last first) and the
+// reverse of n bytes, 16 at a time with SSSE3 byte shuffles: a block whose
+// bytes are all A, C, G, T, U or N (checked by mapping each byte's low nibble
+// back to the letter it must be; the unused nibbles map to a byte with another
+// nibble, so they never pass) is complemented by a low-nibble lookup; any other
+// block takes the table.
+__attribute__((target("ssse3"))) void revcomp_ssse3(char* dst, const char* src, size_t n) {
+    const __m128i rev = _mm_setr_epi8(15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0);
+    // low nibble -> the letter with that nibble, and its complement
+    const __m128i letter = _mm_setr_epi8(1, 'A', 0, 'C', 'T', 'U', 0, 'G', 0, 0, 0, 0, 0, 0, 'N', 0);
+    const __m128i comp = _mm_setr_epi8(0, 'T', 0, 'G', 'A', 'A', 0, 'C', 0, 0, 0, 0, 0, 0, 'N', 0);
+    const __m128i lo = _mm_set1_epi8(0x0F);
+    size_t x = 0;
+    for (; x + 16 <= n; x += 16) {
+        const __m128i v = _mm_loadu_si128((const __m128i*)(src + n - 16 - x));
+        const __m128i nib = _mm_and_si128(v, lo);
+        const __m128i ok = _mm_cmpeq_epi8(_mm_shuffle_epi8(letter, nib), v);
+        if (_mm_movemask_epi8(ok) == 0xFFFF) {
+            _mm_storeu_si128((__m128i*)(dst + x), _mm_shuffle_epi8(_mm_shuffle_epi8(comp, nib), rev));
+        } else {
+            for (size_t y = 0; y < 16; ++y) dst[x + y] = kComp.t[(uint8_t)src[n - 1 - x - y]];
+        }
+    }
+    for (; x < n; ++x) dst[x] = kComp.t[(uint8_t)src[n - 1 - x]];
+}
+
+__attribute__((target("ssse3"))) void reverse_ssse3(char* dst, const char* src, size_t n) {
+    const __m128i rev = _mm_setr_epi8(15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0);
+    size_t x = 0;
+    for (; x + 16 <= n; x += 16)
+        _mm_storeu_si128((__m128i*)(dst + x),
+                         _mm_shuffle_epi8(_mm_loadu_si128((const __m128i*)(src + n - 16 - x)), rev));
+    for (; x < n; ++x) dst[x] = src[n - 1 - x];
+}
+
+const bool kSsse3 = __builtin_cpu_supports("ssse3");
+
+void revcomp_into(char* dst, const char* src, size_t n) {
+    if (kSsse3) return revcomp_ssse3(dst, src, n);
+    for (size_t x = 0; x < n; ++x) dst[x] = kComp.t[(uint8_t)src[n - 1 - x]];
+}
+
+void reverse_into(char* dst, const char* src, size_t n) {
+    if (kSsse3) return reverse_ssse3(dst, src, n);
+    for (size_t x = 0; x < n; ++x) dst[x] = src[n - 1 - x];
+}
+
+// Raw output cursor for SAM records: the caller reserves enough room.
+struct Out {
+    char* p;
+    void str(const std::string& x) { std::memcpy(p, x.data(), x.size()); p += x.size(); }
+    void chr(char c) { *p++ = c; }
+    void lit(const char* x, size_t n) { std::memcpy(p, x, n); p += n; }
+    // decimal, two digits per step from the pair table
+    void u(uint64_t v) {
+        char b[24];
+        int n = 24;
+        while (v >= 100) {
+            const uint64_t q = v / 100;
+            const uint32_t r = (uint32_t)(v - 100 * q);
+            n -= 2;
+            std::memcpy(b + n, kPairs.d + 2 * r, 2);
+            v = q;
+        }
+        if (v >= 10) {
+            n -= 2;
+            std::memcpy(b + n, kPairs.d + 2 * v, 2);
+        } else {
+            b[--n] = (char)('0' + v);
+        }
+        std::memcpy(p, b + n, (size_t)(24 - n));
+        p += 24 - n;
+    }
+    void i(int64_t v) {
+        if (v < 0) { *p++ = '-'; u((uint64_t)(-v)); }
+        else u((uint64_t)v);
+    }
+};
 
 // SAM record fields (src/io/sam.rs:21-43)
 struct Sam {
@@ -442,16 +558,14 @@ void write_sam(std::string& out, const Sam& r, const Ctx& c) {
     const bool sec = (r.flags & SECONDARY) != 0;
     if (sec || seq.empty()) s.chr('*');
     else if (r.revseq) {
-        const size_t n = seq.size();
-        for (size_t x = 0; x < n; ++x) s.p[x] = kComp.t[(uint8_t)seq[n - 1 - x]];
-        s.p += n;
+        revcomp_into(s.p, seq.data(), seq.size());
+        s.p += seq.size();
     } else s.str(seq);
     s.chr('\t');
     if (sec || !r.rec->has_qual || q.empty()) s.chr('*');
     else if (r.revseq) {
-        const size_t n = q.size();
-        for (size_t x = 0; x < n; ++x) s.p[x] = q[n - 1 - x];
-        s.p += n;
+        reverse_into(s.p, q.data(), q.size());
+        s.p += q.size();
     } else s.str(q);
     if (r.al) { s.lit("\tNM:i:", 6); s.u(r.al->edit_distance); s.lit("\tAS:i:", 6); s.u(r.al->score); }
     if (c.o.details && r.det) {
@@ -1350,7 +1464,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
     Ctx base{o, ref, nb, nullptr, nullptr, paired, 0, k, nullptr, false, false, nullptr, nullptr};
     for (size_t x = 0; x < nch; ++x) {
         chs[x]->abundances.assign(ref.names.size(), 0.0);
-        chs[x]->out.reserve(chs[x]->count * (paired ? 1400 : 700));
+        chs[x]->out = g_out_pool.take(chs[x]->count * (paired ? 1400 : 700));
     }
     for (int round = 0;; ++round) {
         double h0 = now_ms();
@@ -1425,7 +1539,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         Chunk& ch = *chs[x];
         decltype(ch.cache)().swap(ch.cache);
         std::vector<uint32_t>().swap(ch.cigar_pool);
-        if (!keep) std::string().swap(ch.out);
+        if (!keep) g_out_pool.give(ch.out);
     });
 }
 
@@ -1523,7 +1637,7 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
         parallel_for(o.threads, chunks.size(), [&](size_t x) {
             Chunk& ch = chunks[x];
             if (!ch.out.empty()) std::memcpy(&(*out)[pos[x]], ch.out.data(), ch.out.size());
-            std::string().swap(ch.out);
+            g_out_pool.give(ch.out);
         });
     }
     if (o.mode == 2 && out) abundance_table(ref, chunks, *out);
