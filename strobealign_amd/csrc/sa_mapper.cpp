@@ -670,6 +670,7 @@ void shuffle_best(std::vector<Nam>& nams, Rng& rng) {
 
 void nams_of(const Ctx& c, uint64_t read_id, std::vector<Nam>& v) {
     v.clear();
+    v.reserve(c.nb.nam_off[read_id + 1] - c.nb.nam_off[read_id]);
     for (uint64_t x = c.nb.nam_off[read_id]; x < c.nb.nam_off[read_id + 1]; ++x) v.push_back(to_nam(c.nb.nams[x]));
 }
 
@@ -680,14 +681,16 @@ void align_single_end(Ctx& c, uint64_t rec_idx, std::vector<Sam>& out, std::dequ
     const Record& rec = c.r1[rec_idx];
     const uint32_t read_id = (uint32_t)(c.rel0 + rec_idx);
     const uint32_t L = (uint32_t)rec.seq.size();
-    std::vector<Nam> nams;
+    // per-thread vectors reused from record to record (a record's call does not nest)
+    thread_local std::vector<Nam> nams;
     nams_of(c, read_id, nams);
     shuffle_best(nams, *c.rng);
     dstore.emplace_back();
     Details& det = dstore.back()[0];
     det.nam = c.nb.info[read_id];
     if (nams.empty()) { out.push_back(make_unmapped(rec, &det)); return; }
-    std::vector<Alignment> alignments;
+    thread_local std::vector<Alignment> alignments;
+    alignments.clear();
     float nam_max = nams[0].score;
     uint32_t best_ed = 0xFFFFFFFFu, best_score = 0, second = 0, n_best = 0;
     size_t best_index = 0;
@@ -1004,7 +1007,7 @@ void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Al
     const uint32_t L[2] = {(uint32_t)recs[0]->seq.size(), (uint32_t)recs[1]->seq.size()};
     dstore.emplace_back();
     Details* det = dstore.back().data();
-    std::vector<Nam> nams[2];
+    thread_local std::vector<Nam> nams[2];
     for (int i = 0; i < 2; ++i) {
         nams_of(c, rid[i], nams[i]);
         shuffle_best(nams[i], *c.rng);
@@ -1173,7 +1176,7 @@ void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
     if (!c.paired) {
         const Record& r1 = c.r1[idx];
         uint32_t id = (uint32_t)(c.rel0 + idx);
-        std::vector<Nam> nams;
+        thread_local std::vector<Nam> nams;
         nams_of(c, id, nams);
         shuffle_best(nams, *c.rng);
         if (c.o.mode == 1) {
@@ -1190,7 +1193,7 @@ void map_only(Ctx& c, uint64_t idx, std::string& s, std::vector<double>& ab) {
     const Record& r2 = c.r2[idx];
     uint64_t rel = c.rel0 + idx;
     uint32_t id1 = (uint32_t)(2 * rel), id2 = (uint32_t)(2 * rel + 1);
-    std::vector<Nam> n1, n2;
+    thread_local std::vector<Nam> n1, n2;
     nams_of(c, id1, n1);
     nams_of(c, id2, n2);
     if (n1.empty() && n2.empty()) return;
