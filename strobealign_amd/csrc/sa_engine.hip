@@ -619,6 +619,8 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
 }
 
 namespace {
+// lanes per piecewise-extension job
+constexpr int PW_LANES = 16;
 // anchor capacities of the LDS chaining tiers
 constexpr uint32_t CHAIN_CAP1 = 64, CHAIN_CAP2 = 256, CHAIN_CAP3 = 1024;
 
@@ -1221,27 +1223,26 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
             });
         });
     }
-    // 4. piecewise extension, one thread per job
+    // 4. piecewise extension, a group of PW_LANES lanes per job
     if (npw) {
-        // per thread: chain anchors, traceback ops (the DP itself uses the blocks below)
+        // per job: chain anchors, traceback ops (the DP itself uses the blocks below)
         uint64_t per = 16 * L + 256 + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
         per = (per + 255) & ~255ull;
-        // DP rows and direction bytes of the end extensions and gaps, one
-        // lane-interleaved block of each per 64 threads (a DP with more rows or
-        // columns falls back to one thread with a large scratch)
+        // DP row buffer and direction bytes of the end extensions and gaps, one
+        // block of each per job (a DP with more rows or columns falls back to one
+        // thread with a large scratch)
         const uint32_t rcap = (uint32_t)(L + L / 10 + 3), drows = (uint32_t)(L + 2);
         const uint64_t pw_per = per + (uint64_t)rcap * sizeof(DpCol) + (uint64_t)rcap * drows;
         const uint64_t slice = std::max<uint64_t>(1, std::min<uint64_t>(npw, budget / pw_per));
-        const uint64_t slice_threads = (slice + 127) / 128 * 128;
         auto* scr = (uint8_t*)grow(S_SCRATCH, slice * per);
-        auto* rows = (DpCol*)grow(S_PWROWS, slice_threads * rcap * sizeof(DpCol));
-        auto* dirs = (uint8_t*)grow(S_PWDIRS, slice_threads * rcap * drows);
+        auto* rows = (DpCol*)grow(S_PWROWS, slice * rcap * sizeof(DpCol));
+        auto* dirs = (uint8_t*)grow(S_PWDIRS, slice * rcap * drows);
         timed("k_ext_pw", npw, [&] {
             for (uint64_t first = 0; first < npw; first += slice) {
                 const uint32_t cnt_ = (uint32_t)std::min<uint64_t>(slice, npw - first);
-                k_ext_pw<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res,
-                                                                d_slots, d_chain, scr, per, (uint32_t)first, rows,
-                                                                dirs, rcap, drows);
+                k_ext_pw_grp<PW_LANES><<<blocks((uint64_t)cnt_ * PW_LANES, 128), 128, 0, stream_>>>(
+                    d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res, d_slots, d_chain, scr, per,
+                    (uint32_t)first, rows, dirs, rcap, drows);
             }
         });
     }

@@ -902,9 +902,9 @@ struct alignas(16) DpCol {
 // Where the DP keeps its row and its direction bytes: row element j at
 // p[j * stride] for j < cap, direction byte of cell (i, j) at
 // dir[(i * cap + j) * stride] for i < dir_rows. The piecewise kernel gives each
-// wave one block of each with its 64 lanes interleaved (stride 64), so lanes at
-// the same cell touch consecutive addresses; without blocks both are placed in
-// the scratch (the direction bytes first, row pitch n + 1).
+// job one block of each (stride 1), filled by the job's lane group; without
+// blocks both are placed in the scratch (the direction bytes first, row pitch
+// n + 1).
 struct DpRows {
     DpCol* p = nullptr;
     uint32_t stride = 1, cap = 0;
@@ -1080,12 +1080,22 @@ SA_HD void emit_tb(const uint8_t* tb, uint32_t ntb, bool reversed_space, CigarOu
     else for (uint32_t x = ntb; x-- > 0;) cig.push(tb[x], 1);
 }
 
+// The DP of extend_piecewise in one thread.
+struct DpScalar {
+    template <typename Q, typename R>
+    SA_HD DpOut operator()(const Q& q, uint32_t m, const R& r, uint32_t n, const Scores& sc, bool local_end,
+                           uint8_t* scratch, uint64_t scratch_bytes, uint8_t* tb, uint32_t tb_cap, DpRows rows) const {
+        return affine_dp(q, m, r, n, sc, local_end, scratch, scratch_bytes, tb, tb_cap, rows);
+    }
+};
+
 // src/piecewisealigner.rs:396-510. `anch` holds (ref_start, query_start) pairs,
 // last anchor first, with absolute reference coordinates; `win` is the decoded
-// reference window (its start is the decode start).
+// reference window (its start is the decode start). `dp` runs the affine DPs.
+template <typename Dp = DpScalar>
 SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t* anch, uint32_t na, uint32_t padding,
                                  const Scores& sc, uint32_t k, uint8_t* scratch, uint64_t scratch_bytes,
-                                 CigarOut& cig, DpRows rows = DpRows{}) {
+                                 CigarOut& cig, DpRows rows = DpRows{}, const Dp& dp = Dp{}) {
     AlignInfo res{false, 0, 0, 0, 0, 0, 0, 0};
     cig.reset();
     if (na == 0) return res;
@@ -1112,7 +1122,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             SubView<RefWin> rv{&win, rs};
             RevView<SubView<QueryView>> qr{&qv, qlen};
             RevView<SubView<RefWin>> rr{&rv, rlen};
-            DpOut d = affine_dp(qr, qlen, rr, rlen, sc, true, dps, dps_bytes, tb, tb_cap, rows);
+            DpOut d = dp(qr, qlen, rr, rlen, sc, true, dps, dps_bytes, tb, tb_cap, rows);
             if (d.status) { res.status = d.status; return res; }
             if (d.score == 0) { query_start = fq; ref_start = fr; }
             else {
@@ -1153,7 +1163,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             if (!done) {
                 SubView<QueryView> qv{&q, pqe};
                 SubView<RefWin> rv{&win, pre};
-                DpOut d = affine_dp(qv, (uint32_t)qd, rv, (uint32_t)rd, sc, false, dps, dps_bytes, tb, tb_cap, rows);
+                DpOut d = dp(qv, (uint32_t)qd, rv, (uint32_t)rd, sc, false, dps, dps_bytes, tb, tb_cap, rows);
                 if (d.status) { res.status = d.status; return res; }
                 score += d.score;
                 emit_tb(tb, d.ntb, false, cig);
@@ -1188,7 +1198,7 @@ SA_HD AlignInfo extend_piecewise(const QueryView& q, const RefWin& win, uint32_t
             uint32_t re = rl < lre + qlen + padding ? rl : lre + qlen + padding;
             SubView<QueryView> qv{&q, lqe};
             SubView<RefWin> rv{&win, lre};
-            DpOut d = affine_dp(qv, qlen, rv, re - lre, sc, true, dps, dps_bytes, tb, tb_cap, rows);
+            DpOut d = dp(qv, qlen, rv, re - lre, sc, true, dps, dps_bytes, tb, tb_cap, rows);
             if (d.status) { res.status = d.status; return res; }
             if (d.score == 0) { query_end = lqe; ref_end = lre; }
             else {

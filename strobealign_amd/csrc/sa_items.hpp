@@ -315,48 +315,6 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
     res[t] = out;
 }
 
-// Piecewise extension job (thread per job).
-// rows, dirs: per 64 consecutive threads one block of row_cap DP columns and
-// one of dir_rows x row_cap direction bytes, lanes interleaved (thread t's
-// column j at block t / 64, index j * 64 + t % 64; cell (i, j) at
-// (i * row_cap + j) * 64 + t % 64).
-SA_HD void item_ext_pw(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc, uint32_t k,
-                       const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res, uint32_t* cigars,
-                       const uint32_t* chain_anchors, uint8_t* scratch, uint64_t scratch_per, uint32_t first,
-                       DpCol* rows, uint8_t* dirs, uint32_t row_cap, uint32_t dir_rows) {
-    if (t >= njobs) return;
-    const ExtJob j = jobs[first + t];
-    const ExtRequest r = req[j.req];
-    uint8_t* scr = scratch + (uint64_t)t * scratch_per;
-    const uint64_t ro = off[r.read_id];
-    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
-    QueryView q{seq + ro, L, (r.flags & 4) != 0};
-    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
-    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    RefWin w{&ref, r.ref_id, j.ws, j.wlen};
-    const uint64_t wave = t / 64, lane = t % 64;
-    uint32_t* an = (uint32_t*)scr;
-    uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
-    if (r.kind != EXT_SEED) {
-        const DpRows dr{rows + wave * 64 * row_cap + lane, 64, row_cap, dirs + wave * 64 * row_cap * dir_rows + lane,
-                        dir_rows};
-        AlignInfo a = pw_probe(r, q, w, sc, k, chain_anchors, scr, scratch_per, cig, dr);
-        out.status = (uint32_t)a.status;
-        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
-    } else if (an_bytes >= scratch_per) {
-        out.status = 2;
-    } else {
-        for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
-        uint32_t na = remove_spurious_anchors(an, r.n_anchors);
-        const DpRows dr{rows + wave * 64 * row_cap + lane, 64, row_cap, dirs + wave * 64 * row_cap * dir_rows + lane,
-                        dir_rows};
-        AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig, dr);
-        out.status = (uint32_t)a.status;
-        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
-    }
-    res[j.req] = out;
-}
-
 // SSW job in one thread (scalar lanes): queries longer than SSW_GROUP_MAX_QUERY.
 SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
                                const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res,

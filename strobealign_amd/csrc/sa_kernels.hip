@@ -6,6 +6,7 @@
 
 #include "sa_chain_group.hpp"
 #include "sa_hits_group.hpp"
+#include "sa_pw_group.hpp"
 #include "sa_items.hpp"
 #include "sa_ssw_group.hpp"
 
@@ -439,16 +440,51 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
     }
 }
 
-__global__ void __launch_bounds__(128) k_ext_pw(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
-                                               RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
-                                               const ExtJob* __restrict__ jobs, uint32_t njobs,
-                                               ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
-                                               const uint32_t* __restrict__ chain_anchors,
-                                               uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first,
-                                               DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
-                                               uint32_t row_cap, uint32_t dir_rows) {
-    item_ext_pw(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, k, req, jobs, njobs, res, cigars,
-                chain_anchors, scratch, scratch_per, first, rows, dirs, row_cap, dir_rows);
+// Piecewise extension with G lanes per job (sa_pw_group.hpp); job x of the
+// launch keeps its DP row buffer at rows + x * row_cap and its direction bytes
+// at dirs + x * row_cap * dir_rows. The lanes run the job's scalar steps alike
+// and fill its DPs together; lane 0 writes the result.
+template <int G>
+__global__ void __launch_bounds__(128) k_ext_pw_grp(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                    RefView ref, Scores sc, uint32_t k,
+                                                    const ExtRequest* __restrict__ req,
+                                                    const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                                    ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                                    const uint32_t* __restrict__ chain_anchors,
+                                                    uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first,
+                                                    DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
+                                                    uint32_t row_cap, uint32_t dir_rows) {
+    const uint32_t lane = threadIdx.x % 64;
+    const Grp<G> gp{lane % G, lane / G * G};
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (t >= njobs) return;
+    const ExtJob j = jobs[first + t];
+    const ExtRequest r = req[j.req];
+    uint8_t* scr = scratch + (uint64_t)t * scratch_per;
+    const uint64_t ro = off[r.read_id];
+    const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
+    QueryView q{seq + ro, L, (r.flags & 4) != 0};
+    CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
+    ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    RefWin w{&ref, r.ref_id, j.ws, j.wlen};
+    const DpRows dr{rows + (uint64_t)t * row_cap, 1, row_cap, dirs + (uint64_t)t * row_cap * dir_rows, dir_rows};
+    uint32_t* an = (uint32_t*)scr;
+    const uint64_t an_bytes = ((uint64_t)r.n_anchors * 8 + 15) & ~15ull;
+    if (r.kind != EXT_SEED) {
+        const AlignInfo a = pw_probe(r, q, w, sc, k, chain_anchors, scr, scratch_per, cig, dr);
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    } else if (an_bytes >= scratch_per) {
+        out.status = 2;
+    } else {
+        for (uint32_t x = 0; x < 2 * r.n_anchors; ++x) an[x] = chain_anchors[2 * r.anchor_off + x];
+        const uint32_t na = remove_spurious_anchors(an, r.n_anchors);
+        const AlignInfo a = extend_piecewise(q, w, an, na, L / 10, sc, k, scr + an_bytes, scratch_per - an_bytes, cig,
+                                             dr, DpGroup<G>{gp});
+        out.status = (uint32_t)a.status;
+        if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
+    }
+    if (gp.r == 0) res[j.req] = out;
 }
 
 __global__ void __launch_bounds__(128) k_ext_long(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
