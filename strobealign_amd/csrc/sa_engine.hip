@@ -729,10 +729,10 @@ void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, c
     tiers.cap[2] = CHAIN_CAP3;
     if (ev) SA_CHECK(hipEventRecord(ev[3], st));
     k_zero_u32<<<1, 64, 0, st>>>(tiers.count, 4);
-    k_chain_t1<8, CHAIN_CAP1><<<blocks(2ull * n, 8), 64, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b,
-                                                                  o.namn, o.araw, log2tab, same_diag, tiers);
+    k_chain_t1<16, CHAIN_CAP1><<<blocks(2ull * n, 4), 64, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b,
+                                                                   o.namn, o.araw, log2tab, same_diag, tiers);
     SA_CHECK(hipGetLastError());
-    k_chain_tn<16, CHAIN_CAP2><<<2048, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
+    k_chain_tn<64, CHAIN_CAP2><<<8192, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
                                                     same_diag, tiers, 0);
     SA_CHECK(hipGetLastError());
     k_chain_tn<64, CHAIN_CAP3><<<1024, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
