@@ -128,6 +128,14 @@ typedef struct sa_map_options {
      * makes the orientation's lookups in turn) or 8, 16, 64 (a lane per
      * randstrobe); 0: default 16. The hits never depend on it, only the speed. */
     int32_t lookup_lanes;
+    /* Batch pass: 0 the plain per-batch stages (seeding to NAMs, then the
+     * decisions request each extension they need); 1 the fused pass, which runs
+     * seeding, lookups, chaining, NAMs and the extensions the decisions are
+     * expected to need (each single-end read's leading NAMs, each mate's top
+     * NAMs) as one recorded graph replayed per batch, with sizes kept on the
+     * device; the decisions then request only what it did not foresee. The
+     * output is identical either way. */
+    int32_t fused;
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */
@@ -179,6 +187,11 @@ void sa_reads_destroy(sa_reads* reads);
 typedef struct sa_run_stats {
     double total_ms, upload_ms, nam_ms, extend_ms, host_ms;
     uint64_t records, requests, rounds, nams;
+    /* fused pass (sa_map_options.fused): batches it ran, its speculative
+     * extensions, how many of them the decisions used, the passes that replayed
+     * a recorded graph, and the batches whose anchors did not fit its buffers
+     * (run by the plain stages instead) */
+    uint64_t fused_batches, spec_requests, spec_used, graph_replays, fused_overflows;
 } sa_run_stats;
 
 /* Maps a read set. With out == NULL the output is produced and discarded
@@ -189,14 +202,15 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
 /* Schedule autotuner: maps the first sample_pairs records of `reads` (0: all;
  * resident if uploaded, output dropped). Unless opt->lookup_lanes is set, it
  * first times the lookup stage on one engine with 1, 8, 16 and 64 lanes per
- * read orientation and keeps the fastest; then it maps the sample with each
- * engine count in {1, 2, 3, 4, 6, 8} and batch size in {16, 32, 64} Mbases, and
- * writes the fastest choices into opt->lookup_lanes, opt->slots and
- * opt->batch_bases. The outcome never depends on the schedule (the output is
- * identical for every choice); only the speed does. report (may be NULL)
- * receives the trials as JSON [{"lookup_lanes", "lookup_ms"}, ...,
- * {"slots", "batch_bases", "ms"}, ...]. No counterpart in the reference, which
- * has no device schedule. */
+ * read orientation (a thread, lane groups, or a wave per orientation) and keeps
+ * the fastest; then it maps the sample with each batch pass (plain and fused;
+ * the fused pass only in alignment mode), engine count in {1, 2, 3, 4, 6, 8}
+ * and batch size in {16, 32, 64} Mbases, and writes the fastest choices into
+ * opt->lookup_lanes, opt->fused, opt->slots and opt->batch_bases. The outcome
+ * never depends on the schedule (the output is identical for every choice);
+ * only the speed does. report (may be NULL) receives the trials as JSON
+ * [{"lookup_lanes", "lookup_ms"}, ..., {"fused", "slots", "batch_bases", "ms"},
+ * ...]. No counterpart in the reference, which has no device schedule. */
 int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, sa_map_options* opt, char* report,
                 size_t report_cap);
 

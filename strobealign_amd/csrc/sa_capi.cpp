@@ -104,6 +104,8 @@ sa::Options options_of(const sa_map_options* o) {
             throw std::invalid_argument("lookup_lanes must be 0, 1, 8, 16 or 64");
         r.lookup_lanes = (uint32_t)o->lookup_lanes;
     }
+    if (o->fused != 0 && o->fused != 1) throw std::invalid_argument("fused must be 0 or 1");
+    r.fused = o->fused == 1;
     return r;
 }
 
@@ -566,6 +568,9 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
             stats->total_ms = st.total_ms; stats->upload_ms = st.upload_ms; stats->nam_ms = st.nam_ms;
             stats->extend_ms = st.extend_ms; stats->host_ms = st.host_ms; stats->records = st.records;
             stats->requests = st.requests; stats->rounds = st.rounds; stats->nams = st.nams;
+            stats->fused_batches = st.fused_batches; stats->spec_requests = st.spec_requests;
+            stats->spec_used = st.spec_used; stats->graph_replays = st.graph_replays;
+            stats->fused_overflows = st.fused_overflows;
         }
         if (out) {
             if (!out_len) return fail(SA_ERR_ARG, "out_len is NULL");
@@ -631,34 +636,42 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
         const int slot_set[] = {1, 2, 3, 4, 6, 8};
         const uint64_t batch_set[] = {16ull << 20, 32ull << 20, 64ull << 20};
         double best = 1e300;
-        int best_slots = 0;
+        int best_slots = 0, best_fused = 0;
         uint64_t best_batch = 0;
-        for (int sl : slot_set)
-            for (uint64_t bb : batch_set) {
-                sa::Options o = options_of(opt);
-                o.slots = sl;
-                o.batch_bases = bb;
-                o.lookup_lanes = lanes;
-                double ms = 0;
-                for (int rep_i = 0; rep_i < 2; ++rep_i) {  // the first run grows the buffers
-                    sa::RunStats st;
-                    sa::map_records(engines(index, sl), index->ref, r1, r2, o, nullptr, &st, pdr);
-                    ms = st.total_ms;
+        // the fused pass only changes alignment mode (the other modes extend nothing)
+        const int n_pass = options_of(opt).mode == 0 ? 2 : 1;
+        for (int fz = 0; fz < n_pass; ++fz)
+            for (int sl : slot_set)
+                for (uint64_t bb : batch_set) {
+                    sa::Options o = options_of(opt);
+                    o.slots = sl;
+                    o.batch_bases = bb;
+                    o.lookup_lanes = lanes;
+                    o.fused = fz == 1;
+                    double ms = 0;
+                    // the first run grows the buffers (and sizes and records the fused pass)
+                    for (int rep_i = 0; rep_i < 2; ++rep_i) {
+                        sa::RunStats st;
+                        sa::map_records(engines(index, sl), index->ref, r1, r2, o, nullptr, &st, pdr);
+                        ms = st.total_ms;
+                    }
+                    char tmp[160];
+                    std::snprintf(tmp, sizeof tmp,
+                                  "%s{\"fused\": %d, \"slots\": %d, \"batch_bases\": %llu, \"ms\": %.2f}",
+                                  rep.size() > 1 ? ", " : "", fz, sl, (unsigned long long)bb, ms);
+                    rep += tmp;
+                    if (ms < best) {
+                        best = ms;
+                        best_slots = sl;
+                        best_batch = bb;
+                        best_fused = fz;
+                    }
                 }
-                char tmp[128];
-                std::snprintf(tmp, sizeof tmp, "%s{\"slots\": %d, \"batch_bases\": %llu, \"ms\": %.2f}",
-                              rep.size() > 1 ? ", " : "", sl, (unsigned long long)bb, ms);
-                rep += tmp;
-                if (ms < best) {
-                    best = ms;
-                    best_slots = sl;
-                    best_batch = bb;
-                }
-            }
         rep += "]";
         opt->slots = best_slots;
         opt->batch_bases = best_batch;
         opt->lookup_lanes = (int32_t)lanes;
+        opt->fused = best_fused;
         if (report && report_cap) {
             const size_t m = std::min(report_cap - 1, rep.size());
             std::memcpy(report, rep.data(), m);

@@ -210,6 +210,80 @@ void device_free(int device, void* p) {
     (void)hipFree(p);
 }
 
+// ------------------------------------------------------------ fused pass ---
+namespace {
+// Device buffers of the fused pass beyond the plain pass's slots.
+enum FusedSlot {
+    F_SEQ = S_COUNT, F_OFF, F_CTL, F_SUM0, F_SUM1, F_SUM2, F_SUM3, F_SUM4, F_SPECN, F_SPECS, F_ROFF, F_SOFF, F_LSCR,
+    F_PSCR, F_COUNT
+};
+// Control words of the fused pass (u64 each, zeroed at its start, copied to the
+// host at its end): anchor overflow flag (u32), speculative requests written
+// (u32), NAMs, anchor capacity, requests and CIGAR slots the selection wanted,
+// CIGAR ops of the results.
+enum FusedCtl { C_FLAG, C_NREQ, C_NNAMS, C_ANCH, C_WANT, C_WANTS, C_NCIG, C_COUNT };
+// Fixed grids of the launches whose counts exist only on the device.
+constexpr uint32_t F_PREP_BLOCKS = 4096;   // x 4 waves, a request per wave
+constexpr uint32_t F_SSW_BLOCKS = 2048;    // x 256 lanes
+constexpr uint32_t F_FIN_SLOTS = 32768;    // traceback threads (scratch slots), at most
+constexpr uint32_t F_LONG_SLOTS = 2048;    // one-thread SSW of long queries, at most
+constexpr uint64_t F_FIN_BUDGET = 2ull << 30;   // bytes of traceback scratch, at most
+constexpr uint64_t F_LONG_BUDGET = 2ull << 30;  // bytes of one-thread SSW scratch, at most
+constexpr uint64_t F_PW_BUDGET = 4ull << 30;    // bytes of piecewise scratch
+constexpr uint32_t EXT_NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;
+uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+}  // namespace
+
+struct Engine::Fused {
+    // what the pass was sized and recorded for
+    uint32_t n_cap = 0, len_cap = 0;
+    uint64_t bases_cap = 0, a_cap = 0, req_cap = 0, slot_cap = 0;
+    MapParams mp{};
+    ChainParams cp{};
+    SpecParams sp{};
+    Scores sc{};
+    const RefEntry* entries = nullptr;
+    bool stale = true;  // resize before the next pass
+    int runs = 0;       // passes with this shape (the first runs directly)
+    // sizing from what the passes saw
+    double req_per_read = 0, slots_per_req = 0;
+    // own device buffers by slot (they only grow, and only outside a recording)
+    std::vector<void*> buf = std::vector<void*>(F_COUNT, nullptr);
+    std::vector<uint64_t> cap = std::vector<uint64_t>(F_COUNT, 0);
+    bool recording = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t* h_ctl = nullptr;  // pinned: C_COUNT control words, then the EXT_NCNT extension counters
+    int device = 0;
+
+    void* take(int slot, uint64_t bytes) {
+        bytes = std::max<uint64_t>(bytes, 16);
+        if (bytes > cap[slot]) {
+            if (recording) throw std::logic_error("fused pass: a buffer would grow while the pass is recorded");
+            if (buf[slot]) SA_CHECK(hipFree(buf[slot]));
+            buf[slot] = nullptr;
+            cap[slot] = 0;
+            const uint64_t want = round_up(bytes + bytes / 8, 1 << 16);
+            SA_CHECK(hipMalloc(&buf[slot], want));
+            cap[slot] = want;
+        }
+        return buf[slot];
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~Fused() {
+        (void)hipSetDevice(device);
+        drop_graph();
+        for (void* p : buf)
+            if (p) (void)hipFree(p);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+    }
+};
+
 Engine::Engine(int device) : device_(device) {
     int n = 0;
     SA_CHECK(hipGetDeviceCount(&n));
@@ -241,6 +315,7 @@ Engine::Engine(ShareIndex, const Engine& owner) : Engine(owner.device_) {
 
 Engine::~Engine() {
     (void)hipSetDevice(device_);  // destructor: errors are not reported
+    delete fused_;
     for (void* p : buf_)
         if (p) (void)hipFree(p);
     if (owns_index_)
@@ -592,6 +667,7 @@ void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_rea
     d_seq_ = d_seq;
     void* o = grow(S_OFF, (uint64_t)(n_reads + 1) * 8);
     SA_CHECK(hipMemcpyAsync(o, off, (uint64_t)(n_reads + 1) * 8, hipMemcpyHostToDevice, stream_));
+    cur_off_ = (const uint64_t*)o;
     sync();
 }
 
@@ -599,6 +675,7 @@ void Engine::set_chain_anchors(const uint32_t* pairs, uint64_t n_pairs) {
     SA_CHECK(hipSetDevice(device_));
     void* d = grow(S_CHAIN, n_pairs * 8 + 8);
     if (n_pairs) SA_CHECK(hipMemcpyAsync(d, pairs, n_pairs * 8, hipMemcpyHostToDevice, stream_));
+    cur_chain_ = (const uint32_t*)d;
     sync();
 }
 
@@ -698,10 +775,13 @@ void hit_pass(const IndexView& ix, const uint8_t* d_seq, const uint64_t* d_off, 
 // exclusive prefix sum and returns its total: a host wait), the chaining tiers,
 // then NAM counts scanned (the second host wait) and nams_out(N) supplies the
 // NAM output for k_finalize. ev (or null) gets [3, 4] chain, [5, 6] finalize.
+// abort (or null): a device flag that empties every orientation's results (the
+// fused pass sets it when the anchors exceed their fixed buffer).
 template <typename Take, typename Scan, typename NamsOut>
 void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, const float* same_diag,
                 const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const ChainParams& cp, hipStream_t st,
-                Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev, const HitPass& h, NamPass& o) {
+                Take take, Scan scan, NamsOut nams_out, hipEvent_t* ev, const HitPass& h, NamPass& o,
+                const uint32_t* abort = nullptr) {
     const uint32_t k = (uint32_t)ix.p.k;
     auto* d_aoff = (uint64_t*)take(S_AOFF, (2ull * n + 1) * 8);
     const uint64_t A = scan(h.acap, d_aoff, 2ull * n);
@@ -730,7 +810,8 @@ void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, c
     if (ev) SA_CHECK(hipEventRecord(ev[3], st));
     k_zero_u32<<<1, 64, 0, st>>>(tiers.count, 4);
     k_chain_t1<16, CHAIN_CAP1><<<blocks(2ull * n, 4), 64, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b,
-                                                                   o.namn, o.araw, log2tab, same_diag, tiers);
+                                                                   o.namn, o.araw, log2tab, same_diag, tiers,
+                                                                   abort);
     SA_CHECK(hipGetLastError());
     k_chain_tn<64, CHAIN_CAP2><<<8192, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
                                                     same_diag, tiers, 0);
@@ -805,9 +886,13 @@ void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out
         [&](const uint64_t* in, uint64_t* outp, uint64_t m) {
             uint64_t t = 0;
             scan_u64(in, outp, m, &t);
+            // the first scan totals the anchor capacities
+            if (in == (const uint64_t*)buf_[S_ACAP] && n_bases_)
+                anchors_per_base_ = std::max(anchors_per_base_, (double)t / (double)n_bases_);
             return t;
         },
         [&](uint64_t N) { return (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec)); }, ev, o);
+    cur_chain_ = (const uint32_t*)buf_[S_CHAIN];
     finish_nams(n, o, ev, out);
 }
 
@@ -864,6 +949,7 @@ void Engine::chain_hits(const Hit* hits, const HitCounts* hc, const ChainParams&
             return t;
         },
         [&](uint64_t N) { return (NamRec*)grow(S_NAMS, (N + 1) * sizeof(NamRec)); }, ev, h, o);
+    cur_chain_ = (const uint32_t*)buf_[S_CHAIN];
     finish_nams(n, o, ev, out);
 }
 
@@ -906,6 +992,7 @@ void Engine::finish_nams(uint32_t n, const NamPass& o, hipEvent_t* ev, NamBatch&
         r.n_nams = namn[2 * i] + namn[2 * i + 1];
         r.both_orientations = !better_orientation(hc[2 * i], hc[2 * i + 1]) && !better_orientation(hc[2 * i + 1], hc[2 * i]);
     }
+    if (!ev) return;  // the fused pass times itself
     float t;
     // algorithmic bytes: read bases in, query randstrobes out (seed); randstrobes,
     // bucket_starts words and index entries in, hits out (lookups); hits/entries in,
@@ -1028,19 +1115,23 @@ void Engine::find_nams_noalloc(const uint8_t* d_seq, const uint64_t* d_off, uint
 
 namespace {
 // Launches k_ssw_pass<G, BYTE, seg> for a run-time stripe count 1 <= seg <= MAX.
+// dev_grid > 0: the slot counts are read on the device (k_ssw_pass with
+// dev_counts) by a grid of dev_grid blocks.
 template <int G, bool BYTE, int S, int MAX>
 void launch_ssw_pass(uint32_t seg, const SswLaunchLists& ll, hipStream_t st, const uint8_t* seq, const uint64_t* off,
                      RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, uint32_t rev,
-                     SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap) {
+                     SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap,
+                     uint32_t dev_grid = 0) {
     if constexpr (S > MAX) {
         throw std::logic_error("SSW stripe count out of range");
     } else if (seg != S) {
         launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, ll, st, seq, off, ref, sc, req, jobs, rev, fwd, ends, lists, counts,
-                                             cap);
+                                             cap, dev_grid);
     } else {
         const uint64_t count = ll.start[SSW_WCLASSES];
-        k_ssw_pass<G, BYTE, S><<<blocks(count * G, 256), 256, 0, st>>>(seq, off, ref, sc, req, jobs, ll, rev, fwd,
-                                                                       ends, lists, counts, cap);
+        const uint32_t grid = dev_grid ? dev_grid : blocks(count * G, 256);
+        k_ssw_pass<G, BYTE, S><<<grid, 256, 0, st>>>(seq, off, ref, sc, req, jobs, ll, rev, fwd, ends, lists, counts,
+                                                     cap, dev_grid ? 1u : 0u);
     }
 }
 }  // namespace
@@ -1090,8 +1181,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     SA_CHECK(hipMemsetAsync(d_cnt, 0, NCNT * 4, stream_));
     RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
     const auto* d_seq = d_seq_;
-    const auto* d_off = (const uint64_t*)buf_[S_OFF];
-    const uint32_t* d_chain = (const uint32_t*)buf_[S_CHAIN];
+    const auto* d_off = cur_off_;
+    const uint32_t* d_chain = cur_chain_;
     const uint32_t k = (uint32_t)p_.k;
     const uint64_t L = max_len_;
     // kernel times are read once the work is done (no wait per kernel)
@@ -1112,8 +1203,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     };
     // 1. Hamming fast path, rescue prefilter, job lists; SSW jobs into forward-byte buckets
     timed("k_ext_prep", n, [&] {
-        k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, d_res,
-                                                                      d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
+        k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, nullptr,
+                                                                      d_res, d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
     k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
                                                        (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
@@ -1202,13 +1293,13 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         // jobs grouped by the band width their traceback starts with
         auto* d_fin = (uint32_t*)grow(S_FINLISTS, (uint64_t)FIN_CLASSES * nssw * 4);
         uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
-        k_fin_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_ends, nssw, d_fin, d_fcnt, nssw);
+        k_fin_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_ends, nssw, nullptr, d_fin, d_fcnt, nssw);
         SA_CHECK(hipGetLastError());
         timed("k_ssw_finish", nssw, [&] {
             waves(nssw, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
                 k_ssw_finish<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, cnt_,
-                                                                    d_res, d_slots, scr, per, first, d_fin, d_fcnt,
-                                                                    nssw);
+                                                                    nullptr, d_res, d_slots, scr, per, first, d_fin,
+                                                                    d_fcnt, nssw);
             });
         });
     }
@@ -1218,8 +1309,8 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         per = (per + 255) & ~255ull;
         timed("k_ext_long", nlong, [&] {
             waves(nlong, per, [&](uint32_t cnt_, uint32_t first, uint8_t* scr) {
-                k_ext_long<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_ljobs, cnt_, d_res,
-                                                                  d_slots, scr, per, first);
+                k_ext_long<<<blocks(cnt_, 128), 128, 0, stream_>>>(d_seq, d_off, ref, sc, d_req, d_ljobs, cnt_,
+                                                                  nullptr, d_res, d_slots, scr, per, first);
             });
         });
     }
@@ -1241,7 +1332,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
             for (uint64_t first = 0; first < npw; first += slice) {
                 const uint32_t cnt_ = (uint32_t)std::min<uint64_t>(slice, npw - first);
                 k_ext_pw_grp<PW_LANES><<<blocks((uint64_t)cnt_ * PW_LANES, 128), 128, 0, stream_>>>(
-                    d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, d_res, d_slots, d_chain, scr, per,
+                    d_seq, d_off, ref, sc, k, d_req, d_pjobs, cnt_, nullptr, d_res, d_slots, d_chain, scr, per,
                     (uint32_t)first, rows, dirs, rcap, drows);
             }
         });
@@ -1301,6 +1392,296 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     cigars.resize(total);
     if (total) SA_CHECK(hipMemcpyAsync(cigars.data(), d_cout, total * 4, hipMemcpyDeviceToHost, stream_));
     sync();
+}
+
+
+// Every launch of one fused pass on the engine stream, from the reads in
+// F_SEQ / F_OFF (n_cap reads, the ones past the batch empty); either run
+// directly or recorded. Nothing in it waits for the device or reads a count
+// on the host.
+void Engine::fused_sequence(Fused& f) {
+    auto take = [&](int slot, uint64_t bytes) { return f.take(slot, bytes); };
+    hipStream_t st = stream_;
+    const uint32_t n = f.n_cap;
+    const auto* d_seq = (const uint8_t*)f.buf[F_SEQ];
+    const auto* d_off = (const uint64_t*)f.buf[F_OFF];
+    auto* ctl = (uint64_t*)take(F_CTL, C_COUNT * 8);
+    auto* flag = (uint32_t*)(ctl + C_FLAG);
+    auto* nreq = (uint32_t*)(ctl + C_NREQ);
+    k_zero_u32<<<1, 64, 0, st>>>((uint32_t*)ctl, 2 * C_COUNT);
+    SA_CHECK(hipGetLastError());
+    // exclusive scan of m values, total to *total (and *fl set when it exceeds cap)
+    auto dscan = [&](const uint64_t* in, uint64_t* out, uint64_t m, uint64_t* total, uint64_t capv, uint32_t* fl,
+                     int sum_slot) {
+        const uint32_t nb = (uint32_t)std::max<uint64_t>(1, (m + SCAN_PER - 1) / SCAN_PER);
+        auto* sums = (uint64_t*)take(sum_slot, (uint64_t)nb * 8);
+        k_scan_sums<<<nb, 256, 0, st>>>(in, m, sums);
+        k_scan_top<<<1, 256, 0, st>>>(sums, nb, total, capv, fl);
+        k_scan_apply<<<nb, 256, 0, st>>>(in, m, sums, out);
+        SA_CHECK(hipGetLastError());
+    };
+    IndexView ix{d_entries_, d_buckets_, p_};
+    RefView ref{d_words_, d_word_off_, d_len_, n_contigs_};
+    NamPass o;
+    HitPass h;
+    hit_pass(ix, d_seq, d_off, n, f.bases_cap, f.mp, st, take, nullptr, h, o);
+    int scans = 0;
+    chain_pass(
+        ix, ref, d_log2_, d_same_diag_, d_seq, d_off, n, f.cp, st, take,
+        [&](const uint64_t* in, uint64_t* outp, uint64_t m) -> uint64_t {
+            // anchors: checked against their fixed capacity; NAMs fit the same
+            if (scans++ == 0) dscan(in, outp, m, ctl + C_ANCH, f.a_cap, flag, F_SUM0);
+            else dscan(in, outp, m, ctl + C_NNAMS, 0, nullptr, F_SUM1);
+            return f.a_cap;
+        },
+        [&](uint64_t N) { return (NamRec*)take(S_NAMS, (N + 1) * sizeof(NamRec)); }, nullptr, h, o, flag);
+    const auto* nam_cnt = (const uint64_t*)f.buf[S_TOT];
+    // speculative requests
+    auto* rq_n = (uint64_t*)take(F_SPECN, (uint64_t)(n + 1) * 8);
+    auto* rq_s = (uint64_t*)take(F_SPECS, (uint64_t)(n + 1) * 8);
+    auto* roff = (uint64_t*)take(F_ROFF, (uint64_t)(n + 1) * 8);
+    auto* soff = (uint64_t*)take(F_SOFF, (uint64_t)(n + 1) * 8);
+    k_spec_count<<<blocks(n, 256), 256, 0, st>>>(o.nams, o.outoff, nam_cnt, d_off, n, f.sp, rq_n, rq_s);
+    SA_CHECK(hipGetLastError());
+    dscan(rq_n, roff, n, ctl + C_WANT, 0, nullptr, F_SUM2);
+    dscan(rq_s, soff, n, ctl + C_WANTS, 0, nullptr, F_SUM3);
+    const uint32_t cap = (uint32_t)f.req_cap;
+    auto* d_req = (ExtRequest*)take(S_REQ, (uint64_t)cap * sizeof(ExtRequest));
+    k_spec_write<<<blocks(n, 256), 256, 0, st>>>(o.nams, o.outoff, nam_cnt, d_off, n, f.sp, roff, soff, f.req_cap,
+                                                  f.slot_cap, d_req, nreq);
+    SA_CHECK(hipGetLastError());
+    // extension of the requests [0, *nreq): Engine::extend with the job counts
+    // read by the kernels
+    const Scores& sc = f.sc;
+    const uint32_t k = (uint32_t)p_.k;
+    auto* d_res = (ExtResult*)take(S_RES, (uint64_t)cap * sizeof(ExtResult));
+    auto* d_slots = (uint32_t*)take(S_CIGSLOT, f.slot_cap * 4 + 4);
+    auto* d_jobs = (ExtJob*)take(S_JOBS, (uint64_t)cap * sizeof(ExtJob));
+    auto* d_ljobs = (ExtJob*)take(S_LJOBS, (uint64_t)cap * sizeof(ExtJob));
+    auto* d_pjobs = (ExtJob*)take(S_PJOBS, (uint64_t)cap * sizeof(ExtJob));
+    auto* d_cnt = (uint32_t*)take(S_XCNT, EXT_NCNT * 4);
+    uint32_t* d_bcnt = d_cnt + 4;
+    auto* d_lists = (uint32_t*)take(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
+    auto* d_ends = (SswEnds*)take(S_ENDS, (uint64_t)cap * sizeof(SswEnds));
+    auto* d_fwd = (SswEnd*)take(S_FWD, (uint64_t)cap * sizeof(SswEnd));
+    const auto* d_chain = (const uint32_t*)f.buf[S_CHAIN];
+    k_zero_u32<<<1, 64, 0, st>>>(d_cnt, EXT_NCNT);
+    k_ext_prep<<<F_PREP_BLOCKS, 256, 0, st>>>(d_seq, d_off, ref, sc, k, d_req, 0, nreq, d_res, d_slots, d_jobs,
+                                               d_ljobs, d_pjobs, d_cnt);
+    k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
+                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
+    SA_CHECK(hipGetLastError());
+    // every stripe count a query of this shape can have, byte then word, forward
+    // then reverse (a pass only adds jobs to the buckets of later launches)
+    const uint32_t Lq = std::min<uint32_t>(f.len_cap, SSW_GROUP_MAX_QUERY);
+    const uint32_t sb = (Lq + 15) / 16, sw = (Lq + 7) / 8;
+    for (int word = 0; word < 2; ++word)
+        for (int rev = 0; rev < 2; ++rev)
+            for (uint32_t seg = 1; seg <= (word ? sw : sb); ++seg) {
+                SswLaunchLists ll{};
+                for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+                    const uint32_t b = rev ? ssw_bucket(word, true, seg, 0)
+                                           : ssw_bucket(word, false, seg, SSW_WCLASSES - 1 - x);
+                    ll.list[x] = d_lists + (uint64_t)b * cap;
+                }
+                if (word)
+                    launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req, d_jobs,
+                                                                rev, d_fwd, d_ends, d_lists, d_bcnt, cap,
+                                                                F_SSW_BLOCKS);
+                else
+                    launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req, d_jobs,
+                                                                rev, d_fwd, d_ends, d_lists, d_bcnt, cap,
+                                                                F_SSW_BLOCKS);
+                SA_CHECK(hipGetLastError());
+            }
+    // banded traceback of the lane-group jobs (queries of at most
+    // SSW_GROUP_MAX_QUERY; scratch per slot as in Engine::extend)
+    const uint64_t L = f.len_cap;
+    uint64_t per = 4ull * (3 * (2 * 64 + 4) + 17ull * Lq + 2 * (2ull * Lq + 64) + 64);
+    per = (per + 255) & ~255ull;
+    const uint32_t fin_slots =
+        (uint32_t)std::max<uint64_t>(128, std::min<uint64_t>(F_FIN_SLOTS, F_FIN_BUDGET / per) / 128 * 128);
+    auto* d_fin = (uint32_t*)take(S_FINLISTS, (uint64_t)FIN_CLASSES * cap * 4);
+    uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
+    k_fin_bucket<<<256, 256, 0, st>>>(d_ends, 0, d_cnt, d_fin, d_fcnt, cap);
+    auto* fscr = (uint8_t*)take(S_SCRATCH, (uint64_t)fin_slots * per);
+    k_ssw_finish<<<fin_slots / 128, 128, 0, st>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, 0, d_cnt, d_res,
+                                                  d_slots, fscr, per, 0, d_fin, d_fcnt, cap);
+    SA_CHECK(hipGetLastError());
+    // one-thread SSW: only queries too long (or scored too high) for a lane group
+    if (f.len_cap > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * f.len_cap > 30000 || sc.match > 127 ||
+        sc.mismatch > 127) {
+        uint64_t lper = 16ull * ((L + 15) / 16 * 16) + 256 + 65536 + 8 * (2 * L + 64) + 129ull * L * 3;
+        lper = (lper + 255) & ~255ull;
+        const uint32_t long_slots =
+            (uint32_t)std::max<uint64_t>(128, std::min<uint64_t>(F_LONG_SLOTS, F_LONG_BUDGET / lper) / 128 * 128);
+        auto* lscr = (uint8_t*)take(F_LSCR, (uint64_t)long_slots * lper);
+        k_ext_long<<<long_slots / 128, 128, 0, st>>>(d_seq, d_off, ref, sc, d_req, d_ljobs, 0, d_cnt + 1, d_res,
+                                                     d_slots, lscr, lper, 0);
+        SA_CHECK(hipGetLastError());
+    }
+    // piecewise extension: single-end seeds without --ssw
+    if (!f.sp.paired && !f.sp.use_ssw) {
+        uint64_t pper = 16 * L + 256 + 16 * (2 * L + L / 10 + 16) + 16 * L + 4096;
+        pper = (pper + 255) & ~255ull;
+        const uint32_t rcap = (uint32_t)(L + L / 10 + 3), drows = (uint32_t)(L + 2);
+        const uint64_t pw_per = pper + (uint64_t)rcap * sizeof(DpCol) + (uint64_t)rcap * drows;
+        // lane-group slots: whole blocks of 128 lanes
+        const uint64_t groups = std::max<uint64_t>(128 / PW_LANES, F_PW_BUDGET / pw_per / (128 / PW_LANES) * (128 / PW_LANES));
+        auto* pscr = (uint8_t*)take(F_PSCR, groups * pper);
+        auto* rows = (DpCol*)take(S_PWROWS, groups * rcap * sizeof(DpCol));
+        auto* dirs = (uint8_t*)take(S_PWDIRS, groups * rcap * drows);
+        k_ext_pw_grp<PW_LANES><<<(uint32_t)(groups * PW_LANES / 128), 128, 0, st>>>(
+            d_seq, d_off, ref, sc, k, d_req, d_pjobs, 0, d_cnt + 2, d_res, d_slots, d_chain, pscr, pper, 0, rows,
+            dirs, rcap, drows);
+        SA_CHECK(hipGetLastError());
+    }
+    // CIGARs packed in request order
+    auto* d_ncig = (uint64_t*)take(S_NCIG, (uint64_t)cap * 8);
+    auto* d_coff = (uint64_t*)take(S_CIGOFF, (uint64_t)(cap + 1) * 8);
+    k_ncigar_dev<<<1024, 256, 0, st>>>(d_res, nreq, d_ncig, cap);
+    dscan(d_ncig, d_coff, cap, ctl + C_NCIG, 0, nullptr, F_SUM4);
+    auto* d_cout = (uint32_t*)take(S_CIGOUT, f.slot_cap * 4 + 4);
+    k_compact_cigars_dev<<<1024, 256, 0, st>>>(d_req, d_res, d_slots, d_coff, d_cout, nreq);
+    SA_CHECK(hipGetLastError());
+    SA_CHECK(hipMemcpyAsync(f.h_ctl, ctl, C_COUNT * 8, hipMemcpyDeviceToHost, st));
+    SA_CHECK(hipMemcpyAsync(f.h_ctl + C_COUNT, d_cnt, EXT_NCNT * 4, hipMemcpyDeviceToHost, st));
+}
+
+bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecParams& sp, const Scores& sc,
+                       NamBatch& out, SpecBatch& spec) {
+    SA_CHECK(hipSetDevice(device_));
+    const uint32_t n = n_reads_;
+    spec = SpecBatch{};
+    // the first batch of an engine (and any empty one) runs the plain pass, which
+    // measures the anchors per base that size this one
+    if (n == 0 || anchors_per_base_ <= 0) return false;
+    if (!fused_) {
+        fused_ = new Fused;
+        fused_->device = device_;
+        SA_CHECK(hipHostMalloc((void**)&fused_->h_ctl, C_COUNT * 8 + EXT_NCNT * 4 + 64, hipHostMallocDefault));
+    }
+    Fused& f = *fused_;
+    chain_tables(cp);
+    auto same = [](const auto& a, const auto& b) { return std::memcmp(&a, &b, sizeof a) == 0; };
+    const bool fits = !f.stale && n <= f.n_cap && n_bases_ <= f.bases_cap && max_len_ <= f.len_cap &&
+                      same(mp, f.mp) && same(cp, f.cp) && same(sp, f.sp) && same(sc, f.sc) &&
+                      d_entries_ == f.entries;
+    if (!fits) {
+        f.drop_graph();
+        f.runs = 0;
+        f.stale = false;
+        f.mp = mp;
+        f.cp = cp;
+        f.sp = sp;
+        f.sc = sc;
+        f.entries = d_entries_;
+        // room for batches a little larger than this one, so that one recorded
+        // shape serves the batches of a run
+        f.n_cap = (uint32_t)round_up(n + n / 8 + 64, 1024);
+        f.bases_cap = round_up(n_bases_ + n_bases_ / 8 + 4096, 1 << 16);
+        f.len_cap = std::max<uint32_t>(f.len_cap, (uint32_t)round_up(max_len_, 16));
+        f.a_cap = (uint64_t)(anchors_per_base_ * 1.5 * (double)f.bases_cap) + 4096;
+        const double rpr = std::max(f.req_per_read, sp.paired ? 1.25 : 2.0);
+        f.req_cap = round_up((uint64_t)(rpr * 1.25 * f.n_cap) + 1024, 1024);
+        const double avg_len = (double)n_bases_ / (double)n;
+        const double spr = std::max(f.slots_per_req, 2.0 * avg_len + 8.0);
+        f.slot_cap = (uint64_t)(spr * 1.25 * (double)f.req_cap) + 4096;
+        if (f.req_cap > 0xFFFFFFFFull / 2) throw std::runtime_error("fused pass: too many speculative requests");
+    }
+    // the batch's reads into the pass's fixed buffers; the reads past the batch are empty
+    auto* fseq = (uint8_t*)f.take(F_SEQ, f.bases_cap + 16);
+    auto* foff = (uint64_t*)f.take(F_OFF, (uint64_t)(f.n_cap + 1) * 8);
+    if (n_bases_) SA_CHECK(hipMemcpyAsync(fseq, d_seq_, n_bases_, hipMemcpyDeviceToDevice, stream_));
+    SA_CHECK(hipMemcpyAsync(foff, cur_off_, (uint64_t)(n + 1) * 8, hipMemcpyDeviceToDevice, stream_));
+    if (f.n_cap > n) k_pad_off<<<blocks(f.n_cap - n, 256), 256, 0, stream_>>>(foff, n, f.n_cap);
+    SA_CHECK(hipGetLastError());
+    hipEvent_t e0 = event(), e1 = event();
+    SA_CHECK(hipEventRecord(e0, stream_));
+    if (f.runs == 0) {
+        fused_sequence(f);
+    } else {
+        if (!f.exec) {
+            SA_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            f.recording = true;
+            try {
+                fused_sequence(f);
+            } catch (...) {
+                f.recording = false;
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream_, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            f.recording = false;
+            SA_CHECK(hipStreamEndCapture(stream_, &f.graph));
+            SA_CHECK(hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+        } else {
+            ++fused_replays_;
+            spec.replayed = true;
+        }
+        SA_CHECK(hipGraphLaunch(f.exec, stream_));
+    }
+    ++f.runs;
+    SA_CHECK(hipEventRecord(e1, stream_));
+    sync();
+    float ms = 0;
+    SA_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    release_event(e0);
+    release_event(e1);
+    const uint64_t* ctl = f.h_ctl;
+    const uint32_t overflow = (uint32_t)ctl[C_FLAG];
+    const uint64_t n_req = (uint32_t)ctl[C_NREQ], n_nams = ctl[C_NNAMS], anchors = ctl[C_ANCH];
+    const uint64_t wanted = ctl[C_WANT], wanted_slots = ctl[C_WANTS], n_cig = ctl[C_NCIG];
+    // what the next shape is sized from
+    anchors_per_base_ = std::max(anchors_per_base_, (double)anchors / (double)std::max<uint64_t>(1, n_bases_));
+    f.req_per_read = std::max(f.req_per_read, (double)wanted / (double)n);
+    if (wanted) f.slots_per_req = std::max(f.slots_per_req, (double)wanted_slots / (double)wanted);
+    if (wanted > f.req_cap || wanted_slots > f.slot_cap) f.stale = true;
+    times_ = StageTimes{};
+    times_.fused = ms;
+    total_ms_ += ms;
+    add_kernel("fused_pass", ms, n);
+    if (overflow) {
+        f.stale = true;
+        spec.overflow = true;
+        return false;
+    }
+    // speculative results: requests, results, CIGAR offsets and CIGARs
+    const uint64_t b_req = round_up(n_req * sizeof(ExtRequest), 64), b_res = round_up(n_req * sizeof(ExtResult), 64);
+    const uint64_t b_off = round_up((n_req + 1) * 8, 64);
+    auto* hs = (uint8_t*)host_stage(2, b_req + b_res + b_off + n_cig * 4 + 64);
+    spec.req = HostView<ExtRequest>{(ExtRequest*)hs, n_req};
+    spec.res = HostView<ExtResult>{(ExtResult*)(hs + b_req), n_req};
+    spec.cigar_off = HostView<uint64_t>{(uint64_t*)(hs + b_req + b_res), n_req + 1};
+    spec.cigars = HostView<uint32_t>{(uint32_t*)(hs + b_req + b_res + b_off), n_cig};
+    spec.wanted = wanted;
+    if (n_req) {
+        SA_CHECK(hipMemcpyAsync(spec.req.data(), f.buf[S_REQ], n_req * sizeof(ExtRequest), hipMemcpyDeviceToHost,
+                                stream_));
+        SA_CHECK(hipMemcpyAsync(spec.res.data(), f.buf[S_RES], n_req * sizeof(ExtResult), hipMemcpyDeviceToHost,
+                                stream_));
+        SA_CHECK(hipMemcpyAsync(spec.cigar_off.data(), f.buf[S_CIGOFF], n_req * 8, hipMemcpyDeviceToHost, stream_));
+    }
+    if (n_cig) SA_CHECK(hipMemcpyAsync(spec.cigars.data(), f.buf[S_CIGOUT], n_cig * 4, hipMemcpyDeviceToHost, stream_));
+    // the NAMs (finish_nams waits for all of the copies)
+    NamPass o;
+    o.qrn = (uint32_t*)f.buf[S_QRN];
+    o.hc = (HitCounts*)f.buf[S_HC];
+    o.namn = (uint32_t*)f.buf[S_NAMN];
+    o.araw = (uint32_t*)f.buf[S_ARAW];
+    o.cbytes = (uint32_t*)f.buf[S_CBYTES];
+    o.hbytes = (uint32_t*)f.buf[S_HBYTES];
+    o.outoff = (uint64_t*)f.buf[S_OUTOFF];
+    o.nams = (NamRec*)f.buf[S_NAMS];
+    o.n_nams = n_nams;
+    finish_nams(n, o, nullptr, out);
+    spec.cigar_off[n_req] = n_cig;
+    // the extension calls that follow work on this pass's reads and anchors
+    d_seq_ = fseq;
+    cur_off_ = foff;
+    cur_chain_ = (const uint32_t*)f.buf[S_CHAIN];
+    return true;
 }
 
 }  // namespace sa

@@ -39,9 +39,24 @@ struct NamBatch {
     std::vector<ReadNamInfo> info;  // n_reads
 };
 
+// Extensions computed by the fused batch pass ahead of the read-level
+// decisions (SpecParams): views into the engine's pinned staging memory, valid
+// until its next fused pass. Requests are in read order; result y's CIGAR is
+// cigars[cigar_off[y], cigar_off[y + 1]). A result with a nonzero status has
+// to be computed again by extend().
+struct SpecBatch {
+    HostView<ExtRequest> req;
+    HostView<ExtResult> res;
+    HostView<uint64_t> cigar_off;  // req.size() + 1
+    HostView<uint32_t> cigars;
+    uint64_t wanted = 0;  // requests the selection made (more than req.size() when the buffers were full)
+    bool replayed = false;  // the pass ran as a replay of its recorded graph
+    bool overflow = false;  // map_fused returned false: the anchors did not fit
+};
+
 // Per-stage timings of the last call, in milliseconds (HIP events on the engine stream).
 struct StageTimes {
-    float seed = 0, hits = 0, chain = 0, finalize = 0, extend = 0, scans = 0;
+    float seed = 0, hits = 0, chain = 0, finalize = 0, extend = 0, scans = 0, fused = 0;
 };
 
 // Plain device buffers outside an engine (a read set staged once for many calls).
@@ -116,6 +131,22 @@ public:
     // hc[2i + t % 2].n_hits of them; find_hits, src/hit.rs:270-333), and NAMs from
     // hits in that layout (src/chainer.rs:182-267 after find_hits).
     void find_hits(const MapParams& mp, std::vector<Hit>& hits, std::vector<HitCounts>& hc);
+
+    // The fused batch pass: seeding -> lookups -> chaining -> NAMs -> the
+    // speculative extensions of `sp` -> their traceback and CIGARs, with every
+    // size that depends on the data kept on the device, so the whole pass is one
+    // recorded graph: run directly the first time for a batch shape (reads,
+    // bases, longest read, parameters), recorded as a hipGraph the second time
+    // and replayed from then on, with one wait at its end. Its buffers have fixed
+    // capacities sized from what this engine has seen; a batch whose anchors do
+    // not fit returns false (nothing written; the next pass is sized larger) and
+    // is run by find_nams instead, as is an engine's first batch. Speculative
+    // requests beyond the buffers are left out (spec.wanted counts them). Reads
+    // as for find_nams; the chain anchors stay for extend().
+    bool map_fused(const MapParams& mp, const ChainParams& cp, const SpecParams& sp, const Scores& sc, NamBatch& out,
+                   SpecBatch& spec);
+    // Passes run by map_fused as a graph replay (not counting recordings).
+    uint64_t fused_replays() const { return fused_replays_; }
     void chain_hits(const Hit* hits, const HitCounts* hc, const ChainParams& cp, NamBatch& out);
 
     // find_nams for reads and outputs in device memory, on the caller's stream,
@@ -159,6 +190,8 @@ public:
     uint64_t total_lookup_bytes() const { return lookup_bytes_; }
 
 private:
+    struct Fused;
+    void fused_sequence(Fused& f);
     void chain_tables(const ChainParams& cp);
     void finish_nams(uint32_t n, const NamPass& o, ihipEvent_t** ev, NamBatch& out);
     void* grow(int slot, uint64_t bytes);
@@ -189,12 +222,20 @@ private:
     uint32_t max_len_ = 0;
     std::vector<uint64_t> h_off_;
     const uint8_t* d_seq_ = nullptr;
+    // offsets and chain anchors of the reads the extension calls work on (the
+    // plain buffers, or the fused pass's after map_fused)
+    const uint64_t* cur_off_ = nullptr;
+    const uint32_t* cur_chain_ = nullptr;
+    // largest anchors-per-base ratio of a plain NAM pass (sizes the fused pass)
+    double anchors_per_base_ = 0;
+    Fused* fused_ = nullptr;
+    uint64_t fused_replays_ = 0;
     bool owns_index_ = true;
     // growable device buffers indexed by slot
     std::vector<void*> buf_;
     std::vector<uint64_t> cap_;
-    void* h_stage_[2] = {nullptr, nullptr};
-    uint64_t h_stage_cap_[2] = {0, 0};
+    void* h_stage_[3] = {nullptr, nullptr, nullptr};
+    uint64_t h_stage_cap_[3] = {0, 0, 0};
     StageTimes times_;
     std::vector<std::pair<std::string, KernelStat>> kstats_;
     double total_ms_ = 0;

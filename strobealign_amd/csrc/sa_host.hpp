@@ -94,6 +94,8 @@ struct Options {
     uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
     int slots = 4;                        // engines working on batches concurrently
     uint32_t lookup_lanes = 16;           // lanes per read orientation in the lookup stage (1, 8, 16, 64)
+    bool fused = false;                   // fused batch pass (Engine::map_fused) instead of find_nams
+    uint32_t spec_per_read = 8;           // most speculative extensions per read in the fused pass
     // -A/-B/-O/-E/-L and --xdrop (src/main.rs:246-272; defaults src/aligner.rs:20-30)
     Scores scores{2, 8, 12, 1, 10, 500};
 };
@@ -104,6 +106,11 @@ std::string sam_header(const PackedRef& ref, const Options& o);
 struct RunStats {
     double nam_ms = 0, extend_ms = 0, host_ms = 0, upload_ms = 0, total_ms = 0;
     uint64_t records = 0, requests = 0, rounds = 0, nams = 0;
+    // fused pass: batches run by it, its speculative extensions, and how many of
+    // them the decisions used
+    uint64_t fused_batches = 0, spec_requests = 0, spec_used = 0;
+    // fused passes that replayed a recorded graph; passes whose anchors did not fit
+    uint64_t graph_replays = 0, fused_overflows = 0;
 };
 
 // Maps all records (paired when r2 is non-empty) and appends the output

@@ -316,11 +316,11 @@ SA_HD void item_ext_prep(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
 }
 
 // SSW job in one thread (scalar lanes): queries longer than SSW_GROUP_MAX_QUERY.
-SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
-                               const ExtRequest* req, const ExtJob* jobs, uint32_t njobs, ExtResult* res,
-                               uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first) {
-    if (t >= njobs) return;
-    const ExtJob j = jobs[first + t];
+// Job `job` of the list with scratch slot `slot` (scratch_per bytes each).
+SA_HD void item_ext_long(uint32_t job, uint32_t slot, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
+                         const ExtRequest* req, const ExtJob* jobs, ExtResult* res, uint32_t* cigars,
+                         uint8_t* scratch, uint64_t scratch_per) {
+    const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
     const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
@@ -328,7 +328,7 @@ SA_HD void item_ext_long(uint32_t t, const uint8_t* seq, const uint64_t* off, Re
     CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     RefWin w{&ref, r.ref_id, j.ws, j.wlen};
-    AlignInfo a = ssw_aligner_align(q, w, sc, scratch + (uint64_t)t * scratch_per, scratch_per, cig);
+    AlignInfo a = ssw_aligner_align(q, w, sc, scratch + (uint64_t)slot * scratch_per, scratch_per, cig);
     out.status = (uint32_t)a.status;
     if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);
     res[j.req] = out;
@@ -349,20 +349,19 @@ SA_HD uint32_t fin_class(const SswEnds& e) {
     return bw > 32 ? 0u : bw > 16 ? 1u : bw > 8 ? 2u : bw > 4 ? 3u : bw > 2 ? 4u : bw > 1 ? 5u : 6u;
 }
 // lists: FIN_CLASSES lists of `cap` entries; counts: their lengths (zeroed before)
-SA_HD void item_fin_bucket(uint32_t t, const SswEnds* ends, uint32_t njobs, uint32_t* lists, uint32_t* counts,
-                           uint32_t cap) {
-    if (t >= njobs) return;
+SA_HD void item_fin_bucket(uint32_t t, const SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap) {
     const uint32_t c = fin_class(ends[t]);
     lists[(uint64_t)c * cap + SA_ATOMIC_ADD(&counts[c], 1u)] = t;
 }
 
-// Thread t runs the job at position first + t of the class lists taken in order.
-SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, RefView ref, Scores sc,
-                           const ExtRequest* req, const ExtJob* jobs, const SswEnds* ends, uint32_t njobs,
-                           ExtResult* res, uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, uint32_t first,
-                           const uint32_t* lists, const uint32_t* counts, uint32_t cap) {
-    if (t >= njobs) return;
-    uint32_t pos = first + t, c = 0;
+// The job at position pos of the class lists taken in order, with scratch slot
+// `slot` (the slots of a wave's 64 threads interleave their scratch words, so
+// the scratch covers whole waves of scratch_per bytes per slot).
+SA_HD void item_ssw_finish(uint32_t pos, uint32_t slot, const uint8_t* seq, const uint64_t* off, RefView ref,
+                           Scores sc, const ExtRequest* req, const ExtJob* jobs, const SswEnds* ends, ExtResult* res,
+                           uint32_t* cigars, uint8_t* scratch, uint64_t scratch_per, const uint32_t* lists,
+                           const uint32_t* counts, uint32_t cap) {
+    uint32_t c = 0;
     while (c + 1 < FIN_CLASSES && pos >= counts[c]) pos -= counts[c++];
     const uint32_t job = lists[(uint64_t)c * cap + pos];
     const ExtJob j = jobs[job];
@@ -373,9 +372,7 @@ SA_HD void item_ssw_finish(uint32_t t, const uint8_t* seq, const uint64_t* off, 
     CigarOut cig{cigars + r.cigar_off, r.cigar_cap, 0, false};
     ExtResult out{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     RefWin w{&ref, r.ref_id, j.ws, j.wlen};
-    // the 64 jobs of a wave interleave their scratch words (the launch's scratch
-    // covers whole waves of scratch_per bytes per job)
-    const LaneMem m{(uint32_t*)(scratch + (uint64_t)(t / 64) * 64 * scratch_per) + t % 64, 64, scratch_per / 4};
+    const LaneMem m{(uint32_t*)(scratch + (uint64_t)(slot / 64) * 64 * scratch_per) + slot % 64, 64, scratch_per / 4};
     AlignInfo a = ssw_finish(q, w, ends[job], sc, m, cig);
     out.status = (uint32_t)a.status;
     if (a.found && a.status == 0) write_found(out, a, j.ws, true, cig.n);

@@ -66,7 +66,7 @@ template __global__ void k_hits_grp<64>(IndexView, MapParams, const uint64_t*, u
 // Chaining, first tier: one read orientation per group of G lanes with its
 // anchors in LDS (sa_chain_group.hpp), for anchor capacities up to tiers.cap[0]
 // (CAP); orientations not chained (src/chainer.rs:206-216) get empty results,
-// larger ones are listed for the later tiers.
+// larger ones are listed for the later tiers (which run only what is listed).
 template <int G, int CAP>
 __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, const uint64_t* __restrict__ off,
                                                  uint32_t n, const Hit* __restrict__ hits,
@@ -74,13 +74,16 @@ __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, c
                                                  const uint64_t* __restrict__ aoff, ChainBufs b,
                                                  uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
                                                  const float* __restrict__ log2tab,
-                                                 const float* __restrict__ same_diag, ChainTiers tiers) {
+                                                 const float* __restrict__ same_diag, ChainTiers tiers,
+                                                 const uint32_t* __restrict__ abort) {
     __shared__ ChainLds<CAP> lds[64 / G];
     const uint32_t g = threadIdx.x / G;
     const Grp<G> gp{threadIdx.x % G, g * G};
     const uint32_t t = blockIdx.x * (64 / G) + g;
     if (t >= 2 * n) return;
-    if (!orientation_chained(hc, t)) {
+    // abort (when given and set): the anchors do not fit their buffer; every
+    // orientation gets empty results and the batch is run again another way
+    if (!orientation_chained(hc, t) || (abort && *abort)) {
         if (gp.r == 0) {
             nam_n[t] = 0;
             anchors_raw[t] = 0;
@@ -272,15 +275,12 @@ __device__ int32_t two_diagonal_score_wave(const Q& q, const RefWin& w, uint32_t
 // compares the read with the reference in 64-base ballots, and the rescue
 // prefilter has_shared_substring (src/mapper.rs:1156-1170) tests 64 window
 // positions per step. Gapped alignments are queued as jobs.
-__global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
-                                                 RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
-                                                 uint32_t nreq, ExtResult* __restrict__ res,
-                                                 uint32_t* __restrict__ cigars, ExtJob* __restrict__ ssw_jobs,
-                                                 ExtJob* __restrict__ long_jobs, ExtJob* __restrict__ pw_jobs,
-                                                 uint32_t* __restrict__ counters) {
-    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t lane = threadIdx.x & 63u;
-    if (t >= nreq) return;
+__device__ inline void ext_prep_one(uint32_t t, uint32_t lane, const uint8_t* __restrict__ seq,
+                                    const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc, uint32_t k,
+                                    const ExtRequest* __restrict__ req, ExtResult* __restrict__ res,
+                                    uint32_t* __restrict__ cigars, ExtJob* __restrict__ ssw_jobs,
+                                    ExtJob* __restrict__ long_jobs, ExtJob* __restrict__ pw_jobs,
+                                    uint32_t* __restrict__ counters) {
     const ExtRequest r = req[t];
     const uint64_t ro = off[r.read_id];
     const uint32_t L = (uint32_t)(off[r.read_id + 1] - ro);
@@ -440,25 +440,29 @@ __global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ se
     }
 }
 
-// Piecewise extension with G lanes per job (sa_pw_group.hpp); job x of the
-// launch keeps its DP row buffer at rows + x * row_cap and its direction bytes
-// at dirs + x * row_cap * dir_rows. The lanes run the job's scalar steps alike
-// and fill its DPs together; lane 0 writes the result.
+// The requests [0, nreq), or [0, *dnreq) when dnreq is given, a wave each, grid-stride.
+__global__ void __launch_bounds__(256) k_ext_prep(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                 RefView ref, Scores sc, uint32_t k, const ExtRequest* __restrict__ req,
+                                                 uint32_t nreq, const uint32_t* __restrict__ dnreq,
+                                                 ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                                 ExtJob* __restrict__ ssw_jobs, ExtJob* __restrict__ long_jobs,
+                                                 ExtJob* __restrict__ pw_jobs, uint32_t* __restrict__ counters) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t n = dnreq ? *dnreq : nreq;
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n; t += gridDim.x * (blockDim.x >> 6))
+        ext_prep_one(t, lane, seq, off, ref, sc, k, req, res, cigars, ssw_jobs, long_jobs, pw_jobs, counters);
+}
+
+// Piecewise job `job` of the list with scratch slot t on the lane group gp.
 template <int G>
-__global__ void __launch_bounds__(128) k_ext_pw_grp(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
-                                                    RefView ref, Scores sc, uint32_t k,
-                                                    const ExtRequest* __restrict__ req,
-                                                    const ExtJob* __restrict__ jobs, uint32_t njobs,
-                                                    ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
-                                                    const uint32_t* __restrict__ chain_anchors,
-                                                    uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first,
-                                                    DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
-                                                    uint32_t row_cap, uint32_t dir_rows) {
-    const uint32_t lane = threadIdx.x % 64;
-    const Grp<G> gp{lane % G, lane / G * G};
-    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
-    if (t >= njobs) return;
-    const ExtJob j = jobs[first + t];
+__device__ inline void ext_pw_job(const Grp<G>& gp, uint32_t job, uint32_t t, const uint8_t* __restrict__ seq,
+                                  const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc, uint32_t k,
+                                  const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs,
+                                  ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                  const uint32_t* __restrict__ chain_anchors, uint8_t* __restrict__ scratch,
+                                  uint64_t scratch_per, DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
+                                  uint32_t row_cap, uint32_t dir_rows) {
+    const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     uint8_t* scr = scratch + (uint64_t)t * scratch_per;
     const uint64_t ro = off[r.read_id];
@@ -487,31 +491,84 @@ __global__ void __launch_bounds__(128) k_ext_pw_grp(const uint8_t* __restrict__ 
     if (gp.r == 0) res[j.req] = out;
 }
 
+// Piecewise extension with G lanes per job (sa_pw_group.hpp); the lane group
+// with scratch slot x keeps its DP row buffer at rows + x * row_cap and its
+// direction bytes at dirs + x * row_cap * dir_rows. The lanes run the job's
+// scalar steps alike and fill its DPs together; lane 0 writes the result.
+// Job counts as for k_ext_long below.
+template <int G>
+__global__ void __launch_bounds__(128) k_ext_pw_grp(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                    RefView ref, Scores sc, uint32_t k,
+                                                    const ExtRequest* __restrict__ req,
+                                                    const ExtJob* __restrict__ jobs, uint32_t njobs,
+                                                    const uint32_t* __restrict__ dn,
+                                                    ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                                    const uint32_t* __restrict__ chain_anchors,
+                                                    uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first,
+                                                    DpCol* __restrict__ rows, uint8_t* __restrict__ dirs,
+                                                    uint32_t row_cap, uint32_t dir_rows) {
+    const uint32_t lane = threadIdx.x % 64;
+    const Grp<G> gp{lane % G, lane / G * G};
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    if (dn) {
+        const uint32_t n = *dn;
+        for (uint32_t x = t; x < n; x += gridDim.x * blockDim.x / G)
+            ext_pw_job<G>(gp, x, t, seq, off, ref, sc, k, req, jobs, res, cigars, chain_anchors, scratch, scratch_per,
+                          rows, dirs, row_cap, dir_rows);
+    } else if (t < njobs) {
+        ext_pw_job<G>(gp, first + t, t, seq, off, ref, sc, k, req, jobs, res, cigars, chain_anchors, scratch,
+                      scratch_per, rows, dirs, row_cap, dir_rows);
+    }
+}
+
+// Job counts: a kernel either runs the host-given range [first, first + njobs),
+// a thread (lane group) per job with scratch slot = its index in the launch, or,
+// when `dn` is given, the device-side count *dn from job 0, grid-stride, each
+// thread (lane group) keeping its own scratch slot across its jobs. The second
+// form lets a launch be recorded once with a fixed grid and replayed for
+// batches whose job counts exist only on the device.
 __global__ void __launch_bounds__(128) k_ext_long(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                                  RefView ref, Scores sc, const ExtRequest* __restrict__ req,
                                                  const ExtJob* __restrict__ jobs, uint32_t njobs,
-                                                 ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
-                                                 uint8_t* __restrict__ scratch, uint64_t scratch_per, uint32_t first) {
-    item_ext_long(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, req, jobs, njobs, res, cigars,
-                        scratch, scratch_per, first);
+                                                 const uint32_t* __restrict__ dn, ExtResult* __restrict__ res,
+                                                 uint32_t* __restrict__ cigars, uint8_t* __restrict__ scratch,
+                                                 uint64_t scratch_per, uint32_t first) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dn) {
+        const uint32_t n = *dn;
+        for (uint32_t x = tid; x < n; x += gridDim.x * blockDim.x)
+            item_ext_long(x, tid, seq, off, ref, sc, req, jobs, res, cigars, scratch, scratch_per);
+    } else if (tid < njobs) {
+        item_ext_long(first + tid, tid, seq, off, ref, sc, req, jobs, res, cigars, scratch, scratch_per);
+    }
 }
 
 __global__ void __launch_bounds__(128) k_ssw_finish(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                                    RefView ref, Scores sc, const ExtRequest* __restrict__ req,
                                                    const ExtJob* __restrict__ jobs, const SswEnds* __restrict__ ends,
-                                                   uint32_t njobs, ExtResult* __restrict__ res,
-                                                   uint32_t* __restrict__ cigars, uint8_t* __restrict__ scratch,
-                                                   uint64_t scratch_per, uint32_t first,
-                                                   const uint32_t* __restrict__ lists,
+                                                   uint32_t njobs, const uint32_t* __restrict__ dn,
+                                                   ExtResult* __restrict__ res, uint32_t* __restrict__ cigars,
+                                                   uint8_t* __restrict__ scratch, uint64_t scratch_per,
+                                                   uint32_t first, const uint32_t* __restrict__ lists,
                                                    const uint32_t* __restrict__ counts, uint32_t cap) {
-    item_ssw_finish(blockIdx.x * blockDim.x + threadIdx.x, seq, off, ref, sc, req, jobs, ends, njobs, res, cigars,
-                    scratch, scratch_per, first, lists, counts, cap);
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dn) {
+        const uint32_t n = *dn;
+        for (uint32_t x = tid; x < n; x += gridDim.x * blockDim.x)
+            item_ssw_finish(x, tid, seq, off, ref, sc, req, jobs, ends, res, cigars, scratch, scratch_per, lists,
+                            counts, cap);
+    } else if (tid < njobs) {
+        item_ssw_finish(first + tid, tid, seq, off, ref, sc, req, jobs, ends, res, cigars, scratch, scratch_per,
+                        lists, counts, cap);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_fin_bucket(const SswEnds* __restrict__ ends, uint32_t njobs,
-                                                   uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
-                                                   uint32_t cap) {
-    item_fin_bucket(blockIdx.x * blockDim.x + threadIdx.x, ends, njobs, lists, counts, cap);
+                                                   const uint32_t* __restrict__ dn, uint32_t* __restrict__ lists,
+                                                   uint32_t* __restrict__ counts, uint32_t cap) {
+    const uint32_t n = dn ? *dn : njobs;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x)
+        item_fin_bucket(x, ends, lists, counts, cap);
 }
 
 // SSW score-pass buckets: jobs are listed by (pass, stripe count) so that each
@@ -536,22 +593,22 @@ SA_HD uint32_t ssw_bucket(bool word, bool rev, uint32_t seg, uint32_t wc) {
 }
 
 // Forward byte-pass buckets of the SSW jobs (queries of at most SSW_GROUP_MAX_QUERY);
-// the job count is counters[0] on the device, the grid covers every request. Also
+// the job count is counters[0] on the device (grid-stride). Also
 // sums the jobs' forward DP cells (query length x window length) into *cells.
 __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__ off,
                                                    const ExtRequest* __restrict__ req,
                                                    const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ counters,
                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
                                                    uint32_t cap, unsigned long long* __restrict__ cells) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t c = 0;
-    if (t < counters[0]) {
+    const uint32_t n = counters[0];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const uint32_t rid = req[jobs[t].req].read_id;
         const uint32_t L = (uint32_t)(off[rid + 1] - off[rid]);
         const bool word = (jobs[t].flags & SSW_WORD_FIRST) != 0;
         const uint32_t b = ssw_bucket(word, false, word ? (L + 7) / 8 : (L + 15) / 16, ssw_wclass(jobs[t].wlen));
         lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
-        c = word ? 0 : (uint64_t)L * jobs[t].wlen;  // cells of the byte forward pass
+        c += word ? 0 : (uint64_t)L * jobs[t].wlen;  // cells of the byte forward pass
     }
     // one atomic per wave
     SA_UNROLL
@@ -572,20 +629,56 @@ struct SswLaunchLists {
     uint32_t start[SSW_WCLASSES + 1];  // slot ranges of the lists
 };
 
+//
+// With dev_counts set, the launch was recorded before the counts existed: the
+// slot ranges come from the bucket counts on the device (the window classes of
+// stripe count SEG, longest first, for a forward pass; the one reverse bucket
+// of SEG for a reverse pass) and the lane groups stride over the slots.
+template <int G, bool BYTE, int SEG>
+__device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __restrict__ seq,
+                                    const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
+                                    const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
+                                    SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
+                                    uint32_t* __restrict__ counts, uint32_t cap);
+
 template <int G, bool BYTE, int SEG>
 __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                                  RefView ref, Scores sc, const ExtRequest* __restrict__ req,
                                                  const ExtJob* __restrict__ jobs, SswLaunchLists ll,
                                                  uint32_t rev, SswEnd* __restrict__ fwd,
                                                  SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                                 uint32_t* __restrict__ counts, uint32_t cap) {
+                                                 uint32_t* __restrict__ counts, uint32_t cap, uint32_t dev_counts) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t slot = tid / G, l = tid % G;
-    if (slot >= ll.start[SSW_WCLASSES]) return;
-    uint32_t c = 0;
-    SA_UNROLL
-    for (uint32_t x = 1; x < SSW_WCLASSES; ++x) c += slot >= ll.start[x] ? 1u : 0u;
-    const uint32_t job = ll.list[c][slot - ll.start[c]];
+    const uint32_t l = tid % G;
+    uint32_t stride = 0xFFFFFFFFu;
+    if (dev_counts) {
+        uint32_t at = 0;
+        SA_UNROLL
+        for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+            ll.start[x] = at;
+            if (!rev) at += counts[ssw_bucket(!BYTE, false, SEG, SSW_WCLASSES - 1 - x)];
+            else if (x == 0) at += counts[ssw_bucket(!BYTE, true, SEG, 0)];
+        }
+        ll.start[SSW_WCLASSES] = at;
+        stride = gridDim.x * blockDim.x / G;
+    }
+    const uint32_t total = ll.start[SSW_WCLASSES];
+    for (uint32_t slot = tid / G; slot < total; slot += stride) {
+        uint32_t c = 0;
+        SA_UNROLL
+        for (uint32_t x = 1; x < SSW_WCLASSES; ++x) c += slot >= ll.start[x] ? 1u : 0u;
+        ssw_pass_job<G, BYTE, SEG>(ll.list[c][slot - ll.start[c]], l, seq, off, ref, sc, req, jobs, rev, fwd, ends,
+                                   lists, counts, cap);
+        if (stride == 0xFFFFFFFFu) break;
+    }
+}
+
+template <int G, bool BYTE, int SEG>
+__device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __restrict__ seq,
+                                    const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
+                                    const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
+                                    SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
+                                    uint32_t* __restrict__ counts, uint32_t cap) {
     const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
@@ -673,6 +766,213 @@ __global__ void __launch_bounds__(256) k_ix_buckets_head(const RefEntry* __restr
 __global__ void __launch_bounds__(256) k_ix_runs(const RefEntry* __restrict__ e, uint64_t n,
                                                 unsigned long long* __restrict__ hist) {
     item_ix_runs(blockIdx.x * blockDim.x + threadIdx.x, e, n, hist);
+}
+
+
+// ------------------------------------------------------------ fused pass ---
+// Kernels of the batch pass that is recorded once as a graph and replayed:
+// every size that depends on the data stays on the device.
+
+// Exclusive prefix sum of m u64 values (m fixed when the launches are
+// recorded), in three launches: per-block sums of SCAN_PER values, one block
+// scanning those sums (total to *total; *flag set when it exceeds cap), and each
+// block scanning its values from its sum's offset.
+constexpr uint32_t SCAN_PER = 2048;  // 256 threads x 8 values
+
+__device__ inline uint64_t block_sum_256(uint64_t v, uint64_t* sh) {
+    SA_UNROLL
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    if ((threadIdx.x & 63u) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const uint64_t t = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return t;
+}
+
+// Exclusive scan of one value per thread over a block of 256; returns the
+// thread's exclusive prefix and the block total in *tot.
+__device__ inline uint64_t block_excl_256(uint64_t v, uint64_t* sh, uint64_t* tot) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint64_t inc = v;
+    SA_UNROLL
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t u = __shfl_up(inc, d);
+        if (lane >= (uint32_t)d) inc += u;
+    }
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    uint64_t before = 0;
+    for (uint32_t x = 0; x < w; ++x) before += sh[x];
+    *tot = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ void __launch_bounds__(256) k_scan_sums(const uint64_t* __restrict__ in, uint64_t m,
+                                                  uint64_t* __restrict__ sums) {
+    __shared__ uint64_t sh[4];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER;
+    uint64_t v = 0;
+    SA_UNROLL
+    for (uint32_t x = 0; x < SCAN_PER / 256; ++x) {
+        const uint64_t i = base + x * 256 + threadIdx.x;
+        if (i < m) v += in[i];
+    }
+    const uint64_t t = block_sum_256(v, sh);
+    if (threadIdx.x == 0) sums[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_scan_top(uint64_t* __restrict__ sums, uint32_t nb, uint64_t* __restrict__ total,
+                                                 uint64_t cap, uint32_t* __restrict__ flag) {
+    __shared__ uint64_t sh[4];
+    uint64_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 256) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint64_t v = i < nb ? sums[i] : 0;
+        uint64_t tot = 0;
+        const uint64_t e = block_excl_256(v, sh, &tot);
+        if (i < nb) sums[i] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) {
+        *total = carry;
+        if (flag && carry > cap) *flag = 1;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_apply(const uint64_t* __restrict__ in, uint64_t m,
+                                                   const uint64_t* __restrict__ sums, uint64_t* __restrict__ out) {
+    __shared__ uint64_t sh[4];
+    constexpr uint32_t PER = SCAN_PER / 256;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_PER + (uint64_t)threadIdx.x * PER;
+    uint64_t v[PER], s = 0;
+    SA_UNROLL
+    for (uint32_t x = 0; x < PER; ++x) {
+        v[x] = base + x < m ? in[base + x] : 0;
+        s += v[x];
+    }
+    uint64_t tot = 0;
+    uint64_t e = sums[blockIdx.x] + block_excl_256(s, sh, &tot);
+    SA_UNROLL
+    for (uint32_t x = 0; x < PER; ++x) {
+        if (base + x < m) out[base + x] = e;
+        e += v[x];
+    }
+}
+
+// Offsets of the padding reads: reads n..cap-1 are empty (off[i] = off[n]).
+__global__ void __launch_bounds__(256) k_pad_off(uint64_t* __restrict__ off, uint32_t n, uint32_t cap) {
+    const uint32_t i = n + 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= cap) off[i] = off[n];
+}
+
+// The speculative extend_seed requests of read i (SpecParams), handed to emit(nam,
+// flipped, consistent) in order; returns how many.
+template <typename Emit>
+__device__ inline uint32_t spec_select(const SpecParams& sp, const NamRec* __restrict__ nams, uint64_t cnt, Emit emit) {
+    if (cnt == 0) return 0;
+    const float top = nams[0].score;
+    uint64_t ntie = 1;
+    while (ntie < cnt && nams[ntie].score == top) ++ntie;
+    uint32_t made = 0;
+    if (!sp.paired) {
+        const uint64_t lim = cnt < (ntie > sp.max_tries ? ntie : sp.max_tries) ? cnt
+                                                                               : (ntie > sp.max_tries ? ntie : sp.max_tries);
+        for (uint64_t x = 0; x < lim && made < sp.max_per_read; ++x) {
+            if (nams[x].score / top < sp.dropoff) break;
+            if (nams[x].flags & 2u) {
+                emit(nams[x], false, true);
+                ++made;
+            }
+        }
+    } else {
+        // reverse_nam_if_needed outcome 1 (consistent once flipped): the
+        // proper-pair path extends the NAM as it is, the other path flipped
+        for (uint64_t x = 0; x < ntie; ++x) {
+            const uint32_t rev = (nams[x].flags >> 2) & 3u;
+            if (made + (rev == 1 ? 2u : 1u) > sp.max_per_read) break;
+            emit(nams[x], false, rev == 0);
+            ++made;
+            if (rev == 1) {
+                emit(nams[x], true, true);
+                ++made;
+            }
+        }
+    }
+    return made;
+}
+
+// Per read: speculative request count and their CIGAR slots (2 L + 8 each).
+__global__ void __launch_bounds__(256) k_spec_count(const NamRec* __restrict__ nams, const uint64_t* __restrict__ nam_off,
+                                                   const uint64_t* __restrict__ nam_cnt, const uint64_t* __restrict__ off,
+                                                   uint32_t n, SpecParams sp, uint64_t* __restrict__ nreq,
+                                                   uint64_t* __restrict__ nslots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t made = spec_select(sp, nams + nam_off[i], nam_cnt[i], [](const NamRec&, bool, bool) {});
+    const uint64_t L = off[i + 1] - off[i];
+    nreq[i] = made;
+    nslots[i] = made * (2 * L + 8);
+}
+
+// Writes the requests of every read whose requests and CIGAR slots fit the
+// buffers (a prefix of the reads, since the offsets grow with i) and raises
+// *n_req to their end.
+__global__ void __launch_bounds__(256) k_spec_write(const NamRec* __restrict__ nams, const uint64_t* __restrict__ nam_off,
+                                                   const uint64_t* __restrict__ nam_cnt, const uint64_t* __restrict__ off,
+                                                   uint32_t n, SpecParams sp, const uint64_t* __restrict__ roff,
+                                                   const uint64_t* __restrict__ soff, uint64_t req_cap,
+                                                   uint64_t slot_cap, ExtRequest* __restrict__ req,
+                                                   uint32_t* __restrict__ n_req) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t L = (uint32_t)(off[i + 1] - off[i]);
+    const uint32_t made = spec_select(sp, nams + nam_off[i], nam_cnt[i], [](const NamRec&, bool, bool) {});
+    if (!made || roff[i] + made > req_cap || soff[i] + made * (2ull * L + 8) > slot_cap) return;
+    uint64_t at = roff[i], slot = soff[i];
+    spec_select(sp, nams + nam_off[i], nam_cnt[i], [&](const NamRec& m, bool flip, bool cons) {
+        ExtRequest q;
+        q.kind = EXT_SEED;
+        q.read_id = i;
+        const bool rc = ((m.flags & 1u) != 0) != flip;
+        q.flags = (sp.paired || sp.use_ssw ? 1u : 0u) | (cons ? 2u : 0u) | (rc ? 4u : 0u);
+        q.ref_id = m.ref_id;
+        q.ref_start = m.ref_start;
+        q.ref_end = m.ref_end;
+        q.query_start = flip ? L - m.query_end : m.query_start;
+        q.query_end = flip ? L - m.query_start : m.query_end;
+        q.anchor_off = m.anchor_off;
+        q.n_anchors = m.n_anchors;
+        q.win_start = q.win_end = 0;
+        q.cigar_cap = 2 * L + 8;
+        q.cigar_off = slot;
+        req[at++] = q;
+        slot += q.cigar_cap;
+    });
+    atomicMax(n_req, (uint32_t)at);
+}
+
+// CIGAR op counts of the results [0, *n) and zeros up to cap (the scan's input).
+__global__ void __launch_bounds__(256) k_ncigar_dev(const ExtResult* __restrict__ r, const uint32_t* __restrict__ n,
+                                                   uint64_t* __restrict__ out, uint32_t cap) {
+    const uint32_t m = *n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cap; i += gridDim.x * blockDim.x)
+        out[i] = i < m && r[i].found ? r[i].n_cigar : 0;
+}
+
+// The CIGARs of results [0, *n) packed at their scanned offsets.
+__global__ void __launch_bounds__(256) k_compact_cigars_dev(const ExtRequest* __restrict__ req,
+                                                           const ExtResult* __restrict__ res,
+                                                           const uint32_t* __restrict__ slots,
+                                                           const uint64_t* __restrict__ coff,
+                                                           uint32_t* __restrict__ out, const uint32_t* __restrict__ n) {
+    const uint32_t m = *n;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        if (!res[i].found) continue;
+        const uint32_t* s = slots + req[i].cigar_off;
+        uint32_t* d = out + coff[i];
+        for (uint32_t x = 0; x < res[i].n_cigar; ++x) d[x] = s[x];
+    }
 }
 
 }  // namespace sa

@@ -236,6 +236,7 @@ private:
 struct CachedResult {
     ExtResult r;
     uint64_t cigar_off;  // into Chunk::cigar_pool
+    uint32_t spec = 0;   // from the fused pass: 1 not looked up yet, 2 looked up
 };
 
 // Chunk output buffers kept for reuse: a fresh multi-megabyte string per chunk
@@ -295,6 +296,7 @@ struct Chunk {
     std::vector<uint32_t> cigar_pool;  // CIGARs of the cached results (grows only between passes)
     std::vector<Key> pending;
     KeyTable<char> pending_set;
+    uint64_t spec_used = 0;  // fused-pass results the decisions looked up
     bool done() const { return next >= count; }
 };
 
@@ -318,7 +320,13 @@ struct Ctx {
 // reported as no alignment; the record then runs on as a probe that collects
 // its other requests, and is not committed in this pass.
 const CachedResult* lookup(Ctx& c, const Key& key) {
-    if (CachedResult* hit = c.ch->cache.find(key)) return hit;
+    if (CachedResult* hit = c.ch->cache.find(key)) {
+        if (hit->spec == 1) {
+            hit->spec = 2;
+            c.ch->spec_used++;
+        }
+        return hit;
+    }
     if (c.ch->pending_set.emplace(key, 1)) c.ch->pending.push_back(key);
     c.missed = true;
     return nullptr;
@@ -1458,7 +1466,15 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
     double t1 = now_ms();
     const MapParams mp{o.mcs, o.rescue_distance, o.lookup_lanes};
     NamBatch nb;
-    eng.find_nams(mp, kChain, nb);
+    SpecBatch spec;
+    // the fused pass also computes the extensions the decisions are expected to
+    // need (alignment mode only: the other modes extend nothing)
+    const SpecParams sp{paired ? 1u : 0u, o.use_ssw ? 1u : 0u, (uint32_t)std::min<uint64_t>(o.max_tries, 1u << 20),
+                        o.dropoff, o.spec_per_read};
+    const bool fused = o.fused && o.mode == 0 && eng.map_fused(mp, kChain, sp, o.scores, nb, spec);
+    if (!fused) eng.find_nams(mp, kChain, nb);
+    st.graph_replays += spec.replayed ? 1 : 0;
+    st.fused_overflows += spec.overflow ? 1 : 0;
     double t2 = now_ms();
     st.upload_ms += t1 - t0;
     st.nam_ms += t2 - t1;
@@ -1468,6 +1484,38 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
     for (size_t x = 0; x < nch; ++x) {
         chs[x]->abundances.assign(ref.names.size(), 0.0);
         chs[x]->out = g_out_pool.take(chs[x]->count * (paired ? 1400 : 700));
+        chs[x]->spec_used = 0;
+    }
+    if (fused) {
+        st.fused_batches++;
+        st.spec_requests += spec.req.size();
+        // the fused pass's results into the chunks' caches, under the keys the
+        // decisions look them up by (extend_seed); results that failed are left
+        // for the decisions to request again
+        const uint64_t per = paired ? 2 : 1;
+        parallel_for(o.threads, nch, [&](size_t x) {
+            Chunk& ch = *chs[x];
+            const uint64_t r0 = per * ch.pos, r1 = per * (ch.pos + ch.count);
+            auto by_read = [](const ExtRequest& q, uint64_t r) { return q.read_id < r; };
+            const ExtRequest* end = spec.req.end();
+            const ExtRequest* lo = std::lower_bound((const ExtRequest*)spec.req.begin(), end, r0, by_read);
+            const ExtRequest* hi = std::lower_bound(lo, end, r1, by_read);
+            ch.cache.reserve(ch.cache.size() + (size_t)(hi - lo));
+            for (const ExtRequest* q = lo; q != hi; ++q) {
+                const uint64_t y = (uint64_t)(q - spec.req.begin());
+                if (spec.res[y].status != 0) continue;
+                Key key{EXT_SEED, q->read_id, q->flags, q->ref_id, q->ref_start, q->ref_end, q->query_start,
+                        q->query_end, 0};
+                if (!(q->flags & 1)) key.e = q->anchor_off | (uint64_t)q->n_anchors << 48;
+                CachedResult cr;
+                cr.r = spec.res[y];
+                cr.cigar_off = ch.cigar_pool.size();
+                cr.spec = 1;
+                ch.cigar_pool.insert(ch.cigar_pool.end(), spec.cigars.begin() + spec.cigar_off[y],
+                                     spec.cigars.begin() + spec.cigar_off[y + 1]);
+                ch.cache.emplace(key, cr);
+            }
+        });
     }
     for (int round = 0;; ++round) {
         double h0 = now_ms();
@@ -1537,6 +1585,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         });
         st.host_ms += now_ms() - g1;
     }
+    for (size_t x = 0; x < nch; ++x) st.spec_used += chs[x]->spec_used;
     // the batch's caches are no longer needed; output text stays until it is joined
     parallel_for(o.threads, nch, [&](size_t x) {
         Chunk& ch = *chs[x];
@@ -1568,6 +1617,11 @@ void add_stats(RunStats& to, const RunStats& b) {
     to.requests += b.requests;
     to.rounds += b.rounds;
     to.nams += b.nams;
+    to.fused_batches += b.fused_batches;
+    to.spec_requests += b.spec_requests;
+    to.spec_used += b.spec_used;
+    to.graph_replays += b.graph_replays;
+    to.fused_overflows += b.fused_overflows;
 }
 
 }  // namespace

@@ -72,6 +72,21 @@ struct ChainParams {
     float matches_weight, max_diagonal_ratio;
 };
 
+// Which extensions the fused batch pass computes ahead of the read-level
+// decisions (speculatively: the decisions take what they need and request the
+// rest). Single-end: extend_seed of each consistent NAM among the first
+// max_tries (all NAMs tied at the top score when there are more of them) down
+// to the score dropoff (src/mapper.rs:387-406). Paired-end: extend_seed of each
+// mate's top-scoring NAMs, as the proper-pair and individual paths of
+// src/mapper.rs:808-866 call it. At most max_per_read per read.
+struct SpecParams {
+    uint32_t paired;
+    uint32_t use_ssw;  // single-end: SSW instead of piecewise extension (--ssw)
+    uint32_t max_tries;
+    float dropoff;
+    uint32_t max_per_read;
+};
+
 struct MapParams {
     uint32_t mcs;  // 0 Always, 1 Rescue, 2 Off, 3 FirstStrobe (src/mcsstrategy.rs:4-19)
     uint32_t rescue_distance;

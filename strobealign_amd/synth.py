@@ -251,6 +251,24 @@ def make_human(out_dir, n_pairs=1_000_000):
     return {"ref": ref, "reads1": r1, "reads2": r2}
 
 
+def make_human_small(out_dir, n_pairs=2000, total_len=50_000_000):
+    """The human-scale generator's shape on a smaller reference (24 contigs of
+    total_len / 24, 45% repeats of 300 bp-50 kbp at 0-15% divergence; 2x250
+    pairs at 3% substitutions, 0.3% indels, fragments Normal(550, 50)), for
+    tests that need its extension-heavy reads without the 3.1 Gbp reference."""
+    os.makedirs(out_dir, exist_ok=True)
+    ref = os.path.join(out_dir, "human_small.fa")
+    r1, r2 = os.path.join(out_dir, "hs_%d_1.fq" % n_pairs), os.path.join(out_dir, "hs_%d_2.fq" % n_pairs)
+    if os.path.exists(ref) and os.path.exists(r1) and os.path.exists(r2):
+        return {"ref": ref, "reads1": r1, "reads2": r2}
+    rng = np.random.default_rng(11)
+    contigs, _ = random_reference(rng, 24, total_len // 24, repeat_fraction=0.45, seg_min=300, seg_max=50_000,
+                                  divergence=0.0, max_divergence=0.15)
+    write_fasta(ref, ["chr%d" % (i + 1) for i in range(24)], contigs)
+    paired_end(np.random.default_rng(110), contigs, n_pairs, 250, 0.03, 0.003, r1, r2, mean=550, sd=50, prefix="h")
+    return {"ref": ref, "reads1": r1, "reads2": r2}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("workload", choices=["small", "pe100m", "se_var", "human"])
