@@ -321,7 +321,7 @@ def schedule_note(trials):
     n_lanes = sum(1 for t in trials if "lookup_lanes" in t)
     n_sched = len(trials) - n_lanes
     lanes = "lookup stage form over %d lane counts, then " % n_lanes if n_lanes else ""
-    return "autotuned: %s(engines, batch) over %d pairs" % (lanes, n_sched)
+    return "autotuned: %s(pass, engines, batch) over %d triples" % (lanes, n_sched)
 
 
 def pct(v, p):
@@ -415,6 +415,26 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
     return roof, lookup, extension
 
 
+def other_pass(ix, sa, reads, opts, threads, steps, n_units):
+    """Times the batch pass the line did not use, on the same schedule (engines,
+    batch size, lookup lanes): (units/s, the last step's stats)."""
+    o = sa.options(threads=threads, slots=opts.slots, batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes,
+                   fused=not opts.fused)
+    ix.map_reads(reads, o, keep_output=False)  # sizes (and records) the pass's buffers
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ix.map_reads(reads, o, keep_output=False)
+    return n_units * steps / (time.perf_counter() - t0), st
+
+
+def pass_note(st, steps_stats=None):
+    """What the fused pass did in a step (its batches, graph replays, speculative
+    extensions made and used by the decisions)."""
+    return {"fused_batches": st.fused_batches, "graph_replays": st.graph_replays,
+            "speculative_extensions": st.spec_requests, "speculative_used": st.spec_used,
+            "anchor_overflows": st.fused_overflows}
+
+
 def fastq_mean_length(path, n=500):
     """Mean length of the first n records (the reference's read-length estimate,
     src/main.rs:1001-1012)."""
@@ -472,12 +492,15 @@ def bench_workload(a, sa, synth, cores, threads):
         print("autotune: %d lookup lanes, %d engines, %d Mbases per batch (%d trials, %.1f s)" %
               (opts.lookup_lanes, opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0),
               file=sys.stderr, flush=True)
+    if a.pass_ != "auto":
+        opts.fused = int(a.pass_ == "fused")
     runs = [("sam", 0, opts.slots), ("paf", 1, opts.slots)] if human else [("sam", 0, opts.slots)]
     if not human and a.single_steps > 0:
         runs.append(("plain", 0, 1))
     res = {}
     for name, mode, slots in runs:
-        o = sa.options(mode=mode, threads=threads, slots=slots, batch_bases=a.batch_bases if slots == 1 else opts.batch_bases)
+        o = sa.options(mode=mode, threads=threads, slots=slots, batch_bases=a.batch_bases if slots == 1 else opts.batch_bases,
+                       lookup_lanes=opts.lookup_lanes, fused=bool(opts.fused) and name == "sam")
         steps = a.steps if name != "plain" else a.single_steps
         for i in range(a.warmup if name != "plain" else 1):
             _, st = ix.map_reads(reads, o, keep_output=False)
@@ -494,6 +517,10 @@ def bench_workload(a, sa, synth, cores, threads):
         kd = {k: tuple(k1[k][x] - k0.get(k, z)[x] for x in range(5)) for k in k1}
         res[name] = (n_units * steps / wall, wall, st, kd, steps)
     value, wall, st, kd, _ = res["sam"]
+    other, other_st = other_pass(ix, sa, reads, opts, threads, max(1, min(a.steps, 3)), n_units)
+    fused_rate, plain_rate = (value, other) if opts.fused else (other, value)
+    print("fused pass %.0f, plain pass %.0f %s (same schedule)" % (fused_rate, plain_rate, unit), file=sys.stderr,
+          flush=True)
     if "plain" in res:  # per-kernel figures from the one-engine pass (kernels alone on the device)
         _, _, _, kdp, sp = res["plain"]
         roof, lookup, extension = kernel_report(ctx, kdp, sp, "the one-engine pass (%d steps)" % sp,
@@ -517,10 +544,13 @@ def bench_workload(a, sa, synth, cores, threads):
                    "cpu_model": cpu_model(), "nproc": os.cpu_count(),
                    "single_thread_value": round(t["t1_records"] / t["t1_s"], 1) if t.get("t1_s") else None,
                    "stage_seconds_summed_over_threads": t.get("stages_s")}
-            mine = ix.map_files(s1, s2, threads=threads)
+            mine = ix.map_files(s1, s2, threads=threads, fused=bool(opts.fused), slots=opts.slots,
+                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes)
             with open(osam, "rb") as f:
                 theirs = f.read()
-            parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs)}
+            parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs),
+                      "compared": "library SAM (the measured pass and schedule) vs oracle SAM (--no-PG) of the "
+                                  "first %d records" % sample}
             print("oracle sample: %.0f %s; SAM identical: %s" % (v, unit, parity["identical_sam"]), file=sys.stderr,
                   flush=True)
         except Exception as e:  # reported, never fatal
@@ -539,7 +569,11 @@ def bench_workload(a, sa, synth, cores, threads):
               "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
               "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
               "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-              "lookup_lanes": opts.lookup_lanes or 16, "schedule": schedule_note(trials)}
+              "lookup_lanes": opts.lookup_lanes or 16, "schedule": schedule_note(trials),
+              "pass": "fused" if opts.fused else "plain",
+              "fused_pass_per_s": round(fused_rate, 1), "plain_pass_per_s": round(plain_rate, 1),
+              "fused_over_plain_pass": round(fused_rate / plain_rate, 3),
+              "fused_pass_step": pass_note(st if opts.fused else other_st)}
     if human:
         config["paf_pairs_per_s"] = round(res["paf"][0], 1)
     if "plain" in res:
@@ -591,6 +625,9 @@ def main():
                     help="human/se_var: build the index on the host instead of the device")
     ap.add_argument("--slots", type=int, default=0, help="device batches in flight (0: library default)")
     ap.add_argument("--batch-bases", type=int, default=0, help="bases per device batch (0: library default)")
+    ap.add_argument("--pass", dest="pass_", choices=["auto", "fused", "plain"], default="auto",
+                    help="batch pass: fused (seeding to the expected extensions as one replayed graph per batch), "
+                         "plain (per-batch stages, extensions requested by the decisions), or auto (autotuner)")
     ap.add_argument("--lookup-lanes", type=int, default=0, choices=[0, 1, 8, 16, 64],
                     help="lanes per read orientation in the lookup stage (0: autotuned, else library default)")
     ap.add_argument("--single-steps", type=int, default=2, help="timed steps of the one-engine comparison (0: skip)")
@@ -659,9 +696,11 @@ def main():
         # (sa_autotune; outputs are identical for every schedule)
         t0 = time.perf_counter()
         trials = ix.autotune(reads, opts, sample_pairs=a.tune_pairs)
-        print("autotune: %d lookup lanes, %d engines, %d Mbases per batch (%d trials, %.1f s)" %
-              (opts.lookup_lanes, opts.slots, opts.batch_bases >> 20, len(trials), time.perf_counter() - t0),
-              file=sys.stderr, flush=True)
+        print("autotune: %s pass, %d lookup lanes, %d engines, %d Mbases per batch (%d trials, %.1f s)" %
+              ("fused" if opts.fused else "plain", opts.lookup_lanes, opts.slots, opts.batch_bases >> 20,
+               len(trials), time.perf_counter() - t0), file=sys.stderr, flush=True)
+    if a.pass_ != "auto":
+        opts.fused = int(a.pass_ == "fused")
     print("index %.1fs, %d pairs loaded, staged on the device in %.1f ms" % (index_s, len(reads), 1000 * upload_s),
           file=sys.stderr, flush=True)
     for i in range(a.warmup):
@@ -679,6 +718,10 @@ def main():
     k1 = ix.kernel_times()
     n_pairs = len(reads)
     value = n_pairs * a.steps / wall
+    other, other_st = other_pass(ix, sa, reads, opts, threads, max(1, min(a.steps, 3)), n_pairs)
+    fused_rate, plain_rate = (value, other) if opts.fused else (other, value)
+    print("fused pass %.0f, plain pass %.0f pairs/s (same schedule)" % (fused_rate, plain_rate), file=sys.stderr,
+          flush=True)
     # the plain per-batch drop-in beside it: one engine, one batch at a time (as a
     # caller handing batches to sa_map_reads one after another would run it)
     single = None
@@ -747,11 +790,13 @@ def main():
             v, t, (s1, s2) = cpu_baseline(w["ref"], w["reads1"], w["reads2"], sample, cores, a.data_dir, out=osam)
             # the headline claims bit-exact SAM: the library maps the same sample
             # records and its SAM is compared byte for byte with the oracle's
-            mine = ix.map_files(s1, s2, threads=threads)
+            mine = ix.map_files(s1, s2, threads=threads, fused=bool(opts.fused), slots=opts.slots,
+                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes)
             with open(osam, "rb") as f:
                 theirs = f.read()
             parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs),
-                      "compared": "library SAM vs oracle SAM (--no-PG) of the first %d pairs" % sample}
+                      "compared": "library SAM (the measured pass and schedule) vs oracle SAM (--no-PG) of the "
+                                  "first %d pairs" % sample}
             print("oracle sample: %.0f pairs/s; SAM identical: %s" % (v, parity["identical_sam"]), file=sys.stderr,
                   flush=True)
             cpu = {"value": round(v, 1), "unit": "read pairs/s", "cores": cores, "kind": "port",
@@ -787,6 +832,10 @@ def main():
                    "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
                    "lookup_lanes": opts.lookup_lanes or 16,
                    "schedule": schedule_note(trials),
+                   "pass": "fused" if opts.fused else "plain",
+                   "fused_pass_pairs_per_s": round(fused_rate, 1), "plain_pass_pairs_per_s": round(plain_rate, 1),
+                   "fused_over_plain_pass": round(fused_rate / plain_rate, 3),
+                   "fused_pass_step": pass_note(stats[-1] if opts.fused else other_st),
                    "plain_one_engine_pairs_per_s": round(single, 1) if single else None,
                    "batched_over_plain": round(value / single, 3) if single else None},
         # request latency of the serving configuration as specified (one read length

@@ -649,16 +649,27 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
                     o.lookup_lanes = lanes;
                     o.fused = fz == 1;
                     double ms = 0;
-                    // the first run grows the buffers (and sizes and records the fused pass)
-                    for (int rep_i = 0; rep_i < 2; ++rep_i) {
-                        sa::RunStats st;
-                        sa::map_records(engines(index, sl), index->ref, r1, r2, o, nullptr, &st, pdr);
-                        ms = st.total_ms;
+                    // the first run grows the buffers (and sizes and records the fused pass);
+                    // a schedule the device memory cannot hold is skipped
+                    try {
+                        for (int rep_i = 0; rep_i < 2; ++rep_i) {
+                            sa::RunStats st;
+                            sa::map_records(engines(index, sl), index->ref, r1, r2, o, nullptr, &st, pdr);
+                            ms = st.total_ms;
+                        }
+                    } catch (const std::runtime_error& e) {
+                        if (std::string(e.what()).find("out of memory") == std::string::npos) throw;
+                        ms = 1e300;
                     }
                     char tmp[160];
-                    std::snprintf(tmp, sizeof tmp,
-                                  "%s{\"fused\": %d, \"slots\": %d, \"batch_bases\": %llu, \"ms\": %.2f}",
-                                  rep.size() > 1 ? ", " : "", fz, sl, (unsigned long long)bb, ms);
+                    if (ms < 1e300)
+                        std::snprintf(tmp, sizeof tmp,
+                                      "%s{\"fused\": %d, \"slots\": %d, \"batch_bases\": %llu, \"ms\": %.2f}",
+                                      rep.size() > 1 ? ", " : "", fz, sl, (unsigned long long)bb, ms);
+                    else
+                        std::snprintf(tmp, sizeof tmp,
+                                      "%s{\"fused\": %d, \"slots\": %d, \"batch_bases\": %llu, \"ms\": null}",
+                                      rep.size() > 1 ? ", " : "", fz, sl, (unsigned long long)bb);
                     rep += tmp;
                     if (ms < best) {
                         best = ms;

@@ -74,7 +74,10 @@ inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) /
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
-    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_PWROWS, S_PWDIRS, S_TIERS, S_COUNT
+    S_NCIG, S_CIGOFF, S_CIGOUT, S_SCRATCH, S_JOBS, S_LJOBS, S_PJOBS, S_ENDS, S_FWD, S_LISTS, S_BCNT, S_XCNT, S_CBYTES, S_HBYTES, S_FINLISTS, S_PWROWS, S_PWDIRS, S_TIERS,
+    // used by the fused pass only (it shares the others with the plain stages)
+    F_SEQ, F_OFF, F_CTL, F_SUM0, F_SUM1, F_SUM2, F_SUM3, F_SUM4, F_SPECN, F_SPECS, F_ROFF, F_SOFF, F_LSCR, F_PSCR,
+    S_COUNT
 };
 }  // namespace
 
@@ -212,11 +215,6 @@ void device_free(int device, void* p) {
 
 // ------------------------------------------------------------ fused pass ---
 namespace {
-// Device buffers of the fused pass beyond the plain pass's slots.
-enum FusedSlot {
-    F_SEQ = S_COUNT, F_OFF, F_CTL, F_SUM0, F_SUM1, F_SUM2, F_SUM3, F_SUM4, F_SPECN, F_SPECS, F_ROFF, F_SOFF, F_LSCR,
-    F_PSCR, F_COUNT
-};
 // Control words of the fused pass (u64 each, zeroed at its start, copied to the
 // host at its end): anchor overflow flag (u32), speculative requests written
 // (u32), NAMs, anchor capacity, requests and CIGAR slots the selection wanted,
@@ -247,28 +245,16 @@ struct Engine::Fused {
     int runs = 0;       // passes with this shape (the first runs directly)
     // sizing from what the passes saw
     double req_per_read = 0, slots_per_req = 0;
-    // own device buffers by slot (they only grow, and only outside a recording)
-    std::vector<void*> buf = std::vector<void*>(F_COUNT, nullptr);
-    std::vector<uint64_t> cap = std::vector<uint64_t>(F_COUNT, 0);
-    bool recording = false;
+    // The pass works in the engine's buffers (the slots it shares with the
+    // plain stages are sized for whichever needs more). The recorded graph
+    // holds their addresses, so it is valid while the engine's allocation
+    // generation is the one it was recorded at.
+    uint64_t gen = ~0ull;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t* h_ctl = nullptr;  // pinned: C_COUNT control words, then the EXT_NCNT extension counters
     int device = 0;
 
-    void* take(int slot, uint64_t bytes) {
-        bytes = std::max<uint64_t>(bytes, 16);
-        if (bytes > cap[slot]) {
-            if (recording) throw std::logic_error("fused pass: a buffer would grow while the pass is recorded");
-            if (buf[slot]) SA_CHECK(hipFree(buf[slot]));
-            buf[slot] = nullptr;
-            cap[slot] = 0;
-            const uint64_t want = round_up(bytes + bytes / 8, 1 << 16);
-            SA_CHECK(hipMalloc(&buf[slot], want));
-            cap[slot] = want;
-        }
-        return buf[slot];
-    }
     void drop_graph() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -278,8 +264,6 @@ struct Engine::Fused {
     ~Fused() {
         (void)hipSetDevice(device);
         drop_graph();
-        for (void* p : buf)
-            if (p) (void)hipFree(p);
         if (h_ctl) (void)hipHostFree(h_ctl);
     }
 };
@@ -315,6 +299,7 @@ Engine::Engine(ShareIndex, const Engine& owner) : Engine(owner.device_) {
 
 Engine::~Engine() {
     (void)hipSetDevice(device_);  // destructor: errors are not reported
+    drain();  // no kernel of this engine may outlive its buffers
     delete fused_;
     for (void* p : buf_)
         if (p) (void)hipFree(p);
@@ -370,12 +355,25 @@ void Engine::add_kernel(const char* name, double ms, uint64_t items, uint64_t by
 void* Engine::grow(int slot, uint64_t bytes) {
     if (bytes == 0) bytes = 16;
     if (cap_[slot] < bytes) {
-        if (buf_[slot]) SA_CHECK(hipFree(buf_[slot]));
+        if (recording_) throw std::logic_error("a device buffer would grow while the fused pass is recorded");
+        ++alloc_gen_;
+        void* old = buf_[slot];
+        buf_[slot] = nullptr;
+        cap_[slot] = 0;
+        if (old) SA_CHECK(hipFree(old));
         uint64_t want = bytes + bytes / 4;
-        SA_CHECK(hipMalloc(&buf_[slot], want));
+        void* p = nullptr;
+        SA_CHECK(hipMalloc(&p, want));
+        buf_[slot] = p;
         cap_[slot] = want;
     }
     return buf_[slot];
+}
+
+void Engine::drain() noexcept {
+    (void)hipSetDevice(device_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    (void)hipGetLastError();
 }
 
 void* Engine::host_stage(int slot, uint64_t bytes) {
@@ -1400,11 +1398,11 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
 // directly or recorded. Nothing in it waits for the device or reads a count
 // on the host.
 void Engine::fused_sequence(Fused& f) {
-    auto take = [&](int slot, uint64_t bytes) { return f.take(slot, bytes); };
+    auto take = [&](int slot, uint64_t bytes) { return grow(slot, bytes); };
     hipStream_t st = stream_;
     const uint32_t n = f.n_cap;
-    const auto* d_seq = (const uint8_t*)f.buf[F_SEQ];
-    const auto* d_off = (const uint64_t*)f.buf[F_OFF];
+    const auto* d_seq = (const uint8_t*)buf_[F_SEQ];
+    const auto* d_off = (const uint64_t*)buf_[F_OFF];
     auto* ctl = (uint64_t*)take(F_CTL, C_COUNT * 8);
     auto* flag = (uint32_t*)(ctl + C_FLAG);
     auto* nreq = (uint32_t*)(ctl + C_NREQ);
@@ -1435,7 +1433,7 @@ void Engine::fused_sequence(Fused& f) {
             return f.a_cap;
         },
         [&](uint64_t N) { return (NamRec*)take(S_NAMS, (N + 1) * sizeof(NamRec)); }, nullptr, h, o, flag);
-    const auto* nam_cnt = (const uint64_t*)f.buf[S_TOT];
+    const auto* nam_cnt = (const uint64_t*)buf_[S_TOT];
     // speculative requests
     auto* rq_n = (uint64_t*)take(F_SPECN, (uint64_t)(n + 1) * 8);
     auto* rq_s = (uint64_t*)take(F_SPECS, (uint64_t)(n + 1) * 8);
@@ -1464,7 +1462,7 @@ void Engine::fused_sequence(Fused& f) {
     auto* d_lists = (uint32_t*)take(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
     auto* d_ends = (SswEnds*)take(S_ENDS, (uint64_t)cap * sizeof(SswEnds));
     auto* d_fwd = (SswEnd*)take(S_FWD, (uint64_t)cap * sizeof(SswEnd));
-    const auto* d_chain = (const uint32_t*)f.buf[S_CHAIN];
+    const auto* d_chain = (const uint32_t*)buf_[S_CHAIN];
     k_zero_u32<<<1, 64, 0, st>>>(d_cnt, EXT_NCNT);
     k_ext_prep<<<F_PREP_BLOCKS, 256, 0, st>>>(d_seq, d_off, ref, sc, k, d_req, 0, nreq, d_res, d_slots, d_jobs,
                                                d_ljobs, d_pjobs, d_cnt);
@@ -1588,39 +1586,56 @@ bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecPar
         const double spr = std::max(f.slots_per_req, 2.0 * avg_len + 8.0);
         f.slot_cap = (uint64_t)(spr * 1.25 * (double)f.req_cap) + 4096;
         if (f.req_cap > 0xFFFFFFFFull / 2) throw std::runtime_error("fused pass: too many speculative requests");
+    } else if (f.gen != alloc_gen_) {
+        // a buffer moved (the plain stages grew it): record the pass again; its
+        // buffers are at least as large as when it ran directly
+        f.drop_graph();
     }
     // the batch's reads into the pass's fixed buffers; the reads past the batch are empty
-    auto* fseq = (uint8_t*)f.take(F_SEQ, f.bases_cap + 16);
-    auto* foff = (uint64_t*)f.take(F_OFF, (uint64_t)(f.n_cap + 1) * 8);
+    auto* fseq = (uint8_t*)grow(F_SEQ, f.bases_cap + 16);
+    auto* foff = (uint64_t*)grow(F_OFF, (uint64_t)(f.n_cap + 1) * 8);
     if (n_bases_) SA_CHECK(hipMemcpyAsync(fseq, d_seq_, n_bases_, hipMemcpyDeviceToDevice, stream_));
     SA_CHECK(hipMemcpyAsync(foff, cur_off_, (uint64_t)(n + 1) * 8, hipMemcpyDeviceToDevice, stream_));
     if (f.n_cap > n) k_pad_off<<<blocks(f.n_cap - n, 256), 256, 0, stream_>>>(foff, n, f.n_cap);
     SA_CHECK(hipGetLastError());
     hipEvent_t e0 = event(), e1 = event();
-    SA_CHECK(hipEventRecord(e0, stream_));
-    if (f.runs == 0) {
-        fused_sequence(f);
-    } else {
-        if (!f.exec) {
-            SA_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            f.recording = true;
-            try {
-                fused_sequence(f);
-            } catch (...) {
-                f.recording = false;
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(stream_, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            f.recording = false;
-            SA_CHECK(hipStreamEndCapture(stream_, &f.graph));
-            SA_CHECK(hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+    try {
+        SA_CHECK(hipEventRecord(e0, stream_));
+        if (f.runs == 0) {
+            // run directly: grows the buffers to this shape
+            fused_sequence(f);
+            f.gen = alloc_gen_;
         } else {
-            ++fused_replays_;
-            spec.replayed = true;
+            if (!f.exec) {
+                SA_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                recording_ = true;
+                try {
+                    fused_sequence(f);
+                } catch (...) {
+                    recording_ = false;
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(stream_, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                recording_ = false;
+                SA_CHECK(hipStreamEndCapture(stream_, &f.graph));
+                SA_CHECK(hipGraphInstantiate(&f.exec, f.graph, nullptr, nullptr, 0));
+                f.gen = alloc_gen_;
+            } else {
+                ++fused_replays_;
+                spec.replayed = true;
+            }
+            SA_CHECK(hipGraphLaunch(f.exec, stream_));
         }
-        SA_CHECK(hipGraphLaunch(f.exec, stream_));
+    } catch (...) {
+        // nothing of a pass that failed part-way may still run once this returns
+        drain();
+        f.drop_graph();
+        f.stale = true;
+        release_event(e0);
+        release_event(e1);
+        throw;
     }
     ++f.runs;
     SA_CHECK(hipEventRecord(e1, stream_));
@@ -1657,30 +1672,30 @@ bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecPar
     spec.cigars = HostView<uint32_t>{(uint32_t*)(hs + b_req + b_res + b_off), n_cig};
     spec.wanted = wanted;
     if (n_req) {
-        SA_CHECK(hipMemcpyAsync(spec.req.data(), f.buf[S_REQ], n_req * sizeof(ExtRequest), hipMemcpyDeviceToHost,
+        SA_CHECK(hipMemcpyAsync(spec.req.data(), buf_[S_REQ], n_req * sizeof(ExtRequest), hipMemcpyDeviceToHost,
                                 stream_));
-        SA_CHECK(hipMemcpyAsync(spec.res.data(), f.buf[S_RES], n_req * sizeof(ExtResult), hipMemcpyDeviceToHost,
+        SA_CHECK(hipMemcpyAsync(spec.res.data(), buf_[S_RES], n_req * sizeof(ExtResult), hipMemcpyDeviceToHost,
                                 stream_));
-        SA_CHECK(hipMemcpyAsync(spec.cigar_off.data(), f.buf[S_CIGOFF], n_req * 8, hipMemcpyDeviceToHost, stream_));
+        SA_CHECK(hipMemcpyAsync(spec.cigar_off.data(), buf_[S_CIGOFF], n_req * 8, hipMemcpyDeviceToHost, stream_));
     }
-    if (n_cig) SA_CHECK(hipMemcpyAsync(spec.cigars.data(), f.buf[S_CIGOUT], n_cig * 4, hipMemcpyDeviceToHost, stream_));
+    if (n_cig) SA_CHECK(hipMemcpyAsync(spec.cigars.data(), buf_[S_CIGOUT], n_cig * 4, hipMemcpyDeviceToHost, stream_));
     // the NAMs (finish_nams waits for all of the copies)
     NamPass o;
-    o.qrn = (uint32_t*)f.buf[S_QRN];
-    o.hc = (HitCounts*)f.buf[S_HC];
-    o.namn = (uint32_t*)f.buf[S_NAMN];
-    o.araw = (uint32_t*)f.buf[S_ARAW];
-    o.cbytes = (uint32_t*)f.buf[S_CBYTES];
-    o.hbytes = (uint32_t*)f.buf[S_HBYTES];
-    o.outoff = (uint64_t*)f.buf[S_OUTOFF];
-    o.nams = (NamRec*)f.buf[S_NAMS];
+    o.qrn = (uint32_t*)buf_[S_QRN];
+    o.hc = (HitCounts*)buf_[S_HC];
+    o.namn = (uint32_t*)buf_[S_NAMN];
+    o.araw = (uint32_t*)buf_[S_ARAW];
+    o.cbytes = (uint32_t*)buf_[S_CBYTES];
+    o.hbytes = (uint32_t*)buf_[S_HBYTES];
+    o.outoff = (uint64_t*)buf_[S_OUTOFF];
+    o.nams = (NamRec*)buf_[S_NAMS];
     o.n_nams = n_nams;
     finish_nams(n, o, nullptr, out);
     spec.cigar_off[n_req] = n_cig;
     // the extension calls that follow work on this pass's reads and anchors
     d_seq_ = fseq;
     cur_off_ = foff;
-    cur_chain_ = (const uint32_t*)f.buf[S_CHAIN];
+    cur_chain_ = (const uint32_t*)buf_[S_CHAIN];
     return true;
 }
 

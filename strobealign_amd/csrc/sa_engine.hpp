@@ -182,6 +182,9 @@ public:
     const SeedParams& params() const { return p_; }
     uint64_t n_entries() const { return n_entries_; }
     ihipStream_t* stream() const { return stream_; }
+    // Waits until nothing of this engine runs on the device (errors ignored);
+    // after a call failed part-way, before the engine's buffers are reused or freed.
+    void drain() noexcept;
     // Runs the engine's work on `st` (null: its own stream) from now on; returns
     // the stream it used before. The caller keeps `st` alive while it is set.
     ihipStream_t* set_stream(ihipStream_t* st);
@@ -230,6 +233,8 @@ private:
     double anchors_per_base_ = 0;
     Fused* fused_ = nullptr;
     uint64_t fused_replays_ = 0;
+    uint64_t alloc_gen_ = 0;   // bumped whenever a device buffer is reallocated
+    bool recording_ = false;   // the fused pass is being recorded: no buffer may grow
     bool owns_index_ = true;
     // growable device buffers indexed by slot
     std::vector<void*> buf_;

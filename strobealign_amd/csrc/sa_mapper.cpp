@@ -1668,7 +1668,12 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
     // batches are dealt to the engines; each engine works on its own stream, so one
     // batch's device stages overlap another's host replay
     if (engines.size() == 1 || batches.size() <= 1) {
-        for (size_t b = 0; b < batches.size(); ++b) one(*engines[0], b);
+        try {
+            for (size_t b = 0; b < batches.size(); ++b) one(*engines[0], b);
+        } catch (...) {
+            engines[0]->drain();
+            throw;
+        }
     } else {
         std::atomic<size_t> next{0};
         std::vector<std::thread> th;
@@ -1680,6 +1685,7 @@ void map_records(const std::vector<Engine*>& engines, const PackedRef& ref, cons
                 } catch (...) {
                     err[e] = std::current_exception();
                     next.store(batches.size());
+                    engines[e]->drain();
                 }
             });
         for (auto& t : th) t.join();

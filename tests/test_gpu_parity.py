@@ -253,9 +253,9 @@ def test_resident_reads_and_concurrent_batches(sa, ctx, tmp_path_factory):
 
 
 def test_autotuned_schedule_gives_the_same_sam(sa, ctx, tmp_path_factory):
-    """sa_autotune picks the lookup stage's lane count and an (engines, batch
-    size) schedule from its candidate sets; mapping with them gives the
-    oracle's SAM like every schedule does."""
+    """sa_autotune picks the lookup stage's lane count and a (batch pass,
+    engines, batch size) schedule from its candidate sets; mapping with them
+    gives the oracle's SAM like every schedule does."""
     say("case: autotuned schedule, 100 Mbp PE 3000 pairs")
     sys.path.insert(0, ROOT)
     from strobealign_amd import synth
@@ -271,13 +271,14 @@ def test_autotuned_schedule_gives_the_same_sam(sa, ctx, tmp_path_factory):
         trials = [t for t in trials if "slots" in t]
         assert sorted(t["lookup_lanes"] for t in lanes) == [1, 8, 16, 64] and all(t["lookup_ms"] > 0 for t in lanes)
         assert opts.lookup_lanes == min(lanes, key=lambda t: t["lookup_ms"])["lookup_lanes"]
-        assert len(trials) == 18 and all(t["ms"] > 0 for t in trials)
+        assert len(trials) == 36 and all(t["ms"] > 0 for t in trials)
+        assert sorted({t["fused"] for t in trials}) == [0, 1]
         assert opts.slots in (1, 2, 3, 4, 6, 8) and opts.batch_bases in (16 << 20, 32 << 20, 64 << 20)
         # the chosen schedule is a fastest trial (times are reported to 0.01 ms,
         # so several trials may show the minimum)
         fastest = min(t["ms"] for t in trials)
-        assert any(t["ms"] == fastest and t["slots"] == opts.slots and t["batch_bases"] == opts.batch_bases
-                   for t in trials)
+        assert any(t["ms"] == fastest and t["slots"] == opts.slots and t["batch_bases"] == opts.batch_bases and
+                   t["fused"] == opts.fused for t in trials)
         out, _ = ix.map_reads(reads, opts)
         reads.close()
     assert out == exp
