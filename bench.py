@@ -318,9 +318,11 @@ def schedule_note(trials):
     """How the schedule was chosen: the autotuner's trials, or fixed."""
     if not trials:
         return "fixed"
+    n_seed = sum(1 for t in trials if "seed_lanes" in t)
     n_lanes = sum(1 for t in trials if "lookup_lanes" in t)
-    n_sched = len(trials) - n_lanes
-    lanes = "lookup stage form over %d lane counts, then " % n_lanes if n_lanes else ""
+    n_sched = len(trials) - n_lanes - n_seed
+    lanes = ("seeding form over %d lane counts, " % n_seed if n_seed else "") + \
+            ("lookup stage form over %d lane counts, then " % n_lanes if n_lanes else "")
     return "autotuned: %s(pass, engines, batch) over %d triples" % (lanes, n_sched)
 
 
@@ -419,7 +421,7 @@ def other_pass(ix, sa, reads, opts, threads, steps, n_units):
     """Times the batch pass the line did not use, on the same schedule (engines,
     batch size, lookup lanes): (units/s, the last step's stats)."""
     o = sa.options(threads=threads, slots=opts.slots, batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes,
-                   fused=not opts.fused)
+                   seed_lanes=opts.seed_lanes, fused=not opts.fused)
     ix.map_reads(reads, o, keep_output=False)  # sizes (and records) the pass's buffers
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -484,7 +486,8 @@ def bench_workload(a, sa, synth, cores, threads):
     upload_s = time.perf_counter() - t0
     n_units = len(reads)
     unit = "read pairs/s" if human else "reads/s"
-    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes,
+                      seed_lanes=a.seed_lanes)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
         t0 = time.perf_counter()
@@ -500,7 +503,8 @@ def bench_workload(a, sa, synth, cores, threads):
     res = {}
     for name, mode, slots in runs:
         o = sa.options(mode=mode, threads=threads, slots=slots, batch_bases=a.batch_bases if slots == 1 else opts.batch_bases,
-                       lookup_lanes=opts.lookup_lanes, fused=bool(opts.fused) and name == "sam")
+                       lookup_lanes=opts.lookup_lanes, seed_lanes=opts.seed_lanes,
+                       fused=bool(opts.fused) and name == "sam")
         steps = a.steps if name != "plain" else a.single_steps
         for i in range(a.warmup if name != "plain" else 1):
             _, st = ix.map_reads(reads, o, keep_output=False)
@@ -545,7 +549,8 @@ def bench_workload(a, sa, synth, cores, threads):
                    "single_thread_value": round(t["t1_records"] / t["t1_s"], 1) if t.get("t1_s") else None,
                    "stage_seconds_summed_over_threads": t.get("stages_s")}
             mine = ix.map_files(s1, s2, threads=threads, fused=bool(opts.fused), slots=opts.slots,
-                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes)
+                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes,
+                                seed_lanes=opts.seed_lanes)
             with open(osam, "rb") as f:
                 theirs = f.read()
             parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs),
@@ -569,7 +574,8 @@ def bench_workload(a, sa, synth, cores, threads):
               "data_generation_s": round(gen_s, 1), "read_upload_ms": round(1000 * upload_s, 2),
               "extension_requests_per_step": st.requests, "rounds_per_step": st.rounds,
               "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-              "lookup_lanes": opts.lookup_lanes or 16, "schedule": schedule_note(trials),
+              "lookup_lanes": opts.lookup_lanes or 16, "seed_lanes": opts.seed_lanes or 64,
+              "schedule": schedule_note(trials),
               "pass": "fused" if opts.fused else "plain",
               "fused_pass_per_s": round(fused_rate, 1), "plain_pass_per_s": round(plain_rate, 1),
               "fused_over_plain_pass": round(fused_rate / plain_rate, 3),
@@ -628,6 +634,8 @@ def main():
     ap.add_argument("--pass", dest="pass_", choices=["auto", "fused", "plain"], default="auto",
                     help="batch pass: fused (seeding to the expected extensions as one replayed graph per batch), "
                          "plain (per-batch stages, extensions requested by the decisions), or auto (autotuner)")
+    ap.add_argument("--seed-lanes", type=int, default=0, choices=[0, 1, 64],
+                    help="lanes per read in seeding: 1 (a thread) or 64 (a wave); 0: autotuner or library default")
     ap.add_argument("--lookup-lanes", type=int, default=0, choices=[0, 1, 8, 16, 64],
                     help="lanes per read orientation in the lookup stage (0: autotuned, else library default)")
     ap.add_argument("--single-steps", type=int, default=2, help="timed steps of the one-engine comparison (0: skip)")
@@ -689,7 +697,8 @@ def main():
     print("ingest: host parse %.0f ms + upload %.0f ms; device ingest %.0f ms wall (%.1f ms on the device)" %
           (1000 * host_parse_s, 1000 * upload_s, 1000 * dev_ingest_s, ingest["device_parse_ms"]),
           file=sys.stderr, flush=True)
-    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes)
+    opts = sa.options(threads=threads, slots=a.slots, batch_bases=a.batch_bases, lookup_lanes=a.lookup_lanes,
+                      seed_lanes=a.seed_lanes)
     trials = None
     if a.autotune and not a.slots and not a.batch_bases:
         # schedule (engines x batch size) chosen by timing the workload itself
@@ -726,7 +735,8 @@ def main():
     # caller handing batches to sa_map_reads one after another would run it)
     single = None
     if a.single_steps > 0:
-        o1 = sa.options(threads=threads, slots=1, batch_bases=a.batch_bases)
+        o1 = sa.options(threads=threads, slots=1, batch_bases=a.batch_bases, lookup_lanes=opts.lookup_lanes,
+                        seed_lanes=opts.seed_lanes)
         ix.map_reads(reads, o1, keep_output=False)  # grows the engine's buffers to this schedule
         k1b = ix.kernel_times()
         t1 = time.perf_counter()
@@ -791,7 +801,8 @@ def main():
             # the headline claims bit-exact SAM: the library maps the same sample
             # records and its SAM is compared byte for byte with the oracle's
             mine = ix.map_files(s1, s2, threads=threads, fused=bool(opts.fused), slots=opts.slots,
-                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes)
+                                batch_bases=opts.batch_bases, lookup_lanes=opts.lookup_lanes,
+                                seed_lanes=opts.seed_lanes)
             with open(osam, "rb") as f:
                 theirs = f.read()
             parity = {"records": sample, "identical_sam": mine == theirs, "bytes": len(theirs),
@@ -830,7 +841,7 @@ def main():
                    "ingest": ingest,
                    "rounds_per_step": stats[-1].rounds, "extension_requests_per_step": stats[-1].requests,
                    "engines": opts.slots or 4, "batch_bases": opts.batch_bases or (64 << 20),
-                   "lookup_lanes": opts.lookup_lanes or 16,
+                   "lookup_lanes": opts.lookup_lanes or 16, "seed_lanes": opts.seed_lanes or 64,
                    "schedule": schedule_note(trials),
                    "pass": "fused" if opts.fused else "plain",
                    "fused_pass_pairs_per_s": round(fused_rate, 1), "plain_pass_pairs_per_s": round(plain_rate, 1),

@@ -136,6 +136,11 @@ typedef struct sa_map_options {
      * device; the decisions then request only what it did not foresee. The
      * output is identical either way. */
     int32_t fused;
+    /* Lanes per read in seeding (randstrobes of the query): 1 (a thread scans
+     * the read) or 64 (a wave: a lane per position for the syncmers, then a
+     * lane per syncmer for the strobes; reads above 512 bp take the thread
+     * form); 0: default 64. The randstrobes never depend on it. */
+    int32_t seed_lanes;
 } sa_map_options;
 
 /* Fills `opt` with the reference defaults. */
@@ -200,16 +205,19 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
                  sa_run_stats* stats);
 
 /* Schedule autotuner: maps the first sample_pairs records of `reads` (0: all;
- * resident if uploaded, output dropped). Unless opt->lookup_lanes is set, it
- * first times the lookup stage on one engine with 1, 8, 16 and 64 lanes per
- * read orientation (a thread, lane groups, or a wave per orientation) and keeps
- * the fastest; then it maps the sample with each batch pass (plain and fused;
+ * resident if uploaded, output dropped). Unless opt->seed_lanes is set, it
+ * first times seeding on one engine with 1 and 64 lanes per read (a thread or
+ * a wave per read) and keeps the faster; unless opt->lookup_lanes is set, it
+ * times the lookup stage with 1, 8, 16 and 64 lanes per read orientation (a
+ * thread, lane groups, or a wave per orientation) and keeps the fastest; then
+ * it maps the sample with each batch pass (plain and fused;
  * the fused pass only in alignment mode), engine count in {1, 2, 3, 4, 6, 8}
  * and batch size in {16, 32, 64} Mbases, and writes the fastest choices into
- * opt->lookup_lanes, opt->fused, opt->slots and opt->batch_bases. The outcome
- * never depends on the schedule (the output is identical for every choice);
- * only the speed does. report (may be NULL) receives the trials as JSON
- * [{"lookup_lanes", "lookup_ms"}, ..., {"fused", "slots", "batch_bases", "ms"},
+ * opt->seed_lanes, opt->lookup_lanes, opt->fused, opt->slots and
+ * opt->batch_bases. The outcome never depends on the schedule (the output is
+ * identical for every choice); only the speed does. report (may be NULL)
+ * receives the trials as JSON [{"seed_lanes", "seed_ms"}, ...,
+ * {"lookup_lanes", "lookup_ms"}, ..., {"fused", "slots", "batch_bases", "ms"},
  * ...]. No counterpart in the reference, which has no device schedule. */
 int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, sa_map_options* opt, char* report,
                 size_t report_cap);

@@ -40,7 +40,7 @@ class MapOptions(ctypes.Structure):
                 ("slots", ctypes.c_int32), ("match_score", ctypes.c_int32), ("mismatch_penalty", ctypes.c_int32),
                 ("gap_open_penalty", ctypes.c_int32), ("gap_extend_penalty", ctypes.c_int32),
                 ("end_bonus", ctypes.c_int32), ("xdrop", ctypes.c_int32), ("lookup_lanes", ctypes.c_int32),
-                ("fused", ctypes.c_int32)]
+                ("fused", ctypes.c_int32), ("seed_lanes", ctypes.c_int32)]
 
 
 class Scores(ctypes.Structure):
@@ -145,11 +145,12 @@ class Reads:
 
 def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000, use_ssw=False, max_secondary=0,
             mcs=0, details=False, fastq_comments=False, only_mapped=False, batch_bases=0, slots=0, scores=None,
-            xdrop=None, lookup_lanes=0, fused=False):
+            xdrop=None, lookup_lanes=0, fused=False, seed_lanes=0):
     """Mapping options; scores = (match, mismatch, gap_open, gap_extend, end_bonus)
     as -A/-B/-O/-E/-L (defaults 2, 8, 12, 1, 10); lookup_lanes: lanes per read
     orientation in the lookup stage (0: default); fused: the fused batch pass
-    (one replayed graph per batch) instead of the plain per-batch stages."""
+    (one replayed graph per batch) instead of the plain per-batch stages;
+    seed_lanes: lanes per read in seeding, 1 or 64 (0: default)."""
     o = MapOptions()
     _lib.sa_map_options_default(ctypes.byref(o))
     if scores is not None:
@@ -159,6 +160,7 @@ def options(mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
     o.mode, o.eqx, o.details, o.fastq_comments, o.only_mapped = mode, int(eqx), int(details), int(fastq_comments), int(only_mapped)
     o.use_ssw, o.max_secondary, o.mcs, o.threads, o.chunk_size = int(use_ssw), max_secondary, mcs, threads, chunk_size
     o.batch_bases, o.slots, o.lookup_lanes, o.fused = batch_bases, slots, lookup_lanes, int(fused)
+    o.seed_lanes = seed_lanes
     if rg_id is not None:
         o.rg_id = rg_id.encode()
     o._rg_keep = (ctypes.c_char_p * max(1, len(rg)))(*[x.encode() for x in rg])
@@ -344,11 +346,11 @@ class Index:
 
     def map_files(self, reads1, reads2=None, mode=0, eqx=False, rg_id=None, rg=(), threads=4, chunk_size=1000000,
                   use_ssw=False, max_secondary=0, mcs=0, details=False, fastq_comments=False, only_mapped=False,
-                  batch_bases=0, scores=None, xdrop=None, lookup_lanes=0, fused=False, slots=0):
+                  batch_bases=0, scores=None, xdrop=None, lookup_lanes=0, fused=False, slots=0, seed_lanes=0):
         o = options(mode=mode, eqx=eqx, rg_id=rg_id, rg=rg, threads=threads, chunk_size=chunk_size, use_ssw=use_ssw,
                     max_secondary=max_secondary, mcs=mcs, details=details, fastq_comments=fastq_comments,
                     only_mapped=only_mapped, batch_bases=batch_bases, scores=scores, xdrop=xdrop,
-                    lookup_lanes=lookup_lanes, fused=fused, slots=slots)
+                    lookup_lanes=lookup_lanes, fused=fused, slots=slots, seed_lanes=seed_lanes)
         out = ctypes.c_void_p()
         n = ctypes.c_size_t()
         _check(_lib.sa_map_files(self._p, ctypes.byref(o), reads1.encode(),

@@ -104,6 +104,10 @@ sa::Options options_of(const sa_map_options* o) {
             throw std::invalid_argument("lookup_lanes must be 0, 1, 8, 16 or 64");
         r.lookup_lanes = (uint32_t)o->lookup_lanes;
     }
+    if (o->seed_lanes) {
+        if (o->seed_lanes != 1 && o->seed_lanes != 64) throw std::invalid_argument("seed_lanes must be 0, 1 or 64");
+        r.seed_lanes = (uint32_t)o->seed_lanes;
+    }
     if (o->fused != 0 && o->fused != 1) throw std::invalid_argument("fused must be 0 or 1");
     r.fused = o->fused == 1;
     return r;
@@ -603,14 +607,44 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
             pdr = &dr;
         }
         std::string rep = "[";
-        // the lookup stage's form first (unless the caller fixed it): the sample
-        // on one engine with each lane count, timed by the lookup kernel's events
+        // the stage forms first (unless the caller fixed them), each timed by
+        // its kernel's events on one engine: seeding a thread or a wave per read
+        auto kernel_ms = [&](sa::Engine& e, const char* name) {
+            double ms = 0;
+            for (const auto& k : e.kernel_stats())
+                if (k.first == name) ms += k.second.ms;
+            return ms;
+        };
+        uint32_t seed_lanes = options_of(opt).seed_lanes;
+        if (opt->seed_lanes == 0) {
+            double best_s = 1e300;
+            for (uint32_t ln : {1u, 64u}) {
+                sa::Options o = options_of(opt);
+                o.slots = 1;
+                o.seed_lanes = ln;
+                const std::vector<sa::Engine*> eng = engines(index, 1);
+                const double before = kernel_ms(*eng[0], "k_seed");
+                sa::RunStats st;
+                sa::map_records(eng, index->ref, r1, r2, o, nullptr, &st, pdr);
+                const double ms = kernel_ms(*eng[0], "k_seed") - before;
+                char tmp[128];
+                std::snprintf(tmp, sizeof tmp, "%s{\"seed_lanes\": %u, \"seed_ms\": %.3f}",
+                              rep.size() > 1 ? ", " : "", ln, ms);
+                rep += tmp;
+                if (ms < best_s) {
+                    best_s = ms;
+                    seed_lanes = ln;
+                }
+            }
+        }
+        // then the lookup stage: a thread, lane groups or a wave per read orientation
         uint32_t lanes = options_of(opt).lookup_lanes;
         if (opt->lookup_lanes == 0) {
             double best_k = 1e300;
             for (uint32_t ln : {1u, 8u, 16u, 64u}) {
                 sa::Options o = options_of(opt);
                 o.slots = 1;
+                o.seed_lanes = seed_lanes;
                 o.lookup_lanes = ln;
                 const std::vector<sa::Engine*> eng = engines(index, 1);
                 auto lookup_ms = [&] {
@@ -647,6 +681,7 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
                     o.slots = sl;
                     o.batch_bases = bb;
                     o.lookup_lanes = lanes;
+                    o.seed_lanes = seed_lanes;
                     o.fused = fz == 1;
                     double ms = 0;
                     // the first run grows the buffers (and sizes and records the fused pass);
@@ -682,6 +717,7 @@ int sa_autotune(sa_index* index, const sa_reads* reads, uint64_t sample_pairs, s
         opt->slots = best_slots;
         opt->batch_bases = best_batch;
         opt->lookup_lanes = (int32_t)lanes;
+        opt->seed_lanes = (int32_t)seed_lanes;
         opt->fused = best_fused;
         if (report && report_cap) {
             const size_t m = std::min(report_cap - 1, rep.size());
@@ -832,7 +868,7 @@ int sa_map_batch(sa_index* index, const sa_map_options* opt, const uint8_t* seqs
         eng.upload_reads(seqs, offsets, n);
         sa::NamBatch nb;
         sa::ChainParams cp{50, 0.1f, 0.05f, 0.7f, -1, 0.01f, 10.0f};
-        eng.find_nams(sa::MapParams{o.mcs, o.rescue_distance, o.lookup_lanes}, cp, nb);
+        eng.find_nams(sa::MapParams{o.mcs, o.rescue_distance, o.lookup_lanes, o.seed_lanes}, cp, nb);
         const uint64_t N = nb.nams.size();
         sa_nam* pn = (sa_nam*)std::malloc(std::max<uint64_t>(1, N) * sizeof(sa_nam));
         sa_nam_extension* pe = (sa_nam_extension*)std::calloc(std::max<uint64_t>(1, N), sizeof(sa_nam_extension));

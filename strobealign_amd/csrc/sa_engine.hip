@@ -71,6 +71,20 @@ __global__ void k_compact_cigars(const ExtRequest* __restrict__ req, const ExtRe
 
 inline uint32_t blocks(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
+// Seeding of n reads: a wave per read up to SEED_WAVE_MAX_LEN bases (lanes 64)
+// and a thread per read beyond, or a thread per read throughout (lanes 1).
+void launch_seed(const uint8_t* d_seq, const uint64_t* d_off, uint32_t n, const SeedParams& p, uint32_t lanes,
+                 hipStream_t st, Syncmer* d_sy, QRand* d_qr, uint32_t* d_qrn) {
+    if (lanes != 1 && lanes != 64) throw std::invalid_argument("seed lanes must be 1 or 64");
+    if (lanes == 64) {
+        k_seed_wave<<<blocks(n, 4), 256, 0, st>>>(d_seq, d_off, n, p, d_qr, d_qrn);
+        SA_CHECK(hipGetLastError());
+    }
+    k_seed<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, p, d_sy, d_qr, d_qrn,
+                                           lanes == 64 ? SEED_WAVE_MAX_LEN : 0);
+    SA_CHECK(hipGetLastError());
+}
+
 enum Slot {
     S_SEQ, S_OFF, S_SY, S_QR, S_QRN, S_HITS, S_CNT, S_HC, S_ACAP, S_AOFF, S_ANCH, S_ANCH2, S_DP, S_PRED, S_CAND, S_USED,
     S_NAMS_IN, S_CHAIN, S_NAMN, S_ARAW, S_TOT, S_OUTOFF, S_NAMS, S_SCANTMP, S_SCANTOT, S_REQ, S_RES, S_CIGSLOT,
@@ -684,8 +698,7 @@ void Engine::randstrobes(std::vector<QRand>& qr, std::vector<uint32_t>& counts) 
     auto* d_sy = (Syncmer*)grow(S_SY, (p_.w_max < 24 ? 1 : n_bases_ + 1) * sizeof(Syncmer));
     auto* d_qr = (QRand*)grow(S_QR, (2 * n_bases_ + 1) * sizeof(QRand));
     auto* d_qrn = (uint32_t*)grow(S_QRN, (2ull * n + 1) * 4);
-    if (n) k_seed<<<blocks(n, 256), 256, 0, stream_>>>(d_seq, d_off, n, p_, d_sy, d_qr, d_qrn);
-    SA_CHECK(hipGetLastError());
+    if (n) launch_seed(d_seq, d_off, n, p_, 64, stream_, d_sy, d_qr, d_qrn);
     qr.resize(2 * n_bases_);
     counts.resize(2ull * n);
     if (n_bases_) SA_CHECK(hipMemcpyAsync(qr.data(), d_qr, 2 * n_bases_ * sizeof(QRand), hipMemcpyDeviceToHost, stream_));
@@ -741,8 +754,7 @@ void hit_pass(const IndexView& ix, const uint8_t* d_seq, const uint64_t* d_off, 
     h.acap = (uint64_t*)take(S_ACAP, (2ull * n + 1) * 8);
     o.hbytes = (uint32_t*)take(S_HBYTES, (2ull * n + 1) * 4);
     if (ev) SA_CHECK(hipEventRecord(ev[0], st));
-    k_seed<<<blocks(n, 256), 256, 0, st>>>(d_seq, d_off, n, p, d_sy, d_qr, o.qrn);
-    SA_CHECK(hipGetLastError());
+    launch_seed(d_seq, d_off, n, p, mp.seed_lanes, st, d_sy, d_qr, o.qrn);
     if (ev) SA_CHECK(hipEventRecord(ev[1], st));
     switch (mp.lookup_lanes) {
     case 1:

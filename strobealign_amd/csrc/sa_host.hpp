@@ -94,6 +94,7 @@ struct Options {
     uint64_t batch_bases = 64ull << 20;  // bases per device batch (about 12 GB of device buffers per engine)
     int slots = 4;                        // engines working on batches concurrently
     uint32_t lookup_lanes = 16;           // lanes per read orientation in the lookup stage (1, 8, 16, 64)
+    uint32_t seed_lanes = 64;             // lanes per read in seeding (1, 64)
     bool fused = false;                   // fused batch pass (Engine::map_fused) instead of find_nams
     uint32_t spec_per_read = 8;           // most speculative extensions per read in the fused pass
     // -A/-B/-O/-E/-L and --xdrop (src/main.rs:246-272; defaults src/aligner.rs:20-30)

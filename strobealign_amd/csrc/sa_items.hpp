@@ -7,12 +7,16 @@
 namespace sa {
 
 // --------------------------------------------------------------- seeding ---
+// With skip_len > 0, reads of at most skip_len bases are left to the wave
+// form (k_seed_wave); with 0, every read is seeded here, empty ones included
+// (their counts must be written: the next stage reads them).
 SA_HD void item_seed(uint32_t i, const uint8_t* seq, const uint64_t* off,
                                              uint32_t n, SeedParams p, Syncmer* sy,
-                                             QRand* qr, uint32_t* qr_n) {
+                                             QRand* qr, uint32_t* qr_n, uint32_t skip_len) {
     if (i >= n) return;
     uint64_t o = off[i];
     uint32_t L = (uint32_t)(off[i + 1] - o);
+    if (skip_len > 0 && L <= skip_len) return;
     if (p.w_max < 24)
         read_randstrobes_stream<24>(seq + o, L, p, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1]);
     else

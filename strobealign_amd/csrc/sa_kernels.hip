@@ -8,14 +8,33 @@
 #include "sa_hits_group.hpp"
 #include "sa_pw_group.hpp"
 #include "sa_items.hpp"
+#include "sa_seed_wave.hpp"
 #include "sa_ssw_group.hpp"
 
 namespace sa {
 
+// Seeding a thread per read (reads of at most skip_len bases are skipped: the
+// wave form seeds them).
 __global__ void __launch_bounds__(256) k_seed(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
                                              uint32_t n, SeedParams p, Syncmer* __restrict__ sy,
-                                             QRand* __restrict__ qr, uint32_t* __restrict__ qr_n) {
-    item_seed(blockIdx.x * blockDim.x + threadIdx.x, seq, off, n, p, sy, qr, qr_n);
+                                             QRand* __restrict__ qr, uint32_t* __restrict__ qr_n,
+                                             uint32_t skip_len) {
+    item_seed(blockIdx.x * blockDim.x + threadIdx.x, seq, off, n, p, sy, qr, qr_n, skip_len);
+}
+
+// Seeding a wave per read (sa_seed_wave.hpp) for reads of at most
+// SEED_WAVE_MAX_LEN bases; longer ones are left to k_seed.
+__global__ void __launch_bounds__(256) k_seed_wave(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
+                                                  uint32_t n, SeedParams p, QRand* __restrict__ qr,
+                                                  uint32_t* __restrict__ qr_n) {
+    __shared__ SeedWaveLds lds[4];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 4 + w;
+    if (i >= n) return;
+    const uint64_t o = off[i];
+    const uint32_t L = (uint32_t)(off[i + 1] - o);
+    if (L > SEED_WAVE_MAX_LEN) return;
+    read_randstrobes_wave(seq + o, L, p, qr + 2 * o, qr + 2 * o + L, &qr_n[2 * i], &qr_n[2 * i + 1], lds[w], lane);
 }
 
 __global__ void __launch_bounds__(256) k_hits(IndexView ix, MapParams mp, const uint64_t* __restrict__ off,

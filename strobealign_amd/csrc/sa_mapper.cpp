@@ -1464,7 +1464,7 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         eng.upload_reads(seq.data(), off.data(), n_reads);
     }
     double t1 = now_ms();
-    const MapParams mp{o.mcs, o.rescue_distance, o.lookup_lanes};
+    const MapParams mp{o.mcs, o.rescue_distance, o.lookup_lanes, o.seed_lanes};
     NamBatch nb;
     SpecBatch spec;
     // the fused pass also computes the extensions the decisions are expected to

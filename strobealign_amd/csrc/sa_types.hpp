@@ -94,6 +94,9 @@ struct MapParams {
     // orientation's lookups in turn) or 8, 16, 64 (a lane per randstrobe); the
     // hits are the same either way
     uint32_t lookup_lanes = 16;
+    // lanes per read in seeding: 1 (a thread scans the read) or 64 (a wave, a
+    // lane per position and then per syncmer); the randstrobes are the same
+    uint32_t seed_lanes = 64;
 };
 
 // NAM (src/nam.rs:18-31). Chain anchors are stored separately as

@@ -64,15 +64,16 @@ def check_fused(st, batches_at_least=3):
 
 def test_fused_paired_end_100m(sa, ctx, tmp_path_factory):
     """100 Mbp repetitive reference, 3000 pairs 2x150, 100 kbase batches on one
-    and on three engines."""
+    and on three engines, seeding a wave per read (default) and a thread per
+    read (which then also seeds the empty reads that pad each batch)."""
     w = synth().make("pe100m", str(tmp_path_factory.mktemp("fpe")), 3000)
     exp = oracle("-t", "8", "--no-PG", "--chunk-size=20000", w["ref"], w["reads1"], w["reads2"])
     with sa.Index(ctx, w["ref"], read_length=150, threads=8) as ix:
-        for slots in (1, 3):
-            say("case: fused pass, 100 Mbp PE 3000 pairs, %d engine(s)" % slots)
+        for slots, seed_lanes in ((1, 0), (3, 0), (1, 1), (3, 1)):
+            say("case: fused pass, 100 Mbp PE 3000 pairs, %d engine(s), seed lanes %d" % (slots, seed_lanes))
             out, st = run(sa, ix, w["reads1"], w["reads2"], fused=True, batch_bases=100_000, slots=slots,
-                          chunk_size=20_000)
-            assert out == exp, slots
+                          chunk_size=20_000, seed_lanes=seed_lanes)
+            assert out == exp, (slots, seed_lanes)
             check_fused(st)
             assert st.fused_overflows == 0
 

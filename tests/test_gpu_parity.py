@@ -81,10 +81,15 @@ def test_phix_tags(sa, ctx):
 
 
 def test_randstrobes_match_oracle(phix, orc):
+    """Query randstrobes of reads of 0-1500 bp (some with N): up to 512 bp the
+    wave-per-read seeding, beyond it the thread-per-read form."""
     say("case: randstrobes vs oracle")
     rng = random.Random(11)
     seqs = [b""] + [bytes(rng.choice(b"ACGTN" if rng.random() < 0.1 else b"ACGT") for _ in range(rng.randint(10, 500)))
                     for _ in range(200)] + [b"A" * 300]
+    seqs += [bytes(rng.choice(b"ACGTN" if rng.random() < 0.02 else b"ACGT") for _ in range(L))
+             for L in (511, 512, 513, 700, 1500)]
+    seqs += [b"N" * 40 + bytes(rng.choice(b"ACGT") for _ in range(100)), b"ACGT" * 5, b"acgtn" * 30]
     got = phix.randstrobes(seqs)
     from test_oracle import randstrobes as orc_randstrobes
     for s, g in zip(seqs, got):

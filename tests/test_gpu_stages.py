@@ -134,6 +134,23 @@ def test_lookup_lanes_give_the_same_nams(sa, ctx, workloads, mcs):
         assert sum(len(x) for x in res[16]) > len(seqs) // 2
 
 
+def test_seed_lanes_give_the_same_nams(sa, ctx, workloads):
+    """Seeding a thread per read and a wave per read (a lane per position, then
+    per syncmer) give the same NAM lists, f32 score bits included; empty reads
+    and reads shorter than the strobe window among them (their counts must be
+    written by either form, or the lookups read stale ones)."""
+    for fasta, seqs in workloads:
+        say("case: seed lanes 1/64 on %s" % fasta)
+        seqs = [b""] + seqs[:50] + [b"", b"ACGTACGT", b""] + seqs[50:] + [b""]
+        with sa.Index(ctx, fasta, read_length=150, threads=8) as ix:
+            res = {}
+            for ln in (1, 64, 1):  # the thread form again, over buffers the wave form has filled
+                res[ln] = ix.map_batch(seqs, sa.options(seed_lanes=ln), speculative=0)[0]
+                assert res[ln][0] == [] and res[ln][-1] == []
+        assert res[1] == res[64]
+        assert sum(len(x) for x in res[64]) > len(seqs) // 2
+
+
 def test_lookup_lanes_out_of_range(sa, ctx, workloads):
     fasta, seqs = workloads[0]
     with sa.Index(ctx, fasta, read_length=150) as ix:
