@@ -439,6 +439,17 @@ def pass_note(st, steps_stats=None):
             "anchor_overflows": st.fused_overflows}
 
 
+def replay_note(st):
+    """The host replay of the decisions in a step: records decided per pass
+    (committed or as probes) and thread time per record."""
+    return {"records_committed": st.replay_committed, "records_probed": st.replay_probed,
+            "probes_taken_over": st.replay_accepted,
+            "us_per_committed": round(1000 * st.replay_commit_ms / max(1, st.replay_committed), 3),
+            "us_per_probed": round(1000 * st.replay_probe_ms / max(1, st.replay_probed), 3),
+            "us_writing_per_committed": round(1000 * st.replay_write_ms / max(1, st.replay_committed), 3),
+            "thread_ms": round(st.replay_commit_ms + st.replay_probe_ms, 1)}
+
+
 def fastq_mean_length(path, n=500):
     """Mean length of the first n records (the reference's read-length estimate,
     src/main.rs:1001-1012)."""
@@ -581,7 +592,8 @@ def bench_workload(a, sa, synth, cores, threads):
               "pass": "fused" if opts.fused else "plain",
               "fused_pass_per_s": round(fused_rate, 1), "plain_pass_per_s": round(plain_rate, 1),
               "fused_over_plain_pass": round(fused_rate / plain_rate, 3),
-              "fused_pass_step": pass_note(st if opts.fused else other_st)}
+              "fused_pass_step": pass_note(st if opts.fused else other_st),
+              "host_replay_step": replay_note(st)}
     if human:
         config["paf_pairs_per_s"] = round(res["paf"][0], 1)
     if "plain" in res:
@@ -849,6 +861,7 @@ def main():
                    "fused_pass_pairs_per_s": round(fused_rate, 1), "plain_pass_pairs_per_s": round(plain_rate, 1),
                    "fused_over_plain_pass": round(fused_rate / plain_rate, 3),
                    "fused_pass_step": pass_note(stats[-1] if opts.fused else other_st),
+                   "host_replay_step": replay_note(stats[-1]),
                    "plain_one_engine_pairs_per_s": round(single, 1) if single else None,
                    "batched_over_plain": round(value / single, 3) if single else None},
         # request latency of the serving configuration as specified (one read length

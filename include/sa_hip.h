@@ -197,6 +197,14 @@ typedef struct sa_run_stats {
      * a recorded graph, and the batches whose anchors did not fit its buffers
      * (run by the plain stages instead) */
     uint64_t fused_batches, spec_requests, spec_used, graph_replays, fused_overflows;
+    /* host replay of the read-level decisions: records decided and committed,
+     * records decided as probes (each pass counts), and the thread time spent
+     * on each and on writing the committed records' output text */
+    uint64_t replay_committed, replay_probed;
+    double replay_commit_ms, replay_probe_ms, replay_write_ms;
+    /* probes the next pass took over unchanged (same generator and insert-size
+     * state at their start, so the same decisions) */
+    uint64_t replay_accepted;
 } sa_run_stats;
 
 /* Maps a read set. With out == NULL the output is produced and discarded

@@ -101,7 +101,10 @@ class RunStats(ctypes.Structure):
                 ("extend_ms", ctypes.c_double), ("host_ms", ctypes.c_double), ("records", ctypes.c_uint64),
                 ("requests", ctypes.c_uint64), ("rounds", ctypes.c_uint64), ("nams", ctypes.c_uint64),
                 ("fused_batches", ctypes.c_uint64), ("spec_requests", ctypes.c_uint64), ("spec_used", ctypes.c_uint64),
-                ("graph_replays", ctypes.c_uint64), ("fused_overflows", ctypes.c_uint64)]
+                ("graph_replays", ctypes.c_uint64), ("fused_overflows", ctypes.c_uint64),
+                ("replay_committed", ctypes.c_uint64), ("replay_probed", ctypes.c_uint64),
+                ("replay_commit_ms", ctypes.c_double), ("replay_probe_ms", ctypes.c_double),
+                ("replay_write_ms", ctypes.c_double), ("replay_accepted", ctypes.c_uint64)]
 
 
 class Reads:

@@ -575,6 +575,9 @@ int sa_map_reads(sa_index* index, const sa_map_options* opt, const sa_reads* rea
             stats->fused_batches = st.fused_batches; stats->spec_requests = st.spec_requests;
             stats->spec_used = st.spec_used; stats->graph_replays = st.graph_replays;
             stats->fused_overflows = st.fused_overflows;
+            stats->replay_committed = st.replay_committed; stats->replay_probed = st.replay_probed;
+            stats->replay_commit_ms = st.replay_commit_ms; stats->replay_probe_ms = st.replay_probe_ms;
+            stats->replay_write_ms = st.replay_write_ms; stats->replay_accepted = st.replay_accepted;
         }
         if (out) {
             if (!out_len) return fail(SA_ERR_ARG, "out_len is NULL");

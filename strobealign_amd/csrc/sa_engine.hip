@@ -236,7 +236,7 @@ namespace {
 enum FusedCtl { C_FLAG, C_NREQ, C_NNAMS, C_ANCH, C_WANT, C_WANTS, C_NCIG, C_COUNT };
 // Fixed grids of the launches whose counts exist only on the device.
 constexpr uint32_t F_PREP_BLOCKS = 4096;   // x 4 waves, a request per wave
-constexpr uint32_t F_SSW_BLOCKS = 2048;    // x 256 lanes
+constexpr uint32_t F_SSW_BLOCKS = 1024;    // x 256 lanes
 constexpr uint32_t F_FIN_SLOTS = 32768;    // traceback threads (scratch slots), at most
 constexpr uint32_t F_LONG_SLOTS = 2048;    // one-thread SSW of long queries, at most
 constexpr uint64_t F_FIN_BUDGET = 2ull << 30;   // bytes of traceback scratch, at most
@@ -247,8 +247,9 @@ uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 }  // namespace
 
 struct Engine::Fused {
-    // what the pass was sized and recorded for
-    uint32_t n_cap = 0, len_cap = 0;
+    // what the pass was sized and recorded for (len_min: no read of a batch is
+    // shorter, so the forward SSW passes of fewer stripes are not recorded)
+    uint32_t n_cap = 0, len_cap = 0, len_min = 0;
     uint64_t bases_cap = 0, a_cap = 0, req_cap = 0, slot_cap = 0;
     MapParams mp{};
     ChainParams cp{};
@@ -674,7 +675,12 @@ void Engine::use_reads(const uint8_t* d_seq, const uint64_t* off, uint32_t n_rea
     h_off_.assign(off, off + n_reads + 1);
     n_bases_ = h_off_[n_reads];
     max_len_ = 0;
-    for (uint32_t i = 0; i < n_reads; ++i) max_len_ = std::max<uint32_t>(max_len_, (uint32_t)(off[i + 1] - off[i]));
+    min_len_ = 0;
+    for (uint32_t i = 0; i < n_reads; ++i) {
+        const uint32_t L = (uint32_t)(off[i + 1] - off[i]);
+        max_len_ = std::max<uint32_t>(max_len_, L);
+        if (L && (min_len_ == 0 || L < min_len_)) min_len_ = L;
+    }
     if (max_len_ >= 65536) throw std::runtime_error("reads longer than 65535 bp are not supported");
     d_seq_ = d_seq;
     void* o = grow(S_OFF, (uint64_t)(n_reads + 1) * 8);
@@ -1471,56 +1477,73 @@ void Engine::fused_sequence(Fused& f) {
     auto* d_pjobs = (ExtJob*)take(S_PJOBS, (uint64_t)cap * sizeof(ExtJob));
     auto* d_cnt = (uint32_t*)take(S_XCNT, EXT_NCNT * 4);
     uint32_t* d_bcnt = d_cnt + 4;
-    auto* d_lists = (uint32_t*)take(S_LISTS, (uint64_t)SSW_BUCKETS * cap * 4);
-    auto* d_ends = (SswEnds*)take(S_ENDS, (uint64_t)cap * sizeof(SswEnds));
-    auto* d_fwd = (SswEnd*)take(S_FWD, (uint64_t)cap * sizeof(SswEnd));
+    // SSW buffers only when the requests extend with SSW
+    const bool ssw_jobs = f.sp.paired || f.sp.use_ssw;
+    auto* d_lists = (uint32_t*)take(S_LISTS, ssw_jobs ? (uint64_t)SSW_BUCKETS * cap * 4 : 16);
+    auto* d_ends = (SswEnds*)take(S_ENDS, ssw_jobs ? (uint64_t)cap * sizeof(SswEnds) : 16);
+    auto* d_fwd = (SswEnd*)take(S_FWD, ssw_jobs ? (uint64_t)cap * sizeof(SswEnd) : 16);
     const auto* d_chain = (const uint32_t*)buf_[S_CHAIN];
     k_zero_u32<<<1, 64, 0, st>>>(d_cnt, EXT_NCNT);
     k_ext_prep<<<F_PREP_BLOCKS, 256, 0, st>>>(d_seq, d_off, ref, sc, k, d_req, 0, nreq, d_res, d_slots, d_jobs,
                                                d_ljobs, d_pjobs, d_cnt);
-    k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
-                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
     SA_CHECK(hipGetLastError());
-    // every stripe count a query of this shape can have, byte then word, forward
-    // then reverse (a pass only adds jobs to the buckets of later launches)
-    const uint32_t Lq = std::min<uint32_t>(f.len_cap, SSW_GROUP_MAX_QUERY);
-    const uint32_t sb = (Lq + 15) / 16, sw = (Lq + 7) / 8;
-    for (int word = 0; word < 2; ++word)
-        for (int rev = 0; rev < 2; ++rev)
-            for (uint32_t seg = 1; seg <= (word ? sw : sb); ++seg) {
-                SswLaunchLists ll{};
-                for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
-                    const uint32_t b = rev ? ssw_bucket(word, true, seg, 0)
-                                           : ssw_bucket(word, false, seg, SSW_WCLASSES - 1 - x);
-                    ll.list[x] = d_lists + (uint64_t)b * cap;
+    // the speculative requests are extend_seed only: SSW jobs exist when they
+    // extend with SSW (paired-end, or single-end with --ssw), piecewise jobs
+    // otherwise
+    const bool ssw = ssw_jobs;
+    if (ssw) {
+        k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
+                                           (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
+        SA_CHECK(hipGetLastError());
+        // every stripe count a job of this shape can have, byte then word,
+        // forward then reverse (a pass only adds jobs to the buckets of later
+        // launches): forward passes from the shortest read's count to the
+        // longest's, reverse passes from 1 (an alignment may end anywhere)
+        const uint32_t Lq = std::min<uint32_t>(f.len_cap, SSW_GROUP_MAX_QUERY);
+        const uint32_t Lm = std::min<uint32_t>(std::max<uint32_t>(f.len_min, 1), Lq);
+        for (int word = 0; word < 2; ++word)
+            for (int rev = 0; rev < 2; ++rev) {
+                const uint32_t g = word ? 8 : 16;
+                const uint32_t s0 = rev ? 1 : (Lm + g - 1) / g, s1 = (Lq + g - 1) / g;
+                for (uint32_t seg = s0; seg <= s1; ++seg) {
+                    SswLaunchLists ll{};
+                    for (uint32_t x = 0; x < SSW_WCLASSES; ++x) {
+                        const uint32_t b = rev ? ssw_bucket(word, true, seg, 0)
+                                               : ssw_bucket(word, false, seg, SSW_WCLASSES - 1 - x);
+                        ll.list[x] = d_lists + (uint64_t)b * cap;
+                    }
+                    if (word)
+                        launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
+                                                                    d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
+                                                                    cap, F_SSW_BLOCKS);
+                    else
+                        launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
+                                                                    d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
+                                                                    cap, F_SSW_BLOCKS);
+                    SA_CHECK(hipGetLastError());
                 }
-                if (word)
-                    launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req, d_jobs,
-                                                                rev, d_fwd, d_ends, d_lists, d_bcnt, cap,
-                                                                F_SSW_BLOCKS);
-                else
-                    launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req, d_jobs,
-                                                                rev, d_fwd, d_ends, d_lists, d_bcnt, cap,
-                                                                F_SSW_BLOCKS);
-                SA_CHECK(hipGetLastError());
             }
+    }
     // banded traceback of the lane-group jobs (queries of at most
     // SSW_GROUP_MAX_QUERY; scratch per slot as in Engine::extend)
     const uint64_t L = f.len_cap;
+    const uint32_t Lq = std::min<uint32_t>(f.len_cap, SSW_GROUP_MAX_QUERY);
     uint64_t per = 4ull * (3 * (2 * 64 + 4) + 17ull * Lq + 2 * (2ull * Lq + 64) + 64);
     per = (per + 255) & ~255ull;
-    const uint32_t fin_slots =
-        (uint32_t)std::max<uint64_t>(128, std::min<uint64_t>(F_FIN_SLOTS, F_FIN_BUDGET / per) / 128 * 128);
-    auto* d_fin = (uint32_t*)take(S_FINLISTS, (uint64_t)FIN_CLASSES * cap * 4);
-    uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
-    k_fin_bucket<<<256, 256, 0, st>>>(d_ends, 0, d_cnt, d_fin, d_fcnt, cap);
-    auto* fscr = (uint8_t*)take(S_SCRATCH, (uint64_t)fin_slots * per);
-    k_ssw_finish<<<fin_slots / 128, 128, 0, st>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, 0, d_cnt, d_res,
-                                                  d_slots, fscr, per, 0, d_fin, d_fcnt, cap);
-    SA_CHECK(hipGetLastError());
+    if (ssw) {
+        const uint32_t fin_slots =
+            (uint32_t)std::max<uint64_t>(128, std::min<uint64_t>(F_FIN_SLOTS, F_FIN_BUDGET / per) / 128 * 128);
+        auto* d_fin = (uint32_t*)take(S_FINLISTS, (uint64_t)FIN_CLASSES * cap * 4);
+        uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
+        k_fin_bucket<<<256, 256, 0, st>>>(d_ends, 0, d_cnt, d_fin, d_fcnt, cap);
+        auto* fscr = (uint8_t*)take(S_SCRATCH, (uint64_t)fin_slots * per);
+        k_ssw_finish<<<fin_slots / 128, 128, 0, st>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, 0, d_cnt, d_res,
+                                                      d_slots, fscr, per, 0, d_fin, d_fcnt, cap);
+        SA_CHECK(hipGetLastError());
+    }
     // one-thread SSW: only queries too long (or scored too high) for a lane group
-    if (f.len_cap > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * f.len_cap > 30000 || sc.match > 127 ||
-        sc.mismatch > 127) {
+    if (ssw && (f.len_cap > SSW_GROUP_MAX_QUERY || (uint64_t)sc.match * f.len_cap > 30000 || sc.match > 127 ||
+                sc.mismatch > 127)) {
         uint64_t lper = 16ull * ((L + 15) / 16 * 16) + 256 + 65536 + 8 * (2 * L + 64) + 129ull * L * 3;
         lper = (lper + 255) & ~255ull;
         const uint32_t long_slots =
@@ -1575,6 +1598,7 @@ bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecPar
     chain_tables(cp);
     auto same = [](const auto& a, const auto& b) { return std::memcmp(&a, &b, sizeof a) == 0; };
     const bool fits = !f.stale && n <= f.n_cap && n_bases_ <= f.bases_cap && max_len_ <= f.len_cap &&
+                      (min_len_ == 0 || min_len_ >= f.len_min) &&
                       same(mp, f.mp) && same(cp, f.cp) && same(sp, f.sp) && same(sc, f.sc) &&
                       d_entries_ == f.entries;
     if (!fits) {
@@ -1591,6 +1615,9 @@ bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecPar
         f.n_cap = (uint32_t)round_up(n + n / 8 + 64, 1024);
         f.bases_cap = round_up(n_bases_ + n_bases_ / 8 + 4096, 1 << 16);
         f.len_cap = std::max<uint32_t>(f.len_cap, (uint32_t)round_up(max_len_, 16));
+        // a little room below this batch's shortest read (a multiple of 16)
+        const uint32_t lmin = min_len_ > 32 ? (min_len_ - 16) / 16 * 16 : 1;
+        f.len_min = f.len_min ? std::min(f.len_min, lmin) : lmin;
         f.a_cap = (uint64_t)(anchors_per_base_ * 1.5 * (double)f.bases_cap) + 4096;
         const double rpr = std::max(f.req_per_read, sp.paired ? 1.25 : 2.0);
         f.req_cap = round_up((uint64_t)(rpr * 1.25 * f.n_cap) + 1024, 1024);

@@ -223,6 +223,7 @@ private:
     uint32_t n_reads_ = 0;
     uint64_t n_bases_ = 0;
     uint32_t max_len_ = 0;
+    uint32_t min_len_ = 0;  // shortest non-empty read of the batch (0: none)
     std::vector<uint64_t> h_off_;
     const uint8_t* d_seq_ = nullptr;
     // offsets and chain anchors of the reads the extension calls work on (the

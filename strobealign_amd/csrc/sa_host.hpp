@@ -112,6 +112,11 @@ struct RunStats {
     uint64_t fused_batches = 0, spec_requests = 0, spec_used = 0;
     // fused passes that replayed a recorded graph; passes whose anchors did not fit
     uint64_t graph_replays = 0, fused_overflows = 0;
+    // host replay: records decided and committed / decided as probes (counted
+    // per pass) / probes taken over by the next pass unchanged, and thread time
+    // in deciding and in writing output text
+    uint64_t replay_committed = 0, replay_probed = 0, replay_accepted = 0;
+    double replay_commit_ms = 0, replay_probe_ms = 0, replay_write_ms = 0;
 };
 
 // Maps all records (paired when r2 is non-empty) and appends the output

@@ -11,6 +11,7 @@
 // chunk resumes. Output is only written by a replay without misses, so it is
 // identical to a sequential run.
 #include <algorithm>
+#include <sched.h>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -281,6 +282,23 @@ private:
 };
 OutPool g_out_pool;
 
+// The replay state a record starts from: the chunk's generator and insert-size
+// estimate (bit for bit).
+bool same_state(const Rng& a, const InsertSize& ia, const Rng& b, const InsertSize& ib) {
+    static_assert(sizeof(InsertSize) == 24, "InsertSize: u64 then four f32, no padding");
+    return a.s == b.s && std::memcmp(&ia, &ib, sizeof(InsertSize)) == 0;
+}
+
+// A record decided as a probe without a miss: its output and end state are
+// exact if the record really starts from the state it was probed with (its
+// decisions read only that state and cached results, which never change).
+struct Tentative {
+    uint64_t r;            // record (relative to the chunk)
+    Rng rng0, rng1;        // state it was probed from, and its state after
+    InsertSize isz0, isz1;
+    uint64_t t0, t1;       // its output text in the pass's tentative buffer
+};
+
 // Per-chunk state of the replay.
 struct Chunk {
     uint64_t first, count;  // records (pairs) [first, first+count) of the caller's read set
@@ -297,6 +315,14 @@ struct Chunk {
     std::vector<Key> pending;
     KeyTable<char> pending_set;
     uint64_t spec_used = 0;  // fused-pass results the decisions looked up
+    // probes of the last pass without a miss (in record order) and their text;
+    // the current pass writes the next ones
+    std::vector<Tentative> tent_prev, tent_cur;
+    std::string text_prev, text_cur;
+    // replay cost: records decided and committed, records decided as probes,
+    // probes taken over unchanged, and the nanoseconds spent on each and on
+    // writing output text
+    uint64_t n_commit = 0, n_probe = 0, n_accept = 0, ns_commit = 0, ns_probe = 0, ns_write = 0;
     bool done() const { return next >= count; }
 };
 
@@ -1263,8 +1289,31 @@ void run_chunk(Ctx base, Chunk& ch) {
     Rng rng = ch.rng;
     InsertSize isz = ch.isize;
     bool probing = false;
+    using clk = std::chrono::steady_clock;
+    auto t_rec = clk::now();
+    ch.tent_cur.clear();
+    ch.text_cur.clear();
+    size_t tp = 0;  // next of the last pass's probes to check
     for (uint64_t r = ch.next; r < ch.count; ++r) {
         const uint64_t idx = r;
+        if (!probing && c.o.mode == 0) {
+            // a record the last pass probed from the state it now starts from:
+            // take its output and end state instead of deciding it again
+            while (tp < ch.tent_prev.size() && ch.tent_prev[tp].r < r) ++tp;
+            if (tp < ch.tent_prev.size() && ch.tent_prev[tp].r == r &&
+                same_state(rng, isz, ch.tent_prev[tp].rng0, ch.tent_prev[tp].isz0)) {
+                const Tentative& t = ch.tent_prev[tp];
+                ch.out.append(ch.text_prev, t.t0, t.t1 - t.t0);
+                rng = t.rng1;
+                isz = t.isz1;
+                ch.next = r + 1;
+                ch.rng = rng;
+                ch.isize = isz;
+                ch.n_accept++;
+                t_rec = clk::now();
+                continue;
+            }
+        }
         const Rng rng0 = rng;
         const InsertSize isz0 = isz;
         c.rng = &rng;
@@ -1282,8 +1331,17 @@ void run_chunk(Ctx base, Chunk& ch) {
                 align_single_end(c, idx, recs, store, dstore);
                 if (c.o.only_mapped && !recs[0].is_mapped()) recs.clear();
             }
-            if (!probing && !c.missed && !c.abandon) {
-                for (const Sam& s : recs) write_sam(ch.out, s, c);
+            if (!c.missed && !c.abandon) {
+                const auto w0 = clk::now();
+                if (!probing) {
+                    for (const Sam& s : recs) write_sam(ch.out, s, c);
+                } else {
+                    // a probe without a miss: kept for the next pass
+                    const uint64_t t0 = ch.text_cur.size();
+                    for (const Sam& s : recs) write_sam(ch.text_cur, s, c);
+                    ch.tent_cur.push_back(Tentative{r, rng0, rng, isz0, isz, t0, ch.text_cur.size()});
+                }
+                ch.ns_write += (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - w0).count();
             }
         } else {
             // mapping-only modes do no extension, so they never miss
@@ -1299,7 +1357,31 @@ void run_chunk(Ctx base, Chunk& ch) {
             ch.rng = rng;
             ch.isize = isz;
         }
+        const auto t1 = clk::now();
+        const uint64_t ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t_rec).count();
+        t_rec = t1;
+        if (probing) {
+            ch.n_probe++;
+            ch.ns_probe += ns;
+        } else {
+            ch.n_commit++;
+            ch.ns_commit += ns;
+        }
     }
+    ch.tent_prev.swap(ch.tent_cur);
+    ch.text_prev.swap(ch.text_cur);
+}
+
+// CPUs this process may run on (its affinity mask; hardware_concurrency counts
+// every CPU of the machine, also those a container or scheduler withholds).
+unsigned usable_cpus() {
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof set, &set) == 0) {
+        const int n = CPU_COUNT(&set);
+        if (n > 0) return (unsigned)n;
+    }
+    return std::max(1u, std::thread::hardware_concurrency());
 }
 
 // Worker threads shared by every parallel_for of the process (the engines' host
@@ -1337,7 +1419,7 @@ public:
 
 private:
     HostPool() {
-        const unsigned n = std::max(2u, std::thread::hardware_concurrency());
+        const unsigned n = std::max(2u, usable_cpus());
         for (unsigned t = 0; t < n; ++t) th_.emplace_back([this] { loop(); });
     }
     ~HostPool() {
@@ -1485,6 +1567,14 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         chs[x]->abundances.assign(ref.names.size(), 0.0);
         chs[x]->out = g_out_pool.take(chs[x]->count * (paired ? 1400 : 700));
         chs[x]->spec_used = 0;
+        chs[x]->n_commit = chs[x]->n_probe = chs[x]->n_accept = chs[x]->ns_commit = chs[x]->ns_probe =
+            chs[x]->ns_write = 0;
+        chs[x]->tent_prev.clear();
+        chs[x]->tent_cur.clear();
+        if (o.mode == 0) {
+            chs[x]->text_prev = g_out_pool.take(chs[x]->count * (paired ? 1400 : 700));
+            chs[x]->text_cur = g_out_pool.take(chs[x]->count * (paired ? 1400 : 700));
+        }
     }
     if (fused) {
         st.fused_batches++;
@@ -1585,12 +1675,25 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         });
         st.host_ms += now_ms() - g1;
     }
-    for (size_t x = 0; x < nch; ++x) st.spec_used += chs[x]->spec_used;
+    for (size_t x = 0; x < nch; ++x) {
+        const Chunk& ch = *chs[x];
+        st.spec_used += ch.spec_used;
+        st.replay_committed += ch.n_commit;
+        st.replay_accepted += ch.n_accept;
+        st.replay_probed += ch.n_probe;
+        st.replay_commit_ms += ch.ns_commit * 1e-6;
+        st.replay_probe_ms += ch.ns_probe * 1e-6;
+        st.replay_write_ms += ch.ns_write * 1e-6;
+    }
     // the batch's caches are no longer needed; output text stays until it is joined
     parallel_for(o.threads, nch, [&](size_t x) {
         Chunk& ch = *chs[x];
         decltype(ch.cache)().swap(ch.cache);
         std::vector<uint32_t>().swap(ch.cigar_pool);
+        g_out_pool.give(ch.text_prev);
+        g_out_pool.give(ch.text_cur);
+        std::vector<Tentative>().swap(ch.tent_prev);
+        std::vector<Tentative>().swap(ch.tent_cur);
         if (!keep) g_out_pool.give(ch.out);
     });
 }
@@ -1622,6 +1725,12 @@ void add_stats(RunStats& to, const RunStats& b) {
     to.spec_used += b.spec_used;
     to.graph_replays += b.graph_replays;
     to.fused_overflows += b.fused_overflows;
+    to.replay_committed += b.replay_committed;
+    to.replay_accepted += b.replay_accepted;
+    to.replay_probed += b.replay_probed;
+    to.replay_commit_ms += b.replay_commit_ms;
+    to.replay_probe_ms += b.replay_probe_ms;
+    to.replay_write_ms += b.replay_write_ms;
 }
 
 }  // namespace
