@@ -207,6 +207,7 @@ def serve_varied(a, sa, synth, ctx, threads, slots):
 STAGE_KERNELS = {"k_hits": ("k_hits", "k_hits_grp"), "k_chain": ("k_zero_u32", "k_chain_t1", "k_chain_tn", "k_chain"),
                  "k_seed": ("k_seed",), "k_finalize": ("k_finalize",)}
 COPY_BYTES = 1 << 30  # k_hbm_copy reads and writes 1 GiB per dispatch
+VALU_INT_PEAK_TOPS = 39.3  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, int32 ops
 VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each
 
 
@@ -386,16 +387,23 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
                 tb = int(tb * run_batch / prof_batch)
             lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1),
                            "traffic_source": "carried from profiles/pmc_traffic.json"})
-    # forward SSW pass: DP cells per second against the VALU integer peak
-    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; about 10 ops per
-    # cell: profile add, clamp, two 3-way maxima, column max, gap subtractions)
-    extension = None
-    if kd.get("k_ssw_byte_fwd", z)[4] > 0 and kd["k_ssw_byte_fwd"][0] > 0:
-        ms_, _, _, _, cells = kd["k_ssw_byte_fwd"]
+    # forward SSW passes: DP cells per second against the VALU integer peak
+    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; 10 ops per
+    # cell of the striped recurrence: profile add, H from three maxima, the
+    # column maximum, E and F each two subtractions and a maximum)
+    def ssw_rate(name):
+        ms_, _, _, _, cells = kd.get(name, z)
+        if not cells or not ms_:
+            return None
         gcups = cells / (ms_ / 1000.0) / 1e9
-        extension = {"kernel": "k_ssw_byte_fwd", "forward_cells_per_step": int(cells / kd_steps),
-                     "gcups": round(gcups, 1), "ops_per_cell": 10, "valu_int_peak_tops": 39.3,
-                     "frac": round(gcups * 10 / 39300.0, 4)}
+        return {"kernel": name, "forward_cells_per_step": int(cells / kd_steps), "gcups": round(gcups, 1),
+                "ops_per_cell": 10, "valu_int_peak_tops": VALU_INT_PEAK_TOPS,
+                "frac": round(gcups * 10 / (VALU_INT_PEAK_TOPS * 1000.0), 4)}
+    extension = ssw_rate("k_ssw_byte_fwd")
+    if extension is not None:
+        word = ssw_rate("k_ssw_word_fwd")
+        if word is not None:
+            extension["word"] = word
         # VALU issue of the score passes by their counters (SQ_INSTS_VALU over
         # the kernel's duration), from the committed profile of this workload
         if tj.get("workload") == workload:
@@ -416,6 +424,19 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
             "kernel_ms_per_step": {k: round(v[0] / kd_steps, 2) for k, v in kd.items()},
             "kernel_gbps": {k: round(v[3] / (v[0] / 1000.0) / 1e9, 2) for k, v in kd.items() if v[3] and v[0]},
             "kernel_times_from": kd_source}
+    if dom.startswith("k_ssw_"):
+        # an SSW score pass is bound by the VALU, not by HBM (its jobs stay in
+        # registers): cells x 10 int32 ops against the VALU integer peak, the
+        # counter-measured instruction issue beside it
+        ms_, launches_, _, _, cells = kd[dom]
+        tops = cells * 10 / (ms_ / 1000.0) / 1e12 if ms_ and cells else 0.0
+        roof.update({"bound": "valu", "unit": "TOP/s", "achieved": round(tops, 3), "peak": VALU_INT_PEAK_TOPS,
+                     "frac": round(tops / VALU_INT_PEAK_TOPS, 5), "traffic": None,
+                     "traffic_source": "an SSW score pass keeps its job in registers (counter traffic < 0.01 GB "
+                                       "per launch)",
+                     "bytes_per_launch": 0,
+                     "cells_per_launch": int(cells / max(1, launches_)),
+                     "valu_note": "10 int32 ops per DP cell; VALU issue by the counters in extension"})
     return roof, lookup, extension
 
 

@@ -242,7 +242,9 @@ constexpr uint32_t F_LONG_SLOTS = 2048;    // one-thread SSW of long queries, at
 constexpr uint64_t F_FIN_BUDGET = 2ull << 30;   // bytes of traceback scratch, at most
 constexpr uint64_t F_LONG_BUDGET = 2ull << 30;  // bytes of one-thread SSW scratch, at most
 constexpr uint64_t F_PW_BUDGET = 4ull << 30;    // bytes of piecewise scratch
-constexpr uint32_t EXT_NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;
+constexpr uint32_t EXT_NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES + 2;
+constexpr uint32_t EXT_WCELLS = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;  // word index of the u64 word-pass cells
+static_assert(EXT_WCELLS % 2 == 0, "word-pass cell counter alignment");
 uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 }  // namespace
 
@@ -1137,17 +1139,17 @@ template <int G, bool BYTE, int S, int MAX>
 void launch_ssw_pass(uint32_t seg, const SswLaunchLists& ll, hipStream_t st, const uint8_t* seq, const uint64_t* off,
                      RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, uint32_t rev,
                      SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap,
-                     uint32_t dev_grid = 0) {
+                     unsigned long long* wcells, uint32_t dev_grid = 0) {
     if constexpr (S > MAX) {
         throw std::logic_error("SSW stripe count out of range");
     } else if (seg != S) {
         launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, ll, st, seq, off, ref, sc, req, jobs, rev, fwd, ends, lists, counts,
-                                             cap, dev_grid);
+                                             cap, wcells, dev_grid);
     } else {
         const uint64_t count = ll.start[SSW_WCLASSES];
         const uint32_t grid = dev_grid ? dev_grid : blocks(count * G, 256);
         k_ssw_pass<G, BYTE, S><<<grid, 256, 0, st>>>(seq, off, ref, sc, req, jobs, ll, rev, fwd, ends, lists, counts,
-                                                     cap, dev_grid ? 1u : 0u);
+                                                     cap, dev_grid ? 1u : 0u, wcells);
     }
 }
 }  // namespace
@@ -1183,8 +1185,9 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_ljobs = (ExtJob*)grow(S_LJOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
     // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets,
-    // then (64-bit, at word 4 + SSW_BUCKETS) the DP cells of the forward SSW passes
-    constexpr uint32_t NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;
+    // then (64-bit, at word 4 + SSW_BUCKETS) the DP cells of the forward byte passes,
+    // the traceback classes, and (64-bit) the DP cells of the forward word passes
+    constexpr uint32_t NCNT = EXT_NCNT;
     static_assert((4 + SSW_BUCKETS) % 2 == 0, "cell counter alignment");
     auto* d_cnt = (uint32_t*)grow(S_XCNT, NCNT * 4);
     uint32_t* d_bcnt = d_cnt + 4;
@@ -1222,8 +1225,9 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, nullptr,
                                                                       d_res, d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
+    auto* d_wcells = (unsigned long long*)(d_cnt + EXT_WCELLS);
     k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
-                                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
+                                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS), d_wcells);
     SA_CHECK(hipGetLastError());
     read_counts();
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
@@ -1283,22 +1287,25 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 if (!at) continue;
                 if (word)
                     launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
-                                                     d_fwd, d_ends, d_lists, d_bcnt, cap);
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_wcells);
                 else
                     launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
-                                                     d_fwd, d_ends, d_lists, d_bcnt, cap);
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_wcells);
                 SA_CHECK(hipGetLastError());
             }
         };
         timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); }, 0, fwd_cells);
         read_counts();
+        // the word forward passes' cells: word-first jobs and the byte passes' saturated ones
+        uint64_t word_cells = 0;
+        std::memcpy(&word_cells, cnt + EXT_WCELLS, 8);
         uint64_t nrb = 0, nfw = 0;
         for (uint32_t x = 0; x < SSW_BYTE_SEGS; ++x) nrb += bc[ssw_bucket(false, true, x + 1, 0)];
         for (uint32_t x = 0; x < SSW_WORD_SEGS; ++x)
             for (uint32_t wc = 0; wc < SSW_WCLASSES; ++wc) nfw += bc[ssw_bucket(true, false, x + 1, wc)];
         if (nrb) timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
         if (nfw) {
-            timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });
+            timed("k_ssw_word_fwd", nfw, [&] { run(true, false); }, 0, word_cells);
             read_counts();
             timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
         }
@@ -1493,7 +1500,8 @@ void Engine::fused_sequence(Fused& f) {
     const bool ssw = ssw_jobs;
     if (ssw) {
         k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
-                                           (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS));
+                                           (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS),
+                                           (unsigned long long*)(d_cnt + EXT_WCELLS));
         SA_CHECK(hipGetLastError());
         // every stripe count a job of this shape can have, byte then word,
         // forward then reverse (a pass only adds jobs to the buckets of later
@@ -1515,11 +1523,13 @@ void Engine::fused_sequence(Fused& f) {
                     if (word)
                         launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
                                                                     d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
-                                                                    cap, F_SSW_BLOCKS);
+                                                                    cap, (unsigned long long*)(d_cnt + EXT_WCELLS),
+                                                                    F_SSW_BLOCKS);
                     else
                         launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
                                                                     d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
-                                                                    cap, F_SSW_BLOCKS);
+                                                                    cap, (unsigned long long*)(d_cnt + EXT_WCELLS),
+                                                                    F_SSW_BLOCKS);
                     SA_CHECK(hipGetLastError());
                 }
             }

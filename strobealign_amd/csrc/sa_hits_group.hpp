@@ -47,9 +47,10 @@ SA_HD uint32_t lookup_one(const IndexView& ix, const QRand& r, bool full, bool p
     const uint64_t cutoff = ix.p.filter_cutoff;
     bool try_partial = !full;
     if (full) {
-        const uint64_t pos = get_masked_from(ix, r.hash, REF_HASH_MASK, NONE64);
+        uint64_t pend = 0;
+        const uint64_t pos = get_masked_from(ix, r.hash, REF_HASH_MASK, NONE64, &pend);
         if (pos != NONE64) {
-            const bool f = too_frequent(ix, pos, cutoff, r.hash_rc);
+            const bool f = too_frequent(ix, pos, cutoff, r.hash_rc, pend);
             h = Hit{r.hash, r.hash_rc, pos, NONE64, 0, r.start, r.end, f ? HIT_FILTERED : 0u};
             return f ? 2u : 1u;
         }
@@ -57,9 +58,10 @@ SA_HD uint32_t lookup_one(const IndexView& ix, const QRand& r, bool full, bool p
         try_partial = partial_fallback;
     }
     if (try_partial) {
-        const uint64_t pos = get_masked_from(ix, r.hash, ix.p.main_mask, NONE64);
+        uint64_t pend = 0;
+        const uint64_t pos = get_masked_from(ix, r.hash, ix.p.main_mask, NONE64, &pend);
         if (pos != NONE64) {
-            const bool f = too_frequent_partial(ix, pos, cutoff);
+            const bool f = too_frequent_partial(ix, pos, cutoff, pend);
             h = Hit{r.hash, r.hash_rc, pos, NONE64, 0, r.start, r.start + (uint32_t)ix.p.k,
                     HIT_PARTIAL | (f ? HIT_FILTERED : 0u)};
             return f ? 4u : 3u;

@@ -618,8 +618,9 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
                                                    const ExtRequest* __restrict__ req,
                                                    const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ counters,
                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
-                                                   uint32_t cap, unsigned long long* __restrict__ cells) {
-    uint64_t c = 0;
+                                                   uint32_t cap, unsigned long long* __restrict__ cells,
+                                                   unsigned long long* __restrict__ wcells) {
+    uint64_t c = 0, cw = 0;
     const uint32_t n = counters[0];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const uint32_t rid = req[jobs[t].req].read_id;
@@ -627,12 +628,17 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
         const bool word = (jobs[t].flags & SSW_WORD_FIRST) != 0;
         const uint32_t b = ssw_bucket(word, false, word ? (L + 7) / 8 : (L + 15) / 16, ssw_wclass(jobs[t].wlen));
         lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
-        c += word ? 0 : (uint64_t)L * jobs[t].wlen;  // cells of the byte forward pass
+        // cells of the byte forward pass, or of the word forward pass for word-first jobs
+        (word ? cw : c) += (uint64_t)L * jobs[t].wlen;
     }
     // one atomic per wave
     SA_UNROLL
-    for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+    for (int s = 32; s > 0; s >>= 1) {
+        c += __shfl_xor(c, s);
+        cw += __shfl_xor(cw, s);
+    }
     if ((threadIdx.x & 63u) == 0 && c) atomicAdd(cells, (unsigned long long)c);
+    if ((threadIdx.x & 63u) == 0 && cw) atomicAdd(wcells, (unsigned long long)cw);
 }
 
 // One striped score pass (ext/ssw/ssw.c:197-588) per group of G lanes: the
@@ -658,7 +664,7 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
                                     const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
                                     const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
                                     SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                    uint32_t* __restrict__ counts, uint32_t cap);
+                                    uint32_t* __restrict__ counts, uint32_t cap, unsigned long long* __restrict__ wcells);
 
 template <int G, bool BYTE, int SEG>
 __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
@@ -666,7 +672,8 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
                                                  const ExtJob* __restrict__ jobs, SswLaunchLists ll,
                                                  uint32_t rev, SswEnd* __restrict__ fwd,
                                                  SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                                 uint32_t* __restrict__ counts, uint32_t cap, uint32_t dev_counts) {
+                                                 uint32_t* __restrict__ counts, uint32_t cap, uint32_t dev_counts,
+                                                 unsigned long long* __restrict__ wcells) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t l = tid % G;
     uint32_t stride = 0xFFFFFFFFu;
@@ -687,7 +694,7 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
         SA_UNROLL
         for (uint32_t x = 1; x < SSW_WCLASSES; ++x) c += slot >= ll.start[x] ? 1u : 0u;
         ssw_pass_job<G, BYTE, SEG>(ll.list[c][slot - ll.start[c]], l, seq, off, ref, sc, req, jobs, rev, fwd, ends,
-                                   lists, counts, cap);
+                                   lists, counts, cap, wcells);
         if (stride == 0xFFFFFFFFu) break;
     }
 }
@@ -697,7 +704,7 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
                                     const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
                                     const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
                                     SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                    uint32_t* __restrict__ counts, uint32_t cap) {
+                                    uint32_t* __restrict__ counts, uint32_t cap, unsigned long long* __restrict__ wcells) {
     const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
@@ -712,6 +719,7 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
             uint32_t b;
             if (BYTE && f.score == 255) {
                 b = ssw_bucket(true, false, (uint32_t)(L + 7) / 8, ssw_wclass(j.wlen));
+                atomicAdd(wcells, (unsigned long long)L * j.wlen);  // cells of its word forward pass
             } else {
                 fwd[job] = f;
                 b = ssw_bucket(!BYTE, true, (uint32_t)(f.read + 1 + G - 1) / G, 0);

@@ -361,11 +361,14 @@ SA_HD uint64_t upper_bound_masked(const IndexView& ix, uint64_t lo, uint64_t hi,
 
 // First entry whose masked hash equals hash&mask, searching the bucket of `hash`
 // from `start` (or the bucket start); NONE64 when absent (src/index.rs:159-197).
-SA_HD uint64_t get_masked_from(const IndexView& ix, uint64_t hash, uint64_t mask, uint64_t start) {
+// *bucket_end (if given) receives the end of that bucket.
+SA_HD uint64_t get_masked_from(const IndexView& ix, uint64_t hash, uint64_t mask, uint64_t start,
+                               uint64_t* bucket_end = nullptr) {
     uint64_t masked = hash & mask;
     uint64_t top = hash >> (64 - ix.p.bits);
     uint64_t ps = start == NONE64 ? ix.bs[top] : start;
     uint64_t pe = ix.bs[top + 1];
+    if (bucket_end) *bucket_end = pe;
     ix.bucket_reads += start == NONE64 ? 2 : 1;
     if (ps >= pe) return NONE64;
     if (pe - ps < 4) {
@@ -405,27 +408,41 @@ SA_HD uint64_t get_count_full(const IndexView& ix, uint64_t position, uint64_t h
     return rev + get_count(ix, position, REF_HASH_MASK);
 }
 
-// src/index.rs:298-307 (the guard uses filter_cutoff, the probe uses cutoff)
-SA_HD bool too_frequent_forward(const IndexView& ix, uint64_t position, uint64_t cutoff) {
-    if (position + ix.p.filter_cutoff < ix.p.n_entries) return ehash(ix, position) == ehash(ix, position + cutoff);
+// src/index.rs:298-307 (the guard uses filter_cutoff, the probe uses cutoff).
+// bucket_end: the end of position's bucket. An entry at or past it lies in a
+// later bucket, whose entries differ from position's in the hash's top bits,
+// so the probe is not read there (it cannot compare equal).
+SA_HD bool too_frequent_forward(const IndexView& ix, uint64_t position, uint64_t cutoff, uint64_t bucket_end) {
+    if (position + ix.p.filter_cutoff < ix.p.n_entries) {
+        if (position + cutoff >= bucket_end) return false;
+        return ehash(ix, position) == ehash(ix, position + cutoff);
+    }
     return false;
 }
 
-// src/index.rs:309-323
-SA_HD bool too_frequent(const IndexView& ix, uint64_t position, uint64_t cutoff, uint64_t hash_rc) {
-    if (too_frequent_forward(ix, position, cutoff)) return true;
-    uint64_t rc = get_masked_from(ix, hash_rc, REF_HASH_MASK, NONE64);
+// src/index.rs:309-323 (bucket_end as for too_frequent_forward)
+SA_HD bool too_frequent(const IndexView& ix, uint64_t position, uint64_t cutoff, uint64_t hash_rc,
+                        uint64_t bucket_end) {
+    if (too_frequent_forward(ix, position, cutoff, bucket_end)) return true;
+    uint64_t rc_end = 0;
+    uint64_t rc = get_masked_from(ix, hash_rc, REF_HASH_MASK, NONE64, &rc_end);
     if (rc != NONE64) {
-        if (too_frequent_forward(ix, rc, cutoff)) return true;
+        if (too_frequent_forward(ix, rc, cutoff, rc_end)) return true;
         return get_count(ix, position, REF_HASH_MASK) + get_count(ix, rc, REF_HASH_MASK) > cutoff;
     }
     return false;
 }
 
-// src/index.rs:325-338
-SA_HD bool too_frequent_partial(const IndexView& ix, uint64_t position, uint64_t cutoff) {
-    if (position + cutoff < ix.p.n_entries)
-        return (ehash(ix, position) & ix.p.main_mask) == (ehash(ix, position + cutoff) & ix.p.main_mask);
+// src/index.rs:325-338. When the bucket bits lie within main_mask's leading
+// ones, entries of different buckets differ under main_mask too, so a probe
+// past position's bucket (bucket_end) compares unequal without being read.
+SA_HD bool too_frequent_partial(const IndexView& ix, uint64_t position, uint64_t cutoff, uint64_t bucket_end) {
+    if (position + cutoff < ix.p.n_entries) {
+        const uint64_t m = ix.p.main_mask;
+        const bool bucket_in_mask = ~m == 0 || (uint64_t)ix.p.bits <= (uint64_t)__builtin_clzll(~m);
+        if (bucket_in_mask && position + cutoff >= bucket_end) return false;
+        return (ehash(ix, position) & m) == (ehash(ix, position + cutoff) & m);
+    }
     return false;
 }
 
@@ -487,9 +504,10 @@ SA_HD uint32_t find_hits(const IndexView& ix, const QRand* qr, uint32_t n, const
     const uint32_t k = (uint32_t)ix.p.k;
     uint32_t nh = 0;
     auto partial = [&](const QRand& r) {
-        uint64_t pos = get_masked_from(ix, r.hash, ix.p.main_mask, NONE64);
+        uint64_t pend = 0;
+        uint64_t pos = get_masked_from(ix, r.hash, ix.p.main_mask, NONE64, &pend);
         if (pos != NONE64) {
-            bool f = too_frequent_partial(ix, pos, cutoff);
+            bool f = too_frequent_partial(ix, pos, cutoff, pend);
             if (f) d.partial_filtered++; else d.partial_found++;
             hits[nh++] = Hit{r.hash, r.hash_rc, pos, NONE64, 0, r.start, r.start + k, HIT_PARTIAL | (f ? HIT_FILTERED : 0u)};
         } else {
@@ -499,9 +517,10 @@ SA_HD uint32_t find_hits(const IndexView& ix, const QRand* qr, uint32_t n, const
     if (mp.mcs != 3) {
         for (uint32_t i = 0; i < n; ++i) {
             const QRand& r = qr[i];
-            uint64_t pos = get_masked_from(ix, r.hash, REF_HASH_MASK, NONE64);
+            uint64_t pend = 0;
+            uint64_t pos = get_masked_from(ix, r.hash, REF_HASH_MASK, NONE64, &pend);
             if (pos != NONE64) {
-                bool f = too_frequent(ix, pos, cutoff, r.hash_rc);
+                bool f = too_frequent(ix, pos, cutoff, r.hash_rc, pend);
                 if (f) d.full_filtered++; else d.full_found++;
                 hits[nh++] = Hit{r.hash, r.hash_rc, pos, NONE64, 0, r.start, r.end, f ? HIT_FILTERED : 0u};
             } else {
