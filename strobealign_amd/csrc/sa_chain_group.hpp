@@ -170,6 +170,23 @@ struct Grp {
     __device__ __forceinline__ T bcast(T v, uint32_t src) const {
         return __shfl(v, (int)src, G);
     }
+    // a value equal on every lane of the group; on a whole wave it is moved to
+    // a scalar register, so the code that depends on it runs once per wave
+    __device__ __forceinline__ uint32_t uni(uint32_t v) const {
+        if constexpr (G == 64) return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+        else return v;
+    }
+    __device__ __forceinline__ float uni(float v) const {
+        if constexpr (G == 64)
+            return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+        else return v;
+    }
+    // lane src's value to every lane of the group (src the same on all of them)
+    __device__ __forceinline__ float lane(float v, uint32_t src) const {
+        if constexpr (G == 64)
+            return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), (int)src));
+        else return __shfl(v, (int)(base + src));
+    }
 };
 
 SA_HD bool key_less(uint32_t ar, uint32_t as, uint32_t aq, uint32_t br, uint32_t bs, uint32_t bq) {
@@ -218,8 +235,9 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
                           const uint64_t* __restrict__ off, const Hit* __restrict__ hits,
                           const HitCounts* __restrict__ hc, const uint64_t* __restrict__ aoff, const ChainBufs& b,
                           uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
-                          const float* __restrict__ log2tab, const float* __restrict__ same_diag) {
+                          const float* __restrict__ log2tab) {
     const uint32_t r = gp.r;
+    t = gp.uni(t);
     const uint32_t i_read = t >> 1, o = t & 1;
     const uint64_t ro = off[i_read];
     const uint32_t L_read = (uint32_t)(off[i_read + 1] - ro);
@@ -249,6 +267,7 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
     }
     entries = gp.sum(entries);
     if (na > CAP) na = CAP;  // capacity bound: cannot be exceeded (2 per entry at most)
+    na = gp.uni(na);
 
     // 2. bitonic sort, padded to a power of two with keys above every anchor
     uint32_t P = 1;
@@ -290,31 +309,48 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         m += gp.sum(keep ? 1u : 0u);
         lds_fence();
     }
+    m = gp.uni(m);
 
-    // 4. collinear chaining DP (src/chainer.rs:85-180)
+    // 4. collinear chaining DP (src/chainer.rs:85-180). What the scalar loop
+    // carries from one anchor to the next (the previous anchor and its score,
+    // the best anchor) is equal on every lane of the group: on a whole wave it
+    // is held in scalar registers and the common steps run as scalar code.
+    // A same-diagonal predecessor (dq == dr >= 1: the quotient is exactly 1)
+    // passes the ratio test exactly when 1 does, and its score is chain_score's
+    // with dd = 0, the value of the precomputed same-diagonal table
+    // (src/chainer.rs:63-74).
     const uint64_t max_ref_gap = cp.max_ref_gap >= 0 ? (uint64_t)cp.max_ref_gap : L_read;
+    // dr >= max_ref_gap for a 32-bit dr, in 32 bits (no dr reaches a larger gap)
+    const bool gap_fits = max_ref_gap <= 0xFFFFFFFFull;
+    const uint32_t gap32 = gap_fits ? (uint32_t)max_ref_gap : 0u;
+    auto gap_breaks = [&](uint32_t dr) { return gap_fits && dr >= gap32; };
     const RatioTest rt = make_ratio_test(cp.max_diagonal_ratio);
+    const bool diag_ratio_ok = !ratio_exceeds(1u, 1u, rt);
     const float NEG = -__builtin_huge_valf();
     float best = 0.0f;
     uint32_t best_i = NONE32;
-    AnchorL bA{0, 0, 0, 0.0f};
+    uint32_t b_ref = 0, b_rs = 0, b_q = 0;  // the best anchor
+    float b_dp = 0.0f;
+    uint32_t p_ref = 0, p_rs = 0, p_q = 0;  // anchor i - 1
+    float p_dp = 0.0f;
+    AnchorL nx = L.a[0];  // anchor i, read one step ahead
     for (uint32_t i = 0; i < m; ++i) {
-        const AnchorL ai = L.a[i];
+        const uint32_t ai_ref = gp.uni(nx.ref), ai_rs = gp.uni(nx.rs), ai_q = gp.uni(nx.q);
+        if (i + 1 < m) nx = L.a[i + 1];
         float dpi = (float)k;
         uint32_t pi = NONE32;
         const uint32_t lookup_end = i > cp.max_lookback ? i - cp.max_lookback : 0;
         const uint32_t span = i - lookup_end;
-        // The scalar loop's first predecessor j = i - 1 settles most steps (the
-        // group decides it uniformly, without the lane scan): a break there ends
-        // the loop with nothing taken; taken on the same diagonal, the loop stops.
+        // The scalar loop's first predecessor j = i - 1 settles most steps: a
+        // break there ends the loop with nothing taken; taken on the same
+        // diagonal, the loop stops.
         bool settled = span == 0;
         if (!settled) {
-            const AnchorL aj = L.a[i - 1];
-            const uint32_t dq = ai.q - aj.q, dr = ai.rs - aj.rs;
-            if (aj.ref != ai.ref || (ai.q >= aj.q && dr >= max_ref_gap)) {
+            const uint32_t dq = ai_q - p_q, dr = ai_rs - p_rs;
+            if (p_ref != ai_ref || (ai_q >= p_q && gap_breaks(dr))) {
                 settled = true;
-            } else if (ai.q >= aj.q && dq == dr && !ratio_exceeds(dq, dr, rt)) {
-                const float ns = aj.dp + (dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab));
+            } else if (ai_q >= p_q && dq == dr && diag_ratio_ok) {
+                const float ns = p_dp + chain_score(dq, dq, k, cp, log2tab);
                 if (ns >= dpi) {
                     dpi = ns;
                     pi = i - 1;
@@ -322,24 +358,24 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
                 }
             }
         }
+        // otherwise the lookback window j = i-1, i-2, ... spread over the lanes
         for (uint32_t c0 = 0; c0 < span && !settled; c0 += G) {
             const uint32_t l = c0 + r;
             bool brk = false, cand = false, diag = false;
             float ns = NEG;
             if (l < span) {
                 const AnchorL aj = L.a[i - 1 - l];
-                const uint32_t dq = ai.q - aj.q, dr = ai.rs - aj.rs;
-                const bool qok = ai.q >= aj.q;
+                const uint32_t dq = ai_q - aj.q, dr = ai_rs - aj.rs;
+                const bool qok = ai_q >= aj.q;
                 // the scalar loop: ref_id change breaks; a later query start is
                 // skipped; otherwise dr >= max_ref_gap breaks
-                brk = aj.ref != ai.ref || (qok && dr >= max_ref_gap);
+                brk = aj.ref != ai_ref || (qok && gap_breaks(dr));
                 if (!brk && qok) {
                     const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
                     diag = dq == dr;
                     if (!ratio_exceeds(mx, mn, rt)) {
                         cand = true;
-                        const float sc = diag && dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab);
-                        ns = aj.dp + sc;
+                        ns = aj.dp + chain_score(dq, dr, k, cp, log2tab);
                     }
                 }
             }
@@ -362,15 +398,16 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
             }
             if (sm || bm) break;
         }
+        dpi = gp.uni(dpi);
+        pi = gp.uni(pi);
         // the best-index re-check; when the loop took the best anchor itself it
         // recomputes the same value, which cannot be strictly larger
-        if (best_i != NONE32 && best_i != pi && bA.ref == ai.ref && ai.q > bA.q) {
-            const uint32_t dq = ai.q - bA.q, dr = ai.rs - bA.rs;
+        if (best_i != NONE32 && best_i != pi && b_ref == ai_ref && ai_q > b_q) {
+            const uint32_t dq = ai_q - b_q, dr = ai_rs - b_rs;
             const uint32_t mx = dq > dr ? dq : dr, mn = dq < dr ? dq : dr;
             // ratio <= R: dq >= 1 and dr >= 1 here, so the quotient is a number
-            if (dr < max_ref_gap && dr > 0 && !ratio_exceeds(mx, mn, rt)) {
-                const float sc = dq == dr && dq < 1024 ? same_diag[dq] : chain_score(dq, dr, k, cp, log2tab);
-                const float ns = bA.dp + sc;
+            if (!gap_breaks(dr) && dr > 0 && !ratio_exceeds(mx, mn, rt)) {
+                const float ns = b_dp + chain_score(dq, dr, k, cp, log2tab);
                 if (ns > dpi) {
                     dpi = ns;
                     pi = best_i;
@@ -384,8 +421,15 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         if (dpi > best) {
             best = dpi;
             best_i = i;
-            bA = AnchorL{ai.ref, ai.rs, ai.q, dpi};
+            b_ref = ai_ref;
+            b_rs = ai_rs;
+            b_q = ai_q;
+            b_dp = dpi;
         }
+        p_ref = ai_ref;
+        p_rs = ai_rs;
+        p_q = ai_q;
+        p_dp = dpi;
         lds_fence();
     }
 
@@ -402,70 +446,116 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
         nc += gp.sum(c ? 1u : 0u);
     }
     lds_fence();
-    for (uint32_t c = r; c < nc; c += G) {
-        const float dx = L.a[L.cand[c]].dp;
+    nc = gp.uni(nc);
+    // the stable rank of each candidate (the candidates with a larger score, or
+    // an equal one earlier in the list): the group holds the scores of G
+    // candidates at a time in registers and every lane counts against them
+    for (uint32_t c0 = 0; c0 < nc; c0 += G) {
+        const uint32_t c = c0 + r;
+        const uint16_t xi = c < nc ? L.cand[c] : (uint16_t)0;
+        const float dx = c < nc ? L.a[xi].dp : 0.0f;
         uint32_t rank = 0;
-        for (uint32_t d = 0; d < nc; ++d) {
-            const float dy = L.a[L.cand[d]].dp;
-            rank += (dy > dx || (dy == dx && d < c)) ? 1u : 0u;
+        for (uint32_t d0 = 0; d0 < nc; d0 += G) {
+            const float dyl = d0 + r < nc ? L.a[L.cand[d0 + r]].dp : 0.0f;
+            const uint32_t dn = nc - d0 < (uint32_t)G ? nc - d0 : (uint32_t)G;
+            for (uint32_t s = 0; s < dn; ++s) {
+                const float dy = gp.lane(dyl, s);
+                rank += (dy > dx || (dy == dx && d0 + s < c)) ? 1u : 0u;
+            }
         }
-        L.order[rank] = L.cand[c];
+        if (c < nc) L.order[rank] = xi;
     }
     lds_fence();
-    if (r == 0) {
-        uint32_t nn = 0;
-        uint64_t ca = 0;
-        uint32_t* cha = b.chain_anchors + 4 * ab;
-        NamRec* nams = b.nams + ab;
-        for (uint32_t c = 0; c < nc; ++c) {
-            const uint32_t i = L.order[c];
-            if (L.used[i]) continue;
-            const AnchorL ai = L.a[i];
-            uint32_t j = i;
-            AnchorL aj = ai;
-            bool overlaps = false;
-            const uint64_t start = ca;
-            cha[2 * ca] = ai.rs;
-            cha[2 * ca + 1] = ai.q;
-            ++ca;
-            uint32_t matching = k, cov = ai.rs;
-            while (L.pred[j] != NONE16) {
-                j = L.pred[j];
-                if (L.used[j]) {
-                    overlaps = true;
-                    break;
-                }
-                aj = L.a[j];
-                cha[2 * ca] = aj.rs;
-                cha[2 * ca + 1] = aj.q;
-                ++ca;
-                L.used[j] = 1;
-                const uint32_t gap = cov > aj.rs ? cov - aj.rs : 0;
-                matching += gap < k ? gap : k;
-                cov = aj.rs;
+    // The greedy backtrack, by the group together. From a chain's head the
+    // links are followed G nodes at a time where they run through consecutive
+    // anchors (pred[x] == x - 1, the common case along a diagonal): lane l
+    // takes node j - l. A run ends at the first node whose link goes
+    // elsewhere, or before the first node already used: the chain then
+    // overlaps an earlier one and is dropped (the nodes it visited stay used).
+    uint32_t nn = 0, ca = 0;
+    uint32_t* cha = b.chain_anchors + 4 * ab;
+    NamRec* nams = b.nams + ab;
+    for (uint32_t c = 0; c < nc; ++c) {
+        const uint32_t i = gp.uni((uint32_t)L.order[c]);
+        if (gp.uni((uint32_t)L.used[i])) continue;
+        const AnchorL hd = L.a[i];
+        const uint32_t ai_ref = gp.uni(hd.ref), ai_rs = gp.uni(hd.rs), ai_q = gp.uni(hd.q);
+        const float ai_dp = gp.uni(hd.dp);
+        const uint32_t start = ca;
+        if (r == 0) {
+            cha[2 * ca] = ai_rs;
+            cha[2 * ca + 1] = ai_q;
+        }
+        ++ca;
+        uint32_t matching = k, cov = ai_rs, last_q = ai_q;
+        bool overlaps = false;
+        for (uint32_t j = gp.uni((uint32_t)L.pred[i]); j != NONE16;) {
+            const bool valid = r <= j;
+            const uint32_t x = j - r;
+            uint32_t px = NONE16, used = 0;
+            AnchorL ax{0, 0, 0, 0.0f};
+            if (valid) {
+                px = L.pred[x];
+                used = L.used[x];
+                ax = L.a[x];
             }
-            if (overlaps) {
-                ca = start;
-                continue;
+            const bool link = valid && px != NONE16 && px + 1 == x;
+            const uint64_t um = gp.ballot(used != 0), lm = gp.ballot(!link);
+            const uint32_t fu = um ? (uint32_t)__builtin_ctzll(um) : (uint32_t)G;
+            const uint32_t fl = lm ? (uint32_t)__builtin_ctzll(lm) : (uint32_t)G;
+            const bool stop = fu <= fl && fu < (uint32_t)G;  // a used node before the run's end
+            const uint32_t R = stop ? fu : (fl + 1 < (uint32_t)G ? fl + 1 : (uint32_t)G);
+            // every lane takes part in the shuffle (a lane whose source lane is
+            // inactive would read nothing defined); lane 0 then takes cov
+            const uint32_t up = __shfl_up(ax.rs, 1, G);
+            const uint32_t prev = r == 0 ? cov : up;
+            uint32_t g = 0;
+            if (r < R) {
+                cha[2 * (ca + r)] = ax.rs;
+                cha[2 * (ca + r) + 1] = ax.q;
+                L.used[x] = 1;
+                const uint32_t gap = prev > ax.rs ? prev - ax.rs : 0;
+                g = gap < k ? gap : k;
             }
+            matching += gp.sum(g);
+            ca += R;
+            lds_fence();
+            if (R) {
+                cov = gp.uni(gp.bcast(ax.rs, R - 1));
+                last_q = gp.uni(gp.bcast(ax.q, R - 1));
+            }
+            if (stop) {
+                overlaps = true;
+                break;
+            }
+            j = gp.uni(gp.bcast(px, R - 1));
+        }
+        if (overlaps) {
+            ca = start;
+            continue;
+        }
+        if (r == 0) {
             NamRec rec;
             rec.nam_id = nn;
-            rec.query_start = aj.q;
-            rec.query_end = ai.q + k;
-            rec.ref_start = aj.rs;
-            rec.ref_end = ai.rs + k;
+            rec.query_start = last_q;
+            rec.query_end = ai_q + k;
+            rec.ref_start = cov;
+            rec.ref_end = ai_rs + k;
             rec.matching_bases = matching;
-            rec.ref_id = ai.ref;
-            rec.n_anchors = (uint32_t)(ca - start);
-            rec.score = ai.dp + (float)rec.n_anchors * cp.matches_weight;
+            rec.ref_id = ai_ref;
+            rec.n_anchors = ca - start;
+            rec.score = ai_dp + (float)rec.n_anchors * cp.matches_weight;
             rec.flags = o ? 1u : 0u;
             rec.anchor_off = 2 * ab + start;
-            nams[nn++] = rec;
+            nams[nn] = rec;
         }
+        ++nn;
+    }
+    if (r == 0) {
         nam_n[t] = nn;
         anchors_raw[t] = na;
         b.alg_bytes[t] = (uint32_t)sizeof(Hit) * nh + (uint32_t)sizeof(RefEntry) * entries +
-                         (uint32_t)sizeof(NamRec) * nn + 8u * (uint32_t)ca;
+                         (uint32_t)sizeof(NamRec) * nn + 8u * ca;
     }
     lds_fence();
 }

@@ -815,9 +815,9 @@ void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, c
     b.alg_bytes = o.cbytes = (uint32_t*)take(S_CBYTES, (2ull * n + 1) * 4);
     o.namn = (uint32_t*)take(S_NAMN, (2ull * n + 1) * 4);
     o.araw = (uint32_t*)take(S_ARAW, (2ull * n + 1) * 4);
-    // chaining tiers by anchor capacity, all in LDS: an 8-lane group per
-    // orientation up to 64 anchors, a 16-lane group up to 256, a wave up to 1024;
-    // a thread per orientation over global memory beyond
+    // chaining tiers by anchor capacity, all in LDS: a 16-lane group per
+    // orientation up to 64 anchors, a wave up to 256 and up to 1024; a thread
+    // per orientation over global memory beyond
     ChainTiers tiers;
     auto* d_tiers = (uint32_t*)take(S_TIERS, (6ull * n + 4) * 4);
     tiers.count = d_tiers;
@@ -828,14 +828,13 @@ void chain_pass(const IndexView& ix, const RefView& ref, const float* log2tab, c
     if (ev) SA_CHECK(hipEventRecord(ev[3], st));
     k_zero_u32<<<1, 64, 0, st>>>(tiers.count, 4);
     k_chain_t1<16, CHAIN_CAP1><<<blocks(2ull * n, 4), 64, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b,
-                                                                   o.namn, o.araw, log2tab, same_diag, tiers,
-                                                                   abort);
+                                                                   o.namn, o.araw, log2tab, tiers, abort);
     SA_CHECK(hipGetLastError());
     k_chain_tn<64, CHAIN_CAP2><<<8192, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
-                                                    same_diag, tiers, 0);
+                                                    tiers, 0);
     SA_CHECK(hipGetLastError());
     k_chain_tn<64, CHAIN_CAP3><<<1024, 64, 0, st>>>(ix, cp, d_off, h.hits, h.hc, d_aoff, b, o.namn, o.araw, log2tab,
-                                                    same_diag, tiers, 1);
+                                                    tiers, 1);
     SA_CHECK(hipGetLastError());
     k_chain<<<256, 256, 0, st>>>(ix, cp, d_off, n, h.hits, h.hc, h.acap, d_aoff, b, o.namn, o.araw, log2tab,
                                  same_diag, tiers);

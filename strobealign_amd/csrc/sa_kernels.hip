@@ -92,8 +92,7 @@ __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, c
                                                  const HitCounts* __restrict__ hc, const uint64_t* __restrict__ acap,
                                                  const uint64_t* __restrict__ aoff, ChainBufs b,
                                                  uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
-                                                 const float* __restrict__ log2tab,
-                                                 const float* __restrict__ same_diag, ChainTiers tiers,
+                                                 const float* __restrict__ log2tab, ChainTiers tiers,
                                                  const uint32_t* __restrict__ abort) {
     __shared__ ChainLds<CAP> lds[64 / G];
     const uint32_t g = threadIdx.x / G;
@@ -118,7 +117,7 @@ __global__ void __launch_bounds__(64) k_chain_t1(IndexView ix, ChainParams cp, c
         }
         return;
     }
-    chain_grp<G, CAP>(gp, lds[g], t, ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab, same_diag);
+    chain_grp<G, CAP>(gp, lds[g], t, ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw, log2tab);
 }
 
 // Chaining, later LDS tiers: the orientations of list `which`, a group of G
@@ -128,8 +127,7 @@ __global__ void __launch_bounds__(64) k_chain_tn(IndexView ix, ChainParams cp, c
                                                  const Hit* __restrict__ hits, const HitCounts* __restrict__ hc,
                                                  const uint64_t* __restrict__ aoff, ChainBufs b,
                                                  uint32_t* __restrict__ nam_n, uint32_t* __restrict__ anchors_raw,
-                                                 const float* __restrict__ log2tab,
-                                                 const float* __restrict__ same_diag, ChainTiers tiers,
+                                                 const float* __restrict__ log2tab, ChainTiers tiers,
                                                  uint32_t which) {
     __shared__ ChainLds<CAP> lds[64 / G];
     const uint32_t g = threadIdx.x / G;
@@ -137,7 +135,7 @@ __global__ void __launch_bounds__(64) k_chain_tn(IndexView ix, ChainParams cp, c
     const uint32_t cnt = tiers.count[which];
     for (uint32_t x = blockIdx.x * (64 / G) + g; x < cnt; x += gridDim.x * (64 / G))
         chain_grp<G, CAP>(gp, lds[g], tiers.list[which][x], ix, cp, off, hits, hc, aoff, b, nam_n, anchors_raw,
-                          log2tab, same_diag);
+                          log2tab);
 }
 
 // Chaining, last tier: the listed orientations above the LDS tiers, a thread each
