@@ -204,8 +204,10 @@ def serve_varied(a, sa, synth, ctx, threads, slots):
 # Counter profiles -> per-launch traffic and VALU figures (bench.py --summarize-pmc).
 # Engine stages (the names the engine times) and the HIP kernels each launches
 # once per batch: a stage's traffic per launch is the sum over its kernels.
-STAGE_KERNELS = {"k_hits": ("k_hits", "k_hits_grp"), "k_chain": ("k_zero_u32", "k_chain_t1", "k_chain_tn", "k_chain"),
-                 "k_seed": ("k_seed",), "k_finalize": ("k_finalize",)}
+# the kernels of each stage (helpers shared between stages, such as the counter
+# clear k_zero_u32, are left out: their dispatch counts are not the stage's)
+STAGE_KERNELS = {"k_hits": ("k_hits", "k_hits_grp"), "k_chain": ("k_chain_t1", "k_chain_tn", "k_chain"),
+                 "k_seed": ("k_seed", "k_seed_wave"), "k_finalize": ("k_finalize",)}
 COPY_BYTES = 1 << 30  # k_hbm_copy reads and writes 1 GiB per dispatch
 VALU_INT_PEAK_TOPS = 39.3  # 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, int32 ops
 VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each
@@ -213,6 +215,24 @@ VALU_WAVE_INSTR_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions/s: 256 CU
 
 def _short(kernel_name):
     return kernel_name.split("(")[0].replace("void ", "").replace("sa::", "").strip()
+
+
+def pmc_stages(kernels):
+    """Per-stage traffic per launch from the per-kernel counter entries: a stage
+    launches once per batch, as often as its most frequent kernel."""
+    stages = {}
+    for stage, ks in STAGE_KERNELS.items():
+        members = [k for k in kernels if k.split("<")[0] in ks]
+        if not members or any("hbm_bytes" not in kernels[k] for k in members):
+            continue
+        launches = max(kernels[k]["dispatches"] for k in members)
+        tot = lambda f: sum(kernels[k][f] * kernels[k]["dispatches"] for k in members) / launches
+        st = {"launches": launches, "kernels": sorted(members), "hbm_bytes_per_launch": int(tot("hbm_bytes")),
+              "dram_read_bytes": int(tot("dram_read_bytes")), "dram_write_bytes": int(tot("dram_write_bytes"))}
+        if all("fetch_size_bytes_scaled" in kernels[k] and "write_size_bytes_scaled" in kernels[k] for k in members):
+            st["fetch_write_size_bytes_scaled"] = int(tot("fetch_size_bytes_scaled") + tot("write_size_bytes_scaled"))
+        stages[stage] = st
+    return stages
 
 
 def summarize_pmc(dbs, workload, batch_bases):
@@ -289,19 +309,7 @@ def summarize_pmc(dbs, workload, batch_bases):
                 e["valu_active_lane_pct"] = 100.0 * thr / (act * 64) if thr is not None else None
         out["kernels"][k] = e
         rows.append((k, e))
-    for stage, ks in STAGE_KERNELS.items():
-        members = [k for k in per if k.split("<")[0] in ks]
-        main = max(members, key=ndisp, default=None)
-        if not main or any("hbm_bytes" not in out["kernels"][k] for k in members):
-            continue
-        launches = ndisp(main)
-        tot = lambda f: sum(out["kernels"][k][f] * out["kernels"][k]["dispatches"] for k in members) / launches
-        st = {"launches": launches, "kernels": sorted(members), "hbm_bytes_per_launch": int(tot("hbm_bytes")),
-              "dram_read_bytes": int(tot("dram_read_bytes")), "dram_write_bytes": int(tot("dram_write_bytes"))}
-        if all("fetch_size_bytes_scaled" in out["kernels"][k] and "write_size_bytes_scaled" in out["kernels"][k]
-               for k in members):
-            st["fetch_write_size_bytes_scaled"] = int(tot("fetch_size_bytes_scaled") + tot("write_size_bytes_scaled"))
-        out["stages"][stage] = st
+    out["stages"] = pmc_stages(out["kernels"])
     lines = ["%-28s %6s %14s %14s %12s %12s %9s %7s" % ("kernel", "disp", "dram_rd_MB", "dram_wr_MB", "hbm_MB",
                                                         "fetchws_MB", "valu_iss", "lanes%")]
     for k, e in rows:
