@@ -5,9 +5,9 @@
 // DP (85-180) and extracts the chains as NAMs (387-448). Here G consecutive
 // lanes of a wave (G = 16: four orientations per wave; G = 64: one) do all of
 // that for one orientation with every intermediate array in LDS:
-//   - hits -> anchors: a lane per hit walks the hit's run of index entries (the
-//     length-difference filter is sequential within a run), a group prefix sum
-//     places its anchors;
+//   - hits -> anchors: a lane per hit walks the hit's run of index entries once
+//     (the length-difference filter is sequential within a run) and writes its
+//     anchors into the slots its capacity reserves, placed by a group prefix sum;
 //   - sort: a bitonic sort of the (ref_id, ref_start, query_start) keys, padded
 //     to a power of two; dedup: a compaction by group prefix sums;
 //   - DP: anchors in order, the lookback window j = i-1, i-2, ... spread over the
@@ -15,8 +15,8 @@
 //     >= the running best, stop after a taken same-diagonal predecessor" rule by a
 //     group prefix maximum and two ballots, so the result equals the scalar loop;
 //   - extraction: candidates by compaction, their stable order by dp by a rank
-//     count, and the greedy backtrack (a pointer chase in LDS) on the group's
-//     first lane.
+//     count, and the greedy backtrack by the whole group, G links at a time
+//     along runs of consecutive anchors.
 // An orientation is handled here when its anchor capacity (2 per entry of a
 // full hit's run, 1 per entry of a partial hit's, computed by the lookups) is
 // at most CAP; the kernel of the first tier lists the larger ones for the next
@@ -193,34 +193,39 @@ SA_HD bool key_less(uint32_t ar, uint32_t as, uint32_t aq, uint32_t br, uint32_t
     return ar < br || (ar == br && (as < bs || (as == bs && aq < bq)));
 }
 
-// Anchors of a hit (src/chainer.rs:304-384) into the group's array at `pos`, or
-// only counted when `write` is false.
+// The key above every anchor: the sort's padding, and the slots of the entries
+// the length filter drops.
+constexpr AnchorL ANCHOR_PAD{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f};
+
+// Anchors of a hit (src/chainer.rs:304-384) into its slots [pos, pos + its
+// capacity) of the group's array: one per entry of a partial hit's run; two per
+// entry of a full hit's that passes the length filter, ANCHOR_PAD in the two
+// slots of an entry it drops. Returns the anchors written.
 template <int CAP>
-__device__ uint32_t grp_hit_anchors(const IndexView& ix, const Hit& h, bool write, uint32_t pos, ChainLds<CAP>& L) {
+__device__ uint32_t grp_hit_anchors(const IndexView& ix, const Hit& h, uint32_t pos, ChainLds<CAP>& L) {
     const uint32_t k = (uint32_t)ix.p.k;
-    uint32_t c = 0;
     if (h.flags & HIT_PARTIAL) {
-        if (write)
-            for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-                const RefEntry x = ix.e[j];
-                L.a[pos + c++] = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
-            }
+        for (uint32_t e = 0; e < h.run_len; ++e) {
+            const RefEntry x = ix.e[h.run_start + e];
+            L.a[pos + e] = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
+        }
         return h.run_len;
     }
-    uint32_t min_diff = 0xFFFFFFFFu;
+    uint32_t min_diff = 0xFFFFFFFFu, c = 0;
     const uint32_t ql = h.query_end - h.query_start;
-    for (uint64_t j = h.run_start; j < h.run_start + h.run_len; ++j) {
-        const RefEntry x = ix.e[j];
+    for (uint32_t e = 0; e < h.run_len; ++e) {
+        const RefEntry x = ix.e[h.run_start + e];
         const uint32_t rl = (uint32_t)(x.hash_offset & 0xFF) + k;
         const uint32_t diff = ql > rl ? ql - rl : rl - ql;
+        AnchorL a0 = ANCHOR_PAD, a1 = ANCHOR_PAD;
         if (diff <= min_diff) {
-            if (write) {
-                L.a[pos + c] = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
-                L.a[pos + c + 1] = AnchorL{x.ref_index, x.position + rl - k, h.query_end - k, 0.0f};
-            }
+            a0 = AnchorL{x.ref_index, x.position, h.query_start, 0.0f};
+            a1 = AnchorL{x.ref_index, x.position + rl - k, h.query_end - k, 0.0f};
             c += 2;
             min_diff = diff;
         }
+        L.a[pos + 2 * e] = a0;
+        L.a[pos + 2 * e + 1] = a1;
     }
     return c;
 }
@@ -246,33 +251,42 @@ __device__ void chain_grp(const Grp<G>& gp, ChainLds<CAP>& L, uint32_t t, const 
     const uint64_t ab = aoff[t];
     const uint32_t k = (uint32_t)ix.p.k;
 
-    // 1. hits -> anchors (unsorted; any order sorts to the same list)
-    uint32_t na = 0, entries = 0;
+    // 1. hits -> anchors (unsorted; any order sorts to the same list), each
+    // hit's in the slots its capacity reserves (2 per entry of a full hit's run,
+    // 1 per entry of a partial hit's, the capacities the lookups summed), placed
+    // by a group prefix sum of the capacities: each entry is read once
+    uint32_t na = 0, entries = 0, ncap = 0;
     for (uint32_t h0 = 0; h0 < nh; h0 += G) {
         const uint32_t h = h0 + r;
         Hit hh;
-        uint32_t c = 0;
         bool has = false;
         if (h < nh) {
             hh = H[h];
             has = hh.run_start != NONE64;
         }
+        const uint32_t c = has ? ((hh.flags & HIT_PARTIAL) ? hh.run_len : 2 * hh.run_len) : 0u;
+        const uint32_t pos = ncap + gp.excl_sum(c);
         if (has) {
             entries += hh.run_len;
-            c = grp_hit_anchors<CAP>(ix, hh, false, 0, L);
+            if (pos + c <= CAP) {
+                na += grp_hit_anchors<CAP>(ix, hh, pos, L);
+            } else {
+                // beyond the tier's capacity bound, which the lookups' sum of
+                // the same capacities keeps the orientation within
+                for (uint32_t x = pos; x < CAP; ++x) L.a[x] = ANCHOR_PAD;
+            }
         }
-        const uint32_t pos = na + gp.excl_sum(c);
-        if (has && c && pos + c <= CAP) grp_hit_anchors<CAP>(ix, hh, true, pos, L);
-        na += gp.sum(c);
+        ncap += gp.sum(c);
     }
     entries = gp.sum(entries);
-    if (na > CAP) na = CAP;  // capacity bound: cannot be exceeded (2 per entry at most)
-    na = gp.uni(na);
+    na = gp.uni(gp.sum(na));
+    ncap = gp.uni(ncap < (uint32_t)CAP ? ncap : (uint32_t)CAP);
 
-    // 2. bitonic sort, padded to a power of two with keys above every anchor
+    // 2. bitonic sort of the capacity slots, padded to a power of two: the
+    // anchors come first, the padding of dropped entries after them
     uint32_t P = 1;
-    while (P < na) P <<= 1;
-    for (uint32_t x = na + r; x < P; x += G) L.a[x] = AnchorL{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0.0f};
+    while (P < ncap) P <<= 1;
+    for (uint32_t x = ncap + r; x < P; x += G) L.a[x] = ANCHOR_PAD;
     lds_fence();
     for (uint32_t k2 = 2; k2 <= P; k2 <<= 1) {
         for (uint32_t j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
