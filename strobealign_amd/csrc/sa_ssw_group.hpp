@@ -161,13 +161,23 @@ __device__ __forceinline__ SswEnd ssw_pass_seg(const QueryView& q, int32_t ql, b
 #pragma unroll
     for (int w = 0; w < W; ++w) prof[0][w] = prof[1][w] = prof[2][w] = prof[3][w] = 0;
     const uint32_t mt = (uint32_t)(sc.match & 0xFF), mm = (uint32_t)((-sc.mismatch) & 0xFF);
+    // the lane's query bytes, all loads issued before the first is used
+    // (positions past the query end read a byte inside it and are not used);
+    // query position of the pass -> read position, read as given or reverse
+    // complemented
+    uint8_t qb[SEG];
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+        const int32_t qi = j + (int32_t)l * SEG;
+        const int32_t qc = qi < ql ? qi : (ql > 0 ? ql - 1 : 0);
+        const uint32_t t = (uint32_t)(rev ? qend - qc : qc);
+        qb[j] = q.seq[q.rc ? q.len - 1 - t : t];
+    }
 #pragma unroll
     for (int j = 0; j < SEG; ++j) {
         const int32_t qi = j + (int32_t)l * SEG;
         if (qi < ql) {
-            // query position of the pass -> read position, read as given or reverse complemented
-            const uint32_t t = (uint32_t)(rev ? qend - qi : qi);
-            const int32_t c = q.rc ? ssw_code_comp(q.seq[q.len - 1 - t]) : ssw_code(q.seq[t]);
+            const int32_t c = q.rc ? ssw_code_comp(qb[j]) : ssw_code(qb[j]);
 #pragma unroll
             for (int b = 0; b < 4; ++b) prof[b][j >> 2] |= (c == b ? mt : mm) << (8 * (j & 3));
         }
