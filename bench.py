@@ -395,23 +395,26 @@ def kernel_report(ctx, kd, kd_steps, kd_source, run_batch, workload):
                 tb = int(tb * run_batch / prof_batch)
             lookup.update({"traffic_bytes_per_launch": tb, "traffic_gbps": round(tb / (avg / 1000.0) / 1e9, 1),
                            "traffic_source": "carried from profiles/pmc_traffic.json"})
-    # forward SSW passes: DP cells per second against the VALU integer peak
-    # (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; 10 ops per
-    # cell of the striped recurrence: profile add, H from three maxima, the
-    # column maximum, E and F each two subtractions and a maximum)
+    # SSW score passes: DP cells computed (query rows x the columns each pass ran,
+    # counted on the device) per second against the VALU integer peak (256 CUs x
+    # 4 SIMDs x 16 lanes x 2.4 GHz = 39.3 T int32 ops/s; 10 ops per cell of the
+    # striped recurrence: profile add, H from three maxima, the column maximum,
+    # E and F each two subtractions and a maximum)
     def ssw_rate(name):
         ms_, _, _, _, cells = kd.get(name, z)
         if not cells or not ms_:
             return None
         gcups = cells / (ms_ / 1000.0) / 1e9
-        return {"kernel": name, "forward_cells_per_step": int(cells / kd_steps), "gcups": round(gcups, 1),
+        return {"kernel": name, "cells_per_step": int(cells / kd_steps), "gcups": round(gcups, 1),
                 "ops_per_cell": 10, "valu_int_peak_tops": VALU_INT_PEAK_TOPS,
                 "frac": round(gcups * 10 / (VALU_INT_PEAK_TOPS * 1000.0), 4)}
     extension = ssw_rate("k_ssw_byte_fwd")
     if extension is not None:
-        word = ssw_rate("k_ssw_word_fwd")
-        if word is not None:
-            extension["word"] = word
+        for key, name in (("word", "k_ssw_word_fwd"), ("byte_reverse", "k_ssw_byte_rev"),
+                          ("word_reverse", "k_ssw_word_rev")):
+            rate = ssw_rate(name)
+            if rate is not None:
+                extension[key] = rate
         # VALU issue of the score passes by their counters (SQ_INSTS_VALU over
         # the kernel's duration), from the committed profile of this workload
         if tj.get("workload") == workload:

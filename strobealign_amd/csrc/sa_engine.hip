@@ -242,9 +242,13 @@ constexpr uint32_t F_LONG_SLOTS = 2048;    // one-thread SSW of long queries, at
 constexpr uint64_t F_FIN_BUDGET = 2ull << 30;   // bytes of traceback scratch, at most
 constexpr uint64_t F_LONG_BUDGET = 2ull << 30;  // bytes of one-thread SSW scratch, at most
 constexpr uint64_t F_PW_BUDGET = 4ull << 30;    // bytes of piecewise scratch
-constexpr uint32_t EXT_NCNT = 4 + SSW_BUCKETS + 2 + FIN_CLASSES + 2;
-constexpr uint32_t EXT_WCELLS = 4 + SSW_BUCKETS + 2 + FIN_CLASSES;  // word index of the u64 word-pass cells
-static_assert(EXT_WCELLS % 2 == 0, "word-pass cell counter alignment");
+// extension counters (32-bit words): [0..3] prep (ssw, long, pw jobs; one unused),
+// the SSW buckets, four 64-bit DP-cell counts (byte forward, byte reverse, word
+// forward, word reverse), then the traceback classes
+constexpr uint32_t EXT_CELLS = 4 + SSW_BUCKETS;
+constexpr uint32_t EXT_FIN = EXT_CELLS + 8;
+constexpr uint32_t EXT_NCNT = EXT_FIN + FIN_CLASSES;
+static_assert(EXT_CELLS % 2 == 0, "cell counter alignment");
 uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 }  // namespace
 
@@ -1138,17 +1142,17 @@ template <int G, bool BYTE, int S, int MAX>
 void launch_ssw_pass(uint32_t seg, const SswLaunchLists& ll, hipStream_t st, const uint8_t* seq, const uint64_t* off,
                      RefView ref, const Scores& sc, const ExtRequest* req, const ExtJob* jobs, uint32_t rev,
                      SswEnd* fwd, SswEnds* ends, uint32_t* lists, uint32_t* counts, uint32_t cap,
-                     unsigned long long* wcells, uint32_t dev_grid = 0) {
+                     unsigned long long* cells, uint32_t dev_grid = 0) {
     if constexpr (S > MAX) {
         throw std::logic_error("SSW stripe count out of range");
     } else if (seg != S) {
         launch_ssw_pass<G, BYTE, S + 1, MAX>(seg, ll, st, seq, off, ref, sc, req, jobs, rev, fwd, ends, lists, counts,
-                                             cap, wcells, dev_grid);
+                                             cap, cells, dev_grid);
     } else {
         const uint64_t count = ll.start[SSW_WCLASSES];
         const uint32_t grid = dev_grid ? dev_grid : blocks(count * G, 256);
         k_ssw_pass<G, BYTE, S><<<grid, 256, 0, st>>>(seq, off, ref, sc, req, jobs, ll, rev, fwd, ends, lists, counts,
-                                                     cap, dev_grid ? 1u : 0u, wcells);
+                                                     cap, dev_grid ? 1u : 0u, cells);
     }
 }
 }  // namespace
@@ -1183,11 +1187,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     auto* d_jobs = (ExtJob*)grow(S_JOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_ljobs = (ExtJob*)grow(S_LJOBS, (uint64_t)n * sizeof(ExtJob));
     auto* d_pjobs = (ExtJob*)grow(S_PJOBS, (uint64_t)n * sizeof(ExtJob));
-    // counters: [0..3] prep (ssw, long, pw jobs; word queue unused), then the SSW buckets,
-    // then (64-bit, at word 4 + SSW_BUCKETS) the DP cells of the forward byte passes,
-    // the traceback classes, and (64-bit) the DP cells of the forward word passes
     constexpr uint32_t NCNT = EXT_NCNT;
-    static_assert((4 + SSW_BUCKETS) % 2 == 0, "cell counter alignment");
     auto* d_cnt = (uint32_t*)grow(S_XCNT, NCNT * 4);
     uint32_t* d_bcnt = d_cnt + 4;
     // SSW bucket lists sized for every request (the job count is known only on the device)
@@ -1224,14 +1224,11 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         k_ext_prep<<<blocks((uint64_t)n * 64, 256), 256, 0, stream_>>>(d_seq, d_off, ref, sc, k, d_req, n, nullptr,
                                                                       d_res, d_slots, d_jobs, d_ljobs, d_pjobs, d_cnt);
     }, prep_bytes);
-    auto* d_wcells = (unsigned long long*)(d_cnt + EXT_WCELLS);
-    k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
-                                                       (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS), d_wcells);
+    auto* d_cells = (unsigned long long*)(d_cnt + EXT_CELLS);
+    k_ssw_bucket<<<blocks(n, 256), 256, 0, stream_>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap);
     SA_CHECK(hipGetLastError());
     read_counts();
     const uint32_t nssw = cnt[0], nlong = cnt[1], npw = cnt[2];
-    uint64_t fwd_cells = 0;
-    std::memcpy(&fwd_cells, cnt + 4 + SSW_BUCKETS, 8);
     const uint32_t* bc = cnt + 4;
     // thread-per-job kernels run in waves sized to the scratch budget
     const uint64_t budget = 16ull << 30;
@@ -1286,25 +1283,22 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
                 if (!at) continue;
                 if (word)
                     launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
-                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_wcells);
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_cells);
                 else
                     launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, stream_, d_seq, d_off, ref, sc, d_req, d_jobs, rev,
-                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_wcells);
+                                                     d_fwd, d_ends, d_lists, d_bcnt, cap, d_cells);
                 SA_CHECK(hipGetLastError());
             }
         };
-        timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); }, 0, fwd_cells);
+        timed("k_ssw_byte_fwd", nssw, [&] { run(false, false); });
         read_counts();
-        // the word forward passes' cells: word-first jobs and the byte passes' saturated ones
-        uint64_t word_cells = 0;
-        std::memcpy(&word_cells, cnt + EXT_WCELLS, 8);
         uint64_t nrb = 0, nfw = 0;
         for (uint32_t x = 0; x < SSW_BYTE_SEGS; ++x) nrb += bc[ssw_bucket(false, true, x + 1, 0)];
         for (uint32_t x = 0; x < SSW_WORD_SEGS; ++x)
             for (uint32_t wc = 0; wc < SSW_WCLASSES; ++wc) nfw += bc[ssw_bucket(true, false, x + 1, wc)];
         if (nrb) timed("k_ssw_byte_rev", nrb, [&] { run(false, true); });
         if (nfw) {
-            timed("k_ssw_word_fwd", nfw, [&] { run(true, false); }, 0, word_cells);
+            timed("k_ssw_word_fwd", nfw, [&] { run(true, false); });
             read_counts();
             timed("k_ssw_word_rev", nfw, [&] { run(true, true); });
         }
@@ -1314,7 +1308,7 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
         per = (per + 255) & ~255ull;
         // jobs grouped by the band width their traceback starts with
         auto* d_fin = (uint32_t*)grow(S_FINLISTS, (uint64_t)FIN_CLASSES * nssw * 4);
-        uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
+        uint32_t* d_fcnt = d_cnt + EXT_FIN;
         k_fin_bucket<<<blocks(nssw, 256), 256, 0, stream_>>>(d_ends, nssw, nullptr, d_fin, d_fcnt, nssw);
         SA_CHECK(hipGetLastError());
         timed("k_ssw_finish", nssw, [&] {
@@ -1372,7 +1366,17 @@ void Engine::extend(const ExtRequest* req_in, uint32_t n, const Scores& sc, std:
     }
     SA_CHECK(hipMemcpyAsync(res.data(), d_res, (uint64_t)n * sizeof(ExtResult), hipMemcpyDeviceToHost, stream_));
     SA_CHECK(hipMemcpyAsync(cigar_off.data(), d_coff, (uint64_t)(n + 1) * 8, hipMemcpyDeviceToHost, stream_));
+    SA_CHECK(hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, stream_));
     sync();
+    // the DP cells each kind of score pass computed
+    {
+        uint64_t cells[4];
+        std::memcpy(cells, cnt + EXT_CELLS, sizeof cells);
+        static const char* const kind[4] = {"k_ssw_byte_fwd", "k_ssw_byte_rev", "k_ssw_word_fwd", "k_ssw_word_rev"};
+        for (Timed& x : tl)
+            for (int c = 0; c < 4; ++c)
+                if (std::strcmp(x.name, kind[c]) == 0) x.cells = cells[c];
+    }
     float ext_ms = 0;
     for (Timed& x : tl) {
         float t = 0;
@@ -1498,9 +1502,7 @@ void Engine::fused_sequence(Fused& f) {
     // otherwise
     const bool ssw = ssw_jobs;
     if (ssw) {
-        k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap,
-                                           (unsigned long long*)(d_cnt + 4 + SSW_BUCKETS),
-                                           (unsigned long long*)(d_cnt + EXT_WCELLS));
+        k_ssw_bucket<<<1024, 256, 0, st>>>(d_off, d_req, d_jobs, d_cnt, d_lists, d_bcnt, cap);
         SA_CHECK(hipGetLastError());
         // every stripe count a job of this shape can have, byte then word,
         // forward then reverse (a pass only adds jobs to the buckets of later
@@ -1522,12 +1524,12 @@ void Engine::fused_sequence(Fused& f) {
                     if (word)
                         launch_ssw_pass<8, false, 1, SSW_WORD_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
                                                                     d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
-                                                                    cap, (unsigned long long*)(d_cnt + EXT_WCELLS),
+                                                                    cap, (unsigned long long*)(d_cnt + EXT_CELLS),
                                                                     F_SSW_BLOCKS);
                     else
                         launch_ssw_pass<16, true, 1, SSW_BYTE_SEGS>(seg, ll, st, d_seq, d_off, ref, sc, d_req,
                                                                     d_jobs, rev, d_fwd, d_ends, d_lists, d_bcnt,
-                                                                    cap, (unsigned long long*)(d_cnt + EXT_WCELLS),
+                                                                    cap, (unsigned long long*)(d_cnt + EXT_CELLS),
                                                                     F_SSW_BLOCKS);
                     SA_CHECK(hipGetLastError());
                 }
@@ -1543,7 +1545,7 @@ void Engine::fused_sequence(Fused& f) {
         const uint32_t fin_slots =
             (uint32_t)std::max<uint64_t>(128, std::min<uint64_t>(F_FIN_SLOTS, F_FIN_BUDGET / per) / 128 * 128);
         auto* d_fin = (uint32_t*)take(S_FINLISTS, (uint64_t)FIN_CLASSES * cap * 4);
-        uint32_t* d_fcnt = d_cnt + 4 + SSW_BUCKETS + 2;
+        uint32_t* d_fcnt = d_cnt + EXT_FIN;
         k_fin_bucket<<<256, 256, 0, st>>>(d_ends, 0, d_cnt, d_fin, d_fcnt, cap);
         auto* fscr = (uint8_t*)take(S_SCRATCH, (uint64_t)fin_slots * per);
         k_ssw_finish<<<fin_slots / 128, 128, 0, st>>>(d_seq, d_off, ref, sc, d_req, d_jobs, d_ends, 0, d_cnt, d_res,

@@ -616,9 +616,7 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
                                                    const ExtRequest* __restrict__ req,
                                                    const ExtJob* __restrict__ jobs, const uint32_t* __restrict__ counters,
                                                    uint32_t* __restrict__ lists, uint32_t* __restrict__ counts,
-                                                   uint32_t cap, unsigned long long* __restrict__ cells,
-                                                   unsigned long long* __restrict__ wcells) {
-    uint64_t c = 0, cw = 0;
+                                                   uint32_t cap) {
     const uint32_t n = counters[0];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const uint32_t rid = req[jobs[t].req].read_id;
@@ -626,17 +624,7 @@ __global__ void __launch_bounds__(256) k_ssw_bucket(const uint64_t* __restrict__
         const bool word = (jobs[t].flags & SSW_WORD_FIRST) != 0;
         const uint32_t b = ssw_bucket(word, false, word ? (L + 7) / 8 : (L + 15) / 16, ssw_wclass(jobs[t].wlen));
         lists[(uint64_t)b * cap + atomicAdd(&counts[b], 1u)] = t;
-        // cells of the byte forward pass, or of the word forward pass for word-first jobs
-        (word ? cw : c) += (uint64_t)L * jobs[t].wlen;
     }
-    // one atomic per wave
-    SA_UNROLL
-    for (int s = 32; s > 0; s >>= 1) {
-        c += __shfl_xor(c, s);
-        cw += __shfl_xor(cw, s);
-    }
-    if ((threadIdx.x & 63u) == 0 && c) atomicAdd(cells, (unsigned long long)c);
-    if ((threadIdx.x & 63u) == 0 && cw) atomicAdd(wcells, (unsigned long long)cw);
 }
 
 // One striped score pass (ext/ssw/ssw.c:197-588) per group of G lanes: the
@@ -662,7 +650,11 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
                                     const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
                                     const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
                                     SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                    uint32_t* __restrict__ counts, uint32_t cap, unsigned long long* __restrict__ wcells);
+                                    uint32_t* __restrict__ counts, uint32_t cap, uint64_t& cells);
+
+// DP cells a pass computed (query rows x columns run), one counter per pass kind:
+// [byte forward, byte reverse, word forward, word reverse]
+__device__ __forceinline__ uint32_t ssw_cell_kind(bool byte, uint32_t rev) { return (byte ? 0u : 2u) + (rev ? 1u : 0u); }
 
 template <int G, bool BYTE, int SEG>
 __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ seq, const uint64_t* __restrict__ off,
@@ -671,7 +663,7 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
                                                  uint32_t rev, SswEnd* __restrict__ fwd,
                                                  SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
                                                  uint32_t* __restrict__ counts, uint32_t cap, uint32_t dev_counts,
-                                                 unsigned long long* __restrict__ wcells) {
+                                                 unsigned long long* __restrict__ cells) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t l = tid % G;
     uint32_t stride = 0xFFFFFFFFu;
@@ -687,14 +679,19 @@ __global__ void __launch_bounds__(256) k_ssw_pass(const uint8_t* __restrict__ se
         stride = gridDim.x * blockDim.x / G;
     }
     const uint32_t total = ll.start[SSW_WCLASSES];
+    uint64_t my_cells = 0;
     for (uint32_t slot = tid / G; slot < total; slot += stride) {
         uint32_t c = 0;
         SA_UNROLL
         for (uint32_t x = 1; x < SSW_WCLASSES; ++x) c += slot >= ll.start[x] ? 1u : 0u;
         ssw_pass_job<G, BYTE, SEG>(ll.list[c][slot - ll.start[c]], l, seq, off, ref, sc, req, jobs, rev, fwd, ends,
-                                   lists, counts, cap, wcells);
+                                   lists, counts, cap, my_cells);
         if (stride == 0xFFFFFFFFu) break;
     }
+    // every lane is here: one atomic per wave
+    SA_UNROLL
+    for (int s = 32; s > 0; s >>= 1) my_cells += __shfl_xor(my_cells, s);
+    if ((threadIdx.x & 63u) == 0 && my_cells) atomicAdd(cells + ssw_cell_kind(BYTE, rev), (unsigned long long)my_cells);
 }
 
 template <int G, bool BYTE, int SEG>
@@ -702,7 +699,7 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
                                     const uint64_t* __restrict__ off, const RefView& ref, const Scores& sc,
                                     const ExtRequest* __restrict__ req, const ExtJob* __restrict__ jobs, uint32_t rev,
                                     SswEnd* __restrict__ fwd, SswEnds* __restrict__ ends, uint32_t* __restrict__ lists,
-                                    uint32_t* __restrict__ counts, uint32_t cap, unsigned long long* __restrict__ wcells) {
+                                    uint32_t* __restrict__ counts, uint32_t cap, uint64_t& cells) {
     const ExtJob j = jobs[job];
     const ExtRequest r = req[j.req];
     const uint64_t ro = off[r.read_id];
@@ -710,14 +707,15 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
     QueryView q{seq + ro, (uint32_t)L, (r.flags & 4) != 0};
     RefCursor w{ref.words + ref.word_off[j.ref_id], j.ws, -1, 0};
     const uint32_t gbase = (__lane_id() & 63u) & ~(uint32_t)(G - 1);
+    int32_t cols = 0;
     if (!rev) {
         const SswEnd f = ssw_pass_seg<G, BYTE, SEG>(q, L, false, 0, w, (int32_t)j.wlen, sc, BYTE ? 255 : 65535, l,
-                                                    gbase);
+                                                    gbase, cols);
         if (l == 0) {
+            cells += (uint64_t)L * (uint32_t)cols;
             uint32_t b;
             if (BYTE && f.score == 255) {
                 b = ssw_bucket(true, false, (uint32_t)(L + 7) / 8, ssw_wclass(j.wlen));
-                atomicAdd(wcells, (unsigned long long)L * j.wlen);  // cells of its word forward pass
             } else {
                 fwd[job] = f;
                 b = ssw_bucket(!BYTE, true, (uint32_t)(f.read + 1 + G - 1) / G, 0);
@@ -726,8 +724,12 @@ __device__ inline void ssw_pass_job(uint32_t job, uint32_t l, const uint8_t* __r
         }
     } else {
         const SswEnd f = fwd[job];
-        const SswEnd b = ssw_pass_seg<G, BYTE, SEG>(q, f.read + 1, true, f.read, w, f.ref + 1, sc, f.score, l, gbase);
-        if (l == 0) ends[job] = SswEnds{f.score, f.ref, f.read, b.ref, f.read - b.read, f.score > b.score};
+        const SswEnd b = ssw_pass_seg<G, BYTE, SEG>(q, f.read + 1, true, f.read, w, f.ref + 1, sc, f.score, l, gbase,
+                                                    cols);
+        if (l == 0) {
+            cells += (uint64_t)(f.read + 1) * (uint32_t)cols;
+            ends[job] = SswEnds{f.score, f.ref, f.read, b.ref, f.read - b.read, f.score > b.score};
+        }
     }
 }
 

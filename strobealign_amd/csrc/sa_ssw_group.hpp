@@ -77,7 +77,8 @@ struct RefCursor {
 // derives from that copy once the pass ends.
 template <int G, bool BYTE, int SEG>
 __device__ SswEnd ssw_columns(const uint32_t (&prof)[4][(SEG + 3) / 4], int32_t qlen, RefCursor rc_cur, int32_t rlen,
-                              bool reverse, const Scores& sc, int32_t terminate, uint32_t l, uint32_t gbase) {
+                              bool reverse, const Scores& sc, int32_t terminate, uint32_t l, uint32_t gbase,
+                              int32_t& cols) {
     constexpr int W = (SEG + 3) / 4;
     const int32_t bias = BYTE ? sc.mismatch : 0;
     const int32_t gapO = sc.gap_open, gapE = sc.gap_extend;
@@ -88,7 +89,9 @@ __device__ SswEnd ssw_columns(const uint32_t (&prof)[4][(SEG + 3) / 4], int32_t 
     int32_t maxv = 0, end_ref = BYTE ? -1 : 0;
     int32_t end_read = 0;  // smallest query position holding maxv in the best column
     const int32_t begin = reverse ? rlen - 1 : 0, end = reverse ? -1 : rlen, step = reverse ? -1 : 1;
+    cols = 0;  // columns computed (the pass may stop before the window's end)
     for (int32_t i = begin; i != end; i += step) {
+        ++cols;
         int32_t vH = grp_shift_up<G>(H[SEG - 1], l);
         int32_t vF = 0, maxcol = 0;
         const int32_t rc = rc_cur.at(i);
@@ -155,7 +158,7 @@ __device__ SswEnd ssw_columns(const uint32_t (&prof)[4][(SEG + 3) / 4], int32_t 
 // Builds the lane's profile for SEG segments and runs the column loop.
 template <int G, bool BYTE, int SEG>
 __device__ __forceinline__ SswEnd ssw_pass_seg(const QueryView& q, int32_t ql, bool rev, int32_t qend, RefCursor rc_cur, int32_t rl,
-                               const Scores& sc, int32_t term, uint32_t l, uint32_t gbase) {
+                               const Scores& sc, int32_t term, uint32_t l, uint32_t gbase, int32_t& cols) {
     constexpr int W = (SEG + 3) / 4;
     uint32_t prof[4][W];
 #pragma unroll
@@ -182,7 +185,7 @@ __device__ __forceinline__ SswEnd ssw_pass_seg(const QueryView& q, int32_t ql, b
             for (int b = 0; b < 4; ++b) prof[b][j >> 2] |= (c == b ? mt : mm) << (8 * (j & 3));
         }
     }
-    return ssw_columns<G, BYTE, SEG>(prof, ql, rc_cur, rl, rev, sc, term, l, gbase);
+    return ssw_columns<G, BYTE, SEG>(prof, ql, rc_cur, rl, rev, sc, term, l, gbase, cols);
 }
 
 }  // namespace sa
