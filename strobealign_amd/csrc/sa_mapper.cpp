@@ -237,7 +237,11 @@ private:
 struct CachedResult {
     ExtResult r;
     uint64_t cigar_off;  // into Chunk::cigar_pool
-    uint32_t spec = 0;   // from the fused pass: 1 not looked up yet, 2 looked up
+};
+// A result a decision found: the result and its CIGAR words (r null: a miss)
+struct ExtHit {
+    const ExtResult* r = nullptr;
+    const uint32_t* cigar = nullptr;
 };
 
 // Chunk output buffers kept for reuse: a fresh multi-megabyte string per chunk
@@ -312,6 +316,16 @@ struct Chunk {
     std::vector<double> abundances;
     KeyTable<CachedResult> cache;
     std::vector<uint32_t> cigar_pool;  // CIGARs of the cached results (grows only between passes)
+    // the fused pass's results for the chunk's reads, looked up where they lie
+    // (its requests are sorted by read): those of read spec_r0 + x are
+    // spec_req[spec_at[x] .. spec_at[x + 1]) with their results and CIGARs
+    const ExtRequest* spec_req = nullptr;
+    const ExtResult* spec_res = nullptr;
+    const uint64_t* spec_cigoff = nullptr;
+    const uint32_t* spec_cig = nullptr;
+    uint64_t spec_r0 = 0;
+    std::vector<uint32_t> spec_at;
+    std::vector<uint8_t> spec_seen;  // looked up by a decision (counted once)
     std::vector<Key> pending;
     KeyTable<char> pending_set;
     uint64_t spec_used = 0;  // fused-pass results the decisions looked up
@@ -342,20 +356,33 @@ struct Ctx {
     InsertSize* isize;
 };
 
-// Look up an extension result. A missing one is registered as pending and
-// reported as no alignment; the record then runs on as a probe that collects
-// its other requests, and is not committed in this pass.
-const CachedResult* lookup(Ctx& c, const Key& key) {
-    if (CachedResult* hit = c.ch->cache.find(key)) {
-        if (hit->spec == 1) {
-            hit->spec = 2;
-            c.ch->spec_used++;
+// Look up an extension result: a seed extension first among the fused pass's
+// results for its read (a few requests, read in order as the records are), then
+// in the chunk's cache of requested ones. A missing one is registered as
+// pending and reported as no alignment; the record then runs on as a probe that
+// collects its other requests, and is not committed in this pass. (A fused-pass
+// result that failed is not taken: the decision requests it again.)
+ExtHit lookup(Ctx& c, const Key& key) {
+    Chunk& ch = *c.ch;
+    if (key.kind == EXT_SEED && ch.spec_req && key.read >= ch.spec_r0 && key.read - ch.spec_r0 + 1 < ch.spec_at.size()) {
+        const uint64_t x = key.read - ch.spec_r0;
+        for (uint32_t y = ch.spec_at[x]; y < ch.spec_at[x + 1]; ++y) {
+            const ExtRequest& q = ch.spec_req[y];
+            const uint64_t e = (q.flags & 1) ? 0 : (q.anchor_off | (uint64_t)q.n_anchors << 48);
+            if (q.flags == key.flags && q.ref_id == key.ref_id && q.ref_start == key.a && q.ref_end == key.b &&
+                q.query_start == key.c && q.query_end == key.d && e == key.e && ch.spec_res[y].status == 0) {
+                if (!ch.spec_seen[y]) {
+                    ch.spec_seen[y] = 1;
+                    ch.spec_used++;
+                }
+                return ExtHit{&ch.spec_res[y], ch.spec_cig + ch.spec_cigoff[y]};
+            }
         }
-        return hit;
     }
-    if (c.ch->pending_set.emplace(key, 1)) c.ch->pending.push_back(key);
+    if (const CachedResult* hit = ch.cache.find(key)) return ExtHit{&hit->r, ch.cigar_pool.data() + hit->cigar_off};
+    if (ch.pending_set.emplace(key, 1)) ch.pending.push_back(key);
     c.missed = true;
-    return nullptr;
+    return ExtHit{};
 }
 
 // src/mapper.rs:532-620 (device request kind EXT_SEED)
@@ -364,20 +391,21 @@ std::optional<Alignment> extend_seed(Ctx& c, const Nam& nam, uint32_t read_id, u
     Key key{EXT_SEED, read_id, (use_ssw ? 1u : 0u) | (consistent ? 2u : 0u) | (nam.is_revcomp ? 4u : 0u), nam.ref_id,
             nam.ref_start, nam.ref_end, nam.query_start, nam.query_end, use_ssw ? 0 : nam.anchor_off};
     if (!use_ssw) key.e |= (uint64_t)nam.n_anchors << 48;
-    const CachedResult* r = lookup(c, key);
-    if (!r || !r->r.found) return std::nullopt;
+    const ExtHit h = lookup(c, key);
+    if (!h.r || !h.r->found) return std::nullopt;
+    const ExtResult& r = *h.r;
     Alignment a;
     a.ref_id = nam.ref_id;
-    a.ref_start = r->r.ref_start;
-    a.cigar = c.ch->cigar_pool.data() + r->cigar_off;
-    a.n_cigar = r->r.found ? r->r.n_cigar : 0;
-    a.edit_distance = r->r.edit_distance;
-    a.soft_clip_left = r->r.query_start;
-    a.soft_clip_right = L - r->r.query_end;
-    a.score = r->r.score;
-    a.length = r->r.length;
+    a.ref_start = r.ref_start;
+    a.cigar = h.cigar;
+    a.n_cigar = r.n_cigar;
+    a.edit_distance = r.edit_distance;
+    a.soft_clip_left = r.query_start;
+    a.soft_clip_right = L - r.query_end;
+    a.score = r.score;
+    a.length = r.length;
     a.is_revcomp = nam.is_revcomp;
-    a.gapped = r->r.gapped != 0;
+    a.gapped = r.gapped != 0;
     return a;
 }
 
@@ -403,18 +431,19 @@ std::optional<Alignment> rescue_align(Ctx& c, const Nam& mate, uint32_t read_id,
     re = std::min(re, rl);
     if (re < rs + c.k) return std::nullopt;
     Key key{EXT_RESCUE, read_id, rc ? 4u : 0u, mate.ref_id, (uint32_t)rs, (uint32_t)re, 0, 0, 0};
-    const CachedResult* r = lookup(c, key);
-    if (!r || !r->r.found) return std::nullopt;
+    const ExtHit h = lookup(c, key);
+    if (!h.r || !h.r->found) return std::nullopt;
+    const ExtResult& r = *h.r;
     Alignment a;
     a.ref_id = mate.ref_id;
-    a.ref_start = r->r.ref_start;
-    a.cigar = c.ch->cigar_pool.data() + r->cigar_off;
-    a.n_cigar = r->r.found ? r->r.n_cigar : 0;
-    a.edit_distance = r->r.edit_distance;
-    a.soft_clip_left = r->r.query_start;
-    a.soft_clip_right = L - r->r.query_end;
-    a.score = r->r.score;
-    a.length = r->r.length;
+    a.ref_start = r.ref_start;
+    a.cigar = h.cigar;
+    a.n_cigar = r.n_cigar;
+    a.edit_distance = r.edit_distance;
+    a.soft_clip_left = r.query_start;
+    a.soft_clip_right = L - r.query_end;
+    a.score = r.score;
+    a.length = r.length;
     a.is_revcomp = !mate.is_revcomp;
     a.gapped = true;
     return a;
@@ -1579,9 +1608,9 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
     if (fused) {
         st.fused_batches++;
         st.spec_requests += spec.req.size();
-        // the fused pass's results into the chunks' caches, under the keys the
-        // decisions look them up by (extend_seed); results that failed are left
-        // for the decisions to request again
+        // each chunk's view of the fused pass's results (they stay in the
+        // engine's staging memory until its next batch): the requests of each of
+        // its reads, found by one walk over the requests, which are sorted by read
         const uint64_t per = paired ? 2 : 1;
         parallel_for(o.threads, nch, [&](size_t x) {
             Chunk& ch = *chs[x];
@@ -1590,21 +1619,19 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
             const ExtRequest* end = spec.req.end();
             const ExtRequest* lo = std::lower_bound((const ExtRequest*)spec.req.begin(), end, r0, by_read);
             const ExtRequest* hi = std::lower_bound(lo, end, r1, by_read);
-            ch.cache.reserve(ch.cache.size() + (size_t)(hi - lo));
-            for (const ExtRequest* q = lo; q != hi; ++q) {
-                const uint64_t y = (uint64_t)(q - spec.req.begin());
-                if (spec.res[y].status != 0) continue;
-                Key key{EXT_SEED, q->read_id, q->flags, q->ref_id, q->ref_start, q->ref_end, q->query_start,
-                        q->query_end, 0};
-                if (!(q->flags & 1)) key.e = q->anchor_off | (uint64_t)q->n_anchors << 48;
-                CachedResult cr;
-                cr.r = spec.res[y];
-                cr.cigar_off = ch.cigar_pool.size();
-                cr.spec = 1;
-                ch.cigar_pool.insert(ch.cigar_pool.end(), spec.cigars.begin() + spec.cigar_off[y],
-                                     spec.cigars.begin() + spec.cigar_off[y + 1]);
-                ch.cache.emplace(key, cr);
+            const uint64_t y0 = (uint64_t)(lo - spec.req.begin());
+            ch.spec_req = lo;
+            ch.spec_res = spec.res.begin() + y0;
+            ch.spec_cigoff = spec.cigar_off.begin() + y0;
+            ch.spec_cig = spec.cigars.begin();
+            ch.spec_r0 = r0;
+            ch.spec_at.assign(r1 - r0 + 1, 0);
+            uint32_t y = 0;
+            for (uint64_t rd = 0; rd <= r1 - r0; ++rd) {
+                while (lo + y < hi && lo[y].read_id < r0 + rd) ++y;
+                ch.spec_at[rd] = y;
             }
+            ch.spec_seen.assign((size_t)(hi - lo), 0);
         });
     }
     for (int round = 0;; ++round) {
@@ -1690,6 +1717,9 @@ void run_batch(Engine& eng, const PackedRef& ref, const Options& o, bool paired,
         Chunk& ch = *chs[x];
         decltype(ch.cache)().swap(ch.cache);
         std::vector<uint32_t>().swap(ch.cigar_pool);
+        ch.spec_req = nullptr;
+        std::vector<uint32_t>().swap(ch.spec_at);
+        std::vector<uint8_t>().swap(ch.spec_seen);
         g_out_pool.give(ch.text_prev);
         g_out_pool.give(ch.text_cur);
         std::vector<Tentative>().swap(ch.tent_prev);
