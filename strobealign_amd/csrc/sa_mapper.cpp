@@ -812,12 +812,32 @@ struct ScoredPair {
     std::optional<Alignment> a1, a2;
 };
 
+// A stable sort by score, largest first (as std::stable_sort with a > b), in
+// place for the short lists of one read or pair; longer lists take
+// std::stable_sort (which allocates a buffer).
+template <typename T>
+void sort_by_score_desc(std::vector<T>& v) {
+    if (v.size() > 32) {
+        std::stable_sort(v.begin(), v.end(), [](const T& a, const T& b) { return a.score > b.score; });
+        return;
+    }
+    for (size_t i = 1; i < v.size(); ++i) {
+        T x = std::move(v[i]);
+        size_t j = i;
+        for (; j > 0 && v[j - 1].score < x.score; --j) v[j] = std::move(v[j - 1]);
+        v[j] = std::move(x);
+    }
+}
+
 // src/mapper.rs:1018-1091
-std::vector<ScoredPair> rescue_read(Ctx& c, uint32_t read2, uint32_t L2, uint32_t read1, uint32_t L1,
-                                    std::vector<Nam>& nams1, Details det[2], float mu, float sigma) {
+void rescue_read(Ctx& c, uint32_t read2, uint32_t L2, uint32_t read1, uint32_t L1, std::vector<Nam>& nams1,
+                 Details det[2], float mu, float sigma, std::vector<ScoredPair>& pairs) {
     float max1 = nams1[0].score;
-    std::vector<Alignment> al1;
-    std::vector<std::optional<Alignment>> al2;
+    // per-thread buffers reused from pair to pair (the call does not nest)
+    thread_local std::vector<Alignment> al1;
+    thread_local std::vector<std::optional<Alignment>> al2;
+    al1.clear();
+    al2.clear();
     for (size_t t = 0; t < nams1.size() && t < c.o.max_tries; ++t) {
         Nam& nam = nams1[t];
         if (nam.score / max1 < c.o.dropoff) break;
@@ -833,7 +853,7 @@ std::vector<ScoredPair> rescue_read(Ctx& c, uint32_t read2, uint32_t L2, uint32_
             al2.push_back(b);
         }
     }
-    std::vector<ScoredPair> pairs;
+    pairs.clear();
     for (const Alignment& a1 : al1) {
         for (const auto& a2 : al2) {
             if (a2) {
@@ -849,7 +869,6 @@ std::vector<ScoredPair> rescue_read(Ctx& c, uint32_t read2, uint32_t L2, uint32_
             }
         }
     }
-    return pairs;
 }
 
 // src/mapper.rs:1184-1198
@@ -876,10 +895,13 @@ bool is_proper_nam_pair(const Nam& n1, const Nam& n2, float mu, float sigma) {
 struct NamPair { float score; const Nam* n1; const Nam* n2; };
 
 // src/mapper.rs:1224-1299
-std::vector<NamPair> best_scoring_nam_pairs(const std::vector<Nam>& n1, const std::vector<Nam>& n2, float mu, float sigma) {
-    std::vector<NamPair> pairs;
-    if (n1.empty() && n2.empty()) return pairs;
-    std::vector<uint32_t> added1, added2;
+void best_scoring_nam_pairs(const std::vector<Nam>& n1, const std::vector<Nam>& n2, float mu, float sigma,
+                            std::vector<NamPair>& pairs) {
+    pairs.clear();
+    if (n1.empty() && n2.empty()) return;
+    thread_local std::vector<uint32_t> added1, added2;
+    added1.clear();
+    added2.clear();
     size_t best_joint = 0;
     for (size_t a = 0; a < n1.size() && a < 1000; ++a) {
         for (size_t b = 0; b < n2.size() && b < 1000; ++b) {
@@ -910,8 +932,7 @@ std::vector<NamPair> best_scoring_nam_pairs(const std::vector<Nam>& n1, const st
             pairs.push_back({n.score, nullptr, &n});
         }
     }
-    std::stable_sort(pairs.begin(), pairs.end(), [](const NamPair& x, const NamPair& y) { return x.score > y.score; });
-    return pairs;
+    sort_by_score_desc(pairs);
 }
 
 // src/mapper.rs:1456-1470
@@ -925,25 +946,28 @@ float top_dropoff(const std::vector<Nam>& nams) {
 struct PairedResult {
     bool proper = false;
     Alignment p1, p2;
-    std::vector<ScoredPair> pairs;
+    std::vector<ScoredPair>* pairs = nullptr;  // the caller's buffer
 };
 
 // src/mapper.rs:778-1014
+// The scored pairs go to `pairs` (cleared here; a per-thread buffer of the caller).
 PairedResult extend_paired_seeds(Ctx& c, std::vector<Nam> nams[2], const uint32_t rid[2], const uint32_t L[2],
-                                 Details det[2], float mu, float sigma) {
+                                 Details det[2], float mu, float sigma, std::vector<ScoredPair>& pairs) {
     PairedResult res;
+    res.pairs = &pairs;
+    pairs.clear();
     const float dropoff = c.o.dropoff;
     if (nams[0].empty() && nams[1].empty()) return res;
     if (!nams[0].empty() && nams[1].empty()) {
-        res.pairs = rescue_read(c, rid[1], L[1], rid[0], L[0], nams[0], det, mu, sigma);
+        rescue_read(c, rid[1], L[1], rid[0], L[0], nams[0], det, mu, sigma, pairs);
         return res;
     }
     if (nams[0].empty() && !nams[1].empty()) {
         Details sw[2] = {det[1], det[0]};
-        res.pairs = rescue_read(c, rid[0], L[0], rid[1], L[1], nams[1], sw, mu, sigma);
+        rescue_read(c, rid[0], L[0], rid[1], L[1], nams[1], sw, mu, sigma, pairs);
         det[0] = sw[1];
         det[1] = sw[0];
-        for (auto& p : res.pairs) std::swap(p.a1, p.a2);
+        for (auto& p : pairs) std::swap(p.a1, p.a2);
         return res;
     }
     if (top_dropoff(nams[0]) < dropoff && top_dropoff(nams[1]) < dropoff &&
@@ -972,18 +996,27 @@ PairedResult extend_paired_seeds(Ctx& c, std::vector<Nam> nams[2], const uint32_
             return res;
         }
     }
-    std::map<uint32_t, std::optional<Alignment>> cache[2];
+    // extension results by NAM id, per read (a few NAMs each: a list, searched
+    // linearly, in per-thread buffers; the call does not nest)
+    thread_local std::vector<std::pair<uint32_t, std::optional<Alignment>>> cache[2];
+    auto cached = [&](int i, uint32_t id) -> std::optional<Alignment>* {
+        for (auto& e : cache[i])
+            if (e.first == id) return &e.second;
+        return nullptr;
+    };
     std::optional<Alignment> indv[2];
     for (int i = 0; i < 2; ++i) {
+        cache[i].clear();
         bool cons = reverse_nam_if_needed(nams[i][0], L[i]);
         det[i].inconsistent_nams += !cons;
         indv[i] = extend_seed(c, nams[i][0], rid[i], L[i], cons, true);
         det[i].tried_alignment++;
         det[i].gapped += indv[i] ? indv[i]->gapped : 0;
-        cache[i][nams[i][0].nam_id] = indv[i];
+        cache[i].emplace_back(nams[i][0].nam_id, indv[i]);
     }
-    std::vector<NamPair> np = best_scoring_nam_pairs(nams[0], nams[1], mu, sigma);
-    std::vector<ScoredPair>& out = res.pairs;
+    thread_local std::vector<NamPair> np;
+    best_scoring_nam_pairs(nams[0], nams[1], mu, sigma, np);
+    std::vector<ScoredPair>& out = pairs;
     float max_score = np[0].score;
     for (const NamPair& p : np) {
         float sd = p.score / max_score;
@@ -994,16 +1027,15 @@ PairedResult extend_paired_seeds(Ctx& c, std::vector<Nam> nams[2], const uint32_
             std::optional<Alignment> al;
             if (ns[i]) {
                 Nam th = *ns[i];
-                auto it = cache[i].find(th.nam_id);
-                if (it == cache[i].end()) {
+                if (std::optional<Alignment>* hit = cached(i, th.nam_id)) {
+                    al = *hit;
+                } else {
                     bool cons = reverse_nam_if_needed(th, L[i]);
                     det[i].inconsistent_nams += !cons;
                     al = extend_seed(c, th, rid[i], L[i], cons, true);
                     det[i].tried_alignment++;
                     det[i].gapped += al ? al->gapped : 0;
-                    cache[i][th.nam_id] = al;
-                } else {
-                    al = it->second;
+                    cache[i].emplace_back(th.nam_id, al);
                 }
             } else {
                 Nam other = *ns[1 - i];
@@ -1078,7 +1110,8 @@ void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Al
     }
     InsertSize& isz = *c.isize;
     float mu = isz.mu, sigma = isz.sigma;
-    PairedResult pr = extend_paired_seeds(c, nams, rid, L, det, mu, sigma);
+    thread_local std::vector<ScoredPair> pair_buf;
+    PairedResult pr = extend_paired_seeds(c, nams, rid, L, det, mu, sigma, pair_buf);
     if (c.abandon) return;
     if (pr.proper) {
         bool ps = is_proper_pair(&pr.p1, &pr.p2, mu, sigma);
@@ -1095,8 +1128,8 @@ void align_paired_end(Ctx& c, uint64_t idx, std::vector<Sam>& out, std::deque<Al
         make_paired(al, recs, mq, det, false, ps, out);
         return;
     }
-    std::vector<ScoredPair>& v = pr.pairs;
-    std::stable_sort(v.begin(), v.end(), [](const ScoredPair& a, const ScoredPair& b) { return a.score > b.score; });
+    std::vector<ScoredPair>& v = *pr.pairs;
+    sort_by_score_desc(v);
     dedup_pairs(v);
     size_t nb = 0;
     while (nb < v.size() && v[nb].score == v[0].score) ++nb;
