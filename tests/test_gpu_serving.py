@@ -51,10 +51,13 @@ def no_header(sam):
     return b"".join(l for l in sam.splitlines(keepends=True) if not l.startswith(b"@"))
 
 
-def test_requests_on_chunk_boundaries_match_oracle(sa, pe):
+@pytest.mark.parametrize("fused", [False, True])
+def test_requests_on_chunk_boundaries_match_oracle(sa, pe, fused):
     """Requests cut at the chunk boundaries of the whole read set, submitted
-    together: in order, their outputs are the oracle's SAM of the whole set."""
-    say("case: serving, 3000 pairs in 12 requests on chunk boundaries, vs oracle")
+    together: in order, their outputs are the oracle's SAM of the whole set
+    (device batches by the plain stages, or by the fused pass, recorded for the
+    batch shapes the requests make)."""
+    say("case: serving, 3000 pairs in 12 requests on chunk boundaries, vs oracle, fused %s" % fused)
     import subprocess
     exp = subprocess.run([ORACLE_BIN, "-t", "8", "--no-PG", "--chunk-size", str(CHUNK), pe["ref"], pe["reads1"],
                           pe["reads2"]], check=True, capture_output=True).stdout
@@ -65,7 +68,7 @@ def test_requests_on_chunk_boundaries_match_oracle(sa, pe):
     cuts.append(3000)
     with sa.Context(0) as ctx, sa.Index(ctx, pe["ref"], read_length=150, threads=8) as ix:
         reads = sa.Reads(pe["reads1"], pe["reads2"])
-        opts = sa.options(threads=8, chunk_size=CHUNK, batch_bases=60_000, slots=2)
+        opts = sa.options(threads=8, chunk_size=CHUNK, batch_bases=60_000, slots=2, fused=fused)
         with sa.Server(ix, opts) as srv:
             tickets = [srv.submit(reads, a, b - a) for a, b in zip(cuts, cuts[1:])]
             outs = [srv.wait(t) for t in tickets]
@@ -143,11 +146,13 @@ def test_admission_waits_for_arena(sa, pe):
     assert st["arena_peak"] <= 200_000
 
 
-def test_requests_of_several_read_lengths(sa, tmp_path):
+@pytest.mark.parametrize("fused", [False, True])
+def test_requests_of_several_read_lengths(sa, tmp_path, fused):
     """The serving configuration's request shape: each request one read length
     (100 to 300 bp, including queries past 256 bp) on an index built for read
-    length 250; every request's output equals mapping its records alone."""
-    say("case: serving, requests of 100-300 bp pairs on an r250 index vs mapped alone")
+    length 250; every request's output equals mapping its records alone (the
+    server's batches by the plain stages or by the fused pass)."""
+    say("case: serving, requests of 100-300 bp pairs on an r250 index vs mapped alone, fused %s" % fused)
     sys.path.insert(0, ROOT)
     from strobealign_amd import synth
     lengths = [100, 175, 250, 275, 300]
@@ -162,7 +167,7 @@ def test_requests_of_several_read_lengths(sa, tmp_path):
             write_fastq(p1, recs[L][0][a:a + n])
             write_fastq(p2, recs[L][1][a:a + n])
             expected.append(no_header(ix.map_files(p1, p2, threads=8, chunk_size=20_000)))
-        opts = sa.options(threads=8, chunk_size=20_000, batch_bases=200_000, slots=3)
+        opts = sa.options(threads=8, chunk_size=20_000, batch_bases=200_000, slots=3, fused=fused)
         with sa.Server(ix, opts) as srv:
             tickets = [srv.submit_records(recs[L][0][a:a + n], recs[L][1][a:a + n]) for L, a, n in reqs]
             outs = [srv.wait(t)[0] for t in tickets]
