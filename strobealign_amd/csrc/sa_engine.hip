@@ -887,8 +887,21 @@ void Engine::chain_tables(const ChainParams& cp) {
     }
 }
 
+// Longest read the mapping passes take. Single-end mapping of 520-2000 bp reads
+// did not finish on the device and no cause is known yet, so batches with longer
+// reads are refused with an error instead of being run. Seeding, hit lookup and
+// the extension kernels themselves take longer queries.
+constexpr uint32_t MAP_MAX_READ_LEN = 512;
+
+static void require_map_len(uint32_t max_len) {
+    if (max_len > MAP_MAX_READ_LEN)
+        throw std::runtime_error("reads longer than 512 bp are not supported by the mapping path (longest: " +
+                                 std::to_string(max_len) + " bp)");
+}
+
 void Engine::find_nams(const MapParams& mp, const ChainParams& cp, NamBatch& out) {
     SA_CHECK(hipSetDevice(device_));
+    require_map_len(max_len_);
     const uint32_t n = n_reads_;
     out.nams = HostView<NamRec>{};
     out.nam_off = HostView<uint64_t>{(uint64_t*)host_stage(0, 8), 1};
@@ -1595,6 +1608,7 @@ void Engine::fused_sequence(Fused& f) {
 bool Engine::map_fused(const MapParams& mp, const ChainParams& cp, const SpecParams& sp, const Scores& sc,
                        NamBatch& out, SpecBatch& spec) {
     SA_CHECK(hipSetDevice(device_));
+    require_map_len(max_len_);
     const uint32_t n = n_reads_;
     spec = SpecBatch{};
     // the first batch of an engine (and any empty one) runs the plain pass, which

@@ -5,8 +5,9 @@ take (the one-thread extension). Needs an MI355X.
 
 Single-end mapping of reads of 520-2000 bp has not finished on the device:
 its first case produced no output for three minutes where the oracle takes
-0.3 s, and no cause has been found in the code yet. The long-read cases are
-skipped until one is; reads of at most 512 bp are unaffected."""
+0.3 s, and no cause has been found in the code yet. The mapping passes refuse
+reads above 512 bp with an error until one is, and the long-read parity cases
+are skipped; reads of at most 512 bp are unaffected."""
 import os
 import subprocess
 import sys
@@ -111,7 +112,8 @@ def test_wildcard_reference_vs_oracle(sa, ctx, small, flags, kw):
         assert b"\tna:i:" in got
 
 
-LONG_READS_OPEN = "single-end mapping of 520-2000 bp reads did not finish on the device; cause not yet found"
+LONG_READS_OPEN = ("single-end mapping of 520-2000 bp reads did not finish on the device; cause not yet found, "
+                   "so the mapping passes refuse reads above 512 bp")
 
 
 @pytest.mark.skip(reason=LONG_READS_OPEN)
@@ -130,6 +132,26 @@ def test_long_reads_vs_oracle(sa, ctx, small, flags, kw):
         recs = [l.split(b"\t") for l in got.splitlines() if not l.startswith(b"@")]
         assert sum(int(r[1]) & 4 == 0 for r in recs) > 300
         assert got == exp
+
+
+def test_long_reads_are_refused(sa, ctx, small, tmp_path):
+    """Reads above 512 bp are refused with an error before any mapping pass
+    runs (single-end and paired-end, plain and fused), and the index then maps
+    short reads as before."""
+    say("case: reads above 512 bp refused by the mapping passes")
+    with open(small["long"], "rb") as f:
+        lines = f.read().splitlines(keepends=True)
+    one = str(tmp_path / "one_long.fq")
+    with open(one, "wb") as f:
+        f.writelines(lines[:4])
+    with sa.Index(ctx, small["ref"], read_length=150) as ix:
+        for kw in ({}, {"fused": True}, {"use_ssw": True}, {"mode": 1}):
+            with pytest.raises(sa.SaError, match="longer than 512 bp"):
+                ix.map_files(one, **kw)
+        with pytest.raises(sa.SaError, match="longer than 512 bp"):
+            ix.map_files(one, one)
+        exp = oracle("--no-PG", small["ref"], small["reads"])
+        assert ix.map_files(small["reads"]) == exp
 
 
 def test_paired_end_details_and_only_mapped_vs_oracle(sa, ctx, tmp_path_factory):
