@@ -151,7 +151,10 @@ void sa_map_options_default(sa_map_options* opt);
  * table) in *out, which the caller releases with sa_free. Replaces the per-chunk
  * Mapper::map_chunk loop of src/main.rs:873-988 with its calls into
  * align_single_end_read / align_paired_end_read (src/mapper.rs:349, 624) and
- * map_*_end_read / abundances_* (src/maponly.rs:18, 57, 114, 198). */
+ * map_*_end_read / abundances_* (src/maponly.rs:18, 57, 114, 198). Reads of
+ * at most 512 bp: a batch with a longer read fails (non-zero return, the
+ * reason in sa_last_error); so do sa_map_records, sa_map_reads and the
+ * serving loop's requests. */
 int sa_map_files(sa_index* index, const sa_map_options* opt, const char* reads1, const char* reads2, char** out,
                  size_t* out_len);
 
